@@ -1,0 +1,236 @@
+/*
+ * lgame_gpu.h -- C-ABI of the MI355X HEVI predictor-corrector step.
+ *
+ * This is the drop-in boundary that replaces the reference time-stepping entry
+ *   subroutine pchevi(state_old,state_new,tend,bc,grid,diag,total_step_counter,lrad_update,t_0)
+ *   (src/time_stepping/mo_manage_pchevi.F90:27-118), called from the serial time loop at
+ *   src/control.F90:520-524 with the two state slots swapped on odd steps.
+ *
+ * Everything crossing the boundary is plain C: ints, doubles and pointers to host arrays that
+ * keep the reference's own layout (Fortran column-major, 1-based in Fortran, y fastest):
+ *   scalar fields  (ny,   nx,   n_layers)          e.g. rhotheta_v, exner_pert
+ *   u-points       (ny,   nx+1, n_layers)          wind_u, dx, area_x, slope_x, ...
+ *   v-points       (ny+1, nx,   n_layers)          wind_v, dy, area_y, slope_y, ...
+ *   w-points       (ny,   nx,   n_layers+1)        wind_w, dz, z_w, area_z, ...
+ *   rho            (ny,   nx,   n_layers, n_constituents)
+ *   inner_product_weights (6, ny, nx, n_layers)    (6 is the fastest index)
+ * A Fortran caller passes c_loc() of its allocatable arrays (allocatables are contiguous).
+ *
+ * The module globals the reference step reads (mo_run_nml, mo_constituents_nml, mo_diff_nml,
+ * mo_surface_nml, mo_bc_nml, mo_rad_nml) become the lgame_dims / lgame_cfg structs below;
+ * Fortran LOGICALs cross as int (0 = .false., nonzero = .true.).
+ *
+ * Error convention: every entry point returns 0 on success and a nonzero LGAME_E* code
+ * otherwise; nothing is thrown across the ABI.  lgame_last_error() gives the message.  The
+ * reference aborts in place with "call exit(1)" (mo_run_nml.F90:104-141, mo_write_out.F90:63-67);
+ * a Fortran shim keeps that by printing lgame_last_error() and calling exit(1) on nonzero.
+ *
+ * Threading: one host thread per handle; a handle is not re-entrant.  Each handle owns one HIP
+ * stream; all device work of a handle is ordered on it.
+ */
+#ifndef LGAME_GPU_H
+#define LGAME_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGAME_ABI_VERSION 1
+
+/* return codes */
+#define LGAME_OK          0
+#define LGAME_EINVAL      1   /* bad argument or unsupported configuration   */
+#define LGAME_EHIP        2   /* a HIP runtime call failed                    */
+#define LGAME_ENOMEM      3   /* device allocation failed                     */
+#define LGAME_ENONFINITE  4   /* NaN/Inf found in a prognostic field          */
+#define LGAME_ESTATE      5   /* call out of order (e.g. step before upload)  */
+
+/* precision modes */
+#define LGAME_PREC_FP64   0   /* plain form: FP64 everywhere                                   */
+#define LGAME_PREC_MIXED  1   /* FP32 stencils/explicit tendencies, FP64 solves/updates/sums   */
+
+/* radiation modes (the step's lrad_update input only has an effect when not OFF) */
+#define LGAME_RAD_OFF     0
+
+/*
+ * Dimensions.  Mirrors mo_run_nml.F90:13-19 (ny, nx, n_layers, n_oro_layers; n_levels and
+ * n_flat_layers are derived as in mo_run_nml.F90:89-90), mo_constituents_nml.F90:12-15 and
+ * mo_surface_nml.F90 (nsoillays).
+ */
+typedef struct lgame_dims {
+  int ny;
+  int nx;
+  int n_layers;
+  int n_oro_layers;
+  int n_constituents;            /* 1 dry, 7 moist                                   */
+  int n_condensed_constituents;  /* 0 dry, 5 moist                                   */
+  int nsoillays;
+} lgame_dims;
+
+/*
+ * Step configuration.  Names and meaning follow the reference namelist variables:
+ * mo_run_nml.F90:13-41, mo_diff_nml.F90, mo_surface_nml.F90, mo_bc_nml.F90, mo_rad_nml.F90.
+ * dtime is passed explicitly (the reference derives it from dx, dy at mo_run_nml.F90:93-94).
+ */
+typedef struct lgame_cfg {
+  double dtime;
+  double toa;
+  double dx;                 /* namelist dx: used by theta_v_adv_3rd_order (mo_multiplications.F90:142) */
+  double dy;                 /* namelist dy                                                            */
+  double klemp_damp_max;
+  double klemp_begin_rel;
+  double dtime_rad;
+  int theta_adv_order;       /* 2 or 3                                                                 */
+  int luse_bg_state;
+  int llinear;
+  int lcorio;
+  int lmoist;
+  int lklemp;
+  int lmom_diff_h, lmom_diff_v;
+  int ltemp_diff_h, ltemp_diff_v;
+  int lmass_diff_h, lmass_diff_v;
+  int lprog_soil_temp;
+  int lsfc_sensible_heat_flux;
+  int lsfc_phase_trans;
+  int lpbl;
+  int lperiodic;
+  int lfreeslip;
+  int lrad;
+  int precision;             /* LGAME_PREC_*                                                           */
+  int rad_mode;              /* LGAME_RAD_*                                                            */
+} lgame_cfg;
+
+/*
+ * Read-only grid metrics consumed by the step (the t_grid components of
+ * src/mo_definitions.F90:29-91 that the hot path reads), in the reference layout.
+ */
+typedef struct lgame_grid_view {
+  const double *z_scalar;              /* (ny,nx,nl)     */
+  const double *z_u;                   /* (ny,nx+1,nl)   */
+  const double *z_v;                   /* (ny+1,nx,nl)   */
+  const double *z_w;                   /* (ny,nx,nl+1)   */
+  const double *dx;                    /* (ny,nx+1,nl)   */
+  const double *dy;                    /* (ny+1,nx,nl)   */
+  const double *dz;                    /* (ny,nx,nl+1)   */
+  const double *area_x;                /* (ny,nx+1,nl)   */
+  const double *area_y;                /* (ny+1,nx,nl)   */
+  const double *area_z;                /* (ny,nx,nl+1)   */
+  const double *volume;                /* (ny,nx,nl)     */
+  const double *slope_x;               /* (ny,nx+1,nl)   */
+  const double *slope_y;               /* (ny+1,nx,nl)   */
+  const double *inner_product_weights; /* (6,ny,nx,nl)   */
+  const double *area_dual_x;           /* (ny+1,nx,nl+1) */
+  const double *area_dual_y;           /* (ny,nx+1,nl+1) */
+  const double *area_dual_z;           /* (ny+1,nx+1,nl) */
+  const double *z_area_dual_z;         /* (ny+1,nx+1,nl) */
+  const double *fvec_x;                /* (ny+1,nx)      */
+  const double *fvec_y;                /* (ny,nx+1)      */
+  const double *fvec_z;                /* (ny+1,nx+1)    */
+  const double *trsk_weights_u;        /* (3,ny)         */
+  const double *trsk_weights_v;        /* (2,ny+1)       */
+  const double *exner_bg_grad_u;       /* (ny,nx+1,nl)   */
+  const double *exner_bg_grad_v;       /* (ny+1,nx,nl)   */
+  const double *exner_bg_grad_w;       /* (ny,nx,nl+1)   */
+  const double *theta_v_bg;            /* (ny,nx,nl)     */
+  const double *exner_bg;              /* (ny,nx,nl)     */
+  const double *gravity_m_v;           /* (ny,nx,nl+1)   */
+  const double *gravity_potential;     /* (ny,nx,nl)     */
+} lgame_grid_view;
+
+/* Prognostic state (t_state, src/mo_definitions.F90:94-105), reference layout. */
+typedef struct lgame_state_view {
+  double *rho;               /* (ny,nx,nl,n_constituents) */
+  double *rhotheta_v;        /* (ny,nx,nl)                */
+  double *theta_v_pert;      /* (ny,nx,nl)                */
+  double *exner_pert;        /* (ny,nx,nl)                */
+  double *wind_u;            /* (ny,nx+1,nl)              */
+  double *wind_v;            /* (ny+1,nx,nl)              */
+  double *wind_w;            /* (ny,nx,nl+1)              */
+  double *temperature_soil;  /* (ny,nx,nsoillays); may be NULL when nsoillays==0 or soil is off */
+} lgame_state_view;
+
+typedef struct lgame_handle lgame_handle;
+
+/* Budget-sum slots written by lgame_integrals (src/io/mo_write_out.F90:226-304). */
+#define LGAME_INT_RHOTHETA   0   /* sum(rhotheta_v*volume)                                  */
+#define LGAME_INT_KINETIC    1   /* 0.5*sum(rho_g*(ipw-weighted |v|^2)*volume)              */
+#define LGAME_INT_POTENTIAL  2   /* sum(rho_g*gravity_potential*volume)                     */
+#define LGAME_INT_INTERNAL   3   /* sum(c_d_v*rho_g*temperature*volume)                     */
+#define LGAME_INT_MASS0      4   /* masses: slot LGAME_INT_MASS0 + jc-1 for constituent jc  */
+
+/* Diagnostic fields readable with lgame_get_diag (reference t_diag components). */
+#define LGAME_DIAG_TEMPERATURE   0   /* diag%temperature (ny,nx,nl), of the last step's state_old */
+
+/* Version of this ABI (LGAME_ABI_VERSION). */
+int lgame_abi_version(void);
+
+/*
+ * Create a handle: validate dims/cfg, allocate device memory, upload the grid once and
+ * precompute grid-only factors.  Replaces the set-up half of control.F90 that feeds pchevi.
+ * Device 0 of the calling process's visible devices is used.
+ */
+int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_view *grid,
+               lgame_handle **out);
+
+/*
+ * Host -> device copy of a full state into resident slot 0 or 1 (lgame_sync's upload half).
+ */
+int lgame_set_state(lgame_handle *h, int slot, const lgame_state_view *state);
+
+/*
+ * Device -> host copy of resident slot 0 or 1 (the reference refreshes host state only before
+ * write_output / write_out_integrals, control.F90:529-550).
+ */
+int lgame_get_state(lgame_handle *h, int slot, lgame_state_view *state);
+
+/*
+ * One HEVI predictor-corrector step on the resident slots: state_new = step(state_old).
+ * Same arguments and meaning as pchevi (mo_manage_pchevi.F90:27): total_step_counter == 0
+ * selects the first-step branches (mo_p_grad.F90:38-43,97-102; mo_vector_tend_expl.F90:58);
+ * lrad_update asks for a radiation update before the substeps.  The carried diagnostics
+ * (previous pressure gradient, lagged momentum advection, tendencies) live in the handle and
+ * persist between calls exactly as t_diag/t_tend do between reference pchevi calls.
+ * Asynchronous on the handle's stream; use lgame_sync to wait.
+ */
+int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_counter,
+                 int lrad_update, double t_0);
+
+/*
+ * Strict drop-in form: upload state_old and state_new, run one step, download state_new.
+ * Mirrors the reference call signature with host arrays; costs two PCIe round trips.
+ */
+int lgame_pchevi_host(lgame_handle *h, const lgame_state_view *state_old,
+                      lgame_state_view *state_new, int total_step_counter, int lrad_update,
+                      double t_0);
+
+/* Wait for all work queued on the handle's stream. */
+int lgame_sync(lgame_handle *h);
+
+/*
+ * Domain budget sums of resident slot `slot` (write_out_integrals, mo_write_out.F90:226-304),
+ * computed on the device in FP64 with compensated (two-sum) accumulation.  `out` receives
+ * 4 + n_constituents doubles at the LGAME_INT_* slots.  The internal-energy sum uses the
+ * handle's diagnosed temperature, as the reference does (control.F90:529-535 after pchevi).
+ */
+int lgame_integrals(lgame_handle *h, int slot, double *out);
+
+/*
+ * Finite check of resident slot `slot`: *n_bad = number of NaN/Inf values in the prognostic
+ * fields (the reference's crash check, mo_write_out.F90:63-67, over every field).
+ */
+int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad);
+
+/* Copy a diagnostic field (LGAME_DIAG_*) to the host in the reference layout. */
+int lgame_get_diag(lgame_handle *h, int which, double *out);
+
+/* Release the handle and its device memory. */
+void lgame_finalize(lgame_handle *h);
+
+/* Message of the last failing call on `h` (or of lgame_init when h is NULL). */
+const char *lgame_last_error(const lgame_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LGAME_GPU_H */

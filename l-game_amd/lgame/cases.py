@@ -1,0 +1,75 @@
+"""Named workloads: run configuration + grid + initial state.
+
+* ``schaer``: the Schaer mountain-wave test (run_scripts/schaer.sh:12-65) on the ny=3 plane with
+  nx=201, 40 layers, dx=500 m, toa 21 km.  dtime defaults to the reference's own
+  1.61*sqrt(dx*dy)/1000 = 0.805 s (mo_run_nml.F90:93-94); 4 s is unstable in the reference scheme.
+* ``dry3d``: 3-D dry run on the rotated lat-lon grid of run_scripts/standard.sh:14-15, ICAO
+  standard atmosphere, 5 seeded Gaussian hills (heights U[0,1500] m, half-widths U[5,30] km),
+  seeded theta_v (0.5 K) and wind (1 m/s) perturbations with 20 km correlation length.
+* ``moist3d``: as dry3d with the 7 constituents and seeded humidity (60-95 % RH below 3 km).
+* ``tiny``: small random-ish 3-D case for fast parity tests.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .config import RunConfig
+from .grid import grid_setup
+from .init_state import add_moisture, ideal_init, perturb
+
+LAT_CENTER_STD = 0.8929595951304794   # run_scripts/standard.sh:14
+LON_CENTER_STD = 0.1199133716060684   # run_scripts/standard.sh:15
+
+
+def schaer_config(nx=201, n_layers=40, dtime=None, **kw) -> RunConfig:
+    return RunConfig(ny=3, nx=nx, n_layers=n_layers, n_oro_layers=n_layers - 1, toa=21000.0, dy=500.0,
+                     dx=500.0, scenario="schaer", lcorio=False, stretching_parameter=1.0, lplane=True,
+                     theta_adv_order=3, lmoist=False, klemp_begin_rel=0.5, orography_id=2, lsleve=True,
+                     lperiodic=True, lfreeslip=True, dtime=dtime, **kw)
+
+
+def random_hills(cfg: RunConfig, seed: int, n=5, hmax=1500.0, wmin=5e3, wmax=30e3):
+    rng = np.random.default_rng(seed)
+    Lx, Ly = cfg.nx * cfg.dx, cfg.ny * cfg.dy
+    hills = []
+    for _ in range(n):
+        x0 = rng.uniform(-0.5 * Lx, 0.5 * Lx)
+        y0 = rng.uniform(-0.5 * Ly, 0.5 * Ly)
+        hills.append((x0, y0, rng.uniform(0.0, hmax), rng.uniform(wmin, wmax)))
+    return tuple(hills)
+
+
+def dry3d_config(n=129, n_layers=50, dtime=3.22, seed=2, **kw) -> RunConfig:
+    cfg = RunConfig(ny=n, nx=n, n_layers=n_layers, n_oro_layers=int(round(0.8 * n_layers)), dy=2000.0,
+                    dx=2000.0, toa=40000.0, stretching_parameter=1.3, scenario="standard",
+                    lat_center=LAT_CENTER_STD, lon_center=LON_CENTER_STD, theta_adv_order=2,
+                    lmoist=False, orography_id=5, lsleve=True, lperiodic=True, lfreeslip=False,
+                    dtime=dtime, **kw)
+    return dataclasses.replace(cfg, hills=random_hills(cfg, seed))
+
+
+def moist3d_config(n=513, n_layers=60, dtime=3.22, seed=3, **kw) -> RunConfig:
+    cfg = dry3d_config(n=n, n_layers=n_layers, dtime=dtime, seed=seed, **kw)
+    return dataclasses.replace(cfg, lmoist=True)
+
+
+def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True, cloud_frac=0.0):
+    """Grid and initial state for ``cfg`` (perturbations only for the standard scenario)."""
+    g = grid_setup(cfg)
+    st = ideal_init(cfg, g)
+    if cfg.scenario == "standard" and perturbation:
+        perturb(st, g, cfg, seed=(seed if seed is not None else 2))
+    if cfg.lmoist:
+        add_moisture(st, g, cfg, seed=(seed if seed is not None else 3) + 1000, cloud_frac=cloud_frac)
+    return g, st
+
+
+def tiny_config(ny=5, nx=7, n_layers=6, moist=False, order=2, freeslip=False, corio=True, plane=False, **kw) -> RunConfig:
+    cfg = RunConfig(ny=ny, nx=nx, n_layers=n_layers, n_oro_layers=n_layers - 2, dy=2000.0, dx=2000.0,
+                    toa=12000.0, stretching_parameter=1.3, scenario="standard", lplane=plane,
+                    lat_center=LAT_CENTER_STD, lon_center=LON_CENTER_STD, theta_adv_order=order,
+                    lmoist=moist, lcorio=corio, orography_id=5, lsleve=True, lperiodic=True,
+                    lfreeslip=freeslip, dtime=2.0, klemp_begin_rel=0.5, **kw)
+    return dataclasses.replace(cfg, hills=((0.0, 0.0, 400.0, 4000.0), (3000.0, -2000.0, 250.0, 3000.0)))

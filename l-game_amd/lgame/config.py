@@ -1,0 +1,176 @@
+"""Run configuration: the reference namelist variables that the step and its set-up read.
+
+Defaults follow the reference namelist set-up routines:
+``mo_run_nml.F90:55-80`` (run), ``mo_diff_nml.F90:39-53`` (diff), ``mo_surface_nml.F90:31-37``
+(surface), ``mo_bc_nml.F90:30-35`` (bc), ``mo_rad_nml.F90:31-36`` (rad),
+``mo_constituents_nml.F90:26-43`` (constituents).  ``dtime`` is derived as in
+``mo_run_nml.F90:93-94`` unless set explicitly.
+
+``to_abi`` produces the ``lgame_dims`` / ``lgame_cfg`` C structs of ``include/lgame_gpu.h``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+RAD_OFF = 0
+
+
+class LgameDims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "ny", "nx", "n_layers", "n_oro_layers", "n_constituents", "n_condensed_constituents",
+        "nsoillays")]
+
+
+class LgameCfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "dtime", "toa", "dx", "dy", "klemp_damp_max", "klemp_begin_rel", "dtime_rad")] + \
+        [(n, ctypes.c_int) for n in (
+            "theta_adv_order", "luse_bg_state", "llinear", "lcorio", "lmoist", "lklemp",
+            "lmom_diff_h", "lmom_diff_v", "ltemp_diff_h", "ltemp_diff_v", "lmass_diff_h",
+            "lmass_diff_v", "lprog_soil_temp", "lsfc_sensible_heat_flux", "lsfc_phase_trans",
+            "lpbl", "lperiodic", "lfreeslip", "lrad", "precision", "rad_mode")]
+
+
+@dataclasses.dataclass
+class RunConfig:
+    # &run
+    ny: int = 35
+    nx: int = 37
+    n_layers: int = 50
+    n_oro_layers: int = 40
+    dy: float = 25e3
+    dx: float = 25e3
+    toa: float = 40000.0
+    stretching_parameter: float = 1.3
+    scenario: str = "standard"
+    llinear: bool = False
+    lcorio: bool = True
+    lplane: bool = False
+    lat_center: float = 0.0
+    lon_center: float = 0.0
+    theta_adv_order: int = 2
+    luse_bg_state: bool = True
+    dtime: float | None = None
+    # &constituents
+    lmoist: bool = False
+    # &diff (diffusion is outside the accelerated path: off)
+    lklemp: bool = True
+    klemp_damp_max: float = 0.25
+    klemp_begin_rel: float = 0.53
+    # &surface
+    orography_id: int = 0
+    lsleve: bool = False
+    nsoillays: int = 5
+    # &bc
+    lperiodic: bool = True
+    lfreeslip: bool = False
+    # &rad
+    lrad: bool = False
+    dtime_rad: float | None = None
+    # set-up extensions (not reference namelist variables)
+    hills: tuple = ()            # ((x0, y0, height, half_width), ...) in metres, orography_id == 5
+    precision: int = PREC_FP64
+
+    @property
+    def n_levels(self) -> int:
+        return self.n_layers + 1
+
+    @property
+    def n_flat_layers(self) -> int:
+        return self.n_layers - self.n_oro_layers
+
+    @property
+    def n_condensed_constituents(self) -> int:
+        return 5 if self.lmoist else 0
+
+    @property
+    def n_constituents(self) -> int:
+        return 7 if self.lmoist else 1
+
+    @property
+    def dt(self) -> float:
+        if self.dtime is not None:
+            return float(self.dtime)
+        # mo_run_nml.F90:93-94
+        return 1.61 * math.sqrt(self.dx * self.dy) / 1000.0
+
+    @property
+    def gridpoints(self) -> int:
+        return self.ny * self.nx * self.n_layers
+
+    def to_abi(self):
+        d = LgameDims(self.ny, self.nx, self.n_layers, self.n_oro_layers, self.n_constituents,
+                      self.n_condensed_constituents, self.nsoillays)
+        c = LgameCfg()
+        c.dtime = self.dt
+        c.toa = self.toa
+        c.dx = self.dx
+        c.dy = self.dy
+        c.klemp_damp_max = self.klemp_damp_max
+        c.klemp_begin_rel = self.klemp_begin_rel
+        c.dtime_rad = self.dtime_rad if self.dtime_rad is not None else 60.0 * self.dy / 1000.0
+        c.theta_adv_order = self.theta_adv_order
+        c.luse_bg_state = int(self.luse_bg_state)
+        c.llinear = int(self.llinear)
+        c.lcorio = int(self.lcorio)
+        c.lmoist = int(self.lmoist)
+        c.lklemp = int(self.lklemp)
+        c.lperiodic = int(self.lperiodic)
+        c.lfreeslip = int(self.lfreeslip)
+        c.lrad = int(self.lrad)
+        c.precision = self.precision
+        c.rad_mode = RAD_OFF
+        return d, c
+
+
+# Reference extents of every array that crosses the boundary, as functions of (ny, nx, nl).
+GRID_FIELDS = {
+    "z_scalar": lambda ny, nx, nl: (ny, nx, nl),
+    "z_u": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "z_v": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "z_w": lambda ny, nx, nl: (ny, nx, nl + 1),
+    "dx": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "dy": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "dz": lambda ny, nx, nl: (ny, nx, nl + 1),
+    "area_x": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "area_y": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "area_z": lambda ny, nx, nl: (ny, nx, nl + 1),
+    "volume": lambda ny, nx, nl: (ny, nx, nl),
+    "slope_x": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "slope_y": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "inner_product_weights": lambda ny, nx, nl: (6, ny, nx, nl),
+    "area_dual_x": lambda ny, nx, nl: (ny + 1, nx, nl + 1),
+    "area_dual_y": lambda ny, nx, nl: (ny, nx + 1, nl + 1),
+    "area_dual_z": lambda ny, nx, nl: (ny + 1, nx + 1, nl),
+    "z_area_dual_z": lambda ny, nx, nl: (ny + 1, nx + 1, nl),
+    "fvec_x": lambda ny, nx, nl: (ny + 1, nx),
+    "fvec_y": lambda ny, nx, nl: (ny, nx + 1),
+    "fvec_z": lambda ny, nx, nl: (ny + 1, nx + 1),
+    "trsk_weights_u": lambda ny, nx, nl: (3, ny),
+    "trsk_weights_v": lambda ny, nx, nl: (2, ny + 1),
+    "exner_bg_grad_u": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "exner_bg_grad_v": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "exner_bg_grad_w": lambda ny, nx, nl: (ny, nx, nl + 1),
+    "theta_v_bg": lambda ny, nx, nl: (ny, nx, nl),
+    "exner_bg": lambda ny, nx, nl: (ny, nx, nl),
+    "gravity_m_v": lambda ny, nx, nl: (ny, nx, nl + 1),
+    "gravity_potential": lambda ny, nx, nl: (ny, nx, nl),
+}
+
+STATE_FIELDS = ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w")
+
+
+def state_shape(name: str, ny: int, nx: int, nl: int, nc: int):
+    return {
+        "rho": (ny, nx, nl, nc),
+        "rhotheta_v": (ny, nx, nl),
+        "theta_v_pert": (ny, nx, nl),
+        "exner_pert": (ny, nx, nl),
+        "wind_u": (ny, nx + 1, nl),
+        "wind_v": (ny + 1, nx, nl),
+        "wind_w": (ny, nx, nl + 1),
+    }[name]

@@ -1,0 +1,1644 @@
+/*
+ * lgame_oracle.c -- CPU restatement of the L-GAME HEVI predictor-corrector step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the GPU step is checked against: only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark may load it, and never
+ * as the thing measured or shipped.  The product (liblgame_gpu.so) does not link or call it.
+ *
+ * It restates, loop for loop and in the same floating-point operation order, the reference
+ * Fortran of the hot path (paths relative to the reference's src/):
+ *   time_stepping/mo_manage_pchevi.F90, mo_p_grad.F90, mo_vector_tend_expl.F90,
+ *   mo_scalar_tend_expl.F90, mo_column_solvers.F90;
+ *   spatial_operators/mo_gradient_operators.F90, mo_divergence_operators.F90,
+ *   mo_multiplications.F90, mo_averaging.F90, mo_vorticities.F90, mo_vorticity_flux.F90,
+ *   mo_inner_product.F90, mo_momentum_diff_diss.F90 (simple_dissipation_rate);
+ *   constituents/mo_derived.F90, mo_dictionary.F90; io/mo_write_out.F90 (budget sums).
+ * Arrays keep the reference layout (column-major, 1-based indices through the A3 macro).
+ * Build with -O2 -ffp-contract=off: the reference build has no FMA contraction.
+ *
+ * Supported configuration: periodic lateral boundaries, luse_bg_state, diffusion/PBL/surface
+ * fluxes/soil off (the reference switches used by the Schaer and standard-atmosphere runs).
+ * Anything else is rejected by orc_create.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/lgame_gpu.h"
+
+/* mo_constants.F90:16-36 */
+static const double R_E = 6371000.789927;
+static const double N_A = 6.02214076e23;
+static const double T_0 = 273.15;
+static const double RHO_H2O = 1024.0;
+static const double R_D = 287.057811;
+static const double C_D_V = 717.942189;
+static const double C_V_V = 1396.475121;
+static const double C_D_P = 1005.0;
+static const double K_B = 1.380649e-23;
+#define M_D (N_A * 0.004810e-23)
+#define M_V (N_A * 0.002991e-23)
+#define IMPL_THERMO_WEIGHT (C_D_V / C_D_P)
+#define MPI 3.14159265358979323846
+
+typedef struct {
+  double *p;
+  int n1, n2, n3;
+} F3;
+
+#define A3(a, i, j, k) \
+  ((a).p[((size_t)((i)-1)) + (size_t)(a).n1 * (((size_t)((j)-1)) + (size_t)(a).n2 * ((size_t)((k)-1)))])
+
+typedef struct {
+  double *rho; /* (ny,nx,nl,nc) */
+  F3 rhotheta_v, theta_v_pert, exner_pert, wind_u, wind_v, wind_w;
+} ostate;
+
+typedef struct oracle {
+  int ny, nx, nl, nlev, noro, nflat, nc, ncond;
+  double dtime, toa, dxn, dyn, klemp_damp_max, klemp_begin_rel;
+  int order, use_bg, llinear, lcorio, lmoist, lklemp, lfreeslip;
+  int n_damping_levels;
+  /* grid */
+  F3 z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume, slope_x, slope_y;
+  F3 area_dual_x, area_dual_y, area_dual_z, z_area_dual_z, fvec_x, fvec_y, fvec_z;
+  F3 ebg_u, ebg_v, ebg_w, theta_v_bg, exner_bg, gravity_m_v, gravity_potential;
+  double *ipw;    /* (6,ny,nx,nl) */
+  double *trsk_u; /* (3,ny) */
+  double *trsk_v; /* (2,ny+1) */
+  /* states */
+  ostate st[2];
+  /* tendencies (t_tend) */
+  double *tend_rho;
+  F3 tend_rhotheta_v, tend_u, tend_v, tend_w;
+  /* diagnostics (t_diag components used by the step) */
+  F3 v_squared, vsg_x, vsg_y, vsg_z, pl_u, pl_v, pl_w, pnl_u, pnl_v, pnl_w, pold_u, pold_v;
+  F3 pgc_w, pvt_x, pvt_y, pvt_z, scalar_ph, temperature, u_ph, v_ph, w_ph, theta_v_u, theta_v_v;
+  F3 zeta_x, zeta_y, zeta_z, eta_x, eta_y, eta_z, radiation_tendency;
+  F3 flux_u, flux_v, flux_div, decel, mom_x, mom_y, mom_z, heating_diss;
+  F3 phase_trans_heating_rate, temp_diff_heating, condensates_sediment_heat, a_rain;
+  double *phase_trans_rates;  /* (ny,nx,nl,ncond+1) */
+  double *mass_diff_tendency; /* (ny,nx,nl,nc)     */
+  /* per-column work vectors */
+  double *work;
+  double *allocs[160];
+  int nallocs;
+} oracle;
+
+static double *oalloc(oracle *o, size_t n) {
+  double *p = (double *)calloc(n ? n : 1, sizeof(double));
+  if (o->nallocs < (int)(sizeof(o->allocs) / sizeof(o->allocs[0]))) o->allocs[o->nallocs++] = p;
+  return p;
+}
+
+#define IPW(o, c, i, j, k) \
+  ((o)->ipw[((size_t)(c)-1) + 6 * (((size_t)(i)-1) + (size_t)(o)->ny * (((size_t)(j)-1) + (size_t)(o)->nx * ((size_t)(k)-1)))])
+#define TRSKU(o, c, i) ((o)->trsk_u[((c)-1) + 3 * ((i)-1)])
+#define TRSKV(o, c, i) ((o)->trsk_v[((c)-1) + 2 * ((i)-1)])
+
+static double *oalloc(struct oracle *o, size_t n);
+
+static F3 f3alloc_o(struct oracle *o, int n1, int n2, int n3) {
+  F3 a;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  a.p = oalloc(o, (size_t)n1 * n2 * n3);
+  return a;
+}
+
+static F3 rho_slice(const oracle *o, double *rho, int jc) {
+  F3 a;
+  a.n1 = o->ny;
+  a.n2 = o->nx;
+  a.n3 = o->nl;
+  a.p = rho + (size_t)(jc - 1) * o->ny * o->nx * o->nl;
+  return a;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_averaging.F90                                                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* vertical_contravariant_corr, mo_averaging.F90:16-54 */
+static double vcc(const oracle *o, F3 vx, F3 vy, int ji, int jk, int jl) {
+  double r = 0.0;
+  if (jl >= o->nflat + 1) {
+    if (jl == o->nflat + 1) {
+      r = r - 0.5 * A3(vx, ji, jk + 1, jl) * A3(o->slope_x, ji, jk + 1, jl) * IPW(o, 1, ji, jk, jl)
+            - 0.5 * A3(vy, ji, jk, jl) * A3(o->slope_y, ji, jk, jl) * IPW(o, 2, ji, jk, jl)
+            - 0.5 * A3(vx, ji, jk, jl) * A3(o->slope_x, ji, jk, jl) * IPW(o, 3, ji, jk, jl)
+            - 0.5 * A3(vy, ji + 1, jk, jl) * A3(o->slope_y, ji + 1, jk, jl) * IPW(o, 4, ji, jk, jl);
+    } else {
+      r = r - 0.5 * A3(vx, ji, jk + 1, jl - 1) * A3(o->slope_x, ji, jk + 1, jl - 1) * IPW(o, 1, ji, jk, jl - 1)
+            - 0.5 * A3(vy, ji, jk, jl - 1) * A3(o->slope_y, ji, jk, jl - 1) * IPW(o, 2, ji, jk, jl - 1)
+            - 0.5 * A3(vx, ji, jk, jl - 1) * A3(o->slope_x, ji, jk, jl - 1) * IPW(o, 3, ji, jk, jl - 1)
+            - 0.5 * A3(vy, ji + 1, jk, jl - 1) * A3(o->slope_y, ji + 1, jk, jl - 1) * IPW(o, 4, ji, jk, jl - 1);
+      r = r - 0.5 * A3(vx, ji, jk + 1, jl) * A3(o->slope_x, ji, jk + 1, jl) * IPW(o, 1, ji, jk, jl)
+            - 0.5 * A3(vy, ji, jk, jl) * A3(o->slope_y, ji, jk, jl) * IPW(o, 2, ji, jk, jl)
+            - 0.5 * A3(vx, ji, jk, jl) * A3(o->slope_x, ji, jk, jl) * IPW(o, 3, ji, jk, jl)
+            - 0.5 * A3(vy, ji + 1, jk, jl) * A3(o->slope_y, ji + 1, jk, jl) * IPW(o, 4, ji, jk, jl);
+    }
+  }
+  return r;
+}
+
+/* remap_ver2hor_x, mo_averaging.F90:56-95 (periodic branch; lperiodic is required) */
+static double remap_x(const oracle *o, F3 vc, int ji, int jk, int jl) {
+  double r = 0.0;
+  if (jk == 1 || jk == o->nx + 1) {
+    r = r + IPW(o, 5, ji, o->nx, jl) * A3(vc, ji, o->nx, jl);
+    r = r + IPW(o, 5, ji, 1, jl) * A3(vc, ji, 1, jl);
+    if (jl < o->nl || o->lfreeslip) {
+      r = r + IPW(o, 6, ji, o->nx, jl) * A3(vc, ji, o->nx, jl + 1);
+      r = r + IPW(o, 6, ji, 1, jl) * A3(vc, ji, 1, jl + 1);
+    }
+  } else {
+    r = r + IPW(o, 5, ji, jk - 1, jl) * A3(vc, ji, jk - 1, jl);
+    r = r + IPW(o, 5, ji, jk, jl) * A3(vc, ji, jk, jl);
+    if (jl < o->nl || o->lfreeslip) {
+      r = r + IPW(o, 6, ji, jk - 1, jl) * A3(vc, ji, jk - 1, jl + 1);
+      r = r + IPW(o, 6, ji, jk, jl) * A3(vc, ji, jk, jl + 1);
+    }
+  }
+  return 0.5 * r;
+}
+
+/* remap_ver2hor_y, mo_averaging.F90:97-136 */
+static double remap_y(const oracle *o, F3 vc, int ji, int jk, int jl) {
+  double r = 0.0;
+  if (ji == 1 || ji == o->ny + 1) {
+    r = r + IPW(o, 5, o->ny, jk, jl) * A3(vc, o->ny, jk, jl);
+    r = r + IPW(o, 5, 1, jk, jl) * A3(vc, 1, jk, jl);
+    if (jl < o->nl || o->lfreeslip) {
+      r = r + IPW(o, 6, o->ny, jk, jl) * A3(vc, o->ny, jk, jl + 1);
+      r = r + IPW(o, 6, 1, jk, jl) * A3(vc, 1, jk, jl + 1);
+    }
+  } else {
+    r = r + IPW(o, 5, ji - 1, jk, jl) * A3(vc, ji - 1, jk, jl);
+    r = r + IPW(o, 5, ji, jk, jl) * A3(vc, ji, jk, jl);
+    if (jl < o->nl || o->lfreeslip) {
+      r = r + IPW(o, 6, ji - 1, jk, jl) * A3(vc, ji - 1, jk, jl + 1);
+      r = r + IPW(o, 6, ji, jk, jl) * A3(vc, ji, jk, jl + 1);
+    }
+  }
+  return 0.5 * r;
+}
+
+/* horizontal_covariant_x/_y, mo_averaging.F90:138-176 */
+static double hcov_x(const oracle *o, F3 hx, F3 vz, int ji, int jk, int jl) {
+  double r = A3(hx, ji, jk, jl);
+  if (jl > o->nflat) r = r + A3(o->slope_x, ji, jk, jl) * remap_x(o, vz, ji, jk, jl);
+  return r;
+}
+static double hcov_y(const oracle *o, F3 hy, F3 vz, int ji, int jk, int jl) {
+  double r = A3(hy, ji, jk, jl);
+  if (jl > o->nflat) r = r + A3(o->slope_y, ji, jk, jl) * remap_y(o, vz, ji, jk, jl);
+  return r;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_gradient_operators.F90                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* grad_hor_cov, mo_gradient_operators.F90:17-63 */
+static void grad_hor_cov(const oracle *o, F3 s, F3 rx, F3 ry) {
+  int ny = o->ny, nx = o->nx, nl = s.n3;
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(rx, ji, jk, jl) = (A3(s, ji, jk, jl) - A3(s, ji, jk - 1, jl)) / A3(o->dx, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(ry, ji, jk, jl) = (A3(s, ji - 1, jk, jl) - A3(s, ji, jk, jl)) / A3(o->dy, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      A3(rx, ji, 1, jl) = (A3(s, ji, 1, jl) - A3(s, ji, nx, jl)) / A3(o->dx, ji, 1, jl);
+      A3(rx, ji, nx + 1, jl) = A3(rx, ji, 1, jl);
+    }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      A3(ry, 1, jk, jl) = (A3(s, ny, jk, jl) - A3(s, 1, jk, jl)) / A3(o->dy, 1, jk, jl);
+      A3(ry, ny + 1, jk, jl) = A3(ry, 1, jk, jl);
+    }
+}
+
+/* grad_vert, mo_gradient_operators.F90:65-83 */
+static void grad_vert(const oracle *o, F3 s, F3 r) {
+  for (int jl = 2; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(r, ji, jk, jl) = (A3(s, ji, jk, jl - 1) - A3(s, ji, jk, jl)) / A3(o->dz, ji, jk, jl);
+}
+
+/* grad_hor, mo_gradient_operators.F90:85-130 */
+static void grad_hor(const oracle *o, F3 s, F3 rx, F3 ry, F3 rz) {
+  grad_hor_cov(o, s, rx, ry);
+  for (int jl = o->nflat + 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx + 1; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(rx, ji, jk, jl) = A3(rx, ji, jk, jl) - A3(o->slope_x, ji, jk, jl) * remap_x(o, rz, ji, jk, jl);
+  for (int jl = o->nflat + 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny + 1; ++ji)
+        A3(ry, ji, jk, jl) = A3(ry, ji, jk, jl) - A3(o->slope_y, ji, jk, jl) * remap_y(o, rz, ji, jk, jl);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_divergence_operators.F90                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* div_h, mo_divergence_operators.F90:16-68 */
+static void div_h(const oracle *o, F3 vx, F3 vy, F3 res) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double comp_h = A3(vx, ji, jk + 1, jl) * A3(o->area_x, ji, jk + 1, jl) + A3(vy, ji, jk, jl) * A3(o->area_y, ji, jk, jl) -
+                        A3(vx, ji, jk, jl) * A3(o->area_x, ji, jk, jl) - A3(vy, ji + 1, jk, jl) * A3(o->area_y, ji + 1, jk, jl);
+        double comp_v = 0.0;
+        if (jl == o->nflat) {
+          double cl = vcc(o, vx, vy, ji, jk, jl + 1);
+          comp_v = -cl * A3(o->area_z, ji, jk, jl + 1);
+        } else if (jl == o->nl) {
+          double cu = vcc(o, vx, vy, ji, jk, jl);
+          comp_v = cu * A3(o->area_z, ji, jk, jl);
+        } else if (jl > o->nflat) {
+          double cu = vcc(o, vx, vy, ji, jk, jl);
+          double cl = vcc(o, vx, vy, ji, jk, jl + 1);
+          comp_v = cu * A3(o->area_z, ji, jk, jl) - cl * A3(o->area_z, ji, jk, jl + 1);
+        }
+        A3(res, ji, jk, jl) = (comp_h + comp_v) / A3(o->volume, ji, jk, jl);
+      }
+}
+
+/* div_h_tracers, mo_divergence_operators.F90:70-147 */
+static void div_h_tracers(const oracle *o, F3 vx, F3 vy, F3 dens, F3 wx, F3 wy, F3 res) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double comp_h = A3(vx, ji, jk + 1, jl) * A3(o->area_x, ji, jk + 1, jl) + A3(vy, ji, jk, jl) * A3(o->area_y, ji, jk, jl) -
+                        A3(vx, ji, jk, jl) * A3(o->area_x, ji, jk, jl) - A3(vy, ji + 1, jk, jl) * A3(o->area_y, ji + 1, jk, jl);
+        double comp_v = 0.0, du, dl;
+        if (jl == o->nflat) {
+          double cl = vcc(o, wx, wy, ji, jk, jl + 1);
+          dl = (cl <= 0.0) ? A3(dens, ji, jk, jl) : A3(dens, ji, jk, jl + 1);
+          comp_v = -dl * cl * A3(o->area_z, ji, jk, jl + 1);
+        } else if (jl == o->nl) {
+          double cu = vcc(o, wx, wy, ji, jk, jl);
+          du = (cu <= 0.0) ? A3(dens, ji, jk, jl - 1) : A3(dens, ji, jk, jl);
+          comp_v = du * cu * A3(o->area_z, ji, jk, jl);
+        } else if (jl > o->nflat) {
+          double cu = vcc(o, wx, wy, ji, jk, jl);
+          du = (cu <= 0.0) ? A3(dens, ji, jk, jl - 1) : A3(dens, ji, jk, jl);
+          double cl = vcc(o, wx, wy, ji, jk, jl + 1);
+          dl = (cl <= 0.0) ? A3(dens, ji, jk, jl) : A3(dens, ji, jk, jl + 1);
+          comp_v = du * cu * A3(o->area_z, ji, jk, jl) - dl * cl * A3(o->area_z, ji, jk, jl + 1);
+        }
+        A3(res, ji, jk, jl) = (comp_h + comp_v) / A3(o->volume, ji, jk, jl);
+      }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_multiplications.F90                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/* scalar_times_vector_h, mo_multiplications.F90:18-66 */
+static void stv_h(const oracle *o, F3 s, F3 ix, F3 iy, F3 rx, F3 ry) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(rx, ji, jk, jl) = 0.5 * (A3(s, ji, jk - 1, jl) + A3(s, ji, jk, jl)) * A3(ix, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(ry, ji, jk, jl) = 0.5 * (A3(s, ji - 1, jk, jl) + A3(s, ji, jk, jl)) * A3(iy, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      A3(rx, ji, 1, jl) = 0.5 * (A3(s, ji, 1, jl) + A3(s, ji, nx, jl)) * A3(ix, ji, 1, jl);
+      A3(rx, ji, nx + 1, jl) = A3(rx, ji, 1, jl);
+    }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      A3(ry, 1, jk, jl) = 0.5 * (A3(s, 1, jk, jl) + A3(s, ny, jk, jl)) * A3(iy, 1, jk, jl);
+      A3(ry, ny + 1, jk, jl) = A3(ry, 1, jk, jl);
+    }
+}
+
+/* scalar_times_vector_h2, mo_multiplications.F90:68-114 */
+static void stv_h2(const oracle *o, F3 s, F3 vx, F3 vy) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(vx, ji, jk, jl) = 0.5 * (A3(s, ji, jk - 1, jl) + A3(s, ji, jk, jl)) * A3(vx, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(vy, ji, jk, jl) = 0.5 * (A3(s, ji - 1, jk, jl) + A3(s, ji, jk, jl)) * A3(vy, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      A3(vx, ji, 1, jl) = 0.5 * (A3(s, ji, 1, jl) + A3(s, ji, nx, jl)) * A3(vx, ji, 1, jl);
+      A3(vx, ji, nx + 1, jl) = A3(vx, ji, 1, jl);
+    }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      A3(vy, 1, jk, jl) = 0.5 * (A3(s, 1, jk, jl) + A3(s, ny, jk, jl)) * A3(vy, 1, jk, jl);
+      A3(vy, ny + 1, jk, jl) = A3(vy, 1, jk, jl);
+    }
+}
+
+/* theta_v_adv_3rd_order, mo_multiplications.F90:116-204 (scalar_placeholder holds theta_v) */
+static void theta_v_adv_3rd_order(oracle *o, F3 wu, F3 wv) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  const double c8 = 1.0 / 8.0;
+  grad_vert(o, o->scalar_ph, o->w_ph);
+  grad_hor(o, o->scalar_ph, o->u_ph, o->v_ph, o->w_ph);
+  div_h(o, o->u_ph, o->v_ph, o->flux_div);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji) {
+        A3(o->theta_v_u, ji, jk, jl) = 0.5 * (A3(o->scalar_ph, ji, jk - 1, jl) + A3(o->scalar_ph, ji, jk, jl));
+        if (A3(wu, ji, jk, jl) >= 0.0)
+          A3(o->theta_v_u, ji, jk, jl) = A3(o->theta_v_u, ji, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji, jk - 1, jl);
+        else
+          A3(o->theta_v_u, ji, jk, jl) = A3(o->theta_v_u, ji, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji, jk, jl);
+      }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 2; ji <= ny; ++ji) {
+        A3(o->theta_v_v, ji, jk, jl) = 0.5 * (A3(o->scalar_ph, ji, jk, jl) + A3(o->scalar_ph, ji - 1, jk, jl));
+        if (A3(wv, ji, jk, jl) >= 0.0)
+          A3(o->theta_v_v, ji, jk, jl) = A3(o->theta_v_v, ji, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji, jk, jl);
+        else
+          A3(o->theta_v_v, ji, jk, jl) = A3(o->theta_v_v, ji, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji - 1, jk, jl);
+      }
+  for (int ji = 1; ji <= ny; ++ji)
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(o->theta_v_u, ji, 1, jl) = 0.5 * (A3(o->scalar_ph, ji, nx, jl) + A3(o->scalar_ph, ji, 1, jl));
+      if (A3(wu, ji, 1, jl) >= 0.0)
+        A3(o->theta_v_u, ji, 1, jl) = A3(o->theta_v_u, ji, 1, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji, nx, jl);
+      else
+        A3(o->theta_v_u, ji, 1, jl) = A3(o->theta_v_u, ji, 1, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ji, 1, jl);
+      A3(o->theta_v_u, ji, nx + 1, jl) = A3(o->theta_v_u, ji, 1, jl);
+    }
+  for (int jk = 1; jk <= nx; ++jk)
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(o->theta_v_v, 1, jk, jl) = 0.5 * (A3(o->scalar_ph, 1, jk, jl) + A3(o->scalar_ph, ny, jk, jl));
+      if (A3(wv, 1, jk, jl) >= 0.0)
+        A3(o->theta_v_v, 1, jk, jl) = A3(o->theta_v_v, 1, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, 1, jk, jl);
+      else
+        A3(o->theta_v_v, 1, jk, jl) = A3(o->theta_v_v, 1, jk, jl) - c8 * o->dyn * o->dxn * A3(o->flux_div, ny, jk, jl);
+      A3(o->theta_v_v, ny + 1, jk, jl) = A3(o->theta_v_v, 1, jk, jl);
+    }
+}
+
+/* scalar_times_vector_h_upstream, mo_multiplications.F90:206-287 */
+static void stv_h_upstream(const oracle *o, F3 s, F3 ix, F3 iy, F3 rx, F3 ry) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(rx, ji, jk, jl) = (A3(ix, ji, jk, jl) >= 0.0 ? A3(s, ji, jk - 1, jl) : A3(s, ji, jk, jl)) * A3(ix, ji, jk, jl);
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(ry, ji, jk, jl) = (A3(iy, ji, jk, jl) >= 0.0 ? A3(s, ji, jk, jl) : A3(s, ji - 1, jk, jl)) * A3(iy, ji, jk, jl);
+  for (int ji = 1; ji <= ny; ++ji)
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(rx, ji, 1, jl) = (A3(ix, ji, 1, jl) >= 0.0 ? A3(s, ji, nx, jl) : A3(s, ji, 1, jl)) * A3(ix, ji, 1, jl);
+      A3(rx, ji, nx + 1, jl) = A3(rx, ji, 1, jl);
+    }
+  for (int jk = 1; jk <= nx; ++jk)
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(ry, 1, jk, jl) = (A3(iy, 1, jk, jl) >= 0.0 ? A3(s, 1, jk, jl) : A3(s, ny, jk, jl)) * A3(iy, 1, jk, jl);
+      A3(ry, ny + 1, jk, jl) = A3(ry, 1, jk, jl);
+    }
+}
+
+/* scalar_times_vector_v, mo_multiplications.F90:289-312 */
+static void stv_v(const oracle *o, F3 s, F3 iz, F3 rz) {
+  for (int jl = 2; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(rz, ji, jk, jl) = 0.5 * (A3(s, ji, jk, jl - 1) + A3(s, ji, jk, jl)) * A3(iz, ji, jk, jl);
+}
+
+/* scalar_times_vector_v2, mo_multiplications.F90:314-336 */
+static void stv_v2(const oracle *o, F3 s, F3 vz) {
+  for (int jl = 2; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(vz, ji, jk, jl) = 0.5 * (A3(s, ji, jk, jl - 1) + A3(s, ji, jk, jl)) * A3(vz, ji, jk, jl);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_inner_product.F90                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* inner_product, mo_inner_product.F90:16-50 */
+static void inner_product(const oracle *o, F3 u1, F3 v1, F3 w1, F3 u2, F3 v2, F3 w2, F3 out) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(out, ji, jk, jl) = IPW(o, 1, ji, jk, jl) * A3(u1, ji, jk + 1, jl) * A3(u2, ji, jk + 1, jl) +
+                              IPW(o, 2, ji, jk, jl) * A3(v1, ji, jk, jl) * A3(v2, ji, jk, jl) +
+                              IPW(o, 3, ji, jk, jl) * A3(u1, ji, jk, jl) * A3(u2, ji, jk, jl) +
+                              IPW(o, 4, ji, jk, jl) * A3(v1, ji + 1, jk, jl) * A3(v2, ji + 1, jk, jl) +
+                              IPW(o, 5, ji, jk, jl) * A3(w1, ji, jk, jl) * A3(w2, ji, jk, jl) +
+                              IPW(o, 6, ji, jk, jl) * A3(w1, ji, jk, jl + 1) * A3(w2, ji, jk, jl + 1);
+}
+
+/* w_free_slip, mo_inner_product.F90:52-75 */
+static void w_free_slip(const oracle *o, ostate *s) {
+  int nl = o->nl;
+  for (int jk = 1; jk <= o->nx; ++jk)
+    for (int ji = 1; ji <= o->ny; ++ji)
+      A3(s->wind_w, ji, jk, nl + 1) = IPW(o, 1, ji, jk, nl) * A3(s->wind_u, ji, jk + 1, nl) * A3(o->slope_x, ji, jk + 1, nl) +
+                                      IPW(o, 2, ji, jk, nl) * A3(s->wind_v, ji, jk, nl) * A3(o->slope_y, ji, jk, nl) +
+                                      IPW(o, 3, ji, jk, nl) * A3(s->wind_u, ji, jk, nl) * A3(o->slope_x, ji, jk, nl) +
+                                      IPW(o, 4, ji, jk, nl) * A3(s->wind_v, ji + 1, jk, nl) * A3(o->slope_y, ji + 1, jk, nl);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_vorticities.F90                                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* rel_vort_z_local, mo_vorticities.F90:423-538 */
+static double rel_vort_z_local(const oracle *o, const ostate *s, int ji, int jk, int jl) {
+  int ny = o->ny, nx = o->nx;
+  int j_i[4] = {ji, ji - 1, ji, ji};
+  int j_k[4] = {jk, jk, jk - 1, jk};
+  const int sgn[4] = {1, -1, -1, 1};
+  double r = 0.0;
+  if (ji == 1 || jk == 1 || ji == ny + 1 || jk == nx + 1) {
+    j_i[0] = ji;
+    j_k[0] = (jk == nx + 1) ? 1 : jk;
+    j_i[1] = (ji == 1) ? ny : ji - 1;
+    j_k[1] = jk;
+    j_i[2] = ji;
+    j_k[2] = (jk == 1) ? nx : jk - 1;
+    j_i[3] = (ji == ny + 1) ? 1 : ji;
+    j_k[3] = jk;
+  }
+  if (jl <= o->nflat) {
+    r = r + sgn[0] * A3(o->dy, j_i[0], j_k[0], jl) * A3(s->wind_v, j_i[0], j_k[0], jl) +
+        sgn[1] * A3(o->dx, j_i[1], j_k[1], jl) * A3(s->wind_u, j_i[1], j_k[1], jl) +
+        sgn[2] * A3(o->dy, j_i[2], j_k[2], jl) * A3(s->wind_v, j_i[2], j_k[2], jl) +
+        sgn[3] * A3(o->dx, j_i[3], j_k[3], jl) * A3(s->wind_u, j_i[3], j_k[3], jl);
+  } else {
+    for (int jm = 0; jm < 4; ++jm) {
+      double l_rescale, delta_z, vg;
+      int sh;
+      if (jm == 0 || jm == 2) {
+        l_rescale = (R_E + A3(o->z_area_dual_z, ji, jk, jl)) / (R_E + A3(o->z_v, j_i[jm], j_k[jm], jl));
+        delta_z = A3(o->z_area_dual_z, ji, jk, jl) - A3(o->z_v, j_i[jm], j_k[jm], jl);
+        sh = 1;
+        if (delta_z > 0.0 || jl == o->nl) sh = -1;
+        if (jl == 1) sh = 1;
+        vg = (A3(s->wind_v, j_i[jm], j_k[jm], jl) - A3(s->wind_v, j_i[jm], j_k[jm], jl + sh)) /
+             (A3(o->z_v, j_i[jm], j_k[jm], jl) - A3(o->z_v, j_i[jm], j_k[jm], jl + sh));
+        r = r + sgn[jm] * l_rescale * A3(o->dy, j_i[jm], j_k[jm], jl) * (A3(s->wind_v, j_i[jm], j_k[jm], jl) + delta_z * vg);
+      } else {
+        l_rescale = (R_E + A3(o->z_area_dual_z, ji, jk, jl)) / (R_E + A3(o->z_u, j_i[jm], j_k[jm], jl));
+        delta_z = A3(o->z_area_dual_z, ji, jk, jl) - A3(o->z_u, j_i[jm], j_k[jm], jl);
+        sh = 1;
+        if (delta_z > 0.0 || jl == o->nl) sh = -1;
+        if (jl == 1) sh = 1;
+        vg = (A3(s->wind_u, j_i[jm], j_k[jm], jl) - A3(s->wind_u, j_i[jm], j_k[jm], jl + sh)) /
+             (A3(o->z_u, j_i[jm], j_k[jm], jl) - A3(o->z_u, j_i[jm], j_k[jm], jl + sh));
+        r = r + sgn[jm] * l_rescale * A3(o->dx, j_i[jm], j_k[jm], jl) * (A3(s->wind_u, j_i[jm], j_k[jm], jl) + delta_z * vg);
+      }
+    }
+  }
+  return r;
+}
+
+/* rel_vort, mo_vorticities.F90:19-187 (periodic branches) */
+static void rel_vort(oracle *o, const ostate *s) {
+  int ny = o->ny, nx = o->nx, nl = o->nl, nlev = o->nlev;
+  F3 zx = o->zeta_x, zy = o->zeta_y, zz = o->zeta_z;
+  for (int jl = 2; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(zx, ji, jk, jl) = A3(o->dz, ji - 1, jk, jl) * A3(s->wind_w, ji - 1, jk, jl) -
+                             A3(o->dy, ji, jk, jl - 1) * hcov_y(o, s->wind_v, s->wind_w, ji, jk, jl - 1) -
+                             A3(o->dz, ji, jk, jl) * A3(s->wind_w, ji, jk, jl) +
+                             A3(o->dy, ji, jk, jl) * hcov_y(o, s->wind_v, s->wind_w, ji, jk, jl);
+      A3(zx, 1, jk, jl) = A3(o->dz, ny, jk, jl) * A3(s->wind_w, ny, jk, jl) -
+                          A3(o->dy, 1, jk, jl - 1) * hcov_y(o, s->wind_v, s->wind_w, 1, jk, jl - 1) -
+                          A3(o->dz, 1, jk, jl) * A3(s->wind_w, 1, jk, jl) +
+                          A3(o->dy, 1, jk, jl) * hcov_y(o, s->wind_v, s->wind_w, 1, jk, jl);
+      A3(zx, ny + 1, jk, jl) = A3(zx, 1, jk, jl);
+      for (int ji = 1; ji <= ny + 1; ++ji)
+        A3(zx, ji, jk, nlev) = -A3(o->dy, ji, jk, nl) * hcov_y(o, s->wind_v, s->wind_w, ji, jk, nl);
+    }
+  if (o->lfreeslip) {
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(zx, ji, jk, nlev) = A3(zx, ji, jk, nlev) + 0.5 * A3(o->dz, ji - 1, jk, nlev) * A3(s->wind_w, ji - 1, jk, nlev) -
+                               0.5 * A3(o->dz, ji, jk, nlev) * A3(s->wind_w, ji, jk, nlev) +
+                               A3(o->dy, ji, jk, nl) *
+                                   (A3(s->wind_v, ji, jk, nl) +
+                                    (A3(s->wind_v, ji, jk, nl - 1) - A3(s->wind_v, ji, jk, nl)) / (A3(o->z_v, ji, jk, nl - 1) - A3(o->z_v, ji, jk, nl)) *
+                                        (0.5 * (A3(o->z_w, ji - 1, jk, nlev) + A3(o->z_w, ji, jk, nlev)) - A3(o->z_v, ji, jk, nl)) +
+                                    0.5 * (A3(s->wind_w, ji - 1, jk, nlev) + A3(s->wind_w, ji, jk, nlev)) *
+                                        (A3(o->z_w, ji - 1, jk, nlev) - A3(o->z_w, ji, jk, nlev)) / A3(o->dy, ji, jk, nl));
+      A3(zx, 1, jk, nlev) = A3(zx, 1, jk, nlev) + 0.5 * A3(o->dz, ny, jk, nlev) * A3(s->wind_w, ny, jk, nlev) -
+                            0.5 * A3(o->dz, 1, jk, nlev) * A3(s->wind_w, 1, jk, nlev) +
+                            A3(o->dy, 1, jk, nl) *
+                                (A3(s->wind_v, 1, jk, nl) +
+                                 (A3(s->wind_v, 1, jk, nl - 1) - A3(s->wind_v, 1, jk, nl)) / (A3(o->z_v, 1, jk, nl - 1) - A3(o->z_v, 1, jk, nl)) *
+                                     (0.5 * (A3(o->z_w, ny, jk, nlev) + A3(o->z_w, 1, jk, nlev)) - A3(o->z_v, 1, jk, nl)) +
+                                 0.5 * (A3(s->wind_w, ny, jk, nlev) + A3(s->wind_w, 1, jk, nlev)) *
+                                     (A3(o->z_w, ny, jk, nlev) - A3(o->z_w, 1, jk, nlev)) / A3(o->dy, 1, jk, nl));
+      A3(zx, ny + 1, jk, nlev) = A3(zx, 1, jk, nlev);
+    }
+  }
+  for (int jl = 1; jl <= nlev; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny + 1; ++ji) A3(zx, ji, jk, jl) = A3(zx, ji, jk, jl) / A3(o->area_dual_x, ji, jk, jl);
+  for (int jk = 1; jk <= nx; ++jk)
+    for (int ji = 1; ji <= ny + 1; ++ji) A3(zx, ji, jk, 1) = A3(zx, ji, jk, 2);
+
+  for (int jl = 2; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(zy, ji, jk, jl) = -A3(o->dz, ji, jk, jl) * A3(s->wind_w, ji, jk, jl) -
+                             A3(o->dx, ji, jk, jl) * hcov_x(o, s->wind_u, s->wind_w, ji, jk, jl) +
+                             A3(o->dz, ji, jk - 1, jl) * A3(s->wind_w, ji, jk - 1, jl) +
+                             A3(o->dx, ji, jk, jl - 1) * hcov_x(o, s->wind_u, s->wind_w, ji, jk, jl - 1);
+      A3(zy, ji, 1, jl) = -A3(o->dz, ji, 1, jl) * A3(s->wind_w, ji, 1, jl) -
+                          A3(o->dx, ji, 1, jl) * hcov_x(o, s->wind_u, s->wind_w, ji, 1, jl) +
+                          A3(o->dz, ji, nx, jl) * A3(s->wind_w, ji, nx, jl) +
+                          A3(o->dx, ji, 1, jl - 1) * hcov_x(o, s->wind_u, s->wind_w, ji, 1, jl - 1);
+      A3(zy, ji, nx + 1, jl) = A3(zy, ji, 1, jl);
+      for (int jk = 1; jk <= nx + 1; ++jk)
+        A3(zy, ji, jk, nlev) = A3(o->dx, ji, jk, nl) * hcov_x(o, s->wind_u, s->wind_w, ji, jk, nl);
+    }
+  if (o->lfreeslip) {
+    for (int ji = 1; ji <= ny; ++ji) {
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(zy, ji, jk, nlev) = A3(zy, ji, jk, nlev) + 0.5 * A3(o->dz, ji, jk - 1, nlev) * A3(s->wind_w, ji, jk - 1, nlev) -
+                               0.5 * A3(o->dz, ji, jk, nlev) * A3(s->wind_w, ji, jk, nlev) -
+                               A3(o->dx, ji, jk, nl) *
+                                   (A3(s->wind_u, ji, jk, nl) +
+                                    (A3(s->wind_u, ji, jk, nl - 1) - A3(s->wind_u, ji, jk, nl)) / (A3(o->z_u, ji, jk, nl - 1) - A3(o->z_u, ji, jk, nl)) *
+                                        (0.5 * (A3(o->z_w, ji, jk - 1, nlev) + A3(o->z_w, ji, jk, nlev)) - A3(o->z_u, ji, jk, nl)) +
+                                    0.5 * (A3(s->wind_w, ji, jk - 1, nlev) + A3(s->wind_w, ji, jk, nlev)) *
+                                        (A3(o->z_w, ji, jk, nlev) - A3(o->z_w, ji, jk - 1, nlev)) / A3(o->dx, ji, jk, nl));
+      A3(zy, ji, 1, nlev) = A3(zy, ji, 1, nlev) + 0.5 * A3(o->dz, ji, nx, nlev) * A3(s->wind_w, ji, nx, nlev) -
+                            0.5 * A3(o->dz, ji, 1, nlev) * A3(s->wind_w, ji, 1, nlev) -
+                            A3(o->dx, ji, 1, nl) *
+                                (A3(s->wind_u, ji, 1, nl) +
+                                 (A3(s->wind_u, ji, 1, nl - 1) - A3(s->wind_u, ji, 1, nl)) / (A3(o->z_u, ji, 1, nl - 1) - A3(o->z_u, ji, 1, nl)) *
+                                     (0.5 * (A3(o->z_w, ji, nx, nlev) + A3(o->z_w, ji, 1, nlev)) - A3(o->z_u, ji, 1, nl)) +
+                                 0.5 * (A3(s->wind_w, ji, nx, nlev) + A3(s->wind_w, ji, 1, nlev)) *
+                                     (A3(o->z_w, ji, 1, nlev) - A3(o->z_w, ji, nx, nlev)) / A3(o->dx, ji, 1, nl));
+      A3(zy, ji, nx + 1, nlev) = A3(zy, ji, 1, nlev);
+    }
+  }
+  for (int jl = 1; jl <= nlev; ++jl)
+    for (int jk = 1; jk <= nx + 1; ++jk)
+      for (int ji = 1; ji <= ny; ++ji) A3(zy, ji, jk, jl) = A3(zy, ji, jk, jl) / A3(o->area_dual_y, ji, jk, jl);
+  for (int jk = 1; jk <= nx + 1; ++jk)
+    for (int ji = 1; ji <= ny; ++ji) A3(zy, ji, jk, 1) = A3(zy, ji, jk, 2);
+
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx + 1; ++jk)
+      for (int ji = 1; ji <= ny + 1; ++ji)
+        A3(zz, ji, jk, jl) = rel_vort_z_local(o, s, ji, jk, jl) / A3(o->area_dual_z, ji, jk, jl);
+}
+
+static double rhog(const oracle *o, const ostate *s, int ji, int jk, int jl) {
+  return s->rho[((size_t)(ji - 1)) + (size_t)o->ny * (((size_t)(jk - 1)) + (size_t)o->nx * (((size_t)(jl - 1)) + (size_t)o->nl * (size_t)o->ncond))];
+}
+
+/* eta_x denominator pieces, mo_vorticities.F90:240-266 (ja = ji-1 wrapped, jb = ji) */
+static double pv_den_h(const oracle *o, const ostate *s, int ia, int ka, int ib, int kb, int jl) {
+  if (jl == 1) {
+    return 0.5 * (rhog(o, s, ia, ka, jl) +
+                  (o->toa - A3(o->z_scalar, ia, ka, jl)) * (rhog(o, s, ia, ka, jl) - rhog(o, s, ia, ka, jl + 1)) /
+                      (A3(o->z_scalar, ia, ka, jl) - A3(o->z_scalar, ia, ka, jl + 1)) +
+                  rhog(o, s, ib, kb, jl) +
+                  (o->toa - A3(o->z_scalar, ib, kb, jl)) * (rhog(o, s, ib, kb, jl) - rhog(o, s, ib, kb, jl + 1)) /
+                      (A3(o->z_scalar, ib, kb, jl) - A3(o->z_scalar, ib, kb, jl + 1)));
+  } else if (jl == o->nlev) {
+    return 0.5 * (rhog(o, s, ia, ka, jl - 1) +
+                  (A3(o->z_w, ia, ka, jl) - A3(o->z_scalar, ia, ka, jl - 1)) * (rhog(o, s, ia, ka, jl - 2) - rhog(o, s, ia, ka, jl - 1)) /
+                      (A3(o->z_scalar, ia, ka, jl - 2) - A3(o->z_scalar, ia, ka, jl - 1)) +
+                  rhog(o, s, ib, kb, jl - 1) +
+                  (A3(o->z_w, ib, kb, jl) - A3(o->z_scalar, ib, kb, jl - 1)) * (rhog(o, s, ib, kb, jl - 2) - rhog(o, s, ib, kb, jl - 1)) /
+                      (A3(o->z_scalar, ib, kb, jl - 2) - A3(o->z_scalar, ib, kb, jl - 1)));
+  }
+  return 0.25 * (rhog(o, s, ia, ka, jl - 1) + rhog(o, s, ib, kb, jl - 1) + rhog(o, s, ia, ka, jl) + rhog(o, s, ib, kb, jl));
+}
+
+/* calc_pot_vort, mo_vorticities.F90:189-421 */
+static void calc_pot_vort(oracle *o, const ostate *s) {
+  int ny = o->ny, nx = o->nx, nl = o->nl, nlev = o->nlev;
+  if (!o->llinear) {
+    rel_vort(o, s);
+  } else {
+    memset(o->zeta_x.p, 0, sizeof(double) * (size_t)(ny + 1) * nx * nlev);
+    memset(o->zeta_y.p, 0, sizeof(double) * (size_t)ny * (nx + 1) * nlev);
+    memset(o->zeta_z.p, 0, sizeof(double) * (size_t)(ny + 1) * (nx + 1) * nl);
+  }
+  if (o->lcorio) {
+    for (int jl = 1; jl <= nlev; ++jl) {
+      for (int jk = 1; jk <= nx; ++jk)
+        for (int ji = 1; ji <= ny + 1; ++ji) A3(o->eta_x, ji, jk, jl) = A3(o->zeta_x, ji, jk, jl) + A3(o->fvec_x, ji, jk, 1);
+      for (int jk = 1; jk <= nx + 1; ++jk)
+        for (int ji = 1; ji <= ny; ++ji) A3(o->eta_y, ji, jk, jl) = A3(o->zeta_y, ji, jk, jl) + A3(o->fvec_y, ji, jk, 1);
+    }
+    for (int jl = 1; jl <= nl; ++jl)
+      for (int jk = 1; jk <= nx + 1; ++jk)
+        for (int ji = 1; ji <= ny + 1; ++ji) A3(o->eta_z, ji, jk, jl) = A3(o->zeta_z, ji, jk, jl) + A3(o->fvec_z, ji, jk, 1);
+  } else {
+    memcpy(o->eta_x.p, o->zeta_x.p, sizeof(double) * (size_t)(ny + 1) * nx * nlev);
+    memcpy(o->eta_y.p, o->zeta_y.p, sizeof(double) * (size_t)ny * (nx + 1) * nlev);
+    memcpy(o->eta_z.p, o->zeta_z.p, sizeof(double) * (size_t)(ny + 1) * (nx + 1) * nl);
+  }
+  for (int jl = 1; jl <= nlev; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int ji = 2; ji <= ny; ++ji) A3(o->eta_x, ji, jk, jl) = A3(o->eta_x, ji, jk, jl) / pv_den_h(o, s, ji - 1, jk, ji, jk, jl);
+      A3(o->eta_x, 1, jk, jl) = A3(o->eta_x, 1, jk, jl) / pv_den_h(o, s, ny, jk, 1, jk, jl);
+      A3(o->eta_x, ny + 1, jk, jl) = A3(o->eta_x, 1, jk, jl);
+    }
+  for (int jl = 1; jl <= nlev; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      for (int jk = 2; jk <= nx; ++jk) A3(o->eta_y, ji, jk, jl) = A3(o->eta_y, ji, jk, jl) / pv_den_h(o, s, ji, jk - 1, ji, jk, jl);
+      A3(o->eta_y, ji, 1, jl) = A3(o->eta_y, ji, 1, jl) / pv_den_h(o, s, ji, nx, ji, 1, jl);
+      A3(o->eta_y, ji, nx + 1, jl) = A3(o->eta_y, ji, 1, jl);
+    }
+  /* vertical vorticity, :378-419 */
+  for (int jl = 1; jl <= nl; ++jl) {
+    for (int ji = 2; ji <= ny; ++ji) {
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(o->eta_z, ji, jk, jl) = A3(o->eta_z, ji, jk, jl) /
+                                   (0.25 * (rhog(o, s, ji, jk - 1, jl) + rhog(o, s, ji, jk, jl) + rhog(o, s, ji - 1, jk, jl) + rhog(o, s, ji - 1, jk - 1, jl)));
+      A3(o->eta_z, ji, 1, jl) = A3(o->eta_z, ji, 1, jl) /
+                                (0.25 * (rhog(o, s, ji, nx, jl) + rhog(o, s, ji, 1, jl) + rhog(o, s, ji - 1, 1, jl) + rhog(o, s, ji - 1, nx, jl)));
+      A3(o->eta_z, ji, nx + 1, jl) = A3(o->eta_z, ji, 1, jl);
+    }
+    for (int jk = 2; jk <= nx; ++jk) {
+      A3(o->eta_z, 1, jk, jl) = A3(o->eta_z, 1, jk, jl) /
+                                (0.25 * (rhog(o, s, 1, jk - 1, jl) + rhog(o, s, 1, jk, jl) + rhog(o, s, ny, jk, jl) + rhog(o, s, ny, jk - 1, jl)));
+      A3(o->eta_z, ny + 1, jk, jl) = A3(o->eta_z, 1, jk, jl);
+    }
+    A3(o->eta_z, 1, 1, jl) = A3(o->eta_z, 1, 1, jl) /
+                             (0.25 * (rhog(o, s, 1, nx, jl) + rhog(o, s, 1, 1, jl) + rhog(o, s, ny, 1, jl) + rhog(o, s, ny, nx, jl)));
+    A3(o->eta_z, 1, nx + 1, jl) = A3(o->eta_z, 1, 1, jl);
+    A3(o->eta_z, ny + 1, 1, jl) = A3(o->eta_z, 1, 1, jl);
+    A3(o->eta_z, ny + 1, nx + 1, jl) = A3(o->eta_z, 1, 1, jl);
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_vorticity_flux.F90                                                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* TRSK u-term for one u point, mo_vorticity_flux.F90:33-45 (km = jk-1 wrapped, kp = jk+1) */
+static double trsk_u_term(const oracle *o, int ji, int jk, int km, int kp, int jl) {
+  F3 ez = o->eta_z, up = o->u_ph, vp = o->v_ph;
+  return TRSKU(o, 1, ji) * A3(vp, ji, km, jl) * 0.25 * (A3(ez, ji, km, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl)) +
+         TRSKU(o, 2, ji) * A3(up, ji, km, jl) * 0.25 * (A3(ez, ji, km, jl) + A3(ez, ji + 1, km, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl)) +
+         TRSKU(o, 3, ji) * A3(vp, ji + 1, km, jl) * 0.25 * (A3(ez, ji + 1, km, jl) + A3(ez, ji + 1, jk, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl)) +
+         TRSKU(o, 3, ji) * A3(vp, ji + 1, jk, jl) * 0.25 * (A3(ez, ji + 1, jk, jl) + A3(ez, ji + 1, kp, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl)) +
+         TRSKU(o, 2, ji) * A3(up, ji, kp, jl) * 0.25 * (A3(ez, ji + 1, kp, jl) + A3(ez, ji, kp, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl)) +
+         TRSKU(o, 1, ji) * A3(vp, ji, jk, jl) * 0.25 * (A3(ez, ji, jk, jl) + A3(ez, ji, kp, jl) + A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl));
+}
+
+/* TRSK v-term, mo_vorticity_flux.F90:72-80 (im = ji-1 wrapped, tw = row used for the weights) */
+static double trsk_v_term(const oracle *o, int ji, int im, int tw, int jk, int jl) {
+  F3 ez = o->eta_z, up = o->u_ph;
+  return TRSKV(o, 1, tw) * A3(up, ji, jk, jl) * 0.25 * (A3(ez, ji, jk, jl) + A3(ez, ji + 1, jk, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk + 1, jl)) +
+         TRSKV(o, 1, tw) * A3(up, ji, jk + 1, jl) * 0.25 * (A3(ez, ji, jk + 1, jl) + A3(ez, ji + 1, jk + 1, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk + 1, jl)) +
+         TRSKV(o, 2, tw) * A3(up, im, jk + 1, jl) * 0.25 * (A3(ez, im, jk + 1, jl) + A3(ez, ji, jk + 1, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk + 1, jl)) +
+         TRSKV(o, 2, tw) * A3(up, im, jk, jl) * 0.25 * (A3(ez, im, jk, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk, jl) + A3(ez, ji, jk + 1, jl));
+}
+
+/* calc_vorticity_flux_term, mo_vorticity_flux.F90:16-171 */
+static void calc_vorticity_flux_term(oracle *o) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  F3 wp = o->w_ph;
+  for (int ji = 1; ji <= ny; ++ji) {
+    for (int jk = 2; jk <= nx; ++jk)
+      for (int jl = 1; jl <= nl; ++jl) A3(o->pvt_x, ji, jk, jl) = trsk_u_term(o, ji, jk, jk - 1, jk + 1, jl);
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(o->pvt_x, ji, 1, jl) = trsk_u_term(o, ji, 1, nx, 2, jl);
+      A3(o->pvt_x, ji, nx + 1, jl) = A3(o->pvt_x, ji, 1, jl);
+    }
+  }
+  for (int jk = 1; jk <= nx; ++jk) {
+    for (int ji = 2; ji <= ny; ++ji)
+      for (int jl = 1; jl <= nl; ++jl) A3(o->pvt_y, ji, jk, jl) = trsk_v_term(o, ji, ji - 1, ji, jk, jl);
+    for (int jl = 1; jl <= nl; ++jl) {
+      A3(o->pvt_y, 1, jk, jl) = trsk_v_term(o, 1, ny, 1, jk, jl);
+      A3(o->pvt_y, ny + 1, jk, jl) = A3(o->pvt_y, 1, jk, jl);
+    }
+  }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(o->pvt_x, ji, jk, jl) = A3(o->pvt_x, ji, jk, jl) - 0.5 * IPW(o, 5, ji, jk - 1, jl) * A3(wp, ji, jk - 1, jl) * A3(o->eta_y, ji, jk, jl) -
+                                   0.5 * IPW(o, 6, ji, jk - 1, jl) * A3(wp, ji, jk - 1, jl + 1) * A3(o->eta_y, ji, jk, jl + 1) -
+                                   0.5 * IPW(o, 5, ji, jk, jl) * A3(wp, ji, jk, jl) * A3(o->eta_y, ji, jk, jl) -
+                                   0.5 * IPW(o, 6, ji, jk, jl) * A3(wp, ji, jk, jl + 1) * A3(o->eta_y, ji, jk, jl + 1);
+      A3(o->pvt_x, ji, 1, jl) = A3(o->pvt_x, ji, 1, jl) - 0.5 * IPW(o, 5, ji, nx, jl) * A3(wp, ji, nx, jl) * A3(o->eta_y, ji, 1, jl) -
+                                0.5 * IPW(o, 6, ji, nx, jl) * A3(wp, ji, nx, jl + 1) * A3(o->eta_y, ji, 1, jl + 1) -
+                                0.5 * IPW(o, 5, ji, 1, jl) * A3(wp, ji, 1, jl) * A3(o->eta_y, ji, 1, jl) -
+                                0.5 * IPW(o, 6, ji, 1, jl) * A3(wp, ji, 1, jl + 1) * A3(o->eta_y, ji, 1, jl + 1);
+      A3(o->pvt_x, ji, nx + 1, jl) = A3(o->pvt_x, ji, 1, jl);
+    }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int ji = 2; ji <= ny; ++ji)
+        A3(o->pvt_y, ji, jk, jl) = A3(o->pvt_y, ji, jk, jl) + 0.5 * IPW(o, 5, ji - 1, jk, jl) * A3(wp, ji - 1, jk, jl) * A3(o->eta_x, ji, jk, jl) +
+                                   0.5 * IPW(o, 6, ji - 1, jk, jl) * A3(wp, ji - 1, jk, jl + 1) * A3(o->eta_x, ji, jk, jl + 1) +
+                                   0.5 * IPW(o, 5, ji, jk, jl) * A3(wp, ji, jk, jl) * A3(o->eta_x, ji, jk, jl) +
+                                   0.5 * IPW(o, 6, ji, jk, jl) * A3(wp, ji, jk, jl + 1) * A3(o->eta_x, ji, jk, jl + 1);
+      A3(o->pvt_y, 1, jk, jl) = A3(o->pvt_y, 1, jk, jl) + 0.5 * IPW(o, 5, ny, jk, jl) * A3(wp, ny, jk, jl) * A3(o->eta_x, 1, jk, jl) +
+                                0.5 * IPW(o, 6, ny, jk, jl) * A3(wp, ny, jk, jl + 1) * A3(o->eta_x, 1, jk, jl + 1) +
+                                0.5 * IPW(o, 5, 1, jk, jl) * A3(wp, 1, jk, jl) * A3(o->eta_x, 1, jk, jl) +
+                                0.5 * IPW(o, 6, 1, jk, jl) * A3(wp, 1, jk, jl + 1) * A3(o->eta_x, 1, jk, jl + 1);
+      A3(o->pvt_y, ny + 1, jk, jl) = A3(o->pvt_y, 1, jk, jl);
+    }
+  for (int jl = 2; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(o->pvt_z, ji, jk, jl) =
+            0.5 * (IPW(o, 1, ji, jk, jl - 1) * A3(o->u_ph, ji, jk + 1, jl - 1) * A3(o->eta_y, ji, jk + 1, jl) -
+                   IPW(o, 2, ji, jk, jl - 1) * A3(o->v_ph, ji, jk, jl - 1) * A3(o->eta_x, ji, jk, jl) +
+                   IPW(o, 3, ji, jk, jl - 1) * A3(o->u_ph, ji, jk, jl - 1) * A3(o->eta_y, ji, jk, jl) -
+                   IPW(o, 4, ji, jk, jl - 1) * A3(o->v_ph, ji + 1, jk, jl - 1) * A3(o->eta_x, ji + 1, jk, jl) +
+                   IPW(o, 1, ji, jk, jl) * A3(o->u_ph, ji, jk + 1, jl) * A3(o->eta_y, ji, jk + 1, jl) -
+                   IPW(o, 2, ji, jk, jl) * A3(o->v_ph, ji, jk, jl) * A3(o->eta_x, ji, jk, jl) +
+                   IPW(o, 3, ji, jk, jl) * A3(o->u_ph, ji, jk, jl) * A3(o->eta_y, ji, jk, jl) -
+                   IPW(o, 4, ji, jk, jl) * A3(o->v_ph, ji + 1, jk, jl) * A3(o->eta_x, ji + 1, jk, jl));
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* mo_derived.F90 / mo_dictionary.F90                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* c_p_water, mo_dictionary.F90:112-145 (integer powers by binary exponentiation) */
+static double c_p_water(double temperature) {
+  double temp_c = temperature - T_0, r;
+  if (temp_c >= 0.0) {
+    if (temp_c > 35.0) temp_c = 35.0;
+    double d = temp_c - 35.0, d2 = d * d, d4 = d2 * d2;
+    r = 0.9979 + 3.1e-6 * d2 + 3.8e-9 * d4;
+  } else {
+    if (temp_c < -37.0) temp_c = -37.0;
+    double t2 = temp_c * temp_c, t3 = t2 * temp_c, t4 = t2 * t2;
+    r = 1.000938 - 2.7052e-3 * temp_c - 2.3235e-5 * t2 + 4.3778e-6 * t3 + 2.7136e-7 * t4;
+  }
+  return 4186.8 * r;
+}
+
+/* c_p_ice, mo_dictionary.F90:147-173 */
+static double c_p_ice(double temperature) {
+  double t = temperature;
+  if (t > T_0) t = T_0;
+  if (t < 20.0) t = 20.0;
+  double q = t / 125.1;
+  double r = -2.0572 + 0.14644 * t + 0.06163 * t * exp(-(q * q));
+  return r / M_V;
+}
+
+/* c_p_cond, mo_dictionary.F90:175-189 */
+static double c_p_cond(int jc, double temperature) {
+  return ((jc - 1) % 2 == 0) ? c_p_ice(temperature) : c_p_water(temperature);
+}
+
+/* temperature_diagnostics, mo_derived.F90:18-38 */
+static void temperature_diagnostics(oracle *o, const ostate *s) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  const double *tb = o->theta_v_bg.p, *tp = s->theta_v_pert.p, *eb = o->exner_bg.p, *ep = s->exner_pert.p;
+  double *t = o->temperature.p;
+  if (!o->lmoist) {
+    for (size_t i = 0; i < n; ++i) t[i] = (tb[i] + tp[i]) * (eb[i] + ep[i]);
+  } else {
+    const double *rg = s->rho + (size_t)o->ncond * n, *rv = s->rho + (size_t)(o->ncond + 1) * n;
+    for (size_t i = 0; i < n; ++i) t[i] = (tb[i] + tp[i]) * (eb[i] + ep[i]) / (1.0 + rv[i] / rg[i] * (M_D / M_V - 1.0));
+  }
+}
+
+/* c_v_mass_weighted_air, mo_derived.F90:157-187 */
+static double c_v_mass_weighted_air(const oracle *o, const double *rho, int ji, int jk, int jl) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  size_t idx = (size_t)(ji - 1) + (size_t)o->ny * ((size_t)(jk - 1) + (size_t)o->nx * (size_t)(jl - 1));
+  double r = 0.0;
+  for (int jc = 1; jc <= o->ncond; ++jc) r = r + rho[idx + (size_t)(jc - 1) * n] * c_p_cond(jc, o->temperature.p[idx]);
+  if (o->lmoist) {
+    r = r + (rho[idx + (size_t)o->ncond * n] - rho[idx + (size_t)(o->ncond + 1) * n]) * C_D_V;
+    r = r + rho[idx + (size_t)(o->ncond + 1) * n] * C_V_V;
+  } else {
+    r = rho[idx + (size_t)o->ncond * n] * C_D_V;
+  }
+  return r;
+}
+
+/* calc_diff_coeff, mo_derived.F90:189-216 */
+static double calc_diff_coeff(double temperature, double density) {
+  const double particle_radius = 130e-12;
+  double particle_mass = M_D / N_A;
+  double thermal_velocity = sqrt(8.0 * K_B * temperature / (MPI * particle_mass));
+  double particle_density = density / particle_mass;
+  double cross_section = 4.0 * MPI * (particle_radius * particle_radius);
+  double mean_free_path = 1.0 / (sqrt(2.0) * particle_density * cross_section);
+  return (1.0 / 3.0) * thermal_velocity * mean_free_path;
+}
+
+/* v_fall_liquid, mo_derived.F90:218-263 */
+static double v_fall_liquid(const oracle *o, const ostate *s, double radius, int ji, int jk, int jl) {
+  const double reynolds_crit = 10.0;
+  double g;
+  if (jl == 1)
+    g = A3(o->gravity_m_v, ji, jk, jl + 1);
+  else if (jl == o->nl)
+    g = A3(o->gravity_m_v, ji, jk, jl);
+  else
+    g = IPW(o, 5, ji, jk, jl) * A3(o->gravity_m_v, ji, jk, jl) + IPW(o, 6, ji, jk, jl) * A3(o->gravity_m_v, ji, jk, jl + 1);
+  double rho = rhog(o, s, ji, jk, jl);
+  double kin = calc_diff_coeff(A3(o->temperature, ji, jk, jl), rho);
+  double v = 2.0 * (radius * radius) * (RHO_H2O - rho) * g / (9.0 * rho * kin);
+  double re = v * radius / kin;
+  if (re > reynolds_crit) {
+    const double c_w = 0.8;
+    v = sqrt(8.0 * radius * RHO_H2O * g / (3.0 * rho * c_w));
+  }
+  return v;
+}
+
+/* v_fall_solid, mo_derived.F90:265-283 (the radius is compared against a default-real literal) */
+static double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
+
+/* ------------------------------------------------------------------------------------------ */
+/* time_stepping/mo_p_grad.F90                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+static double rho_sum(const oracle *o, const double *rho, size_t i) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  double r = 0.0;
+  for (int jc = 1; jc <= o->ncond + 1; ++jc) r = r + rho[i + (size_t)(jc - 1) * n];
+  return r;
+}
+
+/* manage_p_grad, mo_p_grad.F90:20-104 (luse_bg_state only) */
+static void manage_p_grad(oracle *o, const ostate *s, int lfirst) {
+  size_t nu = (size_t)o->ny * (o->nx + 1) * o->nl, nv = (size_t)(o->ny + 1) * o->nx * o->nl, n = (size_t)o->ny * o->nx * o->nl;
+  if (!lfirst) {
+    for (size_t i = 0; i < nu; ++i) o->pold_u.p[i] = -o->pnl_u.p[i] - 1.0 * o->pl_u.p[i];
+    for (size_t i = 0; i < nv; ++i) o->pold_v.p[i] = -o->pnl_v.p[i] - 1.0 * o->pl_v.p[i];
+  }
+  grad_vert(o, s->exner_pert, o->pnl_w);
+  grad_hor(o, s->exner_pert, o->pnl_u, o->pnl_v, o->pnl_w);
+  for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = o->theta_v_bg.p[i] + s->theta_v_pert.p[i];
+  if (o->order == 2) {
+    stv_h2(o, o->scalar_ph, o->pnl_u, o->pnl_v);
+  } else {
+    theta_v_adv_3rd_order(o, s->wind_u, s->wind_v);
+    for (size_t i = 0; i < nu; ++i) o->pnl_u.p[i] = o->theta_v_u.p[i] * o->pnl_u.p[i];
+    for (size_t i = 0; i < nv; ++i) o->pnl_v.p[i] = o->theta_v_v.p[i] * o->pnl_v.p[i];
+  }
+  stv_v2(o, o->scalar_ph, o->pnl_w);
+  for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = s->theta_v_pert.p[i];
+  if (o->order == 2) {
+    stv_h(o, o->scalar_ph, o->ebg_u, o->ebg_v, o->pl_u, o->pl_v);
+  } else {
+    theta_v_adv_3rd_order(o, s->wind_u, s->wind_v);
+    for (size_t i = 0; i < nu; ++i) o->pl_u.p[i] = o->theta_v_u.p[i] * o->ebg_u.p[i];
+    for (size_t i = 0; i < nv; ++i) o->pl_v.p[i] = o->theta_v_v.p[i] * o->ebg_v.p[i];
+  }
+  stv_v(o, o->scalar_ph, o->ebg_w, o->pl_w);
+  for (size_t i = 0; i < n; ++i) o->decel.p[i] = C_D_P * s->rho[i + (size_t)o->ncond * n] / rho_sum(o, s->rho, i);
+  stv_h2(o, o->decel, o->pnl_u, o->pnl_v);
+  stv_v2(o, o->decel, o->pnl_w);
+  stv_h2(o, o->decel, o->pl_u, o->pl_v);
+  stv_v2(o, o->decel, o->pl_w);
+  if (lfirst) {
+    for (size_t i = 0; i < nu; ++i) o->pold_u.p[i] = -o->pnl_u.p[i] - 1.0 * o->pl_u.p[i];
+    for (size_t i = 0; i < nv; ++i) o->pold_v.p[i] = -o->pnl_v.p[i] - 1.0 * o->pl_v.p[i];
+  }
+}
+
+/* calc_p_grad_condensates_v, mo_p_grad.F90:106-120 */
+static void p_grad_condensates_v(oracle *o, const ostate *s) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  for (size_t i = 0; i < n; ++i) o->decel.p[i] = s->rho[i + (size_t)o->ncond * n] / rho_sum(o, s->rho, i) - 1.0;
+  stv_v(o, o->decel, o->gravity_m_v, o->pgc_w);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* time_stepping/mo_vector_tend_expl.F90 (+ simple_dissipation_rate)                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Klemp damping prefactor, mo_column_solvers.F90:190-196 and mo_momentum_diff_diss.F90:362-369 */
+static double klemp_prefactor(const oracle *o, double z) {
+  double start = o->klemp_begin_rel * o->toa;
+  double above = z - start, coeff;
+  if (above < 0.0 || !o->lklemp) {
+    coeff = 0.0;
+  } else {
+    double sn = sin(0.5 * MPI * above / (o->toa - start));
+    coeff = o->klemp_damp_max * (sn * sn);
+  }
+  return 1.0 + coeff * o->dtime;
+}
+
+/* simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377 */
+static void simple_dissipation_rate(oracle *o, const ostate *s) {
+  size_t nw = (size_t)o->ny * o->nx * o->nlev, n = (size_t)o->ny * o->nx * o->nl;
+  memcpy(o->w_ph.p, o->mom_z.p, nw * sizeof(double));
+  if (o->lklemp) {
+    double start = o->klemp_begin_rel * o->toa;
+    for (int jl = 2; jl <= o->n_damping_levels; ++jl)
+      for (int jk = 1; jk <= o->nx; ++jk)
+        for (int ji = 1; ji <= o->ny; ++ji) {
+          double z_above = A3(o->z_w, ji, jk, jl) - start, coeff;
+          if (z_above < 0.0) {
+            coeff = 0.0;
+          } else {
+            double sn = sin(0.5 * MPI * z_above / (o->toa - start));
+            coeff = o->klemp_damp_max * (sn * sn);
+          }
+          double pref = 1.0 + o->dtime * coeff;
+          double w = A3(s->wind_w, ji, jk, jl);
+          A3(o->w_ph, ji, jk, jl) = A3(o->w_ph, ji, jk, jl) + (w / pref - w) / o->dtime;
+        }
+  }
+  inner_product(o, s->wind_u, s->wind_v, s->wind_w, o->mom_x, o->mom_y, o->w_ph, o->heating_diss);
+  for (size_t i = 0; i < n; ++i) o->heating_diss.p[i] = -rho_sum(o, s->rho, i) * o->heating_diss.p[i];
+}
+
+/* vector_tend_expl, mo_vector_tend_expl.F90:30-182 */
+static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) {
+  int ny = o->ny, nx = o->nx, nl = o->nl, nlev = o->nlev;
+  if ((rk_step == 2 || step == 0) && ((!o->llinear) || o->lcorio)) {
+    calc_pot_vort(o, s);
+    stv_h(o, rho_slice(o, s->rho, o->ncond + 1), s->wind_u, s->wind_v, o->u_ph, o->v_ph);
+    stv_v(o, rho_slice(o, s->rho, o->ncond + 1), s->wind_w, o->w_ph);
+    if (o->lfreeslip) {
+      for (int jk = 1; jk <= nx; ++jk)
+        for (int ji = 1; ji <= ny; ++ji) {
+          double dv = rhog(o, s, ji, jk, nl) + (rhog(o, s, ji, jk, nl - 1) - rhog(o, s, ji, jk, nl)) /
+                                                   (A3(o->z_scalar, ji, jk, nl - 1) - A3(o->z_scalar, ji, jk, nl)) *
+                                                   (A3(o->z_w, ji, jk, nlev) - A3(o->z_scalar, ji, jk, nl));
+          A3(o->w_ph, ji, jk, nlev) = dv * A3(s->wind_w, ji, jk, nlev);
+        }
+    }
+    calc_vorticity_flux_term(o);
+    if (o->lfreeslip)
+      for (int jk = 1; jk <= nx; ++jk)
+        for (int ji = 1; ji <= ny; ++ji) A3(o->w_ph, ji, jk, nlev) = 0.0;
+    if (!o->llinear) {
+      inner_product(o, s->wind_u, s->wind_v, s->wind_w, s->wind_u, s->wind_v, s->wind_w, o->v_squared);
+      grad_vert(o, o->v_squared, o->vsg_z);
+      grad_hor(o, o->v_squared, o->vsg_x, o->vsg_y, o->vsg_z);
+    }
+  }
+  if (rk_step == 1) {
+    if (o->lklemp) simple_dissipation_rate(o, s);
+  }
+  double new_weight = (rk_step == 2) ? 0.5 : 1.0;
+  double old_weight = 1.0 - new_weight;
+  double cur_hor = 0.5 + IMPL_THERMO_WEIGHT;
+  double old_hor = 1.0 - cur_hor;
+  double cur_ver = 1.0 - IMPL_THERMO_WEIGHT;
+  double no_bg = 0.0;
+  size_t nu = (size_t)ny * (nx + 1) * nl, nv = (size_t)(ny + 1) * nx * nl, nw = (size_t)ny * nx * nlev;
+  for (size_t i = 0; i < nu; ++i)
+    o->tend_u.p[i] = old_weight * o->tend_u.p[i] +
+                     new_weight * (old_hor * o->pold_u.p[i] - cur_hor * (o->pnl_u.p[i] + o->pl_u.p[i]) - 0.5 * o->vsg_x.p[i] + o->pvt_x.p[i] + o->mom_x.p[i]);
+  for (size_t i = 0; i < nv; ++i)
+    o->tend_v.p[i] = old_weight * o->tend_v.p[i] +
+                     new_weight * (old_hor * o->pold_v.p[i] - cur_hor * (o->pnl_v.p[i] + o->pl_v.p[i]) - 0.5 * o->vsg_y.p[i] + o->pvt_y.p[i] + o->mom_y.p[i]);
+  for (size_t i = 0; i < nw; ++i)
+    o->tend_w.p[i] = old_weight * o->tend_w.p[i] +
+                     new_weight * (-cur_ver * (o->pnl_w.p[i] + o->pl_w.p[i]) - no_bg * (1.0 - cur_ver) * o->pl_w.p[i] - 0.5 * o->vsg_z.p[i] +
+                                   o->pvt_z.p[i] + o->pgc_w.p[i] + o->mom_z.p[i]);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* time_stepping/mo_scalar_tend_expl.F90                                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* scalar_tend_expl, mo_scalar_tend_expl.F90:24-181 (diffusion off) */
+static void scalar_tend_expl(oracle *o, const ostate *old, const ostate *nw, int rk_step) {
+  const ostate *ss = (rk_step == 1) ? old : nw;
+  int ny = o->ny, nx = o->nx, nl = o->nl, nc = o->nc;
+  size_t n = (size_t)ny * nx * nl, nu = (size_t)ny * (nx + 1) * nl, nv = (size_t)(ny + 1) * nx * nl;
+  for (int jc = 1; jc <= nc; ++jc) {
+    double new_w = 1.0;
+    if (rk_step == 2 && jc != o->ncond + 1) new_w = 0.5;
+    double old_w = 1.0 - new_w;
+    F3 rc = rho_slice(o, ss->rho, jc);
+    if (jc == o->ncond + 1) {
+      stv_h(o, rc, nw->wind_u, nw->wind_v, o->flux_u, o->flux_v);
+      div_h(o, o->flux_u, o->flux_v, o->flux_div);
+    } else {
+      stv_h_upstream(o, rc, nw->wind_u, nw->wind_v, o->flux_u, o->flux_v);
+      div_h_tracers(o, o->flux_u, o->flux_v, rc, nw->wind_u, nw->wind_v, o->flux_div);
+    }
+    int ptr_c = jc < o->ncond + 1 ? jc : o->ncond + 1;
+    double *tr = o->tend_rho + (size_t)(jc - 1) * n;
+    const double *mdt = o->mass_diff_tendency + (size_t)(jc - 1) * n;
+    const double *ptr = o->phase_trans_rates + (size_t)(ptr_c - 1) * n;
+    for (size_t i = 0; i < n; ++i) tr[i] = old_w * tr[i] + new_w * (-o->flux_div.p[i] + mdt[i] + ptr[i]);
+    if (jc == o->ncond + 1) {
+      for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = o->theta_v_bg.p[i] + ss->theta_v_pert.p[i];
+      if (o->order == 2) {
+        stv_h(o, o->scalar_ph, o->flux_u, o->flux_v, o->u_ph, o->v_ph);
+      } else {
+        theta_v_adv_3rd_order(o, ss->wind_u, ss->wind_v);
+        for (size_t i = 0; i < nu; ++i) o->u_ph.p[i] = o->theta_v_u.p[i] * o->flux_u.p[i];
+        for (size_t i = 0; i < nv; ++i) o->v_ph.p[i] = o->theta_v_v.p[i] * o->flux_v.p[i];
+      }
+      div_h(o, o->u_ph, o->v_ph, o->flux_div);
+      const double *ptrc = o->phase_trans_rates + (size_t)(jc - 1) * n;
+      for (int jl = 1; jl <= nl; ++jl)
+        for (int jk = 1; jk <= nx; ++jk)
+          for (int ji = 1; ji <= ny; ++ji) {
+            size_t i = (size_t)(ji - 1) + (size_t)ny * ((size_t)(jk - 1) + (size_t)nx * (size_t)(jl - 1));
+            A3(o->tend_rhotheta_v, ji, jk, jl) =
+                -o->flux_div.p[i] +
+                C_D_V * rc.p[i] / c_v_mass_weighted_air(o, ss->rho, ji, jk, jl) *
+                    (+o->heating_diss.p[i] + o->radiation_tendency.p[i] + o->phase_trans_heating_rate.p[i] +
+                     o->condensates_sediment_heat.p[i] + o->temp_diff_heating.p[i]) /
+                    (C_D_P * (o->exner_bg.p[i] + ss->exner_pert.p[i])) +
+                (ptrc[i] + mdt[i]) * o->scalar_ph.p[i];
+          }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* time_stepping/mo_column_solvers.F90                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* thomas_algorithm, mo_column_solvers.F90:606-640 */
+static void thomas(const double *c, const double *d, const double *e, const double *r, double *x, int n, double *ep, double *rp) {
+  ep[0] = e[0] / d[0];
+  for (int j = 1; j < n - 1; ++j) ep[j] = e[j] / (d[j] - ep[j - 1] * c[j - 1]);
+  rp[0] = r[0] / d[0];
+  for (int j = 1; j < n; ++j) rp[j] = (r[j] - rp[j - 1] * c[j - 1]) / (d[j] - ep[j - 1] * c[j - 1]);
+  x[n - 1] = rp[n - 1];
+  for (int j = n - 2; j >= 0; --j) x[j] = rp[j] - ep[j] * x[j + 1];
+}
+
+/* three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off) */
+static void three_band_solver_ver(oracle *o, const ostate *old, ostate *nw, int rk_step) {
+  const ostate *nu_ = (rk_step == 1) ? old : nw;
+  int ny = o->ny, nx = o->nx, nl = o->nl, g = o->ncond + 1;
+  double dt = o->dtime;
+  const double pw = 0.5;
+  double *wk = o->work;
+  double *c = wk, *d = c + nl, *e = d + nl, *r = e + nl, *x = r + nl, *rho_expl = x + nl, *rt_expl = rho_expl + nl;
+  double *ep_expl = rt_expl + nl, *tp_expl = ep_expl + nl, *rio = tp_expl + nl, *rie = rio + nl, *tin = rie + nl;
+  double *al = tin + nl, *be = al + nl, *ga = be + nl, *dp = ga + nl, *eprime = dp + nl, *rprime = eprime + nl;
+  F3 trho = rho_slice(o, o->tend_rho, g);
+  for (int ji = 1; ji <= ny; ++ji)
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int jl = 1; jl <= nl; ++jl) {
+        int j = jl - 1;
+        double V = A3(o->volume, ji, jk, jl);
+        rho_expl[j] = rhog(o, old, ji, jk, jl) + dt * A3(trho, ji, jk, jl);
+        rt_expl[j] = A3(old->rhotheta_v, ji, jk, jl) + dt * A3(o->tend_rhotheta_v, ji, jk, jl);
+        if (rk_step == 1) {
+          double ro = rhog(o, old, ji, jk, jl);
+          al[j] = -A3(old->rhotheta_v, ji, jk, jl) / (ro * ro) / V;
+          be[j] = 1.0 / ro / V;
+          ga[j] = R_D / (C_D_V * A3(old->rhotheta_v, ji, jk, jl)) * (A3(o->exner_bg, ji, jk, jl) + A3(old->exner_pert, ji, jk, jl)) / V;
+        } else {
+          double ro = rhog(o, old, ji, jk, jl), rn = rhog(o, nu_, ji, jk, jl);
+          double a_old = -A3(old->rhotheta_v, ji, jk, jl) / (ro * ro);
+          double b_old = 1.0 / ro;
+          double g_old = R_D / (C_D_V * A3(old->rhotheta_v, ji, jk, jl)) * (A3(o->exner_bg, ji, jk, jl) + A3(old->exner_pert, ji, jk, jl));
+          double a_new = -A3(nu_->rhotheta_v, ji, jk, jl) / (rn * rn);
+          double b_new = 1.0 / rn;
+          double g_new = R_D / (C_D_V * A3(nu_->rhotheta_v, ji, jk, jl)) * (A3(o->exner_bg, ji, jk, jl) + A3(nu_->exner_pert, ji, jk, jl));
+          al[j] = ((1.0 - pw) * a_old + pw * a_new) / V;
+          be[j] = ((1.0 - pw) * b_old + pw * b_new) / V;
+          ga[j] = ((1.0 - pw) * g_old + pw * g_new) / V;
+        }
+        tp_expl[j] = A3(old->theta_v_pert, ji, jk, jl) + dt * V * (al[j] * A3(trho, ji, jk, jl) + be[j] * A3(o->tend_rhotheta_v, ji, jk, jl));
+        ep_expl[j] = A3(old->exner_pert, ji, jk, jl) + dt * V * ga[j] * A3(o->tend_rhotheta_v, ji, jk, jl);
+      }
+      for (int jl = 1; jl <= nl - 1; ++jl) {
+        int j = jl - 1;
+        rio[j] = 0.5 * (rhog(o, old, ji, jk, jl) + rhog(o, old, ji, jk, jl + 1));
+        rie[j] = 0.5 * (rho_expl[j] + rho_expl[j + 1]);
+        tin[j] = 0.5 * (A3(nu_->rhotheta_v, ji, jk, jl) / rhog(o, nu_, ji, jk, jl) + A3(nu_->rhotheta_v, ji, jk, jl + 1) / rhog(o, nu_, ji, jk, jl + 1));
+      }
+      for (int jl = 1; jl <= nl - 1; ++jl) {
+        int j = jl - 1;
+        dp[j] = klemp_prefactor(o, A3(o->z_w, ji, jk, jl + 1));
+        double dzs = A3(o->z_scalar, ji, jk, jl) - A3(o->z_scalar, ji, jk, jl + 1);
+        double deb = A3(o->exner_bg, ji, jk, jl) - A3(o->exner_bg, ji, jk, jl + 1);
+        d[j] = -tin[j] * tin[j] * (ga[j] + ga[j + 1]) + 0.5 * deb * (al[j + 1] - al[j] + tin[j] * (be[j + 1] - be[j])) -
+               dzs / (IMPL_THERMO_WEIGHT * (dt * dt) * C_D_P * rio[j]) *
+                   (2.0 / A3(o->area_z, ji, jk, jl + 1) +
+                    dt * A3(old->wind_w, ji, jk, jl + 1) * 0.5 * (-1.0 / A3(o->volume, ji, jk, jl) + 1.0 / A3(o->volume, ji, jk, jl + 1))) *
+                   dp[j];
+        r[j] = -(A3(old->wind_w, ji, jk, jl + 1) + dt * A3(o->tend_w, ji, jk, jl + 1)) * dzs / (IMPL_THERMO_WEIGHT * (dt * dt) * C_D_P) +
+               tin[j] * (ep_expl[j] - ep_expl[j + 1]) / dt + 0.5 / dt * (tp_expl[j] + tp_expl[j + 1]) * deb -
+               dzs / (IMPL_THERMO_WEIGHT * (dt * dt) * C_D_P) * A3(old->wind_w, ji, jk, jl + 1) * rie[j] / rio[j];
+      }
+      for (int jl = 1; jl <= nl - 2; ++jl) {
+        int j = jl - 1;
+        c[j] = tin[j + 1] * ga[j + 1] * tin[j] + 0.5 * (A3(o->exner_bg, ji, jk, jl + 1) - A3(o->exner_bg, ji, jk, jl + 2)) * (al[j + 1] + be[j + 1] * tin[j]) -
+               (A3(o->z_scalar, ji, jk, jl + 1) - A3(o->z_scalar, ji, jk, jl + 2)) / (IMPL_THERMO_WEIGHT * dt * C_D_P) * 0.5 *
+                   A3(old->wind_w, ji, jk, jl + 2) / (A3(o->volume, ji, jk, jl + 1) * rio[j + 1]) * dp[j + 1];
+        e[j] = tin[j] * ga[j + 1] * tin[j + 1] - 0.5 * (A3(o->exner_bg, ji, jk, jl) - A3(o->exner_bg, ji, jk, jl + 1)) * (al[j + 1] + be[j + 1] * tin[j + 1]) +
+               (A3(o->z_scalar, ji, jk, jl) - A3(o->z_scalar, ji, jk, jl + 1)) / (IMPL_THERMO_WEIGHT * dt * C_D_P) * 0.5 *
+                   A3(old->wind_w, ji, jk, jl + 1) / (A3(o->volume, ji, jk, jl + 1) * rio[j]) * dp[j];
+      }
+      thomas(c, d, e, r, x, nl - 1, eprime, rprime);
+      for (int jl = 2; jl <= nl - 1; ++jl) {
+        int j = jl - 1;
+        double V = A3(o->volume, ji, jk, jl);
+        nw->rho[(size_t)(ji - 1) + (size_t)ny * ((size_t)(jk - 1) + (size_t)nx * ((size_t)(jl - 1) + (size_t)nl * (g - 1)))] =
+            rho_expl[j] + dt * (-x[j - 1] + x[j]) / V;
+        A3(nw->rhotheta_v, ji, jk, jl) = rt_expl[j] + dt * (-tin[j - 1] * x[j - 1] + tin[j] * x[j]) / V;
+      }
+      {
+        size_t base = (size_t)(ji - 1) + (size_t)ny * ((size_t)(jk - 1) + (size_t)nx * (size_t)nl * (g - 1));
+        nw->rho[base] = rho_expl[0] + dt * x[0] / A3(o->volume, ji, jk, 1);
+        A3(nw->rhotheta_v, ji, jk, 1) = rt_expl[0] + dt * tin[0] * x[0] / A3(o->volume, ji, jk, 1);
+        size_t bl = (size_t)(ji - 1) + (size_t)ny * ((size_t)(jk - 1) + (size_t)nx * ((size_t)(nl - 1) + (size_t)nl * (g - 1)));
+        nw->rho[bl] = rho_expl[nl - 1] - dt * x[nl - 2] / A3(o->volume, ji, jk, nl);
+        A3(nw->rhotheta_v, ji, jk, nl) = rt_expl[nl - 1] - dt * tin[nl - 2] * x[nl - 2] / A3(o->volume, ji, jk, nl);
+      }
+      for (int jl = 2; jl <= nl; ++jl) {
+        double rin = 0.5 * (rhog(o, nw, ji, jk, jl - 1) + rhog(o, nw, ji, jk, jl));
+        A3(nw->wind_w, ji, jk, jl) = (2.0 * x[jl - 2] / A3(o->area_z, ji, jk, jl) - rin * A3(old->wind_w, ji, jk, jl)) / rio[jl - 2];
+      }
+      for (int jl = 1; jl <= nl; ++jl)
+        A3(nw->exner_pert, ji, jk, jl) = A3(old->exner_pert, ji, jk, jl) +
+                                         A3(o->volume, ji, jk, jl) * ga[jl - 1] * (A3(nw->rhotheta_v, ji, jk, jl) - A3(old->rhotheta_v, ji, jk, jl));
+    }
+  size_t n = (size_t)ny * nx * nl;
+  for (size_t i = 0; i < n; ++i) nw->theta_v_pert.p[i] = nw->rhotheta_v.p[i] / nw->rho[i + (size_t)(g - 1) * n] - o->theta_v_bg.p[i];
+}
+
+/* three_band_solver_gen_densities, mo_column_solvers.F90:353-604 */
+static void three_band_solver_gen_densities(oracle *o, const ostate *old, ostate *nw, int rk_step) {
+  int ny = o->ny, nx = o->nx, nl = o->nl, ncond = o->ncond;
+  size_t n = (size_t)ny * nx * nl;
+  const double itw = 0.5, ew = 1.0 - itw;
+  double dt = o->dtime;
+  double *wk = o->work;
+  double *c = wk, *d = c + nl, *e = d + nl, *r = e + nl, *x = r + nl, *fi = x + nl, *fr = fi + nl, *fe = fr + nl, *vf = fe + nl;
+  double *ep = vf + nl, *rp = ep + nl;
+  const double cloud_r = 12.e-6, ice_r = 95.e-6, snow_r = 500.e-6;
+  for (int jc = 1; jc <= o->nc; ++jc) {
+    if (jc == ncond + 1) continue;
+    const double *rc_old = old->rho + (size_t)(jc - 1) * n;
+    const double *trc = o->tend_rho + (size_t)(jc - 1) * n;
+    double *rc_new = nw->rho + (size_t)(jc - 1) * n;
+    for (int ji = 1; ji <= ny; ++ji)
+      for (int jk = 1; jk <= nx; ++jk) {
+#define IX(l) ((size_t)(ji - 1) + (size_t)ny * ((size_t)(jk - 1) + (size_t)nx * (size_t)((l)-1)))
+        for (int jl = 1; jl <= nl - 1; ++jl) {
+          int j = jl - 1;
+          if (rk_step == 1 && jc == 1) A3(o->condensates_sediment_heat, ji, jk, jl) = 0.0;
+          fi[j] = A3(old->wind_w, ji, jk, jl + 1);
+          fr[j] = A3(nw->wind_w, ji, jk, jl + 1);
+          double vu, vl;
+          if (jc == 1) {
+            vu = v_fall_solid(snow_r);
+            vl = v_fall_solid(snow_r);
+            vf[j] = 0.5 * (vu + vl);
+          } else if (jc == 2 || jc == 5) {
+            vu = v_fall_liquid(o, old, A3(o->a_rain, ji, jk, jl), ji, jk, jl);
+            vl = v_fall_liquid(o, old, A3(o->a_rain, ji, jk, jl + 1), ji, jk, jl + 1);
+            for (int q = 0; q < nl; ++q) vf[q] = 0.5 * (vu + vl);
+          } else if (jc == 3) {
+            vu = v_fall_solid(ice_r);
+            vl = v_fall_solid(ice_r);
+            for (int q = 0; q < nl; ++q) vf[q] = 0.5 * (vu + vl);
+          } else if (jc == 4) {
+            vu = v_fall_liquid(o, old, cloud_r, ji, jk, jl);
+            vl = v_fall_liquid(o, old, cloud_r, ji, jk, jl + 1);
+            for (int q = 0; q < nl; ++q) vf[q] = 0.5 * (vu + vl);
+          } else {
+            vf[j] = 0.0;
+          }
+          fi[j] = fi[j] - vf[j];
+          fr[j] = fr[j] - vf[j];
+          fi[j] = A3(o->area_z, ji, jk, jl + 1) * fi[j];
+          fr[j] = A3(o->area_z, ji, jk, jl + 1) * fr[j];
+          double dens, tmp;
+          if (fr[j] >= 0.0) {
+            dens = rc_old[IX(jl + 1)];
+            tmp = A3(o->temperature, ji, jk, jl + 1);
+          } else {
+            dens = rc_old[IX(jl)];
+            tmp = A3(o->temperature, ji, jk, jl + 1);
+          }
+          fr[j] = dens * fr[j];
+          fe[j] = c_p_cond(jc, tmp) * tmp * fr[j];
+        }
+        if (rk_step == 1 && jc == 1) A3(o->condensates_sediment_heat, ji, jk, nl) = 0.0;
+        if (jc == 1)
+          vf[nl - 1] = v_fall_solid(snow_r);
+        else if (jc == 2 || jc == 5)
+          vf[nl - 1] = v_fall_liquid(o, old, A3(o->a_rain, ji, jk, nl), ji, jk, nl);
+        else if (jc == 3)
+          vf[nl - 1] = v_fall_solid(ice_r);
+        else if (jc == 4)
+          vf[nl - 1] = v_fall_liquid(o, old, cloud_r, ji, jk, nl);
+        for (int jl = 1; jl <= nl - 1; ++jl) {
+          int j = jl - 1;
+          if (fi[j] >= 0.0) {
+            c[j] = 0.0;
+            e[j] = -itw * dt / A3(o->volume, ji, jk, jl) * fi[j];
+          } else {
+            c[j] = itw * dt / A3(o->volume, ji, jk, jl + 1) * fi[j];
+            e[j] = 0.0;
+          }
+        }
+        for (int jl = 1; jl <= nl; ++jl) {
+          int j = jl - 1;
+          double V = A3(o->volume, ji, jk, jl);
+          if (jl == 1) {
+            d[j] = (fi[0] >= 0.0) ? 1.0 : 1.0 - itw * dt / V * fi[0];
+          } else if (jl == nl) {
+            d[j] = (fi[j - 1] >= 0.0) ? 1.0 + itw * dt / V * fi[j - 1] : 1.0;
+            if (jc <= ncond / 4 || jc <= ncond / 2 || jc <= ncond)
+              d[j] = d[j] + itw * vf[j] * dt * A3(o->area_z, ji, jk, jl + 1) / V;
+          } else {
+            d[j] = 1.0;
+            if (fi[j - 1] >= 0.0) d[j] = d[j] + itw * dt / V * fi[j - 1];
+            if (fi[j] < 0.0) d[j] = d[j] - itw * dt / V * fi[j];
+          }
+          r[j] = rc_old[IX(jl)] + dt * trc[IX(jl)];
+          if (jl == 1) {
+            r[j] = r[j] + ew * dt * fr[j] / V;
+            if (rk_step == 1 && jc <= ncond)
+              A3(o->condensates_sediment_heat, ji, jk, jl) = A3(o->condensates_sediment_heat, ji, jk, jl) + fe[j] / V;
+          } else if (jl == nl) {
+            r[j] = r[j] - ew * dt * fr[j - 1] / V;
+            if (rk_step == 1 && jc <= ncond)
+              A3(o->condensates_sediment_heat, ji, jk, jl) = A3(o->condensates_sediment_heat, ji, jk, jl) - fe[j - 1] / V;
+            if (jc <= ncond / 4) {
+              r[j] = r[j] - ew * vf[j] * dt * rc_old[IX(jl)] * A3(o->area_z, ji, jk, jl + 1) / V;
+            } else if (jc <= ncond / 2) {
+              r[j] = r[j] - ew * vf[j] * dt * rc_old[IX(jl)] * A3(o->area_z, ji, jk, jl + 1) / V;
+              if (rk_step == 1)
+                A3(o->condensates_sediment_heat, ji, jk, jl) =
+                    A3(o->condensates_sediment_heat, ji, jk, jl) -
+                    vf[j] * A3(o->temperature, ji, jk, nl) * c_p_cond(jc, A3(o->temperature, ji, jk, nl)) * rc_old[IX(nl)] *
+                        A3(o->area_z, ji, jk, nl + 1) / V;
+            } else if (jc <= ncond) {
+              r[j] = r[j] - ew * vf[j] * dt * rc_old[IX(jl)] * A3(o->area_z, ji, jk, jl + 1) / V;
+              if (rk_step == 1)
+                A3(o->condensates_sediment_heat, ji, jk, jl) =
+                    A3(o->condensates_sediment_heat, ji, jk, jl) -
+                    vf[j] * A3(o->temperature, ji, jk, nl) * c_p_cond(jc, A3(o->temperature, ji, jk, nl)) * rc_old[IX(nl)] *
+                        A3(o->area_z, ji, jk, nl + 1) / V;
+            }
+          } else {
+            r[j] = r[j] + ew * dt * (-fr[j - 1] + fr[j]) / V;
+            if (rk_step == 1 && jc <= ncond)
+              A3(o->condensates_sediment_heat, ji, jk, jl) = A3(o->condensates_sediment_heat, ji, jk, jl) + (-fe[j - 1] + fe[j]) / V;
+          }
+        }
+        thomas(c, d, e, r, x, nl, ep, rp);
+        for (int jl = 1; jl <= nl; ++jl)
+          if (x[jl - 1] < 0.0) x[jl - 1] = 0.0;
+        for (int jl = 1; jl <= nl; ++jl) rc_new[IX(jl)] = x[jl - 1];
+#undef IX
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* time_stepping/mo_manage_pchevi.F90                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* pchevi, mo_manage_pchevi.F90:27-118 (radiation, surface and boundary updates off) */
+static void pchevi(oracle *o, ostate *old, ostate *nw, int step) {
+  size_t nu = (size_t)o->ny * (o->nx + 1) * o->nl, nv = (size_t)(o->ny + 1) * o->nx * o->nl;
+  temperature_diagnostics(o, old);
+  for (int rk = 1; rk <= 2; ++rk) {
+    if (rk == 1) manage_p_grad(o, old, step == 0);
+    p_grad_condensates_v(o, old);
+    vector_tend_expl(o, rk == 1 ? old : nw, rk, step);
+    for (size_t i = 0; i < nu; ++i) nw->wind_u.p[i] = old->wind_u.p[i] + o->dtime * o->tend_u.p[i];
+    for (size_t i = 0; i < nv; ++i) nw->wind_v.p[i] = old->wind_v.p[i] + o->dtime * o->tend_v.p[i];
+    scalar_tend_expl(o, old, nw, rk);
+    three_band_solver_ver(o, old, nw, rk);
+    if (o->lfreeslip) w_free_slip(o, nw);
+    if (o->nc > 1) three_band_solver_gen_densities(o, old, nw, rk);
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* public API (ctypes)                                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+static void state_alloc(oracle *o, ostate *s) {
+  s->rho = oalloc(o, (size_t)o->ny * o->nx * o->nl * o->nc);
+  s->rhotheta_v = f3alloc_o(o, o->ny, o->nx, o->nl);
+  s->theta_v_pert = f3alloc_o(o, o->ny, o->nx, o->nl);
+  s->exner_pert = f3alloc_o(o, o->ny, o->nx, o->nl);
+  s->wind_u = f3alloc_o(o, o->ny, o->nx + 1, o->nl);
+  s->wind_v = f3alloc_o(o, o->ny + 1, o->nx, o->nl);
+  s->wind_w = f3alloc_o(o, o->ny, o->nx, o->nlev);
+}
+
+static char g_err[256];
+
+const char *orc_last_error(void) { return g_err; }
+
+oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
+  g_err[0] = 0;
+  if (dm->ny < 3 || dm->nx < 3 || dm->n_layers < 3 || dm->n_oro_layers >= dm->n_layers || dm->n_oro_layers < 0) {
+    snprintf(g_err, sizeof g_err, "bad dimensions");
+    return NULL;
+  }
+  if (!cf->lperiodic || !cf->luse_bg_state || cf->lmom_diff_h || cf->lmom_diff_v || cf->ltemp_diff_h || cf->ltemp_diff_v ||
+      cf->lmass_diff_h || cf->lmass_diff_v || cf->lpbl || cf->lsfc_sensible_heat_flux || cf->lsfc_phase_trans || cf->lprog_soil_temp ||
+      (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
+    snprintf(g_err, sizeof g_err, "unsupported configuration");
+    return NULL;
+  }
+  if ((dm->n_constituents == 1) == (cf->lmoist != 0) || dm->n_constituents != dm->n_condensed_constituents + 1 + (cf->lmoist ? 1 : 0)) {
+    snprintf(g_err, sizeof g_err, "constituent counts do not match lmoist");
+    return NULL;
+  }
+  oracle *o = (oracle *)calloc(1, sizeof(oracle));
+  int ny = dm->ny, nx = dm->nx, nl = dm->n_layers, nlev = nl + 1;
+  o->ny = ny;
+  o->nx = nx;
+  o->nl = nl;
+  o->nlev = nlev;
+  o->noro = dm->n_oro_layers;
+  o->nflat = nl - dm->n_oro_layers;
+  o->nc = dm->n_constituents;
+  o->ncond = dm->n_condensed_constituents;
+  o->dtime = cf->dtime;
+  o->toa = cf->toa;
+  o->dxn = cf->dx;
+  o->dyn = cf->dy;
+  o->klemp_damp_max = cf->klemp_damp_max;
+  o->klemp_begin_rel = cf->klemp_begin_rel;
+  o->order = cf->theta_adv_order;
+  o->use_bg = 1;
+  o->llinear = cf->llinear;
+  o->lcorio = cf->lcorio;
+  o->lmoist = cf->lmoist;
+  o->lklemp = cf->lklemp;
+  o->lfreeslip = cf->lfreeslip;
+#define G3(f, a, b, c) o->f = f3alloc_o(o, a, b, c)
+  G3(z_scalar, ny, nx, nl);
+  G3(z_u, ny, nx + 1, nl);
+  G3(z_v, ny + 1, nx, nl);
+  G3(z_w, ny, nx, nlev);
+  G3(dx, ny, nx + 1, nl);
+  G3(dy, ny + 1, nx, nl);
+  G3(dz, ny, nx, nlev);
+  G3(area_x, ny, nx + 1, nl);
+  G3(area_y, ny + 1, nx, nl);
+  G3(area_z, ny, nx, nlev);
+  G3(volume, ny, nx, nl);
+  G3(slope_x, ny, nx + 1, nl);
+  G3(slope_y, ny + 1, nx, nl);
+  G3(area_dual_x, ny + 1, nx, nlev);
+  G3(area_dual_y, ny, nx + 1, nlev);
+  G3(area_dual_z, ny + 1, nx + 1, nl);
+  G3(z_area_dual_z, ny + 1, nx + 1, nl);
+  G3(fvec_x, ny + 1, nx, 1);
+  G3(fvec_y, ny, nx + 1, 1);
+  G3(fvec_z, ny + 1, nx + 1, 1);
+  G3(ebg_u, ny, nx + 1, nl);
+  G3(ebg_v, ny + 1, nx, nl);
+  G3(ebg_w, ny, nx, nlev);
+  G3(theta_v_bg, ny, nx, nl);
+  G3(exner_bg, ny, nx, nl);
+  G3(gravity_m_v, ny, nx, nlev);
+  G3(gravity_potential, ny, nx, nl);
+  o->ipw = oalloc(o, (size_t)6 * ny * nx * nl);
+  o->trsk_u = oalloc(o, (size_t)3 * ny);
+  o->trsk_v = oalloc(o, (size_t)2 * (ny + 1));
+  state_alloc(o, &o->st[0]);
+  state_alloc(o, &o->st[1]);
+  o->tend_rho = oalloc(o, (size_t)ny * nx * nl * o->nc);
+  G3(tend_rhotheta_v, ny, nx, nl);
+  G3(tend_u, ny, nx + 1, nl);
+  G3(tend_v, ny + 1, nx, nl);
+  G3(tend_w, ny, nx, nlev);
+  G3(v_squared, ny, nx, nl);
+  G3(vsg_x, ny, nx + 1, nl);
+  G3(vsg_y, ny + 1, nx, nl);
+  G3(vsg_z, ny, nx, nlev);
+  G3(pl_u, ny, nx + 1, nl);
+  G3(pl_v, ny + 1, nx, nl);
+  G3(pl_w, ny, nx, nlev);
+  G3(pnl_u, ny, nx + 1, nl);
+  G3(pnl_v, ny + 1, nx, nl);
+  G3(pnl_w, ny, nx, nlev);
+  G3(pold_u, ny, nx + 1, nl);
+  G3(pold_v, ny + 1, nx, nl);
+  G3(pgc_w, ny, nx, nlev);
+  G3(pvt_x, ny, nx + 1, nl);
+  G3(pvt_y, ny + 1, nx, nl);
+  G3(pvt_z, ny, nx, nlev);
+  G3(scalar_ph, ny, nx, nl);
+  G3(temperature, ny, nx, nl);
+  G3(u_ph, ny, nx + 1, nl);
+  G3(v_ph, ny + 1, nx, nl);
+  G3(w_ph, ny, nx, nlev);
+  G3(theta_v_u, ny, nx + 1, nl);
+  G3(theta_v_v, ny + 1, nx, nl);
+  G3(zeta_x, ny + 1, nx, nlev);
+  G3(zeta_y, ny, nx + 1, nlev);
+  G3(zeta_z, ny + 1, nx + 1, nl);
+  G3(eta_x, ny + 1, nx, nlev);
+  G3(eta_y, ny, nx + 1, nlev);
+  G3(eta_z, ny + 1, nx + 1, nl);
+  G3(radiation_tendency, ny, nx, nl);
+  G3(flux_u, ny, nx + 1, nl);
+  G3(flux_v, ny + 1, nx, nl);
+  G3(flux_div, ny, nx, nl);
+  G3(decel, ny, nx, nl);
+  G3(mom_x, ny, nx + 1, nl);
+  G3(mom_y, ny + 1, nx, nl);
+  G3(mom_z, ny, nx, nlev);
+  G3(heating_diss, ny, nx, nl);
+  G3(phase_trans_heating_rate, ny, nx, nl);
+  G3(temp_diff_heating, ny, nx, nl);
+  G3(condensates_sediment_heat, ny, nx, nl);
+  G3(a_rain, ny, nx, nl);
+#undef G3
+  o->phase_trans_rates = oalloc(o, (size_t)ny * nx * nl * (o->ncond + 1));
+  o->mass_diff_tendency = oalloc(o, (size_t)ny * nx * nl * o->nc);
+  o->work = oalloc(o, (size_t)24 * (nl + 2));
+  return o;
+}
+
+static F3 *grid_field(oracle *o, const char *name) {
+#define GF(f) \
+  if (!strcmp(name, #f)) return &o->f;
+  GF(z_scalar) GF(z_u) GF(z_v) GF(z_w) GF(dx) GF(dy) GF(dz) GF(area_x) GF(area_y) GF(area_z) GF(volume) GF(slope_x) GF(slope_y)
+  GF(area_dual_x) GF(area_dual_y) GF(area_dual_z) GF(z_area_dual_z) GF(fvec_x) GF(fvec_y) GF(fvec_z) GF(theta_v_bg) GF(exner_bg)
+  GF(gravity_m_v) GF(gravity_potential)
+  if (!strcmp(name, "exner_bg_grad_u")) return &o->ebg_u;
+  if (!strcmp(name, "exner_bg_grad_v")) return &o->ebg_v;
+  if (!strcmp(name, "exner_bg_grad_w")) return &o->ebg_w;
+#undef GF
+  return NULL;
+}
+
+int orc_set_grid(oracle *o, const char *name, const double *src) {
+  if (!strcmp(name, "inner_product_weights")) {
+    memcpy(o->ipw, src, sizeof(double) * 6 * (size_t)o->ny * o->nx * o->nl);
+    return 0;
+  }
+  if (!strcmp(name, "trsk_weights_u")) {
+    memcpy(o->trsk_u, src, sizeof(double) * 3 * (size_t)o->ny);
+    return 0;
+  }
+  if (!strcmp(name, "trsk_weights_v")) {
+    memcpy(o->trsk_v, src, sizeof(double) * 2 * (size_t)(o->ny + 1));
+    return 0;
+  }
+  F3 *f = grid_field(o, name);
+  if (!f) return 1;
+  memcpy(f->p, src, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
+  return 0;
+}
+
+/* Must be called after z_w is set: n_damping_levels as in mo_grid_generator.F90:1050-1058. */
+void orc_finish_grid(oracle *o) {
+  o->n_damping_levels = 0;
+  for (int jl = 1; jl <= o->nlev; ++jl) {
+    double mx = -1e300;
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        if (A3(o->z_w, ji, jk, jl) > mx) mx = A3(o->z_w, ji, jk, jl);
+    if (mx > o->klemp_begin_rel * o->toa) o->n_damping_levels += 1;
+  }
+}
+
+static double *state_field(oracle *o, int slot, const char *name, size_t *count) {
+  ostate *s = &o->st[slot];
+#define SF(f)                                                                 \
+  if (!strcmp(name, #f)) {                                                    \
+    *count = (size_t)s->f.n1 * s->f.n2 * s->f.n3;                             \
+    return s->f.p;                                                            \
+  }
+  SF(rhotheta_v) SF(theta_v_pert) SF(exner_pert) SF(wind_u) SF(wind_v) SF(wind_w)
+#undef SF
+  if (!strcmp(name, "rho")) {
+    *count = (size_t)o->ny * o->nx * o->nl * o->nc;
+    return s->rho;
+  }
+  return NULL;
+}
+
+int orc_set_state(oracle *o, int slot, const char *name, const double *src) {
+  size_t cnt;
+  double *p = state_field(o, slot, name, &cnt);
+  if (!p) return 1;
+  memcpy(p, src, cnt * sizeof(double));
+  return 0;
+}
+
+int orc_get_state(oracle *o, int slot, const char *name, double *dst) {
+  size_t cnt;
+  double *p = state_field(o, slot, name, &cnt);
+  if (!p) return 1;
+  memcpy(dst, p, cnt * sizeof(double));
+  return 0;
+}
+
+int orc_get_diag(oracle *o, const char *name, double *dst) {
+  F3 *f = NULL;
+  if (!strcmp(name, "temperature")) f = &o->temperature;
+  if (!strcmp(name, "heating_diss")) f = &o->heating_diss;
+  if (!strcmp(name, "pot_vort_tend_x")) f = &o->pvt_x;
+  if (!strcmp(name, "pot_vort_tend_y")) f = &o->pvt_y;
+  if (!strcmp(name, "pot_vort_tend_z")) f = &o->pvt_z;
+  if (!strcmp(name, "p_grad_acc_old_u")) f = &o->pold_u;
+  if (!strcmp(name, "p_grad_acc_old_v")) f = &o->pold_v;
+  if (!strcmp(name, "tend_wind_w")) f = &o->tend_w;
+  if (!strcmp(name, "condensates_sediment_heat")) f = &o->condensates_sediment_heat;
+  if (!f) return 1;
+  memcpy(dst, f->p, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
+  return 0;
+}
+
+/* One reference step: state slot `nw` = pchevi(slot `old`). */
+int orc_step(oracle *o, int old, int nw, int total_step_counter) {
+  pchevi(o, &o->st[old], &o->st[nw], total_step_counter);
+  return 0;
+}
+
+/* n steps with the reference's ping-pong (control.F90:520-524), starting at counter `first`. */
+int orc_run(oracle *o, int first, int nsteps) {
+  for (int s = first; s < first + nsteps; ++s) {
+    if (s % 2 == 0)
+      pchevi(o, &o->st[0], &o->st[1], s);
+    else
+      pchevi(o, &o->st[1], &o->st[0], s);
+  }
+  return 0;
+}
+
+/* state_2 = state_1 (control.F90:500) */
+void orc_copy_slot(oracle *o, int from, int to) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  ostate *a = &o->st[from], *b = &o->st[to];
+  memcpy(b->rho, a->rho, n * o->nc * sizeof(double));
+  memcpy(b->rhotheta_v.p, a->rhotheta_v.p, n * sizeof(double));
+  memcpy(b->theta_v_pert.p, a->theta_v_pert.p, n * sizeof(double));
+  memcpy(b->exner_pert.p, a->exner_pert.p, n * sizeof(double));
+  memcpy(b->wind_u.p, a->wind_u.p, (size_t)o->ny * (o->nx + 1) * o->nl * sizeof(double));
+  memcpy(b->wind_v.p, a->wind_v.p, (size_t)(o->ny + 1) * o->nx * o->nl * sizeof(double));
+  memcpy(b->wind_w.p, a->wind_w.p, (size_t)o->ny * o->nx * o->nlev * sizeof(double));
+}
+
+/*
+ * Budget sums of write_out_integrals (mo_write_out.F90:246-302), evaluated with plain
+ * left-to-right summation in the reference's array order.  out: [rhotheta, 0.5*kinetic,
+ * potential, internal, mass_1..mass_nc]; the internal energy uses the diagnosed temperature.
+ */
+void orc_integrals(oracle *o, int slot, double *out) {
+  ostate *s = &o->st[slot];
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  for (int jc = 1; jc <= o->nc; ++jc) {
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) acc += s->rho[i + (size_t)(jc - 1) * n] * o->volume.p[i];
+    out[4 + jc - 1] = acc;
+  }
+  double acc = 0.0;
+  for (size_t i = 0; i < n; ++i) acc += s->rhotheta_v.p[i] * o->volume.p[i];
+  out[0] = acc;
+  inner_product(o, s->wind_u, s->wind_v, s->wind_w, s->wind_u, s->wind_v, s->wind_w, o->scalar_ph);
+  double ke = 0.0, pe = 0.0, ie = 0.0;
+  const double *rg = s->rho + (size_t)o->ncond * n;
+  for (size_t i = 0; i < n; ++i) {
+    ke += (rg[i] * o->scalar_ph.p[i]) * o->volume.p[i];
+    pe += (rg[i] * o->gravity_potential.p[i]) * o->volume.p[i];
+    ie += (C_D_V * rg[i] * o->temperature.p[i]) * o->volume.p[i];
+  }
+  out[1] = 0.5 * ke;
+  out[2] = pe;
+  out[3] = ie;
+}
+
+void orc_diagnose_temperature(oracle *o, int slot) { temperature_diagnostics(o, &o->st[slot]); }
+
+int orc_n_damping_levels(const oracle *o) { return o->n_damping_levels; }
+
+void orc_destroy(oracle *o) {
+  if (!o) return;
+  for (int i = 0; i < o->nallocs; ++i) free(o->allocs[i]);
+  free(o);
+}

@@ -1,0 +1,199 @@
+"""ctypes bindings of the two CPU oracles (TEST INFRASTRUCTURE ONLY).
+
+* ``CpuOracle``  -- oracle/liboracle.so, the C restatement (lgame_oracle.c).
+* ``RefHarness`` -- oracle/_ref/libref.so, the reference's own modules driven by ref_harness.F90
+  (exists only where the reference tree was available to build it).
+
+Only the test suite, the smoke check and the CPU-baseline leg of the benchmark use this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+
+REF_GRID_IDS = {name: i + 1 for i, name in enumerate([
+    "z_scalar", "z_u", "z_v", "z_w", "dx", "dy", "dz", "area_x", "area_y", "area_z", "volume",
+    "slope_x", "slope_y", "inner_product_weights", "area_dual_x", "area_dual_y", "area_dual_z",
+    "z_area_dual_z", "fvec_x", "fvec_y", "fvec_z", "trsk_weights_u", "trsk_weights_v",
+    "exner_bg_grad_u", "exner_bg_grad_v", "exner_bg_grad_w", "theta_v_bg", "exner_bg",
+    "gravity_m_v", "gravity_potential"])}
+REF_STATE_IDS = {"rho": 1, "rhotheta_v": 2, "theta_v_pert": 3, "exner_pert": 4, "wind_u": 5,
+                 "wind_v": 6, "wind_w": 7}
+REF_DIAG_IDS = {"temperature": 1, "heating_diss": 2, "pot_vort_tend_x": 3, "pot_vort_tend_z": 4,
+                "condensates_sediment_heat": 5}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fortran(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+class CpuOracle:
+    """The C restatement of the reference step on one configuration."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(ORACLE_LIB):
+                raise FileNotFoundError(f"{ORACLE_LIB} is not built (run `make -C oracle`)")
+            lib = ctypes.CDLL(ORACLE_LIB)
+            lib.orc_create.restype = ctypes.c_void_p
+            lib.orc_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+            for f in ("orc_set_grid", "orc_set_state", "orc_get_state", "orc_get_diag"):
+                getattr(lib, f).restype = ctypes.c_int
+            lib.orc_set_grid.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_get_diag.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_finish_grid.argtypes = [ctypes.c_void_p]
+            lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            lib.orc_copy_slot.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            lib.orc_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+            lib.orc_diagnose_temperature.argtypes = [ctypes.c_void_p, ctypes.c_int]
+            lib.orc_n_damping_levels.argtypes = [ctypes.c_void_p]
+            lib.orc_destroy.argtypes = [ctypes.c_void_p]
+            lib.orc_last_error.restype = ctypes.c_char_p
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, cfg, grid: dict):
+        from lgame.config import GRID_FIELDS
+        self.cfg = cfg
+        self.dims, self.ccfg = cfg.to_abi()
+        lib = self.lib()
+        self.h = lib.orc_create(ctypes.byref(self.dims), ctypes.byref(self.ccfg))
+        if not self.h:
+            raise ValueError(lib.orc_last_error().decode())
+        for name in GRID_FIELDS:
+            a = _fortran(grid[name])
+            if lib.orc_set_grid(self.h, name.encode(), _ptr(a)) != 0:
+                raise KeyError(name)
+        lib.orc_finish_grid(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self._lib is not None:
+            self._lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, slot: int, st: dict):
+        for k, v in st.items():
+            a = _fortran(v)
+            if self.lib().orc_set_state(self.h, slot, k.encode(), _ptr(a)) != 0:
+                raise KeyError(k)
+
+    def get_state(self, slot: int) -> dict:
+        from lgame.config import STATE_FIELDS, state_shape
+        c = self.cfg
+        out = {}
+        for k in STATE_FIELDS:
+            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents), order="F")
+            self.lib().orc_get_state(self.h, slot, k.encode(), _ptr(a))
+            out[k] = a
+        return out
+
+    def get_diag(self, name: str, shape) -> np.ndarray:
+        a = np.zeros(shape, order="F")
+        if self.lib().orc_get_diag(self.h, name.encode(), _ptr(a)) != 0:
+            raise KeyError(name)
+        return a
+
+    def start(self, st: dict):
+        """Load the initial state into slot 0 and copy it to slot 1 (control.F90:500)."""
+        self.set_state(0, st)
+        self.lib().orc_copy_slot(self.h, 0, 1)
+
+    def run(self, first: int, nsteps: int):
+        self.lib().orc_run(self.h, first, nsteps)
+
+    def step(self, old: int, new: int, counter: int):
+        self.lib().orc_step(self.h, old, new, counter)
+
+    def integrals(self, slot: int) -> np.ndarray:
+        out = np.zeros(4 + self.cfg.n_constituents)
+        self.lib().orc_integrals(self.h, slot, _ptr(out))
+        return out
+
+    @property
+    def n_damping_levels(self) -> int:
+        return self.lib().orc_n_damping_levels(self.h)
+
+
+class RefHarness:
+    """The reference's compiled modules behind ref_harness.F90 (one instance per process)."""
+
+    _lib = None
+    _used = False
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_LIB)
+
+    def __init__(self, cfg, grid: dict):
+        from lgame.config import GRID_FIELDS
+        if RefHarness._used:
+            raise RuntimeError("the reference harness holds module state; use one per process")
+        RefHarness._used = True
+        lib = ctypes.CDLL(REF_LIB)
+        lib.ref_set_grid.argtypes = [ctypes.c_int, ctypes.c_void_p]
+        lib.ref_state_io.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        lib.ref_get_diag.argtypes = [ctypes.c_int, ctypes.c_void_p]
+        lib.ref_run.argtypes = [ctypes.c_int, ctypes.c_int]
+        self.lib = lib
+        self.cfg = cfg
+        idims = np.array([cfg.ny, cfg.nx, cfg.n_layers, cfg.n_oro_layers, cfg.n_constituents,
+                          cfg.n_condensed_constituents, cfg.nsoillays], dtype=np.int32)
+        icfg = np.array([cfg.theta_adv_order, int(cfg.luse_bg_state), int(cfg.llinear), int(cfg.lcorio),
+                         int(cfg.lmoist), int(cfg.lklemp), int(cfg.lperiodic), int(cfg.lfreeslip)], dtype=np.int32)
+        rcfg = np.array([cfg.dt, cfg.toa, cfg.dx, cfg.dy, cfg.klemp_damp_max, cfg.klemp_begin_rel])
+        lib.ref_init(_ptr(idims), _ptr(icfg), _ptr(rcfg))
+        for name in GRID_FIELDS:
+            a = _fortran(grid[name])
+            lib.ref_set_grid(REF_GRID_IDS[name], _ptr(a))
+        lib.ref_finish_grid()
+
+    def start(self, st: dict):
+        for k, v in st.items():
+            a = _fortran(v)
+            self.lib.ref_state_io(0, REF_STATE_IDS[k], 0, _ptr(a))
+        self.lib.ref_copy_slot0_to_1()
+
+    def run(self, first: int, nsteps: int):
+        self.lib.ref_run(first, nsteps)
+
+    def get_state(self, slot: int) -> dict:
+        from lgame.config import STATE_FIELDS, state_shape
+        c = self.cfg
+        out = {}
+        for k in STATE_FIELDS:
+            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents), order="F")
+            self.lib.ref_state_io(slot, REF_STATE_IDS[k], 1, _ptr(a))
+            out[k] = a
+        return out
+
+    def get_diag(self, name: str, shape) -> np.ndarray:
+        a = np.zeros(shape, order="F")
+        self.lib.ref_get_diag(REF_DIAG_IDS[name], _ptr(a))
+        return a
+
+
+def thomas_reference(c, d, e, r):
+    """The reference's thomas_algorithm (mo_column_solvers.F90:606-640) via the harness."""
+    lib = ctypes.CDLL(REF_LIB)
+    lib.ref_thomas.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int]
+    n = len(d)
+    x = np.zeros(n)
+    c, d, e, r = (np.ascontiguousarray(v, dtype=np.float64) for v in (c, d, e, r))
+    lib.ref_thomas(_ptr(c), _ptr(d), _ptr(e), _ptr(r), _ptr(x), n)
+    return x
