@@ -223,6 +223,19 @@ int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad);
 /* Copy a diagnostic field (LGAME_DIAG_*) to the host in the reference layout. */
 int lgame_get_diag(lgame_handle *h, int which, double *out);
 
+/*
+ * Event timing of kernel groups on the handle's stream.  No counterpart in the reference, which
+ * only prints a model speed (control.F90:544-547); for profiling tools and benchmarks.
+ * lgame_profile(h, 1) starts recording (and clears earlier records); lgame_profile_read waits for
+ * the stream and returns the summed milliseconds and launch count of one group.
+ */
+#define LGAME_PROF_SOLVE_VER   0   /* three_band_solver_ver kernels                  */
+#define LGAME_PROF_SOLVE_GEN   1   /* three_band_solver_gen_densities kernels        */
+#define LGAME_PROF_STEP        2   /* whole lgame_pchevi calls                       */
+int lgame_profile(lgame_handle *h, int enable);
+int lgame_profile_read(lgame_handle *h, int group, double *total_ms, long long *count);
+int lgame_profile_reset(lgame_handle *h);
+
 /* Release the handle and its device memory. */
 void lgame_finalize(lgame_handle *h);
 

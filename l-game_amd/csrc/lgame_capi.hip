@@ -1,0 +1,746 @@
+// lgame_capi.hip -- C-ABI of the MI355X HEVI step (include/lgame_gpu.h): handle, device memory,
+// layout conversion between the reference's Fortran arrays and the device layout, the step
+// sequence, budget sums and the finite check.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lgame_gpu.h"
+#include "lgame_ops.h"
+#include "lgame_step.h"
+
+using namespace lgame;
+
+struct lgame_handle {
+  lgame_dims dims;
+  lgame_cfg cfg;
+  Ctx c;
+  StateDev slot[2];
+  double *slab = nullptr;        // all fields, one allocation
+  size_t slab_fields = 0;
+  double *staging = nullptr;     // device staging for host<->device transposes
+  size_t staging_elems = 0;
+  double *red = nullptr;         // reduction partials
+  unsigned long long *counter = nullptr;
+  double *trsk = nullptr;
+  hipStream_t stream = nullptr;
+  int have_state[2] = {0, 0};
+  int stepped = 0;
+  // optional event timing of kernel groups (LGAME_PROF_*)
+  int prof = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<int, int>>> ev_marks;  // group, (start, stop) pool index
+  size_t ev_next = 0;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_init_error;
+
+int fail(lgame_handle *h, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h)
+    h->err = buf;
+  else
+    g_init_error = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return fail(h, LGAME_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// Fortran array (n1, n2, n3) with n1 = y extent, n2 = x extent, optional leading component
+// dimension of size `stride` -> device layout
+__global__ void k_from_ref(double *dst, const double *src, int n1, int n2, int n3, int stride, int comp, Geom G) {
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int jl = blockIdx.z + 1;
+  if (jk > n2 || ji > n1 || jl > n3) return;
+  const size_t s = (size_t)comp + (size_t)stride * ((size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1)));
+  dst[LG_IX(G, ji, jk, jl)] = src[s];
+}
+
+__global__ void k_to_ref(double *dst, const double *src, int n1, int n2, int n3, Geom G) {
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int jl = blockIdx.z + 1;
+  if (jk > n2 || ji > n1 || jl > n3) return;
+  dst[(size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1))] = src[LG_IX(G, ji, jk, jl)];
+}
+
+// Klemp damping prefactor at w levels, 1 + damping_coeff*dtime (mo_column_solvers.F90:190-196)
+double klemp_prefactor(const lgame_cfg &cf, double z) {
+  const double start = cf.klemp_begin_rel * cf.toa;
+  const double above = z - start;
+  double coeff;
+  if (above < 0.0 || !cf.lklemp) {
+    coeff = 0.0;
+  } else {
+    const double sn = std::sin(0.5 * MPI * above / (cf.toa - start));
+    coeff = cf.klemp_damp_max * (sn * sn);
+  }
+  return 1.0 + coeff * cf.dtime;
+}
+
+int upload(lgame_handle *h, double *dst, const double *host, int n1, int n2, int n3, int stride = 1, int ncomp = 1) {
+  const size_t n = (size_t)n1 * n2 * n3 * stride;
+  if (n > h->staging_elems) return fail(h, LGAME_EINVAL, "internal: staging buffer too small");
+  HIPCHK(h, hipMemcpyAsync(h->staging, host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  dim3 blk(BX, BY, 1), grd((n2 + BX - 1) / BX, (n1 + BY - 1) / BY, n3);
+  for (int comp = 0; comp < ncomp; ++comp)
+    hipLaunchKernelGGL(k_from_ref, grd, blk, 0, h->stream, dst + (size_t)comp * h->c.G.fsize, (const double *)h->staging, n1, n2, n3,
+                       stride, comp, h->c.G);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int download(lgame_handle *h, double *host, const double *src, int n1, int n2, int n3) {
+  const size_t n = (size_t)n1 * n2 * n3;
+  if (n > h->staging_elems) return fail(h, LGAME_EINVAL, "internal: staging buffer too small");
+  dim3 blk(BX, BY, 1), grd((n2 + BX - 1) / BX, (n1 + BY - 1) / BY, n3);
+  hipLaunchKernelGGL(k_to_ref, grd, blk, 0, h->stream, h->staging, src, n1, n2, n3, h->c.G);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(host, h->staging, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// ---- budget sums (write_out_integrals, mo_write_out.F90:226-304), two-sum compensated -------
+struct DD {
+  double s, c;
+};
+__device__ __forceinline__ void dd_add(DD &a, double x) {
+  const double t = a.s + x;
+  const double bp = t - a.s;
+  const double err = (a.s - (t - bp)) + (x - bp);
+  a.s = t;
+  a.c = a.c + err;
+}
+
+constexpr int NRED = 4 + MAX_CONST;
+constexpr int RED_BLOCK = 256;
+constexpr int RED_GRID = 512;
+
+__global__ void __launch_bounds__(RED_BLOCK) k_integrals(const Ctx c, const StateDev s, double *partial) {
+  DD acc[NRED];
+#pragma unroll
+  for (int q = 0; q < NRED; ++q) acc[q] = {0.0, 0.0};
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond, nc = c.G.nc;
+  const long long ncell = (long long)nx * ny * nl;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long long)gridDim.x * blockDim.x) {
+    const int jk = (int)(t % nx) + 1;
+    const int ji = (int)((t / nx) % ny) + 1;
+    const int jl = (int)(t / ((long long)nx * ny)) + 1;
+    const size_t i = LG_IX(c.G, ji, jk, jl);
+    const double V = c.g.volume[i];
+    const double rg = s.rho[ncond][i];
+    dd_add(acc[0], s.rhotheta_v[i] * V);
+    dd_add(acc[1], (rg * ke_point(c, s, ji, jk, jl)) * V);
+    dd_add(acc[2], (rg * c.g.gravity_potential[i]) * V);
+    dd_add(acc[3], (C_D_V * rg * c.w.temperature[i]) * V);
+    for (int jc = 0; jc < nc; ++jc) dd_add(acc[4 + jc], s.rho[jc][i] * V);
+  }
+  __shared__ double sh_s[RED_BLOCK], sh_c[RED_BLOCK];
+  for (int q = 0; q < 4 + nc; ++q) {
+    sh_s[threadIdx.x] = acc[q].s;
+    sh_c[threadIdx.x] = acc[q].c;
+    __syncthreads();
+    for (int st = RED_BLOCK / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) {
+        DD a = {sh_s[threadIdx.x], sh_c[threadIdx.x]};
+        dd_add(a, sh_s[threadIdx.x + st]);
+        a.c = a.c + sh_c[threadIdx.x + st];
+        sh_s[threadIdx.x] = a.s;
+        sh_c[threadIdx.x] = a.c;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partial[2 * (q * RED_GRID + blockIdx.x)] = sh_s[0];
+      partial[2 * (q * RED_GRID + blockIdx.x) + 1] = sh_c[0];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_integrals_final(const double *partial, int nq, double *out) {
+  __shared__ double sh_s[RED_BLOCK], sh_c[RED_BLOCK];
+  for (int q = 0; q < nq; ++q) {
+    DD a = {0.0, 0.0};
+    for (int b = threadIdx.x; b < RED_GRID; b += RED_BLOCK) {
+      dd_add(a, partial[2 * (q * RED_GRID + b)]);
+      a.c = a.c + partial[2 * (q * RED_GRID + b) + 1];
+    }
+    sh_s[threadIdx.x] = a.s;
+    sh_c[threadIdx.x] = a.c;
+    __syncthreads();
+    for (int st = RED_BLOCK / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) {
+        DD x = {sh_s[threadIdx.x], sh_c[threadIdx.x]};
+        dd_add(x, sh_s[threadIdx.x + st]);
+        x.c = x.c + sh_c[threadIdx.x + st];
+        sh_s[threadIdx.x] = x.s;
+        sh_c[threadIdx.x] = x.c;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh_s[0] + sh_c[0];
+    __syncthreads();
+  }
+}
+
+// number of non-finite values in the prognostic fields of a slot
+__global__ void k_count_nonfinite(const Ctx c, const StateDev s, unsigned long long *count) {
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int jl = blockIdx.z + 1;
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
+  if (jk > nx + 1 || ji > ny + 1 || jl > nl + 1) return;
+  const size_t i = LG_IX(c.G, ji, jk, jl);
+  unsigned long long bad = 0;
+  if (ji <= ny && jk <= nx && jl <= nl) {
+    for (int jc = 0; jc < c.G.nc; ++jc) bad += !isfinite(s.rho[jc][i]);
+    bad += !isfinite(s.rhotheta_v[i]) + !isfinite(s.theta_v_pert[i]) + !isfinite(s.exner_pert[i]);
+  }
+  if (ji <= ny && jl <= nl) bad += !isfinite(s.wind_u[i]);
+  if (jk <= nx && jl <= nl) bad += !isfinite(s.wind_v[i]);
+  if (ji <= ny && jk <= nx) bad += !isfinite(s.wind_w[i]);
+  if (bad) atomicAdd(count, bad);
+}
+
+inline dim3 grid3(int nxr, int nyr, int nzr) { return dim3((nxr + BX - 1) / BX, (nyr + BY - 1) / BY, nzr); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// the step sequence, pchevi (mo_manage_pchevi.F90:27-118)
+// ---------------------------------------------------------------------------------------------
+namespace lgame {
+
+void enqueue_pchevi(const Ctx &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof) {
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
+  const dim3 blk(BX, BY, 1);
+  const dim3 g_cell = grid3(nx, ny, nl);
+  const dim3 g_uv = grid3(nx + 1, ny + 1, nl);
+  const dim3 g_all = grid3(nx + 1, ny + 1, nlev);
+  const dim3 g_col((nx * ny + BCOL - 1) / BCOL), b_col(BCOL);
+  const bool o3 = c.C.order == 3;
+
+  hipLaunchKernelGGL(k_temperature, g_cell, blk, 0, st, c, old);
+  for (int rk = 1; rk <= 2; ++rk) {
+    const StateDev &S = rk == 1 ? old : nw;
+    if (rk == 1) {
+      // manage_p_grad (mo_p_grad.F90:20-104); theta_v_adv_3rd_order of theta_v and theta_v_pert
+      if (o3) {
+        hipLaunchKernelGGL(k_theta_fill, g_cell, blk, 0, st, c, old, 2);
+        hipLaunchKernelGGL(k_theta_grad, g_uv, blk, 0, st, c, 2);
+        hipLaunchKernelGGL(k_theta_div, g_cell, blk, 0, st, c, 2);
+        hipLaunchKernelGGL(k_theta_edge, g_uv, blk, 0, st, c, (const double *)old.wind_u, (const double *)old.wind_v, 2);
+      }
+      hipLaunchKernelGGL(k_pgrad, g_all, blk, 0, st, c, old, step == 0 ? 1 : 0);
+    }
+    // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
+    if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
+      hipLaunchKernelGGL(k_pot_vort, g_all, blk, 0, st, c, S);
+      hipLaunchKernelGGL(k_mass_flux, g_all, blk, 0, st, c, S);
+      hipLaunchKernelGGL(k_vort_flux, g_uv, blk, 0, st, c);
+      if (!c.C.llinear) {
+        hipLaunchKernelGGL(k_ke, g_cell, blk, 0, st, c, S);
+        hipLaunchKernelGGL(k_ke_grad, g_uv, blk, 0, st, c);
+      }
+    }
+    if (rk == 1 && c.C.lklemp) hipLaunchKernelGGL(k_dissipation, g_cell, blk, 0, st, c, old);
+    hipLaunchKernelGGL(k_vec_tend, g_uv, blk, 0, st, c, old, nw, rk);
+    // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
+    // pressure gradient are reused (same state, same winds)
+    if (o3 && rk == 2) {
+      hipLaunchKernelGGL(k_theta_fill, g_cell, blk, 0, st, c, nw, 1);
+      hipLaunchKernelGGL(k_theta_grad, g_uv, blk, 0, st, c, 1);
+      hipLaunchKernelGGL(k_theta_div, g_cell, blk, 0, st, c, 1);
+      hipLaunchKernelGGL(k_theta_edge, g_uv, blk, 0, st, c, (const double *)nw.wind_u, (const double *)nw.wind_v, 1);
+    }
+    hipLaunchKernelGGL(k_scalar_flux, g_uv, blk, 0, st, c, S, nw);
+    hipLaunchKernelGGL(k_scalar_div, g_cell, blk, 0, st, c, S, nw, rk);
+    // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
+    if (prof) prof->begin(0);
+    hipLaunchKernelGGL(k_solve_ver, g_col, b_col, 0, st, c, old, nw, rk);
+    if (prof) prof->end(0);
+    if (c.G.nc > 1) {
+      if (prof) prof->begin(1);
+      hipLaunchKernelGGL(k_solve_gen, g_col, b_col, 0, st, c, old, nw, rk);
+      if (prof) prof->end(1);
+    }
+  }
+}
+
+}  // namespace lgame
+
+namespace {
+
+// Records a pair of events around each timed kernel group on the handle's stream.
+struct EventProf : ProfHook {
+  lgame_handle *h;
+  int open_idx[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  bool failed = false;
+  explicit EventProf(lgame_handle *hh) : h(hh) {}
+  int next_event() {
+    if (h->ev_next == h->ev_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        failed = true;
+        return -1;
+      }
+      h->ev_pool.push_back(e);
+    }
+    return (int)h->ev_next++;
+  }
+  void begin(int group) override {
+    const int i = next_event();
+    if (i < 0) return;
+    if (hipEventRecord(h->ev_pool[i], h->stream) != hipSuccess) failed = true;
+    open_idx[group] = i;
+  }
+  void end(int group) override {
+    const int i = next_event();
+    if (i < 0 || open_idx[group] < 0) return;
+    if (hipEventRecord(h->ev_pool[i], h->stream) != hipSuccess) failed = true;
+    h->ev_marks.push_back({group, {open_idx[group], i}});
+    open_idx[group] = -1;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int lgame_profile(lgame_handle *h, int enable) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->prof = enable != 0;
+  h->ev_marks.clear();
+  h->ev_next = 0;
+  return LGAME_OK;
+}
+
+int lgame_profile_read(lgame_handle *h, int group, double *total_ms, long long *count) {
+  if (!h || !total_ms || !count) return fail(h, LGAME_EINVAL, "null argument");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double tot = 0.0;
+  long long n = 0;
+  for (auto &m : h->ev_marks) {
+    if (m.first != group) continue;
+    float ms = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev_pool[m.second.first], h->ev_pool[m.second.second]));
+    tot += ms;
+    ++n;
+  }
+  *total_ms = tot;
+  *count = n;
+  return LGAME_OK;
+}
+
+int lgame_profile_reset(lgame_handle *h) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ev_marks.clear();
+  h->ev_next = 0;
+  return LGAME_OK;
+}
+
+int lgame_abi_version(void) { return LGAME_ABI_VERSION; }
+
+const char *lgame_last_error(const lgame_handle *h) { return h ? h->err.c_str() : g_init_error.c_str(); }
+
+int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_view *grid, lgame_handle **out) {
+  g_init_error.clear();
+  if (!dims || !cfg || !grid || !out) return fail(nullptr, LGAME_EINVAL, "null argument");
+  *out = nullptr;
+  const lgame_dims &d = *dims;
+  const lgame_cfg &f = *cfg;
+  if (d.ny < 3 || d.nx < 3 || d.n_layers < 3) return fail(nullptr, LGAME_EINVAL, "ny, nx and n_layers must be >= 3");
+  if (d.n_oro_layers < 0 || d.n_oro_layers >= d.n_layers)
+    return fail(nullptr, LGAME_EINVAL, "it must be 0 <= n_oro_layers < n_layers (mo_run_nml.F90:133-136)");
+  if (f.theta_adv_order != 2 && f.theta_adv_order != 3) return fail(nullptr, LGAME_EINVAL, "theta_adv_order must be 2 or 3");
+  const bool moist = f.lmoist != 0;
+  if (moist ? (d.n_condensed_constituents != 5 || d.n_constituents != 7) : (d.n_condensed_constituents != 0 || d.n_constituents != 1))
+    return fail(nullptr, LGAME_EINVAL, "constituent counts must be 1/0 (dry) or 7/5 (moist)");
+  if (!f.lperiodic) return fail(nullptr, LGAME_EINVAL, "unsupported: lperiodic=.false. (lateral boundary relaxation)");
+  if (!f.luse_bg_state) return fail(nullptr, LGAME_EINVAL, "unsupported: luse_bg_state=.false.");
+  if (f.lmom_diff_h || f.lmom_diff_v || f.ltemp_diff_h || f.ltemp_diff_v || f.lmass_diff_h || f.lmass_diff_v)
+    return fail(nullptr, LGAME_EINVAL, "unsupported: diffusion switches must be off");
+  if (f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
+    return fail(nullptr, LGAME_EINVAL, "unsupported: PBL, surface fluxes and soil must be off");
+  if (f.lrad || f.rad_mode != LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "unsupported: radiation");
+  if (f.precision != LGAME_PREC_FP64) return fail(nullptr, LGAME_EINVAL, "unsupported precision mode %d", f.precision);
+  if (!(f.dtime > 0.0) || !(f.toa > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime and toa must be positive");
+#define NEED(p) \
+  if (!grid->p) return fail(nullptr, LGAME_EINVAL, "grid field " #p " is null");
+  NEED(z_scalar) NEED(z_u) NEED(z_v) NEED(z_w) NEED(dx) NEED(dy) NEED(dz) NEED(area_x) NEED(area_y) NEED(area_z) NEED(volume)
+  NEED(slope_x) NEED(slope_y) NEED(inner_product_weights) NEED(area_dual_x) NEED(area_dual_y) NEED(area_dual_z) NEED(z_area_dual_z)
+  NEED(fvec_x) NEED(fvec_y) NEED(fvec_z) NEED(trsk_weights_u) NEED(trsk_weights_v) NEED(exner_bg_grad_u) NEED(exner_bg_grad_v)
+  NEED(exner_bg_grad_w) NEED(theta_v_bg) NEED(exner_bg) NEED(gravity_m_v) NEED(gravity_potential)
+#undef NEED
+
+  lgame_handle *h = new lgame_handle();
+  h->dims = d;
+  h->cfg = f;
+  Geom &G = h->c.G;
+  G.ny = d.ny;
+  G.nx = d.nx;
+  G.nl = d.n_layers;
+  G.nlev = d.n_layers + 1;
+  G.nflat = d.n_layers - d.n_oro_layers;
+  G.nc = d.n_constituents;
+  G.ncond = d.n_condensed_constituents;
+  G.pitch = ((d.nx + 1) + 15) / 16 * 16;
+  G.lvl = (long long)(d.ny + 1) * G.pitch;
+  G.fsize = (long long)G.nlev * G.lvl;
+  Cfg &C = h->c.C;
+  C.dtime = f.dtime;
+  C.toa = f.toa;
+  C.dxn = f.dx;
+  C.dyn = f.dy;
+  C.klemp_damp_max = f.klemp_damp_max;
+  C.klemp_begin_rel = f.klemp_begin_rel;
+  C.order = f.theta_adv_order;
+  C.llinear = f.llinear != 0;
+  C.lcorio = f.lcorio != 0;
+  C.lmoist = moist;
+  C.lklemp = f.lklemp != 0;
+  C.lfreeslip = f.lfreeslip != 0;
+
+  // n_damping_levels, as mo_grid_generator.F90:1050-1058 derives it from z_w
+  {
+    const int ny = d.ny, nx = d.nx, nlev = G.nlev;
+    C.n_damping_levels = 0;
+    for (int jl = 1; jl <= nlev; ++jl) {
+      double mx = -1e300;
+      for (size_t q = 0; q < (size_t)ny * nx; ++q) mx = std::fmax(mx, grid->z_w[q + (size_t)(jl - 1) * ny * nx]);
+      if (mx > f.klemp_begin_rel * f.toa) C.n_damping_levels += 1;
+    }
+  }
+
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    fail(nullptr, LGAME_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    delete h;
+    return LGAME_EHIP;
+  }
+  // field budget: 2 states, tendencies, carried diagnostics, scratch and the grid
+  const int nc = d.n_constituents;
+  const size_t nstate = (size_t)2 * (nc + 6);
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 7;
+  const size_t ngrid = 40;
+  h->slab_fields = nstate + nwork + ngrid;
+  const size_t bytes = h->slab_fields * (size_t)G.fsize * sizeof(double);
+  e = hipMalloc(&h->slab, bytes);
+  if (e != hipSuccess) {
+    fail(nullptr, LGAME_ENOMEM, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LGAME_ENOMEM;
+  }
+  size_t used = 0;
+  auto take = [&]() { return h->slab + (used++) * (size_t)G.fsize; };
+  for (int s = 0; s < 2; ++s) {
+    StateDev &S = h->slot[s];
+    for (int jc = 0; jc < MAX_CONST; ++jc) S.rho[jc] = jc < nc ? take() : nullptr;
+    S.rhotheta_v = take();
+    S.theta_v_pert = take();
+    S.exner_pert = take();
+    S.wind_u = take();
+    S.wind_v = take();
+    S.wind_w = take();
+  }
+  WorkDev &W = h->c.w;
+  for (int jc = 0; jc < MAX_CONST; ++jc) W.tend_rho[jc] = jc < nc ? take() : nullptr;
+  W.tend_rhotheta_v = take();
+  W.tend_u = take();
+  W.tend_v = take();
+  W.tend_w = take();
+  W.pnl_u = take();
+  W.pnl_v = take();
+  W.pnl_w = take();
+  W.pl_u = take();
+  W.pl_v = take();
+  W.pl_w = take();
+  W.pold_u = take();
+  W.pold_v = take();
+  W.vsg_x = take();
+  W.vsg_y = take();
+  W.vsg_z = take();
+  W.pvt_x = take();
+  W.pvt_y = take();
+  W.pvt_z = take();
+  W.heating_diss = take();
+  W.csh = take();
+  W.temperature = take();
+  W.radiation_tendency = take();
+  W.eta_x = take();
+  W.eta_y = take();
+  W.eta_z = take();
+  W.u_ph = take();
+  W.v_ph = take();
+  W.w_ph = take();
+  W.v_squared = take();
+  for (int q = 0; q < 2; ++q) {
+    W.tth[q] = take();
+    W.tgu[q] = take();
+    W.tgv[q] = take();
+    W.tfd[q] = take();
+    W.tu[q] = take();
+    W.tv[q] = take();
+  }
+  W.fu = take();
+  W.fv = take();
+  W.thu = take();
+  W.thv = take();
+  for (int q = 0; q < 7; ++q) W.col[q] = take();
+  W.col[7] = nullptr;
+  GridDev &g = h->c.g;
+  double *gz_scalar = take(), *gz_u = take(), *gz_v = take(), *gz_w = take(), *gdx = take(), *gdy = take(), *gdz = take();
+  double *garea_x = take(), *garea_y = take(), *garea_z = take(), *gvolume = take(), *gslope_x = take(), *gslope_y = take();
+  double *gipw = take();
+  for (int q = 1; q < 6; ++q) take();
+  double *gadx = take(), *gady = take(), *gadz = take(), *gzadz = take(), *gfx = take(), *gfy = take(), *gfz = take();
+  double *gebu = take(), *gebv = take(), *gebw = take(), *gtb = take(), *geb = take(), *ggmv = take(), *ggp = take(), *gkp = take();
+  if (used > h->slab_fields) {
+    fail(nullptr, LGAME_EINVAL, "internal: field budget exceeded");
+    lgame_finalize(h);
+    return LGAME_EINVAL;
+  }
+  e = hipMemsetAsync(h->slab, 0, bytes, h->stream);
+  h->staging_elems = (size_t)d.ny * d.nx * d.n_layers * (nc > 6 ? nc : 6) + (size_t)(d.ny + 1) * (d.nx + 1) * (d.n_layers + 1);
+  if (e == hipSuccess) e = hipMalloc(&h->staging, h->staging_elems * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->red, (size_t)2 * NRED * RED_GRID * sizeof(double) + NRED * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&h->trsk, (size_t)(3 * d.ny + 2 * (d.ny + 1)) * sizeof(double));
+  if (e != hipSuccess) {
+    fail(nullptr, LGAME_ENOMEM, "device allocation: %s", hipGetErrorString(e));
+    lgame_finalize(h);
+    return LGAME_ENOMEM;
+  }
+  const int ny = d.ny, nx = d.nx, nl = d.n_layers, nlev = nl + 1;
+  int rc = 0;
+  rc = rc ? rc : upload(h, gz_scalar, grid->z_scalar, ny, nx, nl);
+  rc = rc ? rc : upload(h, gz_u, grid->z_u, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gz_v, grid->z_v, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gz_w, grid->z_w, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gdx, grid->dx, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gdy, grid->dy, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gdz, grid->dz, ny, nx, nlev);
+  rc = rc ? rc : upload(h, garea_x, grid->area_x, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, garea_y, grid->area_y, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, garea_z, grid->area_z, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gvolume, grid->volume, ny, nx, nl);
+  rc = rc ? rc : upload(h, gslope_x, grid->slope_x, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gslope_y, grid->slope_y, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gipw, grid->inner_product_weights, ny, nx, nl, 6, 6);
+  rc = rc ? rc : upload(h, gadx, grid->area_dual_x, ny + 1, nx, nlev);
+  rc = rc ? rc : upload(h, gady, grid->area_dual_y, ny, nx + 1, nlev);
+  rc = rc ? rc : upload(h, gadz, grid->area_dual_z, ny + 1, nx + 1, nl);
+  rc = rc ? rc : upload(h, gzadz, grid->z_area_dual_z, ny + 1, nx + 1, nl);
+  rc = rc ? rc : upload(h, gfx, grid->fvec_x, ny + 1, nx, 1);
+  rc = rc ? rc : upload(h, gfy, grid->fvec_y, ny, nx + 1, 1);
+  rc = rc ? rc : upload(h, gfz, grid->fvec_z, ny + 1, nx + 1, 1);
+  rc = rc ? rc : upload(h, gebu, grid->exner_bg_grad_u, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gebv, grid->exner_bg_grad_v, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gebw, grid->exner_bg_grad_w, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gtb, grid->theta_v_bg, ny, nx, nl);
+  rc = rc ? rc : upload(h, geb, grid->exner_bg, ny, nx, nl);
+  rc = rc ? rc : upload(h, ggmv, grid->gravity_m_v, ny, nx, nlev);
+  rc = rc ? rc : upload(h, ggp, grid->gravity_potential, ny, nx, nl);
+  if (!rc) {
+    std::vector<double> kp((size_t)ny * nx * nlev);
+    for (size_t q = 0; q < kp.size(); ++q) kp[q] = klemp_prefactor(f, grid->z_w[q]);
+    rc = upload(h, gkp, kp.data(), ny, nx, nlev);
+  }
+  if (!rc) {
+    std::vector<double> tw((size_t)3 * ny + 2 * (ny + 1));
+    std::memcpy(tw.data(), grid->trsk_weights_u, sizeof(double) * 3 * ny);
+    std::memcpy(tw.data() + 3 * ny, grid->trsk_weights_v, sizeof(double) * 2 * (ny + 1));
+    e = hipMemcpy(h->trsk, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(h, LGAME_EHIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    g_init_error = h->err;
+    lgame_finalize(h);
+    return rc;
+  }
+  g.z_scalar = gz_scalar;
+  g.z_u = gz_u;
+  g.z_v = gz_v;
+  g.z_w = gz_w;
+  g.dx = gdx;
+  g.dy = gdy;
+  g.dz = gdz;
+  g.area_x = garea_x;
+  g.area_y = garea_y;
+  g.area_z = garea_z;
+  g.volume = gvolume;
+  g.slope_x = gslope_x;
+  g.slope_y = gslope_y;
+  for (int q = 0; q < 6; ++q) g.ipw[q] = gipw + (size_t)q * G.fsize;
+  g.area_dual_x = gadx;
+  g.area_dual_y = gady;
+  g.area_dual_z = gadz;
+  g.z_area_dual_z = gzadz;
+  g.fvec_x = gfx;
+  g.fvec_y = gfy;
+  g.fvec_z = gfz;
+  g.ebg_u = gebu;
+  g.ebg_v = gebv;
+  g.ebg_w = gebw;
+  g.theta_v_bg = gtb;
+  g.exner_bg = geb;
+  g.gravity_m_v = ggmv;
+  g.gravity_potential = ggp;
+  g.klemp_pref = gkp;
+  g.trsk_u = h->trsk;
+  g.trsk_v = h->trsk + 3 * ny;
+  *out = h;
+  return LGAME_OK;
+}
+
+static int state_io(lgame_handle *h, int slot, lgame_state_view *st, bool up) {
+  if (!h || !st) return fail(h, LGAME_EINVAL, "null argument");
+  if (slot != 0 && slot != 1) return fail(h, LGAME_EINVAL, "slot must be 0 or 1");
+  const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers, nc = h->dims.n_constituents;
+  StateDev &S = h->slot[slot];
+  if (!st->rho || !st->rhotheta_v || !st->theta_v_pert || !st->exner_pert || !st->wind_u || !st->wind_v || !st->wind_w)
+    return fail(h, LGAME_EINVAL, "null state field");
+  int rc = 0;
+  const size_t ncell = (size_t)ny * nx * nl;
+  for (int jc = 0; jc < nc && !rc; ++jc)
+    rc = up ? upload(h, S.rho[jc], st->rho + jc * ncell, ny, nx, nl) : download(h, st->rho + jc * ncell, S.rho[jc], ny, nx, nl);
+  struct F {
+    double *host, *dev;
+    int n1, n2, n3;
+  } fs[] = {{st->rhotheta_v, S.rhotheta_v, ny, nx, nl},     {st->theta_v_pert, S.theta_v_pert, ny, nx, nl},
+            {st->exner_pert, S.exner_pert, ny, nx, nl},     {st->wind_u, S.wind_u, ny, nx + 1, nl},
+            {st->wind_v, S.wind_v, ny + 1, nx, nl},         {st->wind_w, S.wind_w, ny, nx, nl + 1}};
+  for (auto &x : fs) {
+    if (rc) break;
+    rc = up ? upload(h, x.dev, x.host, x.n1, x.n2, x.n3) : download(h, x.host, x.dev, x.n1, x.n2, x.n3);
+  }
+  if (!rc && up) h->have_state[slot] = 1;
+  return rc;
+}
+
+int lgame_set_state(lgame_handle *h, int slot, const lgame_state_view *state) {
+  return state_io(h, slot, const_cast<lgame_state_view *>(state), true);
+}
+
+int lgame_get_state(lgame_handle *h, int slot, lgame_state_view *state) {
+  if (h && (slot == 0 || slot == 1) && !h->have_state[slot]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot);
+  return state_io(h, slot, state, false);
+}
+
+int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_counter, int lrad_update, double t_0) {
+  (void)t_0;
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  if ((slot_old != 0 && slot_old != 1) || (slot_new != 0 && slot_new != 1) || slot_old == slot_new)
+    return fail(h, LGAME_EINVAL, "slots must be 0 and 1, distinct");
+  if (!h->have_state[slot_old]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot_old);
+  if (!h->have_state[slot_new]) return fail(h, LGAME_ESTATE, "slot %d holds no state (copy state_old there first)", slot_new);
+  if (lrad_update && h->cfg.lrad) return fail(h, LGAME_EINVAL, "radiation is not available");
+  if (total_step_counter < 0) return fail(h, LGAME_EINVAL, "negative step counter");
+  if (h->prof) {
+    EventProf ep(h);
+    ep.begin(LGAME_PROF_STEP);
+    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
+    ep.end(LGAME_PROF_STEP);
+    if (ep.failed) return fail(h, LGAME_EHIP, "event recording failed");
+  } else {
+    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
+  }
+  HIPCHK(h, hipGetLastError());
+  h->stepped = 1;
+  return LGAME_OK;
+}
+
+int lgame_pchevi_host(lgame_handle *h, const lgame_state_view *state_old, lgame_state_view *state_new, int total_step_counter,
+                      int lrad_update, double t_0) {
+  const int a = total_step_counter % 2 == 0 ? 0 : 1, b = 1 - a;
+  int rc = lgame_set_state(h, a, state_old);
+  if (!rc) rc = lgame_set_state(h, b, state_new);
+  if (!rc) rc = lgame_pchevi(h, a, b, total_step_counter, lrad_update, t_0);
+  if (!rc) rc = lgame_get_state(h, b, state_new);
+  return rc;
+}
+
+int lgame_sync(lgame_handle *h) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return LGAME_OK;
+}
+
+int lgame_integrals(lgame_handle *h, int slot, double *out) {
+  if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
+  if ((slot != 0 && slot != 1) || !h->have_state[slot]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot);
+  const int nq = 4 + h->dims.n_constituents;
+  double *partial = h->red, *res = h->red + 2 * NRED * RED_GRID;
+  if (!h->stepped) {  // before the first step the temperature is diagnosed from this state (control.F90:484)
+    const Geom &G = h->c.G;
+    hipLaunchKernelGGL(k_temperature, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->c, h->slot[slot]);
+  }
+  hipLaunchKernelGGL(k_integrals, dim3(RED_GRID), dim3(RED_BLOCK), 0, h->stream, h->c, h->slot[slot], partial);
+  hipLaunchKernelGGL(k_integrals_final, dim3(1), dim3(RED_BLOCK), 0, h->stream, (const double *)partial, nq, res);
+  HIPCHK(h, hipGetLastError());
+  double tmp[NRED];
+  HIPCHK(h, hipMemcpyAsync(tmp, res, nq * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  tmp[LGAME_INT_KINETIC] = 0.5 * tmp[LGAME_INT_KINETIC];
+  std::memcpy(out, tmp, nq * sizeof(double));
+  return LGAME_OK;
+}
+
+int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad) {
+  if (!h || !n_bad) return fail(h, LGAME_EINVAL, "null argument");
+  if ((slot != 0 && slot != 1) || !h->have_state[slot]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot);
+  HIPCHK(h, hipMemsetAsync(h->counter, 0, sizeof(unsigned long long), h->stream));
+  const Geom &G = h->c.G;
+  hipLaunchKernelGGL(k_count_nonfinite, grid3(G.nx + 1, G.ny + 1, G.nlev), dim3(BX, BY, 1), 0, h->stream, h->c, h->slot[slot], h->counter);
+  HIPCHK(h, hipGetLastError());
+  unsigned long long v = 0;
+  HIPCHK(h, hipMemcpyAsync(&v, h->counter, sizeof v, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *n_bad = (long long)v;
+  return v ? LGAME_ENONFINITE : LGAME_OK;
+}
+
+int lgame_get_diag(lgame_handle *h, int which, double *out) {
+  if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
+  if (which != LGAME_DIAG_TEMPERATURE) return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
+  return download(h, out, h->c.w.temperature, h->dims.ny, h->dims.nx, h->dims.n_layers);
+}
+
+void lgame_finalize(lgame_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->slab) (void)hipFree(h->slab);
+  if (h->staging) (void)hipFree(h->staging);
+  if (h->red) (void)hipFree(h->red);
+  if (h->counter) (void)hipFree(h->counter);
+  if (h->trsk) (void)hipFree(h->trsk);
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"

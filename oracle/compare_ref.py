@@ -1,0 +1,53 @@
+"""Run the C oracle and the reference harness on one case and print the comparison as JSON.
+
+TEST INFRASTRUCTURE ONLY.  The reference harness keeps Fortran module state, so each comparison
+runs in its own process:  python oracle/compare_ref.py '<json case>'
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "l-game_amd"))
+sys.path.insert(0, HERE)
+
+from lgame.cases import make_case, schaer_config, tiny_config, dry3d_config  # noqa: E402
+from pyoracle import CpuOracle, RefHarness  # noqa: E402
+
+
+def build_case(spec):
+    kind = spec.get("kind", "tiny")
+    kw = spec.get("kw", {})
+    if kind == "tiny":
+        cfg = tiny_config(**kw)
+    elif kind == "schaer":
+        cfg = schaer_config(**kw)
+    else:
+        cfg = dry3d_config(**kw)
+    return cfg, make_case(cfg, cloud_frac=spec.get("cloud_frac", 0.0))
+
+
+def main():
+    spec = json.loads(sys.argv[1])
+    cfg, (g, st) = build_case(spec)
+    n = spec.get("steps", 3)
+    o = CpuOracle(cfg, g)
+    o.start(st)
+    o.run(0, n)
+    r = RefHarness(cfg, g)
+    r.start(st)
+    r.run(0, n)
+    slot = n % 2
+    a, b = o.get_state(slot), r.get_state(slot)
+    res = {}
+    for k in a:
+        d = np.abs(a[k] - b[k]).max()
+        res[k] = {"bitwise": bool(np.array_equal(a[k], b[k])), "maxabs": float(d),
+                  "moved": float(np.abs(a[k] - st[k]).max())}
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,31 @@
+"""Loading of the golden fixtures in tests/golden (written by tests/golden/make_golden.py)."""
+import json
+import os
+
+import numpy as np
+
+from lgame.config import GRID_FIELDS, STATE_FIELDS, RunConfig
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    c = dict(meta["config"])
+    c["hills"] = tuple(tuple(h) for h in c["hills"])
+    cfg = RunConfig(**c)
+    grid = {k: np.asfortranarray(z[f"grid_{k}"]) for k in GRID_FIELDS}
+    st_in = {k: np.asfortranarray(z[f"in_{k}"]) for k in STATE_FIELDS}
+    st_out = {k: np.asfortranarray(z[f"out_{k}"]) for k in STATE_FIELDS}
+    return cfg, grid, st_in, st_out, meta["steps"]
+
+
+def max_rel(a, b):
+    """max |a-b| / max |b| over the fields of two states."""
+    worst = 0.0
+    for k in b:
+        scale = max(float(np.abs(b[k]).max()), 1e-300)
+        worst = max(worst, float(np.abs(a[k] - b[k]).max()) / scale)
+    return worst

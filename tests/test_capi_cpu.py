@@ -1,0 +1,64 @@
+"""C-ABI checks that need no GPU: the library loads, exports what include/lgame_gpu.h declares,
+and rejects invalid or unsupported configurations before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lgame.cases import tiny_config
+from lgame.config import GRID_FIELDS
+from lgame.gpu import EXPORTED, LIB_PATH, GridView, load_library
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lgame_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*(lgame_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_bound_functions():
+    assert declared_functions() == sorted(EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def _init(mutate_dims=None, mutate_cfg=None):
+    lib = load_library()
+    cfg = tiny_config()
+    d, c = cfg.to_abi()
+    if mutate_dims:
+        mutate_dims(d)
+    if mutate_cfg:
+        mutate_cfg(c)
+    buf = ctypes.create_string_buffer(8 * 8)  # never read: validation fails first
+    gv = GridView(*([ctypes.addressof(buf)] * len(GRID_FIELDS)))
+    h = ctypes.c_void_p()
+    rc = lib.lgame_init(ctypes.byref(d), ctypes.byref(c), ctypes.byref(gv), ctypes.byref(h))
+    return rc, lib.lgame_last_error(None).decode(), h
+
+
+@pytest.mark.parametrize("mutate_dims,mutate_cfg,msg", [
+    (lambda d: setattr(d, "n_oro_layers", d.n_layers), None, "n_oro_layers"),
+    (lambda d: setattr(d, "ny", 2), None, ">= 3"),
+    (None, lambda c: setattr(c, "theta_adv_order", 4), "theta_adv_order"),
+    (None, lambda c: setattr(c, "lperiodic", 0), "lperiodic"),
+    (None, lambda c: setattr(c, "lmom_diff_h", 1), "diffusion"),
+    (None, lambda c: setattr(c, "lmoist", 1), "constituent"),
+    (None, lambda c: setattr(c, "dtime", 0.0), "dtime"),
+])
+def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
+    rc, err, h = _init(mutate_dims, mutate_cfg)
+    assert rc == 1  # LGAME_EINVAL
+    assert msg in err
+    assert not h.value
+
+
+def test_abi_version():
+    assert load_library().lgame_abi_version() == 1

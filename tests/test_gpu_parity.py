@@ -1,0 +1,134 @@
+"""Parity of the HIP step (through the C-ABI) with the CPU oracle and the golden fixtures.
+
+Bar: FP64 plain step within 1e-12 relative (max |gpu - oracle| / max |oracle| per prognostic
+field) after N steps; the kernels restate the reference's expression trees without FMA, so the
+fields are expected to agree bit for bit, which is reported separately.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from golden_util import NAMES, load, max_rel
+from lgame.cases import dry3d_config, make_case, schaer_config, tiny_config
+from lgame.gpu import LgameError, Step
+from pyoracle import CpuOracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def run_both(cfg, nsteps, cloud_frac=0.0, seed=None):
+    grid, st = make_case(cfg, seed=seed, cloud_frac=cloud_frac)
+    print(f"case ny={cfg.ny} nx={cfg.nx} nl={cfg.n_layers} moist={cfg.lmoist} order={cfg.theta_adv_order} "
+          f"freeslip={cfg.lfreeslip} steps={nsteps}", flush=True)
+    g = Step(cfg, grid)
+    g.start(st)
+    g.run(0, nsteps)
+    got = g.get_state(nsteps % 2)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.run(0, nsteps)
+    want = o.get_state(nsteps % 2)
+    g.close()
+    return got, want, st
+
+
+CASES = {
+    "tiny_dry_o2": (tiny_config(), 4, 0.0),
+    "tiny_dry_o3_freeslip": (tiny_config(order=3, freeslip=True), 5, 0.0),
+    "tiny_plane_nocorio": (tiny_config(corio=False, plane=True), 4, 0.0),
+    "tiny_linear": (tiny_config(ny=7, nx=5, n_layers=8, llinear=True), 3, 0.0),
+    "tiny_moist_o3_freeslip": (tiny_config(moist=True, order=3, freeslip=True), 5, 0.3),
+    "tiny_moist_o2": (tiny_config(ny=9, nx=11, n_layers=7, moist=True), 4, 0.2),
+    "schaer_61x24": (schaer_config(nx=61, n_layers=24), 30, 0.0),
+    "dry3d_33x33x20": (dry3d_config(n=33, n_layers=20), 8, 0.0),
+    "dry3d_66x40x18_even": (dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66), 6, 0.0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_matches_oracle(name):
+    cfg, n, cf = CASES[name]
+    got, want, st = run_both(cfg, n, cloud_frac=cf)
+    rel = max_rel(got, want)
+    bitwise = all(np.array_equal(got[k], want[k]) for k in want)
+    print(f"{name}: max relative deviation {rel:.3e}, bitwise={bitwise}")
+    assert rel <= TOL
+    assert np.abs(got["wind_w"] - st["wind_w"]).max() > 0.0
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gpu_matches_golden(name):
+    cfg, grid, st_in, st_out, steps = load(name)
+    g = Step(cfg, grid)
+    g.start(st_in)
+    g.run(0, steps)
+    got = g.get_state(steps % 2)
+    g.close()
+    rel = max_rel(got, st_out)
+    print(f"{name}: max relative deviation from the reference fixture {rel:.3e}")
+    assert rel <= TOL
+
+
+def test_host_drop_in_path_matches_resident_path():
+    cfg = tiny_config(order=3, freeslip=True)
+    grid, st = make_case(cfg)
+    a = Step(cfg, grid)
+    a.start(st)
+    a.run(0, 3)
+    res = a.get_state(1)
+    b = Step(cfg, grid)
+    new = {k: v.copy() for k, v in st.items()}
+    old = {k: v.copy() for k, v in st.items()}
+    from lgame.gpu import StateView
+    import ctypes
+    for s in range(3):
+        ao = {k: np.asfortranarray(v) for k, v in old.items()}
+        an = {k: np.asfortranarray(v) for k, v in new.items()}
+        vo = StateView(*[ao[k].ctypes.data for k in ao], None)
+        vn = StateView(*[an[k].ctypes.data for k in an], None)
+        rc = b.lib.lgame_pchevi_host(b.h, ctypes.byref(vo), ctypes.byref(vn), s, 0, 0.0)
+        assert rc == 0, b.lib.lgame_last_error(b.h)
+        old, new = an, ao
+    for k in res:
+        assert np.array_equal(res[k], old[k]), k
+
+
+def test_integrals_match_oracle():
+    cfg = tiny_config(moist=True, order=3, freeslip=True)
+    grid, st = make_case(cfg, cloud_frac=0.2)
+    g = Step(cfg, grid)
+    g.start(st)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    i0 = g.integrals(0)
+    o.lib().orc_diagnose_temperature(o.h, 0)
+    np.testing.assert_allclose(i0, o.integrals(0), rtol=1e-13)
+    g.run(0, 3)
+    o.run(0, 3)
+    np.testing.assert_allclose(g.integrals(1), o.integrals(1), rtol=1e-13)
+
+
+def test_check_finite_flags_injected_nan():
+    cfg = tiny_config()
+    grid, st = make_case(cfg)
+    g = Step(cfg, grid)
+    g.start(st)
+    assert g.count_nonfinite(0) == 0
+    bad = {k: v.copy() for k, v in st.items()}
+    bad["exner_pert"][1, 2, 3] = np.nan
+    bad["wind_u"][0, 0, 0] = np.inf
+    g.set_state(1, bad)
+    assert g.count_nonfinite(1) == 2
+
+
+def test_step_argument_errors():
+    cfg = tiny_config()
+    grid, st = make_case(cfg)
+    g = Step(cfg, grid)
+    with pytest.raises(LgameError, match="holds no state"):
+        g.pchevi(0, 1, 0)
+    g.start(st)
+    with pytest.raises(LgameError, match="distinct"):
+        g.pchevi(0, 0, 0)
