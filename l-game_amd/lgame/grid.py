@@ -128,28 +128,29 @@ def _grad_hor(s, rz, g, cfg):
     below = np.ones(nl, dtype=bool)
     if not cfg.lfreeslip:
         below[nl - 1] = False
-    # x: neighbours (jk-1, jk) with periodic wrap for jk = 1 and nx+1
-    km = np.concatenate([[nx - 1], np.arange(0, nx - 1), [nx - 1]])
-    kp = np.concatenate([[0], np.arange(1, nx), [0]])
-    r = np.zeros((ny, nx + 1, nl))
-    r += ipw[4][:, km, :] * rz[:, km, 0:nl]
-    r += ipw[4][:, kp, :] * rz[:, kp, 0:nl]
-    add = ipw[5][:, km, :] * rz[:, km, 1:nl + 1] + 0.0
-    r2 = r + add
-    r2 = r2 + ipw[5][:, kp, :] * rz[:, kp, 1:nl + 1]
+    p5, p6 = ipw[4], ipw[5]
+
+    def wrapx(a):  # columns (nx, 1, 2, ..., nx, 1): left and right neighbours of u points
+        return np.concatenate([a[:, nx - 1:nx], a, a[:, 0:1]], axis=1)
+
+    def wrapy(a):
+        return np.concatenate([a[ny - 1:ny], a, a[0:1]], axis=0)
+
+    # remap_ver2hor_x: neighbours (jk-1, jk), periodic at jk = 1 and nx+1
+    a5, z0 = wrapx(p5), wrapx(rz[:, :, 0:nl])
+    a6, z1 = wrapx(p6), wrapx(rz[:, :, 1:nl + 1])
+    left, right = slice(0, nx + 1), slice(1, nx + 2)
+    r = (0.0 + a5[:, left] * z0[:, left]) + a5[:, right] * z0[:, right]
+    r2 = (r + a6[:, left] * z1[:, left]) + a6[:, right] * z1[:, right]
     r = np.where(below[None, None, :], r2, r)
-    rem = 0.5 * r
-    rx[:, :, nflat:] = rx[:, :, nflat:] - g["slope_x"][:, :, nflat:] * rem[:, :, nflat:]
-    im = np.concatenate([[ny - 1], np.arange(0, ny - 1), [ny - 1]])
-    ip = np.concatenate([[0], np.arange(1, ny), [0]])
-    r = np.zeros((ny + 1, nx, nl))
-    r += ipw[4][im, :, :] * rz[im, :, 0:nl]
-    r += ipw[4][ip, :, :] * rz[ip, :, 0:nl]
-    r2 = r + ipw[5][im, :, :] * rz[im, :, 1:nl + 1]
-    r2 = r2 + ipw[5][ip, :, :] * rz[ip, :, 1:nl + 1]
+    rx[:, :, nflat:] = rx[:, :, nflat:] - g["slope_x"][:, :, nflat:] * (0.5 * r)[:, :, nflat:]
+    a5, z0 = wrapy(p5), wrapy(rz[:, :, 0:nl])
+    a6, z1 = wrapy(p6), wrapy(rz[:, :, 1:nl + 1])
+    up, dn = slice(0, ny + 1), slice(1, ny + 2)
+    r = (0.0 + a5[up] * z0[up]) + a5[dn] * z0[dn]
+    r2 = (r + a6[up] * z1[up]) + a6[dn] * z1[dn]
     r = np.where(below[None, None, :], r2, r)
-    rem = 0.5 * r
-    ry[:, :, nflat:] = ry[:, :, nflat:] - g["slope_y"][:, :, nflat:] * rem[:, :, nflat:]
+    ry[:, :, nflat:] = ry[:, :, nflat:] - g["slope_y"][:, :, nflat:] * (0.5 * r)[:, :, nflat:]
     return rx, ry
 
 

@@ -93,11 +93,11 @@ def _smooth_periodic(field, sigma_cells):
     """Gaussian filter with periodic wrap along axes 0 and 1 (FFT), unit-variance rescaled."""
     ny, nx = field.shape[0], field.shape[1]
     ky = np.fft.fftfreq(ny)[:, None]
-    kx = np.fft.fftfreq(nx)[None, :]
+    kx = np.fft.rfftfreq(nx)[None, :]
     kern = np.exp(-2.0 * (np.pi * sigma_cells) ** 2 * (kx ** 2 + ky ** 2))
     if field.ndim == 3:
         kern = kern[:, :, None]
-    out = np.real(np.fft.ifft2(np.fft.fft2(field, axes=(0, 1)) * kern, axes=(0, 1)))
+    out = np.fft.irfft2(np.fft.rfft2(field, axes=(0, 1)) * kern, s=(ny, nx), axes=(0, 1))
     s = out.std()
     return out / s if s > 0 else out
 
