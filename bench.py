@@ -29,6 +29,10 @@ sys.path.insert(0, os.path.join(ROOT, "l-game_amd"))
 HBM_PEAK_GBS = 8000.0
 SOLVER_BYTES = {1: 152.0, 2: 176.0}          # per grid point, FP64, predictor / corrector
 STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step traffic model
+# HBM bytes per k_solve_ver dispatch, averaged over predictor and corrector, from two rocprofv3
+# --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB device copy gives);
+# profiles/moist512_fp64_hbm_traffic.md holds the commands and the per-kernel table.
+SOLVER_TRAFFIC = {"moist512": 4.929e9}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
 
 
@@ -82,6 +86,24 @@ def cpu_baseline(cfg, nsteps):
     return cfg.gridpoints * nsteps / dt, dt, worst
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of a known size (the reference kernel for the HBM byte counters)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +112,8 @@ def main():
     ap.add_argument("--workload", default="moist512", choices=["moist512", "dry128", "schaer"])
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -99,6 +123,8 @@ def main():
     from lgame.cases import make_case
     from lgame.gpu import PROF_SOLVE_VER, PROF_STEP, Step
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     cfg, sample_cfg = workload(args.workload)
     t0 = time.perf_counter()
     log(f"setting up {describe(cfg)}")
@@ -168,7 +194,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": solver_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": SOLVER_TRAFFIC.get(args.workload),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes_per_point": {"predictor": SOLVER_BYTES[1], "corrector": SOLVER_BYTES[2]},
             "avg_launch_us": avg_launch_us,
             "launches_timed": n_solver,

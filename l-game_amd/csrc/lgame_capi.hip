@@ -444,7 +444,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   // field budget: 2 states, tendencies, carried diagnostics, scratch and the grid
   const int nc = d.n_constituents;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 7;
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 12;
   const size_t ngrid = 40;
   h->slab_fields = nstate + nwork + ngrid;
   const size_t bytes = h->slab_fields * (size_t)G.fsize * sizeof(double);
@@ -510,8 +510,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   W.fv = take();
   W.thu = take();
   W.thv = take();
-  for (int q = 0; q < 7; ++q) W.col[q] = take();
-  W.col[7] = nullptr;
+  for (int q = 0; q < 12; ++q) W.col[q] = take();
+  W.col[12] = W.col[13] = nullptr;
   GridDev &g = h->c.g;
   double *gz_scalar = take(), *gz_u = take(), *gz_v = take(), *gz_w = take(), *gdx = take(), *gdy = take(), *gdz = take();
   double *garea_x = take(), *garea_y = take(), *garea_z = take(), *gvolume = take(), *gslope_x = take(), *gslope_y = take();

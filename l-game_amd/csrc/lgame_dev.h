@@ -64,7 +64,7 @@ struct WorkDev {
   double *eta_x, *eta_y, *eta_z, *u_ph, *v_ph, *w_ph, *v_squared;
   double *tth[2], *tgu[2], *tgv[2], *tfd[2], *tu[2], *tv[2];  // theta_v_adv_3rd_order: full, pert
   double *fu, *fv, *thu, *thv;                       // gas mass flux and theta_v flux
-  double *col[8];                                    // column-solver per-level scratch
+  double *col[14];  // column-solver per-level scratch
 };
 
 struct Cfg {

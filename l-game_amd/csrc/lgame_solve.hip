@@ -209,25 +209,15 @@ __device__ double v_fall_liquid(const Ctx &c, const StateDev &s, double radius, 
 
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
 
-// sink velocity of condensate jc (1-based) in layer jl; radii from mo_dictionary.F90:475-527
-__device__ __forceinline__ double v_fall(const Ctx &c, const StateDev &old, int jc, int ji, int jk, int jl) {
-  switch (jc) {
-    case 1:
-      return v_fall_solid(500.e-6);
-    case 2:
-    case 5:  // radius of rain drops and graupel: diag%a_rain, which the reference keeps at zero
-      return v_fall_liquid(c, old, 0.0, ji, jk, jl);
-    case 3:
-      return v_fall_solid(95.e-6);
-    case 4:
-      return v_fall_liquid(c, old, 12.e-6, ji, jk, jl);
-    default:
-      return 0.0;
-  }
-}
+// three_band_solver_gen_densities, mo_column_solvers.F90:353-604.  All tracers of a column are
+// eliminated in one sweep down the column (they share the winds, metrics, temperature and fall
+// speeds of each level); per level the tracers are visited in the reference's constituent order,
+// so the sediment heating accumulates in the same order.  Sink velocities: snow v_fall_solid of
+// 500e-6 m = 1 m/s, cloud ice v_fall_solid of 95e-6 m = 0.001 m/s, cloud water v_fall_liquid of
+// 12e-6 m; rain and graupel use the radius diag%a_rain, which the reference keeps at zero, so their
+// v_fall_liquid is exactly zero (mo_dictionary.F90:475-527, control.F90:443).
+constexpr int NTR = 6;  // tracers: snow, rain, cloud ice, cloud water, graupel, vapour
 
-// three_band_solver_gen_densities, mo_column_solvers.F90:353-604; all tracers of a column in one
-// lane so the sediment heating accumulates over the constituents in the reference's order
 __global__ void __launch_bounds__(BCOL) k_solve_gen(const Ctx c, const StateDev old, const StateDev nw, int rk) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond;
@@ -235,103 +225,127 @@ __global__ void __launch_bounds__(BCOL) k_solve_gen(const Ctx c, const StateDev 
   const int jk = col % nx + 1, ji = col / nx + 1;
   const double itw = 0.5, ew = 1.0 - itw;
   const double dt = c.C.dtime;
-  double *sep = c.w.col[0], *srp = c.w.col[1];
-  for (int jc = 1; jc <= c.G.nc; ++jc) {
-    if (jc == ncond + 1) continue;
-    const double *ro = old.rho[jc - 1];
-    const double *tr = c.w.tend_rho[jc - 1];
-    // interface fluxes of interface q (between layers q and q+1)
-    auto iface = [&](int q, double &fi, double &fr, double &fe) {
-      const size_t iw = LG_IX(c.G, ji, jk, q + 1);
-      fi = old.wind_w[iw];
-      fr = nw.wind_w[iw];
-      double vf = 0.0;
-      if (jc <= 5) vf = 0.5 * (v_fall(c, old, jc, ji, jk, q) + v_fall(c, old, jc, ji, jk, q + 1));
-      fi = fi - vf;
-      fr = fr - vf;
-      fi = c.g.area_z[iw] * fi;
-      fr = c.g.area_z[iw] * fr;
-      double dens;
-      const double tmp = AT(c.w.temperature, ji, jk, q + 1);
-      if (fr >= 0.0)
-        dens = AT(ro, ji, jk, q + 1);
-      else
-        dens = AT(ro, ji, jk, q);
-      fr = dens * fr;
-      fe = c_p_cond(jc, tmp) * tmp * fr;
-    };
-    double fi_up = 0.0, fr_up = 0.0, fe_up = 0.0;  // interface above the current layer
-    double fi_dn, fr_dn, fe_dn;                    // interface below
-    double ep_prev = 0.0, rp_prev = 0.0, c_prev = 0.0;
-    for (int jl = 1; jl <= nl; ++jl) {
-      const size_t i = LG_IX(c.G, ji, jk, jl);
-      const double V = c.g.volume[i];
-      if (jl < nl) iface(jl, fi_dn, fr_dn, fe_dn);
-      if (rk == 1 && jc == 1) c.w.csh[i] = 0.0;
+  const int jcs[NTR] = {1, 2, 3, 4, 5, 7};  // 1-based constituent index of each tracer
+  const double vf_snow = v_fall_solid(500.e-6), vf_ice = v_fall_solid(95.e-6);
+  double ep[NTR], rp[NTR], cprev[NTR], fi_up[NTR], fr_up[NTR], fe_up[NTR];
+#pragma unroll
+  for (int t = 0; t < NTR; ++t) ep[t] = rp[t] = cprev[t] = fi_up[t] = fr_up[t] = fe_up[t] = 0.0;
+  double vcw_lo = v_fall_liquid(c, old, 12.e-6, ji, jk, 1);  // cloud-water fall speed of the layer
+  for (int jl = 1; jl <= nl; ++jl) {
+    const size_t i = LG_IX(c.G, ji, jk, jl);
+    const double V = c.g.volume[i];
+    // interface below this layer (q = jl), shared by all tracers
+    double wo = 0.0, wn = 0.0, az = 0.0, tif = 0.0, cpi = 0.0, cpw = 0.0, vcw_up = vcw_lo, Vb = 0.0;
+    const bool below = jl < nl;
+    if (below) {
+      const size_t iw = LG_IX(c.G, ji, jk, jl + 1);
+      wo = old.wind_w[iw];
+      wn = nw.wind_w[iw];
+      az = c.g.area_z[iw];
+      tif = c.w.temperature[iw];  // temperature of layer jl+1 (used for both flux directions)
+      cpi = c_p_ice(tif);
+      cpw = c_p_water(tif);
+      vcw_lo = v_fall_liquid(c, old, 12.e-6, ji, jk, jl + 1);
+      Vb = c.g.volume[iw];
+    }
+    const double vcw_layer = vcw_up;  // v_fall of cloud water in layer jl
+    double csh = 0.0;
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+      const int jc = jcs[t];
+      const double *ro = old.rho[jc - 1];
+      double fi = 0.0, fr = 0.0, fe = 0.0;
+      if (below) {
+        double vf = 0.0;
+        if (jc == 1)
+          vf = 0.5 * (vf_snow + vf_snow);
+        else if (jc == 3)
+          vf = 0.5 * (vf_ice + vf_ice);
+        else if (jc == 4)
+          vf = 0.5 * (vcw_layer + vcw_lo);
+        else if (jc == 2 || jc == 5)
+          vf = 0.5 * (0.0 + 0.0);
+        fi = wo - vf;
+        fr = wn - vf;
+        fi = az * fi;
+        fr = az * fr;
+        const double dens = fr >= 0.0 ? ro[LG_IX(c.G, ji, jk, jl + 1)] : ro[i];
+        fr = dens * fr;
+        const double cp = ((jc - 1) % 2 == 0) ? cpi : cpw;
+        fe = cp * tif * fr;
+      }
       double d, e = 0.0, cq = 0.0;
       if (jl == 1) {
-        d = (fi_dn >= 0.0) ? 1.0 : 1.0 - itw * dt / V * fi_dn;
+        d = (fi >= 0.0) ? 1.0 : 1.0 - itw * dt / V * fi;
       } else if (jl == nl) {
-        d = (fi_up >= 0.0) ? 1.0 + itw * dt / V * fi_up : 1.0;
-        if (jc <= ncond) d = d + itw * v_fall(c, old, jc, ji, jk, nl) * dt * AT(c.g.area_z, ji, jk, nl + 1) / V;
+        d = (fi_up[t] >= 0.0) ? 1.0 + itw * dt / V * fi_up[t] : 1.0;
       } else {
         d = 1.0;
-        if (fi_up >= 0.0) d = d + itw * dt / V * fi_up;
-        if (fi_dn < 0.0) d = d - itw * dt / V * fi_dn;
+        if (fi_up[t] >= 0.0) d = d + itw * dt / V * fi_up[t];
+        if (fi < 0.0) d = d - itw * dt / V * fi;
       }
-      if (jl < nl) {  // off-diagonals of row jl (mo_column_solvers.F90:487-495)
-        if (fi_dn >= 0.0) {
+      double vfs = 0.0;  // sink velocity in the lowest layer
+      if (jl == nl && jc <= ncond) {
+        vfs = jc == 1 ? vf_snow : jc == 3 ? vf_ice : jc == 4 ? vcw_layer : 0.0;
+        d = d + itw * vfs * dt * AT(c.g.area_z, ji, jk, nl + 1) / V;
+      }
+      if (below) {
+        if (fi >= 0.0) {
           cq = 0.0;
-          e = -itw * dt / V * fi_dn;
+          e = -itw * dt / V * fi;
         } else {
-          cq = itw * dt / c.g.volume[LG_IX(c.G, ji, jk, jl + 1)] * fi_dn;
+          cq = itw * dt / Vb * fi;
           e = 0.0;
         }
       }
-      double r = ro[i] + dt * tr[i];
+      double r = ro[i] + dt * c.w.tend_rho[jc - 1][i];
       if (jl == 1) {
-        r = r + ew * dt * fr_dn / V;
-        if (rk == 1 && jc <= ncond) c.w.csh[i] = c.w.csh[i] + fe_dn / V;
+        r = r + ew * dt * fr / V;
+        if (rk == 1 && jc <= ncond) csh = csh + fe / V;
       } else if (jl == nl) {
-        r = r - ew * dt * fr_up / V;
-        if (rk == 1 && jc <= ncond) c.w.csh[i] = c.w.csh[i] - fe_up / V;
+        r = r - ew * dt * fr_up[t] / V;
+        if (rk == 1 && jc <= ncond) csh = csh - fe_up[t] / V;
         if (jc <= ncond) {
-          const double vfs = v_fall(c, old, jc, ji, jk, nl);
           r = r - ew * vfs * dt * ro[i] * AT(c.g.area_z, ji, jk, nl + 1) / V;
           if (jc > ncond / 4 && rk == 1) {
             const double tn = AT(c.w.temperature, ji, jk, nl);
-            c.w.csh[i] = c.w.csh[i] - vfs * tn * c_p_cond(jc, tn) * ro[i] * AT(c.g.area_z, ji, jk, nl + 1) / V;
+            csh = csh - vfs * tn * c_p_cond(jc, tn) * ro[i] * AT(c.g.area_z, ji, jk, nl + 1) / V;
           }
         }
       } else {
-        r = r + ew * dt * (-fr_up + fr_dn) / V;
-        if (rk == 1 && jc <= ncond) c.w.csh[i] = c.w.csh[i] + (-fe_up + fe_dn) / V;
+        r = r + ew * dt * (-fr_up[t] + fr) / V;
+        if (rk == 1 && jc <= ncond) csh = csh + (-fe_up[t] + fe) / V;
       }
       double epq, rpq;
       if (jl == 1) {
         epq = e / d;
         rpq = r / d;
       } else {
-        const double den = d - ep_prev * c_prev;
+        const double den = d - ep[t] * cprev[t];
         epq = e / den;
-        rpq = (r - rp_prev * c_prev) / den;
+        rpq = (r - rp[t] * cprev[t]) / den;
       }
-      sep[i] = epq;
-      srp[i] = rpq;
-      ep_prev = epq;
-      rp_prev = rpq;
-      c_prev = cq;
-      fi_up = fi_dn;
-      fr_up = fr_dn;
-      fe_up = fe_dn;
+      c.w.col[2 * t][i] = epq;
+      c.w.col[2 * t + 1][i] = rpq;
+      ep[t] = epq;
+      rp[t] = rpq;
+      cprev[t] = cq;
+      fi_up[t] = fi;
+      fr_up[t] = fr;
+      fe_up[t] = fe;
     }
-    double x = srp[LG_IX(c.G, ji, jk, nl)];
-    double *rn = nw.rho[jc - 1];
-    rn[LG_IX(c.G, ji, jk, nl)] = x < 0.0 ? 0.0 : x;
-    for (int jl = nl - 1; jl >= 1; --jl) {
-      const size_t i = LG_IX(c.G, ji, jk, jl);
-      x = srp[i] - sep[i] * x;
-      rn[i] = x < 0.0 ? 0.0 : x;
+    if (rk == 1) c.w.csh[i] = csh;
+  }
+  // back substitution and clipping of round-off negatives (mo_column_solvers.F90:584-596)
+  double x[NTR];
+#pragma unroll
+  for (int t = 0; t < NTR; ++t) x[t] = 0.0;
+  for (int jl = nl; jl >= 1; --jl) {
+    const size_t i = LG_IX(c.G, ji, jk, jl);
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+      x[t] = (jl == nl) ? c.w.col[2 * t + 1][i] : c.w.col[2 * t + 1][i] - c.w.col[2 * t][i] * x[t];
+      nw.rho[jcs[t] - 1][i] = x[t] < 0.0 ? 0.0 : x[t];
     }
   }
 }

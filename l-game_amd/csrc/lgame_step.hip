@@ -482,39 +482,66 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_flux(const Ctx c, const Stat
   }
 }
 
-// upstream flux of a tracer at a u face (scalar_times_vector_h_upstream, :206-287)
-__device__ __forceinline__ double upflux_u(const Ctx &c, const double *r, const double *u, int ji, int jk, int jl) {
-  const int k = (jk == c.G.nx + 1) ? 1 : jk;
-  const double uu = AT(u, ji, k, jl);
-  return (uu >= 0.0 ? AT(r, ji, wrapm(k, c.G.nx), jl) : AT(r, ji, k, jl)) * uu;
-}
-__device__ __forceinline__ double upflux_v(const Ctx &c, const double *r, const double *v, int ji, int jk, int jl) {
-  const int j = (ji == c.G.ny + 1) ? 1 : ji;
-  const double vv = AT(v, j, jk, jl);
-  return (vv >= 0.0 ? AT(r, j, jk, jl) : AT(r, wrapm(j, c.G.ny), jk, jl)) * vv;
+// Tracer fluxes of div_h_tracers (mo_divergence_operators.F90:70-147) with upstream face values
+// (scalar_times_vector_h_upstream, mo_multiplications.F90:206-287).  The wind-dependent parts
+// (face winds, upwind direction, contravariant corrections) are shared by all tracers of a cell.
+struct TracerFaces {
+  double ue, uw, vn, vs;   // face winds: east (jk+1), west (jk), v at ji, v at ji+1
+  size_t re, rw, rn, rs;   // upwind cell of each face
+  double cu, cl;           // contravariant corrections at the upper (jl) and lower (jl+1) interface
+};
+
+__device__ __forceinline__ TracerFaces tracer_faces(const Ctx &c, const StateDev &nw, int ji, int jk, int jl) {
+  const int nx = c.G.nx, ny = c.G.ny, nflat = c.G.nflat, nl = c.G.nl;
+  TracerFaces f;
+  // face jk+1 (the copy of face 1 when jk = nx)
+  {
+    const int k = (jk + 1 == nx + 1) ? 1 : jk + 1;
+    f.ue = AT(nw.wind_u, ji, k, jl);
+    f.re = f.ue >= 0.0 ? LG_IX(c.G, ji, wrapm(k, nx), jl) : LG_IX(c.G, ji, k, jl);
+  }
+  {
+    f.uw = AT(nw.wind_u, ji, jk, jl);
+    f.rw = f.uw >= 0.0 ? LG_IX(c.G, ji, wrapm(jk, nx), jl) : LG_IX(c.G, ji, jk, jl);
+  }
+  {
+    f.vn = AT(nw.wind_v, ji, jk, jl);
+    f.rn = f.vn >= 0.0 ? LG_IX(c.G, ji, jk, jl) : LG_IX(c.G, wrapm(ji, ny), jk, jl);
+  }
+  {
+    const int j = (ji + 1 == ny + 1) ? 1 : ji + 1;
+    f.vs = AT(nw.wind_v, j, jk, jl);
+    f.rs = f.vs >= 0.0 ? LG_IX(c.G, j, jk, jl) : LG_IX(c.G, wrapm(j, ny), jk, jl);
+  }
+  f.cu = 0.0;
+  f.cl = 0.0;
+  if (jl == nflat) {
+    f.cl = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl + 1);
+  } else if (jl == nl) {
+    f.cu = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl);
+  } else if (jl > nflat) {
+    f.cu = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl);
+    f.cl = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl + 1);
+  }
+  return f;
 }
 
-// div_h_tracers, mo_divergence_operators.F90:70-147
-__device__ double div_tracer(const Ctx &c, const double *r, const StateDev &nw, int ji, int jk, int jl) {
-  const double *u = nw.wind_u, *v = nw.wind_v;
-  double comp_h = upflux_u(c, r, u, ji, jk + 1, jl) * AT(c.g.area_x, ji, jk + 1, jl) + upflux_v(c, r, v, ji, jk, jl) * AT(c.g.area_y, ji, jk, jl) -
-                  upflux_u(c, r, u, ji, jk, jl) * AT(c.g.area_x, ji, jk, jl) - upflux_v(c, r, v, ji + 1, jk, jl) * AT(c.g.area_y, ji + 1, jk, jl);
+// div_h_tracers of one tracer field r, mo_divergence_operators.F90:94-145
+__device__ __forceinline__ double div_tracer(const Ctx &c, const TracerFaces &f, const double *r, int ji, int jk, int jl) {
+  const double comp_h = (r[f.re] * f.ue) * AT(c.g.area_x, ji, jk + 1, jl) + (r[f.rn] * f.vn) * AT(c.g.area_y, ji, jk, jl) -
+                        (r[f.rw] * f.uw) * AT(c.g.area_x, ji, jk, jl) - (r[f.rs] * f.vs) * AT(c.g.area_y, ji + 1, jk, jl);
   double comp_v = 0.0;
   const int nflat = c.G.nflat, nl = c.G.nl;
   if (jl == nflat) {
-    const double cl = vcc(c, u, v, ji, jk, jl + 1);
-    const double dl = cl <= 0.0 ? AT(r, ji, jk, jl) : AT(r, ji, jk, jl + 1);
-    comp_v = -dl * cl * AT(c.g.area_z, ji, jk, jl + 1);
+    const double dl = f.cl <= 0.0 ? AT(r, ji, jk, jl) : AT(r, ji, jk, jl + 1);
+    comp_v = -dl * f.cl * AT(c.g.area_z, ji, jk, jl + 1);
   } else if (jl == nl) {
-    const double cu = vcc(c, u, v, ji, jk, jl);
-    const double du = cu <= 0.0 ? AT(r, ji, jk, jl - 1) : AT(r, ji, jk, jl);
-    comp_v = du * cu * AT(c.g.area_z, ji, jk, jl);
+    const double du = f.cu <= 0.0 ? AT(r, ji, jk, jl - 1) : AT(r, ji, jk, jl);
+    comp_v = du * f.cu * AT(c.g.area_z, ji, jk, jl);
   } else if (jl > nflat) {
-    const double cu = vcc(c, u, v, ji, jk, jl);
-    const double du = cu <= 0.0 ? AT(r, ji, jk, jl - 1) : AT(r, ji, jk, jl);
-    const double cl = vcc(c, u, v, ji, jk, jl + 1);
-    const double dl = cl <= 0.0 ? AT(r, ji, jk, jl) : AT(r, ji, jk, jl + 1);
-    comp_v = du * cu * AT(c.g.area_z, ji, jk, jl) - dl * cl * AT(c.g.area_z, ji, jk, jl + 1);
+    const double du = f.cu <= 0.0 ? AT(r, ji, jk, jl - 1) : AT(r, ji, jk, jl);
+    const double dl = f.cl <= 0.0 ? AT(r, ji, jk, jl) : AT(r, ji, jk, jl + 1);
+    comp_v = du * f.cu * AT(c.g.area_z, ji, jk, jl) - dl * f.cl * AT(c.g.area_z, ji, jk, jl + 1);
   }
   return (comp_h + comp_v) / AT(c.g.volume, ji, jk, jl);
 }
@@ -534,9 +561,10 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const Ctx c, const State
                                (C_D_P * (c.g.exner_bg[i] + ss.exner_pert[i]));
   if (c.G.nc > 1) {
     const double new_w = rk == 2 ? 0.5 : 1.0, old_w = 1.0 - new_w;
+    const TracerFaces f = tracer_faces(c, nw, ji, jk, jl);
     for (int jc = 0; jc < c.G.nc; ++jc) {
       if (jc == ncond) continue;
-      const double d = div_tracer(c, ss.rho[jc], nw, ji, jk, jl);
+      const double d = div_tracer(c, f, ss.rho[jc], ji, jk, jl);
       c.w.tend_rho[jc][i] = old_w * c.w.tend_rho[jc][i] + new_w * (-d);
     }
   }

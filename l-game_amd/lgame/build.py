@@ -47,6 +47,31 @@ def build_oracles(verbose: bool = False) -> None:
         print(f"built oracle targets {targets}")
 
 
+FORTRAN_DIR = os.path.join(PKG_ROOT, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
+FORTRAN_DEMO = os.path.join(FORTRAN_BUILD, "lgame_driver_demo")
+
+
+def build_fortran_demo(verbose: bool = False) -> str:
+    """The iso_c_binding module and the demo Fortran time loop, linked against liblgame_gpu.so."""
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    fc = os.environ.get("FC", "flang")
+    obj = os.path.join(FORTRAN_BUILD, "mo_lgame_gpu.o")
+    cmds = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "mo_lgame_gpu.F90"), "-o", obj, "-module-dir", FORTRAN_BUILD],
+        [fc, "-O2", "-I", FORTRAN_BUILD, os.path.join(FORTRAN_DIR, "lgame_driver_demo.F90"), obj,
+         "-L", PKG_ROOT, "-llgame_gpu", "-Wl,-rpath,$ORIGIN/../..", "-o", FORTRAN_DEMO],
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_BUILD)
+        if r.returncode != 0:
+            raise RuntimeError(f"{' '.join(cmd)} failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {FORTRAN_DEMO}")
+    return FORTRAN_DEMO
+
+
 def build_all(verbose: bool = False) -> None:
     build_gpu(verbose)
     build_oracles(verbose)
+    build_fortran_demo(verbose)

@@ -67,3 +67,47 @@ def test_oracle_rest_state_stays_at_rest():
     assert np.abs(out["wind_u"]).max() < 1e-10
     assert np.abs(out["wind_v"]).max() < 1e-10
     assert np.abs(out["wind_w"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_oracle_integrals_closed_form(moist):
+    """write_out_integrals (mo_write_out.F90:226-304) on a uniform state over flat ground.
+
+    Constituent masses are rho_c * V_total, the rhotheta integral rho*theta*V_total, kinetic
+    energy 0.5*rho_gas*(u^2+v^2)*V_total (the inner-product weights of a flat Cartesian cell are
+    1/2 on each face), potential energy rho_gas*sum(phi*V), and internal energy
+    c_d_v*rho_gas*T*V_total with T = theta_v*Pi/(1 + rho_v/rho_gas*(m_d/m_v - 1)).
+    """
+    cfg = dataclasses.replace(tiny_config(moist=moist, plane=True), orography_id=0, hills=())
+    grid, st = make_case(cfg, perturbation=False)
+    nc, gc = cfg.n_constituents, cfg.n_condensed_constituents
+    rho_c = np.linspace(1e-3, 7e-3, nc)
+    rho_c[gc] = 1.1
+    theta0, pi0, u0, v0 = 300.0, 0.9, 7.0, -3.0
+    for jc in range(nc):
+        st["rho"][:, :, :, jc] = rho_c[jc]
+    st["rhotheta_v"][:] = rho_c[gc] * theta0
+    st["theta_v_pert"][:] = theta0 - grid["theta_v_bg"]
+    st["exner_pert"][:] = pi0 - grid["exner_bg"]
+    st["wind_u"][:] = u0
+    st["wind_v"][:] = v0
+    st["wind_w"][:] = 0.0
+    ipw = grid["inner_product_weights"]
+    assert np.allclose(ipw[0] + ipw[2], 1.0, rtol=0, atol=1e-14)
+    assert np.allclose(ipw[1] + ipw[3], 1.0, rtol=0, atol=1e-14)
+    o = CpuOracle(cfg, grid)
+    o.set_state(0, st)
+    o.lib().orc_diagnose_temperature(o.h, 0)
+    got = o.integrals(0)
+    vol = float(np.sum(grid["volume"]))
+    temp = theta0 * pi0
+    if moist:
+        m_d, m_v = 0.004810, 0.002991
+        temp /= 1.0 + rho_c[nc - 1] / rho_c[gc] * (m_d / m_v - 1.0)
+    want = np.zeros(4 + nc)
+    want[0] = rho_c[gc] * theta0 * vol
+    want[1] = 0.5 * rho_c[gc] * (u0 * u0 + v0 * v0) * vol
+    want[2] = rho_c[gc] * float(np.sum(grid["gravity_potential"] * grid["volume"]))
+    want[3] = 717.942189 * rho_c[gc] * temp * vol
+    want[4:] = rho_c * vol
+    np.testing.assert_allclose(got, want, rtol=1e-12)
