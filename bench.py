@@ -9,11 +9,15 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
   moist512  512x512x60 moist (7 constituents), dx 2 km, dt 3.22 s   -- the operational-size config
   dry128    128x128x50 dry, rotated lat-lon, dx 2 km, dt 3.22 s
   schaer    ny=3, nx=201, 40 layers, dx 500 m, dt 0.805 s
-The roofline object is the implicit vertical solver (three_band_solver_ver), timed with HIP
-events on the library's own stream; its algorithmic bytes are 152 B (predictor) and 176 B
+Both forms of the step are timed: the plain FP64 form and the mixed form (FP32 stencils and
+explicit tendencies, FP64 column solves, density updates and budget sums).  The headline value is
+the mixed form; `forms` lists both and `speedup_mixed_vs_fp64` their ratio.
+The roofline object is the implicit vertical solver (three_band_solver_ver, FP64 in both forms),
+timed with HIP events on the library's own stream; its algorithmic bytes are 152 B (predictor) and 176 B
 (corrector) per grid point (state, tendency and grid arrays streamed once, FP64).
 The CPU baseline is the oracle (C restatement, one thread) on a bounded sample of the same
-workload type; the GPU result on that sample is compared with it and the deviation reported.
+workload type; each GPU form is run on that sample and its per-field RMS and max error against
+the oracle are reported in `field_error`.
 """
 from __future__ import annotations
 
@@ -58,13 +62,15 @@ def describe(cfg):
     return f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, theta order {cfg.theta_adv_order}"
 
 
-def cpu_baseline(cfg, nsteps):
-    """Oracle timing on the host (one thread) plus the GPU deviation on the same sample."""
-    import numpy as np
+def cpu_baseline(cfg, nsteps, forms):
+    """Oracle timing on the host (one thread) plus the GPU forms' per-field error on the same sample."""
+    import dataclasses
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from lgame.cases import make_case
+    from lgame.config import PREC_FP64, PREC_MIXED
     from lgame.gpu import Step
+    from lgame.metrics import field_errors, worst
     from pyoracle import CpuOracle
 
     grid, st = make_case(cfg)
@@ -74,16 +80,17 @@ def cpu_baseline(cfg, nsteps):
     o.run(0, nsteps)
     dt = time.perf_counter() - t0
     want = o.get_state(nsteps % 2)
-    g = Step(cfg, grid)
-    g.start(st)
-    g.run(0, nsteps)
-    got = g.get_state(nsteps % 2)
-    g.close()
-    worst = 0.0
-    for k in want:
-        scale = max(float(np.abs(want[k]).max()), 1e-300)
-        worst = max(worst, float(np.abs(got[k] - want[k]).max()) / scale)
-    return cfg.gridpoints * nsteps / dt, dt, worst
+    errs = {}
+    for form in forms:
+        g = Step(dataclasses.replace(cfg, precision=PREC_MIXED if form == "mixed" else PREC_FP64), grid)
+        g.start(st)
+        g.run(0, nsteps)
+        got = g.get_state(nsteps % 2)
+        g.close()
+        e = field_errors(got, want)
+        errs[form] = {"max_rel": worst(e, "max_rel"), "rms_rel": worst(e, "rms_rel"),
+                      "per_field": {k: [v["rms_rel"], v["max_rel"]] for k, v in e.items()}}
+    return cfg.gridpoints * nsteps / dt, dt, errs
 
 
 def copy_calibration(nbytes):
@@ -104,12 +111,47 @@ def copy_calibration(nbytes):
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
+def time_form(cfg, grid, st, precision, steps, warmup):
+    """Throughput of one form of the step plus event timing of the FP64 column solver."""
+    import dataclasses
+
+    from lgame.gpu import PROF_SOLVE_GEN, PROF_SOLVE_VER, PROF_STEP, Step
+
+    step = Step(dataclasses.replace(cfg, precision=precision), grid)
+    step.start(st)
+    counter = 0
+    for _ in range(warmup):
+        step.pchevi(counter % 2, 1 - counter % 2, counter)
+        counter += 1
+    step.sync()
+    # timed region: exactly K steps between two synchronizations of the library's stream
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.pchevi(counter % 2, 1 - counter % 2, counter)
+        counter += 1
+    step.sync()
+    elapsed = time.perf_counter() - t0
+    bad = step.count_nonfinite(counter % 2)
+    # kernel timing with HIP events on the library's stream (separate pass)
+    step.profile(True)
+    nprof = max(2, min(steps, 6))
+    for _ in range(nprof):
+        step.pchevi(counter % 2, 1 - counter % 2, counter)
+        counter += 1
+    prof = {g: step.profile_read(g) for g in (PROF_SOLVE_VER, PROF_SOLVE_GEN, PROF_STEP)}
+    step.profile(False)
+    step.close()
+    return elapsed, bad, prof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="moist512", choices=["moist512", "dry128", "schaer"])
+    ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
+                    help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -121,7 +163,8 @@ def main():
         return 2
 
     from lgame.cases import make_case
-    from lgame.gpu import PROF_SOLVE_VER, PROF_STEP, Step
+    from lgame.config import PREC_FP64, PREC_MIXED
+    from lgame.gpu import PROF_SOLVE_VER, PROF_STEP
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -129,45 +172,28 @@ def main():
     t0 = time.perf_counter()
     log(f"setting up {describe(cfg)}")
     grid, st = make_case(cfg)
-    log(f"set-up done in {time.perf_counter() - t0:.1f} s; uploading")
-    step = Step(cfg, grid)
-    step.start(st)
-    del grid, st
-    counter = 0
-    for _ in range(args.warmup):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    step.sync()
-    log(f"warm-up of {args.warmup} steps done")
-
-    # timed region: exactly K steps between two synchronizations of the library's stream
-    step.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    step.sync()
-    elapsed = time.perf_counter() - t0
-    bad = step.count_nonfinite(counter % 2)
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; non-finite values: {bad}")
-
-    # kernel timing of the solver with HIP events on the library's stream (separate pass)
-    step.profile(True)
-    nprof = max(2, min(args.steps, 6))
-    for _ in range(nprof):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    ms_solver, n_solver = step.profile_read(PROF_SOLVE_VER)
-    ms_step, n_step = step.profile_read(PROF_STEP)
-    step.profile(False)
-    step.close()
+    log(f"set-up done in {time.perf_counter() - t0:.1f} s")
     cells = cfg.gridpoints
+    forms = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.form]
+    res = {}
+    for form in forms:
+        prec = PREC_MIXED if form == "mixed" else PREC_FP64
+        elapsed, bad, prof = time_form(cfg, grid, st, prec, args.steps, args.warmup)
+        res[form] = (elapsed, bad, prof)
+        log(f"{form}: {args.steps} steps in {elapsed:.3f} s ({1e3 * elapsed / args.steps:.2f} ms/step), "
+            f"non-finite values: {bad}")
+    del grid, st
+    head = "mixed" if "mixed" in res else "fp64"
+    elapsed, bad, prof = res[head]
+    value = cells * args.steps / elapsed
+    ms_solver, n_solver = prof[PROF_SOLVE_VER]
+    ms_step, n_step = prof[PROF_STEP]
     solver_bytes = cells * (SOLVER_BYTES[1] + SOLVER_BYTES[2]) * (n_solver // 2)
     solver_gbs = solver_bytes / (ms_solver * 1e-3) / 1e9 if ms_solver > 0 else 0.0
     avg_launch_us = 1e3 * ms_solver / max(n_solver, 1)
-
-    value = cells * args.steps / elapsed
     step_model = STEP_BYTES["moist" if cfg.lmoist else "dry"]
+    form_text = "mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates " \
+                "and budget sums)" if head == "mixed" else "plain FP64 form"
     out = {
         "metric": METRIC,
         "value": value,
@@ -179,16 +205,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {describe(cfg)}, FP64 plain form, radiation off",
-            "form": "plain FP64",
+            "workload": f"{args.workload}: {describe(cfg)}, {form_text}, radiation off",
+            "form": head,
             "grid_points": cells,
             "nonfinite_after_run": bad,
         },
+        "forms": {f: {"value": cells * args.steps / r[0], "ms_per_step": 1e3 * r[0] / args.steps,
+                      "solver_ver_us": 1e3 * r[2][PROF_SOLVE_VER][0] / max(r[2][PROF_SOLVE_VER][1], 1)}
+                  for f, r in res.items()},
         "roofline": {
-            "kernel": "k_solve_ver (three_band_solver_ver, both substeps)",
+            "kernel": "k_solve_ver (three_band_solver_ver, both substeps, FP64)",
             "bound": "hbm",
             "achieved": solver_gbs,
             "peak": HBM_PEAK_GBS,
@@ -201,21 +230,21 @@ def main():
             "launches_timed": n_solver,
         },
         "step_roofline": {
-            "bytes_per_gp_step_model": step_model,
-            "achieved_gbs": step_model * value / 1e9,
-            "frac": step_model * value / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_gp_step_model_fp64": step_model,
+            "achieved_gbs_fp64_model": step_model * value / 1e9,
             "event_ms_per_step": ms_step / max(n_step, 1),
         },
     }
+    if "fp64" in res and "mixed" in res:
+        out["speedup_mixed_vs_fp64"] = res["fp64"][0] / res["mixed"][0]
     if not args.no_cpu:
         log(f"CPU baseline: oracle on {describe(sample_cfg)} for {args.cpu_steps} steps")
-        v, secs, dev = cpu_baseline(sample_cfg, args.cpu_steps)
+        v, secs, errs = cpu_baseline(sample_cfg, args.cpu_steps, forms)
         out["cpu_baseline"] = {
             "value": v, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle/lgame_oracle.c, {describe(sample_cfg)}, {args.cpu_steps} steps, {secs:.1f} s",
         }
-        out["field_error"] = {"vs": "FP64 oracle on the CPU-baseline sample", "max_rel": dev,
-                              "steps": args.cpu_steps}
+        out["field_error"] = {"vs": "FP64 oracle on the CPU-baseline sample", "steps": args.cpu_steps, **errs}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -17,15 +17,17 @@ using namespace lgame;
 struct lgame_handle {
   lgame_dims dims;
   lgame_cfg cfg;
-  Ctx c;
+  Ctx c;                         // plain form; its grid (FP64) also serves the budget sums
+  CtxT<float> cf;                // mixed form: FP32 grid copy and work arrays, FP64 grid in g64
+  int mixed = 0;
   StateDev slot[2];
-  double *slab = nullptr;        // all fields, one allocation
-  size_t slab_fields = 0;
+  char *slab = nullptr;          // all fields, one allocation
+  size_t slab_bytes = 0;
   double *staging = nullptr;     // device staging for host<->device transposes
   size_t staging_elems = 0;
   double *red = nullptr;         // reduction partials
   unsigned long long *counter = nullptr;
-  double *trsk = nullptr;
+  double *trsk = nullptr;        // TRSK weights, FP64 then FP32
   hipStream_t stream = nullptr;
   int have_state[2] = {0, 0};
   int stepped = 0;
@@ -71,12 +73,18 @@ __global__ void k_from_ref(double *dst, const double *src, int n1, int n2, int n
   dst[LG_IX(G, ji, jk, jl)] = src[s];
 }
 
-__global__ void k_to_ref(double *dst, const double *src, int n1, int n2, int n3, Geom G) {
+template <class T>
+__global__ void k_to_ref(double *dst, const T *src, int n1, int n2, int n3, Geom G) {
   const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int jl = blockIdx.z + 1;
   if (jk > n2 || ji > n1 || jl > n3) return;
-  dst[(size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1))] = src[LG_IX(G, ji, jk, jl)];
+  dst[(size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1))] = (double)src[LG_IX(G, ji, jk, jl)];
+}
+
+// FP32 copy of a block of FP64 fields (the grid of the mixed form)
+__global__ void k_to_float(float *dst, const double *src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = (float)src[i];
 }
 
 // Klemp damping prefactor at w levels, 1 + damping_coeff*dtime (mo_column_solvers.F90:190-196)
@@ -106,11 +114,12 @@ int upload(lgame_handle *h, double *dst, const double *host, int n1, int n2, int
   return 0;
 }
 
-int download(lgame_handle *h, double *host, const double *src, int n1, int n2, int n3) {
+template <class T>
+int download(lgame_handle *h, double *host, const T *src, int n1, int n2, int n3) {
   const size_t n = (size_t)n1 * n2 * n3;
   if (n > h->staging_elems) return fail(h, LGAME_EINVAL, "internal: staging buffer too small");
   dim3 blk(BX, BY, 1), grd((n2 + BX - 1) / BX, (n1 + BY - 1) / BY, n3);
-  hipLaunchKernelGGL(k_to_ref, grd, blk, 0, h->stream, h->staging, src, n1, n2, n3, h->c.G);
+  hipLaunchKernelGGL(k_to_ref<T>, grd, blk, 0, h->stream, h->staging, src, n1, n2, n3, h->c.G);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(host, h->staging, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -133,7 +142,8 @@ constexpr int NRED = 4 + MAX_CONST;
 constexpr int RED_BLOCK = 256;
 constexpr int RED_GRID = 512;
 
-__global__ void __launch_bounds__(RED_BLOCK) k_integrals(const Ctx c, const StateDev s, double *partial) {
+template <class T>
+__global__ void __launch_bounds__(RED_BLOCK) k_integrals(const Ctx c, const StateDev s, Fld<const T> temperature, double *partial) {
   DD acc[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; ++q) acc[q] = {0.0, 0.0};
@@ -143,13 +153,13 @@ __global__ void __launch_bounds__(RED_BLOCK) k_integrals(const Ctx c, const Stat
     const int jk = (int)(t % nx) + 1;
     const int ji = (int)((t / nx) % ny) + 1;
     const int jl = (int)(t / ((long long)nx * ny)) + 1;
-    const size_t i = LG_IX(c.G, ji, jk, jl);
+    const ix_t i = LG_IX(c.G, ji, jk, jl);
     const double V = c.g.volume[i];
     const double rg = s.rho[ncond][i];
     dd_add(acc[0], s.rhotheta_v[i] * V);
     dd_add(acc[1], (rg * ke_point(c, s, ji, jk, jl)) * V);
     dd_add(acc[2], (rg * c.g.gravity_potential[i]) * V);
-    dd_add(acc[3], (C_D_V * rg * c.w.temperature[i]) * V);
+    dd_add(acc[3], (C_D_V * rg * (double)temperature[i]) * V);
     for (int jc = 0; jc < nc; ++jc) dd_add(acc[4 + jc], s.rho[jc][i] * V);
   }
   __shared__ double sh_s[RED_BLOCK], sh_c[RED_BLOCK];
@@ -208,7 +218,7 @@ __global__ void k_count_nonfinite(const Ctx c, const StateDev s, unsigned long l
   const int jl = blockIdx.z + 1;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl + 1) return;
-  const size_t i = LG_IX(c.G, ji, jk, jl);
+  const ix_t i = LG_IX(c.G, ji, jk, jl);
   unsigned long long bad = 0;
   if (ji <= ny && jk <= nx && jl <= nl) {
     for (int jc = 0; jc < c.G.nc; ++jc) bad += !isfinite(s.rho[jc][i]);
@@ -229,7 +239,8 @@ inline dim3 grid3(int nxr, int nyr, int nzr) { return dim3((nxr + BX - 1) / BX, 
 // ---------------------------------------------------------------------------------------------
 namespace lgame {
 
-void enqueue_pchevi(const Ctx &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof) {
+template <class R>
+void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const dim3 blk(BX, BY, 1);
   const dim3 g_cell = grid3(nx, ny, nl);
@@ -238,52 +249,55 @@ void enqueue_pchevi(const Ctx &c, const StateDev &old, const StateDev &nw, int s
   const dim3 g_col((nx * ny + BCOL - 1) / BCOL), b_col(BCOL);
   const bool o3 = c.C.order == 3;
 
-  hipLaunchKernelGGL(k_temperature, g_cell, blk, 0, st, c, old);
+  hipLaunchKernelGGL(k_temperature<R>, g_cell, blk, 0, st, c, old);
   for (int rk = 1; rk <= 2; ++rk) {
     const StateDev &S = rk == 1 ? old : nw;
     if (rk == 1) {
       // manage_p_grad (mo_p_grad.F90:20-104); theta_v_adv_3rd_order of theta_v and theta_v_pert
       if (o3) {
-        hipLaunchKernelGGL(k_theta_fill, g_cell, blk, 0, st, c, old, 2);
-        hipLaunchKernelGGL(k_theta_grad, g_uv, blk, 0, st, c, 2);
-        hipLaunchKernelGGL(k_theta_div, g_cell, blk, 0, st, c, 2);
-        hipLaunchKernelGGL(k_theta_edge, g_uv, blk, 0, st, c, (const double *)old.wind_u, (const double *)old.wind_v, 2);
+        hipLaunchKernelGGL(k_theta_fill<R>, g_cell, blk, 0, st, c, old, 2);
+        hipLaunchKernelGGL(k_theta_grad<R>, g_uv, blk, 0, st, c, 2);
+        hipLaunchKernelGGL(k_theta_div<R>, g_cell, blk, 0, st, c, 2);
+        hipLaunchKernelGGL(k_theta_edge<R>, g_uv, blk, 0, st, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
       }
-      hipLaunchKernelGGL(k_pgrad, g_all, blk, 0, st, c, old, step == 0 ? 1 : 0);
+      hipLaunchKernelGGL(k_pgrad<R>, g_all, blk, 0, st, c, old, step == 0 ? 1 : 0);
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
-      hipLaunchKernelGGL(k_pot_vort, g_all, blk, 0, st, c, S);
-      hipLaunchKernelGGL(k_mass_flux, g_all, blk, 0, st, c, S);
-      hipLaunchKernelGGL(k_vort_flux, g_uv, blk, 0, st, c);
+      hipLaunchKernelGGL(k_pot_vort<R>, g_all, blk, 0, st, c, S);
+      hipLaunchKernelGGL(k_mass_flux<R>, g_all, blk, 0, st, c, S);
+      hipLaunchKernelGGL(k_vort_flux<R>, g_uv, blk, 0, st, c);
       if (!c.C.llinear) {
-        hipLaunchKernelGGL(k_ke, g_cell, blk, 0, st, c, S);
-        hipLaunchKernelGGL(k_ke_grad, g_uv, blk, 0, st, c);
+        hipLaunchKernelGGL(k_ke<R>, g_cell, blk, 0, st, c, S);
+        hipLaunchKernelGGL(k_ke_grad<R>, g_uv, blk, 0, st, c);
       }
     }
-    if (rk == 1 && c.C.lklemp) hipLaunchKernelGGL(k_dissipation, g_cell, blk, 0, st, c, old);
-    hipLaunchKernelGGL(k_vec_tend, g_uv, blk, 0, st, c, old, nw, rk);
+    if (rk == 1 && c.C.lklemp) hipLaunchKernelGGL(k_dissipation<R>, g_cell, blk, 0, st, c, old);
+    hipLaunchKernelGGL(k_vec_tend<R>, g_uv, blk, 0, st, c, old, nw, rk);
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)
     if (o3 && rk == 2) {
-      hipLaunchKernelGGL(k_theta_fill, g_cell, blk, 0, st, c, nw, 1);
-      hipLaunchKernelGGL(k_theta_grad, g_uv, blk, 0, st, c, 1);
-      hipLaunchKernelGGL(k_theta_div, g_cell, blk, 0, st, c, 1);
-      hipLaunchKernelGGL(k_theta_edge, g_uv, blk, 0, st, c, (const double *)nw.wind_u, (const double *)nw.wind_v, 1);
+      hipLaunchKernelGGL(k_theta_fill<R>, g_cell, blk, 0, st, c, nw, 1);
+      hipLaunchKernelGGL(k_theta_grad<R>, g_uv, blk, 0, st, c, 1);
+      hipLaunchKernelGGL(k_theta_div<R>, g_cell, blk, 0, st, c, 1);
+      hipLaunchKernelGGL(k_theta_edge<R>, g_uv, blk, 0, st, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
     }
-    hipLaunchKernelGGL(k_scalar_flux, g_uv, blk, 0, st, c, S, nw);
-    hipLaunchKernelGGL(k_scalar_div, g_cell, blk, 0, st, c, S, nw, rk);
+    hipLaunchKernelGGL(k_scalar_flux<R>, g_uv, blk, 0, st, c, S, nw);
+    hipLaunchKernelGGL(k_scalar_div<R>, g_cell, blk, 0, st, c, S, nw, rk);
     // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
     if (prof) prof->begin(0);
-    hipLaunchKernelGGL(k_solve_ver, g_col, b_col, 0, st, c, old, nw, rk);
+    hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
     if (prof) prof->end(0);
     if (c.G.nc > 1) {
       if (prof) prof->begin(1);
-      hipLaunchKernelGGL(k_solve_gen, g_col, b_col, 0, st, c, old, nw, rk);
+      hipLaunchKernelGGL(k_solve_gen<R>, g_col, dim3(BCOL, 6), 0, st, c, old, nw, rk);
       if (prof) prof->end(1);
     }
   }
 }
+
+template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *);
+template void enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *);
 
 }  // namespace lgame
 
@@ -386,7 +400,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   if (f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
     return fail(nullptr, LGAME_EINVAL, "unsupported: PBL, surface fluxes and soil must be off");
   if (f.lrad || f.rad_mode != LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "unsupported: radiation");
-  if (f.precision != LGAME_PREC_FP64) return fail(nullptr, LGAME_EINVAL, "unsupported precision mode %d", f.precision);
+  if (f.precision != LGAME_PREC_FP64 && f.precision != LGAME_PREC_MIXED)
+    return fail(nullptr, LGAME_EINVAL, "unknown precision mode %d", f.precision);
   if (!(f.dtime > 0.0) || !(f.toa > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime and toa must be positive");
 #define NEED(p) \
   if (!grid->p) return fail(nullptr, LGAME_EINVAL, "grid field " #p " is null");
@@ -410,6 +425,10 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   G.pitch = ((d.nx + 1) + 15) / 16 * 16;
   G.lvl = (long long)(d.ny + 1) * G.pitch;
   G.fsize = (long long)G.nlev * G.lvl;
+  if ((unsigned long long)G.fsize * sizeof(double) >= (1ull << 32)) {
+    delete h;
+    return fail(nullptr, LGAME_EINVAL, "one field of %lld points exceeds 4 GiB", G.fsize);
+  }
   Cfg &C = h->c.C;
   C.dtime = f.dtime;
   C.toa = f.toa;
@@ -441,178 +460,228 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     delete h;
     return LGAME_EHIP;
   }
-  // field budget: 2 states, tendencies, carried diagnostics, scratch and the grid
+  // field budget: 2 states (FP64), the FP64 grid, column scratch (FP64), the work arrays of the
+  // stencil kernels (FP64 plain, FP32 mixed) and, for the mixed form, the FP32 grid copy
   const int nc = d.n_constituents;
+  const bool mixed = f.precision == LGAME_PREC_MIXED;
+  h->mixed = mixed;
+  constexpr int NGRID = 40;
+  const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 12;
-  const size_t ngrid = 40;
-  h->slab_fields = nstate + nwork + ngrid;
-  const size_t bytes = h->slab_fields * (size_t)G.fsize * sizeof(double);
-  e = hipMalloc(&h->slab, bytes);
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4;
+  const size_t ncol = 12;
+  const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
+  h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
+  e = hipMalloc(&h->slab, h->slab_bytes);
   if (e != hipSuccess) {
-    fail(nullptr, LGAME_ENOMEM, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    fail(nullptr, LGAME_ENOMEM, "hipMalloc of %zu bytes: %s", h->slab_bytes, hipGetErrorString(e));
     (void)hipStreamDestroy(h->stream);
     delete h;
     return LGAME_ENOMEM;
   }
-  size_t used = 0;
-  auto take = [&]() { return h->slab + (used++) * (size_t)G.fsize; };
+  size_t used = 0;  // bytes handed out; every field starts on a 128-byte boundary (pitch is 16 doubles)
+  auto take8 = [&]() {
+    double *p = reinterpret_cast<double *>(h->slab + used);
+    used += fs * sizeof(double);
+    return p;
+  };
+  auto take4 = [&]() {
+    float *p = reinterpret_cast<float *>(h->slab + used);
+    used += fs * sizeof(float);
+    return p;
+  };
   for (int s = 0; s < 2; ++s) {
     StateDev &S = h->slot[s];
-    for (int jc = 0; jc < MAX_CONST; ++jc) S.rho[jc] = jc < nc ? take() : nullptr;
-    S.rhotheta_v = take();
-    S.theta_v_pert = take();
-    S.exner_pert = take();
-    S.wind_u = take();
-    S.wind_v = take();
-    S.wind_w = take();
+    for (int jc = 0; jc < MAX_CONST; ++jc) S.rho[jc] = jc < nc ? take8() : nullptr;
+    S.rhotheta_v = take8();
+    S.theta_v_pert = take8();
+    S.exner_pert = take8();
+    S.wind_u = take8();
+    S.wind_v = take8();
+    S.wind_w = take8();
   }
-  WorkDev &W = h->c.w;
-  for (int jc = 0; jc < MAX_CONST; ++jc) W.tend_rho[jc] = jc < nc ? take() : nullptr;
-  W.tend_rhotheta_v = take();
-  W.tend_u = take();
-  W.tend_v = take();
-  W.tend_w = take();
-  W.pnl_u = take();
-  W.pnl_v = take();
-  W.pnl_w = take();
-  W.pl_u = take();
-  W.pl_v = take();
-  W.pl_w = take();
-  W.pold_u = take();
-  W.pold_v = take();
-  W.vsg_x = take();
-  W.vsg_y = take();
-  W.vsg_z = take();
-  W.pvt_x = take();
-  W.pvt_y = take();
-  W.pvt_z = take();
-  W.heating_diss = take();
-  W.csh = take();
-  W.temperature = take();
-  W.radiation_tendency = take();
-  W.eta_x = take();
-  W.eta_y = take();
-  W.eta_z = take();
-  W.u_ph = take();
-  W.v_ph = take();
-  W.w_ph = take();
-  W.v_squared = take();
-  for (int q = 0; q < 2; ++q) {
-    W.tth[q] = take();
-    W.tgu[q] = take();
-    W.tgv[q] = take();
-    W.tfd[q] = take();
-    W.tu[q] = take();
-    W.tv[q] = take();
+  // grid: NGRID consecutive FP64 fields in a fixed order (the FP32 copy uses the same order)
+  double *grid64 = take8();
+  for (int q = 1; q < NGRID; ++q) take8();
+  Fld<double> col[14];
+  for (int q = 0; q < (int)ncol; ++q) col[q] = take8();
+  auto assign_work = [&](auto &W, auto take) {
+    for (int jc = 0; jc < MAX_CONST; ++jc) W.tend_rho[jc] = jc < nc ? take() : nullptr;
+    W.tend_rhotheta_v = take();
+    W.tend_u = take();
+    W.tend_v = take();
+    W.tend_w = take();
+    W.pnl_u = take();
+    W.pnl_v = take();
+    W.pnl_w = take();
+    W.pl_u = take();
+    W.pl_v = take();
+    W.pl_w = take();
+    W.pold_u = take();
+    W.pold_v = take();
+    W.vsg_x = take();
+    W.vsg_y = take();
+    W.vsg_z = take();
+    W.pvt_x = take();
+    W.pvt_y = take();
+    W.pvt_z = take();
+    W.heating_diss = take();
+    W.csh = take();
+    W.temperature = take();
+    W.radiation_tendency = take();
+    W.eta_x = take();
+    W.eta_y = take();
+    W.eta_z = take();
+    W.u_ph = take();
+    W.v_ph = take();
+    W.w_ph = take();
+    W.v_squared = take();
+    for (int q = 0; q < 2; ++q) {
+      W.tth[q] = take();
+      W.tgu[q] = take();
+      W.tgv[q] = take();
+      W.tfd[q] = take();
+      W.tu[q] = take();
+      W.tv[q] = take();
+    }
+    W.fu = take();
+    W.fv = take();
+    W.thu = take();
+    W.thv = take();
+    for (int q = 0; q < 14; ++q) W.col[q] = col[q];
+  };
+  if (mixed)
+    assign_work(h->cf.w, take4);
+  else
+    assign_work(h->c.w, take8);
+  float *grid32 = nullptr;
+  if (mixed) {
+    grid32 = take4();
+    for (int q = 1; q < NGRID; ++q) take4();
   }
-  W.fu = take();
-  W.fv = take();
-  W.thu = take();
-  W.thv = take();
-  for (int q = 0; q < 12; ++q) W.col[q] = take();
-  W.col[12] = W.col[13] = nullptr;
-  GridDev &g = h->c.g;
-  double *gz_scalar = take(), *gz_u = take(), *gz_v = take(), *gz_w = take(), *gdx = take(), *gdy = take(), *gdz = take();
-  double *garea_x = take(), *garea_y = take(), *garea_z = take(), *gvolume = take(), *gslope_x = take(), *gslope_y = take();
-  double *gipw = take();
-  for (int q = 1; q < 6; ++q) take();
-  double *gadx = take(), *gady = take(), *gadz = take(), *gzadz = take(), *gfx = take(), *gfy = take(), *gfz = take();
-  double *gebu = take(), *gebv = take(), *gebw = take(), *gtb = take(), *geb = take(), *ggmv = take(), *ggp = take(), *gkp = take();
-  if (used > h->slab_fields) {
+  if (used > h->slab_bytes) {
     fail(nullptr, LGAME_EINVAL, "internal: field budget exceeded");
     lgame_finalize(h);
     return LGAME_EINVAL;
   }
-  e = hipMemsetAsync(h->slab, 0, bytes, h->stream);
+  e = hipMemsetAsync(h->slab, 0, h->slab_bytes, h->stream);
   h->staging_elems = (size_t)d.ny * d.nx * d.n_layers * (nc > 6 ? nc : 6) + (size_t)(d.ny + 1) * (d.nx + 1) * (d.n_layers + 1);
   if (e == hipSuccess) e = hipMalloc(&h->staging, h->staging_elems * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->red, (size_t)2 * NRED * RED_GRID * sizeof(double) + NRED * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMalloc(&h->trsk, (size_t)(3 * d.ny + 2 * (d.ny + 1)) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->trsk, (size_t)(3 * d.ny + 2 * (d.ny + 1)) * (sizeof(double) + sizeof(float)));
   if (e != hipSuccess) {
     fail(nullptr, LGAME_ENOMEM, "device allocation: %s", hipGetErrorString(e));
     lgame_finalize(h);
     return LGAME_ENOMEM;
   }
   const int ny = d.ny, nx = d.nx, nl = d.n_layers, nlev = nl + 1;
+  // grid field q of the block
+  enum {
+    G_Z_SCALAR, G_Z_U, G_Z_V, G_Z_W, G_DX, G_DY, G_DZ, G_AREA_X, G_AREA_Y, G_AREA_Z, G_VOLUME, G_SLOPE_X, G_SLOPE_Y,
+    G_IPW, G_AREA_DUAL_X = G_IPW + 6, G_AREA_DUAL_Y, G_AREA_DUAL_Z, G_Z_AREA_DUAL_Z, G_FVEC_X, G_FVEC_Y, G_FVEC_Z,
+    G_EBG_U, G_EBG_V, G_EBG_W, G_THETA_V_BG, G_EXNER_BG, G_GRAVITY_M_V, G_GRAVITY_POTENTIAL, G_KLEMP_PREF, G_COUNT
+  };
+  static_assert(G_COUNT <= NGRID, "grid field budget");
+  auto gf = [&](int q) { return grid64 + (size_t)q * fs; };
   int rc = 0;
-  rc = rc ? rc : upload(h, gz_scalar, grid->z_scalar, ny, nx, nl);
-  rc = rc ? rc : upload(h, gz_u, grid->z_u, ny, nx + 1, nl);
-  rc = rc ? rc : upload(h, gz_v, grid->z_v, ny + 1, nx, nl);
-  rc = rc ? rc : upload(h, gz_w, grid->z_w, ny, nx, nlev);
-  rc = rc ? rc : upload(h, gdx, grid->dx, ny, nx + 1, nl);
-  rc = rc ? rc : upload(h, gdy, grid->dy, ny + 1, nx, nl);
-  rc = rc ? rc : upload(h, gdz, grid->dz, ny, nx, nlev);
-  rc = rc ? rc : upload(h, garea_x, grid->area_x, ny, nx + 1, nl);
-  rc = rc ? rc : upload(h, garea_y, grid->area_y, ny + 1, nx, nl);
-  rc = rc ? rc : upload(h, garea_z, grid->area_z, ny, nx, nlev);
-  rc = rc ? rc : upload(h, gvolume, grid->volume, ny, nx, nl);
-  rc = rc ? rc : upload(h, gslope_x, grid->slope_x, ny, nx + 1, nl);
-  rc = rc ? rc : upload(h, gslope_y, grid->slope_y, ny + 1, nx, nl);
-  rc = rc ? rc : upload(h, gipw, grid->inner_product_weights, ny, nx, nl, 6, 6);
-  rc = rc ? rc : upload(h, gadx, grid->area_dual_x, ny + 1, nx, nlev);
-  rc = rc ? rc : upload(h, gady, grid->area_dual_y, ny, nx + 1, nlev);
-  rc = rc ? rc : upload(h, gadz, grid->area_dual_z, ny + 1, nx + 1, nl);
-  rc = rc ? rc : upload(h, gzadz, grid->z_area_dual_z, ny + 1, nx + 1, nl);
-  rc = rc ? rc : upload(h, gfx, grid->fvec_x, ny + 1, nx, 1);
-  rc = rc ? rc : upload(h, gfy, grid->fvec_y, ny, nx + 1, 1);
-  rc = rc ? rc : upload(h, gfz, grid->fvec_z, ny + 1, nx + 1, 1);
-  rc = rc ? rc : upload(h, gebu, grid->exner_bg_grad_u, ny, nx + 1, nl);
-  rc = rc ? rc : upload(h, gebv, grid->exner_bg_grad_v, ny + 1, nx, nl);
-  rc = rc ? rc : upload(h, gebw, grid->exner_bg_grad_w, ny, nx, nlev);
-  rc = rc ? rc : upload(h, gtb, grid->theta_v_bg, ny, nx, nl);
-  rc = rc ? rc : upload(h, geb, grid->exner_bg, ny, nx, nl);
-  rc = rc ? rc : upload(h, ggmv, grid->gravity_m_v, ny, nx, nlev);
-  rc = rc ? rc : upload(h, ggp, grid->gravity_potential, ny, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_Z_SCALAR), grid->z_scalar, ny, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_Z_U), grid->z_u, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_Z_V), grid->z_v, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_Z_W), grid->z_w, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_DX), grid->dx, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_DY), grid->dy, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_DZ), grid->dz, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_AREA_X), grid->area_x, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_AREA_Y), grid->area_y, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_AREA_Z), grid->area_z, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_VOLUME), grid->volume, ny, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_SLOPE_X), grid->slope_x, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_SLOPE_Y), grid->slope_y, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_IPW), grid->inner_product_weights, ny, nx, nl, 6, 6);
+  rc = rc ? rc : upload(h, gf(G_AREA_DUAL_X), grid->area_dual_x, ny + 1, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_AREA_DUAL_Y), grid->area_dual_y, ny, nx + 1, nlev);
+  rc = rc ? rc : upload(h, gf(G_AREA_DUAL_Z), grid->area_dual_z, ny + 1, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_Z_AREA_DUAL_Z), grid->z_area_dual_z, ny + 1, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_FVEC_X), grid->fvec_x, ny + 1, nx, 1);
+  rc = rc ? rc : upload(h, gf(G_FVEC_Y), grid->fvec_y, ny, nx + 1, 1);
+  rc = rc ? rc : upload(h, gf(G_FVEC_Z), grid->fvec_z, ny + 1, nx + 1, 1);
+  rc = rc ? rc : upload(h, gf(G_EBG_U), grid->exner_bg_grad_u, ny, nx + 1, nl);
+  rc = rc ? rc : upload(h, gf(G_EBG_V), grid->exner_bg_grad_v, ny + 1, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_EBG_W), grid->exner_bg_grad_w, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_THETA_V_BG), grid->theta_v_bg, ny, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_EXNER_BG), grid->exner_bg, ny, nx, nl);
+  rc = rc ? rc : upload(h, gf(G_GRAVITY_M_V), grid->gravity_m_v, ny, nx, nlev);
+  rc = rc ? rc : upload(h, gf(G_GRAVITY_POTENTIAL), grid->gravity_potential, ny, nx, nl);
   if (!rc) {
     std::vector<double> kp((size_t)ny * nx * nlev);
     for (size_t q = 0; q < kp.size(); ++q) kp[q] = klemp_prefactor(f, grid->z_w[q]);
-    rc = upload(h, gkp, kp.data(), ny, nx, nlev);
+    rc = upload(h, gf(G_KLEMP_PREF), kp.data(), ny, nx, nlev);
   }
+  const size_t ntw = (size_t)3 * ny + 2 * (ny + 1);
+  float *trsk32 = reinterpret_cast<float *>(h->trsk + ntw);
   if (!rc) {
-    std::vector<double> tw((size_t)3 * ny + 2 * (ny + 1));
+    std::vector<double> tw(ntw);
     std::memcpy(tw.data(), grid->trsk_weights_u, sizeof(double) * 3 * ny);
     std::memcpy(tw.data() + 3 * ny, grid->trsk_weights_v, sizeof(double) * 2 * (ny + 1));
+    std::vector<float> tw32(tw.begin(), tw.end());
     e = hipMemcpy(h->trsk, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(trsk32, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = fail(h, LGAME_EHIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  if (!rc && mixed) {
+    hipLaunchKernelGGL(k_to_float, dim3(4096), dim3(256), 0, h->stream, grid32, (const double *)grid64, (size_t)NGRID * fs);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(h, LGAME_EHIP, "grid conversion: %s", hipGetErrorString(e));
   }
   if (rc) {
     g_init_error = h->err;
     lgame_finalize(h);
     return rc;
   }
-  g.z_scalar = gz_scalar;
-  g.z_u = gz_u;
-  g.z_v = gz_v;
-  g.z_w = gz_w;
-  g.dx = gdx;
-  g.dy = gdy;
-  g.dz = gdz;
-  g.area_x = garea_x;
-  g.area_y = garea_y;
-  g.area_z = garea_z;
-  g.volume = gvolume;
-  g.slope_x = gslope_x;
-  g.slope_y = gslope_y;
-  for (int q = 0; q < 6; ++q) g.ipw[q] = gipw + (size_t)q * G.fsize;
-  g.area_dual_x = gadx;
-  g.area_dual_y = gady;
-  g.area_dual_z = gadz;
-  g.z_area_dual_z = gzadz;
-  g.fvec_x = gfx;
-  g.fvec_y = gfy;
-  g.fvec_z = gfz;
-  g.ebg_u = gebu;
-  g.ebg_v = gebv;
-  g.ebg_w = gebw;
-  g.theta_v_bg = gtb;
-  g.exner_bg = geb;
-  g.gravity_m_v = ggmv;
-  g.gravity_potential = ggp;
-  g.klemp_pref = gkp;
-  g.trsk_u = h->trsk;
-  g.trsk_v = h->trsk + 3 * ny;
+  auto assign_grid = [&](auto &g, auto *base, auto *trsk) {
+    auto at = [&](int q) { return base + (size_t)q * fs; };
+    g.z_scalar = at(G_Z_SCALAR);
+    g.z_u = at(G_Z_U);
+    g.z_v = at(G_Z_V);
+    g.z_w = at(G_Z_W);
+    g.dx = at(G_DX);
+    g.dy = at(G_DY);
+    g.dz = at(G_DZ);
+    g.area_x = at(G_AREA_X);
+    g.area_y = at(G_AREA_Y);
+    g.area_z = at(G_AREA_Z);
+    g.volume = at(G_VOLUME);
+    g.slope_x = at(G_SLOPE_X);
+    g.slope_y = at(G_SLOPE_Y);
+    for (int q = 0; q < 6; ++q) g.ipw[q] = at(G_IPW + q);
+    g.area_dual_x = at(G_AREA_DUAL_X);
+    g.area_dual_y = at(G_AREA_DUAL_Y);
+    g.area_dual_z = at(G_AREA_DUAL_Z);
+    g.z_area_dual_z = at(G_Z_AREA_DUAL_Z);
+    g.fvec_x = at(G_FVEC_X);
+    g.fvec_y = at(G_FVEC_Y);
+    g.fvec_z = at(G_FVEC_Z);
+    g.ebg_u = at(G_EBG_U);
+    g.ebg_v = at(G_EBG_V);
+    g.ebg_w = at(G_EBG_W);
+    g.theta_v_bg = at(G_THETA_V_BG);
+    g.exner_bg = at(G_EXNER_BG);
+    g.gravity_m_v = at(G_GRAVITY_M_V);
+    g.gravity_potential = at(G_GRAVITY_POTENTIAL);
+    g.klemp_pref = at(G_KLEMP_PREF);
+    g.trsk_u = trsk;
+    g.trsk_v = trsk + 3 * ny;
+  };
+  assign_grid(h->c.g, grid64, h->trsk);
+  h->c.g64 = h->c.g;
+  if (mixed) {
+    h->cf.G = h->c.G;
+    h->cf.C = h->c.C;
+    assign_grid(h->cf.g, grid32, trsk32);
+    h->cf.g64 = h->c.g;
+  }
   *out = h;
   return LGAME_OK;
 }
@@ -627,13 +696,13 @@ static int state_io(lgame_handle *h, int slot, lgame_state_view *st, bool up) {
   int rc = 0;
   const size_t ncell = (size_t)ny * nx * nl;
   for (int jc = 0; jc < nc && !rc; ++jc)
-    rc = up ? upload(h, S.rho[jc], st->rho + jc * ncell, ny, nx, nl) : download(h, st->rho + jc * ncell, S.rho[jc], ny, nx, nl);
+    rc = up ? upload(h, S.rho[jc].p, st->rho + jc * ncell, ny, nx, nl) : download(h, st->rho + jc * ncell, S.rho[jc].p, ny, nx, nl);
   struct F {
     double *host, *dev;
     int n1, n2, n3;
-  } fs[] = {{st->rhotheta_v, S.rhotheta_v, ny, nx, nl},     {st->theta_v_pert, S.theta_v_pert, ny, nx, nl},
-            {st->exner_pert, S.exner_pert, ny, nx, nl},     {st->wind_u, S.wind_u, ny, nx + 1, nl},
-            {st->wind_v, S.wind_v, ny + 1, nx, nl},         {st->wind_w, S.wind_w, ny, nx, nl + 1}};
+  } fs[] = {{st->rhotheta_v, S.rhotheta_v.p, ny, nx, nl},     {st->theta_v_pert, S.theta_v_pert.p, ny, nx, nl},
+            {st->exner_pert, S.exner_pert.p, ny, nx, nl},     {st->wind_u, S.wind_u.p, ny, nx + 1, nl},
+            {st->wind_v, S.wind_v.p, ny + 1, nx, nl},         {st->wind_w, S.wind_w.p, ny, nx, nl + 1}};
   for (auto &x : fs) {
     if (rc) break;
     rc = up ? upload(h, x.dev, x.host, x.n1, x.n2, x.n3) : download(h, x.host, x.dev, x.n1, x.n2, x.n3);
@@ -663,11 +732,17 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
   if (h->prof) {
     EventProf ep(h);
     ep.begin(LGAME_PROF_STEP);
-    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
+    if (h->mixed)
+      enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
+    else
+      enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
     ep.end(LGAME_PROF_STEP);
     if (ep.failed) return fail(h, LGAME_EHIP, "event recording failed");
   } else {
-    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
+    if (h->mixed)
+      enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
+    else
+      enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
   }
   HIPCHK(h, hipGetLastError());
   h->stepped = 1;
@@ -697,9 +772,17 @@ int lgame_integrals(lgame_handle *h, int slot, double *out) {
   double *partial = h->red, *res = h->red + 2 * NRED * RED_GRID;
   if (!h->stepped) {  // before the first step the temperature is diagnosed from this state (control.F90:484)
     const Geom &G = h->c.G;
-    hipLaunchKernelGGL(k_temperature, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->c, h->slot[slot]);
+    if (h->mixed)
+      hipLaunchKernelGGL(k_temperature<float>, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->cf, h->slot[slot]);
+    else
+      hipLaunchKernelGGL(k_temperature<double>, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->c, h->slot[slot]);
   }
-  hipLaunchKernelGGL(k_integrals, dim3(RED_GRID), dim3(RED_BLOCK), 0, h->stream, h->c, h->slot[slot], partial);
+  if (h->mixed)
+    hipLaunchKernelGGL(k_integrals<float>, dim3(RED_GRID), dim3(RED_BLOCK), 0, h->stream, h->c, h->slot[slot],
+                       Fld<const float>(h->cf.w.temperature), partial);
+  else
+    hipLaunchKernelGGL(k_integrals<double>, dim3(RED_GRID), dim3(RED_BLOCK), 0, h->stream, h->c, h->slot[slot],
+                       Fld<const double>(h->c.w.temperature), partial);
   hipLaunchKernelGGL(k_integrals_final, dim3(1), dim3(RED_BLOCK), 0, h->stream, (const double *)partial, nq, res);
   HIPCHK(h, hipGetLastError());
   double tmp[NRED];
@@ -727,7 +810,8 @@ int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad) {
 int lgame_get_diag(lgame_handle *h, int which, double *out) {
   if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
   if (which != LGAME_DIAG_TEMPERATURE) return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
-  return download(h, out, h->c.w.temperature, h->dims.ny, h->dims.nx, h->dims.n_layers);
+  if (h->mixed) return download(h, out, h->cf.w.temperature.p, h->dims.ny, h->dims.nx, h->dims.n_layers);
+  return download(h, out, h->c.w.temperature.p, h->dims.ny, h->dims.nx, h->dims.n_layers);
 }
 
 void lgame_finalize(lgame_handle *h) {

@@ -6,11 +6,16 @@
 // staggered fields (u: nx+1 columns, v: ny+1 rows, w: n_layers+1 levels) share one indexing rule.
 // Device functions take the reference's 1-based (ji, jk, jl) = (y, x, level) indices so the
 // arithmetic reads like the reference loops it restates.
+//
+// Precision: the state is always FP64.  The grid metrics and the work arrays of the stencil
+// kernels have the type R of the form: double in the plain form, float in the mixed form.  The
+// column solvers read FP64 metrics (`g64`), so in the mixed form the grid exists in both types.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <type_traits>
 
 namespace lgame {
 
@@ -31,6 +36,25 @@ constexpr double MPI = 3.14159265358979323846;
 
 constexpr int MAX_CONST = 7;
 
+// Element index inside one field.  Every field is smaller than 4 GiB (checked by lgame_init),
+// so the byte offset fits 32 bits and an access compiles to one load with a scalar base
+// register and a 32-bit vector offset.
+using ix_t = unsigned;
+
+// Pointer to one field; operator[] forms the 32-bit byte offset.
+template <class T>
+struct Fld {
+  T *p = nullptr;
+  Fld() = default;
+  __host__ __device__ Fld(T *q) : p(q) {}
+  template <class U>
+  __host__ __device__ Fld(const Fld<U> &o) : p(o.p) {}
+  __device__ __forceinline__ T &operator[](ix_t i) const {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(const_cast<typename std::remove_const<T>::type *>(p)) +
+                                  i * (ix_t)sizeof(T));
+  }
+};
+
 struct Geom {
   int ny, nx, nl, nlev, nflat, nc, ncond;
   int pitch;       // row pitch in elements
@@ -39,32 +63,35 @@ struct Geom {
 };
 
 // Read-only grid metrics in the device layout.
-struct GridDev {
-  const double *z_scalar, *z_u, *z_v, *z_w, *dx, *dy, *dz, *area_x, *area_y, *area_z, *volume;
-  const double *slope_x, *slope_y, *ipw[6], *area_dual_x, *area_dual_y, *area_dual_z;
-  const double *z_area_dual_z, *fvec_x, *fvec_y, *fvec_z, *ebg_u, *ebg_v, *ebg_w, *theta_v_bg;
-  const double *exner_bg, *gravity_m_v, *gravity_potential;
-  const double *klemp_pref;  // 1 + damping_coeff*dtime at w levels (mo_column_solvers.F90:190-196)
-  const double *trsk_u;      // (3, ny) as in the reference
-  const double *trsk_v;      // (2, ny+1)
+template <class R>
+struct GridT {
+  Fld<const R> z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume;
+  Fld<const R> slope_x, slope_y, ipw[6], area_dual_x, area_dual_y, area_dual_z;
+  Fld<const R> z_area_dual_z, fvec_x, fvec_y, fvec_z, ebg_u, ebg_v, ebg_w, theta_v_bg;
+  Fld<const R> exner_bg, gravity_m_v, gravity_potential;
+  Fld<const R> klemp_pref;  // 1 + damping_coeff*dtime at w levels (mo_column_solvers.F90:190-196)
+  const R *trsk_u;          // (3, ny) as in the reference
+  const R *trsk_v;          // (2, ny+1)
 };
+using GridDev = GridT<double>;
 
 struct StateDev {
-  double *rho[MAX_CONST];
-  double *rhotheta_v, *theta_v_pert, *exner_pert, *wind_u, *wind_v, *wind_w;
+  Fld<double> rho[MAX_CONST];
+  Fld<double> rhotheta_v, theta_v_pert, exner_pert, wind_u, wind_v, wind_w;
 };
 
 // Tendencies, carried diagnostics and scratch of one handle.
-struct WorkDev {
-  double *tend_rho[MAX_CONST];
-  double *tend_rhotheta_v, *tend_u, *tend_v, *tend_w;
-  double *pnl_u, *pnl_v, *pnl_w, *pl_u, *pl_v, *pl_w, *pold_u, *pold_v;
-  double *vsg_x, *vsg_y, *vsg_z, *pvt_x, *pvt_y, *pvt_z;
-  double *heating_diss, *csh, *temperature, *radiation_tendency;
-  double *eta_x, *eta_y, *eta_z, *u_ph, *v_ph, *w_ph, *v_squared;
-  double *tth[2], *tgu[2], *tgv[2], *tfd[2], *tu[2], *tv[2];  // theta_v_adv_3rd_order: full, pert
-  double *fu, *fv, *thu, *thv;                       // gas mass flux and theta_v flux
-  double *col[14];  // column-solver per-level scratch
+template <class R>
+struct WorkT {
+  Fld<R> tend_rho[MAX_CONST];
+  Fld<R> tend_rhotheta_v, tend_u, tend_v, tend_w;
+  Fld<R> pnl_u, pnl_v, pnl_w, pl_u, pl_v, pl_w, pold_u, pold_v;
+  Fld<R> vsg_x, vsg_y, vsg_z, pvt_x, pvt_y, pvt_z;
+  Fld<R> heating_diss, csh, temperature, radiation_tendency;
+  Fld<R> eta_x, eta_y, eta_z, u_ph, v_ph, w_ph, v_squared;
+  Fld<R> tth[2], tgu[2], tgv[2], tfd[2], tu[2], tv[2];  // theta_v_adv_3rd_order: full, pert
+  Fld<R> fu, fv, thu, thv;                              // gas mass flux and theta_v flux
+  Fld<double> col[14];                                  // column-solver per-level scratch (FP64)
 };
 
 struct Cfg {
@@ -72,14 +99,17 @@ struct Cfg {
   int order, llinear, lcorio, lmoist, lklemp, lfreeslip, n_damping_levels;
 };
 
-struct Ctx {
+template <class R>
+struct CtxT {
   Geom G;
   Cfg C;
-  GridDev g;
-  WorkDev w;
+  GridT<R> g;         // metrics read by the stencil kernels
+  GridT<double> g64;  // metrics read by the column solvers and budget sums
+  WorkT<R> w;
 };
+using Ctx = CtxT<double>;
 
 #define LG_IX(G, ji, jk, jl) \
-  ((size_t)((jl)-1) * (size_t)(G).lvl + (size_t)((ji)-1) * (size_t)(G).pitch + (size_t)((jk)-1))
+  ((ix_t)((jl)-1) * (ix_t)(G).lvl + (ix_t)((ji)-1) * (ix_t)(G).pitch + (ix_t)((jk)-1))
 
 }  // namespace lgame
