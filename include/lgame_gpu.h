@@ -49,8 +49,59 @@ extern "C" {
 #define LGAME_PREC_FP64   0   /* plain form: FP64 everywhere                                   */
 #define LGAME_PREC_MIXED  1   /* FP32 stencils/explicit tendencies, FP64 solves/updates/sums   */
 
-/* radiation modes (the step's lrad_update input only has an effect when not OFF) */
-#define LGAME_RAD_OFF     0
+/*
+ * Radiation modes (lgame_cfg.rad_mode; the step's lrad_update input only has an effect when not
+ * OFF and lrad is set).  The reference computes radiative heating with RTE+RRTMGP
+ * (src/radiation/mo_rterrtmgp_coupler.F90:56-493), a third-party library that is not part of the
+ * reference tree.  This library keeps the coupler's own steps (the pressure and temperature
+ * interfaces, the water paths, coszenith, calc_power_density and the surface fluxes) and puts a
+ * gray two-stream solver in the place of the RTE+RRTMGP calls, with the constants below:
+ *   longwave:  per-layer optical depth  dm*(K_LW_DRY*p/P_0 + K_LW_VAPOUR*q_v) + K_LW_LIQUID*LWP
+ *              + K_LW_ICE*IWP, transmissivity exp(-DIFFUSIVITY*tau), layer source sigma*T^4,
+ *              surface emission EMISSIVITY*sigma*T_sfc^4 plus the reflected rest;
+ *   shortwave: direct beam SOLAR_CONSTANT*mu_0 attenuated by exp(-tau_sw/mu_0), tau_sw =
+ *              dm*(K_SW_DRY + K_SW_VAPOUR*q_v) + K_SW_LIQUID*LWP + K_SW_ICE*IWP, reflected at the
+ *              surface with the albedo and attenuated upward by exp(-DIFFUSIVITY*tau_sw);
+ * dm = layer air mass per area from the interface pressures [kg m^-2], q_v = rho_vapour/rho_gas,
+ * LWP/IWP the reference's water paths [g m^-2].
+ *   LGAME_RAD_TEACHER:   that solver on every column (FP64); it is also the trainer's teacher.
+ *   LGAME_RAD_SURROGATE: the per-column MLP attached with lgame_rad_attach_mlp; each call also
+ *                        re-runs the solver on an audit sample of the columns and gives every
+ *                        audited column whose heating-rate RMSE exceeds the tolerance the
+ *                        solver's result (lgame_rad_stats counts both).
+ */
+#define LGAME_RAD_OFF        0
+#define LGAME_RAD_TEACHER    1
+#define LGAME_RAD_SURROGATE  2
+
+#define LGAME_RAD_SOLAR_CONSTANT  1361.0      /* W m^-2                                         */
+#define LGAME_RAD_SCALE_HEIGHT    8.e3        /* m, mo_rterrtmgp_coupler.F90:126                */
+#define LGAME_RAD_T_MIN           160.0       /* K, temperature clipping of the coupler          */
+#define LGAME_RAD_T_MAX           355.0       /* K                                               */
+#define LGAME_RAD_EMISSIVITY      0.98        /* mo_rterrtmgp_coupler.F90:150                    */
+#define LGAME_RAD_DIFFUSIVITY     1.66
+#define LGAME_RAD_P_0             100000.0    /* Pa                                              */
+#define LGAME_RAD_GRAVITY         9.80616     /* m s^-2                                          */
+#define LGAME_RAD_SIGMA           5.670374419e-8
+#define LGAME_RAD_K_LW_DRY        2.0e-4      /* m^2 kg^-1 (times p/P_0)                         */
+#define LGAME_RAD_K_LW_VAPOUR     0.08        /* m^2 kg^-1                                       */
+#define LGAME_RAD_K_LW_LIQUID     0.13        /* m^2 g^-1                                        */
+#define LGAME_RAD_K_LW_ICE        0.07        /* m^2 g^-1                                        */
+#define LGAME_RAD_K_SW_DRY        1.0e-5      /* m^2 kg^-1                                       */
+#define LGAME_RAD_K_SW_VAPOUR     0.02        /* m^2 kg^-1                                       */
+#define LGAME_RAD_K_SW_LIQUID     0.02        /* m^2 g^-1                                        */
+#define LGAME_RAD_K_SW_ICE        0.01        /* m^2 g^-1                                        */
+
+/*
+ * Surrogate column interface (n_layers = 60): inputs are 5 profiles of 60 layers (temperature
+ * [K], pressure [Pa], vapour, liquid and ice mixing ratios [kg/kg]; liquid = rain + cloud water,
+ * ice = snow + cloud ice + graupel) followed by 4 scalars (surface temperature [K], mu_0,
+ * surface albedo, surface pressure [Pa]); outputs are the 60 radiative heating rates [K/day]
+ * followed by sfc_sw_in and sfc_lw_out [W m^-2].
+ */
+#define LGAME_RAD_LEVELS     60
+#define LGAME_RAD_N_IN       (5 * LGAME_RAD_LEVELS + 4)
+#define LGAME_RAD_N_OUT      (LGAME_RAD_LEVELS + 2)
 
 /*
  * Dimensions.  Mirrors mo_run_nml.F90:13-19 (ny, nx, n_layers, n_oro_layers; n_levels and
@@ -151,6 +202,18 @@ typedef struct lgame_state_view {
 
 typedef struct lgame_handle lgame_handle;
 
+/*
+ * Per-column surface and location data of the radiation call (update_rad_fluxes,
+ * src/radiation/mo_manage_radiation_calls.F90:17-59): geographic latitude and longitude [rad],
+ * surface albedo and the surface temperature the coupler receives (temp_sfc_full), each (ny,nx).
+ */
+typedef struct lgame_rad_view {
+  const double *lat_geo;
+  const double *lon_geo;
+  const double *sfc_albedo;
+  const double *temp_sfc;
+} lgame_rad_view;
+
 /* Budget-sum slots written by lgame_integrals (src/io/mo_write_out.F90:226-304). */
 #define LGAME_INT_RHOTHETA   0   /* sum(rhotheta_v*volume)                                  */
 #define LGAME_INT_KINETIC    1   /* 0.5*sum(rho_g*(ipw-weighted |v|^2)*volume)              */
@@ -232,9 +295,71 @@ int lgame_get_diag(lgame_handle *h, int which, double *out);
 #define LGAME_PROF_SOLVE_VER   0   /* three_band_solver_ver kernels                  */
 #define LGAME_PROF_SOLVE_GEN   1   /* three_band_solver_gen_densities kernels        */
 #define LGAME_PROF_STEP        2   /* whole lgame_pchevi calls                       */
+#define LGAME_PROF_RADIATION   3   /* radiation updates (solver or surrogate + audit) */
 int lgame_profile(lgame_handle *h, int enable);
 int lgame_profile_read(lgame_handle *h, int group, double *total_ms, long long *count);
 int lgame_profile_reset(lgame_handle *h);
+
+/* ---- radiation boundary (update_rad_fluxes, src/radiation/mo_manage_radiation_calls.F90:17-59) ---- */
+
+/* Per-column surface and location data; needed before the first step with lrad_update. */
+int lgame_rad_set_surface(lgame_handle *h, const lgame_rad_view *surface);
+
+/*
+ * Inputs of the surrogate and outputs of the gray two-stream solver for every column of a state
+ * slot at time coordinate t (the coszenith time): x (ny*nx, 5*n_layers+4) and y (ny*nx,
+ * n_layers+2), row-major, columns in Fortran order ((ji-1) + ny*(jk-1)).  The temperature is
+ * diagnosed from the slot first (it becomes the handle's diag temperature).  Either pointer may
+ * be NULL.  This is how training data for the surrogate are harvested from the model's states.
+ */
+int lgame_rad_columns(lgame_handle *h, int slot, double t, float *x, float *y);
+
+/* Radiation outputs of the last update: LGAME_RAD_OUT_TENDENCY (ny,nx,nl) [W m^-3],
+   LGAME_RAD_OUT_SW_IN and LGAME_RAD_OUT_LW_OUT (ny,nx) [W m^-2], reference layout. */
+#define LGAME_RAD_OUT_TENDENCY 0
+#define LGAME_RAD_OUT_SW_IN    1
+#define LGAME_RAD_OUT_LW_OUT   2
+int lgame_rad_get(lgame_handle *h, int which, double *out);
+
+typedef struct lgame_mlp lgame_mlp;
+
+/*
+ * Use a trained MLP (n_in = LGAME_RAD_N_IN, n_out = LGAME_RAD_N_OUT, n_layers = 60) as the
+ * radiation of a handle configured with LGAME_RAD_SURROGATE.  At every update a fraction
+ * audit_fraction of the columns (a different sample each call, from seed) is recomputed with the
+ * solver; an audited column whose heating-rate RMSE exceeds tol_k_per_day takes the solver's
+ * result.  The handle keeps a pointer to m: m must outlive the handle's use of it.
+ */
+int lgame_rad_attach_mlp(lgame_handle *h, lgame_mlp *m, double tol_k_per_day, double audit_fraction, unsigned long long seed);
+
+/* Counters since lgame_init: radiation updates, audited columns, columns given back to the
+   solver, and the RMSE [K/day] of the surrogate's heating rates over all audited columns. */
+int lgame_rad_stats(lgame_handle *h, long long *calls, long long *audited, long long *fallbacks, double *audit_rmse);
+
+/* ---- the surrogate network: a per-column MLP trained and evaluated with BF16 MFMA kernels ----
+ * These entry points have no counterpart in the reference (which calls RTE+RRTMGP on every
+ * radiation update); they serve the learned replacement of that call.
+ * n_in -> hidden x n_hidden_layers (GELU) -> n_out; FP32 master weights with Adam, BF16
+ * weights and activations in the GEMMs, FP32 accumulation.  Parameters are ordered W_1 [out][in],
+ * b_1, W_2, b_2, ...  Inputs and targets are normalised with lgame_mlp_set_norm's mean and scale.
+ */
+int lgame_mlp_create(int n_in, int hidden, int n_hidden_layers, int n_out, unsigned long long seed, lgame_mlp **out);
+long long lgame_mlp_n_params(const lgame_mlp *m);
+int lgame_mlp_set_params(lgame_mlp *m, const float *params, long long n);
+int lgame_mlp_get_params(lgame_mlp *m, float *params, long long n);
+int lgame_mlp_set_norm(lgame_mlp *m, const float *x_mean, const float *x_scale, const float *y_mean, const float *y_scale);
+/* y (n, n_out) = network(x (n, n_in)), host arrays, physical units */
+int lgame_mlp_forward(lgame_mlp *m, const float *x, long long n, float *y);
+/* mean-squared error of the normalised outputs over n rows (a multiple of 64) and its gradient
+   with respect to every parameter */
+int lgame_mlp_gradient(lgame_mlp *m, const float *x, const float *y, long long n, float *grad, float *loss);
+/* upload a training set (host arrays, physical units) */
+int lgame_mlp_load_data(lgame_mlp *m, const float *x, const float *y, long long n);
+/* Adam epochs over the loaded set in shuffled batches (a multiple of 64); loss[e] per epoch;
+   *seconds the device time of the run */
+int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long long seed, float *loss, double *seconds);
+const char *lgame_mlp_last_error(const lgame_mlp *m);
+void lgame_mlp_destroy(lgame_mlp *m);
 
 /* Release the handle and its device memory. */
 void lgame_finalize(lgame_handle *h);

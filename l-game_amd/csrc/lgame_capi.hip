@@ -36,6 +36,16 @@ struct lgame_handle {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<int, int>>> ev_marks;  // group, (start, stop) pool index
   size_t ev_next = 0;
+  // radiation: per-column data (one allocation), surrogate and its audit
+  double *rad_mem = nullptr;
+  float *rad_x = nullptr, *rad_y = nullptr;
+  unsigned long long *rad_counts = nullptr;  // audited, fallbacks (device)
+  double *rad_sq = nullptr;                  // squared-error sum (device)
+  int rad_surface = 0;
+  lgame_mlp *mlp = nullptr;
+  double rad_tol = 0.0;
+  unsigned rad_thresh = 0;
+  unsigned long long rad_seed = 0, rad_calls = 0;
   std::string err;
 };
 
@@ -240,7 +250,8 @@ inline dim3 grid3(int nxr, int nyr, int nzr) { return dim3((nxr + BX - 1) / BX, 
 namespace lgame {
 
 template <class R>
-void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof) {
+void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
+                    const RadCall *rad) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const dim3 blk(BX, BY, 1);
   const dim3 g_cell = grid3(nx, ny, nl);
@@ -250,6 +261,12 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
   const bool o3 = c.C.order == 3;
 
   hipLaunchKernelGGL(k_temperature<R>, g_cell, blk, 0, st, c, old);
+  // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
+  if (rad) {
+    if (prof) prof->begin(3);
+    enqueue_radiation(c, old, *rad, st);
+    if (prof) prof->end(3);
+  }
   for (int rk = 1; rk <= 2; ++rk) {
     const StateDev &S = rk == 1 ? old : nw;
     if (rk == 1) {
@@ -296,8 +313,10 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
   }
 }
 
-template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *);
-template void enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *);
+template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
+                                     const RadCall *);
+template void enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
+                                    const RadCall *);
 
 }  // namespace lgame
 
@@ -334,6 +353,51 @@ struct EventProf : ProfHook {
     open_idx[group] = -1;
   }
 };
+
+}  // namespace
+
+namespace {
+
+// column arrays of the radiation data block (one allocation of 6 ncol doubles)
+RadDev rad_dev(lgame_handle *h) {
+  const size_t ncol = (size_t)h->dims.ny * h->dims.nx;
+  RadDev r{};
+  r.lat = h->rad_mem;
+  r.lon = h->rad_mem + ncol;
+  r.albedo = h->rad_mem + 2 * ncol;
+  r.tsfc = h->rad_mem + 3 * ncol;
+  r.sw_in = h->rad_mem + 4 * ncol;
+  r.lw_out = h->rad_mem + 5 * ncol;
+  r.x = h->rad_x;
+  r.y = h->rad_y;
+  r.ypred = h->rad_y;
+  r.counts = h->rad_counts;
+  r.sq = h->rad_sq;
+  r.tol = h->rad_tol;
+  r.audit_thresh = h->rad_thresh;
+  return r;
+}
+
+int rad_alloc(lgame_handle *h) {
+  if (h->rad_mem) return 0;
+  const size_t ncol = (size_t)h->dims.ny * h->dims.nx, nl = h->dims.n_layers;
+  HIPCHK(h, hipMalloc(&h->rad_mem, 6 * ncol * sizeof(double)));
+  HIPCHK(h, hipMemset(h->rad_mem, 0, 6 * ncol * sizeof(double)));
+  HIPCHK(h, hipMalloc(&h->rad_x, ncol * (5 * nl + 4) * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->rad_y, ncol * (nl + 2) * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->rad_counts, 2 * sizeof(unsigned long long)));
+  HIPCHK(h, hipMemset(h->rad_counts, 0, 2 * sizeof(unsigned long long)));
+  HIPCHK(h, hipMalloc(&h->rad_sq, sizeof(double)));
+  HIPCHK(h, hipMemset(h->rad_sq, 0, sizeof(double)));
+  return 0;
+}
+
+// (ny,nx) Fortran-order host array <-> device column order (ji-1)*nx + (jk-1)
+void to_device_cols(const double *host, std::vector<double> &out, int ny, int nx) {
+  out.resize((size_t)ny * nx);
+  for (int jk = 0; jk < nx; ++jk)
+    for (int ji = 0; ji < ny; ++ji) out[(size_t)ji * nx + jk] = host[ji + (size_t)ny * jk];
+}
 
 }  // namespace
 
@@ -399,7 +463,12 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     return fail(nullptr, LGAME_EINVAL, "unsupported: diffusion switches must be off");
   if (f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
     return fail(nullptr, LGAME_EINVAL, "unsupported: PBL, surface fluxes and soil must be off");
-  if (f.lrad || f.rad_mode != LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "unsupported: radiation");
+  if (f.rad_mode != LGAME_RAD_OFF && f.rad_mode != LGAME_RAD_TEACHER && f.rad_mode != LGAME_RAD_SURROGATE)
+    return fail(nullptr, LGAME_EINVAL, "unknown radiation mode %d", f.rad_mode);
+  if (f.lrad && f.rad_mode == LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "lrad needs a radiation mode (LGAME_RAD_*)");
+  if (f.lrad && !(f.dtime_rad > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime_rad must be positive");
+  if (f.lrad && f.rad_mode == LGAME_RAD_SURROGATE && d.n_layers != LGAME_RAD_LEVELS)
+    return fail(nullptr, LGAME_EINVAL, "the radiation surrogate needs n_layers = %d", LGAME_RAD_LEVELS);
   if (f.precision != LGAME_PREC_FP64 && f.precision != LGAME_PREC_MIXED)
     return fail(nullptr, LGAME_EINVAL, "unknown precision mode %d", f.precision);
   if (!(f.dtime > 0.0) || !(f.toa > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime and toa must be positive");
@@ -721,28 +790,36 @@ int lgame_get_state(lgame_handle *h, int slot, lgame_state_view *state) {
 }
 
 int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_counter, int lrad_update, double t_0) {
-  (void)t_0;
   if (!h) return fail(h, LGAME_EINVAL, "null handle");
   if ((slot_old != 0 && slot_old != 1) || (slot_new != 0 && slot_new != 1) || slot_old == slot_new)
     return fail(h, LGAME_EINVAL, "slots must be 0 and 1, distinct");
   if (!h->have_state[slot_old]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot_old);
   if (!h->have_state[slot_new]) return fail(h, LGAME_ESTATE, "slot %d holds no state (copy state_old there first)", slot_new);
-  if (lrad_update && h->cfg.lrad) return fail(h, LGAME_EINVAL, "radiation is not available");
   if (total_step_counter < 0) return fail(h, LGAME_EINVAL, "negative step counter");
-  if (h->prof) {
-    EventProf ep(h);
-    ep.begin(LGAME_PROF_STEP);
-    if (h->mixed)
-      enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
-    else
-      enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, &ep);
+  RadCall rc{};
+  const RadCall *rad = nullptr;
+  if (lrad_update && h->cfg.lrad) {
+    if (!h->rad_surface) return fail(h, LGAME_ESTATE, "radiation update before lgame_rad_set_surface");
+    if (h->cfg.rad_mode == LGAME_RAD_SURROGATE && !h->mlp)
+      return fail(h, LGAME_ESTATE, "radiation surrogate update before lgame_rad_attach_mlp");
+    rc.mode = h->cfg.rad_mode;
+    rc.t_rad = t_0 + 0.5 * h->cfg.dtime_rad;  // mo_manage_radiation_calls.F90:53
+    rc.dev = rad_dev(h);
+    rc.dev.call = h->rad_calls;
+    rc.mlp = h->mlp;
+    rad = &rc;
+    ++h->rad_calls;
+  }
+  EventProf ep(h);
+  ProfHook *ph = h->prof ? &ep : nullptr;
+  if (ph) ep.begin(LGAME_PROF_STEP);
+  if (h->mixed)
+    enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad);
+  else
+    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad);
+  if (ph) {
     ep.end(LGAME_PROF_STEP);
     if (ep.failed) return fail(h, LGAME_EHIP, "event recording failed");
-  } else {
-    if (h->mixed)
-      enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
-    else
-      enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, nullptr);
   }
   HIPCHK(h, hipGetLastError());
   h->stepped = 1;
@@ -814,6 +891,116 @@ int lgame_get_diag(lgame_handle *h, int which, double *out) {
   return download(h, out, h->c.w.temperature.p, h->dims.ny, h->dims.nx, h->dims.n_layers);
 }
 
+}  // extern "C"
+
+extern "C" {
+
+int lgame_rad_set_surface(lgame_handle *h, const lgame_rad_view *v) {
+  if (!h || !v || !v->lat_geo || !v->lon_geo || !v->sfc_albedo || !v->temp_sfc) return fail(h, LGAME_EINVAL, "null argument");
+  int rc = rad_alloc(h);
+  if (rc) return rc;
+  const int ny = h->dims.ny, nx = h->dims.nx;
+  const size_t ncol = (size_t)ny * nx;
+  const double *src[4] = {v->lat_geo, v->lon_geo, v->sfc_albedo, v->temp_sfc};
+  std::vector<double> tmp;
+  for (int q = 0; q < 4; ++q) {
+    to_device_cols(src[q], tmp, ny, nx);
+    HIPCHK(h, hipMemcpy(h->rad_mem + q * ncol, tmp.data(), ncol * sizeof(double), hipMemcpyHostToDevice));
+  }
+  h->rad_surface = 1;
+  return LGAME_OK;
+}
+
+int lgame_rad_columns(lgame_handle *h, int slot, double t, float *x, float *y) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  if ((slot != 0 && slot != 1) || !h->have_state[slot]) return fail(h, LGAME_ESTATE, "slot %d holds no state", slot);
+  if (!h->rad_surface) return fail(h, LGAME_ESTATE, "lgame_rad_set_surface first");
+  const Geom &G = h->c.G;
+  const int ny = G.ny, nx = G.nx, nl = G.nl, nin = 5 * nl + 4, nout = nl + 2;
+  const size_t ncol = (size_t)ny * nx;
+  RadDev r = rad_dev(h);
+  const dim3 gc((unsigned)((ncol + BCOL - 1) / BCOL)), bc(BCOL);
+  if (h->mixed) {
+    hipLaunchKernelGGL(k_temperature<float>, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->cf, h->slot[slot]);
+    hipLaunchKernelGGL(k_rad<float>, gc, bc, 0, h->stream, h->cf, h->slot[slot], r, t, RAD_X | RAD_Y);
+  } else {
+    hipLaunchKernelGGL(k_temperature<double>, grid3(G.nx, G.ny, G.nl), dim3(BX, BY, 1), 0, h->stream, h->c, h->slot[slot]);
+    hipLaunchKernelGGL(k_rad<double>, gc, bc, 0, h->stream, h->c, h->slot[slot], r, t, RAD_X | RAD_Y);
+  }
+  HIPCHK(h, hipGetLastError());
+  std::vector<float> hx(ncol * nin), hy(ncol * nout);
+  HIPCHK(h, hipMemcpyAsync(hx.data(), h->rad_x, hx.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(hy.data(), h->rad_y, hy.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int jk = 0; jk < nx; ++jk)
+    for (int ji = 0; ji < ny; ++ji) {
+      const size_t dc = (size_t)ji * nx + jk, fc = ji + (size_t)ny * jk;
+      if (x) std::memcpy(x + fc * nin, hx.data() + dc * nin, nin * sizeof(float));
+      if (y) std::memcpy(y + fc * nout, hy.data() + dc * nout, nout * sizeof(float));
+    }
+  return LGAME_OK;
+}
+
+int lgame_rad_get(lgame_handle *h, int which, double *out) {
+  if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
+  const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers;
+  if (which == LGAME_RAD_OUT_TENDENCY) {
+    if (h->mixed) return download(h, out, h->cf.w.radiation_tendency.p, ny, nx, nl);
+    return download(h, out, h->c.w.radiation_tendency.p, ny, nx, nl);
+  }
+  if (which != LGAME_RAD_OUT_SW_IN && which != LGAME_RAD_OUT_LW_OUT) return fail(h, LGAME_EINVAL, "unknown radiation output %d", which);
+  if (!h->rad_mem) return fail(h, LGAME_ESTATE, "no radiation data");
+  const size_t ncol = (size_t)ny * nx;
+  std::vector<double> tmp(ncol);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(tmp.data(), h->rad_mem + (which == LGAME_RAD_OUT_SW_IN ? 4 : 5) * ncol, ncol * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  for (int jk = 0; jk < nx; ++jk)
+    for (int ji = 0; ji < ny; ++ji) out[ji + (size_t)ny * jk] = tmp[(size_t)ji * nx + jk];
+  return LGAME_OK;
+}
+
+int lgame_rad_attach_mlp(lgame_handle *h, lgame_mlp *m, double tol_k_per_day, double audit_fraction, unsigned long long seed) {
+  if (!h || !m) return fail(h, LGAME_EINVAL, "null argument");
+  if (h->cfg.rad_mode != LGAME_RAD_SURROGATE) return fail(h, LGAME_EINVAL, "the handle is not configured for the surrogate");
+  int n_in = 0, n_out = 0;
+  mlp_dims(m, &n_in, &n_out);
+  if (h->dims.n_layers != LGAME_RAD_LEVELS || n_in != LGAME_RAD_N_IN || n_out != LGAME_RAD_N_OUT)
+    return fail(h, LGAME_EINVAL, "the network must map %d inputs to %d outputs on %d layers", LGAME_RAD_N_IN, LGAME_RAD_N_OUT,
+                LGAME_RAD_LEVELS);
+  if (!(tol_k_per_day > 0.0) || !(audit_fraction >= 0.0 && audit_fraction <= 1.0))
+    return fail(h, LGAME_EINVAL, "tolerance must be positive and the audit fraction in [0, 1]");
+  int rc = rad_alloc(h);
+  if (rc) return rc;
+  h->mlp = m;
+  h->rad_tol = tol_k_per_day;
+  h->rad_thresh = audit_fraction >= 1.0 ? 0xffffffffu : (unsigned)(audit_fraction * 4294967296.0);
+  h->rad_seed = seed;
+  h->rad_calls = seed * 1000003ull;
+  return LGAME_OK;
+}
+
+int lgame_rad_stats(lgame_handle *h, long long *calls, long long *audited, long long *fallbacks, double *audit_rmse) {
+  if (!h || !calls || !audited || !fallbacks || !audit_rmse) return fail(h, LGAME_EINVAL, "null argument");
+  *calls = *audited = *fallbacks = 0;
+  *audit_rmse = 0.0;
+  if (!h->rad_counts) return LGAME_OK;
+  unsigned long long c[2];
+  double sq = 0.0;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(c, h->rad_counts, sizeof c, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(&sq, h->rad_sq, sizeof sq, hipMemcpyDeviceToHost));
+  *calls = (long long)(h->rad_calls - h->rad_seed * 1000003ull);
+  *audited = (long long)c[0];
+  *fallbacks = (long long)c[1];
+  *audit_rmse = c[0] ? std::sqrt(sq / ((double)c[0] * h->dims.n_layers)) : 0.0;
+  return LGAME_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 void lgame_finalize(lgame_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -822,6 +1009,8 @@ void lgame_finalize(lgame_handle *h) {
   if (h->red) (void)hipFree(h->red);
   if (h->counter) (void)hipFree(h->counter);
   if (h->trsk) (void)hipFree(h->trsk);
+  for (void *p : {(void *)h->rad_mem, (void *)h->rad_x, (void *)h->rad_y, (void *)h->rad_counts, (void *)h->rad_sq})
+    if (p) (void)hipFree(p);
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

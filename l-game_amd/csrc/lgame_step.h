@@ -2,6 +2,7 @@
 #pragma once
 
 #include "lgame_dev.h"
+#include "lgame_rad.h"
 
 namespace lgame {
 
@@ -36,9 +37,10 @@ struct ProfHook {
   virtual ~ProfHook() = default;
 };
 
-// Enqueue one HEVI predictor-corrector step (pchevi, mo_manage_pchevi.F90:27-118) on `stream`.
+// Enqueue one HEVI predictor-corrector step (pchevi, mo_manage_pchevi.F90:27-118) on `stream`;
+// `rad` is non-null when the step updates the radiation (lrad_update).
 template <class R>
 void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int total_step_counter, hipStream_t stream,
-                    ProfHook *prof);
+                    ProfHook *prof, const RadCall *rad);
 
 }  // namespace lgame

@@ -73,3 +73,25 @@ def tiny_config(ny=5, nx=7, n_layers=6, moist=False, order=2, freeslip=False, co
                     lmoist=moist, lcorio=corio, orography_id=5, lsleve=True, lperiodic=True,
                     lfreeslip=freeslip, dtime=2.0, klemp_begin_rel=0.5, **kw)
     return dataclasses.replace(cfg, hills=((0.0, 0.0, 400.0, 4000.0), (3000.0, -2000.0, 250.0, 3000.0)))
+
+
+def radiation_surface(cfg: RunConfig, g: dict, st: dict, seed: int = 7) -> dict:
+    """Per-column surface data of the radiation call (set-up extension): geographic coordinates
+    of the grid, a land/sea mask from seeded random blobs (land where a smoothed Gaussian field
+    exceeds 0.3), albedo 0.2 on land and 0.06 on sea, and a surface temperature 1 K (sea) or 3 K
+    (land) above the temperature of the lowest layer of the initial state."""
+    from .init_state import _smooth_periodic
+
+    rng = np.random.default_rng(seed)
+    land = _smooth_periodic(rng.standard_normal((cfg.ny, cfg.nx)), max(2.0, 20e3 / cfg.dx)) > 0.3
+    nl = cfg.n_layers
+    t_low = (g["theta_v_bg"][:, :, nl - 1] + st["theta_v_pert"][:, :, nl - 1]) * (g["exner_bg"][:, :, nl - 1] +
+                                                                              st["exner_pert"][:, :, nl - 1])
+    return {
+        "lat_geo": g["lat_geo_scalar"],
+        "lon_geo": g["lon_geo_scalar"],
+        "sfc_albedo": np.asfortranarray(np.where(land, 0.2, 0.06)),
+        "temp_sfc": np.asfortranarray(t_low + np.where(land, 3.0, 1.0)),
+        "land_fraction": np.asfortranarray(land.astype(np.float64)),
+    }
+

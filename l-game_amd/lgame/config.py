@@ -17,6 +17,8 @@ import math
 PREC_FP64 = 0
 PREC_MIXED = 1
 RAD_OFF = 0
+RAD_TEACHER = 1
+RAD_SURROGATE = 2
 
 
 class LgameDims(ctypes.Structure):
@@ -74,6 +76,7 @@ class RunConfig:
     # set-up extensions (not reference namelist variables)
     hills: tuple = ()            # ((x0, y0, height, half_width), ...) in metres, orography_id == 5
     precision: int = PREC_FP64
+    rad_mode: int = RAD_OFF      # LGAME_RAD_* of the radiation call (with lrad)
 
     @property
     def n_levels(self) -> int:
@@ -123,7 +126,7 @@ class RunConfig:
         c.lfreeslip = int(self.lfreeslip)
         c.lrad = int(self.lrad)
         c.precision = self.precision
-        c.rad_mode = RAD_OFF
+        c.rad_mode = self.rad_mode
         return d, c
 
 

@@ -19,11 +19,21 @@ LIB_PATH = os.path.join(PKG_ROOT, "liblgame_gpu.so")
 
 EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_state", "lgame_pchevi",
             "lgame_pchevi_host", "lgame_sync", "lgame_integrals", "lgame_check_finite", "lgame_get_diag",
-            "lgame_finalize", "lgame_last_error", "lgame_profile", "lgame_profile_read", "lgame_profile_reset")
+            "lgame_finalize", "lgame_last_error", "lgame_profile", "lgame_profile_read", "lgame_profile_reset",
+            "lgame_rad_set_surface", "lgame_rad_columns", "lgame_rad_get", "lgame_rad_attach_mlp", "lgame_rad_stats",
+            "lgame_mlp_create", "lgame_mlp_n_params", "lgame_mlp_set_params", "lgame_mlp_get_params",
+            "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_gradient", "lgame_mlp_load_data",
+            "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy")
 
 PROF_SOLVE_VER = 0
 PROF_SOLVE_GEN = 1
 PROF_STEP = 2
+PROF_RADIATION = 3
+
+RAD_OUT_TENDENCY = 0
+RAD_OUT_SW_IN = 1
+RAD_OUT_LW_OUT = 2
+RAD_SURFACE_FIELDS = ("lat_geo", "lon_geo", "sfc_albedo", "temp_sfc")
 
 LGAME_DIAG_TEMPERATURE = 0
 
@@ -34,6 +44,10 @@ class LgameError(RuntimeError):
 
 class GridView(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in GRID_FIELDS]
+
+
+class RadView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in RAD_SURFACE_FIELDS]
 
 
 class StateView(ctypes.Structure):
@@ -69,6 +83,26 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_profile.argtypes = [vp, ci]
     lib.lgame_profile_read.argtypes = [vp, ci, ctypes.POINTER(cd), ctypes.POINTER(ctypes.c_longlong)]
     lib.lgame_profile_reset.argtypes = [vp]
+    ll, ull, fp = ctypes.c_longlong, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float)
+    lib.lgame_rad_set_surface.argtypes = [vp, vp]
+    lib.lgame_rad_columns.argtypes = [vp, ci, cd, vp, vp]
+    lib.lgame_rad_get.argtypes = [vp, ci, vp]
+    lib.lgame_rad_attach_mlp.argtypes = [vp, vp, cd, cd, ull]
+    lib.lgame_rad_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(cd)]
+    lib.lgame_mlp_create.argtypes = [ci, ci, ci, ci, ull, ctypes.POINTER(vp)]
+    lib.lgame_mlp_n_params.argtypes = [vp]
+    lib.lgame_mlp_n_params.restype = ll
+    lib.lgame_mlp_set_params.argtypes = [vp, vp, ll]
+    lib.lgame_mlp_get_params.argtypes = [vp, vp, ll]
+    lib.lgame_mlp_set_norm.argtypes = [vp, vp, vp, vp, vp]
+    lib.lgame_mlp_forward.argtypes = [vp, vp, ll, vp]
+    lib.lgame_mlp_gradient.argtypes = [vp, vp, vp, ll, vp, fp]
+    lib.lgame_mlp_load_data.argtypes = [vp, vp, vp, ll]
+    lib.lgame_mlp_train.argtypes = [vp, ci, ci, ctypes.c_float, ull, vp, ctypes.POINTER(cd)]
+    lib.lgame_mlp_last_error.argtypes = [vp]
+    lib.lgame_mlp_last_error.restype = ctypes.c_char_p
+    lib.lgame_mlp_destroy.argtypes = [vp]
+    lib.lgame_mlp_destroy.restype = None
     _lib = lib
     return lib
 
@@ -130,12 +164,59 @@ class Step:
     def pchevi(self, slot_old: int, slot_new: int, total_step_counter: int, lrad_update: bool = False, t_0: float = 0.0):
         self._check(self.lib.lgame_pchevi(self.h, slot_old, slot_new, total_step_counter, int(lrad_update), t_0), "lgame_pchevi")
 
-    def run(self, first: int, nsteps: int):
+    def run(self, first: int, nsteps: int, t_init: float | None = None):
+        """Ping-pong steps first..first+nsteps-1.  With t_init, the radiation is updated as in
+        control.F90:510-517 (when t_0 <= t_rad_update <= t_0 + dtime, t_rad_update starting at
+        t_init and advancing by dtime_rad)."""
+        dt = self.cfg.dt
+        t_rad_update = None
+        if t_init is not None:  # the first update time at or after the first step's t_0
+            k = max(0, int(np.ceil((first * dt) / self.ccfg.dtime_rad - 1e-12)))
+            t_rad_update = t_init + k * self.ccfg.dtime_rad
         for s in range(first, first + nsteps):
+            lrad, t0 = False, 0.0
+            if t_rad_update is not None:
+                t0 = t_init + s * dt
+                if self.cfg.lrad and t0 <= t_rad_update and t0 + dt >= t_rad_update:
+                    lrad = True
+                    t_rad_update += self.ccfg.dtime_rad
             if s % 2 == 0:
-                self.pchevi(0, 1, s)
+                self.pchevi(0, 1, s, lrad, t0)
             else:
-                self.pchevi(1, 0, s)
+                self.pchevi(1, 0, s, lrad, t0)
+
+    # ---- radiation ----
+    def set_rad_surface(self, surface: dict):
+        keep = {k: _f64(surface[k]) for k in RAD_SURFACE_FIELDS}
+        v = RadView(*[keep[k].ctypes.data for k in RAD_SURFACE_FIELDS])
+        self._check(self.lib.lgame_rad_set_surface(self.h, ctypes.byref(v)), "lgame_rad_set_surface")
+
+    def rad_columns(self, slot: int, t: float):
+        """(x, y): surrogate inputs and solver outputs of every column, Fortran column order."""
+        c = self.cfg
+        ncol = c.ny * c.nx
+        x = np.zeros((ncol, 5 * c.n_layers + 4), dtype=np.float32)
+        y = np.zeros((ncol, c.n_layers + 2), dtype=np.float32)
+        self._check(self.lib.lgame_rad_columns(self.h, slot, t, x.ctypes.data, y.ctypes.data), "lgame_rad_columns")
+        return x, y
+
+    def rad_get(self, which: int) -> np.ndarray:
+        c = self.cfg
+        shape = (c.ny, c.nx, c.n_layers) if which == RAD_OUT_TENDENCY else (c.ny, c.nx)
+        out = np.zeros(shape, order="F")
+        self._check(self.lib.lgame_rad_get(self.h, which, out.ctypes.data), "lgame_rad_get")
+        return out
+
+    def attach_mlp(self, mlp: "Mlp", tol_k_per_day: float = 1.0, audit_fraction: float = 0.01, seed: int = 0):
+        self._mlp = mlp  # the handle keeps a pointer to it
+        self._check(self.lib.lgame_rad_attach_mlp(self.h, mlp.m, tol_k_per_day, audit_fraction, seed), "lgame_rad_attach_mlp")
+
+    def rad_stats(self) -> dict:
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        r = ctypes.c_double()
+        self._check(self.lib.lgame_rad_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(r)),
+                    "lgame_rad_stats")
+        return {"calls": a.value, "audited": b.value, "fallbacks": c.value, "audit_rmse_k_per_day": r.value}
 
     def sync(self):
         self._check(self.lib.lgame_sync(self.h), "lgame_sync")
@@ -169,3 +250,73 @@ class Step:
         out = np.zeros((c.ny, c.nx, c.n_layers), order="F")
         self._check(self.lib.lgame_get_diag(self.h, LGAME_DIAG_TEMPERATURE, out.ctypes.data), "lgame_get_diag")
         return out
+
+
+class Mlp:
+    """The per-column MLP of the radiation surrogate (BF16 MFMA kernels, FP32 Adam)."""
+
+    def __init__(self, n_in: int, hidden: int, n_hidden_layers: int, n_out: int, seed: int = 0):
+        self.lib = load_library()
+        m = ctypes.c_void_p()
+        rc = self.lib.lgame_mlp_create(n_in, hidden, n_hidden_layers, n_out, seed, ctypes.byref(m))
+        if rc != 0:
+            raise LgameError(f"lgame_mlp_create failed ({rc})")
+        self.m = m
+        self.n_in, self.n_out = n_in, n_out
+        self.n_params = int(self.lib.lgame_mlp_n_params(m))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LgameError(f"{what} failed ({rc}): {self.lib.lgame_mlp_last_error(self.m).decode()}")
+
+    def close(self):
+        if getattr(self, "m", None):
+            self.lib.lgame_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self.lib.lgame_mlp_get_params(self.m, p.ctypes.data, self.n_params), "lgame_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.lgame_mlp_set_params(self.m, p.ctypes.data, p.size), "lgame_mlp_set_params")
+
+    def set_norm(self, x_mean, x_scale, y_mean, y_scale):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x_mean, x_scale, y_mean, y_scale)]
+        self._norm = arrs
+        self._check(self.lib.lgame_mlp_set_norm(self.m, *[a.ctypes.data for a in arrs]), "lgame_mlp_set_norm")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self._check(self.lib.lgame_mlp_forward(self.m, x.ctypes.data, x.shape[0], y.ctypes.data), "lgame_mlp_forward")
+        return y
+
+    def gradient(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.lgame_mlp_gradient(self.m, x.ctypes.data, y.ctypes.data, x.shape[0], g.ctypes.data,
+                                                ctypes.byref(loss)), "lgame_mlp_gradient")
+        return g, float(loss.value)
+
+    def load_data(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        self._check(self.lib.lgame_mlp_load_data(self.m, x.ctypes.data, y.ctypes.data, x.shape[0]), "lgame_mlp_load_data")
+
+    def train(self, epochs: int, batch: int, lr: float, seed: int = 0):
+        loss = np.zeros(epochs, dtype=np.float32)
+        sec = ctypes.c_double()
+        self._check(self.lib.lgame_mlp_train(self.m, epochs, batch, lr, seed, loss.ctypes.data, ctypes.byref(sec)),
+                    "lgame_mlp_train")
+        return loss, sec.value

@@ -220,6 +220,10 @@ def grid_setup(cfg: RunConfig) -> dict:
     lon_d = np.concatenate([lon_scalar - 0.5 * dlon, [lon_scalar[-1] + 0.5 * dlon]])[None, :].repeat(ny + 1, axis=0)
     lat_d_g, _ = _rotate_points(lat_d, lon_d, rot)
     g["fvec_z"] = np.asfortranarray(2.0 * OMEGA * np.sin(lat_d_g))
+    # geographic coordinates of the scalar points (lat_geo_scalar, lon_geo_scalar), used by radiation
+    lat_geo, lon_geo = _rotate_points(np.repeat(lat_scalar[:, None], nx, axis=1), np.repeat(lon_scalar[None, :], ny, axis=0), rot)
+    g["lat_geo_scalar"] = np.asfortranarray(lat_geo)
+    g["lon_geo_scalar"] = np.asfortranarray(lon_geo)
 
     # vertical grid, :894-951
     oro, oro_s = orography(cfg)

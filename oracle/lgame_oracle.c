@@ -18,7 +18,10 @@
  *
  * Supported configuration: periodic lateral boundaries, luse_bg_state, diffusion/PBL/surface
  * fluxes/soil off (the reference switches used by the Schaer and standard-atmosphere runs).
- * Anything else is rejected by orc_create.
+ * Anything else is rejected by orc_create.  Radiation: the coupler steps of
+ * radiation/mo_rterrtmgp_coupler.F90 are restated; RTE+RRTMGP, which is not in the reference tree,
+ * is replaced by the gray two-stream solver that include/lgame_gpu.h specifies (its outputs are
+ * pinned by closed forms and flux conservation, not by the reference).
  */
 #include <math.h>
 #include <stdio.h>
@@ -80,6 +83,10 @@ typedef struct oracle {
   F3 phase_trans_heating_rate, temp_diff_heating, condensates_sediment_heat, a_rain;
   double *phase_trans_rates;  /* (ny,nx,nl,ncond+1) */
   double *mass_diff_tendency; /* (ny,nx,nl,nc)     */
+  /* radiation: per-column surface data (ny,nx), outputs and switches */
+  int lrad, rad_mode;
+  double dtime_rad;
+  double *lat, *lon, *albedo, *tsfc, *sfc_sw_in, *sfc_lw_out;
   /* per-column work vectors */
   double *work;
   double *allocs[160];
@@ -1318,13 +1325,166 @@ static void three_band_solver_gen_densities(oracle *o, const ostate *old, ostate
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* radiation/mo_manage_radiation_calls.F90, radiation/mo_rterrtmgp_coupler.F90                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* coszenith, mo_rterrtmgp_coupler.F90:550-629 */
+double orc_coszenith(double lat, double lon, double t) {
+  const double omega = 7.292115e-5, omega_rev = 1.99099e-7, obliquity = 0.409092592843564;
+  const double t_0 = 1576843200.0, phi_0_earth_around_sun = 0.0, phi_0_earth_rotation = 0.0;
+  const double t_transformed = t - t_0;
+  const double rot_angle = omega * t_transformed - phi_0_earth_rotation;
+  double ne[3], ns[3], tr[3][3], s2e[3];
+  ne[0] = cos(lat) * cos(lon);
+  ne[1] = cos(lat) * sin(lon);
+  ne[2] = sin(lat);
+  tr[0][0] = -cos(rot_angle) * cos(obliquity);
+  tr[1][0] = -sin(rot_angle);
+  tr[2][0] = cos(rot_angle) * sin(obliquity);
+  tr[0][1] = sin(rot_angle) * cos(obliquity);
+  tr[1][1] = -cos(rot_angle);
+  tr[2][1] = -sin(rot_angle) * sin(obliquity);
+  tr[0][2] = sin(obliquity);
+  tr[1][2] = 0.0;
+  tr[2][2] = cos(obliquity);
+  for (int i = 0; i < 3; ++i) ns[i] = tr[i][0] * ne[0] + tr[i][1] * ne[1] + tr[i][2] * ne[2];
+  s2e[0] = cos(omega_rev * t_transformed + phi_0_earth_around_sun);
+  s2e[1] = sin(omega_rev * t_transformed + phi_0_earth_around_sun);
+  s2e[2] = 0.0;
+  double cz = ns[0] * (-s2e[0]) + ns[1] * (-s2e[1]) + ns[2] * (-s2e[2]);
+  if (cz < 0.0) cz = 0.0;
+  return cz;
+}
+
+static double rad_clip(double t) {
+  if (t > LGAME_RAD_T_MAX) t = LGAME_RAD_T_MAX;
+  if (t < LGAME_RAD_T_MIN) t = LGAME_RAD_T_MIN;
+  return t;
+}
+
+/*
+ * One column of calc_radiative_flux_convergence (mo_rterrtmgp_coupler.F90:56-493) with the gray
+ * two-stream solver in place of RTE+RRTMGP.  Writes the heating [W m^-3] of every layer to q,
+ * the surface fluxes, and (if x/y are non-null) the surrogate's inputs and targets.
+ * Uses o->temperature (diagnosed from the state at the start of the step, as the reference).
+ */
+static void rad_column(oracle *o, const ostate *s, int ji, int jk, double mu0, double *q, double *sw_in, double *lw_out,
+                       double *x, double *y) {
+  const int nl = o->nl, nlev = o->nlev, ncond = o->ncond;
+  const size_t n = (size_t)o->ny * o->nx * nl;
+  const int col = (ji - 1) + o->ny * (jk - 1);
+  double *w = o->work; /* 8 * nlev */
+  double *T = w, *p = w + nlev, *pi = w + 2 * nlev, *tl = w + 3 * nlev, *B = w + 4 * nlev, *ts = w + 5 * nlev;
+  double *dn = w + 6 * nlev, *up = w + 7 * nlev;
+  double *qv = o->work + 8 * nlev, *lwp = qv + nlev, *iwp = lwp + nlev, *ql = iwp + nlev, *qi = ql + nlev;
+#define RHO(jc, jl) s->rho[((size_t)(ji - 1) + (size_t)o->ny * ((size_t)(jk - 1) + (size_t)o->nx * (size_t)((jl) - 1))) + (size_t)((jc) - 1) * n]
+  /* mo_rterrtmgp_coupler.F90:158-167: temperature and pressure (equation of state) */
+  for (int jl = 1; jl <= nl; ++jl) {
+    T[jl - 1] = A3(o->temperature, ji, jk, jl);
+    p[jl - 1] = R_D * RHO(ncond + 1, jl) * T[jl - 1];
+  }
+  /* :172-240 water paths [g m^-2] (dry: zero) */
+  for (int jl = 1; jl <= nl; ++jl) {
+    const double thickness = A3(o->z_w, ji, jk, jl) - A3(o->z_w, ji, jk, jl + 1);
+    if (o->lmoist) {
+      iwp[jl - 1] = thickness * 1000.0 * (RHO(1, jl) + RHO(3, jl) + RHO(5, jl));
+      lwp[jl - 1] = thickness * 1000.0 * (RHO(2, jl) + RHO(4, jl));
+      qv[jl - 1] = RHO(ncond + 2, jl) / RHO(ncond + 1, jl);
+      qi[jl - 1] = (RHO(1, jl) + RHO(3, jl) + RHO(5, jl)) / RHO(ncond + 1, jl);
+      ql[jl - 1] = (RHO(2, jl) + RHO(4, jl)) / RHO(ncond + 1, jl);
+    } else {
+      iwp[jl - 1] = lwp[jl - 1] = qv[jl - 1] = qi[jl - 1] = ql[jl - 1] = 0.0;
+    }
+  }
+  /* :243-257 temperature clipping */
+  for (int jl = 1; jl <= nl; ++jl) T[jl - 1] = rad_clip(T[jl - 1]);
+  /* :259-296 interface pressures (the TOA value uses z_scalar of layer 2, as the reference) */
+  for (int jl = 1; jl <= nlev; ++jl) {
+    if (jl == 1)
+      pi[0] = p[0] * exp(-(A3(o->z_w, ji, jk, 1) - A3(o->z_scalar, ji, jk, 2)) / LGAME_RAD_SCALE_HEIGHT);
+    else if (jl == nlev)
+      pi[nl] = p[nl - 1] * exp(-(A3(o->z_w, ji, jk, nlev) - A3(o->z_scalar, ji, jk, nl)) / LGAME_RAD_SCALE_HEIGHT);
+    else
+      pi[jl - 1] = 0.5 * (p[jl - 2] + p[jl - 1]);
+  }
+  const double t_sfc = rad_clip(o->tsfc[col]);
+  /* gray two-stream longwave */
+  for (int jl = 1; jl <= nl; ++jl) {
+    const double dm = (pi[jl] - pi[jl - 1]) / LGAME_RAD_GRAVITY;
+    const double tau = dm * (LGAME_RAD_K_LW_DRY * p[jl - 1] / LGAME_RAD_P_0 + LGAME_RAD_K_LW_VAPOUR * qv[jl - 1]) +
+                       LGAME_RAD_K_LW_LIQUID * lwp[jl - 1] + LGAME_RAD_K_LW_ICE * iwp[jl - 1];
+    const double tau_sw = dm * (LGAME_RAD_K_SW_DRY + LGAME_RAD_K_SW_VAPOUR * qv[jl - 1]) + LGAME_RAD_K_SW_LIQUID * lwp[jl - 1] +
+                          LGAME_RAD_K_SW_ICE * iwp[jl - 1];
+    tl[jl - 1] = exp(-LGAME_RAD_DIFFUSIVITY * tau);
+    const double t2 = T[jl - 1] * T[jl - 1];
+    B[jl - 1] = LGAME_RAD_SIGMA * (t2 * t2);
+    ts[jl - 1] = tau_sw;
+  }
+  for (int jl = 1; jl <= nl; ++jl) q[jl - 1] = 0.0;
+  /* shortwave on day columns (:326-433); calc_power_density adds SW then LW (:495-547) */
+  double sw = 0.0;
+  if (mu0 > 0.0) {
+    dn[0] = LGAME_RAD_SOLAR_CONSTANT * mu0;
+    for (int jl = 1; jl <= nl; ++jl) dn[jl] = dn[jl - 1] * exp(-ts[jl - 1] / mu0);
+    up[nl] = o->albedo[col] * dn[nl];
+    for (int jl = nl; jl >= 1; --jl) up[jl - 1] = up[jl] * exp(-LGAME_RAD_DIFFUSIVITY * ts[jl - 1]);
+    for (int jl = 1; jl <= nl; ++jl)
+      q[jl - 1] = q[jl - 1] + (up[jl] - up[jl - 1] + dn[jl - 1] - dn[jl]) / (A3(o->z_w, ji, jk, jl) - A3(o->z_w, ji, jk, jl + 1));
+    sw = dn[nl] - up[nl];
+  }
+  dn[0] = 0.0;
+  for (int jl = 1; jl <= nl; ++jl) dn[jl] = dn[jl - 1] * tl[jl - 1] + B[jl - 1] * (1.0 - tl[jl - 1]);
+  const double ts2 = t_sfc * t_sfc;
+  up[nl] = LGAME_RAD_EMISSIVITY * LGAME_RAD_SIGMA * (ts2 * ts2) + (1.0 - LGAME_RAD_EMISSIVITY) * dn[nl];
+  for (int jl = nl; jl >= 1; --jl) up[jl - 1] = up[jl] * tl[jl - 1] + B[jl - 1] * (1.0 - tl[jl - 1]);
+  for (int jl = 1; jl <= nl; ++jl)
+    q[jl - 1] = q[jl - 1] + (up[jl] - up[jl - 1] + dn[jl - 1] - dn[jl]) / (A3(o->z_w, ji, jk, jl) - A3(o->z_w, ji, jk, jl + 1));
+  *sw_in = sw;
+  *lw_out = up[nl] - dn[nl];
+  if (x) {
+    for (int jl = 0; jl < nl; ++jl) {
+      x[jl] = T[jl];
+      x[nl + jl] = p[jl];
+      x[2 * nl + jl] = qv[jl];
+      x[3 * nl + jl] = ql[jl];
+      x[4 * nl + jl] = qi[jl];
+    }
+    x[5 * nl] = t_sfc;
+    x[5 * nl + 1] = mu0;
+    x[5 * nl + 2] = o->albedo[col];
+    x[5 * nl + 3] = pi[nl];
+  }
+  if (y) {
+    for (int jl = 1; jl <= nl; ++jl) y[jl - 1] = 86400.0 * q[jl - 1] / (RHO(ncond + 1, jl) * C_D_P);
+    y[nl] = sw;
+    y[nl + 1] = *lw_out;
+  }
+#undef RHO
+}
+
+/* update_rad_fluxes, mo_manage_radiation_calls.F90:17-59: all columns at time_coordinate +
+   dtime_rad/2 (the surface temperature comes in directly: soil and sea are not prognostic here) */
+static void update_rad_fluxes(oracle *o, const ostate *s, double t) {
+  const double t_rad = t + 0.5 * o->dtime_rad;
+  for (int jk = 1; jk <= o->nx; ++jk)
+    for (int ji = 1; ji <= o->ny; ++ji) {
+      const int col = (ji - 1) + o->ny * (jk - 1);
+      const double mu0 = orc_coszenith(o->lat[col], o->lon[col], t_rad);
+      double q[512];
+      rad_column(o, s, ji, jk, mu0, q, &o->sfc_sw_in[col], &o->sfc_lw_out[col], NULL, NULL);
+      for (int jl = 1; jl <= o->nl; ++jl) A3(o->radiation_tendency, ji, jk, jl) = q[jl - 1];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* time_stepping/mo_manage_pchevi.F90                                                          */
 /* ------------------------------------------------------------------------------------------ */
 
-/* pchevi, mo_manage_pchevi.F90:27-118 (radiation, surface and boundary updates off) */
-static void pchevi(oracle *o, ostate *old, ostate *nw, int step) {
+/* pchevi, mo_manage_pchevi.F90:27-118 (surface and boundary updates off) */
+static void pchevi(oracle *o, ostate *old, ostate *nw, int step, int lrad_update, double t_0) {
   size_t nu = (size_t)o->ny * (o->nx + 1) * o->nl, nv = (size_t)(o->ny + 1) * o->nx * o->nl;
   temperature_diagnostics(o, old);
+  if (lrad_update && o->lrad) update_rad_fluxes(o, old, t_0);
   for (int rk = 1; rk <= 2; ++rk) {
     if (rk == 1) manage_p_grad(o, old, step == 0);
     p_grad_condensates_v(o, old);
@@ -1479,6 +1639,15 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   o->phase_trans_rates = oalloc(o, (size_t)ny * nx * nl * (o->ncond + 1));
   o->mass_diff_tendency = oalloc(o, (size_t)ny * nx * nl * o->nc);
   o->work = oalloc(o, (size_t)24 * (nl + 2));
+  o->lrad = cf->lrad != 0;
+  o->rad_mode = cf->rad_mode;
+  o->dtime_rad = cf->dtime_rad;
+  o->lat = oalloc(o, (size_t)ny * nx);
+  o->lon = oalloc(o, (size_t)ny * nx);
+  o->albedo = oalloc(o, (size_t)ny * nx);
+  o->tsfc = oalloc(o, (size_t)ny * nx);
+  o->sfc_sw_in = oalloc(o, (size_t)ny * nx);
+  o->sfc_lw_out = oalloc(o, (size_t)ny * nx);
   return o;
 }
 
@@ -1569,6 +1738,7 @@ int orc_get_diag(oracle *o, const char *name, double *dst) {
   if (!strcmp(name, "p_grad_acc_old_v")) f = &o->pold_v;
   if (!strcmp(name, "tend_wind_w")) f = &o->tend_w;
   if (!strcmp(name, "condensates_sediment_heat")) f = &o->condensates_sediment_heat;
+  if (!strcmp(name, "radiation_tendency")) f = &o->radiation_tendency;
   if (!f) return 1;
   memcpy(dst, f->p, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
   return 0;
@@ -1576,7 +1746,44 @@ int orc_get_diag(oracle *o, const char *name, double *dst) {
 
 /* One reference step: state slot `nw` = pchevi(slot `old`). */
 int orc_step(oracle *o, int old, int nw, int total_step_counter) {
-  pchevi(o, &o->st[old], &o->st[nw], total_step_counter);
+  pchevi(o, &o->st[old], &o->st[nw], total_step_counter, 0, 0.0);
+  return 0;
+}
+
+/* One step with the radiation switch and time of control.F90:510-524. */
+int orc_step_rad(oracle *o, int old, int nw, int total_step_counter, int lrad_update, double t_0) {
+  pchevi(o, &o->st[old], &o->st[nw], total_step_counter, lrad_update, t_0);
+  return 0;
+}
+
+/* Per-column surface data of the radiation call, each (ny,nx). */
+void orc_set_rad_surface(oracle *o, const double *lat, const double *lon, const double *albedo, const double *tsfc) {
+  const size_t n = (size_t)o->ny * o->nx;
+  memcpy(o->lat, lat, n * sizeof(double));
+  memcpy(o->lon, lon, n * sizeof(double));
+  memcpy(o->albedo, albedo, n * sizeof(double));
+  memcpy(o->tsfc, tsfc, n * sizeof(double));
+}
+
+/* Surrogate inputs x (ncol, LGAME_RAD_N_IN style: 5*nl+4) and teacher targets y (ncol, nl+2) of
+   every column of a slot at time t (column index (ji-1) + ny*(jk-1)); diagnoses the temperature
+   of the slot first. */
+void orc_rad_columns(oracle *o, int slot, double t, double *x, double *y) {
+  temperature_diagnostics(o, &o->st[slot]);
+  const int nin = 5 * o->nl + 4, nout = o->nl + 2;
+  for (int jk = 1; jk <= o->nx; ++jk)
+    for (int ji = 1; ji <= o->ny; ++ji) {
+      const int col = (ji - 1) + o->ny * (jk - 1);
+      const double mu0 = orc_coszenith(o->lat[col], o->lon[col], t);
+      double q[512], a, b;
+      rad_column(o, &o->st[slot], ji, jk, mu0, q, &a, &b, x + (size_t)col * nin, y + (size_t)col * nout);
+    }
+}
+
+int orc_get_rad_sfc(oracle *o, double *sw_in, double *lw_out) {
+  const size_t n = (size_t)o->ny * o->nx;
+  memcpy(sw_in, o->sfc_sw_in, n * sizeof(double));
+  memcpy(lw_out, o->sfc_lw_out, n * sizeof(double));
   return 0;
 }
 
@@ -1584,9 +1791,9 @@ int orc_step(oracle *o, int old, int nw, int total_step_counter) {
 int orc_run(oracle *o, int first, int nsteps) {
   for (int s = first; s < first + nsteps; ++s) {
     if (s % 2 == 0)
-      pchevi(o, &o->st[0], &o->st[1], s);
+      pchevi(o, &o->st[0], &o->st[1], s, 0, 0.0);
     else
-      pchevi(o, &o->st[1], &o->st[0], s);
+      pchevi(o, &o->st[1], &o->st[0], s, 0, 0.0);
   }
   return 0;
 }

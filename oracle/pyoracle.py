@@ -65,6 +65,14 @@ class CpuOracle:
             lib.orc_n_damping_levels.argtypes = [ctypes.c_void_p]
             lib.orc_destroy.argtypes = [ctypes.c_void_p]
             lib.orc_last_error.restype = ctypes.c_char_p
+            lib.orc_step_rad.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+            lib.orc_set_rad_surface.argtypes = [ctypes.c_void_p] * 5
+            lib.orc_set_rad_surface.restype = None
+            lib.orc_rad_columns.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+            lib.orc_rad_columns.restype = None
+            lib.orc_get_rad_sfc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+            lib.orc_coszenith.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+            lib.orc_coszenith.restype = ctypes.c_double
             cls._lib = lib
         return cls._lib
 
@@ -119,6 +127,43 @@ class CpuOracle:
 
     def step(self, old: int, new: int, counter: int):
         self.lib().orc_step(self.h, old, new, counter)
+
+    # ---- radiation (coupler steps of mo_rterrtmgp_coupler.F90 with the gray two-stream solver) ----
+    def set_rad_surface(self, surface: dict):
+        self._rad = [_fortran(surface[k]) for k in ("lat_geo", "lon_geo", "sfc_albedo", "temp_sfc")]
+        self.lib().orc_set_rad_surface(self.h, *[_ptr(a) for a in self._rad])
+
+    def run_rad(self, first: int, nsteps: int, t_init: float):
+        """Steps with the radiation schedule of control.F90:510-517 (same as lgame.gpu.Step.run)."""
+        dt, drad = self.cfg.dt, self.ccfg.dtime_rad
+        k = max(0, int(np.ceil((first * dt) / drad - 1e-12)))
+        t_rad_update = t_init + k * drad
+        for s in range(first, first + nsteps):
+            t0 = t_init + s * dt
+            lrad = 0
+            if self.cfg.lrad and t0 <= t_rad_update and t0 + dt >= t_rad_update:
+                lrad = 1
+                t_rad_update += drad
+            old, new = (0, 1) if s % 2 == 0 else (1, 0)
+            self.lib().orc_step_rad(self.h, old, new, s, lrad, t0)
+
+    def rad_columns(self, slot: int, t: float):
+        c = self.cfg
+        ncol = c.ny * c.nx
+        x = np.zeros((ncol, 5 * c.n_layers + 4))
+        y = np.zeros((ncol, c.n_layers + 2))
+        self.lib().orc_rad_columns(self.h, slot, t, _ptr(x), _ptr(y))
+        return x, y
+
+    def rad_sfc(self):
+        c = self.cfg
+        sw, lw = np.zeros((c.ny, c.nx), order="F"), np.zeros((c.ny, c.nx), order="F")
+        self.lib().orc_get_rad_sfc(self.h, _ptr(sw), _ptr(lw))
+        return sw, lw
+
+    @classmethod
+    def coszenith(cls, lat: float, lon: float, t: float) -> float:
+        return cls.lib().orc_coszenith(lat, lon, t)
 
     def integrals(self, slot: int) -> np.ndarray:
         out = np.zeros(4 + self.cfg.n_constituents)

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "lgame_gpu.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*(lgame_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|long long|const char \*)\s*(lgame_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_the_bound_functions():
