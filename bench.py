@@ -1,27 +1,32 @@
 """Benchmark of the MI355X HEVI predictor-corrector step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--form both|mixed|fp64]
 
 Runs the step on one GPU with the state resident in HBM, times exactly K steps between two
 stream synchronizations, and prints one JSON line (last line of stdout).  Progress goes to stderr.
 
 Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
-  moist512  512x512x60 moist (7 constituents), dx 2 km, dt 3.22 s   -- the operational-size config
-  dry128    128x128x50 dry, rotated lat-lon, dx 2 km, dt 3.22 s
-  schaer    ny=3, nx=201, 40 layers, dx 500 m, dt 0.805 s
-Both forms of the step are timed: the plain FP64 form and the mixed form (FP32 stencils and
-explicit tendencies, FP64 column solves, density updates and budget sums).  The headline value is
-the mixed form; `forms` lists both and `speedup_mixed_vs_fp64` their ratio.
+  moist512        512x512x60 moist (7 constituents), dx 2 km, dt 3.22 s, radiation every 900 s
+  dry128          128x128x50 dry, rotated lat-lon, dx 2 km, dt 3.22 s
+  schaer          ny=3, nx=201, 40 layers, dx 500 m, dt 0.805 s
+  surrogate-train harvest radiation columns from the model and train the 304 -> 3x512 -> 62 MLP
+
+Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
+and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
+and budget sums) with the MLP radiation surrogate and its 1% audit.  Radiation updates follow
+the model's cadence (dtime_rad / dtime steps), the first one inside the first timed step.  The
+headline value is the mixed form; `forms` lists both and `speedup_mixed_vs_fp64` their ratio.
 The roofline object is the implicit vertical solver (three_band_solver_ver, FP64 in both forms),
-timed with HIP events on the library's own stream; its algorithmic bytes are 152 B (predictor) and 176 B
-(corrector) per grid point (state, tendency and grid arrays streamed once, FP64).
+timed with HIP events on the library's own stream; its algorithmic bytes are 152 B (predictor)
+and 176 B (corrector) per grid point (state, tendency and grid arrays streamed once, FP64).
 The CPU baseline is the oracle (C restatement, one thread) on a bounded sample of the same
-workload type; each GPU form is run on that sample and its per-field RMS and max error against
-the oracle are reported in `field_error`.
+workload type, with the same radiation schedule; each GPU form is run on that sample and its
+per-field RMS and max error against the oracle are reported in `field_error`.
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -31,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "l-game_amd"))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 SOLVER_BYTES = {1: 152.0, 2: 176.0}          # per grid point, FP64, predictor / corrector
 STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step traffic model
 # HBM bytes per k_solve_ver dispatch, averaged over predictor and corrector, from two rocprofv3
@@ -38,6 +44,8 @@ STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step 
 # profiles/moist512_fp64_hbm_traffic.md holds the commands and the per-kernel table.
 SOLVER_TRAFFIC = {"moist512": 4.929e9}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
+SURROGATE_WEIGHTS = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
+T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
 
 
 def log(msg):
@@ -46,45 +54,112 @@ def log(msg):
 
 def workload(name):
     from lgame.cases import dry3d_config, moist3d_config, schaer_config
+    from lgame.config import RAD_TEACHER
 
     if name == "moist512":
-        return moist3d_config(n=512, n_layers=60, dtime=3.22), moist3d_config(n=65, n_layers=60, dtime=3.22)
+        rad = dict(lrad=True, rad_mode=RAD_TEACHER, dtime_rad=900.0)
+        return (dataclasses.replace(moist3d_config(n=512, n_layers=60, dtime=3.22), **rad),
+                dataclasses.replace(moist3d_config(n=65, n_layers=60, dtime=3.22), **rad))
     if name == "dry128":
         return dry3d_config(n=128, n_layers=50, dtime=3.22), dry3d_config(n=65, n_layers=50, dtime=3.22)
     if name == "schaer":
-        from lgame.cases import schaer_config as sc
-        return sc(nx=201, n_layers=40), sc(nx=201, n_layers=40)
+        return schaer_config(nx=201, n_layers=40), schaer_config(nx=201, n_layers=40)
     raise SystemExit(f"unknown workload {name}")
 
 
 def describe(cfg):
     kind = "moist" if cfg.lmoist else "dry"
-    return f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, theta order {cfg.theta_adv_order}"
+    rad = f", radiation every {cfg.dtime_rad:g} s" if cfg.lrad else ", radiation off"
+    return f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, theta order {cfg.theta_adv_order}{rad}"
 
 
-def cpu_baseline(cfg, nsteps, forms):
-    """Oracle timing on the host (one thread) plus the GPU forms' per-field error on the same sample."""
-    import dataclasses
+def surrogate_mlp():
+    """The trained radiation surrogate: the committed weights, or a quick training run."""
+    from lgame import surrogate
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from lgame.cases import make_case
-    from lgame.config import PREC_FP64, PREC_MIXED
+    if os.path.exists(SURROGATE_WEIGHTS):
+        log(f"radiation surrogate: {os.path.relpath(SURROGATE_WEIGHTS, ROOT)}")
+        return surrogate.load(SURROGATE_WEIGHTS)
+    log("radiation surrogate: no trained weights, training a small one on the GPU")
+    m, _, rep = surrogate.build(states=8, epochs=30)
+    log(f"surrogate trained: {rep}")
+    return m
+
+
+def make_step(cfg, grid, surface, form, mlp):
+    from lgame.config import PREC_FP64, PREC_MIXED, RAD_SURROGATE, RAD_TEACHER
     from lgame.gpu import Step
+
+    if form == "mixed":
+        c = dataclasses.replace(cfg, precision=PREC_MIXED, rad_mode=RAD_SURROGATE if cfg.lrad else cfg.rad_mode)
+    else:
+        c = dataclasses.replace(cfg, precision=PREC_FP64, rad_mode=RAD_TEACHER if cfg.lrad else cfg.rad_mode)
+    s = Step(c, grid)
+    if cfg.lrad:
+        s.set_rad_surface(surface)
+        if form == "mixed":
+            s.attach_mlp(mlp, tol_k_per_day=1.0, audit_fraction=0.01, seed=7)
+    return s
+
+
+def run_steps(step, cfg, first, n):
+    """n steps from counter `first`, radiation at the model's cadence starting at step `first`."""
+    every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
+    for k in range(first, first + n):
+        lrad = bool(every) and (k - first) % every == 0
+        step.pchevi(k % 2, 1 - k % 2, k, lrad, T_EPOCH + k * cfg.dt)
+
+
+def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
+    """Throughput of one form of the step plus event timing of the FP64 column solver."""
+    from lgame.gpu import PROF_RADIATION, PROF_SOLVE_GEN, PROF_SOLVE_VER, PROF_STEP
+
+    step = make_step(cfg, grid, surface, form, mlp)
+    step.start(st)
+    run_steps(step, cfg, 0, warmup)
+    step.sync()
+    # timed region: exactly K steps between two synchronizations of the library's stream
+    t0 = time.perf_counter()
+    run_steps(step, cfg, warmup, steps)
+    step.sync()
+    elapsed = time.perf_counter() - t0
+    counter = warmup + steps
+    bad = step.count_nonfinite(counter % 2)
+    rad = step.rad_stats() if cfg.lrad else {}
+    # kernel timing with HIP events on the library's stream (separate pass)
+    step.profile(True)
+    run_steps(step, cfg, counter, max(2, min(steps, 6)))
+    prof = {g: step.profile_read(g) for g in (PROF_SOLVE_VER, PROF_SOLVE_GEN, PROF_STEP, PROF_RADIATION)}
+    step.profile(False)
+    step.close()
+    return elapsed, bad, prof, rad
+
+
+def cpu_baseline(cfg, nsteps, forms, mlp):
+    """Oracle timing on the host (one thread) plus the GPU forms' per-field error on the same sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from lgame.cases import make_case, radiation_surface
     from lgame.metrics import field_errors, worst
     from pyoracle import CpuOracle
 
     grid, st = make_case(cfg)
+    surface = radiation_surface(cfg, grid, st)
     o = CpuOracle(cfg, grid)
     o.start(st)
+    o.set_rad_surface(surface)
+    every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
+    lib = o.lib()
     t0 = time.perf_counter()
-    o.run(0, nsteps)
+    for k in range(nsteps):
+        lrad = int(bool(every) and k % every == 0)
+        lib.orc_step_rad(o.h, k % 2, 1 - k % 2, k, lrad, T_EPOCH + k * cfg.dt)
     dt = time.perf_counter() - t0
     want = o.get_state(nsteps % 2)
     errs = {}
     for form in forms:
-        g = Step(dataclasses.replace(cfg, precision=PREC_MIXED if form == "mixed" else PREC_FP64), grid)
+        g = make_step(cfg, grid, surface, form, mlp)
         g.start(st)
-        g.run(0, nsteps)
+        run_steps(g, cfg, 0, nsteps)
         got = g.get_state(nsteps % 2)
         g.close()
         e = field_errors(got, want)
@@ -111,37 +186,27 @@ def copy_calibration(nbytes):
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
-def time_form(cfg, grid, st, precision, steps, warmup):
-    """Throughput of one form of the step plus event timing of the FP64 column solver."""
-    import dataclasses
+def bench_training(args):
+    """The surrogate training run: harvest columns from the model, train with BF16 MFMA GEMMs."""
+    from lgame import surrogate
 
-    from lgame.gpu import PROF_SOLVE_GEN, PROF_SOLVE_VER, PROF_STEP, Step
-
-    step = Step(dataclasses.replace(cfg, precision=precision), grid)
-    step.start(st)
-    counter = 0
-    for _ in range(warmup):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    step.sync()
-    # timed region: exactly K steps between two synchronizations of the library's stream
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    step.sync()
-    elapsed = time.perf_counter() - t0
-    bad = step.count_nonfinite(counter % 2)
-    # kernel timing with HIP events on the library's stream (separate pass)
-    step.profile(True)
-    nprof = max(2, min(steps, 6))
-    for _ in range(nprof):
-        step.pchevi(counter % 2, 1 - counter % 2, counter)
-        counter += 1
-    prof = {g: step.profile_read(g) for g in (PROF_SOLVE_VER, PROF_SOLVE_GEN, PROF_STEP)}
-    step.profile(False)
-    step.close()
-    return elapsed, bad, prof
+    m, norm, rep = surrogate.build(states=args.train_states, epochs=args.train_epochs, batch=args.train_batch)
+    if args.save_surrogate:
+        surrogate.save(args.save_surrogate, m, norm)
+        log(f"saved {args.save_surrogate}")
+    out = {
+        "metric": "surrogate training samples/s at 1 GPU (BF16 MFMA, FP32 Adam)",
+        "value": rep["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": rep["epochs"], "warmup": 0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "surrogate-train: columns from 33x33x60 moist runs, 304 -> 3x512 GELU -> 62",
+                   "batch": rep["batch"], "columns": rep["train_columns"]},
+        "roofline": {"kernel": "k_gemm (all MLP GEMMs of training)", "bound": "mfma", "achieved": rep["train_tflops"],
+                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": rep["train_tflops"] / BF16_PEAK_TFLOPS,
+                     "traffic": None},
+        "report": rep,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def main():
@@ -149,42 +214,48 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="moist512", choices=["moist512", "dry128", "schaer"])
+    ap.add_argument("--workload", default="moist512", choices=["moist512", "dry128", "schaer", "surrogate-train"])
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (for HBM counter scaling)")
+    ap.add_argument("--train-states", type=int, default=24)
+    ap.add_argument("--train-epochs", type=int, default=60)
+    ap.add_argument("--train-batch", type=int, default=512)
+    ap.add_argument("--save-surrogate", default="")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
         return 2
+    if args.workload == "surrogate-train":
+        return bench_training(args)
 
-    from lgame.cases import make_case
-    from lgame.config import PREC_FP64, PREC_MIXED
-    from lgame.gpu import PROF_SOLVE_VER, PROF_STEP
+    from lgame.cases import make_case, radiation_surface
+    from lgame.gpu import PROF_RADIATION, PROF_SOLVE_VER, PROF_STEP
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     cfg, sample_cfg = workload(args.workload)
+    forms = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.form]
+    mlp = surrogate_mlp() if (cfg.lrad and "mixed" in forms) else None
     t0 = time.perf_counter()
     log(f"setting up {describe(cfg)}")
     grid, st = make_case(cfg)
+    surface = radiation_surface(cfg, grid, st) if cfg.lrad else None
     log(f"set-up done in {time.perf_counter() - t0:.1f} s")
     cells = cfg.gridpoints
-    forms = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.form]
     res = {}
     for form in forms:
-        prec = PREC_MIXED if form == "mixed" else PREC_FP64
-        elapsed, bad, prof = time_form(cfg, grid, st, prec, args.steps, args.warmup)
-        res[form] = (elapsed, bad, prof)
+        elapsed, bad, prof, rad = time_form(cfg, grid, st, surface, form, mlp, args.steps, args.warmup)
+        res[form] = (elapsed, bad, prof, rad)
         log(f"{form}: {args.steps} steps in {elapsed:.3f} s ({1e3 * elapsed / args.steps:.2f} ms/step), "
-            f"non-finite values: {bad}")
+            f"non-finite values: {bad}, radiation {rad}")
     del grid, st
     head = "mixed" if "mixed" in res else "fp64"
-    elapsed, bad, prof = res[head]
+    elapsed, bad, prof, rad = res[head]
     value = cells * args.steps / elapsed
     ms_solver, n_solver = prof[PROF_SOLVE_VER]
     ms_step, n_step = prof[PROF_STEP]
@@ -192,8 +263,16 @@ def main():
     solver_gbs = solver_bytes / (ms_solver * 1e-3) / 1e9 if ms_solver > 0 else 0.0
     avg_launch_us = 1e3 * ms_solver / max(n_solver, 1)
     step_model = STEP_BYTES["moist" if cfg.lmoist else "dry"]
-    form_text = "mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates " \
-                "and budget sums)" if head == "mixed" else "plain FP64 form"
+    form_text = ("mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates and budget "
+                 "sums), radiation by the MLP surrogate with a 1% solver audit" if head == "mixed" else
+                 "plain FP64 form, radiation by the gray two-stream solver")
+    if not cfg.lrad:
+        form_text = form_text.split(", radiation")[0] + ", radiation off"
+    every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
+    rad_us = {}
+    for f, r in res.items():
+        ms_rad, n_rad = r[2][PROF_RADIATION]
+        rad_us[f] = 1e3 * ms_rad / n_rad if n_rad else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -208,13 +287,15 @@ def main():
         "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {describe(cfg)}, {form_text}, radiation off",
+            "workload": f"{args.workload}: {describe(cfg)}, {form_text}",
             "form": head,
             "grid_points": cells,
             "nonfinite_after_run": bad,
+            "radiation_updates_timed": (1 + (args.steps - 1) // every) if every else 0,
         },
         "forms": {f: {"value": cells * args.steps / r[0], "ms_per_step": 1e3 * r[0] / args.steps,
-                      "solver_ver_us": 1e3 * r[2][PROF_SOLVE_VER][0] / max(r[2][PROF_SOLVE_VER][1], 1)}
+                      "solver_ver_us": 1e3 * r[2][PROF_SOLVE_VER][0] / max(r[2][PROF_SOLVE_VER][1], 1),
+                      "radiation_update_us": rad_us[f], "radiation": r[3]}
                   for f, r in res.items()},
         "roofline": {
             "kernel": "k_solve_ver (three_band_solver_ver, both substeps, FP64)",
@@ -239,12 +320,13 @@ def main():
         out["speedup_mixed_vs_fp64"] = res["fp64"][0] / res["mixed"][0]
     if not args.no_cpu:
         log(f"CPU baseline: oracle on {describe(sample_cfg)} for {args.cpu_steps} steps")
-        v, secs, errs = cpu_baseline(sample_cfg, args.cpu_steps, forms)
+        v, secs, errs = cpu_baseline(sample_cfg, args.cpu_steps, forms, mlp)
         out["cpu_baseline"] = {
             "value": v, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle/lgame_oracle.c, {describe(sample_cfg)}, {args.cpu_steps} steps, {secs:.1f} s",
         }
-        out["field_error"] = {"vs": "FP64 oracle on the CPU-baseline sample", "steps": args.cpu_steps, **errs}
+        out["field_error"] = {"vs": "FP64 oracle (gray two-stream radiation) on the CPU-baseline sample",
+                              "steps": args.cpu_steps, **errs}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -104,3 +104,47 @@ def load(path) -> Mlp:
     m.set_params(d["params"])
     m.set_norm(d["x_mean"], d["x_scale"], d["y_mean"], d["y_scale"])
     return m
+
+
+def build(states=24, calls=2, times=6, epochs=60, batch=512, lr=1e-3, seed=0, test_states=3):
+    """Harvest, train and evaluate; returns (mlp, norm, report)."""
+    import time
+
+    t0 = time.perf_counter()
+    xtr, ytr = harvest(n=33, n_states=states, calls=calls, times=times, seed=seed)
+    xte, yte = harvest(n=33, n_states=test_states, calls=1, times=times, seed=seed + 1000)
+    t_harvest = time.perf_counter() - t0
+    norm = fit_norm(xtr, ytr)
+    m, loss, seconds = train(xtr, ytr, epochs=epochs, batch=batch, lr=lr, seed=seed, norm=norm)
+    pred = m.forward(xte)
+    n_params = m.n_params
+    flops = 6.0 * n_params * len(xtr) * epochs  # forward 2, backward 4 flops per parameter and sample
+    report = {
+        "train_columns": int(len(xtr)), "test_columns": int(len(xte)), "epochs": epochs, "batch": batch, "lr": lr,
+        "harvest_s": t_harvest, "train_s": seconds, "samples_per_s": len(xtr) * epochs / seconds,
+        "train_tflops": flops / seconds / 1e12, "params": n_params,
+        "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
+        "heating_rmse_k_per_day": heating_rmse(pred, yte),
+        "baseline_mean_profile_rmse_k_per_day": baseline_rmse(ytr, yte),
+        "no_radiation_rmse_k_per_day": heating_rmse(np.zeros_like(yte), yte),
+        "sw_in_rmse_w_m2": float(np.sqrt(np.mean((pred[:, LEVELS] - yte[:, LEVELS]) ** 2))),
+        "lw_out_rmse_w_m2": float(np.sqrt(np.mean((pred[:, LEVELS + 1] - yte[:, LEVELS + 1]) ** 2))),
+    }
+    return m, norm, report
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train the radiation surrogate on the GPU")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--states", type=int, default=24)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    m, norm, rep = build(states=a.states, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+    save(a.out, m, norm)
+    print(json.dumps(rep), flush=True)
