@@ -10,6 +10,10 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
   dry128          128x128x50 dry, rotated lat-lon, dx 2 km, dt 3.22 s
   schaer          ny=3, nx=201, 40 layers, dx 500 m, dt 0.805 s
   surrogate-train harvest radiation columns from the model and train the 304 -> 3x512 -> 62 MLP
+  ensemble        48 moist members of uneven size (nx, ny ~ U[64,384], 40-70 layers), mixed form,
+                  autotuned block shapes, one graph launch per ensemble step vs serial launches
+Block shapes of the stencil kernels are chosen by lgame_autotune before the timed steps
+(--no-autotune keeps the default 64x4).
 
 Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
 and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
@@ -46,6 +50,7 @@ SOLVER_TRAFFIC = {"moist512": 4.929e9}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
 SURROGATE_WEIGHTS = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
 T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
+NO_AUTOTUNE = False
 
 
 def log(msg):
@@ -116,6 +121,7 @@ def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
 
     step = make_step(cfg, grid, surface, form, mlp)
     step.start(st)
+    tiles = None if NO_AUTOTUNE else step.autotune(2)
     run_steps(step, cfg, 0, warmup)
     step.sync()
     # timed region: exactly K steps between two synchronizations of the library's stream
@@ -132,7 +138,7 @@ def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
     prof = {g: step.profile_read(g) for g in (PROF_SOLVE_VER, PROF_SOLVE_GEN, PROF_STEP, PROF_RADIATION)}
     step.profile(False)
     step.close()
-    return elapsed, bad, prof, rad
+    return elapsed, bad, prof, dict(rad, tiles=tiles)
 
 
 def cpu_baseline(cfg, nsteps, forms, mlp):
@@ -209,12 +215,92 @@ def bench_training(args):
     return 0
 
 
+def ensemble_members(n, seed0=5000):
+    """Member configurations: seeded sizes nx, ny ~ U[64,384] (odd), layers ~ U{40..70}."""
+    from lgame.cases import moist3d_config
+
+    out = []
+    for i in range(n):
+        rng = __import__("numpy").random.default_rng(seed0 + i)
+        nx = int(rng.integers(64, 385)) | 1
+        ny = int(rng.integers(64, 385)) | 1
+        nl = int(rng.integers(40, 71))
+        cfg = moist3d_config(n=nx, n_layers=nl, dtime=3.22, seed=seed0 + i)
+        out.append(dataclasses.replace(cfg, ny=ny, nx=nx))
+    return out
+
+
+def bench_ensemble(args):
+    """Config-5 ensemble: aggregate throughput with one graph launch per step vs serial launches."""
+    from lgame.cases import make_case
+    from lgame.config import PREC_MIXED
+    from lgame.gpu import Ensemble, Step
+
+    cfgs = ensemble_members(args.members)
+    steps, cells = [], []
+    t0 = time.perf_counter()
+    tiles = []
+    for i, c in enumerate(cfgs):
+        c = dataclasses.replace(c, precision=PREC_MIXED)
+        g, st = make_case(c)
+        s = Step(c, g)
+        s.start(st)
+        del g
+        tiles.append(s.autotune(2) if not args.no_autotune else None)
+        s.start(st)
+        del st
+        steps.append(s)
+        cells.append(c.gridpoints)
+        log(f"member {i}: {c.ny}x{c.nx}x{c.n_layers} ({c.gridpoints / 1e6:.2f} M points), tiles {tiles[-1]}")
+    log(f"set-up of {len(steps)} members in {time.perf_counter() - t0:.1f} s")
+    total = sum(cells)
+    e = Ensemble(steps)
+    res = {}
+    for mode, name in ((Ensemble.GRAPH, "graph"), (Ensemble.SERIAL, "serial")):
+        e.run(0, 1 + args.warmup, mode=mode)  # counter 0 runs eagerly in both modes
+        e.sync()
+        t1 = time.perf_counter()
+        e.run(1 + args.warmup, args.steps, mode=mode)
+        e.sync()
+        res[name] = time.perf_counter() - t1
+        log(f"{name}: {args.steps} ensemble steps in {res[name]:.3f} s")
+    # per-member wall time: each member alone, eager launches
+    per = []
+    for s, n in zip(steps, cells):
+        s.sync()
+        t1 = time.perf_counter()
+        s.run(1000, 4)
+        s.sync()
+        per.append(1e3 * (time.perf_counter() - t1) / 4)
+    e.close()
+    for s in steps:
+        s.close()
+    value = total * args.steps / res["graph"]
+    out = {
+        "metric": "aggregate grid-point-steps/s of a batched ensemble at 1 GPU",
+        "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * res["graph"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"ensemble: {len(steps)} moist members, nx,ny ~ U[64,384], 40-70 layers, mixed form, "
+                               f"{total / 1e6:.1f} M points in total, radiation off in the timed steps",
+                   "members": len(steps), "grid_points": total, "autotuned": not args.no_autotune},
+        "serial": {"value": total * args.steps / res["serial"], "ms_per_step": 1e3 * res["serial"] / args.steps},
+        "speedup_graph_vs_serial": res["serial"] / res["graph"],
+        "per_member_ms_per_step": {"mean": sum(per) / len(per), "min": min(per), "max": max(per)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="moist512", choices=["moist512", "dry128", "schaer", "surrogate-train"])
+    ap.add_argument("--workload", default="moist512",
+                    choices=["moist512", "dry128", "schaer", "surrogate-train", "ensemble"])
+    ap.add_argument("--members", type=int, default=48)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
@@ -230,8 +316,12 @@ def main():
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
         return 2
+    global NO_AUTOTUNE
+    NO_AUTOTUNE = args.no_autotune
     if args.workload == "surrogate-train":
         return bench_training(args)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     from lgame.cases import make_case, radiation_surface
     from lgame.gpu import PROF_RADIATION, PROF_SOLVE_VER, PROF_STEP

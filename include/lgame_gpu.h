@@ -300,6 +300,34 @@ int lgame_profile(lgame_handle *h, int enable);
 int lgame_profile_read(lgame_handle *h, int group, double *total_ms, long long *count);
 int lgame_profile_reset(lgame_handle *h);
 
+/* ---- launch tuning ----
+ * The horizontal stencil kernels run on blocks of 64 x-points by 1, 2 or 4 rows.  lgame_autotune
+ * times every candidate for each of the LGAME_N_TILES kernels over `steps` steps of the handle's
+ * own configuration and keeps the fastest; it must run after both slots are set and before the
+ * first step, and restores the slots (and the zeroed work arrays) when it is done.  rows (may be
+ * NULL) receives the choice, in the kernel order of lgame_set_tiles.  No reference counterpart:
+ * the reference is a CPU code with OpenMP loops.
+ * Kernel order: temperature, theta_fill, theta_grad, theta_div, theta_edge, pgrad, pot_vort,
+ * mass_flux, vort_flux, ke, ke_grad, dissipation, vec_tend, scalar_flux, scalar_div. */
+#define LGAME_N_TILES 15
+int lgame_autotune(lgame_handle *h, int steps, int *rows);
+int lgame_set_tiles(lgame_handle *h, const int *rows);
+
+/* ---- ensembles ----
+ * Independent domains (handles, each with its own grid, size and state) advanced together.  A
+ * step of the ensemble is one launch of a graph captured across all members' streams (one branch
+ * per member), so the members run concurrently and the per-kernel launch cost is paid once per
+ * ensemble step; steps at counter 0 and steps with a radiation update run eagerly.  mode 0 = the
+ * graph launch, mode 1 = the members one after another with eager launches (the comparison).
+ * The handles must stay alive and unchanged in configuration while the ensemble exists.  No
+ * reference counterpart (the reference runs one domain per process). */
+typedef struct lgame_ensemble lgame_ensemble;
+int lgame_ensemble_create(lgame_handle *const *members, int n, lgame_ensemble **out);
+int lgame_ensemble_step(lgame_ensemble *e, int total_step_counter, int lrad_update, double t_0, int mode);
+int lgame_ensemble_sync(lgame_ensemble *e);
+const char *lgame_ensemble_last_error(const lgame_ensemble *e);
+void lgame_ensemble_destroy(lgame_ensemble *e);
+
 /* ---- radiation boundary (update_rad_fluxes, src/radiation/mo_manage_radiation_calls.F90:17-59) ---- */
 
 /* Per-column surface and location data; needed before the first step with lrad_update. */

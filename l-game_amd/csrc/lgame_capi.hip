@@ -20,6 +20,9 @@ struct lgame_handle {
   Ctx c;                         // plain form; its grid (FP64) also serves the budget sums
   CtxT<float> cf;                // mixed form: FP32 grid copy and work arrays, FP64 grid in g64
   int mixed = 0;
+  Tiles tiles;                   // block shapes of the stencil kernels (lgame_autotune)
+  size_t state_bytes = 0;        // the two state slots: the first bytes of the slab
+  size_t work_off = 0, work_bytes = 0;  // column scratch and work arrays (zero at the first step)
   StateDev slot[2];
   char *slab = nullptr;          // all fields, one allocation
   size_t slab_bytes = 0;
@@ -251,16 +254,22 @@ namespace lgame {
 
 template <class R>
 void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
-                    const RadCall *rad) {
+                    const RadCall *rad, const Tiles &tiles) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  const dim3 blk(BX, BY, 1);
-  const dim3 g_cell = grid3(nx, ny, nl);
-  const dim3 g_uv = grid3(nx + 1, ny + 1, nl);
-  const dim3 g_all = grid3(nx + 1, ny + 1, nlev);
   const dim3 g_col((nx * ny + BCOL - 1) / BCOL), b_col(BCOL);
   const bool o3 = c.C.order == 3;
+  const bool pk = prof && prof->kernels;
+  // a stencil kernel on a (64 x by) block grid covering (ex, ey, ez) points, by from the tile table
+#define LG_LAUNCH(KI, kern, ex, ey, ez, ...)                                        \
+  do {                                                                              \
+    const int by_ = tiles.by[KI];                                                   \
+    const dim3 b_(BX, by_, 1), g_(((ex) + BX - 1) / BX, ((ey) + by_ - 1) / by_, (ez)); \
+    if (pk) prof->begin(PROF_KERNEL0 + (KI));                                       \
+    hipLaunchKernelGGL(kern<R>, g_, b_, 0, st, __VA_ARGS__);                        \
+    if (pk) prof->end(PROF_KERNEL0 + (KI));                                         \
+  } while (0)
 
-  hipLaunchKernelGGL(k_temperature<R>, g_cell, blk, 0, st, c, old);
+  LG_LAUNCH(KI_TEMPERATURE, k_temperature, nx, ny, nl, c, old);
   // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
   if (rad) {
     if (prof) prof->begin(3);
@@ -272,35 +281,35 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     if (rk == 1) {
       // manage_p_grad (mo_p_grad.F90:20-104); theta_v_adv_3rd_order of theta_v and theta_v_pert
       if (o3) {
-        hipLaunchKernelGGL(k_theta_fill<R>, g_cell, blk, 0, st, c, old, 2);
-        hipLaunchKernelGGL(k_theta_grad<R>, g_uv, blk, 0, st, c, 2);
-        hipLaunchKernelGGL(k_theta_div<R>, g_cell, blk, 0, st, c, 2);
-        hipLaunchKernelGGL(k_theta_edge<R>, g_uv, blk, 0, st, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
+        LG_LAUNCH(KI_THETA_FILL, k_theta_fill, nx, ny, nl, c, old, 2);
+        LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, 2);
+        LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 2);
+        LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
       }
-      hipLaunchKernelGGL(k_pgrad<R>, g_all, blk, 0, st, c, old, step == 0 ? 1 : 0);
+      LG_LAUNCH(KI_PGRAD, k_pgrad, nx + 1, ny + 1, nlev, c, old, step == 0 ? 1 : 0);
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
-      hipLaunchKernelGGL(k_pot_vort<R>, g_all, blk, 0, st, c, S);
-      hipLaunchKernelGGL(k_mass_flux<R>, g_all, blk, 0, st, c, S);
-      hipLaunchKernelGGL(k_vort_flux<R>, g_uv, blk, 0, st, c);
+      LG_LAUNCH(KI_POT_VORT, k_pot_vort, nx + 1, ny + 1, nlev, c, S);
+      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx + 1, ny + 1, nlev, c, S);
+      LG_LAUNCH(KI_VORT_FLUX, k_vort_flux, nx + 1, ny + 1, nl, c);
       if (!c.C.llinear) {
-        hipLaunchKernelGGL(k_ke<R>, g_cell, blk, 0, st, c, S);
-        hipLaunchKernelGGL(k_ke_grad<R>, g_uv, blk, 0, st, c);
+        LG_LAUNCH(KI_KE, k_ke, nx, ny, nl, c, S);
+        LG_LAUNCH(KI_KE_GRAD, k_ke_grad, nx + 1, ny + 1, nl, c);
       }
     }
-    if (rk == 1 && c.C.lklemp) hipLaunchKernelGGL(k_dissipation<R>, g_cell, blk, 0, st, c, old);
-    hipLaunchKernelGGL(k_vec_tend<R>, g_uv, blk, 0, st, c, old, nw, rk);
+    if (rk == 1 && c.C.lklemp) LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
+    LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)
     if (o3 && rk == 2) {
-      hipLaunchKernelGGL(k_theta_fill<R>, g_cell, blk, 0, st, c, nw, 1);
-      hipLaunchKernelGGL(k_theta_grad<R>, g_uv, blk, 0, st, c, 1);
-      hipLaunchKernelGGL(k_theta_div<R>, g_cell, blk, 0, st, c, 1);
-      hipLaunchKernelGGL(k_theta_edge<R>, g_uv, blk, 0, st, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
+      LG_LAUNCH(KI_THETA_FILL, k_theta_fill, nx, ny, nl, c, nw, 1);
+      LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, 1);
+      LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 1);
+      LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
     }
-    hipLaunchKernelGGL(k_scalar_flux<R>, g_uv, blk, 0, st, c, S, nw);
-    hipLaunchKernelGGL(k_scalar_div<R>, g_cell, blk, 0, st, c, S, nw, rk);
+    LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx + 1, ny + 1, nl, c, S, nw);
+    LG_LAUNCH(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
     // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
     if (prof) prof->begin(0);
     hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
@@ -311,12 +320,13 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
       if (prof) prof->end(1);
     }
   }
+#undef LG_LAUNCH
 }
 
 template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
-                                     const RadCall *);
+                                     const RadCall *, const Tiles &);
 template void enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
-                                    const RadCall *);
+                                    const RadCall *, const Tiles &);
 
 }  // namespace lgame
 
@@ -325,9 +335,12 @@ namespace {
 // Records a pair of events around each timed kernel group on the handle's stream.
 struct EventProf : ProfHook {
   lgame_handle *h;
-  int open_idx[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int open_idx[PROF_GROUPS];
   bool failed = false;
-  explicit EventProf(lgame_handle *hh) : h(hh) {}
+  explicit EventProf(lgame_handle *hh, bool per_kernel = false) : h(hh) {
+    kernels = per_kernel;
+    for (int &i : open_idx) i = -1;
+  }
   int next_event() {
     if (h->ev_next == h->ev_pool.size()) {
       hipEvent_t e;
@@ -569,10 +582,12 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     S.wind_v = take8();
     S.wind_w = take8();
   }
+  h->state_bytes = used;
   // grid: NGRID consecutive FP64 fields in a fixed order (the FP32 copy uses the same order)
   double *grid64 = take8();
   for (int q = 1; q < NGRID; ++q) take8();
   Fld<double> col[14];
+  h->work_off = used;
   for (int q = 0; q < (int)ncol; ++q) col[q] = take8();
   auto assign_work = [&](auto &W, auto take) {
     for (int jc = 0; jc < MAX_CONST; ++jc) W.tend_rho[jc] = jc < nc ? take() : nullptr;
@@ -623,6 +638,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     assign_work(h->cf.w, take4);
   else
     assign_work(h->c.w, take8);
+  h->work_bytes = used - h->work_off;
   float *grid32 = nullptr;
   if (mixed) {
     grid32 = take4();
@@ -814,9 +830,9 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
   ProfHook *ph = h->prof ? &ep : nullptr;
   if (ph) ep.begin(LGAME_PROF_STEP);
   if (h->mixed)
-    enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad);
+    enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
   else
-    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad);
+    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
   if (ph) {
     ep.end(LGAME_PROF_STEP);
     if (ep.failed) return fail(h, LGAME_EHIP, "event recording failed");
@@ -894,6 +910,224 @@ int lgame_get_diag(lgame_handle *h, int which, double *out) {
 }  // extern "C"
 
 extern "C" {
+
+int lgame_autotune(lgame_handle *h, int steps, int *rows) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  if (steps < 1 || steps > 64) return fail(h, LGAME_EINVAL, "steps must be in 1..64");
+  if (!h->have_state[0] || !h->have_state[1]) return fail(h, LGAME_ESTATE, "set both state slots first");
+  if (h->stepped) return fail(h, LGAME_ESTATE, "autotune before the first step");
+  // the tuning steps advance the state: keep a copy of both slots
+  char *backup = nullptr;
+  HIPCHK(h, hipMalloc(&backup, h->state_bytes));
+  HIPCHK(h, hipMemcpyAsync(backup, h->slab, h->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+  const int cand[3] = {1, 2, 4};
+  double best_ms[KI_COUNT];
+  int best_by[KI_COUNT];
+  for (int k = 0; k < KI_COUNT; ++k) {
+    best_ms[k] = 1e300;
+    best_by[k] = BY;
+  }
+  int rc = 0;
+  for (int ci = 0; ci < 3 && !rc; ++ci) {
+    for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = cand[ci];
+    h->ev_marks.clear();
+    h->ev_next = 0;
+    EventProf ep(h, true);
+    for (int s = 1; s <= steps + 1; ++s) {  // counters >= 1: the steady-state kernel sequence
+      if (s == 2) {                          // the first step warms up and is not counted
+        rc = hipStreamSynchronize(h->stream) == hipSuccess ? 0 : LGAME_EHIP;
+        h->ev_marks.clear();
+        h->ev_next = 0;
+      }
+      const int a = s % 2, b = 1 - a;
+      if (h->mixed)
+        enqueue_pchevi(h->cf, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
+      else
+        enqueue_pchevi(h->c, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
+    }
+    if (rc || ep.failed || hipStreamSynchronize(h->stream) != hipSuccess) {
+      rc = rc ? rc : LGAME_EHIP;
+      break;
+    }
+    double ms[KI_COUNT] = {0.0};
+    for (auto &m : h->ev_marks) {
+      const int g = m.first - PROF_KERNEL0;
+      if (g < 0 || g >= KI_COUNT) continue;
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, h->ev_pool[m.second.first], h->ev_pool[m.second.second]) == hipSuccess) ms[g] += t;
+    }
+    for (int k = 0; k < KI_COUNT; ++k)
+      if (ms[k] > 0.0 && ms[k] < best_ms[k]) {
+        best_ms[k] = ms[k];
+        best_by[k] = cand[ci];
+      }
+  }
+  h->ev_marks.clear();
+  h->ev_next = 0;
+  for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = best_by[k];
+  // back to the state before tuning: both slots, and zero carried work arrays, as at lgame_init
+  hipError_t e = hipMemcpyAsync(h->slab, backup, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(h->slab + h->work_off, 0, h->work_bytes, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(backup);
+  if (e != hipSuccess) return fail(h, LGAME_EHIP, "autotune restore: %s", hipGetErrorString(e));
+  if (rc) return fail(h, rc, "autotune step failed");
+  if (rows)
+    for (int k = 0; k < KI_COUNT; ++k) rows[k] = best_by[k];
+  return LGAME_OK;
+}
+
+int lgame_set_tiles(lgame_handle *h, const int *rows) {
+  if (!h || !rows) return fail(h, LGAME_EINVAL, "null argument");
+  for (int k = 0; k < KI_COUNT; ++k)
+    if (rows[k] != 1 && rows[k] != 2 && rows[k] != 4) return fail(h, LGAME_EINVAL, "rows per block must be 1, 2 or 4");
+  for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = rows[k];
+  return LGAME_OK;
+}
+
+// ---- ensembles -----------------------------------------------------------------------------
+}  // extern "C"
+
+struct lgame_ensemble {
+  std::vector<lgame_handle *> m;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // one graph per ping-pong parity
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  std::string err;
+};
+
+namespace {
+
+int efail(lgame_ensemble *e, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  return code;
+}
+
+#define ECHK(e, call)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return efail(e, LGAME_EHIP, "%s: %s", #call, hipGetErrorString(e_));  \
+  } while (0)
+
+void enqueue_member(lgame_handle *h, int a, int b, int step, const RadCall *rad) {
+  if (h->mixed)
+    enqueue_pchevi(h->cf, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, h->tiles);
+  else
+    enqueue_pchevi(h->c, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, h->tiles);
+}
+
+// all members' steps of one parity as parallel branches of one graph (fork/join over the
+// members' own streams)
+int capture(lgame_ensemble *e, int parity) {
+  const int a = parity, b = 1 - parity;
+  ECHK(e, hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+  ECHK(e, hipEventRecord(e->fork, e->master));
+  for (size_t q = 0; q < e->m.size(); ++q) {
+    lgame_handle *h = e->m[q];
+    ECHK(e, hipStreamWaitEvent(h->stream, e->fork, 0));
+    enqueue_member(h, a, b, 1 + parity, nullptr);  // any counter >= 1 of this parity: the steady sequence
+    ECHK(e, hipEventRecord(e->join[q], h->stream));
+    ECHK(e, hipStreamWaitEvent(e->master, e->join[q], 0));
+  }
+  ECHK(e, hipStreamEndCapture(e->master, &e->graph[parity]));
+  ECHK(e, hipGraphInstantiate(&e->exec[parity], e->graph[parity], nullptr, nullptr, 0));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lgame_ensemble_create(lgame_handle *const *members, int n, lgame_ensemble **out) {
+  if (!members || n < 1 || !out) return LGAME_EINVAL;
+  *out = nullptr;
+  lgame_ensemble *e = new lgame_ensemble();
+  for (int q = 0; q < n; ++q) {
+    if (!members[q]) {
+      delete e;
+      return LGAME_EINVAL;
+    }
+    e->m.push_back(members[q]);
+  }
+  bool ok = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+  e->join.resize(n, nullptr);
+  for (int q = 0; q < n && ok; ++q) ok = hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    lgame_ensemble_destroy(e);
+    return LGAME_EHIP;
+  }
+  *out = e;
+  return LGAME_OK;
+}
+
+// One step of every member (the members' own ping-pong, counter total_step_counter).  Reading a
+// member's state or budget sums needs lgame_ensemble_sync first (graph work runs on the
+// ensemble's own stream).  Steps at
+// counter 0 and steps with a radiation update run eagerly on the members' streams (the radiation
+// surrogate's buffers are shared, so those members run one after another); all other steps
+// replay one captured graph for the whole ensemble.  mode 1 runs every member eagerly and
+// serially instead (the comparison without the batched launch).
+int lgame_ensemble_step(lgame_ensemble *e, int total_step_counter, int lrad_update, double t_0, int mode) {
+  if (!e) return LGAME_EINVAL;
+  const int a = total_step_counter % 2, b = 1 - a;
+  for (auto *h : e->m)
+    if (!h->have_state[0] || !h->have_state[1]) return efail(e, LGAME_ESTATE, "every member needs both state slots");
+  if (mode == 0 && total_step_counter > 0 && !lrad_update) {
+    if (!e->exec[a]) {
+      int rc = capture(e, a);
+      if (rc) return rc;
+    }
+    // the graph runs on the master stream after everything already queued on the members' streams
+    for (size_t q = 0; q < e->m.size(); ++q) {
+      ECHK(e, hipEventRecord(e->join[q], e->m[q]->stream));
+      ECHK(e, hipStreamWaitEvent(e->master, e->join[q], 0));
+    }
+    ECHK(e, hipGraphLaunch(e->exec[a], e->master));
+    for (auto *h : e->m) h->stepped = 1;
+    return LGAME_OK;
+  }
+  // eager steps on the members' streams start after the graph launches queued on the master
+  ECHK(e, hipEventRecord(e->fork, e->master));
+  for (auto *h : e->m) ECHK(e, hipStreamWaitEvent(h->stream, e->fork, 0));
+  const bool serial = mode == 1 || lrad_update;
+  for (auto *h : e->m) {
+    int rc = lgame_pchevi(h, a, b, total_step_counter, lrad_update, t_0);
+    if (rc) return efail(e, rc, "member step: %s", h->err.c_str());
+    if (serial) ECHK(e, hipStreamSynchronize(h->stream));
+  }
+  return LGAME_OK;
+}
+
+int lgame_ensemble_sync(lgame_ensemble *e) {
+  if (!e) return LGAME_EINVAL;
+  ECHK(e, hipStreamSynchronize(e->master));
+  for (auto *h : e->m) ECHK(e, hipStreamSynchronize(h->stream));
+  return LGAME_OK;
+}
+
+const char *lgame_ensemble_last_error(const lgame_ensemble *e) { return e ? e->err.c_str() : ""; }
+
+void lgame_ensemble_destroy(lgame_ensemble *e) {
+  if (!e) return;
+  if (e->master) (void)hipStreamSynchronize(e->master);
+  for (int p = 0; p < 2; ++p) {
+    if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+    if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+  }
+  for (auto ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->master) (void)hipStreamDestroy(e->master);
+  delete e;
+}
 
 int lgame_rad_set_surface(lgame_handle *h, const lgame_rad_view *v) {
   if (!h || !v || !v->lat_geo || !v->lon_geo || !v->sfc_albedo || !v->temp_sfc) return fail(h, LGAME_EINVAL, "null argument");

@@ -24,14 +24,33 @@ template <class R> __global__ void k_scalar_div(const CtxT<R> c, const StateDev 
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 
-// Launch geometry of the horizontal stencil kernels: 64 x-points by 4 rows per block.
+// Launch geometry of the horizontal stencil kernels: 64 x-points by BY rows per block (the
+// default; the tile table can choose 1, 2 or 4 rows per kernel).
 constexpr int BX = 64;
 constexpr int BY = 4;
+
+// stencil kernels of the step, the index of their tile-table entry
+enum KernelId {
+  KI_TEMPERATURE, KI_THETA_FILL, KI_THETA_GRAD, KI_THETA_DIV, KI_THETA_EDGE, KI_PGRAD, KI_POT_VORT, KI_MASS_FLUX,
+  KI_VORT_FLUX, KI_KE, KI_KE_GRAD, KI_DISSIPATION, KI_VEC_TEND, KI_SCALAR_FLUX, KI_SCALAR_DIV, KI_COUNT
+};
+
+// rows per block of every stencil kernel (block = 64 x by threads)
+struct Tiles {
+  int by[KI_COUNT];
+  Tiles() {
+    for (int &b : by) b = BY;
+  }
+};
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
 
-// Optional timing hook around kernel groups (0 = vertical solver, 1 = tracer solver).
+// Optional timing hook around kernel groups (0 = vertical solver, 1 = tracer solver, 2 = step,
+// 3 = radiation; PROF_KERNEL0 + KernelId for every stencil kernel when `kernels` is set).
+constexpr int PROF_KERNEL0 = 8;
+constexpr int PROF_GROUPS = PROF_KERNEL0 + KI_COUNT;
 struct ProfHook {
+  bool kernels = false;
   virtual void begin(int group) = 0;
   virtual void end(int group) = 0;
   virtual ~ProfHook() = default;
@@ -41,6 +60,6 @@ struct ProfHook {
 // `rad` is non-null when the step updates the radiation (lrad_update).
 template <class R>
 void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int total_step_counter, hipStream_t stream,
-                    ProfHook *prof, const RadCall *rad);
+                    ProfHook *prof, const RadCall *rad, const Tiles &tiles);
 
 }  // namespace lgame

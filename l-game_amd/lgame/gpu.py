@@ -23,7 +23,13 @@ EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_sta
             "lgame_rad_set_surface", "lgame_rad_columns", "lgame_rad_get", "lgame_rad_attach_mlp", "lgame_rad_stats",
             "lgame_mlp_create", "lgame_mlp_n_params", "lgame_mlp_set_params", "lgame_mlp_get_params",
             "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_gradient", "lgame_mlp_load_data",
-            "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy")
+            "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy", "lgame_autotune", "lgame_set_tiles",
+            "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
+            "lgame_ensemble_destroy")
+
+N_TILES = 15
+TILE_KERNELS = ("temperature", "theta_fill", "theta_grad", "theta_div", "theta_edge", "pgrad", "pot_vort",
+                "mass_flux", "vort_flux", "ke", "ke_grad", "dissipation", "vec_tend", "scalar_flux", "scalar_div")
 
 PROF_SOLVE_VER = 0
 PROF_SOLVE_GEN = 1
@@ -103,6 +109,15 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_mlp_last_error.restype = ctypes.c_char_p
     lib.lgame_mlp_destroy.argtypes = [vp]
     lib.lgame_mlp_destroy.restype = None
+    lib.lgame_autotune.argtypes = [vp, ci, vp]
+    lib.lgame_set_tiles.argtypes = [vp, vp]
+    lib.lgame_ensemble_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
+    lib.lgame_ensemble_step.argtypes = [vp, ci, ci, cd, ci]
+    lib.lgame_ensemble_sync.argtypes = [vp]
+    lib.lgame_ensemble_last_error.argtypes = [vp]
+    lib.lgame_ensemble_last_error.restype = ctypes.c_char_p
+    lib.lgame_ensemble_destroy.argtypes = [vp]
+    lib.lgame_ensemble_destroy.restype = None
     _lib = lib
     return lib
 
@@ -184,6 +199,16 @@ class Step:
                 self.pchevi(0, 1, s, lrad, t0)
             else:
                 self.pchevi(1, 0, s, lrad, t0)
+
+    def autotune(self, steps: int = 2) -> dict:
+        """Pick the block shape of every stencil kernel by timing (before the first step)."""
+        rows = np.zeros(N_TILES, dtype=np.int32)
+        self._check(self.lib.lgame_autotune(self.h, steps, rows.ctypes.data), "lgame_autotune")
+        return dict(zip(TILE_KERNELS, rows.tolist()))
+
+    def set_tiles(self, rows):
+        r = np.ascontiguousarray([rows[k] for k in TILE_KERNELS] if isinstance(rows, dict) else rows, dtype=np.int32)
+        self._check(self.lib.lgame_set_tiles(self.h, r.ctypes.data), "lgame_set_tiles")
 
     # ---- radiation ----
     def set_rad_surface(self, surface: dict):
@@ -320,3 +345,44 @@ class Mlp:
         self._check(self.lib.lgame_mlp_train(self.m, epochs, batch, lr, seed, loss.ctypes.data, ctypes.byref(sec)),
                     "lgame_mlp_train")
         return loss, sec.value
+
+
+class Ensemble:
+    """Independent domains advanced together, one graph launch per ensemble step."""
+
+    GRAPH, SERIAL = 0, 1
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
+        e = ctypes.c_void_p()
+        rc = self.lib.lgame_ensemble_create(arr, len(self.members), ctypes.byref(e))
+        if rc != 0:
+            raise LgameError(f"lgame_ensemble_create failed ({rc})")
+        self.e = e
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LgameError(f"{what} failed ({rc}): {self.lib.lgame_ensemble_last_error(self.e).decode()}")
+
+    def step(self, counter: int, lrad_update: bool = False, t_0: float = 0.0, mode: int = 0):
+        self._check(self.lib.lgame_ensemble_step(self.e, counter, int(lrad_update), t_0, mode), "lgame_ensemble_step")
+
+    def run(self, first: int, nsteps: int, mode: int = 0):
+        for k in range(first, first + nsteps):
+            self.step(k, mode=mode)
+
+    def sync(self):
+        self._check(self.lib.lgame_ensemble_sync(self.e), "lgame_ensemble_sync")
+
+    def close(self):
+        if getattr(self, "e", None):
+            self.lib.lgame_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

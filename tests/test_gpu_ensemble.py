@@ -1,0 +1,61 @@
+"""Launch tuning and ensembles: the autotuner's block shapes and the graph-captured ensemble step
+must not change any result (every member stays bitwise equal to its own oracle run)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from lgame.cases import dry3d_config, make_case, tiny_config
+from lgame.gpu import TILE_KERNELS, Ensemble, Step
+from pyoracle import CpuOracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(cfg, g, st, n):
+    o = CpuOracle(cfg, g)
+    o.start(st)
+    o.run(0, n)
+    return o.get_state(n % 2)
+
+
+def test_autotune_keeps_results():
+    cfg = tiny_config(ny=9, nx=11, n_layers=7, moist=True, order=3, freeslip=True)
+    g, st = make_case(cfg, cloud_frac=0.2)
+    s = Step(cfg, g)
+    s.start(st)
+    rows = s.autotune(2)
+    print(f"tiles {rows}")
+    assert set(rows) == set(TILE_KERNELS) and all(r in (1, 2, 4) for r in rows.values())
+    s.run(0, 5)
+    got = s.get_state(1)
+    s.close()
+    want = _oracle(cfg, g, st, 5)
+    assert all(np.array_equal(got[k], want[k]) for k in want)
+
+
+@pytest.mark.parametrize("mode", [Ensemble.GRAPH, Ensemble.SERIAL])
+def test_ensemble_members_match_oracle(mode):
+    cfgs = [tiny_config(ny=5, nx=7, n_layers=6, moist=True, order=3),
+            tiny_config(ny=9, nx=11, n_layers=8),
+            dry3d_config(n=17, n_layers=10)]
+    cases = [make_case(c, cloud_frac=0.1 if c.lmoist else 0.0) for c in cfgs]
+    steps = []
+    for c, (g, st) in zip(cfgs, cases):
+        s = Step(c, g)
+        s.start(st)
+        if mode == Ensemble.GRAPH:
+            s.set_tiles({k: (1, 2, 4)[i % 3] for i, k in enumerate(TILE_KERNELS)})
+        steps.append(s)
+    e = Ensemble(steps)
+    n = 7
+    print(f"ensemble of {len(steps)} members, mode {mode}, {n} steps", flush=True)
+    e.run(0, n, mode=mode)
+    e.sync()
+    for c, (g, st), s in zip(cfgs, cases, steps):
+        got = s.get_state(n % 2)
+        want = _oracle(c, g, st, n)
+        assert all(np.array_equal(got[k], want[k]) for k in want), f"member {c.ny}x{c.nx}x{c.n_layers}"
+    e.close()
+    for s in steps:
+        s.close()
