@@ -316,7 +316,8 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     if (prof) prof->end(0);
     if (c.G.nc > 1) {
       if (prof) prof->begin(1);
-      hipLaunchKernelGGL(k_solve_gen<R>, g_col, dim3(BCOL, 6), 0, st, c, old, nw, rk);
+      const dim3 g_gen((nx * ny + BCOL * GEN_GROUPS - 1) / (BCOL * GEN_GROUPS));
+      hipLaunchKernelGGL(k_solve_gen<R>, g_gen, dim3(BCOL, NTR * GEN_GROUPS), 0, st, c, old, nw, rk);
       if (prof) prof->end(1);
     }
   }

@@ -107,6 +107,7 @@ __device__ __forceinline__ Iface make_iface(const Layer &a, const Layer &b, cons
 struct BackIn {
   double V, rhoe, rte, ga, rio, ep_o, rt_o, tb;  // layer Lr
   double rp_u, ep_u, tin_u;                      // row Lr-1 of the system (interface Lr)
+  double az_b, wo_b;                             // interface Lr+1 (below the layer)
 };
 
 // three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off)
@@ -208,6 +209,12 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
       b.ep_u = sep[iu];
       b.tin_u = stin[iu];
     }
+    b.az_b = b.wo_b = 0.0;
+    if (Lr < nl) {
+      const ix_t ib = LG_IX(c.G, ji, jk, Lr + 1);
+      b.az_b = c.g64.area_z[ib];
+      b.wo_b = old.wind_w[ib];
+    }
     return b;
   };
   double x_low = 0.0;  // x(Lr) of the row below the current layer
@@ -217,12 +224,7 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
   for (int Lr = nl; Lr >= 1; --Lr) {
     const BackIn b = bn;
     if (Lr >= 2) bn = load_back(Lr - 1);
-    double az_b = 0.0, wo_b = 0.0;
-    if (Lr < nl) {
-      const ix_t ib = LG_IX(c.G, ji, jk, Lr + 1);
-      az_b = c.g64.area_z[ib];
-      wo_b = old.wind_w[ib];
-    }
+    const double az_b = b.az_b, wo_b = b.wo_b;
     const ix_t i = LG_IX(c.G, ji, jk, Lr);
     const double V = b.V;
     double rho_new, rt_new;
@@ -315,21 +317,24 @@ __device__ __forceinline__ double v_fall_solid(double radius) { return (radius >
 // 0.001 m/s, cloud water v_fall_liquid of 12e-6 m; rain and graupel use the radius diag%a_rain,
 // which the reference keeps at zero, so their v_fall_liquid is exactly zero
 // (mo_dictionary.F90:475-527, control.F90:443).
-constexpr int NTR = 6;    // tracers per moist column
 constexpr int NCS = 5;    // condensates among them (constituents 1..5)
 constexpr int GEN_K = 8;  // levels per exchange of sediment-heating terms
+constexpr int BACK_K = 8;  // levels per chunk of back-substitution loads
 
 template <class R>
-__global__ void __launch_bounds__(BCOL * NTR) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+__global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
   // sediment-heating terms of the condensates: csh(jl) = ((0 + a_1) - b_1) + a_2 - b_2 ...,
   // b only at the lowest layer (the surface sink term)
-  __shared__ double sh_a[GEN_K][NCS][BCOL];
-  __shared__ double sh_b[NCS][BCOL];
+  __shared__ double sh_a_all[GEN_GROUPS][GEN_K][NCS][BCOL];
+  __shared__ double sh_b_all[GEN_GROUPS][NCS][BCOL];
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond;
   // blockDim.x is the wavefront width, so the tracer index is uniform over a wavefront; telling
   // the compiler so keeps the tracer selection in scalar registers and scalar branches
-  const int lane = threadIdx.x, t = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int col_raw = blockIdx.x * BCOL + lane;
+  const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int t = wy % NTR, grp = wy / NTR;
+  double(*sh_a)[NCS][BCOL] = sh_a_all[grp];
+  double(*sh_b)[BCOL] = sh_b_all[grp];
+  const int col_raw = (blockIdx.x * GEN_GROUPS + grp) * BCOL + lane;
   const bool active = col_raw < nx * ny;
   const int col = active ? col_raw : nx * ny - 1;  // idle lanes mirror a valid column, store nothing
   const int jk = col % nx + 1, ji = col / nx + 1;
@@ -371,12 +376,8 @@ __global__ void __launch_bounds__(BCOL * NTR) k_solve_gen(const CtxT<R> c, const
   double V = c.g64.volume[LG_IX(c.G, ji, jk, 1)];
   const double az_sfc = AT(c.g64.area_z, ji, jk, nl + 1);
   const double t_sfc = (heat && cond) ? (double)AT(c.w.temperature, ji, jk, nl) : 0.0;
-  GenIn nxt = load(1);
-  for (int j0 = 1; j0 <= nl; j0 += GEN_K) {
-    const int j1 = j0 + GEN_K - 1 < nl ? j0 + GEN_K - 1 : nl;
-    for (int jl = j0; jl <= j1; ++jl) {
-      const GenIn cur = nxt;
-      if (jl < nl) nxt = load(jl + 1);
+  // one level of the forward sweep; cur holds the inputs of layer jl
+  auto level = [&](const GenIn &cur, int jl, int j0) {
       const ix_t i = LG_IX(c.G, ji, jk, jl);
       const bool below = jl < nl;
       double fi = 0.0, fr = 0.0, fe = 0.0;
@@ -470,6 +471,21 @@ __global__ void __launch_bounds__(BCOL * NTR) k_solve_gen(const CtxT<R> c, const
       fe_up = fe;
       ro_cur = cur.ro_b;
       V = cur.Vb;
+  };
+  // Two input buffers, used by odd and even levels: a buffer is refilled with the inputs two
+  // levels down as soon as its level is done, so the loads of two levels are in flight.
+  GenIn in_a = load(1), in_b = in_a;
+  if (nl >= 2) in_b = load(2);
+  for (int j0 = 1; j0 <= nl; j0 += GEN_K) {
+    const int j1 = j0 + GEN_K - 1 < nl ? j0 + GEN_K - 1 : nl;
+    for (int jl = j0; jl <= j1; ++jl) {
+      if (jl & 1) {
+        level(in_a, jl, j0);
+        if (jl + 2 <= nl) in_a = load(jl + 2);
+      } else {
+        level(in_b, jl, j0);
+        if (jl + 2 <= nl) in_b = load(jl + 2);
+      }
     }
     if (heat) {  // uniform over the launch: every wave reaches both barriers
       __syncthreads();
@@ -488,19 +504,39 @@ __global__ void __launch_bounds__(BCOL * NTR) k_solve_gen(const CtxT<R> c, const
     }
   }
   if (!active) return;
-  // back substitution and clipping of round-off negatives (mo_column_solvers.F90:584-596)
+  // back substitution and clipping of round-off negatives (mo_column_solvers.F90:584-596).
+  // The scratch rows come in chunks of BACK_K levels into two register buffers: one chunk's
+  // 2*BACK_K loads are in flight while the other chunk is substituted.
   double x = 0.0;
   const Fld<double> rn = nw.rho[jc - 1];
-  double p_rp = srp[LG_IX(c.G, ji, jk, nl)], p_ep = 0.0;
-  for (int jl = nl; jl >= 1; --jl) {
-    const double rpl = p_rp, epl = p_ep;
-    if (jl > 1) {
-      const ix_t iu = LG_IX(c.G, ji, jk, jl - 1);
-      p_rp = srp[iu];
-      p_ep = sep[iu];
+  double ar[BACK_K], ae[BACK_K], br[BACK_K], be[BACK_K];
+  auto fetch = [&](int top, double *pr, double *pe) {
+#pragma unroll
+    for (int q = 0; q < BACK_K; ++q) {
+      const int jl = top - q;
+      if (jl >= 1) {
+        const ix_t i = LG_IX(c.G, ji, jk, jl);
+        pr[q] = srp[i];
+        pe[q] = sep[i];
+      }
     }
-    x = (jl == nl) ? rpl : rpl - epl * x;
-    rn[LG_IX(c.G, ji, jk, jl)] = x < 0.0 ? 0.0 : x;
+  };
+  auto subst = [&](int top, const double *pr, const double *pe) {
+#pragma unroll
+    for (int q = 0; q < BACK_K; ++q) {
+      const int jl = top - q;
+      if (jl >= 1) {
+        x = (jl == nl) ? pr[q] : pr[q] - pe[q] * x;
+        rn[LG_IX(c.G, ji, jk, jl)] = x < 0.0 ? 0.0 : x;
+      }
+    }
+  };
+  fetch(nl, ar, ae);
+  for (int top = nl; top >= 1; top -= 2 * BACK_K) {
+    fetch(top - BACK_K, br, be);
+    subst(top, ar, ae);
+    fetch(top - 2 * BACK_K, ar, ae);
+    subst(top - BACK_K, br, be);
   }
 }
 

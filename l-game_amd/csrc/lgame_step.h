@@ -44,6 +44,11 @@ struct Tiles {
 };
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
+// The tracer solver: one wavefront per tracer of a 64-column group, two groups per block. Twelve
+// wavefronts per block land three on each SIMD, which is what the kernel's registers allow; a
+// block of six would land 2,2,1,1 and leave room for only one block per CU.
+constexpr int NTR = 6;         // tracers per moist column
+constexpr int GEN_GROUPS = 2;  // 64-column groups per block
 
 // Optional timing hook around kernel groups (0 = vertical solver, 1 = tracer solver, 2 = step,
 // 3 = radiation; PROF_KERNEL0 + KernelId for every stencil kernel when `kernels` is set).

@@ -501,101 +501,170 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_flux(const CtxT<R> c, const 
   }
 }
 
-// Tracer fluxes of div_h_tracers (mo_divergence_operators.F90:70-147) with upstream face values
-// (scalar_times_vector_h_upstream, mo_multiplications.F90:206-287).  The wind-dependent parts
-// (face winds, upwind direction, contravariant corrections) are shared by all tracers of a cell.
+// The scalar divergences below load every metric and face value once per cell and share them
+// between the gas, theta_v and tracer fluxes: the vector memory path, not arithmetic, bounds
+// these stencils, so each repeated load costs time.  The expressions are those of div_h,
+// vertical_contravariant_corr and div_h_tracers, evaluated in the same order.
+
+// slopes at the four faces of cell (ji, jk) and its inner-product weights 0..3, layer l
 template <class R>
-struct TracerFaces {
-  R ue, uw, vn, vs;   // face winds: east (jk+1), west (jk), v at ji, v at ji+1
-  ix_t re, rw, rn, rs;   // upwind cell of each face
-  R cu, cl;           // contravariant corrections at the upper (jl) and lower (jl+1) interface
+struct VccGeo {
+  R sx1, sy0, sx0, sy1, w0, w1, w2, w3;
 };
-
 template <class R>
-__device__ __forceinline__ TracerFaces<R> tracer_faces(const CtxT<R> &c, const StateDev &nw, int ji, int jk, int jl) {
-  const int nx = c.G.nx, ny = c.G.ny, nflat = c.G.nflat, nl = c.G.nl;
-  TracerFaces<R> f;
-  // face jk+1 (the copy of face 1 when jk = nx)
-  {
-    const int k = (jk + 1 == nx + 1) ? 1 : jk + 1;
-    f.ue = ATR(nw.wind_u, ji, k, jl);
-    f.re = f.ue >= R(0.0) ? LG_IX(c.G, ji, wrapm(k, nx), jl) : LG_IX(c.G, ji, k, jl);
-  }
-  {
-    f.uw = ATR(nw.wind_u, ji, jk, jl);
-    f.rw = f.uw >= R(0.0) ? LG_IX(c.G, ji, wrapm(jk, nx), jl) : LG_IX(c.G, ji, jk, jl);
-  }
-  {
-    f.vn = ATR(nw.wind_v, ji, jk, jl);
-    f.rn = f.vn >= R(0.0) ? LG_IX(c.G, ji, jk, jl) : LG_IX(c.G, wrapm(ji, ny), jk, jl);
-  }
-  {
-    const int j = (ji + 1 == ny + 1) ? 1 : ji + 1;
-    f.vs = ATR(nw.wind_v, j, jk, jl);
-    f.rs = f.vs >= R(0.0) ? LG_IX(c.G, j, jk, jl) : LG_IX(c.G, wrapm(j, ny), jk, jl);
-  }
-  f.cu = R(0.0);
-  f.cl = R(0.0);
-  if (jl == nflat) {
-    f.cl = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl + 1);
-  } else if (jl == nl) {
-    f.cu = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl);
-  } else if (jl > nflat) {
-    f.cu = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl);
-    f.cl = vcc(c, nw.wind_u, nw.wind_v, ji, jk, jl + 1);
-  }
-  return f;
+__device__ __forceinline__ VccGeo<R> vcc_geo(const CtxT<R> &c, int ji, int jk, int l) {
+  return VccGeo<R>{AT(c.g.slope_x, ji, jk + 1, l), AT(c.g.slope_y, ji, jk, l),    AT(c.g.slope_x, ji, jk, l),
+                   AT(c.g.slope_y, ji + 1, jk, l), AT(c.g.ipw[0], ji, jk, l),     AT(c.g.ipw[1], ji, jk, l),
+                   AT(c.g.ipw[2], ji, jk, l),      AT(c.g.ipw[3], ji, jk, l)};
 }
 
-// div_h_tracers of one tracer field r, mo_divergence_operators.F90:94-145
+// a vector field at the four faces of cell (ji, jk): x at jk+1, y at ji, x at jk, y at ji+1
 template <class R>
-__device__ __forceinline__ R div_tracer(const CtxT<R> &c, const TracerFaces<R> &f, Fld<const double> r, int ji, int jk, int jl) {
-  const R comp_h = (R(r[f.re]) * f.ue) * AT(c.g.area_x, ji, jk + 1, jl) + (R(r[f.rn]) * f.vn) * AT(c.g.area_y, ji, jk, jl) -
-                        (R(r[f.rw]) * f.uw) * AT(c.g.area_x, ji, jk, jl) - (R(r[f.rs]) * f.vs) * AT(c.g.area_y, ji + 1, jk, jl);
-  R comp_v = R(0.0);
-  const int nflat = c.G.nflat, nl = c.G.nl;
-  if (jl == nflat) {
-    const R dl = f.cl <= R(0.0) ? ATR(r, ji, jk, jl) : ATR(r, ji, jk, jl + 1);
-    comp_v = -dl * f.cl * AT(c.g.area_z, ji, jk, jl + 1);
-  } else if (jl == nl) {
-    const R du = f.cu <= R(0.0) ? ATR(r, ji, jk, jl - 1) : ATR(r, ji, jk, jl);
-    comp_v = du * f.cu * AT(c.g.area_z, ji, jk, jl);
-  } else if (jl > nflat) {
-    const R du = f.cu <= R(0.0) ? ATR(r, ji, jk, jl - 1) : ATR(r, ji, jk, jl);
-    const R dl = f.cl <= R(0.0) ? ATR(r, ji, jk, jl) : ATR(r, ji, jk, jl + 1);
-    comp_v = du * f.cu * AT(c.g.area_z, ji, jk, jl) - dl * f.cl * AT(c.g.area_z, ji, jk, jl + 1);
-  }
-  return (comp_h + comp_v) / AT(c.g.volume, ji, jk, jl);
+struct Faces {
+  R e, n, w, s;
+};
+template <class R, class P>
+__device__ __forceinline__ Faces<R> faces(P vx, P vy, const Geom &G, int ji, int jk, int l) {
+  return Faces<R>{R(vx[LG_IX(G, ji, jk + 1, l)]), R(vy[LG_IX(G, ji, jk, l)]), R(vx[LG_IX(G, ji, jk, l)]),
+                  R(vy[LG_IX(G, ji + 1, jk, l)])};
 }
 
-// divergences -> tend%rho, tend%rhotheta_v; plus w_free_slip (mo_inner_product.F90:52-75) of
-// the new state, which only depends on the already final u, v of this substep
+// r minus the four terms of one layer of vertical_contravariant_corr (mo_averaging.F90:29-50)
+template <class R>
+__device__ __forceinline__ R vcc_sub(R r, const VccGeo<R> &g, const Faces<R> &f) {
+  return r - R(0.5) * f.e * g.sx1 * g.w0 - R(0.5) * f.n * g.sy0 * g.w1 - R(0.5) * f.w * g.sx0 * g.w2 - R(0.5) * f.s * g.sy1 * g.w3;
+}
+
+// divergences -> tend%rho, tend%rhotheta_v (scalar_tend_expl, mo_scalar_tend_expl.F90:24-181,
+// with div_h, mo_divergence_operators.F90:16-68, and div_h_tracers with upstream face values,
+// :70-147 and scalar_times_vector_h_upstream, mo_multiplications.F90:206-287); plus
+// w_free_slip (mo_inner_product.F90:52-75) of the new state, which only depends on the already
+// final u, v of this substep.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk) {
   KIDX;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond;
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond, nflat = c.G.nflat, nc = c.G.nc;
   if (jk > nx || ji > ny || jl > nl) return;
-  const ix_t i = LG_IX(c.G, ji, jk, jl);
-  // gas (new weight 1 at both substeps)
-  c.w.tend_rho[ncond][i] = -div_h(c, c.w.fu, c.w.fv, ji, jk, jl);
+  const Geom &G = c.G;
+  const ix_t i = LG_IX(G, ji, jk, jl);
+  const bool tracers = nc > 1;
+  const Fld<const double> wu = nw.wind_u, wv = nw.wind_v;
+  // which interfaces carry a contravariant correction, and which layers those need
+  // (vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1)
+  const bool up = jl > nflat;                  // interface jl (jl = nl included)
+  const bool lo = jl >= nflat && jl < nl;      // interface jl+1
+  const bool lay_m = up && jl != nflat + 1;    // layer jl-1
+  const bool lay_c = up || (lo && jl != nflat);  // layer jl
+  const bool lay_p = lo;                       // layer jl+1
+  VccGeo<R> gm{}, gc{}, gp{};
+  Faces<R> fm_g{}, fc_g{}, fp_g{}, fm_h{}, fc_h{}, fp_h{}, fm_w{}, fc_w{}, fp_w{};
+  if (lay_m) {
+    gm = vcc_geo(c, ji, jk, jl - 1);
+    fm_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl - 1);
+    fm_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl - 1);
+    if (tracers) fm_w = faces<R>(wu, wv, G, ji, jk, jl - 1);
+  }
+  if (lay_c) gc = vcc_geo(c, ji, jk, jl);
+  // the flux faces of layer jl are needed for the horizontal divergence anyway
+  fc_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl);
+  fc_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl);
+  if (tracers) fc_w = faces<R>(wu, wv, G, ji, jk, jl);
+  if (lay_p) {
+    gp = vcc_geo(c, ji, jk, jl + 1);
+    fp_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl + 1);
+    fp_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl + 1);
+    if (tracers) fp_w = faces<R>(wu, wv, G, ji, jk, jl + 1);
+  }
+  const R ax1 = AT(c.g.area_x, ji, jk + 1, jl), ay0 = AT(c.g.area_y, ji, jk, jl);
+  const R ax0 = AT(c.g.area_x, ji, jk, jl), ay1 = AT(c.g.area_y, ji + 1, jk, jl);
+  const R az_u = up ? AT(c.g.area_z, ji, jk, jl) : R(0.0);
+  const R az_l = lo ? AT(c.g.area_z, ji, jk, jl + 1) : R(0.0);
+  const R V = AT(c.g.volume, ji, jk, jl);
+  // vcc at the upper and lower interface for a flux whose faces are fm, fc, fp
+  auto vcc_u = [&](const Faces<R> &fm, const Faces<R> &fc) {
+    R r = R(0.0);
+    if (lay_m) r = vcc_sub(r, gm, fm);
+    return vcc_sub(r, gc, fc);
+  };
+  auto vcc_l = [&](const Faces<R> &fc, const Faces<R> &fp) {
+    R r = R(0.0);
+    if (jl != nflat) r = vcc_sub(r, gc, fc);
+    return vcc_sub(r, gp, fp);
+  };
+  // div_h with the corrections of this cell
+  auto div = [&](const Faces<R> &fm, const Faces<R> &fc, const Faces<R> &fp) {
+    const R comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
+    R comp_v = R(0.0);
+    if (jl == nflat) {
+      comp_v = -vcc_l(fc, fp) * az_l;
+    } else if (jl == nl) {
+      comp_v = vcc_u(fm, fc) * az_u;
+    } else if (jl > nflat) {
+      const R cu = vcc_u(fm, fc), cl = vcc_l(fc, fp);
+      comp_v = cu * az_u - cl * az_l;
+    }
+    return (comp_h + comp_v) / V;
+  };
+  const R div_g = div(fm_g, fc_g, fp_g);
+  const R div_h_ = div(fm_h, fc_h, fp_h);
   const R heat = c.w.heating_diss[i] + c.w.radiation_tendency[i] + c.w.csh[i];
-  c.w.tend_rhotheta_v[i] = -div_h(c, c.w.thu, c.w.thv, ji, jk, jl) +
-                           R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
-                               (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
-  if (c.G.nc > 1) {
+  const R rt_tend = -div_h_ + R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
+                                 (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
+  // tracers: upstream densities at the four faces and the two interfaces
+  R dtr[MAX_CONST];
+  if (tracers) {
+    const int k1 = (jk + 1 == nx + 1) ? 1 : jk + 1;  // face jk+1 (the copy of face 1 when jk = nx)
+    const int j1 = (ji + 1 == ny + 1) ? 1 : ji + 1;
+    const R ue = ATR(wu, ji, k1, jl), uw = fc_w.w, vn = fc_w.n, vs = ATR(wv, j1, jk, jl);
+    const ix_t re = ue >= R(0.0) ? LG_IX(G, ji, wrapm(k1, nx), jl) : LG_IX(G, ji, k1, jl);
+    const ix_t rw = uw >= R(0.0) ? LG_IX(G, ji, wrapm(jk, nx), jl) : LG_IX(G, ji, jk, jl);
+    const ix_t rn = vn >= R(0.0) ? LG_IX(G, ji, jk, jl) : LG_IX(G, wrapm(ji, ny), jk, jl);
+    const ix_t rs = vs >= R(0.0) ? LG_IX(G, j1, jk, jl) : LG_IX(G, wrapm(j1, ny), jk, jl);
+    R cu = R(0.0), cl = R(0.0);
+    if (jl == nflat) {
+      cl = vcc_l(fc_w, fp_w);
+    } else if (jl == nl) {
+      cu = vcc_u(fm_w, fc_w);
+    } else if (jl > nflat) {
+      cu = vcc_u(fm_w, fc_w);
+      cl = vcc_l(fc_w, fp_w);
+    }
+    // the upstream layer of each interface, as an offset from this cell
+    const ix_t iu = cu <= R(0.0) ? LG_IX(G, ji, jk, jl - 1) : i;
+    const ix_t il = cl <= R(0.0) ? i : LG_IX(G, ji, jk, jl + 1);
+#pragma unroll
+    for (int q = 0; q < MAX_CONST; ++q) {
+      dtr[q] = R(0.0);
+      if (q >= nc || q == ncond) continue;
+      const Fld<const double> r = ss.rho[q];
+      const R comp_h = (R(r[re]) * ue) * ax1 + (R(r[rn]) * vn) * ay0 - (R(r[rw]) * uw) * ax0 - (R(r[rs]) * vs) * ay1;
+      R comp_v = R(0.0);
+      if (jl == nflat) {
+        comp_v = -R(r[il]) * cl * az_l;
+      } else if (jl == nl) {
+        comp_v = R(r[iu]) * cu * az_u;
+      } else if (jl > nflat) {
+        comp_v = R(r[iu]) * cu * az_u - R(r[il]) * cl * az_l;
+      }
+      dtr[q] = (comp_h + comp_v) / V;
+    }
+  }
+  // stores
+  c.w.tend_rho[ncond][i] = -div_g;
+  c.w.tend_rhotheta_v[i] = rt_tend;
+  if (tracers) {
     const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
-    const TracerFaces<R> f = tracer_faces(c, nw, ji, jk, jl);
-    for (int jc = 0; jc < c.G.nc; ++jc) {
-      if (jc == ncond) continue;
-      const R d = div_tracer(c, f, ss.rho[jc], ji, jk, jl);
-      c.w.tend_rho[jc][i] = old_w * c.w.tend_rho[jc][i] + new_w * (-d);
+#pragma unroll
+    for (int q = 0; q < MAX_CONST; ++q) {
+      if (q >= nc || q == ncond) continue;
+      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * (-dtr[q]);
     }
   }
   if (c.C.lfreeslip && jl == nl) {
-    AT(nw.wind_w, ji, jk, nl + 1) = AT(c.g.ipw[0], ji, jk, nl) * ATR(nw.wind_u, ji, jk + 1, nl) * AT(c.g.slope_x, ji, jk + 1, nl) +
-                                    AT(c.g.ipw[1], ji, jk, nl) * ATR(nw.wind_v, ji, jk, nl) * AT(c.g.slope_y, ji, jk, nl) +
-                                    AT(c.g.ipw[2], ji, jk, nl) * ATR(nw.wind_u, ji, jk, nl) * AT(c.g.slope_x, ji, jk, nl) +
-                                    AT(c.g.ipw[3], ji, jk, nl) * ATR(nw.wind_v, ji + 1, jk, nl) * AT(c.g.slope_y, ji + 1, jk, nl);
+    AT(nw.wind_w, ji, jk, nl + 1) = AT(c.g.ipw[0], ji, jk, nl) * ATR(wu, ji, jk + 1, nl) * AT(c.g.slope_x, ji, jk + 1, nl) +
+                                    AT(c.g.ipw[1], ji, jk, nl) * ATR(wv, ji, jk, nl) * AT(c.g.slope_y, ji, jk, nl) +
+                                    AT(c.g.ipw[2], ji, jk, nl) * ATR(wu, ji, jk, nl) * AT(c.g.slope_x, ji, jk, nl) +
+                                    AT(c.g.ipw[3], ji, jk, nl) * ATR(wv, ji + 1, jk, nl) * AT(c.g.slope_y, ji + 1, jk, nl);
   }
 }
 
