@@ -105,11 +105,129 @@ __device__ __forceinline__ R decel(const CtxT<R> &c, const StateDev &s, int ji, 
   return R(C_D_P) * R(s.rho[c.G.ncond][i]) / rho_sum(c, s, i);
 }
 
+// The pressure-gradient accelerations at the u, v and w points of one interior cell (ji <= ny,
+// jk <= nx). Its west, south and upper neighbours and the cell itself are shared by the three
+// points, so every state value, metric and deceleration factor is loaded or computed once and
+// then combined with the expressions of grad_hor_x/_y, remap_ver2hor_x/_y and the w-point
+// gradient, in their order.
+template <class R>
+__device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev &s, int lfirst, int ji, int jk, int jl) {
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, nflat = c.G.nflat;
+  const bool o3 = c.C.order == 3;
+  const Geom &G = c.G;
+  const ix_t iC = LG_IX(G, ji, jk, jl), iW = LG_IX(G, ji, wrapm(jk, nx), jl), iS = LG_IX(G, wrapm(ji, ny), jk, jl);
+  const ix_t dl = (ix_t)G.lvl;  // one level down
+  const Fld<const double> ep = s.exner_pert, tp = s.theta_v_pert;
+  const bool w_pt = jl >= 2;
+  const bool terr = jl > nflat;
+  const bool below = jl < nl || c.C.lfreeslip;  // the remap reads the interface below
+  // Exner pressure and the vertical gradient of it (grad_vert, zero at levels 1 and n_levels)
+  const R eC = R(ep[iC]), eW = R(ep[iW]), eS = R(ep[iS]);
+  auto vgrad = [&](ix_t ix, R e_here, int l) {  // level l of the column through ix (at level jl)
+    if (l >= nlev || l <= 1) return R(0.0);
+    const ix_t il = ix + (l - jl) * dl;
+    const R up = (l == jl) ? R(ep[il - dl]) : e_here;
+    const R lo = (l == jl) ? e_here : R(ep[il]);
+    return (up - lo) / c.g.dz[il];
+  };
+  R vC = R(0.0), vW = R(0.0), vS = R(0.0), vC1 = R(0.0), vW1 = R(0.0), vS1 = R(0.0);
+  if (terr || w_pt) vC = vgrad(iC, eC, jl);
+  if (terr) {
+    vW = vgrad(iW, eW, jl);
+    vS = vgrad(iS, eS, jl);
+    if (below) {
+      vC1 = vgrad(iC, eC, jl + 1);
+      vW1 = vgrad(iW, eW, jl + 1);
+      vS1 = vgrad(iS, eS, jl + 1);
+    }
+  }
+  // theta_v, full and perturbation, and the deceleration factors
+  auto decel_at = [&](ix_t ix) {
+    R r = R(0.0);
+    for (int jc = 0; jc <= c.G.ncond; ++jc) r = r + R(s.rho[jc][ix]);
+    return R(C_D_P) * R(s.rho[c.G.ncond][ix]) / r;
+  };
+  const R tpC = R(tp[iC]), thC = c.g.theta_v_bg[iC] + tpC, dC = decel_at(iC);
+  // u point
+  {
+    if (!lfirst) c.w.pold_u[iC] = -c.w.pnl_u[iC] - R(1.0) * c.w.pl_u[iC];
+    R g = (eC - eW) / c.g.dx[iC];
+    if (terr) {
+      R r = R(0.0);
+      r = r + c.g.ipw[4][iW] * vW;
+      r = r + c.g.ipw[4][iC] * vC;
+      if (below) {
+        r = r + c.g.ipw[5][iW] * vW1;
+        r = r + c.g.ipw[5][iC] * vC1;
+      }
+      g = g - c.g.slope_x[iC] * (R(0.5) * r);
+    }
+    const R tpW = R(tp[iW]);
+    R t, lu;
+    if (o3) {
+      t = c.w.tu[0][iC] * g;
+      lu = c.w.tu[1][iC] * c.g.ebg_u[iC];
+    } else {
+      t = R(0.5) * ((c.g.theta_v_bg[iW] + tpW) + thC) * g;
+      lu = R(0.5) * (tpW + tpC) * c.g.ebg_u[iC];
+    }
+    const R d = R(0.5) * (decel_at(iW) + dC);
+    const R nlu = d * t, llu = d * lu;
+    c.w.pnl_u[iC] = nlu;
+    c.w.pl_u[iC] = llu;
+    if (lfirst) c.w.pold_u[iC] = -nlu - R(1.0) * llu;
+  }
+  // v point
+  {
+    if (!lfirst) c.w.pold_v[iC] = -c.w.pnl_v[iC] - R(1.0) * c.w.pl_v[iC];
+    R g = (eS - eC) / c.g.dy[iC];
+    if (terr) {
+      R r = R(0.0);
+      r = r + c.g.ipw[4][iS] * vS;
+      r = r + c.g.ipw[4][iC] * vC;
+      if (below) {
+        r = r + c.g.ipw[5][iS] * vS1;
+        r = r + c.g.ipw[5][iC] * vC1;
+      }
+      g = g - c.g.slope_y[iC] * (R(0.5) * r);
+    }
+    const R tpS = R(tp[iS]);
+    R t, lv;
+    if (o3) {
+      t = c.w.tv[0][iC] * g;
+      lv = c.w.tv[1][iC] * c.g.ebg_v[iC];
+    } else {
+      t = R(0.5) * ((c.g.theta_v_bg[iS] + tpS) + thC) * g;
+      lv = R(0.5) * (tpS + tpC) * c.g.ebg_v[iC];
+    }
+    const R d = R(0.5) * (decel_at(iS) + dC);
+    const R nlv = d * t, llv = d * lv;
+    c.w.pnl_v[iC] = nlv;
+    c.w.pl_v[iC] = llv;
+    if (lfirst) c.w.pold_v[iC] = -nlv - R(1.0) * llv;
+  }
+  // w point, levels 2..n_layers
+  if (w_pt) {
+    const ix_t iA = iC - dl;
+    const R tpA = R(tp[iA]);
+    const R t = R(0.5) * ((c.g.theta_v_bg[iA] + tpA) + thC) * vC;
+    const R lw = R(0.5) * (tpA + tpC) * c.g.ebg_w[iC];
+    const R d = R(0.5) * (decel_at(iA) + dC);
+    c.w.pnl_w[iC] = d * t;
+    c.w.pl_w[iC] = d * lw;
+  }
+}
+
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl + 1) return;
+  if (ji <= ny && jk <= nx && jl <= nl) {
+    pgrad_interior(c, s, lfirst, ji, jk, jl);
+    return;
+  }
+  // the edge points: u at jk = nx+1 (the copy of jk = 1) and v at ji = ny+1
   const bool o3 = c.C.order == 3;
   if (ji <= ny && jl <= nl) {  // u point; the final value at nx+1 is the copy of jk = 1
     const ix_t iu = LG_IX(c.G, ji, jk, jl);
