@@ -5,6 +5,8 @@
 // column are built while streaming down the column with a rolling window of three layers; the
 // forward Thomas elimination runs in the same sweep, and only the eliminated coefficients plus
 // the few per-level values the back substitution needs go to coalesced column scratch.
+#include <type_traits>
+
 #include "lgame_ops.h"
 #include "lgame_step.h"
 
@@ -264,18 +266,15 @@ struct VfIn {
 
 template <class R>
 __device__ __forceinline__ VfIn vf_load(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+  // branch-free: every level read lies in 1..nl, whatever jl; the values that the formula of
+  // level jl does not use are never read by vf_liquid
+  const int nl = c.G.nl;
+  const int lm = jl < 2 ? 2 : (jl > nl - 1 ? nl - 1 : jl);  // a middle level near jl
   VfIn v;
-  v.g1 = v.w4 = v.w5 = 0.0;
-  if (jl == 1) {
-    v.g0 = AT(c.g64.gravity_m_v, ji, jk, jl + 1);
-  } else if (jl == c.G.nl) {
-    v.g0 = AT(c.g64.gravity_m_v, ji, jk, jl);
-  } else {
-    v.w4 = AT(c.g64.ipw[4], ji, jk, jl);
-    v.g0 = AT(c.g64.gravity_m_v, ji, jk, jl);
-    v.w5 = AT(c.g64.ipw[5], ji, jk, jl);
-    v.g1 = AT(c.g64.gravity_m_v, ji, jk, jl + 1);
-  }
+  v.g0 = AT(c.g64.gravity_m_v, ji, jk, jl == 1 ? 2 : jl);
+  v.w4 = AT(c.g64.ipw[4], ji, jk, lm);
+  v.w5 = AT(c.g64.ipw[5], ji, jk, lm);
+  v.g1 = AT(c.g64.gravity_m_v, ji, jk, lm + 1);
   v.rho = AT(s.rho[c.G.ncond], ji, jk, jl);
   v.temp = (double)AT(c.w.temperature, ji, jk, jl);
   return v;
@@ -293,13 +292,11 @@ __device__ __forceinline__ double vf_liquid(const VfIn &in, double radius, int j
   const double cross_section = 4.0 * MPI * (particle_radius * particle_radius);
   const double mean_free_path = 1.0 / (sqrt(2.0) * particle_density * cross_section);
   const double kin = (1.0 / 3.0) * thermal_velocity * mean_free_path;
-  double v = 2.0 * (radius * radius) * (RHO_H2O - rho) * gl / (9.0 * rho * kin);
+  const double v = 2.0 * (radius * radius) * (RHO_H2O - rho) * gl / (9.0 * rho * kin);
   const double re = v * radius / kin;
-  if (re > reynolds_crit) {
-    const double c_w = 0.8;
-    v = sqrt(8.0 * radius * RHO_H2O * gl / (3.0 * rho * c_w));
-  }
-  return v;
+  const double c_w = 0.8;
+  const double v_turb = sqrt(8.0 * radius * RHO_H2O * gl / (3.0 * rho * c_w));  // both branches, one selected
+  return re > reynolds_crit ? v_turb : v;
 }
 
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
@@ -317,185 +314,250 @@ __device__ __forceinline__ double v_fall_solid(double radius) { return (radius >
 // 0.001 m/s, cloud water v_fall_liquid of 12e-6 m; rain and graupel use the radius diag%a_rain,
 // which the reference keeps at zero, so their v_fall_liquid is exactly zero
 // (mo_dictionary.F90:475-527, control.F90:443).
-constexpr int NCS = 5;    // condensates among them (constituents 1..5)
-constexpr int GEN_K = 8;  // levels per exchange of sediment-heating terms
+constexpr int NCS = 5;     // condensates among them (constituents 1..5)
+constexpr int GEN_K = 8;   // levels per exchange of sediment-heating terms
+constexpr int GEN_RING = 2 * GEN_K;  // LDS slots of heating terms: the current and the next exchange
 constexpr int BACK_K = 8;  // levels per chunk of back-substitution loads
 
-template <class R>
-__global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
-  // sediment-heating terms of the condensates: csh(jl) = ((0 + a_1) - b_1) + a_2 - b_2 ...,
-  // b only at the lowest layer (the surface sink term)
-  __shared__ double sh_a_all[GEN_GROUPS][GEN_K][NCS][BCOL];
-  __shared__ double sh_b_all[GEN_GROUPS][NCS][BCOL];
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond;
-  // blockDim.x is the wavefront width, so the tracer index is uniform over a wavefront; telling
-  // the compiler so keeps the tracer selection in scalar registers and scalar branches
-  const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int t = wy % NTR, grp = wy / NTR;
-  double(*sh_a)[NCS][BCOL] = sh_a_all[grp];
-  double(*sh_b)[BCOL] = sh_b_all[grp];
-  const int col_raw = (blockIdx.x * GEN_GROUPS + grp) * BCOL + lane;
-  const bool active = col_raw < nx * ny;
-  const int col = active ? col_raw : nx * ny - 1;  // idle lanes mirror a valid column, store nothing
-  const int jk = col % nx + 1, ji = col / nx + 1;
+// The five kinds of tracer, by sink velocity and heat capacity: snow and cloud ice (constant
+// v_fall_solid, ice), cloud water (v_fall_liquid, water), rain (no sink, water), graupel (no sink,
+// ice), vapour (no sink, not a condensate).
+enum { VF_NONE = 0, VF_CONST = 1, VF_LIQUID = 2 };
+enum { CP_NONE = 0, CP_ICE = 1, CP_WATER = 2 };
+// position of a layer in its column
+enum { POS_TOP = 0, POS_MID = 1, POS_BOTTOM = 2 };
+
+// c_p_water with both temperature branches evaluated and the one of the reference selected, so
+// that the sweep below has no divergent branch (mo_dictionary.F90:112-150)
+__device__ __forceinline__ double c_p_water_sel(double temperature) {
+  const double temp_c = temperature - T_0;
+  const double ta = temp_c > 35.0 ? 35.0 : temp_c;
+  const double d = ta - 35.0, d2 = d * d, d4 = d2 * d2;
+  const double ra = 0.9979 + 3.1e-6 * d2 + 3.8e-9 * d4;
+  const double tb = temp_c < -37.0 ? -37.0 : temp_c;
+  const double t2 = tb * tb, t3 = t2 * tb, t4 = t2 * t2;
+  const double rb = 1.000938 - 2.7052e-3 * tb - 2.3235e-5 * t2 + 4.3778e-6 * t3 + 2.7136e-7 * t4;
+  return 4186.8 * (temp_c >= 0.0 ? ra : rb);
+}
+
+// inputs of layer l and of the interface below it (l+1)
+struct GenIn {
+  double tr, wo, wn, az, Vb, ro_b, tif;
+  VfIn vf;
+};
+
+// values carried from one layer to the next: the fluxes through the interface above, the
+// density and volume of the layer and, for cloud water, its sink velocity
+struct GenCarry {
+  double fi_up, fr_up, fe_up, ro_cur, q, vcw_lo;  // q = itw*dt/V of the layer
+};
+
+// one row of the tridiagonal system and the layer's sediment-heating terms
+struct GenRow {
+  double d, e, r, cq, ha, hb;
+};
+
+// the coefficients of layer l (mo_column_solvers.F90:404-560), all selects and no branches
+template <int VF, int CP, bool HEAT, int POS>
+__device__ __forceinline__ GenRow gen_row(const GenIn &cur, GenCarry &s, double vf_const, double dt, double V, double az_sfc,
+                                          double t_sfc, bool hb_on, int l, int nl) {
+  constexpr bool cond = CP != CP_NONE;
   const double itw = 0.5, ew = 1.0 - itw;
+  double fi = 0.0, fr = 0.0, fe = 0.0, q_b = 0.0;
+  const double vcw_layer = s.vcw_lo;  // v_fall of cloud water in layer l
+  if (POS != POS_BOTTOM) {
+    double vf = 0.0;
+    if (VF == VF_CONST) vf = 0.5 * (vf_const + vf_const);
+    if (VF == VF_LIQUID) {
+      s.vcw_lo = vf_liquid(cur.vf, 12.e-6, l + 1, nl);
+      vf = 0.5 * (vcw_layer + s.vcw_lo);
+    }
+    fi = cur.wo - vf;
+    fr = cur.wn - vf;
+    fi = cur.az * fi;
+    fr = cur.az * fr;
+    const double dens = fr >= 0.0 ? cur.ro_b : s.ro_cur;
+    fr = dens * fr;
+    if (HEAT && cond) {
+      const double tif = cur.tif;
+      const double cp = CP == CP_ICE ? c_p_ice(tif) : c_p_water_sel(tif);
+      fe = cp * tif * fr;
+    }
+    q_b = itw * dt / cur.Vb;  // itw*dt/V of the layer below, also its q
+  }
+  const double q = s.q;
+  GenRow w;
+  if (POS == POS_TOP) {
+    w.d = fi >= 0.0 ? 1.0 : 1.0 - q * fi;
+  } else if (POS == POS_BOTTOM) {
+    w.d = s.fi_up >= 0.0 ? 1.0 + q * s.fi_up : 1.0;
+  } else {
+    w.d = s.fi_up >= 0.0 ? 1.0 + q * s.fi_up : 1.0;
+    w.d = fi < 0.0 ? w.d - q * fi : w.d;
+  }
+  double vfs = 0.0;  // sink velocity in the lowest layer
+  if (POS == POS_BOTTOM && cond) {
+    vfs = VF == VF_CONST ? vf_const : VF == VF_LIQUID ? vcw_layer : 0.0;
+    w.d = w.d + itw * vfs * dt * az_sfc / V;
+  }
+  w.e = 0.0;
+  w.cq = 0.0;
+  if (POS != POS_BOTTOM) {
+    w.e = fi >= 0.0 ? -(q * fi) : 0.0;  // -itw*dt/V*fi
+    w.cq = fi >= 0.0 ? 0.0 : q_b * fi;  // itw*dt/V(below)*fi
+  }
+  w.r = s.ro_cur + dt * cur.tr;
+  w.ha = 0.0;
+  w.hb = 0.0;
+  if (POS == POS_TOP) {
+    w.r = w.r + ew * dt * fr / V;
+    w.ha = fe / V;
+  } else if (POS == POS_BOTTOM) {
+    w.r = w.r - ew * dt * s.fr_up / V;
+    w.ha = -(s.fe_up / V);
+    if (cond) {
+      w.r = w.r - ew * vfs * dt * s.ro_cur * az_sfc / V;
+      if (HEAT) {
+        const double cp = CP == CP_ICE ? c_p_ice(t_sfc) : c_p_water_sel(t_sfc);
+        w.hb = hb_on ? vfs * t_sfc * cp * s.ro_cur * az_sfc / V : 0.0;
+      }
+    }
+  } else {
+    w.r = w.r + ew * dt * (-s.fr_up + fr) / V;
+    w.ha = (-s.fe_up + fe) / V;
+  }
+  s.fi_up = fi;
+  s.fr_up = fr;
+  s.fe_up = fe;
+  s.ro_cur = cur.ro_b;
+  s.q = q_b;
+  return w;
+}
+
+// three_band_solver_gen_densities, mo_column_solvers.F90:353-604.
+//
+// A block holds 64-column groups with one wavefront per tracer (snow, rain, cloud ice, cloud
+// water, graupel, vapour), so every lane eliminates one tracer of one column and the per-level
+// loads of the winds, metrics and temperature that all tracers of a column share are served
+// from the CU's cache.  At the predictor substep the five condensates also contribute to the
+// sediment heating, which the reference accumulates per level in constituent order; each
+// condensate wave leaves its per-level terms in LDS, and after every GEN_K levels the vapour
+// wave (which has the least work) adds them up in that order and stores the heating.
+// Sink velocities: snow v_fall_solid of 500e-6 m = 1 m/s, cloud ice v_fall_solid of 95e-6 m =
+// 0.001 m/s, cloud water v_fall_liquid of 12e-6 m; rain and graupel use the radius diag%a_rain,
+// which the reference keeps at zero, so their v_fall_liquid is exactly zero
+// (mo_dictionary.F90:475-527, control.F90:443).
+//
+// The sweep is software-pipelined: the row of layer l+1 is built in the same straight-line block
+// as the elimination of layer l, so the long dependent chain of the elimination (two divisions
+// per layer) overlaps with the independent arithmetic of the next row.  Each tracer kind runs
+// its own instantiation, which has no branches on the kind.
+struct GenShared {
+  double a[GEN_GROUPS][GEN_RING][NCS][BCOL];
+  double b[GEN_GROUPS][NCS][BCOL];
+};
+
+template <class R, int VF, int CP, bool HEAT>
+__device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old, const StateDev &nw, GenShared &sh, int t, int grp,
+                                          int lane, bool active, int ji, int jk, int jc, double vf_const) {
+  constexpr bool cond = CP != CP_NONE;
+  const int nl = c.G.nl;
   const double dt = c.C.dtime;
-  const int jc = t < NCS ? t + 1 : 7;  // 1-based constituent index
-  const bool cond = jc <= ncond;
-  const bool heat = rk == 1;           // the sediment heating is diagnosed at the predictor only
-  const double vf_snow = v_fall_solid(500.e-6), vf_ice = v_fall_solid(95.e-6);
   const Fld<const double> ro = old.rho[jc - 1];
   const Fld<const R> tr = c.w.tend_rho[jc - 1];
   const Fld<double> sep = c.w.col[2 * t], srp = c.w.col[2 * t + 1];
-  // inputs of layer jl and of the interface below it, loaded one level ahead of their use so
-  // that the loads of the next level are in flight while this level is eliminated and stored
-  struct GenIn {
-    double tr, wo, wn, az, Vb, ro_b, tif;
-    VfIn vf;
-  };
-  auto load = [&](int jl) {
+  const bool is_vapour = t == NTR - 1;
+  // Inputs of layer l <= nl, without branches: at l = nl the interface values come from the
+  // padding level nl+1, which every field has, and are not used.
+  auto load = [&](int l) {
     GenIn g;
-    g.tr = (double)tr[LG_IX(c.G, ji, jk, jl)];
-    g.wo = g.wn = g.az = g.Vb = g.ro_b = g.tif = 0.0;
-    g.vf = VfIn{0.0, 0.0, 0.0, 0.0, 1.0, 1.0};
-    if (jl < nl) {
-      const ix_t iw = LG_IX(c.G, ji, jk, jl + 1);
-      g.wo = old.wind_w[iw];
-      g.wn = nw.wind_w[iw];
-      g.az = c.g64.area_z[iw];
-      g.Vb = c.g64.volume[iw];
-      g.ro_b = ro[iw];
-      if (heat && cond) g.tif = (double)c.w.temperature[iw];  // temperature of layer jl+1 (both flux directions)
-      if (jc == 4) g.vf = vf_load(c, old, ji, jk, jl + 1);
-    }
+    g.tr = (double)tr[LG_IX(c.G, ji, jk, l)];
+    const ix_t iw = LG_IX(c.G, ji, jk, l + 1);
+    g.wo = old.wind_w[iw];
+    g.wn = nw.wind_w[iw];
+    g.az = c.g64.area_z[iw];
+    g.Vb = c.g64.volume[iw];
+    g.ro_b = ro[iw];
+    g.tif = (HEAT && cond) ? (double)c.w.temperature[iw] : 0.0;  // temperature of layer l+1
+    g.vf = VF == VF_LIQUID ? vf_load(c, old, ji, jk, l < nl ? l + 1 : nl) : VfIn{0.0, 0.0, 0.0, 0.0, 1.0, 1.0};
     return g;
   };
-  double ep = 0.0, rp = 0.0, cprev = 0.0, fi_up = 0.0, fr_up = 0.0, fe_up = 0.0;
-  double vcw_lo = jc == 4 ? vf_liquid(vf_load(c, old, ji, jk, 1), 12.e-6, 1, nl) : 0.0;  // cloud-water fall speed
-  double ro_cur = ro[LG_IX(c.G, ji, jk, 1)];
-  double V = c.g64.volume[LG_IX(c.G, ji, jk, 1)];
   const double az_sfc = AT(c.g64.area_z, ji, jk, nl + 1);
-  const double t_sfc = (heat && cond) ? (double)AT(c.w.temperature, ji, jk, nl) : 0.0;
-  // one level of the forward sweep; cur holds the inputs of layer jl
-  auto level = [&](const GenIn &cur, int jl, int j0) {
-      const ix_t i = LG_IX(c.G, ji, jk, jl);
-      const bool below = jl < nl;
-      double fi = 0.0, fr = 0.0, fe = 0.0;
-      const double vcw_layer = vcw_lo;  // v_fall of cloud water in layer jl
-      if (below) {
-        double vf = 0.0;
-        if (jc == 1) {
-          vf = 0.5 * (vf_snow + vf_snow);
-        } else if (jc == 3) {
-          vf = 0.5 * (vf_ice + vf_ice);
-        } else if (jc == 4) {
-          vcw_lo = vf_liquid(cur.vf, 12.e-6, jl + 1, nl);
-          vf = 0.5 * (vcw_layer + vcw_lo);
-        } else if (jc == 2 || jc == 5) {
-          vf = 0.5 * (0.0 + 0.0);
-        }
-        fi = cur.wo - vf;
-        fr = cur.wn - vf;
-        fi = cur.az * fi;
-        fr = cur.az * fr;
-        const double dens = fr >= 0.0 ? cur.ro_b : ro_cur;
-        fr = dens * fr;
-        if (heat && cond) {
-          const double tif = cur.tif;
-          const double cp = ((jc - 1) % 2 == 0) ? c_p_ice(tif) : c_p_water(tif);
-          fe = cp * tif * fr;
-        }
-      }
-      double d, e = 0.0, cq = 0.0;
-      if (jl == 1) {
-        d = (fi >= 0.0) ? 1.0 : 1.0 - itw * dt / V * fi;
-      } else if (jl == nl) {
-        d = (fi_up >= 0.0) ? 1.0 + itw * dt / V * fi_up : 1.0;
-      } else {
-        d = 1.0;
-        if (fi_up >= 0.0) d = d + itw * dt / V * fi_up;
-        if (fi < 0.0) d = d - itw * dt / V * fi;
-      }
-      double vfs = 0.0;  // sink velocity in the lowest layer
-      if (jl == nl && cond) {
-        vfs = jc == 1 ? vf_snow : jc == 3 ? vf_ice : jc == 4 ? vcw_layer : 0.0;
-        d = d + itw * vfs * dt * az_sfc / V;
-      }
-      if (below) {
-        if (fi >= 0.0) {
-          cq = 0.0;
-          e = -itw * dt / V * fi;
-        } else {
-          cq = itw * dt / cur.Vb * fi;
-          e = 0.0;
-        }
-      }
-      double r = ro_cur + dt * cur.tr;
-      double ha = 0.0, hb = 0.0;  // this tracer's sediment-heating terms of the layer
-      if (jl == 1) {
-        r = r + ew * dt * fr / V;
-        ha = fe / V;
-      } else if (jl == nl) {
-        r = r - ew * dt * fr_up / V;
-        ha = -(fe_up / V);
-        if (cond) {
-          r = r - ew * vfs * dt * ro_cur * az_sfc / V;
-          if (jc > ncond / 4 && heat) hb = vfs * t_sfc * c_p_cond(jc, t_sfc) * ro_cur * az_sfc / V;
-        }
-      } else {
-        r = r + ew * dt * (-fr_up + fr) / V;
-        ha = (-fe_up + fe) / V;
-      }
-      if (heat && cond) {
-        sh_a[jl - j0][t][lane] = ha;
-        if (jl == nl) sh_b[t][lane] = hb;
-      }
-      double epq, rpq;
-      if (jl == 1) {
-        epq = e / d;
-        rpq = r / d;
-      } else {
-        const double den = d - ep * cprev;
-        epq = e / den;
-        rpq = (r - rp * cprev) / den;
-      }
-      if (active) {
-        sep[i] = epq;
-        srp[i] = rpq;
-      }
-      ep = epq;
-      rp = rpq;
-      cprev = cq;
-      fi_up = fi;
-      fr_up = fr;
-      fe_up = fe;
-      ro_cur = cur.ro_b;
-      V = cur.Vb;
+  const double t_sfc = (HEAT && cond) ? (double)AT(c.w.temperature, ji, jk, nl) : 0.0;
+  const bool hb_on = jc > c.G.ncond / 4;
+  GenCarry s;
+  s.fi_up = s.fr_up = s.fe_up = 0.0;
+  s.ro_cur = ro[LG_IX(c.G, ji, jk, 1)];
+  double V = c.g64.volume[LG_IX(c.G, ji, jk, 1)];
+  s.q = 0.5 * dt / V;
+  s.vcw_lo = VF == VF_LIQUID ? vf_liquid(vf_load(c, old, ji, jk, 1), 12.e-6, 1, nl) : 0.0;
+  // heating terms of layer l go to ring slot (l-1) % GEN_RING
+  auto put_heat = [&](const GenRow &w, int l) {
+    if (HEAT && cond) sh.a[grp][(l - 1) % GEN_RING][t][lane] = w.ha;
   };
-  // Two input buffers, used by odd and even levels: a buffer is refilled with the inputs two
-  // levels down as soon as its level is done, so the loads of two levels are in flight.
-  GenIn in_a = load(1), in_b = in_a;
-  if (nl >= 2) in_b = load(2);
+  // Thomas elimination of layer l (mo_column_solvers.F90:562-582; at l = 1 ep = rp = c = 0
+  // leave d and r unchanged, as the reference's first row)
+  double ep = 0.0, rp = 0.0, cprev = 0.0;
+  const int jk_st = active ? jk : c.G.nx + 1;  // idle lanes store into the padding column
+  auto eliminate = [&](const GenRow &w, int l) {
+    const double den = w.d - ep * cprev;
+    const double epq = w.e / den;
+    const double rpq = (w.r - rp * cprev) / den;
+    const ix_t i = LG_IX(c.G, ji, jk_st, l);
+    sep[i] = epq;
+    srp[i] = rpq;
+    ep = epq;
+    rp = rpq;
+    cprev = w.cq;
+  };
+  // two input buffers for odd and even layers, each refilled two layers ahead once used
+  GenIn in_a = load(1), in_b = load(2);
+  GenRow row = gen_row<VF, CP, HEAT, POS_TOP>(in_a, s, vf_const, dt, V, az_sfc, t_sfc, hb_on, 1, nl);
+  V = in_a.Vb;
+  put_heat(row, 1);
+  in_a = load(3);  // n_layers >= 3
   for (int j0 = 1; j0 <= nl; j0 += GEN_K) {
     const int j1 = j0 + GEN_K - 1 < nl ? j0 + GEN_K - 1 : nl;
     for (int jl = j0; jl <= j1; ++jl) {
-      if (jl & 1) {
-        level(in_a, jl, j0);
-        if (jl + 2 <= nl) in_a = load(jl + 2);
+      const int l = jl + 1;  // the layer whose row is built while layer jl is eliminated
+      if (l < nl) {
+        GenRow nxt;
+        // (past the bottom the refill reloads layer nl, which is then never used)
+        const int lf = l + 2 <= nl ? l + 2 : nl;
+        if (l & 1) {
+          nxt = gen_row<VF, CP, HEAT, POS_MID>(in_a, s, vf_const, dt, V, az_sfc, t_sfc, hb_on, l, nl);
+          V = in_a.Vb;
+          eliminate(row, jl);
+          put_heat(nxt, l);
+          in_a = load(lf);
+        } else {
+          nxt = gen_row<VF, CP, HEAT, POS_MID>(in_b, s, vf_const, dt, V, az_sfc, t_sfc, hb_on, l, nl);
+          V = in_b.Vb;
+          eliminate(row, jl);
+          put_heat(nxt, l);
+          in_b = load(lf);
+        }
+        row = nxt;
+      } else if (l == nl) {
+        const GenIn &cur = (l & 1) ? in_a : in_b;
+        const GenRow nxt = gen_row<VF, CP, HEAT, POS_BOTTOM>(cur, s, vf_const, dt, V, az_sfc, t_sfc, hb_on, l, nl);
+        eliminate(row, jl);
+        put_heat(nxt, l);
+        if (HEAT && cond) sh.b[grp][t][lane] = nxt.hb;
+        row = nxt;
       } else {
-        level(in_b, jl, j0);
-        if (jl + 2 <= nl) in_b = load(jl + 2);
+        eliminate(row, jl);
       }
     }
-    if (heat) {  // uniform over the launch: every wave reaches both barriers
+    if (HEAT) {  // uniform over the launch: every wave reaches both barriers
       __syncthreads();
-      if (t == NTR - 1 && active) {
+      if (is_vapour && active) {
         for (int jl = j0; jl <= j1; ++jl) {
           double csh = 0.0;
 #pragma unroll
           for (int q = 0; q < NCS; ++q) {
-            csh = csh + sh_a[jl - j0][q][lane];
-            if (jl == nl) csh = csh - sh_b[q][lane];
+            csh = csh + sh.a[grp][(jl - 1) % GEN_RING][q][lane];
+            if (jl == nl) csh = csh - sh.b[grp][q][lane];
           }
           c.w.csh[LG_IX(c.G, ji, jk, jl)] = (R)csh;
         }
@@ -503,41 +565,101 @@ __global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const Ctx
       __syncthreads();
     }
   }
-  if (!active) return;
+}
+
+// Back substitution of one tracer.  It is the same for all tracer kinds and is called once, after
+// the kind-specific sweeps, so the kernel holds one copy of its code.
+__device__ __forceinline__ void gen_back(const Geom &G, Fld<const double> srp, Fld<const double> sep, Fld<double> rn, int ji, int jk) {
+  const int nl = G.nl;
   // back substitution and clipping of round-off negatives (mo_column_solvers.F90:584-596).
   // The scratch rows come in chunks of BACK_K levels into two register buffers: one chunk's
-  // 2*BACK_K loads are in flight while the other chunk is substituted.
+  // 2*BACK_K loads are in flight while the other chunk is substituted.  Only the chunks that
+  // reach above level 1 need bounds checks.
   double x = 0.0;
-  const Fld<double> rn = nw.rho[jc - 1];
   double ar[BACK_K], ae[BACK_K], br[BACK_K], be[BACK_K];
-  auto fetch = [&](int top, double *pr, double *pe) {
+  auto fetch = [&](auto guard, int top, double *pr, double *pe) {
 #pragma unroll
     for (int q = 0; q < BACK_K; ++q) {
       const int jl = top - q;
-      if (jl >= 1) {
-        const ix_t i = LG_IX(c.G, ji, jk, jl);
+      if (!decltype(guard)::value || jl >= 1) {
+        const ix_t i = LG_IX(G, ji, jk, jl);
         pr[q] = srp[i];
         pe[q] = sep[i];
       }
     }
   };
-  auto subst = [&](int top, const double *pr, const double *pe) {
+  auto subst = [&](auto guard, int top, const double *pr, const double *pe) {
 #pragma unroll
     for (int q = 0; q < BACK_K; ++q) {
       const int jl = top - q;
-      if (jl >= 1) {
+      if (!decltype(guard)::value || jl >= 1) {
         x = (jl == nl) ? pr[q] : pr[q] - pe[q] * x;
-        rn[LG_IX(c.G, ji, jk, jl)] = x < 0.0 ? 0.0 : x;
+        rn[LG_IX(G, ji, jk, jl)] = x < 0.0 ? 0.0 : x;
       }
     }
   };
-  fetch(nl, ar, ae);
-  for (int top = nl; top >= 1; top -= 2 * BACK_K) {
-    fetch(top - BACK_K, br, be);
-    subst(top, ar, ae);
-    fetch(top - 2 * BACK_K, ar, ae);
-    subst(top - BACK_K, br, be);
+  const std::true_type checked;
+  const std::false_type unchecked;
+  int top = nl;
+  fetch(checked, top, ar, ae);
+  for (; top >= 2 * BACK_K; top -= 2 * BACK_K) {
+    fetch(unchecked, top - BACK_K, br, be);
+    subst(unchecked, top, ar, ae);
+    fetch(checked, top - 2 * BACK_K, ar, ae);
+    subst(unchecked, top - BACK_K, br, be);
   }
+  if (top >= 1) {
+    fetch(checked, top - BACK_K, br, be);
+    subst(checked, top, ar, ae);
+    subst(checked, top - BACK_K, br, be);
+  }
+}
+
+
+template <class R, bool HEAT>
+__device__ __forceinline__ void gen_dispatch(const CtxT<R> &c, const StateDev &old, const StateDev &nw, GenShared &sh, int t, int grp,
+                                             int lane, bool active, int ji, int jk) {
+  const double vf_snow = v_fall_solid(500.e-6), vf_ice = v_fall_solid(95.e-6);
+  // Constituents 1..5 and vapour (7).  Without the heating, rain and graupel (no sink velocity)
+  // take the vapour instantiation: their surface-sink terms are d + 0 and r - 0, which leave d
+  // and r unchanged (r - (+0) = r also for r = -0; the densities are not negative).
+  switch (t) {
+    case 0: gen_sweep<R, VF_CONST, CP_ICE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 1, vf_snow); break;
+    case 1:
+      if (HEAT) gen_sweep<R, VF_NONE, CP_WATER, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 2, 0.0);
+      else gen_sweep<R, VF_NONE, CP_NONE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 2, 0.0);
+      break;
+    case 2: gen_sweep<R, VF_CONST, CP_ICE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 3, vf_ice); break;
+    case 3: gen_sweep<R, VF_LIQUID, CP_WATER, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 4, 0.0); break;
+    case 4:
+      if (HEAT) gen_sweep<R, VF_NONE, CP_ICE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
+      else gen_sweep<R, VF_NONE, CP_NONE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
+      break;
+    default: gen_sweep<R, VF_NONE, CP_NONE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 7, 0.0); break;
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  // sediment-heating terms of the condensates: csh(jl) = ((0 + a_1) - b_1) + a_2 - b_2 ...,
+  // b only at the lowest layer (the surface sink term)
+  __shared__ GenShared sh;
+  const int nx = c.G.nx, ny = c.G.ny;
+  // blockDim.x is the wavefront width, so the tracer index is uniform over a wavefront; telling
+  // the compiler so keeps the tracer selection in scalar registers and scalar branches
+  const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int t = wy % NTR, grp = wy / NTR;
+  const int col_raw = (blockIdx.x * GEN_GROUPS + grp) * BCOL + lane;
+  const bool active = col_raw < nx * ny;
+  const int col = active ? col_raw : nx * ny - 1;  // idle lanes mirror a valid column, store nothing
+  const int jk = col % nx + 1, ji = col / nx + 1;
+  if (rk == 1)  // the sediment heating is diagnosed at the predictor only
+    gen_dispatch<R, true>(c, old, nw, sh, t, grp, lane, active, ji, jk);
+  else
+    gen_dispatch<R, false>(c, old, nw, sh, t, grp, lane, active, ji, jk);
+  if (!active) return;
+  const int jc = t < NCS ? t + 1 : 7;
+  gen_back(c.G, c.w.col[2 * t + 1], c.w.col[2 * t], nw.rho[jc - 1], ji, jk);
 }
 
 template __global__ void k_solve_ver<double>(const CtxT<double>, const StateDev, const StateDev, int);
