@@ -552,7 +552,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
   const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4;
-  const size_t ncol = 12;
+  const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
   h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
   e = hipMalloc(&h->slab, h->slab_bytes);
@@ -633,7 +633,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     W.fv = take();
     W.thu = take();
     W.thv = take();
-    for (int q = 0; q < 14; ++q) W.col[q] = col[q];
+    for (int q = 0; q < 12; ++q) W.col[q] = col[q];
+    W.vfall_cw = col[12];
   };
   if (mixed)
     assign_work(h->cf.w, take4);

@@ -216,6 +216,39 @@ __device__ __forceinline__ R c_v_mass_weighted_air(const CtxT<R> &c, const State
   return r;
 }
 
+// v_fall_liquid (mo_derived.F90:218-283, with calc_diff_coeff :189-216) of a cloud-water droplet
+// of the given radius in layer jl, at the given temperature; FP64 in both forms.  Both branches
+// of the Reynolds-number test are evaluated and one is selected, so callers stay branch-free.
+template <class R>
+__device__ __forceinline__ double v_fall_liquid(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl, double temperature,
+                                                double radius) {
+  const int nl = c.G.nl;
+  double gl;
+  if (jl == 1)
+    gl = AT(c.g64.gravity_m_v, ji, jk, 2);
+  else if (jl == nl)
+    gl = AT(c.g64.gravity_m_v, ji, jk, jl);
+  else
+    gl = AT(c.g64.ipw[4], ji, jk, jl) * AT(c.g64.gravity_m_v, ji, jk, jl) + AT(c.g64.ipw[5], ji, jk, jl) * AT(c.g64.gravity_m_v, ji, jk, jl + 1);
+  const double reynolds_crit = 10.0;
+  const double rho = AT(s.rho[c.G.ncond], ji, jk, jl);
+  const double particle_radius = 130e-12;
+  const double particle_mass = M_D / N_A;
+  const double thermal_velocity = sqrt(8.0 * K_B * temperature / (MPI * particle_mass));
+  const double particle_density = rho / particle_mass;
+  const double cross_section = 4.0 * MPI * (particle_radius * particle_radius);
+  const double mean_free_path = 1.0 / (sqrt(2.0) * particle_density * cross_section);
+  const double kin = (1.0 / 3.0) * thermal_velocity * mean_free_path;
+  const double v = 2.0 * (radius * radius) * (RHO_H2O - rho) * gl / (9.0 * rho * kin);
+  const double re = v * radius / kin;
+  const double c_w = 0.8;
+  const double v_turb = sqrt(8.0 * radius * RHO_H2O * gl / (3.0 * rho * c_w));
+  return re > reynolds_crit ? v_turb : v;
+}
+
+// radius of the cloud-water droplets whose sink velocity the generalized-density solver uses
+constexpr double CLOUD_WATER_RADIUS = 12.e-6;
+
 // inner_product of the wind with itself, mo_inner_product.F90:16-50
 template <class R>
 __device__ __forceinline__ R ke_point(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {

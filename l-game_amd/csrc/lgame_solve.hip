@@ -257,48 +257,6 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
   }
 }
 
-// v_fall_liquid / v_fall_solid, mo_derived.F90:218-283; calc_diff_coeff :189-216.
-// The loads (VfIn, vf_load) are separate from the arithmetic (vf_liquid) so that a column sweep
-// can issue them one level ahead.
-struct VfIn {
-  double g0, g1, w4, w5, rho, temp;
-};
-
-template <class R>
-__device__ __forceinline__ VfIn vf_load(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  // branch-free: every level read lies in 1..nl, whatever jl; the values that the formula of
-  // level jl does not use are never read by vf_liquid
-  const int nl = c.G.nl;
-  const int lm = jl < 2 ? 2 : (jl > nl - 1 ? nl - 1 : jl);  // a middle level near jl
-  VfIn v;
-  v.g0 = AT(c.g64.gravity_m_v, ji, jk, jl == 1 ? 2 : jl);
-  v.w4 = AT(c.g64.ipw[4], ji, jk, lm);
-  v.w5 = AT(c.g64.ipw[5], ji, jk, lm);
-  v.g1 = AT(c.g64.gravity_m_v, ji, jk, lm + 1);
-  v.rho = AT(s.rho[c.G.ncond], ji, jk, jl);
-  v.temp = (double)AT(c.w.temperature, ji, jk, jl);
-  return v;
-}
-
-__device__ __forceinline__ double vf_liquid(const VfIn &in, double radius, int jl, int nl) {
-  const double reynolds_crit = 10.0;
-  const double gl = (jl == 1 || jl == nl) ? in.g0 : in.w4 * in.g0 + in.w5 * in.g1;
-  const double rho = in.rho;
-  const double particle_radius = 130e-12;
-  const double particle_mass = M_D / N_A;
-  const double temperature = in.temp;
-  const double thermal_velocity = sqrt(8.0 * K_B * temperature / (MPI * particle_mass));
-  const double particle_density = rho / particle_mass;
-  const double cross_section = 4.0 * MPI * (particle_radius * particle_radius);
-  const double mean_free_path = 1.0 / (sqrt(2.0) * particle_density * cross_section);
-  const double kin = (1.0 / 3.0) * thermal_velocity * mean_free_path;
-  const double v = 2.0 * (radius * radius) * (RHO_H2O - rho) * gl / (9.0 * rho * kin);
-  const double re = v * radius / kin;
-  const double c_w = 0.8;
-  const double v_turb = sqrt(8.0 * radius * RHO_H2O * gl / (3.0 * rho * c_w));  // both branches, one selected
-  return re > reynolds_crit ? v_turb : v;
-}
-
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
 
 // three_band_solver_gen_densities, mo_column_solvers.F90:353-604.
@@ -342,8 +300,7 @@ __device__ __forceinline__ double c_p_water_sel(double temperature) {
 
 // inputs of layer l and of the interface below it (l+1)
 struct GenIn {
-  double tr, wo, wn, az, Vb, ro_b, tif;
-  VfIn vf;
+  double tr, wo, wn, az, Vb, ro_b, tif, vcw;  // vcw: cloud-water sink velocity of layer l+1
 };
 
 // values carried from one layer to the next: the fluxes through the interface above, the
@@ -369,7 +326,7 @@ __device__ __forceinline__ GenRow gen_row(const GenIn &cur, GenCarry &s, double 
     double vf = 0.0;
     if (VF == VF_CONST) vf = 0.5 * (vf_const + vf_const);
     if (VF == VF_LIQUID) {
-      s.vcw_lo = vf_liquid(cur.vf, 12.e-6, l + 1, nl);
+      s.vcw_lo = cur.vcw;
       vf = 0.5 * (vcw_layer + s.vcw_lo);
     }
     fi = cur.wo - vf;
@@ -479,7 +436,7 @@ __device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old,
     g.Vb = c.g64.volume[iw];
     g.ro_b = ro[iw];
     g.tif = (HEAT && cond) ? (double)c.w.temperature[iw] : 0.0;  // temperature of layer l+1
-    g.vf = VF == VF_LIQUID ? vf_load(c, old, ji, jk, l < nl ? l + 1 : nl) : VfIn{0.0, 0.0, 0.0, 0.0, 1.0, 1.0};
+    g.vcw = VF == VF_LIQUID ? c.w.vfall_cw[iw] : 0.0;
     return g;
   };
   const double az_sfc = AT(c.g64.area_z, ji, jk, nl + 1);
@@ -490,7 +447,7 @@ __device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old,
   s.ro_cur = ro[LG_IX(c.G, ji, jk, 1)];
   double V = c.g64.volume[LG_IX(c.G, ji, jk, 1)];
   s.q = 0.5 * dt / V;
-  s.vcw_lo = VF == VF_LIQUID ? vf_liquid(vf_load(c, old, ji, jk, 1), 12.e-6, 1, nl) : 0.0;
+  s.vcw_lo = VF == VF_LIQUID ? AT(c.w.vfall_cw, ji, jk, 1) : 0.0;
   // heating terms of layer l go to ring slot (l-1) % GEN_RING
   auto put_heat = [&](const GenRow &w, int l) {
     if (HEAT && cond) sh.a[grp][(l - 1) % GEN_RING][t][lane] = w.ha;

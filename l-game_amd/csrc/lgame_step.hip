@@ -18,6 +18,8 @@ namespace lgame {
 // ---------------------------------------------------------------------------------------------
 // temperature_diagnostics, mo_derived.F90:18-38
 // ---------------------------------------------------------------------------------------------
+// With moisture, also the sink velocity of cloud water in each layer, which depends only on this
+// state and the temperature; the generalized-density solver of both substeps reads it.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const StateDev s) {
   KIDX;
@@ -26,6 +28,7 @@ __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const 
   R t = (c.g.theta_v_bg[i] + R(s.theta_v_pert[i])) * (c.g.exner_bg[i] + R(s.exner_pert[i]));
   if (c.C.lmoist) t = t / (R(1.0) + R(s.rho[c.G.ncond + 1][i]) / R(s.rho[c.G.ncond][i]) * (R(M_D / M_V) - R(1.0)));
   c.w.temperature[i] = t;
+  if (c.C.lmoist) c.w.vfall_cw[i] = v_fall_liquid(c, s, ji, jk, jl, (double)t, CLOUD_WATER_RADIUS);
 }
 
 // ---------------------------------------------------------------------------------------------
