@@ -269,6 +269,16 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     if (pk) prof->end(PROF_KERNEL0 + (KI));                                         \
   } while (0)
 
+  // the same on a 1-D grid, for kernels that take their tile from tile_zfast
+#define LG_LAUNCH_ZFAST(KI, kern, ex, ey, ez, ...)                                           \
+  do {                                                                                     \
+    const int by_ = tiles.by[KI];                                                          \
+    const dim3 b_(BX, by_, 1), g_((((ex) + BX - 1) / BX) * (((ey) + by_ - 1) / by_) * (ez)); \
+    if (pk) prof->begin(PROF_KERNEL0 + (KI));                                              \
+    hipLaunchKernelGGL(kern<R>, g_, b_, 0, st, __VA_ARGS__);                               \
+    if (pk) prof->end(PROF_KERNEL0 + (KI));                                                \
+  } while (0)
+
   LG_LAUNCH(KI_TEMPERATURE, k_temperature, nx, ny, nl, c, old);
   // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
   if (rad) {
@@ -309,7 +319,7 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
       LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
     }
     LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx + 1, ny + 1, nl, c, S, nw);
-    LG_LAUNCH(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
+    LG_LAUNCH_ZFAST(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
     // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
     if (prof) prof->begin(0);
     hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
@@ -322,6 +332,7 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     }
   }
 #undef LG_LAUNCH
+#undef LG_LAUNCH_ZFAST
 }
 
 template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,

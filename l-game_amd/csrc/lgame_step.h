@@ -42,6 +42,8 @@ struct Tiles {
     for (int &b : by) b = BY;
   }
 };
+// XCDs of the MI355X; blocks of a launch go to them round-robin.
+constexpr int N_XCD = 8;
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
 // The tracer solver: one wavefront per tracer of a 64-column group, two groups per block. Twelve

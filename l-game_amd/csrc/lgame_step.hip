@@ -15,6 +15,26 @@ namespace lgame {
   const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
   const int jl = blockIdx.z + 1;
 
+// Tile order for kernels that read the levels above and below: the kernel is launched on a 1-D
+// grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8 XCDs.  XCD k takes
+// a contiguous range of tiles, the level running fastest, so the blocks of one tile at adjacent
+// levels run one after another on the same XCD and its L2 still holds the shared layers.
+__device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int &by, int &bz) {
+  const int n = gx * gy * gz, b = blockIdx.x;
+  const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, r = n % N_XCD;
+  const int t = k * q + (k < r ? k : r) + s;  // tile index, level fastest
+  bz = t % gz;
+  const int txy = t / gz;
+  bx = txy % gx;
+  by = txy / gx;
+}
+#define KIDX_ZFAST(ex, ey, ez)                                                            \
+  int bx_, by_, bz_;                                                                      \
+  tile_zfast(((ex) + BX - 1) / BX, ((ey) + (int)blockDim.y - 1) / (int)blockDim.y, (ez), bx_, by_, bz_); \
+  const int jk = bx_ * blockDim.x + threadIdx.x + 1;                                      \
+  const int ji = by_ * blockDim.y + threadIdx.y + 1;                                      \
+  const int jl = bz_ + 1;
+
 // ---------------------------------------------------------------------------------------------
 // temperature_diagnostics, mo_derived.F90:18-38
 // ---------------------------------------------------------------------------------------------
@@ -663,7 +683,7 @@ __device__ __forceinline__ R vcc_sub(R r, const VccGeo<R> &g, const Faces<R> &f)
 // final u, v of this substep.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk) {
-  KIDX;
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond, nflat = c.G.nflat, nc = c.G.nc;
   if (jk > nx || ji > ny || jl > nl) return;
   const Geom &G = c.G;
@@ -676,64 +696,68 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
   const bool lo = jl >= nflat && jl < nl;      // interface jl+1
   const bool lay_m = up && jl != nflat + 1;    // layer jl-1
   const bool lay_c = up || (lo && jl != nflat);  // layer jl
-  const bool lay_p = lo;                       // layer jl+1
-  VccGeo<R> gm{}, gc{}, gp{};
-  Faces<R> fm_g{}, fc_g{}, fp_g{}, fm_h{}, fc_h{}, fp_h{}, fm_w{}, fc_w{}, fp_w{};
-  if (lay_m) {
-    gm = vcc_geo(c, ji, jk, jl - 1);
-    fm_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl - 1);
-    fm_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl - 1);
-    if (tracers) fm_w = faces<R>(wu, wv, G, ji, jk, jl - 1);
-  }
-  if (lay_c) gc = vcc_geo(c, ji, jk, jl);
-  // the flux faces of layer jl are needed for the horizontal divergence anyway
-  fc_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl);
-  fc_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl);
+  // Flux faces of layer jl (needed by the horizontal divergence anyway).  The corrections of
+  // the upper and the lower interface are formed one after the other, with the layers above
+  // and below loaded only for their interface, so that few FP64 values are live at once.
+  const Faces<R> fc_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl);
+  const Faces<R> fc_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl);
+  Faces<R> fc_w{};
   if (tracers) fc_w = faces<R>(wu, wv, G, ji, jk, jl);
-  if (lay_p) {
-    gp = vcc_geo(c, ji, jk, jl + 1);
-    fp_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl + 1);
-    fp_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl + 1);
-    if (tracers) fp_w = faces<R>(wu, wv, G, ji, jk, jl + 1);
+  VccGeo<R> gc{};
+  if (lay_c) gc = vcc_geo(c, ji, jk, jl);
+  R cu_g = R(0.0), cu_h = R(0.0), cu_w = R(0.0), cl_g = R(0.0), cl_h = R(0.0), cl_w = R(0.0);
+  if (up) {  // vcc at interface jl: layer jl-1 (unless jl = nflat+1), then layer jl
+    if (lay_m) {
+      const VccGeo<R> gm = vcc_geo(c, ji, jk, jl - 1);
+      cu_g = vcc_sub(R(0.0), gm, faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl - 1));
+      cu_h = vcc_sub(R(0.0), gm, faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl - 1));
+      if (tracers) cu_w = vcc_sub(R(0.0), gm, faces<R>(wu, wv, G, ji, jk, jl - 1));
+    }
+    cu_g = vcc_sub(cu_g, gc, fc_g);
+    cu_h = vcc_sub(cu_h, gc, fc_h);
+    if (tracers) cu_w = vcc_sub(cu_w, gc, fc_w);
   }
+  __builtin_amdgcn_sched_barrier(0);
+  if (lo) {  // vcc at interface jl+1: layer jl (unless jl = nflat), then layer jl+1
+    if (jl != nflat) {
+      cl_g = vcc_sub(R(0.0), gc, fc_g);
+      cl_h = vcc_sub(R(0.0), gc, fc_h);
+      if (tracers) cl_w = vcc_sub(R(0.0), gc, fc_w);
+    }
+    const VccGeo<R> gp = vcc_geo(c, ji, jk, jl + 1);
+    cl_g = vcc_sub(cl_g, gp, faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl + 1));
+    cl_h = vcc_sub(cl_h, gp, faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl + 1));
+    if (tracers) cl_w = vcc_sub(cl_w, gp, faces<R>(wu, wv, G, ji, jk, jl + 1));
+  }
+  __builtin_amdgcn_sched_barrier(0);
   const R ax1 = AT(c.g.area_x, ji, jk + 1, jl), ay0 = AT(c.g.area_y, ji, jk, jl);
   const R ax0 = AT(c.g.area_x, ji, jk, jl), ay1 = AT(c.g.area_y, ji + 1, jk, jl);
   const R az_u = up ? AT(c.g.area_z, ji, jk, jl) : R(0.0);
   const R az_l = lo ? AT(c.g.area_z, ji, jk, jl + 1) : R(0.0);
   const R V = AT(c.g.volume, ji, jk, jl);
-  // vcc at the upper and lower interface for a flux whose faces are fm, fc, fp
-  auto vcc_u = [&](const Faces<R> &fm, const Faces<R> &fc) {
-    R r = R(0.0);
-    if (lay_m) r = vcc_sub(r, gm, fm);
-    return vcc_sub(r, gc, fc);
-  };
-  auto vcc_l = [&](const Faces<R> &fc, const Faces<R> &fp) {
-    R r = R(0.0);
-    if (jl != nflat) r = vcc_sub(r, gc, fc);
-    return vcc_sub(r, gp, fp);
-  };
-  // div_h with the corrections of this cell
-  auto div = [&](const Faces<R> &fm, const Faces<R> &fc, const Faces<R> &fp) {
+  // div_h with the corrections cu, cl of this cell
+  auto div = [&](const Faces<R> &fc, R cu, R cl) {
     const R comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
     R comp_v = R(0.0);
     if (jl == nflat) {
-      comp_v = -vcc_l(fc, fp) * az_l;
+      comp_v = -cl * az_l;
     } else if (jl == nl) {
-      comp_v = vcc_u(fm, fc) * az_u;
+      comp_v = cu * az_u;
     } else if (jl > nflat) {
-      const R cu = vcc_u(fm, fc), cl = vcc_l(fc, fp);
       comp_v = cu * az_u - cl * az_l;
     }
     return (comp_h + comp_v) / V;
   };
-  const R div_g = div(fm_g, fc_g, fp_g);
-  const R div_h_ = div(fm_h, fc_h, fp_h);
+  const R div_g = div(fc_g, cu_g, cl_g);
+  const R div_h_ = div(fc_h, cu_h, cl_h);
   const R heat = c.w.heating_diss[i] + c.w.radiation_tendency[i] + c.w.csh[i];
   const R rt_tend = -div_h_ + R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
                                  (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
+  c.w.tend_rho[ncond][i] = -div_g;
+  c.w.tend_rhotheta_v[i] = rt_tend;
   // tracers: upstream densities at the four faces and the two interfaces
-  R dtr[MAX_CONST];
   if (tracers) {
+    const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
     const int k1 = (jk + 1 == nx + 1) ? 1 : jk + 1;  // face jk+1 (the copy of face 1 when jk = nx)
     const int j1 = (ji + 1 == ny + 1) ? 1 : ji + 1;
     const R ue = ATR(wu, ji, k1, jl), uw = fc_w.w, vn = fc_w.n, vs = ATR(wv, j1, jk, jl);
@@ -741,22 +765,14 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
     const ix_t rw = uw >= R(0.0) ? LG_IX(G, ji, wrapm(jk, nx), jl) : LG_IX(G, ji, jk, jl);
     const ix_t rn = vn >= R(0.0) ? LG_IX(G, ji, jk, jl) : LG_IX(G, wrapm(ji, ny), jk, jl);
     const ix_t rs = vs >= R(0.0) ? LG_IX(G, j1, jk, jl) : LG_IX(G, wrapm(j1, ny), jk, jl);
-    R cu = R(0.0), cl = R(0.0);
-    if (jl == nflat) {
-      cl = vcc_l(fc_w, fp_w);
-    } else if (jl == nl) {
-      cu = vcc_u(fm_w, fc_w);
-    } else if (jl > nflat) {
-      cu = vcc_u(fm_w, fc_w);
-      cl = vcc_l(fc_w, fp_w);
-    }
+    // the corrections where div_h_tracers uses them, zero elsewhere
+    const R cu = (jl == nl || jl > nflat) && jl != nflat ? cu_w : R(0.0);
+    const R cl = jl == nflat || (jl > nflat && jl != nl) ? cl_w : R(0.0);
     // the upstream layer of each interface, as an offset from this cell
     const ix_t iu = cu <= R(0.0) ? LG_IX(G, ji, jk, jl - 1) : i;
     const ix_t il = cl <= R(0.0) ? i : LG_IX(G, ji, jk, jl + 1);
-#pragma unroll
-    for (int q = 0; q < MAX_CONST; ++q) {
-      dtr[q] = R(0.0);
-      if (q >= nc || q == ncond) continue;
+    for (int q = 0; q < nc; ++q) {
+      if (q == ncond) continue;
       const Fld<const double> r = ss.rho[q];
       const R comp_h = (R(r[re]) * ue) * ax1 + (R(r[rn]) * vn) * ay0 - (R(r[rw]) * uw) * ax0 - (R(r[rs]) * vs) * ay1;
       R comp_v = R(0.0);
@@ -767,18 +783,8 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
       } else if (jl > nflat) {
         comp_v = R(r[iu]) * cu * az_u - R(r[il]) * cl * az_l;
       }
-      dtr[q] = (comp_h + comp_v) / V;
-    }
-  }
-  // stores
-  c.w.tend_rho[ncond][i] = -div_g;
-  c.w.tend_rhotheta_v[i] = rt_tend;
-  if (tracers) {
-    const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
-#pragma unroll
-    for (int q = 0; q < MAX_CONST; ++q) {
-      if (q >= nc || q == ncond) continue;
-      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * (-dtr[q]);
+      const R d = (comp_h + comp_v) / V;
+      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * (-d);
     }
   }
   if (c.C.lfreeslip && jl == nl) {
