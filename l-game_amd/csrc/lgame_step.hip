@@ -15,10 +15,12 @@ namespace lgame {
   const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
   const int jl = blockIdx.z + 1;
 
-// Tile order for kernels that read the levels above and below: the kernel is launched on a 1-D
-// grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8 XCDs.  XCD k takes
-// a contiguous range of tiles, the level running fastest, so the blocks of one tile at adjacent
-// levels run one after another on the same XCD and its L2 still holds the shared layers.
+// Tile order for kernels that read many values of the levels above and below: the kernel is
+// launched on a 1-D grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8
+// XCDs.  XCD k takes a contiguous range of tiles, the level running fastest, so the blocks of one
+// tile at adjacent levels run one after another on the same XCD and its L2 still holds the
+// shared layers.  Kernels that mostly stream level by level keep the x-fastest order, which
+// reads DRAM pages in sequence.
 __device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int &by, int &bz) {
   const int n = gx * gy * gz, b = blockIdx.x;
   const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, r = n % N_XCD;
@@ -243,7 +245,7 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
 
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
-  KIDX;
+  KIDX_ZFAST(c.G.nx + 1, c.G.ny + 1, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl + 1) return;
   if (ji <= ny && jk <= nx && jl <= nl) {
@@ -416,7 +418,7 @@ __device__ R pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, int ka, int i
 // calc_pot_vort (mo_vorticities.F90:189-421) with rel_vort (:19-187) inline
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
-  KIDX;
+  KIDX_ZFAST(c.G.nx + 1, c.G.ny + 1, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   if (jk > nx + 1 || ji > ny + 1 || jl > nlev) return;
   if (jk <= nx) {  // eta_x; row ny+1 is the copy of row 1
@@ -478,53 +480,64 @@ __global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const St
   }
 }
 
-// calc_vorticity_flux_term, mo_vorticity_flux.F90:16-171
+// The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
+// mass-flux and weight value that two of the points share is loaded once, and the three results
+// are stored after all loads.  The expressions and their order are those of the reference's
+// per-point loops.
+template <class R>
+__device__ __forceinline__ void vort_flux_interior(const CtxT<R> &c, int ji, int jk, int jl) {
+  const Geom &G = c.G;
+  const int nx = G.nx, ny = G.ny;
+  const Fld<const R> ez = c.w.eta_z, up = c.w.u_ph, vp = c.w.v_ph, wp = c.w.w_ph;
+  const Fld<const R> ex = c.w.eta_x, ey = c.w.eta_y;
+  const int km = wrapm(jk, nx), kp = jk + 1, jm = wrapm(ji, ny);
+  const ix_t dl = (ix_t)G.lvl;
+  const ix_t iC = LG_IX(G, ji, jk, jl), iW = LG_IX(G, ji, km, jl), iE = LG_IX(G, ji, kp, jl);
+  const ix_t iN = LG_IX(G, ji + 1, jk, jl), iNW = LG_IX(G, ji + 1, km, jl), iNE = LG_IX(G, ji + 1, kp, jl);
+  const ix_t iS = LG_IX(G, jm, jk, jl), iSE = LG_IX(G, jm, kp, jl);
+  // eta_z at the eight dual points around the cell
+  const R zC = ez[iC], zW = ez[iW], zE = ez[iE], zN = ez[iN], zNW = ez[iNW], zNE = ez[iNE], zS = ez[iS], zSE = ez[iSE];
+  const R uW = up[iW], uC = up[iC], uE = up[iE], uS = up[iS], uSE = up[iSE];
+  const R vW = vp[iW], vNW = vp[iNW], vN = vp[iN], vC = vp[iC];
+  const R wW0 = wp[iW], wW1 = wp[iW + dl], wC0 = wp[iC], wC1 = wp[iC + dl], wS0 = wp[iS], wS1 = wp[iS + dl];
+  const R ey0 = ey[iC], ey1 = ey[iC + dl], ex0 = ex[iC], ex1 = ex[iC + dl];
+  const R p4W = c.g.ipw[4][iW], p5W = c.g.ipw[5][iW], p4C = c.g.ipw[4][iC], p5C = c.g.ipw[5][iC];
+  const R p4S = c.g.ipw[4][iS], p5S = c.g.ipw[5][iS];
+  // u point (ji, jk)
+  const R *tu = c.g.trsk_u + 3 * (ji - 1);
+  R ru = tu[0] * vW * R(0.25) * (zW + zC + zC + zN) + tu[1] * uW * R(0.25) * (zW + zNW + zC + zN) +
+         tu[2] * vNW * R(0.25) * (zNW + zN + zC + zN) + tu[2] * vN * R(0.25) * (zN + zNE + zC + zN) +
+         tu[1] * uE * R(0.25) * (zNE + zE + zC + zN) + tu[0] * vC * R(0.25) * (zC + zE + zC + zN);
+  ru = ru - R(0.5) * p4W * wW0 * ey0 - R(0.5) * p5W * wW1 * ey1 - R(0.5) * p4C * wC0 * ey0 - R(0.5) * p5C * wC1 * ey1;
+  // v point (ji, jk)
+  const R *tv = c.g.trsk_v + 2 * (ji - 1);
+  R rv = tv[0] * uC * R(0.25) * (zC + zN + zC + zE) + tv[0] * uE * R(0.25) * (zE + zNE + zC + zE) +
+         tv[1] * uSE * R(0.25) * (zSE + zE + zC + zE) + tv[1] * uS * R(0.25) * (zS + zC + zC + zE);
+  rv = rv + R(0.5) * p4S * wS0 * ex0 + R(0.5) * p5S * wS1 * ex1 + R(0.5) * p4C * wC0 * ex0 + R(0.5) * p5C * wC1 * ex1;
+  // w point, levels 2..n_layers
+  R rw = R(0.0);
+  if (jl >= 2) {
+    const ix_t a = iC - dl;
+    rw = R(0.5) * (c.g.ipw[0][a] * up[iE - dl] * ey[iE] - c.g.ipw[1][a] * vp[a] * ex0 + c.g.ipw[2][a] * up[a] * ey0 -
+                   c.g.ipw[3][a] * vp[iN - dl] * ex[iN] + c.g.ipw[0][iC] * uE * ey[iE] - c.g.ipw[1][iC] * vC * ex0 +
+                   c.g.ipw[2][iC] * uC * ey0 - c.g.ipw[3][iC] * vN * ex[iN]);
+  }
+  c.w.pvt_x[iC] = ru;
+  c.w.pvt_y[iC] = rv;
+  if (jl >= 2) c.w.pvt_z[iC] = rw;
+  if (jk == 1) c.w.pvt_x[LG_IX(G, ji, nx + 1, jl)] = ru;
+  if (ji == 1) c.w.pvt_y[LG_IX(G, ny + 1, jk, jl)] = rv;
+}
+
+// calc_vorticity_flux_term, mo_vorticity_flux.F90:16-171.  The u points of column nx+1 and the
+// v points of row ny+1 are copies of column 1 and row 1 (the reference evaluates them with the
+// indices of column 1 and row 1), so the threads of column 1 and row 1 store them too and the
+// launch covers the nx by ny cells only.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c) {
-  KIDX;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
-  const Fld<const R> ez = c.w.eta_z, up = c.w.u_ph, vp = c.w.v_ph, wp = c.w.w_ph;
-  if (ji <= ny) {  // u points; nx+1 is the copy of column 1
-    const int k = (jk == nx + 1) ? 1 : jk, km = wrapm(k, nx), kp = k + 1;
-    const R *tw = c.g.trsk_u + 3 * (ji - 1);
-    R r = tw[0] * AT(vp, ji, km, jl) * R(0.25) * (AT(ez, ji, km, jl) + AT(ez, ji, k, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl)) +
-               tw[1] * AT(up, ji, km, jl) * R(0.25) * (AT(ez, ji, km, jl) + AT(ez, ji + 1, km, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl)) +
-               tw[2] * AT(vp, ji + 1, km, jl) * R(0.25) * (AT(ez, ji + 1, km, jl) + AT(ez, ji + 1, k, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl)) +
-               tw[2] * AT(vp, ji + 1, k, jl) * R(0.25) * (AT(ez, ji + 1, k, jl) + AT(ez, ji + 1, kp, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl)) +
-               tw[1] * AT(up, ji, kp, jl) * R(0.25) * (AT(ez, ji + 1, kp, jl) + AT(ez, ji, kp, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl)) +
-               tw[0] * AT(vp, ji, k, jl) * R(0.25) * (AT(ez, ji, k, jl) + AT(ez, ji, kp, jl) + AT(ez, ji, k, jl) + AT(ez, ji + 1, k, jl));
-    r = r - R(0.5) * AT(c.g.ipw[4], ji, km, jl) * AT(wp, ji, km, jl) * AT(c.w.eta_y, ji, k, jl) -
-        R(0.5) * AT(c.g.ipw[5], ji, km, jl) * AT(wp, ji, km, jl + 1) * AT(c.w.eta_y, ji, k, jl + 1) -
-        R(0.5) * AT(c.g.ipw[4], ji, k, jl) * AT(wp, ji, k, jl) * AT(c.w.eta_y, ji, k, jl) -
-        R(0.5) * AT(c.g.ipw[5], ji, k, jl) * AT(wp, ji, k, jl + 1) * AT(c.w.eta_y, ji, k, jl + 1);
-    AT(c.w.pvt_x, ji, jk, jl) = r;
-  }
-  if (jk <= nx) {  // v points; ny+1 is the copy of row 1
-    const int j = (ji == ny + 1) ? 1 : ji, jm = wrapm(j, ny);
-    const R *tw = c.g.trsk_v + 2 * (j - 1);
-    R r = tw[0] * AT(up, j, jk, jl) * R(0.25) * (AT(ez, j, jk, jl) + AT(ez, j + 1, jk, jl) + AT(ez, j, jk, jl) + AT(ez, j, jk + 1, jl)) +
-               tw[0] * AT(up, j, jk + 1, jl) * R(0.25) * (AT(ez, j, jk + 1, jl) + AT(ez, j + 1, jk + 1, jl) + AT(ez, j, jk, jl) + AT(ez, j, jk + 1, jl)) +
-               tw[1] * AT(up, jm, jk + 1, jl) * R(0.25) * (AT(ez, jm, jk + 1, jl) + AT(ez, j, jk + 1, jl) + AT(ez, j, jk, jl) + AT(ez, j, jk + 1, jl)) +
-               tw[1] * AT(up, jm, jk, jl) * R(0.25) * (AT(ez, jm, jk, jl) + AT(ez, j, jk, jl) + AT(ez, j, jk, jl) + AT(ez, j, jk + 1, jl));
-    r = r + R(0.5) * AT(c.g.ipw[4], jm, jk, jl) * AT(wp, jm, jk, jl) * AT(c.w.eta_x, j, jk, jl) +
-        R(0.5) * AT(c.g.ipw[5], jm, jk, jl) * AT(wp, jm, jk, jl + 1) * AT(c.w.eta_x, j, jk, jl + 1) +
-        R(0.5) * AT(c.g.ipw[4], j, jk, jl) * AT(wp, j, jk, jl) * AT(c.w.eta_x, j, jk, jl) +
-        R(0.5) * AT(c.g.ipw[5], j, jk, jl) * AT(wp, j, jk, jl + 1) * AT(c.w.eta_x, j, jk, jl + 1);
-    AT(c.w.pvt_y, ji, jk, jl) = r;
-  }
-  if (ji <= ny && jk <= nx && jl >= 2) {  // w points, levels 2..n_layers
-    const int l = jl - 1;
-    AT(c.w.pvt_z, ji, jk, jl) =
-        R(0.5) * (AT(c.g.ipw[0], ji, jk, l) * AT(up, ji, jk + 1, l) * AT(c.w.eta_y, ji, jk + 1, jl) -
-               AT(c.g.ipw[1], ji, jk, l) * AT(vp, ji, jk, l) * AT(c.w.eta_x, ji, jk, jl) +
-               AT(c.g.ipw[2], ji, jk, l) * AT(up, ji, jk, l) * AT(c.w.eta_y, ji, jk, jl) -
-               AT(c.g.ipw[3], ji, jk, l) * AT(vp, ji + 1, jk, l) * AT(c.w.eta_x, ji + 1, jk, jl) +
-               AT(c.g.ipw[0], ji, jk, jl) * AT(up, ji, jk + 1, jl) * AT(c.w.eta_y, ji, jk + 1, jl) -
-               AT(c.g.ipw[1], ji, jk, jl) * AT(vp, ji, jk, jl) * AT(c.w.eta_x, ji, jk, jl) +
-               AT(c.g.ipw[2], ji, jk, jl) * AT(up, ji, jk, jl) * AT(c.w.eta_y, ji, jk, jl) -
-               AT(c.g.ipw[3], ji, jk, jl) * AT(vp, ji + 1, jk, jl) * AT(c.w.eta_x, ji + 1, jk, jl));
-  }
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  vort_flux_interior(c, ji, jk, jl);
 }
 
 template <class R>
