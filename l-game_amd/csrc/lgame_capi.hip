@@ -296,12 +296,12 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
         LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 2);
         LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
       }
-      LG_LAUNCH_ZFAST(KI_PGRAD, k_pgrad, nx + 1, ny + 1, nlev, c, old, step == 0 ? 1 : 0);
+      LG_LAUNCH_ZFAST(KI_PGRAD, k_pgrad, nx, ny, nl, c, old, step == 0 ? 1 : 0);
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
-      LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx + 1, ny + 1, nlev, c, S);
-      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx + 1, ny + 1, nlev, c, S);
+      LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
+      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S);
       LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c);
       if (!c.C.llinear) {
         LG_LAUNCH(KI_KE, k_ke, nx, ny, nl, c, S);
@@ -318,7 +318,7 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
       LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 1);
       LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
     }
-    LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx + 1, ny + 1, nl, c, S, nw);
+    LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx, ny, nl, c, S, nw);
     LG_LAUNCH_ZFAST(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
     // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
     if (prof) prof->begin(0);

@@ -124,14 +124,7 @@ __global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<con
 // manage_p_grad, mo_p_grad.F90:20-104 (luse_bg_state): nonlinear and linear pressure gradient
 // accelerations at u, v and w points, scaled by the condensate deceleration factor.
 // ---------------------------------------------------------------------------------------------
-template <class R>
-__device__ __forceinline__ R decel(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  const ix_t i = LG_IX(c.G, ji, jk, jl);
-  return R(C_D_P) * R(s.rho[c.G.ncond][i]) / rho_sum(c, s, i);
-}
-
-// The pressure-gradient accelerations at the u, v and w points of one interior cell (ji <= ny,
-// jk <= nx). Its west, south and upper neighbours and the cell itself are shared by the three
+// The pressure-gradient accelerations at the u, v and w points of one cell (ji <= ny, jk <= nx). Its west, south and upper neighbours and the cell itself are shared by the three
 // points, so every state value, metric and deceleration factor is loaded or computed once and
 // then combined with the expressions of grad_hor_x/_y, remap_ver2hor_x/_y and the w-point
 // gradient, in their order.
@@ -175,7 +168,11 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
   const R tpC = R(tp[iC]), thC = c.g.theta_v_bg[iC] + tpC, dC = decel_at(iC);
   // u point
   {
-    if (!lfirst) c.w.pold_u[iC] = -c.w.pnl_u[iC] - R(1.0) * c.w.pl_u[iC];
+    R pold = R(0.0);
+    if (!lfirst) {
+      pold = -c.w.pnl_u[iC] - R(1.0) * c.w.pl_u[iC];
+      c.w.pold_u[iC] = pold;
+    }
     R g = (eC - eW) / c.g.dx[iC];
     if (terr) {
       R r = R(0.0);
@@ -200,11 +197,24 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
     const R nlu = d * t, llu = d * lu;
     c.w.pnl_u[iC] = nlu;
     c.w.pl_u[iC] = llu;
-    if (lfirst) c.w.pold_u[iC] = -nlu - R(1.0) * llu;
+    if (lfirst) {
+      pold = -nlu - R(1.0) * llu;
+      c.w.pold_u[iC] = pold;
+    }
+    if (jk == 1) {  // column nx+1 is the copy of column 1
+      const ix_t iX = LG_IX(G, ji, nx + 1, jl);
+      c.w.pnl_u[iX] = nlu;
+      c.w.pl_u[iX] = llu;
+      c.w.pold_u[iX] = pold;
+    }
   }
   // v point
   {
-    if (!lfirst) c.w.pold_v[iC] = -c.w.pnl_v[iC] - R(1.0) * c.w.pl_v[iC];
+    R pold = R(0.0);
+    if (!lfirst) {
+      pold = -c.w.pnl_v[iC] - R(1.0) * c.w.pl_v[iC];
+      c.w.pold_v[iC] = pold;
+    }
     R g = (eS - eC) / c.g.dy[iC];
     if (terr) {
       R r = R(0.0);
@@ -229,7 +239,16 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
     const R nlv = d * t, llv = d * lv;
     c.w.pnl_v[iC] = nlv;
     c.w.pl_v[iC] = llv;
-    if (lfirst) c.w.pold_v[iC] = -nlv - R(1.0) * llv;
+    if (lfirst) {
+      pold = -nlv - R(1.0) * llv;
+      c.w.pold_v[iC] = pold;
+    }
+    if (ji == 1) {  // row ny+1 is the copy of row 1
+      const ix_t iY = LG_IX(G, ny + 1, jk, jl);
+      c.w.pnl_v[iY] = nlv;
+      c.w.pl_v[iY] = llv;
+      c.w.pold_v[iY] = pold;
+    }
   }
   // w point, levels 2..n_layers
   if (w_pt) {
@@ -243,67 +262,14 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
   }
 }
 
+// The u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1 (the
+// reference evaluates them with the indices of column 1 and row 1); the threads of column 1 and
+// row 1 store them, and the launch covers the nx by ny by n_layers cells only.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
-  KIDX_ZFAST(c.G.nx + 1, c.G.ny + 1, c.G.nlev);
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nl + 1) return;
-  if (ji <= ny && jk <= nx && jl <= nl) {
-    pgrad_interior(c, s, lfirst, ji, jk, jl);
-    return;
-  }
-  // the edge points: u at jk = nx+1 (the copy of jk = 1) and v at ji = ny+1
-  const bool o3 = c.C.order == 3;
-  if (ji <= ny && jl <= nl) {  // u point; the final value at nx+1 is the copy of jk = 1
-    const ix_t iu = LG_IX(c.G, ji, jk, jl);
-    if (!lfirst) c.w.pold_u[iu] = -c.w.pnl_u[iu] - R(1.0) * c.w.pl_u[iu];
-    const int k = (jk == nx + 1) ? 1 : jk, km = wrapm(k, nx);
-    const R g = grad_hor_x(c, s.exner_pert, ji, k, jl);
-    R t, lu;
-    if (o3) {
-      t = AT(c.w.tu[0], ji, k, jl) * g;
-      lu = AT(c.w.tu[1], ji, k, jl) * AT(c.g.ebg_u, ji, k, jl);
-    } else {
-      t = R(0.5) * ((AT(c.g.theta_v_bg, ji, km, jl) + ATR(s.theta_v_pert, ji, km, jl)) +
-                 (AT(c.g.theta_v_bg, ji, k, jl) + ATR(s.theta_v_pert, ji, k, jl))) * g;
-      lu = R(0.5) * (ATR(s.theta_v_pert, ji, km, jl) + ATR(s.theta_v_pert, ji, k, jl)) * AT(c.g.ebg_u, ji, k, jl);
-    }
-    const R d = R(0.5) * (decel(c, s, ji, km, jl) + decel(c, s, ji, k, jl));
-    const R nlu = d * t, llu = d * lu;
-    c.w.pnl_u[iu] = nlu;
-    c.w.pl_u[iu] = llu;
-    if (lfirst) c.w.pold_u[iu] = -nlu - R(1.0) * llu;
-  }
-  if (jk <= nx && jl <= nl) {  // v point
-    const ix_t iv = LG_IX(c.G, ji, jk, jl);
-    if (!lfirst) c.w.pold_v[iv] = -c.w.pnl_v[iv] - R(1.0) * c.w.pl_v[iv];
-    const int j = (ji == ny + 1) ? 1 : ji, jm = wrapm(j, ny);
-    const R g = grad_hor_y(c, s.exner_pert, j, jk, jl);
-    R t, lv;
-    if (o3) {
-      t = AT(c.w.tv[0], j, jk, jl) * g;
-      lv = AT(c.w.tv[1], j, jk, jl) * AT(c.g.ebg_v, j, jk, jl);
-    } else {
-      t = R(0.5) * ((AT(c.g.theta_v_bg, jm, jk, jl) + ATR(s.theta_v_pert, jm, jk, jl)) +
-                 (AT(c.g.theta_v_bg, j, jk, jl) + ATR(s.theta_v_pert, j, jk, jl))) * g;
-      lv = R(0.5) * (ATR(s.theta_v_pert, jm, jk, jl) + ATR(s.theta_v_pert, j, jk, jl)) * AT(c.g.ebg_v, j, jk, jl);
-    }
-    const R d = R(0.5) * (decel(c, s, jm, jk, jl) + decel(c, s, j, jk, jl));
-    const R nlv = d * t, llv = d * lv;
-    c.w.pnl_v[iv] = nlv;
-    c.w.pl_v[iv] = llv;
-    if (lfirst) c.w.pold_v[iv] = -nlv - R(1.0) * llv;
-  }
-  if (ji <= ny && jk <= nx && jl >= 2 && jl <= nl) {  // w point, levels 2..n_layers
-    const ix_t iw = LG_IX(c.G, ji, jk, jl);
-    const ix_t ia = LG_IX(c.G, ji, jk, jl - 1);
-    const R g = (R(s.exner_pert[ia]) - R(s.exner_pert[iw])) / c.g.dz[iw];
-    const R t = R(0.5) * ((c.g.theta_v_bg[ia] + R(s.theta_v_pert[ia])) + (c.g.theta_v_bg[iw] + R(s.theta_v_pert[iw]))) * g;
-    const R lw = R(0.5) * (R(s.theta_v_pert[ia]) + R(s.theta_v_pert[iw])) * c.g.ebg_w[iw];
-    const R d = R(0.5) * (decel(c, s, ji, jk, jl - 1) + decel(c, s, ji, jk, jl));
-    c.w.pnl_w[iw] = d * t;
-    c.w.pl_w[iw] = d * lw;
-  }
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  pgrad_interior(c, s, lfirst, ji, jk, jl);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -418,66 +384,74 @@ __device__ R pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, int ka, int i
 // calc_pot_vort (mo_vorticities.F90:189-421) with rel_vort (:19-187) inline
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
-  KIDX_ZFAST(c.G.nx + 1, c.G.ny + 1, c.G.nlev);
+  // The values of row ny+1 and column nx+1 are copies of row 1 and column 1 (the reference
+  // evaluates them with the indices of row 1 and column 1): the threads of row 1 and column 1
+  // store them, and the launch covers the nx by ny cells.  The three potential vorticities of a
+  // cell are formed before any store, so the loads they share are made once.
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nlev) return;
-  if (jk <= nx) {  // eta_x; row ny+1 is the copy of row 1
-    const int j = (ji == ny + 1) ? 1 : ji;
+  if (jk > nx || ji > ny || jl > nlev) return;
+  const int jm = wrapm(ji, ny), km = wrapm(jk, nx);
+  const int le = jl == 1 ? 2 : jl;  // zeta_x,y(:,:,1) = zeta_x,y(:,:,2)
+  // eta_x
+  R zx = R(0.0);
+  if (!c.C.llinear) zx = zeta_x_circ(c, s, ji, jk, le) / AT(c.g.area_dual_x, ji, jk, le);
+  const R ex = (c.C.lcorio ? zx + c.g.fvec_x[LG_IX(c.G, ji, jk, 1)] : zx) / pv_den_h(c, s, jm, jk, ji, jk, jl);
+  // eta_y
+  R zy = R(0.0);
+  if (!c.C.llinear) zy = zeta_y_circ(c, s, ji, jk, le) / AT(c.g.area_dual_y, ji, jk, le);
+  const R ey = (c.C.lcorio ? zy + c.g.fvec_y[LG_IX(c.G, ji, jk, 1)] : zy) / pv_den_h(c, s, ji, km, ji, jk, jl);
+  // eta_z
+  R ez = R(0.0);
+  if (jl <= nl) {
     R z = R(0.0);
-    if (!c.C.llinear) {
-      const int le = jl == 1 ? 2 : jl;  // zeta_x(:,:,1) = zeta_x(:,:,2)
-      z = zeta_x_circ(c, s, j, jk, le) / AT(c.g.area_dual_x, j, jk, le);
-    }
-    R e = c.C.lcorio ? z + c.g.fvec_x[LG_IX(c.G, j, jk, 1)] : z;
-    AT(c.w.eta_x, ji, jk, jl) = e / pv_den_h(c, s, wrapm(j, ny), jk, j, jk, jl);
+    if (!c.C.llinear) z = rel_vort_z(c, s, ji, jk, jl) / AT(c.g.area_dual_z, ji, jk, jl);
+    const R e = c.C.lcorio ? z + c.g.fvec_z[LG_IX(c.G, ji, jk, 1)] : z;
+    ez = e / (R(0.25) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl) + rhog(c, s, jm, jk, jl) + rhog(c, s, jm, km, jl)));
   }
-  if (ji <= ny) {  // eta_y; column nx+1 is the copy of column 1
-    const int k = (jk == nx + 1) ? 1 : jk;
-    R z = R(0.0);
-    if (!c.C.llinear) {
-      const int le = jl == 1 ? 2 : jl;
-      z = zeta_y_circ(c, s, ji, k, le) / AT(c.g.area_dual_y, ji, k, le);
-    }
-    R e = c.C.lcorio ? z + c.g.fvec_y[LG_IX(c.G, ji, k, 1)] : z;
-    AT(c.w.eta_y, ji, jk, jl) = e / pv_den_h(c, s, ji, wrapm(k, nx), ji, k, jl);
+  AT(c.w.eta_x, ji, jk, jl) = ex;
+  AT(c.w.eta_y, ji, jk, jl) = ey;
+  if (jl <= nl) AT(c.w.eta_z, ji, jk, jl) = ez;
+  if (ji == 1) {
+    AT(c.w.eta_x, ny + 1, jk, jl) = ex;
+    if (jl <= nl) AT(c.w.eta_z, ny + 1, jk, jl) = ez;
   }
-  if (jl <= nl) {  // eta_z; duplicates are copies of the canonical point
-    const int j = (ji == ny + 1) ? 1 : ji, k = (jk == nx + 1) ? 1 : jk;
-    const int jm = wrapm(j, ny), km = wrapm(k, nx);
-    R z = R(0.0);
-    if (!c.C.llinear) z = rel_vort_z(c, s, j, k, jl) / AT(c.g.area_dual_z, j, k, jl);
-    R e = c.C.lcorio ? z + c.g.fvec_z[LG_IX(c.G, j, k, 1)] : z;
-    AT(c.w.eta_z, ji, jk, jl) =
-        e / (R(0.25) * (rhog(c, s, j, km, jl) + rhog(c, s, j, k, jl) + rhog(c, s, jm, k, jl) + rhog(c, s, jm, km, jl)));
+  if (jk == 1) {
+    AT(c.w.eta_y, ji, nx + 1, jl) = ey;
+    if (jl <= nl) AT(c.w.eta_z, ji, nx + 1, jl) = ez;
   }
+  if (ji == 1 && jk == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = ez;
 }
 
-// mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80)
+// mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80).  The
+// u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1; the
+// threads of column 1 and row 1 store them, and the launch covers the nx by ny cells.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nlev) return;
-  if (ji <= ny && jl <= nl) {
-    const int k = (jk == nx + 1) ? 1 : jk, km = wrapm(k, nx);
-    AT(c.w.u_ph, ji, jk, jl) = R(0.5) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, k, jl)) * ATR(s.wind_u, ji, k, jl);
+  if (jk > nx || ji > ny || jl > nlev) return;
+  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
+  R fu = R(0.0), fv = R(0.0), fw = R(0.0);
+  if (jl <= nl) {
+    fu = R(0.5) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_u, ji, jk, jl);
+    fv = R(0.5) * (rhog(c, s, jm, jk, jl) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_v, ji, jk, jl);
   }
-  if (jk <= nx && jl <= nl) {
-    const int j = (ji == ny + 1) ? 1 : ji, jm = wrapm(j, ny);
-    AT(c.w.v_ph, ji, jk, jl) = R(0.5) * (rhog(c, s, jm, jk, jl) + rhog(c, s, j, jk, jl)) * ATR(s.wind_v, j, jk, jl);
+  if (jl >= 2 && jl <= nl) {
+    fw = R(0.5) * (rhog(c, s, ji, jk, jl - 1) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_w, ji, jk, jl);
+  } else if (jl == nlev && c.C.lfreeslip) {
+    const R dv = rhog(c, s, ji, jk, nl) + (rhog(c, s, ji, jk, nl - 1) - rhog(c, s, ji, jk, nl)) /
+                                              (AT(c.g.z_scalar, ji, jk, nl - 1) - AT(c.g.z_scalar, ji, jk, nl)) *
+                                              (AT(c.g.z_w, ji, jk, nlev) - AT(c.g.z_scalar, ji, jk, nl));
+    fw = dv * ATR(s.wind_w, ji, jk, nlev);
   }
-  if (ji <= ny && jk <= nx) {
-    R r = R(0.0);
-    if (jl >= 2 && jl <= nl) {
-      r = R(0.5) * (rhog(c, s, ji, jk, jl - 1) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_w, ji, jk, jl);
-    } else if (jl == nlev && c.C.lfreeslip) {
-      const R dv = rhog(c, s, ji, jk, nl) + (rhog(c, s, ji, jk, nl - 1) - rhog(c, s, ji, jk, nl)) /
-                                                     (AT(c.g.z_scalar, ji, jk, nl - 1) - AT(c.g.z_scalar, ji, jk, nl)) *
-                                                     (AT(c.g.z_w, ji, jk, nlev) - AT(c.g.z_scalar, ji, jk, nl));
-      r = dv * ATR(s.wind_w, ji, jk, nlev);
-    }
-    AT(c.w.w_ph, ji, jk, jl) = r;
+  if (jl <= nl) {
+    AT(c.w.u_ph, ji, jk, jl) = fu;
+    AT(c.w.v_ph, ji, jk, jl) = fv;
+    if (jk == 1) AT(c.w.u_ph, ji, nx + 1, jl) = fu;
+    if (ji == 1) AT(c.w.v_ph, ny + 1, jk, jl) = fv;
   }
+  AT(c.w.w_ph, ji, jk, jl) = fw;
 }
 
 // The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
@@ -621,37 +595,44 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
 // scalar_tend_expl, mo_scalar_tend_expl.F90:24-181
 // ---------------------------------------------------------------------------------------------
 
-// gas mass flux with the new winds and the theta_v flux at u and v points (copies at nx+1, ny+1)
+// gas mass flux with the new winds and the theta_v flux at u and v points.  Column nx+1 and row
+// ny+1 are copies of column 1 and row 1; the threads of column 1 and row 1 store them, and the
+// launch covers the nx by ny cells.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_scalar_flux(const CtxT<R> c, const StateDev ss, const StateDev nw) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
+  if (jk > nx || ji > ny || jl > nl) return;
   const bool o3 = c.C.order == 3;
   const Fld<const double> rg = ss.rho[c.G.ncond];
-  if (ji <= ny) {
-    const int k = (jk == nx + 1) ? 1 : jk, km = wrapm(k, nx);
-    const R f = R(0.5) * (ATR(rg, ji, km, jl) + ATR(rg, ji, k, jl)) * ATR(nw.wind_u, ji, k, jl);
-    R th;
-    if (o3)
-      th = AT(c.w.tu[0], ji, k, jl) * f;
-    else
-      th = R(0.5) * ((AT(c.g.theta_v_bg, ji, km, jl) + ATR(ss.theta_v_pert, ji, km, jl)) +
-                  (AT(c.g.theta_v_bg, ji, k, jl) + ATR(ss.theta_v_pert, ji, k, jl))) * f;
-    AT(c.w.fu, ji, jk, jl) = f;
-    AT(c.w.thu, ji, jk, jl) = th;
+  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
+  const R rC = ATR(rg, ji, jk, jl);
+  const R thC = AT(c.g.theta_v_bg, ji, jk, jl) + ATR(ss.theta_v_pert, ji, jk, jl);
+  // u point
+  const R fu = R(0.5) * (ATR(rg, ji, km, jl) + rC) * ATR(nw.wind_u, ji, jk, jl);
+  R thu;
+  if (o3)
+    thu = AT(c.w.tu[0], ji, jk, jl) * fu;
+  else
+    thu = R(0.5) * ((AT(c.g.theta_v_bg, ji, km, jl) + ATR(ss.theta_v_pert, ji, km, jl)) + thC) * fu;
+  // v point
+  const R fv = R(0.5) * (ATR(rg, jm, jk, jl) + rC) * ATR(nw.wind_v, ji, jk, jl);
+  R thv;
+  if (o3)
+    thv = AT(c.w.tv[0], ji, jk, jl) * fv;
+  else
+    thv = R(0.5) * ((AT(c.g.theta_v_bg, jm, jk, jl) + ATR(ss.theta_v_pert, jm, jk, jl)) + thC) * fv;
+  AT(c.w.fu, ji, jk, jl) = fu;
+  AT(c.w.thu, ji, jk, jl) = thu;
+  AT(c.w.fv, ji, jk, jl) = fv;
+  AT(c.w.thv, ji, jk, jl) = thv;
+  if (jk == 1) {
+    AT(c.w.fu, ji, nx + 1, jl) = fu;
+    AT(c.w.thu, ji, nx + 1, jl) = thu;
   }
-  if (jk <= nx) {
-    const int j = (ji == ny + 1) ? 1 : ji, jm = wrapm(j, ny);
-    const R f = R(0.5) * (ATR(rg, jm, jk, jl) + ATR(rg, j, jk, jl)) * ATR(nw.wind_v, j, jk, jl);
-    R th;
-    if (o3)
-      th = AT(c.w.tv[0], j, jk, jl) * f;
-    else
-      th = R(0.5) * ((AT(c.g.theta_v_bg, jm, jk, jl) + ATR(ss.theta_v_pert, jm, jk, jl)) +
-                  (AT(c.g.theta_v_bg, j, jk, jl) + ATR(ss.theta_v_pert, j, jk, jl))) * f;
-    AT(c.w.fv, ji, jk, jl) = f;
-    AT(c.w.thv, ji, jk, jl) = th;
+  if (ji == 1) {
+    AT(c.w.fv, ny + 1, jk, jl) = fv;
+    AT(c.w.thv, ny + 1, jk, jl) = thv;
   }
 }
 
