@@ -562,7 +562,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   constexpr int NGRID = 40;
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4;
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
   h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
@@ -644,6 +644,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     W.fv = take();
     W.thu = take();
     W.thv = take();
+    W.decel = take();
+    W.pgc = take();
     for (int q = 0; q < 12; ++q) W.col[q] = col[q];
     W.vfall_cw = col[12];
   };

@@ -40,17 +40,26 @@ __device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int 
 // ---------------------------------------------------------------------------------------------
 // temperature_diagnostics, mo_derived.F90:18-38
 // ---------------------------------------------------------------------------------------------
-// With moisture, also the sink velocity of cloud water in each layer, which depends only on this
-// state and the temperature; the generalized-density solver of both substeps reads it.
+// Also the per-cell factors of this state that later kernels of the step read at several cells
+// and in both substeps: the condensate deceleration of the pressure gradient C_D_P*rho_gas/rho_sum
+// (mo_p_grad.F90:74-104), and with moisture rho_gas/rho_sum - 1 of calc_p_grad_condensates_v
+// (mo_p_grad.F90:106-120) and the sink velocity of cloud water, which depends only on this state
+// and the temperature (the generalized-density solver of both substeps reads it).
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const StateDev s) {
   KIDX;
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   const ix_t i = LG_IX(c.G, ji, jk, jl);
+  const int ncond = c.G.ncond;
   R t = (c.g.theta_v_bg[i] + R(s.theta_v_pert[i])) * (c.g.exner_bg[i] + R(s.exner_pert[i]));
-  if (c.C.lmoist) t = t / (R(1.0) + R(s.rho[c.G.ncond + 1][i]) / R(s.rho[c.G.ncond][i]) * (R(M_D / M_V) - R(1.0)));
+  if (c.C.lmoist) t = t / (R(1.0) + R(s.rho[ncond + 1][i]) / R(s.rho[ncond][i]) * (R(M_D / M_V) - R(1.0)));
   c.w.temperature[i] = t;
-  if (c.C.lmoist) c.w.vfall_cw[i] = v_fall_liquid(c, s, ji, jk, jl, (double)t, CLOUD_WATER_RADIUS);
+  const R rs = rho_sum(c, s, i), rg = R(s.rho[ncond][i]);
+  c.w.decel[i] = R(C_D_P) * rg / rs;
+  if (c.C.lmoist) {
+    c.w.pgc[i] = rg / rs - R(1.0);
+    c.w.vfall_cw[i] = v_fall_liquid(c, s, ji, jk, jl, (double)t, CLOUD_WATER_RADIUS);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -160,11 +169,8 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
     }
   }
   // theta_v, full and perturbation, and the deceleration factors
-  auto decel_at = [&](ix_t ix) {
-    R r = R(0.0);
-    for (int jc = 0; jc <= c.G.ncond; ++jc) r = r + R(s.rho[jc][ix]);
-    return R(C_D_P) * R(s.rho[c.G.ncond][ix]) / r;
-  };
+  // C_D_P*rho_gas/rho_sum of a cell, formed with the temperature at the start of the step
+  auto decel_at = [&](ix_t ix) { return c.w.decel[ix]; };
   const R tpC = R(tp[iC]), thC = c.g.theta_v_bg[iC] + tpC, dC = decel_at(iC);
   // u point
   {
@@ -583,8 +589,8 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
     R x = -cur_ver * (c.w.pnl_w[i] + c.w.pl_w[i]) - R(0.5) * c.w.vsg_z[i] + c.w.pvt_z[i];
     if (c.C.lmoist) {  // calc_p_grad_condensates_v, mo_p_grad.F90:106-120 (identically 0 when dry)
       const ix_t ia = LG_IX(c.G, ji, jk, jl - 1);
-      const R da = R(old.rho[c.G.ncond][ia]) / rho_sum(c, old, ia) - R(1.0);
-      const R db = R(old.rho[c.G.ncond][i]) / rho_sum(c, old, i) - R(1.0);
+      const R da = c.w.pgc[ia];  // rho_gas/rho_sum - 1 of the old state, from k_temperature
+      const R db = c.w.pgc[i];
       x = x + R(0.5) * (da + db) * c.g.gravity_m_v[i];
     }
     c.w.tend_w[i] = old_w * c.w.tend_w[i] + new_w * x;
