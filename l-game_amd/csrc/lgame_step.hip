@@ -527,17 +527,75 @@ __global__ void __launch_bounds__(BX * BY) k_ke(const CtxT<R> c, const StateDev 
   AT(c.w.v_squared, ji, jk, jl) = ke_point(c, s, ji, jk, jl);
 }
 
+// grad_hor and grad_vert of the kinetic energy at the u, v and w points of one cell (ji <= ny,
+// jk <= nx): the values and vertical gradients that the three share are loaded or formed once and
+// combined with the expressions of grad_hor_x/_y and remap_ver2hor_x/_y, in their order.
+template <class R>
+__device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int jk, int jl) {
+  const Geom &G = c.G;
+  const int nx = G.nx, ny = G.ny, nl = G.nl, nlev = G.nlev, nflat = G.nflat;
+  const Fld<const R> k2 = c.w.v_squared;
+  const ix_t iC = LG_IX(G, ji, jk, jl), iW = LG_IX(G, ji, wrapm(jk, nx), jl), iS = LG_IX(G, wrapm(ji, ny), jk, jl);
+  const ix_t dl = (ix_t)G.lvl;
+  const bool terr = jl > nflat, below = jl < nl || c.C.lfreeslip;
+  const R kC = k2[iC], kW = k2[iW], kS = k2[iS];
+  auto vgrad = [&](ix_t ix, R k_here, int l) {  // grad_vert at level l of the column through ix
+    if (l >= nlev || l <= 1) return R(0.0);
+    const ix_t il = ix + (l - jl) * dl;
+    const R up = (l == jl) ? k2[il - dl] : k_here;
+    const R lo = (l == jl) ? k_here : k2[il];
+    return (up - lo) / c.g.dz[il];
+  };
+  R vC = R(0.0), vW = R(0.0), vS = R(0.0), vC1 = R(0.0), vW1 = R(0.0), vS1 = R(0.0);
+  if (terr || jl >= 2) vC = vgrad(iC, kC, jl);
+  if (terr) {
+    vW = vgrad(iW, kW, jl);
+    vS = vgrad(iS, kS, jl);
+    if (below) {
+      vC1 = vgrad(iC, kC, jl + 1);
+      vW1 = vgrad(iW, kW, jl + 1);
+      vS1 = vgrad(iS, kS, jl + 1);
+    }
+  }
+  R gx = (kC - kW) / c.g.dx[iC];
+  R gy = (kS - kC) / c.g.dy[iC];
+  if (terr) {
+    R r = R(0.0);
+    r = r + c.g.ipw[4][iW] * vW;
+    r = r + c.g.ipw[4][iC] * vC;
+    if (below) {
+      r = r + c.g.ipw[5][iW] * vW1;
+      r = r + c.g.ipw[5][iC] * vC1;
+    }
+    gx = gx - c.g.slope_x[iC] * (R(0.5) * r);
+    r = R(0.0);
+    r = r + c.g.ipw[4][iS] * vS;
+    r = r + c.g.ipw[4][iC] * vC;
+    if (below) {
+      r = r + c.g.ipw[5][iS] * vS1;
+      r = r + c.g.ipw[5][iC] * vC1;
+    }
+    gy = gy - c.g.slope_y[iC] * (R(0.5) * r);
+  }
+  c.w.vsg_x[iC] = gx;
+  c.w.vsg_y[iC] = gy;
+  if (jl >= 2) c.w.vsg_z[iC] = vC;
+}
+
 // grad_vert + grad_hor of the kinetic energy (mo_vector_tend_expl.F90:96-97)
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
+  if (ji <= ny && jk <= nx) {
+    ke_grad_interior(c, ji, jk, jl);
+    return;
+  }
+  // the edge points: u at jk = nx+1 (own terrain metrics) and v at ji = ny+1
   const Fld<const R> k2 = c.w.v_squared;
   if (ji <= ny) AT(c.w.vsg_x, ji, jk, jl) = grad_hor_x(c, k2, ji, jk, jl);
   if (jk <= nx) AT(c.w.vsg_y, ji, jk, jl) = grad_hor_y(c, k2, ji, jk, jl);
-  if (ji <= ny && jk <= nx && jl >= 2)
-    AT(c.w.vsg_z, ji, jk, jl) = (AT(k2, ji, jk, jl - 1) - AT(k2, ji, jk, jl)) / AT(c.g.dz, ji, jk, jl);
 }
 
 // simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377 (momentum diffusion off: only the
