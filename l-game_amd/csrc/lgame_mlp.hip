@@ -2,16 +2,16 @@
 // accumulation, GELU, mean-squared-error backward pass and FP32 Adam master weights.
 //
 // Network: n_in -> hidden x n_hidden_layers (GELU) -> n_out.  Dimensions are padded to multiples
-// of 64 in device storage (zero weights, zero inputs), so every GEMM is a whole number of 64x64
-// output tiles with a K loop in steps of 32.  Weights are stored [out][in] (in contiguous), the
-// activations [column][feature].  Inputs and targets are normalised with per-feature means and
-// scales that the caller sets (lgame_mlp_set_norm); the network works on normalised values.
+// of 64 in device storage (zero weights, zero inputs).  Weights are stored [out][in] (in
+// contiguous) with a BF16 copy in that layout and a transposed one; activations [column][feature]
+// and, for training, also [feature][column].  Inputs and targets are normalised with
+// per-feature means and scales that the caller sets (lgame_mlp_set_norm).
 //
-// GEMM: one 256-thread block per 64x64 tile of C; each wavefront owns a 32x32 quarter and
-// computes it with 2x2 v_mfma_f32_16x16x16_bf16 tiles.  A and B tiles of 32 k-values are staged
-// in LDS with k contiguous, so the MFMA operands of a lane (4 consecutive k) are one 8-byte LDS
-// read.  Operands can be k-contiguous or m/n-contiguous in memory (forward, data gradient and
-// weight gradient use different ones); the global loads are arranged to be contiguous either way.
+// GEMM: every product is C = A B^T with both operands contiguous along k (the forward pass, the
+// data gradient through the transposed weights, the weight gradient through the transposed
+// activations and output gradients, which the epilogues write), computed by k_gemm_nt with
+// gfx950's v_mfma_f32_16x16x32_bf16.  The weight gradient, whose k is the batch, is split along
+// k and the partial sums are added in split order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,9 +26,6 @@
 #include "lgame_mlp.h"
 
 namespace lgame {
-
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
   uint32_t u;
@@ -49,103 +46,181 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));  // 16 bytes, kept in registers
+
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2, EPI_DGELU = 3 };
 
-struct GemmArgs {
-  const uint16_t *A;  // A(m,k) = A[m*sam + k*sak]
-  long long sam, sak;
-  const uint16_t *B;  // B(k,n) = B[k*sbk + n*sbn]
-  long long sbk, sbn;
+// C(m,n) = sum_k A[m*lda + k] * B[n*ldb + k] (both operands k-contiguous), over the K values
+// [z*K, (z+1)*K) of split z of a split-K launch.  Epilogues:
+//   EPI_F32       C[z*split_stride + m*ldc + n] = acc                       (weight gradient partials)
+//   EPI_BIAS_F32  C[m*ldc + n] = acc + bias[n]                              (output layer)
+//   EPI_BIAS_GELU z = acc + bias[n]; Z[m*ldz + n] = z (if Z), A[m*ldz + n] = gelu(z), and the
+//                 transposed AT[n*ldt + m] = gelu(z) (if AT)                (hidden layers)
+//   EPI_DGELU     d = acc * gelu'(Zin[m*ldz + n]); D[m*ldz + n] = d, DT[n*ldt + m] = d (data gradient)
+struct NtArgs {
+  const uint16_t *A;
+  long long lda;
+  const uint16_t *B;
+  long long ldb;
   int M, N, K;
-  float *C;            // EPI_F32 / EPI_BIAS_F32: C[m*ldc + n]
-  long long ldc;
-  const float *bias;   // per n
-  uint16_t *Z, *Aout;  // EPI_BIAS_GELU: pre-activation and activation, [m*ldz + n]
-  const uint16_t *Zin; // EPI_DGELU: pre-activation of the layer
-  uint16_t *D;         // EPI_DGELU: output gradient [m*ldz + n]
-  long long ldz;
+  float *C;
+  long long ldc, split_stride;
+  const float *bias;
+  uint16_t *Z, *Aout, *AT;
+  const uint16_t *Zin;
+  uint16_t *D, *DT;
+  long long ldz, ldt;
 };
 
-constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;  // LDS row: 32 k values + 8 padding
+constexpr int MAX_SPLITS = 16;  // split-K limit of the weight gradients
+constexpr int GK = 64;          // k values per split of the weight gradients: a multiple of the stage depth
 
-template <int EPI>
-__global__ void __launch_bounds__(256) k_gemm(const GemmArgs g) {
-  __shared__ uint16_t As[TM][LDK];
-  __shared__ uint16_t Bs[TN][LDK];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  float4_t acc[2][2];
+// Block tile BM x BN with 4 wavefronts in a 2 x 2 arrangement; a wavefront computes (BM/2) x
+// (BN/2) as (BM/32) x (BN/32) v_mfma_f32_16x16x32_bf16 tiles.  K goes in stages of BK values
+// staged in LDS (rows of BK + 8 values: the 16-B operand reads of a wavefront hit distinct
+// banks); the next stage is loaded from global memory into registers with 16-B loads while the
+// current one is multiplied.  Deep stages (BK = 128) keep the number of exposed load latencies
+// per block small.
+template <int EPI, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) k_gemm_nt(const NtArgs g) {
+  constexpr int LD = BK + 8;
+  constexpr int TMW = BM / 32, TNW = BN / 32;  // MFMA tiles per wavefront along m and n
+  constexpr int CPR = BK / 8;                  // 16-B chunks per row of a stage
+  constexpr int CA = BM * CPR / 256, CB = BN * CPR / 256;  // chunks per thread
+  constexpr int RS = 256 / CPR;                // rows between a thread's chunks
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // Tile of this block.  The 1-D grid is dealt round-robin to the 8 XCDs; XCD k takes a
+  // contiguous range of tiles in (split, m, n) order, so the blocks that share an A tile (the n
+  // tiles of one m tile) run on one XCD, whose L2 then reads that tile from memory once.
+  int bx, by, bz;
+  {
+    const int nb = gridDim.x, b = blockIdx.x, nxb = g.N / BN, nyb = g.M / BM;
+    const int k = b % 8, s = b / 8, q = nb / 8, r = nb % 8;
+    const int tt = k * q + (k < r ? k : r) + s;
+    bx = tt % nxb;
+    by = (tt / nxb) % nyb;
+    bz = tt / (nxb * nyb);
+  }
+  const int m0 = by * BM, n0 = bx * BN;
+  const long long kz = (long long)bz * g.K;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  float4_t acc[TMW][TNW];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TMW; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const bool a_kc = g.sak == 1, b_kc = g.sbk == 1;
-  for (int k0 = 0; k0 < g.K; k0 += TK) {
-    // A tile: 64 m x 32 k, 8 elements per thread, contiguous in memory
-    if (a_kc) {
-      const int m = t >> 2, kk = (t & 3) * 8;
-      const uint16_t *src = g.A + (long long)(m0 + m) * g.sam + (k0 + kk);
+    for (int b = 0; b < TNW; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+  // this thread's chunks of a stage: row crow + RS*q, k values ckc..ckc+7
+  const int crow = t / CPR, ckc = (t % CPR) * 8;
+  const uint16_t *pa = g.A + (long long)(m0 + crow) * g.lda + kz + ckc;
+  const uint16_t *pb = g.B + (long long)(n0 + crow) * g.ldb + kz + ckc;
+  const long long sa = RS * g.lda, sb = RS * g.ldb;
+  u32x4_t ra[CA], rb[CB];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[m][kk + q] = src[q];
-    } else {
-      const int kk = t >> 3, mm = (t & 7) * 8;
-      const uint16_t *src = g.A + (long long)(k0 + kk) * g.sak + (long long)(m0 + mm) * g.sam;
+  for (int q = 0; q < CA; ++q) ra[q] = *reinterpret_cast<const u32x4_t *>(pa + q * sa);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[mm + q][kk] = src[(long long)q * g.sam];
-    }
-    if (b_kc) {
-      const int n = t >> 2, kk = (t & 3) * 8;
-      const uint16_t *src = g.B + (long long)(n0 + n) * g.sbn + (k0 + kk);
+  for (int q = 0; q < CB; ++q) rb[q] = *reinterpret_cast<const u32x4_t *>(pb + q * sb);
+  for (int kt = 0; kt < g.K; kt += BK) {
+    __syncthreads();  // the previous stage has been read
 #pragma unroll
-      for (int q = 0; q < 8; ++q) Bs[n][kk + q] = src[q];
-    } else {
-      const int kk = t >> 3, nn = (t & 7) * 8;
-      const uint16_t *src = g.B + (long long)(k0 + kk) * g.sbk + (long long)(n0 + nn) * g.sbn;
+    for (int q = 0; q < CA; ++q) *reinterpret_cast<u32x4_t *>(&As[(crow + RS * q) * LD + ckc]) = ra[q];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) Bs[nn + q][kk] = src[(long long)q * g.sbn];
-    }
+    for (int q = 0; q < CB; ++q) *reinterpret_cast<u32x4_t *>(&Bs[(crow + RS * q) * LD + ckc]) = rb[q];
     __syncthreads();
+    if (kt + BK < g.K) {  // the next stage, in flight while this one is multiplied
+      pa += BK;
+      pb += BK;
 #pragma unroll
-    for (int ks = 0; ks < TK; ks += 16) {
-      // lane layout of v_mfma_f32_16x16x16_bf16: row/column index lane % 16, k = 4*(lane / 16) + j
-      const int r = lane & 15, kq = ks + 4 * (lane >> 4);
-      short4_t av[2], bv[2];
+      for (int q = 0; q < CA; ++q) ra[q] = *reinterpret_cast<const u32x4_t *>(pa + q * sa);
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = *reinterpret_cast<const short4_t *>(&As[wm + 16 * a + r][kq]);
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const short4_t *>(&Bs[wn + 16 * b + r][kq]);
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+      for (int q = 0; q < CB; ++q) rb[q] = *reinterpret_cast<const u32x4_t *>(pb + q * sb);
     }
-    __syncthreads();
+    // lane layout of v_mfma_f32_16x16x32_bf16: row (A) / column (B) lane % 16, k = 8*(lane/16) + 0..7
+    const int r = lane & 15, kq = 8 * (lane >> 4);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 32) {
+      bf16x8_t av[TMW], bv[TNW];
+#pragma unroll
+      for (int a = 0; a < TMW; ++a) av[a] = *reinterpret_cast<const bf16x8_t *>(&As[(wm + 16 * a + r) * LD + ks + kq]);
+#pragma unroll
+      for (int b = 0; b < TNW; ++b) bv[b] = *reinterpret_cast<const bf16x8_t *>(&Bs[(wn + 16 * b + r) * LD + ks + kq]);
+#pragma unroll
+      for (int a = 0; a < TMW; ++a)
+#pragma unroll
+        for (int b = 0; b < TNW; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
   }
   // result layout: lane holds rows 4*(lane/16) + j (j = 0..3) of column lane % 16
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TMW; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < TNW; ++b) {
+      const int mb = m0 + wm + 16 * a + 4 * (lane >> 4);
+      const int n = n0 + wn + 16 * b + (lane & 15);
+      uint16_t tr[4];  // the lane's four values for the transposed copy (consecutive m)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = m0 + wm + 16 * a + 4 * (lane >> 4) + j;
-        const int n = n0 + wn + 16 * b + (lane & 15);
+        const long long m = mb + j;
         const float v = acc[a][b][j];
         if (EPI == EPI_F32) {
-          g.C[(long long)m * g.ldc + n] = v;
+          g.C[(long long)bz * g.split_stride + m * g.ldc + n] = v;
         } else if (EPI == EPI_BIAS_F32) {
-          g.C[(long long)m * g.ldc + n] = v + g.bias[n];
+          g.C[m * g.ldc + n] = v + g.bias[n];
         } else if (EPI == EPI_BIAS_GELU) {
           const float z = v + g.bias[n];
-          g.Z[(long long)m * g.ldz + n] = f2bf(z);
-          g.Aout[(long long)m * g.ldz + n] = f2bf(gelu(z));
+          if (g.Z) g.Z[m * g.ldz + n] = f2bf(z);
+          const uint16_t h = f2bf(gelu(z));
+          g.Aout[m * g.ldz + n] = h;
+          tr[j] = h;
         } else {
-          const float z = bf2f(g.Zin[(long long)m * g.ldz + n]);
-          g.D[(long long)m * g.ldz + n] = f2bf(v * gelu_grad(z));
+          const float z = bf2f(g.Zin[m * g.ldz + n]);
+          const uint16_t h = f2bf(v * gelu_grad(z));
+          g.D[m * g.ldz + n] = h;
+          tr[j] = h;
         }
       }
+      uint16_t *T = EPI == EPI_BIAS_GELU ? g.AT : EPI == EPI_DGELU ? g.DT : nullptr;
+      if (T) {
+        uint2 pk;
+        pk.x = (uint32_t)tr[0] | ((uint32_t)tr[1] << 16);
+        pk.y = (uint32_t)tr[2] | ((uint32_t)tr[3] << 16);
+        *reinterpret_cast<uint2 *>(T + (long long)n * g.ldt + mb) = pk;
+      }
+    }
+}
+
+// sum of the split-K partials in split order: out[i] = sum_z part[z*n + i]
+__global__ void k_sum_splits(const float *part, int splits, long long n, float *out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int z = 0; z < splits; ++z) s += part[(long long)z * n + i];
+  out[i] = s;
+}
+
+// row sums of a [rows][ld] BF16 matrix over its first `len` columns (one block per row, fixed
+// reduction tree): the bias gradient from the transposed output gradient
+__global__ void __launch_bounds__(256) k_rowsum(const uint16_t *T, long long ld, int len, float *out) {
+  __shared__ float sh[256];
+  const uint16_t *row = T + (long long)blockIdx.x * ld;
+  float s = 0.0f;
+  // 8 values (16 B) per load; len and ld are multiples of 64
+  for (int i = 8 * threadIdx.x; i < len; i += 8 * 256) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(row + i);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s += bf2f((uint16_t)(w[q] & 0xffffu)) + bf2f((uint16_t)(w[q] >> 16));
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
 // normalised BF16 inputs, padded: X[c][i] = (x - mean)/scale for i < n_in, 0 above
@@ -175,10 +250,10 @@ __global__ void k_norm_out(const float *y, long long ncol, int n_out, int ldo, c
   Yn[idx] = o < n_out ? (y[c * n_out + o] - mean[o]) / scale[o] : 0.0f;
 }
 
-// mean-squared-error gradient of a batch: dY = 2 (yhat - y) / (B * n_out) on the valid outputs;
-// per-block partial sums of the squared error
+// mean-squared-error gradient of a batch: dY = 2 (yhat - y) / (B * n_out) on the valid outputs,
+// also transposed; per-block partial sums of the squared error
 __global__ void __launch_bounds__(256) k_mse_grad(const float *yhat, const float *y, const int *rows, int B, int n_out, int ldo,
-                                                  uint16_t *dY, float *partial) {
+                                                  uint16_t *dY, uint16_t *dYT, float *partial) {
   __shared__ float sh[256];
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float se = 0.0f;
@@ -192,6 +267,7 @@ __global__ void __launch_bounds__(256) k_mse_grad(const float *yhat, const float
       d = 2.0f * e / (float)((long long)B * n_out);
     }
     dY[idx] = f2bf(d);
+    dYT[(long long)o * B + b] = f2bf(d);  // transposed copy [ldo][B] for the weight gradient
   }
   sh[threadIdx.x] = se;
   __syncthreads();
@@ -202,13 +278,19 @@ __global__ void __launch_bounds__(256) k_mse_grad(const float *yhat, const float
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// column sums of a [B][ld] BF16 gradient: the bias gradient (FP32)
-__global__ void k_colsum(const uint16_t *D, int B, int ld, float *out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ld) return;
-  float s = 0.0f;
-  for (int b = 0; b < B; ++b) s += bf2f(D[(long long)b * ld + n]);
-  out[n] = s;
+// dst[c*ldd + r] = src[r*ld + c] for r < rows, c < cols (BF16), through 32x32 LDS tiles
+__global__ void __launch_bounds__(256) k_transpose16(const uint16_t *src, long long ld, int rows, int cols, uint16_t *dst, long long ldd) {
+  __shared__ uint16_t tile[32][34];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = src[(long long)r * ld + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) dst[(long long)c * ldd + r] = tile[tx][j];
+  }
 }
 
 // gather a batch of rows of the normalised data set
@@ -220,9 +302,9 @@ __global__ void k_gather(const uint16_t *X, const int *rows, int B, int ld, uint
 }
 
 // Adam on the FP32 master weights of one parameter block (padded [rows][ld] layout; entries
-// outside the valid rows/columns stay zero), then the BF16 copy
-__global__ void k_adam(float *w, float *m, float *v, const float *grad, uint16_t *wbf, long long n, int ld, int valid_cols,
-                       int valid_rows, float lr, float b1, float b2, float eps, float c1, float c2) {
+// outside the valid rows/columns stay zero), then the BF16 copies [rows][ld] and [ld][ldt]
+__global__ void k_adam(float *w, float *m, float *v, const float *grad, uint16_t *wbf, uint16_t *wbfT, long long ldt, long long n,
+                       int ld, int valid_cols, int valid_rows, float lr, float b1, float b2, float eps, float c1, float c2) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int col = (int)(i % ld);
@@ -235,12 +317,19 @@ __global__ void k_adam(float *w, float *m, float *v, const float *grad, uint16_t
   v[i] = vi;
   const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   w[i] = wi;
-  if (wbf) wbf[i] = f2bf(wi);
+  if (wbf) {
+    wbf[i] = f2bf(wi);
+    wbfT[(long long)col * ldt + row] = f2bf(wi);
+  }
 }
 
-__global__ void k_to_bf16(const float *w, uint16_t *wbf, long long n) {
+// BF16 copies of a [rows][ld] FP32 weight block: as it is and transposed ([ld][rows])
+__global__ void k_to_bf16(const float *w, uint16_t *wbf, uint16_t *wbfT, int rows, int ld) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) wbf[i] = f2bf(w[i]);
+  if (i >= (long long)rows * ld) return;
+  const long long row = i / ld, col = i % ld;
+  wbf[i] = f2bf(w[i]);
+  wbfT[col * rows + row] = f2bf(w[i]);
 }
 
 }  // namespace lgame
@@ -256,7 +345,7 @@ struct MlpLayer {
   float *w = nullptr, *b = nullptr;          // FP32 master [out_p][in_p], [out_p]
   float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;  // Adam moments
   float *gw = nullptr, *gb = nullptr;        // gradients
-  uint16_t *wbf = nullptr;                   // BF16 copy
+  uint16_t *wbf = nullptr, *wbfT = nullptr;  // BF16 copies [out_p][in_p] and [in_p][out_p]
 };
 
 struct lgame_mlp {
@@ -265,8 +354,9 @@ struct lgame_mlp {
   float *x_mean = nullptr, *x_scale = nullptr, *y_mean = nullptr, *y_scale = nullptr;
   // activations for up to `cap` rows: per layer Z and A (BF16), gradient buffers
   long long cap = 0;
-  std::vector<uint16_t *> Z, A;
-  uint16_t *X = nullptr, *dA = nullptr, *dB = nullptr, *dYb = nullptr;
+  std::vector<uint16_t *> Z, A, AT;  // AT: activations transposed, [feature][row]
+  uint16_t *X = nullptr, *XT = nullptr, *dA = nullptr, *dB = nullptr, *dYb = nullptr, *dTa = nullptr, *dTb = nullptr;
+  float *gpart = nullptr;  // split-K partials of a weight gradient
   float *Yf = nullptr, *Yt = nullptr, *partial = nullptr, *xin = nullptr, *yout = nullptr;
   int *rows = nullptr;
   // data set (normalised, device)
@@ -303,25 +393,34 @@ int ensure_cap(lgame_mlp *m, long long rows) {
   if (rows <= m->cap) return 0;
   for (auto p : m->Z) (void)hipFree(p);
   for (auto p : m->A) (void)hipFree(p);
+  for (auto p : m->AT) (void)hipFree(p);
   m->Z.clear();
   m->A.clear();
-  for (void *p : {(void *)m->X, (void *)m->dA, (void *)m->dB, (void *)m->dYb, (void *)m->Yf, (void *)m->Yt, (void *)m->partial,
-                  (void *)m->xin, (void *)m->yout, (void *)m->rows})
+  m->AT.clear();
+  for (void *p : {(void *)m->X, (void *)m->XT, (void *)m->dA, (void *)m->dB, (void *)m->dYb, (void *)m->dTa, (void *)m->dTb,
+                  (void *)m->gpart, (void *)m->Yf, (void *)m->Yt, (void *)m->partial, (void *)m->xin, (void *)m->yout,
+                  (void *)m->rows})
     if (p) (void)hipFree(p);
   const int hp = pad64(m->hidden), ip = pad64(m->n_in), op = pad64(m->n_out);
   const int wmax = hp > ip ? hp : ip;
   m->cap = rows;
   for (int l = 0; l < m->nh; ++l) {
-    uint16_t *z, *a;
+    uint16_t *z, *a, *at;
     MCHK(m, hipMalloc(&z, (size_t)rows * hp * 2));
     MCHK(m, hipMalloc(&a, (size_t)rows * hp * 2));
+    MCHK(m, hipMalloc(&at, (size_t)rows * hp * 2));
     m->Z.push_back(z);
     m->A.push_back(a);
+    m->AT.push_back(at);
   }
   MCHK(m, hipMalloc(&m->X, (size_t)rows * ip * 2));
   MCHK(m, hipMemset(m->X, 0, (size_t)rows * ip * 2));
+  MCHK(m, hipMalloc(&m->XT, (size_t)rows * ip * 2));
   MCHK(m, hipMalloc(&m->dA, (size_t)rows * wmax * 2));
   MCHK(m, hipMalloc(&m->dB, (size_t)rows * wmax * 2));
+  MCHK(m, hipMalloc(&m->dTa, (size_t)rows * wmax * 2));
+  MCHK(m, hipMalloc(&m->dTb, (size_t)rows * wmax * 2));
+  MCHK(m, hipMalloc(&m->gpart, (size_t)MAX_SPLITS * wmax * wmax * 4));
   MCHK(m, hipMalloc(&m->dYb, (size_t)rows * op * 2));
   MCHK(m, hipMalloc(&m->Yf, (size_t)rows * op * 4));
   MCHK(m, hipMalloc(&m->Yt, (size_t)rows * op * 4));
@@ -332,82 +431,120 @@ int ensure_cap(lgame_mlp *m, long long rows) {
   return 0;
 }
 
-template <int EPI>
-void gemm(hipStream_t st, const GemmArgs &g) {
-  hipLaunchKernelGGL(k_gemm<EPI>, dim3(g.N / TN, g.M / TM), dim3(256), 0, st, g);
+template <int EPI, int BK>
+void gemm_launch(hipStream_t st, const NtArgs &g, dim3 grid, bool m128, bool n128) {
+  if (m128 && n128)
+    hipLaunchKernelGGL((k_gemm_nt<EPI, 128, 128, BK>), grid, dim3(256), 0, st, g);
+  else if (m128)
+    hipLaunchKernelGGL((k_gemm_nt<EPI, 128, 64, BK>), grid, dim3(256), 0, st, g);
+  else if (n128)
+    hipLaunchKernelGGL((k_gemm_nt<EPI, 64, 128, BK>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((k_gemm_nt<EPI, 64, 64, BK>), grid, dim3(256), 0, st, g);
 }
 
-// forward pass of B normalised rows X (padded) -> Yf (FP32, padded), keeping Z and A
-void forward(lgame_mlp *m, const uint16_t *X, int B) {
+// C = A B^T with both operands k-contiguous; M, N multiples of 64, K (per split) of 64.  The
+// largest tiles that still give at least 4 blocks per CU (1024 blocks): a stage's global loads
+// are in flight for one stage of multiplication only, so the other blocks of a CU hide them.
+template <int EPI>
+void gemm_nt(hipStream_t st, const NtArgs &g, int splits = 1) {
+  const long long want = 1024;
+  auto nblocks = [&](int bm, int bn) { return (long long)(g.M / bm) * (g.N / bn) * splits; };
+  bool m128 = g.M % 128 == 0, n128 = g.N % 128 == 0;
+  if (m128 && n128 && nblocks(128, 128) < want) {  // drop to a 64-wide side first
+    if (nblocks(128, 64) >= want || nblocks(64, 128) >= want) {
+      if (nblocks(128, 64) >= want) n128 = false; else m128 = false;
+    } else {
+      m128 = n128 = false;
+    }
+  } else if ((m128 != n128) && nblocks(m128 ? 128 : 64, n128 ? 128 : 64) < want) {
+    m128 = n128 = false;
+  }
+  const dim3 grid((unsigned)((g.N / (n128 ? 128 : 64)) * (g.M / (m128 ? 128 : 64)) * splits));  // tiles, see k_gemm_nt
+  if (g.K % 128 == 0)
+    gemm_launch<EPI, 128>(st, g, grid, m128, n128);
+  else
+    gemm_launch<EPI, 64>(st, g, grid, m128, n128);
+}
+
+// forward pass of B normalised rows X (padded) -> Yf (FP32, padded); for training it keeps the
+// pre-activations Z and the activations in both layouts (A [row][feature], AT [feature][row])
+void forward(lgame_mlp *m, const uint16_t *X, int B, bool train) {
   const uint16_t *in = X;
   int in_p = pad64(m->n_in);
   for (int l = 0; l <= m->nh; ++l) {
     const MlpLayer &Ly = m->L[l];
-    GemmArgs g{};
+    NtArgs g{};
     g.A = in;
-    g.sam = in_p;
-    g.sak = 1;
+    g.lda = in_p;
     g.B = Ly.wbf;
-    g.sbk = 1;
-    g.sbn = Ly.in_p;
+    g.ldb = Ly.in_p;
     g.M = B;
     g.N = Ly.out_p;
     g.K = Ly.in_p;
     g.bias = Ly.b;
     if (l < m->nh) {
-      g.Z = m->Z[l];
+      g.Z = train ? m->Z[l] : nullptr;
       g.Aout = m->A[l];
+      g.AT = train ? m->AT[l] : nullptr;
       g.ldz = Ly.out_p;
-      gemm<EPI_BIAS_GELU>(m->stream, g);
+      g.ldt = B;
+      gemm_nt<EPI_BIAS_GELU>(m->stream, g);
       in = m->A[l];
       in_p = Ly.out_p;
     } else {
       g.C = m->Yf;
       g.ldc = Ly.out_p;
-      gemm<EPI_BIAS_F32>(m->stream, g);
+      gemm_nt<EPI_BIAS_F32>(m->stream, g);
     }
   }
 }
 
-// backward pass from dYb (BF16, [B][out_p]) with the activations of the last forward pass
-void backward(lgame_mlp *m, const uint16_t *X, int B) {
-  const uint16_t *dcur = m->dYb;
+// backward pass from the output gradient (dYb [B][out_p] and dTa = its transpose) with the
+// activations of the last training forward pass; XT is the transposed input
+void backward(lgame_mlp *m, const uint16_t *XT, int B) {
+  const uint16_t *dcur = m->dYb, *dcurT = m->dTa;
+  // split-K of the weight gradients (K = B): enough blocks to fill the chip, >= 256 k values each
+  int splits = 1;
+  while (splits < MAX_SPLITS && B / (2 * splits) >= 256 && (B / (2 * splits)) % GK == 0) splits *= 2;
   for (int l = m->nh; l >= 0; --l) {
     MlpLayer &Ly = m->L[l];
-    const uint16_t *in = l == 0 ? X : m->A[l - 1];
-    // weight gradient: gW[o][i] = sum_b d[b][o] * in[b][i]
-    GemmArgs gw{};
-    gw.A = dcur;
-    gw.sam = 1;
-    gw.sak = Ly.out_p;
-    gw.B = in;
-    gw.sbk = Ly.in_p;
-    gw.sbn = 1;
+    const uint16_t *inT = l == 0 ? XT : m->AT[l - 1];
+    // weight gradient: gW[o][i] = sum_b d[b][o] * in[b][i] = sum_b dT[o][b] * inT[i][b]
+    NtArgs gw{};
+    gw.A = dcurT;
+    gw.lda = B;
+    gw.B = inT;
+    gw.ldb = B;
     gw.M = Ly.out_p;
     gw.N = Ly.in_p;
-    gw.K = B;
-    gw.C = Ly.gw;
+    gw.K = B / splits;
+    gw.C = m->gpart;
     gw.ldc = Ly.in_p;
-    gemm<EPI_F32>(m->stream, gw);
-    hipLaunchKernelGGL(k_colsum, dim3(nblk(Ly.out_p)), dim3(256), 0, m->stream, dcur, B, Ly.out_p, Ly.gb);
+    gw.split_stride = (long long)Ly.out_p * Ly.in_p;
+    gemm_nt<EPI_F32>(m->stream, gw, splits);
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(gw.split_stride)), dim3(256), 0, m->stream, m->gpart, splits, gw.split_stride, Ly.gw);
+    hipLaunchKernelGGL(k_rowsum, dim3(Ly.out_p), dim3(256), 0, m->stream, dcurT, (long long)B, B, Ly.gb);
     if (l == 0) break;
     // data gradient through the GELU of layer l-1: d_prev[b][i] = (sum_o d[b][o] W[o][i]) gelu'(Z)
     uint16_t *dnext = (dcur == m->dA) ? m->dB : m->dA;
-    GemmArgs gd{};
+    uint16_t *dnextT = (dcurT == m->dTa) ? m->dTb : m->dTa;
+    NtArgs gd{};
     gd.A = dcur;
-    gd.sam = Ly.out_p;
-    gd.sak = 1;
-    gd.B = Ly.wbf;
-    gd.sbk = Ly.in_p;
-    gd.sbn = 1;
+    gd.lda = Ly.out_p;
+    gd.B = Ly.wbfT;
+    gd.ldb = Ly.out_p;
     gd.M = B;
     gd.N = Ly.in_p;
     gd.K = Ly.out_p;
     gd.Zin = m->Z[l - 1];
     gd.D = dnext;
+    gd.DT = dnextT;
     gd.ldz = Ly.in_p;
-    gemm<EPI_DGELU>(m->stream, gd);
+    gd.ldt = B;
+    gemm_nt<EPI_DGELU>(m->stream, gd);
     dcur = dnext;
+    dcurT = dnextT;
   }
 }
 
@@ -439,7 +576,7 @@ int mlp_infer_device(lgame_mlp *m, const float *x, long long ncol, float *y, hip
     // rows beyond nc are left from earlier chunks: finite, and their outputs are not copied
     hipLaunchKernelGGL(k_norm_in, dim3(nblk(nc * ip)), dim3(256), 0, stream, x + c0 * m->n_in, nc, m->n_in, ip, m->x_mean,
                        m->x_scale, m->X);
-    forward(m, m->X, B);
+    forward(m, m->X, B, false);
     hipLaunchKernelGGL(k_denorm_out, dim3(nblk(nc * m->n_out)), dim3(256), 0, stream, m->Yf, nc, m->n_out, op, m->y_mean,
                        m->y_scale, y + c0 * m->n_out);
   }
@@ -491,6 +628,7 @@ int lgame_mlp_create(int n_in, int hidden, int n_hidden_layers, int n_out, unsig
     for (float **p : {&Ly.w, &Ly.mw, &Ly.vw, &Ly.gw}) ok = ok && hipMalloc(p, nw * 4) == hipSuccess;
     for (float **p : {&Ly.b, &Ly.mb, &Ly.vb, &Ly.gb}) ok = ok && hipMalloc(p, nb * 4) == hipSuccess;
     ok = ok && hipMalloc(&Ly.wbf, nw * 2) == hipSuccess;
+    ok = ok && hipMalloc(&Ly.wbfT, nw * 2) == hipSuccess;
     ok = ok && hipMemcpy(Ly.w, w.data(), nw * 4, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(Ly.b, b.data(), nb * 4, hipMemcpyHostToDevice) == hipSuccess;
     for (float *p : {Ly.mw, Ly.vw, Ly.gw}) ok = ok && hipMemset(p, 0, nw * 4) == hipSuccess;
@@ -500,7 +638,7 @@ int lgame_mlp_create(int n_in, int hidden, int n_hidden_layers, int n_out, unsig
       lgame_mlp_destroy(m);
       return LGAME_ENOMEM;
     }
-    hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nw)), dim3(256), 0, m->stream, Ly.w, Ly.wbf, (long long)nw);
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nw)), dim3(256), 0, m->stream, Ly.w, Ly.wbf, Ly.wbfT, Ly.out_p, Ly.in_p);
   }
   bool ok = true;
   for (float **p : {&m->x_mean, &m->x_scale}) ok = ok && hipMalloc(p, (size_t)n_in * 4) == hipSuccess;
@@ -553,7 +691,8 @@ static int params_io(lgame_mlp *m, float *p, long long n, bool set) {
     if (set) {
       MCHK(m, hipMemcpy(Ly.w, w.data(), w.size() * 4, hipMemcpyHostToDevice));
       MCHK(m, hipMemcpy(Ly.b, b.data(), b.size() * 4, hipMemcpyHostToDevice));
-      hipLaunchKernelGGL(k_to_bf16, dim3(nblk((long long)w.size())), dim3(256), 0, m->stream, Ly.w, Ly.wbf, (long long)w.size());
+      hipLaunchKernelGGL(k_to_bf16, dim3(nblk((long long)w.size())), dim3(256), 0, m->stream, Ly.w, Ly.wbf, Ly.wbfT, Ly.out_p,
+                         Ly.in_p);
       MCHK(m, hipStreamSynchronize(m->stream));
     }
   }
@@ -605,10 +744,12 @@ int lgame_mlp_gradient(lgame_mlp *m, const float *x, const float *y, long long n
   MCHK(m, hipMemcpyAsync(dy, y, (size_t)n * m->n_out * 4, hipMemcpyHostToDevice, m->stream));
   hipLaunchKernelGGL(k_norm_in, dim3(nblk(n * ip)), dim3(256), 0, m->stream, m->xin, n, m->n_in, ip, m->x_mean, m->x_scale, m->X);
   hipLaunchKernelGGL(k_norm_out, dim3(nblk(n * op)), dim3(256), 0, m->stream, dy, n, m->n_out, op, m->y_mean, m->y_scale, m->Yt);
-  forward(m, m->X, (int)n);
+  hipLaunchKernelGGL(k_transpose16, dim3(ip / 32, (unsigned)((n + 31) / 32)), dim3(256), 0, m->stream, m->X, (long long)ip, (int)n, ip,
+                     m->XT, (long long)n);
+  forward(m, m->X, (int)n, true);
   hipLaunchKernelGGL(k_mse_grad, dim3(nblk(n * op)), dim3(256), 0, m->stream, m->Yf, m->Yt, (const int *)nullptr, (int)n, m->n_out,
-                     op, m->dYb, m->partial);
-  backward(m, m->X, (int)n);
+                     op, m->dYb, m->dTa, m->partial);
+  backward(m, m->XT, (int)n);
   *loss = loss_sum(m, (int)n) / (float)(n * m->n_out);
   (void)hipFree(dy);
   long long off = 0;
@@ -684,18 +825,21 @@ int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long
     for (long long b = 0; b < nb; ++b) {
       const int *rows = drows + b * batch;
       hipLaunchKernelGGL(k_gather, dim3(nblk((long long)batch * ip)), dim3(256), 0, m->stream, m->DX, rows, batch, ip, m->X);
-      forward(m, m->X, batch);
+      hipLaunchKernelGGL(k_transpose16, dim3(ip / 32, batch / 32), dim3(256), 0, m->stream, m->X, (long long)ip, batch, ip, m->XT,
+                         (long long)batch);
+      forward(m, m->X, batch, true);
       hipLaunchKernelGGL(k_mse_grad, dim3(nblk((long long)batch * op)), dim3(256), 0, m->stream, m->Yf, m->DY, rows, batch,
-                         m->n_out, op, m->dYb, m->partial);
-      backward(m, m->X, batch);
+                         m->n_out, op, m->dYb, m->dTa, m->partial);
+      backward(m, m->XT, batch);
       ++m->adam_t;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
       for (auto &Ly : m->L) {
         const long long nw = (long long)Ly.in_p * Ly.out_p;
-        hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(256), 0, m->stream, Ly.w, Ly.mw, Ly.vw, (const float *)Ly.gw, Ly.wbf, nw,
-                           Ly.in_p, Ly.in, Ly.out, lr, b1, b2, eps, c1, c2);
+        hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(256), 0, m->stream, Ly.w, Ly.mw, Ly.vw, (const float *)Ly.gw, Ly.wbf, Ly.wbfT,
+                           (long long)Ly.out_p, nw, Ly.in_p, Ly.in, Ly.out, lr, b1, b2, eps, c1, c2);
         hipLaunchKernelGGL(k_adam, dim3(nblk(Ly.out_p)), dim3(256), 0, m->stream, Ly.b, Ly.mb, Ly.vb, (const float *)Ly.gb,
-                           (uint16_t *)nullptr, (long long)Ly.out_p, Ly.out_p, Ly.out, 1, lr, b1, b2, eps, c1, c2);
+                           (uint16_t *)nullptr, (uint16_t *)nullptr, 0LL, (long long)Ly.out_p, Ly.out_p, Ly.out, 1, lr, b1, b2, eps, c1,
+                           c2);
       }
       if (b % 16 == 15 || b == nb - 1) {  // read the loss now and then (keeps the queue short)
         MCHK(m, hipMemcpyAsync(part.data(), m->partial, part.size() * 4, hipMemcpyDeviceToHost, m->stream));
@@ -726,13 +870,15 @@ void lgame_mlp_destroy(lgame_mlp *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (auto &Ly : m->L)
     for (void *p : {(void *)Ly.w, (void *)Ly.b, (void *)Ly.mw, (void *)Ly.vw, (void *)Ly.mb, (void *)Ly.vb, (void *)Ly.gw,
-                    (void *)Ly.gb, (void *)Ly.wbf})
+                    (void *)Ly.gb, (void *)Ly.wbf, (void *)Ly.wbfT})
       if (p) (void)hipFree(p);
   for (auto p : m->Z) (void)hipFree(p);
   for (auto p : m->A) (void)hipFree(p);
-  for (void *p : {(void *)m->X, (void *)m->dA, (void *)m->dB, (void *)m->dYb, (void *)m->Yf, (void *)m->Yt, (void *)m->partial,
-                  (void *)m->xin, (void *)m->yout, (void *)m->rows, (void *)m->DX, (void *)m->DY, (void *)m->x_mean,
-                  (void *)m->x_scale, (void *)m->y_mean, (void *)m->y_scale})
+  for (auto p : m->AT) (void)hipFree(p);
+  for (void *p : {(void *)m->X, (void *)m->XT, (void *)m->dA, (void *)m->dB, (void *)m->dYb, (void *)m->dTa, (void *)m->dTb,
+                  (void *)m->gpart, (void *)m->Yf, (void *)m->Yt, (void *)m->partial, (void *)m->xin, (void *)m->yout,
+                  (void *)m->rows, (void *)m->DX, (void *)m->DY, (void *)m->x_mean, (void *)m->x_scale, (void *)m->y_mean,
+                  (void *)m->y_scale})
     if (p) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
