@@ -196,7 +196,8 @@ def bench_training(args):
     """The surrogate training run: harvest columns from the model, train with BF16 MFMA GEMMs."""
     from lgame import surrogate
 
-    m, norm, rep = surrogate.build(states=args.train_states, epochs=args.train_epochs, batch=args.train_batch)
+    m, norm, rep = surrogate.build(states=args.train_states, epochs=args.train_epochs, batch=args.train_batch,
+                                   lr=args.train_lr, progress=log)
     if args.save_surrogate:
         surrogate.save(args.save_surrogate, m, norm)
         log(f"saved {args.save_surrogate}")
@@ -309,7 +310,8 @@ def main():
                     help="first run one device-to-device copy of BYTES (for HBM counter scaling)")
     ap.add_argument("--train-states", type=int, default=24)
     ap.add_argument("--train-epochs", type=int, default=60)
-    ap.add_argument("--train-batch", type=int, default=512)
+    ap.add_argument("--train-batch", type=int, default=8192)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--save-surrogate", default="")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -278,6 +278,20 @@ __global__ void __launch_bounds__(256) k_mse_grad(const float *yhat, const float
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// acc += sum of the per-block squared-error partials of one batch (one block, fixed order)
+__global__ void __launch_bounds__(256) k_loss_add(const float *partial, int n, float *acc) {
+  __shared__ float sh[256];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += sh[0];
+}
+
 // dst[c*ldd + r] = src[r*ld + c] for r < rows, c < cols (BF16), through 32x32 LDS tiles
 __global__ void __launch_bounds__(256) k_transpose16(const uint16_t *src, long long ld, int rows, int cols, uint16_t *dst, long long ldd) {
   __shared__ uint16_t tile[32][34];
@@ -794,24 +808,32 @@ int lgame_mlp_load_data(lgame_mlp *m, const float *x, const float *y, long long 
 }
 
 // Adam training on the loaded data set: `epochs` passes in shuffled batches of `batch` rows (a
-// multiple of 64); loss[e] receives the mean squared error of epoch e.  *seconds (if non-null)
-// the device time of the run.
+// multiple of 64); loss[e] receives the mean squared error of epoch e over all its batches (each
+// taken before its update).  *seconds (if non-null) the device time of the run.
 int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long long seed, float *loss, double *seconds) {
   if (!m || epochs < 1 || batch < 64 || batch % 64 || !loss) return LGAME_EINVAL;
   if (m->ndata < batch) return mfail(m, LGAME_ESTATE, "load at least one batch of data first");
   if (ensure_cap(m, batch)) return mfail(m, LGAME_ENOMEM, "activation buffers");
   const int ip = pad64(m->n_in), op = pad64(m->n_out);
   const long long nb = m->ndata / batch;
-  std::vector<int> perm(m->ndata);
+  const int nparts = (int)nblk((long long)batch * op);
   unsigned long long st = seed * 0x9E3779B97F4A7C15ull + 1;
-  hipEvent_t e0, e1;
+  // The permutation of the next epoch is drawn on the host while the device trains the current
+  // one: two pinned host buffers and two device copies.  The squared error is summed on the
+  // device, batch by batch in order, so nothing waits for the device within the run.
+  int *hperm[2] = {nullptr, nullptr}, *drows[2] = {nullptr, nullptr};
+  float *dloss = nullptr;
+  hipEvent_t e0, e1, copied[2];
   MCHK(m, hipEventCreate(&e0));
   MCHK(m, hipEventCreate(&e1));
-  int *drows;
-  MCHK(m, hipMalloc(&drows, (size_t)m->ndata * 4));
-  MCHK(m, hipEventRecord(e0, m->stream));
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int ep = 0; ep < epochs; ++ep) {
+  for (int q = 0; q < 2; ++q) {
+    MCHK(m, hipHostMalloc(&hperm[q], (size_t)m->ndata * 4, hipHostMallocDefault));
+    MCHK(m, hipMalloc(&drows[q], (size_t)m->ndata * 4));
+    MCHK(m, hipEventCreate(&copied[q]));
+  }
+  MCHK(m, hipMalloc(&dloss, (size_t)epochs * 4));
+  MCHK(m, hipMemsetAsync(dloss, 0, (size_t)epochs * 4, m->stream));
+  auto shuffle = [&](int *perm) {
     for (long long i = 0; i < m->ndata; ++i) perm[i] = (int)i;
     for (long long i = m->ndata - 1; i > 0; --i) {
       st ^= st << 13;
@@ -819,17 +841,23 @@ int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long
       st ^= st << 17;
       std::swap(perm[i], perm[st % (unsigned long long)(i + 1)]);
     }
-    MCHK(m, hipMemcpyAsync(drows, perm.data(), (size_t)m->ndata * 4, hipMemcpyHostToDevice, m->stream));
-    double se = 0.0;
-    std::vector<float> part(nblk((long long)batch * op));
+  };
+  shuffle(hperm[0]);
+  MCHK(m, hipMemcpyAsync(drows[0], hperm[0], (size_t)m->ndata * 4, hipMemcpyHostToDevice, m->stream));
+  MCHK(m, hipEventRecord(copied[0], m->stream));
+  MCHK(m, hipEventRecord(e0, m->stream));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    const int cur = ep % 2;
     for (long long b = 0; b < nb; ++b) {
-      const int *rows = drows + b * batch;
+      const int *rows = drows[cur] + b * batch;
       hipLaunchKernelGGL(k_gather, dim3(nblk((long long)batch * ip)), dim3(256), 0, m->stream, m->DX, rows, batch, ip, m->X);
       hipLaunchKernelGGL(k_transpose16, dim3(ip / 32, batch / 32), dim3(256), 0, m->stream, m->X, (long long)ip, batch, ip, m->XT,
                          (long long)batch);
       forward(m, m->X, batch, true);
-      hipLaunchKernelGGL(k_mse_grad, dim3(nblk((long long)batch * op)), dim3(256), 0, m->stream, m->Yf, m->DY, rows, batch,
-                         m->n_out, op, m->dYb, m->dTa, m->partial);
+      hipLaunchKernelGGL(k_mse_grad, dim3(nparts), dim3(256), 0, m->stream, m->Yf, m->DY, rows, batch, m->n_out, op, m->dYb, m->dTa,
+                         m->partial);
+      hipLaunchKernelGGL(k_loss_add, dim3(1), dim3(256), 0, m->stream, (const float *)m->partial, nparts, dloss + ep);
       backward(m, m->XT, batch);
       ++m->adam_t;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
@@ -841,24 +869,31 @@ int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long
                            (uint16_t *)nullptr, (uint16_t *)nullptr, 0LL, (long long)Ly.out_p, Ly.out_p, Ly.out, 1, lr, b1, b2, eps, c1,
                            c2);
       }
-      if (b % 16 == 15 || b == nb - 1) {  // read the loss now and then (keeps the queue short)
-        MCHK(m, hipMemcpyAsync(part.data(), m->partial, part.size() * 4, hipMemcpyDeviceToHost, m->stream));
-        MCHK(m, hipStreamSynchronize(m->stream));
-        for (float v : part) se += v;
-      }
     }
-    (void)se;
-    // the loss of the epoch: exact mean over the last batch of every 16 (cheap estimate)
-    loss[ep] = (float)(se / ((double)((nb + 15) / 16) * batch * m->n_out));
+    if (ep + 1 < epochs) {  // the next epoch's rows, drawn while this epoch runs
+      const int nxt = (ep + 1) % 2;
+      MCHK(m, hipEventSynchronize(copied[nxt]));  // its buffer's previous copy has completed
+      shuffle(hperm[nxt]);
+      MCHK(m, hipMemcpyAsync(drows[nxt], hperm[nxt], (size_t)m->ndata * 4, hipMemcpyHostToDevice, m->stream));
+      MCHK(m, hipEventRecord(copied[nxt], m->stream));
+    }
   }
   MCHK(m, hipEventRecord(e1, m->stream));
   MCHK(m, hipEventSynchronize(e1));
   float ms = 0.0f;
   MCHK(m, hipEventElapsedTime(&ms, e0, e1));
   if (seconds) *seconds = ms * 1e-3;
+  std::vector<float> se(epochs);
+  MCHK(m, hipMemcpy(se.data(), dloss, (size_t)epochs * 4, hipMemcpyDeviceToHost));
+  for (int ep = 0; ep < epochs; ++ep) loss[ep] = (float)((double)se[ep] / ((double)nb * batch * m->n_out));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(drows);
+  for (int q = 0; q < 2; ++q) {
+    (void)hipEventDestroy(copied[q]);
+    (void)hipHostFree(hperm[q]);
+    (void)hipFree(drows[q]);
+  }
+  (void)hipFree(dloss);
   MCHK(m, hipGetLastError());
   return LGAME_OK;
 }

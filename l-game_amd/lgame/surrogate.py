@@ -20,7 +20,7 @@ HIDDEN, N_HIDDEN = 512, 3
 T_EPOCH = 1576843200.0
 
 
-def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_max=0.01):
+def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_max=0.01, progress=None):
     """Columns (x, y) from runs of the moist n x n x 60 case with the solver as radiation.
 
     Each run starts from a seeded perturbed state whose cloud condensates, surface albedo and
@@ -62,6 +62,8 @@ def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_m
             s.run(step, steps_between, t_init=t0)
             step += steps_between
         s.close()
+        if progress and (k + 1) % 128 == 0:
+            progress(f"harvested {k + 1} of {n_states} states")
     return np.concatenate(xs), np.concatenate(ys)
 
 
@@ -106,12 +108,12 @@ def load(path) -> Mlp:
     return m
 
 
-def build(states=24, calls=2, times=6, epochs=60, batch=512, lr=1e-3, seed=0, test_states=3):
+def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, test_states=3, progress=None):
     """Harvest, train and evaluate; returns (mlp, norm, report)."""
     import time
 
     t0 = time.perf_counter()
-    xtr, ytr = harvest(n=33, n_states=states, calls=calls, times=times, seed=seed)
+    xtr, ytr = harvest(n=33, n_states=states, calls=calls, times=times, seed=seed, progress=progress)
     xte, yte = harvest(n=33, n_states=test_states, calls=1, times=times, seed=seed + 1000)
     t_harvest = time.perf_counter() - t0
     norm = fit_norm(xtr, ytr)
@@ -141,7 +143,7 @@ if __name__ == "__main__":
     ap.add_argument("--out", required=True)
     ap.add_argument("--states", type=int, default=24)
     ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
