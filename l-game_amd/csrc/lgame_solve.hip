@@ -107,54 +107,47 @@ __device__ __forceinline__ Iface make_iface(const Layer &a, const Layer &b, cons
 
 // inputs of one layer of the back substitution, loaded one layer ahead
 struct BackIn {
-  double V, rhoe, rte, ga, rio, ep_o, rt_o, tb;  // layer Lr
-  double rp_u, ep_u, tin_u;                      // row Lr-1 of the system (interface Lr)
-  double az_b, wo_b;                             // interface Lr+1 (below the layer)
+  double V, rho_o, trho, rt_o, trt, ep_o, tb, eb_or_ga;  // layer Lr (exner_bg at rk 1, scratch ga at rk 2)
+  double rp_u, ep_u, tin_u;                              // row Lr-1 of the system (tin from scratch at rk 2)
+  double az_b, wo_b;                                     // interface Lr+1 (below the layer)
 };
 
-// three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off)
-template <class R>
-__global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, g = c.G.ncond;
-  if (col >= nx * ny) return;
-  const int jk = col % nx + 1, ji = col / nx + 1;
-  const StateDev &nu = rk == 1 ? old : nw;
+// three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off).
+//
+// The back substitution needs, per layer, rho_e, rhotheta_e, gamma and, per row, rho at the
+// interface and theta_v at the interface.  It forms again, with the same expressions, those it
+// can from values it loads anyway (rho_old, rhotheta_old, Exner', the volume): rho_e and
+// rhotheta_e from the old densities and the tendencies, the interface rho from the rolling rho of
+// the layer below, and at the predictor also gamma and the interface theta_v from
+// rhotheta_old/rho_old.  Only the eliminated coefficients (and at the corrector gamma and the
+// interface theta_v, which need the predictor state) go through the column scratch.
+template <class R, int RK>
+__device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int ji, int jk) {
+  const int nl = c.G.nl, g = c.G.ncond;
+  const StateDev &nu = RK == 1 ? old : nw;
   const double dt = c.C.dtime;
-  const Fld<double> sep = c.w.col[0], srp = c.w.col[1], srhoe = c.w.col[2], srte = c.w.col[3];
-  const Fld<double> stin = c.w.col[4], sga = c.w.col[5], srio = c.w.col[6];
+  const Fld<double> sep = c.w.col[0], srp = c.w.col[1], stin = c.w.col[4], sga = c.w.col[5];
   const double K2 = IMPL_THERMO_WEIGHT * (dt * dt);
   const double K1 = IMPL_THERMO_WEIGHT * dt * C_D_P;
 
   // forward sweep: rows q = 1..nl-1 of the interface mass-flux system
-  Layer L0 = make_layer(load_raw(c, old, nu, rk, LG_IX(c.G, ji, jk, 1)), rk, dt);
-  const RawLayer raw1 = load_raw(c, old, nu, rk, LG_IX(c.G, ji, jk, 2));
+  Layer L0 = make_layer(load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, 1)), RK, dt);
+  const RawLayer raw1 = load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, 2));
   RawLayer nxt = raw1;  // layer q+2 for the first row
-  if (nl >= 3) nxt = load_raw(c, old, nu, rk, LG_IX(c.G, ji, jk, 3));
-  Layer L1 = make_layer(raw1, rk, dt);
+  if (nl >= 3) nxt = load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, 3));
+  Layer L1 = make_layer(raw1, RK, dt);
   Iface F0 = make_iface(L0, L1, raw1);
   Layer L2 = L1;
   Iface F1 = F0;
   double ep_prev = 0.0, rp_prev = 0.0, c_prev = 0.0;
-  {
-    const ix_t i0 = LG_IX(c.G, ji, jk, 1);
-    srhoe[i0] = L0.rhoe;
-    srte[i0] = L0.rte;
-    sga[i0] = L0.ga;
-  }
+  if (RK == 2) sga[LG_IX(c.G, ji, jk, 1)] = L0.ga;
   for (int q = 1; q <= nl - 1; ++q) {
     const bool has_next = q + 2 <= nl;
     const RawLayer cur = nxt;  // layer q+2
-    if (q + 3 <= nl) nxt = load_raw(c, old, nu, rk, LG_IX(c.G, ji, jk, q + 3));
-    // layer q+1 = L1 (stored for the back substitution); layer q+2 and interface q+1 if present
-    {
-      const ix_t i1 = LG_IX(c.G, ji, jk, q + 1);
-      srhoe[i1] = L1.rhoe;
-      srte[i1] = L1.rte;
-      sga[i1] = L1.ga;
-    }
+    if (q + 3 <= nl) nxt = load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, q + 3));
+    if (RK == 2) sga[LG_IX(c.G, ji, jk, q + 1)] = L1.ga;  // layer q+1
     if (has_next) {
-      L2 = make_layer(cur, rk, dt);
+      L2 = make_layer(cur, RK, dt);
       F1 = make_iface(L1, L2, cur);
     }
     const double dzs = L0.z - L1.z;
@@ -182,8 +175,7 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
     const ix_t iq = LG_IX(c.G, ji, jk, q);
     sep[iq] = epq;
     srp[iq] = rpq;
-    stin[iq] = F0.tin;
-    srio[iq] = F0.rio;
+    if (RK == 2) stin[iq] = F0.tin;
     ep_prev = epq;
     rp_prev = rpq;
     c_prev = cq;
@@ -197,19 +189,19 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
     BackIn b;
     const ix_t i = LG_IX(c.G, ji, jk, Lr);
     b.V = c.g64.volume[i];
-    b.rhoe = srhoe[i];
-    b.rte = srte[i];
-    b.ga = sga[i];
-    b.rio = Lr < nl ? srio[i] : 0.0;
-    b.ep_o = old.exner_pert[i];
+    b.rho_o = old.rho[g][i];
+    b.trho = (double)c.w.tend_rho[g][i];
     b.rt_o = old.rhotheta_v[i];
+    b.trt = (double)c.w.tend_rhotheta_v[i];
+    b.ep_o = old.exner_pert[i];
     b.tb = c.g64.theta_v_bg[i];
+    b.eb_or_ga = RK == 1 ? c.g64.exner_bg[i] : sga[i];
     b.rp_u = b.ep_u = b.tin_u = 0.0;
     if (Lr >= 2) {
       const ix_t iu = LG_IX(c.G, ji, jk, Lr - 1);
       b.rp_u = srp[iu];
       b.ep_u = sep[iu];
-      b.tin_u = stin[iu];
+      if (RK == 2) b.tin_u = stin[iu];
     }
     b.az_b = b.wo_b = 0.0;
     if (Lr < nl) {
@@ -221,40 +213,65 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
   };
   double x_low = 0.0;  // x(Lr) of the row below the current layer
   double tin_low = 0.0;
-  double rho_below = 0.0;
+  double rho_below = 0.0;    // new rho of the layer below
+  double rho_o_below = 0.0;  // old rho of the layer below
   BackIn bn = load_back(nl);
+  double ratio_cur = RK == 1 ? bn.rt_o / bn.rho_o : 0.0;  // rhotheta/rho of layer Lr (predictor)
   for (int Lr = nl; Lr >= 1; --Lr) {
     const BackIn b = bn;
     if (Lr >= 2) bn = load_back(Lr - 1);
-    const double az_b = b.az_b, wo_b = b.wo_b;
+    // the layer and row quantities of the forward sweep (make_layer, make_iface)
+    const double rhoe = b.rho_o + dt * b.trho;
+    const double rte = b.rt_o + dt * b.trt;
+    const double ga = RK == 1 ? R_D / (C_D_V * b.rt_o) * (b.eb_or_ga + b.ep_o) / b.V : b.eb_or_ga;
+    double tin_u = b.tin_u;  // theta_v at interface Lr
+    if (RK == 1 && Lr >= 2) {
+      const double ratio_up = bn.rt_o / bn.rho_o;
+      tin_u = 0.5 * (ratio_up + ratio_cur);
+      ratio_cur = ratio_up;
+    }
     const ix_t i = LG_IX(c.G, ji, jk, Lr);
     const double V = b.V;
     double rho_new, rt_new;
     double x_up = 0.0;
     if (Lr == nl) {
       x_up = b.rp_u;  // x(nl-1)
-      rho_new = b.rhoe - dt * x_up / V;
-      rt_new = b.rte - dt * b.tin_u * x_up / V;
+      rho_new = rhoe - dt * x_up / V;
+      rt_new = rte - dt * tin_u * x_up / V;
     } else if (Lr >= 2) {
       x_up = b.rp_u - b.ep_u * x_low;
-      rho_new = b.rhoe + dt * (-x_up + x_low) / V;
-      rt_new = b.rte + dt * (-b.tin_u * x_up + tin_low * x_low) / V;
+      rho_new = rhoe + dt * (-x_up + x_low) / V;
+      rt_new = rte + dt * (-tin_u * x_up + tin_low * x_low) / V;
     } else {
-      rho_new = b.rhoe + dt * x_low / V;
-      rt_new = b.rte + dt * tin_low * x_low / V;
+      rho_new = rhoe + dt * x_low / V;
+      rt_new = rte + dt * tin_low * x_low / V;
     }
     nw.rho[g][i] = rho_new;
     nw.rhotheta_v[i] = rt_new;
     if (Lr < nl) {  // w at the interface below this layer, level Lr+1
+      const double rio = 0.5 * (b.rho_o + rho_o_below);  // rho at that interface (row Lr)
       const double rin = 0.5 * (rho_new + rho_below);
-      nw.wind_w[LG_IX(c.G, ji, jk, Lr + 1)] = (2.0 * x_low / az_b - rin * wo_b) / b.rio;
+      nw.wind_w[LG_IX(c.G, ji, jk, Lr + 1)] = (2.0 * x_low / b.az_b - rin * b.wo_b) / rio;
     }
-    nw.exner_pert[i] = b.ep_o + V * b.ga * (rt_new - b.rt_o);
+    nw.exner_pert[i] = b.ep_o + V * ga * (rt_new - b.rt_o);
     nw.theta_v_pert[i] = rt_new / rho_new - b.tb;
     rho_below = rho_new;
+    rho_o_below = b.rho_o;
     x_low = x_up;
-    tin_low = b.tin_u;
+    tin_low = tin_u;
   }
+}
+
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = c.G.nx, ny = c.G.ny;
+  if (col >= nx * ny) return;
+  const int jk = col % nx + 1, ji = col / nx + 1;
+  if (rk == 1)
+    ver_column<R, 1>(c, old, nw, ji, jk);
+  else
+    ver_column<R, 2>(c, old, nw, ji, jk);
 }
 
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
