@@ -431,7 +431,7 @@ struct GenShared {
   double b[GEN_GROUPS][NCS][BCOL];
 };
 
-template <class R, int VF, int CP, bool HEAT>
+template <class R, int VF, int CP, bool HEAT, bool OWN_EP = true>
 __device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old, const StateDev &nw, GenShared &sh, int t, int grp,
                                           int lane, bool active, int ji, int jk, int jc, double vf_const) {
   constexpr bool cond = CP != CP_NONE;
@@ -478,7 +478,7 @@ __device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old,
     const double epq = w.e / den;
     const double rpq = (w.r - rp * cprev) / den;
     const ix_t i = LG_IX(c.G, ji, jk_st, l);
-    sep[i] = epq;
+    if (OWN_EP) sep[i] = epq;
     srp[i] = rpq;
     ep = epq;
     rp = rpq;
@@ -606,10 +606,10 @@ __device__ __forceinline__ void gen_dispatch(const CtxT<R> &c, const StateDev &o
     case 2: gen_sweep<R, VF_CONST, CP_ICE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 3, vf_ice); break;
     case 3: gen_sweep<R, VF_LIQUID, CP_WATER, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 4, 0.0); break;
     case 4:
-      if (HEAT) gen_sweep<R, VF_NONE, CP_ICE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
-      else gen_sweep<R, VF_NONE, CP_NONE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
+      if (HEAT) gen_sweep<R, VF_NONE, CP_ICE, HEAT, false>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
+      else gen_sweep<R, VF_NONE, CP_NONE, HEAT, false>(c, old, nw, sh, t, grp, lane, active, ji, jk, 5, 0.0);
       break;
-    default: gen_sweep<R, VF_NONE, CP_NONE, HEAT>(c, old, nw, sh, t, grp, lane, active, ji, jk, 7, 0.0); break;
+    default: gen_sweep<R, VF_NONE, CP_NONE, HEAT, false>(c, old, nw, sh, t, grp, lane, active, ji, jk, 7, 0.0); break;
   }
 }
 
@@ -631,9 +631,14 @@ __global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const Ctx
     gen_dispatch<R, true>(c, old, nw, sh, t, grp, lane, active, ji, jk);
   else
     gen_dispatch<R, false>(c, old, nw, sh, t, grp, lane, active, ji, jk);
+  // Rain, graupel and vapour have no sink velocity, so their rows have the same d, e and c and
+  // their eliminated coefficients e' are the same bits (the surface sink terms of rain and
+  // graupel add and subtract zero); rain's sweep stores them for all three.
+  __syncthreads();  // (uniform: every wavefront of the block reaches it)
   if (!active) return;
   const int jc = t < NCS ? t + 1 : 7;
-  gen_back(c.G, c.w.col[2 * t + 1], c.w.col[2 * t], nw.rho[jc - 1], ji, jk);
+  const bool shared_ep = t == 4 || t == 5;  // graupel, vapour
+  gen_back(c.G, c.w.col[2 * t + 1], c.w.col[shared_ep ? 2 : 2 * t], nw.rho[jc - 1], ji, jk);
 }
 
 template __global__ void k_solve_ver<double>(const CtxT<double>, const StateDev, const StateDev, int);
