@@ -46,7 +46,7 @@ STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step 
 # HBM bytes per k_solve_ver dispatch, averaged over predictor and corrector, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB device copy gives);
 # profiles/moist512_fp64_hbm_traffic.md holds the commands and the per-kernel table.
-SOLVER_TRAFFIC = {"moist512": 4.929e9}
+SOLVER_TRAFFIC = {"moist512": 4.369e+09}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
 SURROGATE_WEIGHTS = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
 T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
