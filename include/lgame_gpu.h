@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define LGAME_ABI_VERSION 1
+#define LGAME_ABI_VERSION 2
 
 /* return codes */
 #define LGAME_OK          0
@@ -122,6 +122,15 @@ typedef struct lgame_dims {
  * Step configuration.  Names and meaning follow the reference namelist variables:
  * mo_run_nml.F90:13-41, mo_diff_nml.F90, mo_surface_nml.F90, mo_bc_nml.F90, mo_rad_nml.F90.
  * dtime is passed explicitly (the reference derives it from dx, dy at mo_run_nml.F90:93-94).
+ *
+ * Diffusion (mo_diff_nml.F90:8-53; the reference default turns all six switches on, with
+ * diff_coeff_scheme_h "smag", diff_coeff_scheme_v "tke", c_s 0.2, bg_shear 1.5e-5): every
+ * combination is accepted in which horizontal temperature or mass diffusion comes with horizontal
+ * momentum diffusion.  Without it, scalar_diff_coeffs (mo_eff_diff_coeffs.F90:540-590) evaluates
+ * the Smagorinsky coefficient from whatever vorticity and divergence the diagnostics last held and
+ * updates the TKE a second time; lgame_init rejects those combinations with LGAME_EINVAL.
+ * diff_coeff_scheme_v has one valid value in the reference ("tke", mo_diff_nml.F90:95-98), so it
+ * has no field here.
  */
 typedef struct lgame_cfg {
   double dtime;
@@ -131,6 +140,8 @@ typedef struct lgame_cfg {
   double klemp_damp_max;
   double klemp_begin_rel;
   double dtime_rad;
+  double c_s;                /* Smagorinsky parameter                                                  */
+  double bg_shear;           /* minimum background shear [1/s]                                         */
   int theta_adv_order;       /* 2 or 3                                                                 */
   int luse_bg_state;
   int llinear;
@@ -149,6 +160,7 @@ typedef struct lgame_cfg {
   int lrad;
   int precision;             /* LGAME_PREC_*                                                           */
   int rad_mode;              /* LGAME_RAD_*                                                            */
+  int diff_coeff_scheme_h;   /* 0 = "smag", 1 = "tke"                                                  */
 } lgame_cfg;
 
 /*
@@ -186,6 +198,10 @@ typedef struct lgame_grid_view {
   const double *exner_bg;              /* (ny,nx,nl)     */
   const double *gravity_m_v;           /* (ny,nx,nl+1)   */
   const double *gravity_potential;     /* (ny,nx,nl)     */
+  const double *dx_dual;               /* (ny+1,nx,nl)   read only with diffusion */
+  const double *dy_dual;               /* (ny,nx+1,nl)   read only with diffusion */
+  const double *layer_thickness;       /* (ny,nx,nl)     read only with diffusion */
+  double mean_velocity_area;           /* grid%mean_velocity_area, mo_grid_generator.F90:1111 */
 } lgame_grid_view;
 
 /* Prognostic state (t_state, src/mo_definitions.F90:94-105), reference layout. */
@@ -222,7 +238,13 @@ typedef struct lgame_rad_view {
 #define LGAME_INT_MASS0      4   /* masses: slot LGAME_INT_MASS0 + jc-1 for constituent jc  */
 
 /* Diagnostic fields readable with lgame_get_diag (reference t_diag components). */
-#define LGAME_DIAG_TEMPERATURE   0   /* diag%temperature (ny,nx,nl), of the last step's state_old */
+#define LGAME_DIAG_TEMPERATURE     0   /* diag%temperature (ny,nx,nl), of the last step's state_old */
+#define LGAME_DIAG_TKE             1   /* diag%tke (ny,nx,nl)                                       */
+#define LGAME_DIAG_VISCOSITY       2   /* diag%viscosity (ny,nx,nl), density times the coefficient  */
+#define LGAME_DIAG_MOM_DIFF_TEND_X 3   /* diag%mom_diff_tend_x (ny,nx+1,nl)                         */
+#define LGAME_DIAG_MOM_DIFF_TEND_Z 4   /* diag%mom_diff_tend_z (ny,nx,nl+1)                         */
+#define LGAME_DIAG_TEMP_DIFF_HEATING 5 /* diag%temp_diff_heating (ny,nx,nl)                         */
+#define LGAME_DIAG_HEATING_DISS    6   /* diag%heating_diss (ny,nx,nl)                              */
 
 /* Version of this ABI (LGAME_ABI_VERSION). */
 int lgame_abi_version(void);
@@ -308,8 +330,9 @@ int lgame_profile_reset(lgame_handle *h);
  * NULL) receives the choice, in the kernel order of lgame_set_tiles.  No reference counterpart:
  * the reference is a CPU code with OpenMP loops.
  * Kernel order: temperature, theta_fill, theta_grad, theta_div, theta_edge, pgrad, pot_vort,
- * mass_flux, vort_flux, ke, ke_grad, dissipation, vec_tend, scalar_flux, scalar_div. */
-#define LGAME_N_TILES 15
+ * mass_flux, vort_flux, ke, ke_grad, dissipation, vec_tend, scalar_flux, scalar_div, diff_zeta,
+ * diff_coef, diff_dual, diff_mom, diff_flux, diff_div (the last six run only with diffusion). */
+#define LGAME_N_TILES 21
 int lgame_autotune(lgame_handle *h, int steps, int *rows);
 int lgame_set_tiles(lgame_handle *h, const int *rows);
 

@@ -308,8 +308,28 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
         LG_LAUNCH(KI_KE_GRAD, k_ke_grad, nx + 1, ny + 1, nl, c);
       }
     }
-    if (rk == 1 && c.C.lklemp) LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
+    // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
+    if (rk == 1 && c.C.lmom_h) {
+      LG_LAUNCH_ZFAST(KI_DIFF_ZETA, k_diff_zeta, nx + 1, ny + 1, nl, c, old);
+      LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
+      LG_LAUNCH(KI_DIFF_DUAL, k_diff_dual, nx, ny, nl, c, old);
+      LG_LAUNCH_ZFAST(KI_DIFF_MOM, k_diff_mom, nx, ny, nl, c, old);
+    }
+    if (rk == 1 && (c.C.lklemp || c.C.lmom_h || c.C.lmom_v)) LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
     LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
+    // temperature and mass diffusion with the coefficients of this substep (mo_scalar_tend_expl.F90:
+    // 60-99).  The temperature diffusion of the second substep repeats the first one with the same
+    // temperature and coefficients, so its result is kept.
+    if (rk == 1 && c.C.ltemp_h) {
+      LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux_temp, nx, ny, nl, c, old);
+      LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c, c.w.tdh, c.C.ltemp_v);
+    }
+    if (rk == 1 && c.C.lmass_h) {
+      for (int jc = 0; jc < c.G.nc; ++jc) {
+        LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux_mass, nx, ny, nl, c, old, jc);
+        LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c, c.w.mdt[jc], c.C.lmass_v);
+      }
+    }
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)
     if (o3 && rk == 2) {
@@ -484,8 +504,14 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     return fail(nullptr, LGAME_EINVAL, "constituent counts must be 1/0 (dry) or 7/5 (moist)");
   if (!f.lperiodic) return fail(nullptr, LGAME_EINVAL, "unsupported: lperiodic=.false. (lateral boundary relaxation)");
   if (!f.luse_bg_state) return fail(nullptr, LGAME_EINVAL, "unsupported: luse_bg_state=.false.");
-  if (f.lmom_diff_h || f.lmom_diff_v || f.ltemp_diff_h || f.ltemp_diff_v || f.lmass_diff_h || f.lmass_diff_v)
-    return fail(nullptr, LGAME_EINVAL, "unsupported: diffusion switches must be off");
+  if ((f.ltemp_diff_h || f.lmass_diff_h) && !f.lmom_diff_h)
+    return fail(nullptr, LGAME_EINVAL,
+                "unsupported: horizontal temperature or mass diffusion without horizontal momentum diffusion "
+                "(scalar_diff_coeffs, mo_eff_diff_coeffs.F90:540-590, then reads stale vorticity and divergence)");
+  if (f.diff_coeff_scheme_h != 0 && f.diff_coeff_scheme_h != 1)
+    return fail(nullptr, LGAME_EINVAL, "diff_coeff_scheme_h must be 0 (smag) or 1 (tke)");
+  const bool diffusion = f.lmom_diff_h || f.lmom_diff_v;  // temperature and mass diffusion need lmom_diff_h
+  if (f.lmom_diff_h && !(grid->mean_velocity_area > 0.0)) return fail(nullptr, LGAME_EINVAL, "mean_velocity_area must be positive");
   if (f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
     return fail(nullptr, LGAME_EINVAL, "unsupported: PBL, surface fluxes and soil must be off");
   if (f.rad_mode != LGAME_RAD_OFF && f.rad_mode != LGAME_RAD_TEACHER && f.rad_mode != LGAME_RAD_SURROGATE)
@@ -503,6 +529,9 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   NEED(slope_x) NEED(slope_y) NEED(inner_product_weights) NEED(area_dual_x) NEED(area_dual_y) NEED(area_dual_z) NEED(z_area_dual_z)
   NEED(fvec_x) NEED(fvec_y) NEED(fvec_z) NEED(trsk_weights_u) NEED(trsk_weights_v) NEED(exner_bg_grad_u) NEED(exner_bg_grad_v)
   NEED(exner_bg_grad_w) NEED(theta_v_bg) NEED(exner_bg) NEED(gravity_m_v) NEED(gravity_potential)
+  if (f.lmom_diff_h) {
+    NEED(dx_dual) NEED(dy_dual) NEED(layer_thickness)
+  }
 #undef NEED
 
   lgame_handle *h = new lgame_handle();
@@ -536,6 +565,17 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   C.lmoist = moist;
   C.lklemp = f.lklemp != 0;
   C.lfreeslip = f.lfreeslip != 0;
+  C.lmom_h = f.lmom_diff_h != 0;
+  C.lmom_v = f.lmom_diff_v != 0;
+  C.ltemp_h = f.ltemp_diff_h != 0;
+  C.ltemp_v = f.ltemp_diff_v != 0;
+  C.lmass_h = f.lmass_diff_h != 0;
+  C.lmass_v = f.lmass_diff_v != 0;
+  C.scheme_tke = f.diff_coeff_scheme_h == 1;
+  C.c_s = f.c_s;
+  C.bg_shear = f.bg_shear;
+  C.eff_hor_res = std::sqrt(f.dx * f.dy);  // mo_run_nml.F90:93
+  C.mva = grid->mean_velocity_area;
 
   // n_damping_levels, as mo_grid_generator.F90:1050-1058 derives it from z_w
   {
@@ -562,7 +602,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   constexpr int NGRID = 40;
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2;
+  const size_t ndiff = diffusion ? (size_t)14 + nc : 0;  // the diffusion fields of WorkT
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
   h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
@@ -646,6 +687,23 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     W.thv = take();
     W.decel = take();
     W.pgc = take();
+    if (diffusion) {
+      W.tke = take();
+      W.tke_next = take();
+      W.visc = take();
+      W.kv = take();
+      W.vdiv = take();
+      W.zeta = take();
+      W.vz = take();
+      W.mom_x = take();
+      W.mom_y = take();
+      W.mom_z = take();
+      W.tdh = take();
+      W.dfu = take();
+      W.dfv = take();
+      W.dfw = take();
+      for (int jc = 0; jc < MAX_CONST; ++jc) W.mdt[jc] = jc < nc ? take() : nullptr;
+    }
     for (int q = 0; q < 12; ++q) W.col[q] = col[q];
     W.vfall_cw = col[12];
   };
@@ -680,7 +738,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   enum {
     G_Z_SCALAR, G_Z_U, G_Z_V, G_Z_W, G_DX, G_DY, G_DZ, G_AREA_X, G_AREA_Y, G_AREA_Z, G_VOLUME, G_SLOPE_X, G_SLOPE_Y,
     G_IPW, G_AREA_DUAL_X = G_IPW + 6, G_AREA_DUAL_Y, G_AREA_DUAL_Z, G_Z_AREA_DUAL_Z, G_FVEC_X, G_FVEC_Y, G_FVEC_Z,
-    G_EBG_U, G_EBG_V, G_EBG_W, G_THETA_V_BG, G_EXNER_BG, G_GRAVITY_M_V, G_GRAVITY_POTENTIAL, G_KLEMP_PREF, G_COUNT
+    G_EBG_U, G_EBG_V, G_EBG_W, G_THETA_V_BG, G_EXNER_BG, G_GRAVITY_M_V, G_GRAVITY_POTENTIAL, G_KLEMP_PREF, G_DX_DUAL,
+    G_DY_DUAL, G_LAYER_THICKNESS, G_COUNT
   };
   static_assert(G_COUNT <= NGRID, "grid field budget");
   auto gf = [&](int q) { return grid64 + (size_t)q * fs; };
@@ -713,6 +772,11 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   rc = rc ? rc : upload(h, gf(G_EXNER_BG), grid->exner_bg, ny, nx, nl);
   rc = rc ? rc : upload(h, gf(G_GRAVITY_M_V), grid->gravity_m_v, ny, nx, nlev);
   rc = rc ? rc : upload(h, gf(G_GRAVITY_POTENTIAL), grid->gravity_potential, ny, nx, nl);
+  if (f.lmom_diff_h) {
+    rc = rc ? rc : upload(h, gf(G_DX_DUAL), grid->dx_dual, ny + 1, nx, nl);
+    rc = rc ? rc : upload(h, gf(G_DY_DUAL), grid->dy_dual, ny, nx + 1, nl);
+    rc = rc ? rc : upload(h, gf(G_LAYER_THICKNESS), grid->layer_thickness, ny, nx, nl);
+  }
   if (!rc) {
     std::vector<double> kp((size_t)ny * nx * nlev);
     for (size_t q = 0; q < kp.size(); ++q) kp[q] = klemp_prefactor(f, grid->z_w[q]);
@@ -771,6 +835,9 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     g.gravity_m_v = at(G_GRAVITY_M_V);
     g.gravity_potential = at(G_GRAVITY_POTENTIAL);
     g.klemp_pref = at(G_KLEMP_PREF);
+    g.dx_dual = at(G_DX_DUAL);
+    g.dy_dual = at(G_DY_DUAL);
+    g.layer_thickness = at(G_LAYER_THICKNESS);
     g.trsk_u = trsk;
     g.trsk_v = trsk + 3 * ny;
   };
@@ -917,14 +984,32 @@ int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad) {
 
 int lgame_get_diag(lgame_handle *h, int which, double *out) {
   if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
-  if (which != LGAME_DIAG_TEMPERATURE) return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
-  if (h->mixed) return download(h, out, h->cf.w.temperature.p, h->dims.ny, h->dims.nx, h->dims.n_layers);
-  return download(h, out, h->c.w.temperature.p, h->dims.ny, h->dims.nx, h->dims.n_layers);
+  const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers;
+  const bool diffusion = h->cfg.lmom_diff_h || h->cfg.lmom_diff_v;
+  auto get = [&](auto &w) -> int {
+    switch (which) {
+      case LGAME_DIAG_TEMPERATURE: return download(h, out, w.temperature.p, ny, nx, nl);
+      case LGAME_DIAG_HEATING_DISS: return download(h, out, w.heating_diss.p, ny, nx, nl);
+      default: break;
+    }
+    if (!diffusion) return fail(h, LGAME_EINVAL, "diagnostic %d exists only with diffusion", which);
+    switch (which) {
+      case LGAME_DIAG_TKE: return download(h, out, w.tke.p, ny, nx, nl);
+      case LGAME_DIAG_VISCOSITY: return download(h, out, w.visc.p, ny, nx, nl);
+      case LGAME_DIAG_MOM_DIFF_TEND_X: return download(h, out, w.mom_x.p, ny, nx + 1, nl);
+      case LGAME_DIAG_MOM_DIFF_TEND_Z: return download(h, out, w.mom_z.p, ny, nx, nl + 1);
+      case LGAME_DIAG_TEMP_DIFF_HEATING: return download(h, out, w.tdh.p, ny, nx, nl);
+      default: return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
+    }
+  };
+  return h->mixed ? get(h->cf.w) : get(h->c.w);
 }
 
 }  // extern "C"
 
 extern "C" {
+
+static_assert(KI_COUNT == LGAME_N_TILES, "tile table and LGAME_N_TILES");
 
 int lgame_autotune(lgame_handle *h, int steps, int *rows) {
   if (!h) return fail(h, LGAME_EINVAL, "null handle");

@@ -36,6 +36,9 @@ constexpr double MPI = 3.14159265358979323846;
 
 constexpr int MAX_CONST = 7;
 
+// XCDs of the MI355X; blocks of a launch go to them round-robin.
+constexpr int N_XCD = 8;
+
 // Element index inside one field.  Every field is smaller than 4 GiB (checked by lgame_init),
 // so the byte offset fits 32 bits and an access compiles to one load with a scalar base
 // register and a 32-bit vector offset.
@@ -70,6 +73,7 @@ struct GridT {
   Fld<const R> z_area_dual_z, fvec_x, fvec_y, fvec_z, ebg_u, ebg_v, ebg_w, theta_v_bg;
   Fld<const R> exner_bg, gravity_m_v, gravity_potential;
   Fld<const R> klemp_pref;  // 1 + damping_coeff*dtime at w levels (mo_column_solvers.F90:190-196)
+  Fld<const R> dx_dual, dy_dual, layer_thickness;  // read by the diffusion kernels only
   const R *trsk_u;          // (3, ny) as in the reference
   const R *trsk_v;          // (2, ny+1)
 };
@@ -94,11 +98,23 @@ struct WorkT {
   Fld<R> decel, pgc;  // per cell of the old state: C_D_P*rho_gas/rho_sum and rho_gas/rho_sum - 1
   Fld<double> col[12];                                  // column-solver per-level scratch (FP64)
   Fld<double> vfall_cw;  // sink velocity of cloud water per layer (moist), with the temperature
+  // diffusion and turbulence (allocated only when a diffusion switch is on).  Carried between
+  // steps as the reference's t_diag carries them: tke, visc (density times the horizontal
+  // kinematic viscosity, diag%viscosity), mom_x/_y/_z (diag%mom_diff_tend_*), tdh
+  // (diag%temp_diff_heating) and mdt (diag%mass_diff_tendency).  Per step: zeta (relative
+  // vorticity at the dual points), tke_next, kv (molecular plus vertical eddy viscosity of a
+  // cell), vdiv (visc times the horizontal divergence), vz (viscosity_dual times zeta) and the
+  // diffusive fluxes dfu, dfv, dfw of one scalar field.
+  Fld<R> tke, tke_next, visc, kv, vdiv, zeta, vz, mom_x, mom_y, mom_z, tdh, mdt[MAX_CONST], dfu, dfv, dfw;
 };
 
 struct Cfg {
   double dtime, toa, dxn, dyn, klemp_damp_max, klemp_begin_rel;
   int order, llinear, lcorio, lmoist, lklemp, lfreeslip, n_damping_levels;
+  // diffusion (mo_diff_nml.F90); eff_hor_res = sqrt(dx*dy) (mo_run_nml.F90:93), mva =
+  // grid%mean_velocity_area (mo_grid_generator.F90:1111)
+  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_tke;
+  double c_s, bg_shear, eff_hor_res, mva;
 };
 
 template <class R>

@@ -258,4 +258,73 @@ __device__ __forceinline__ R ke_point(const CtxT<R> &c, const StateDev &s, int j
          AT(c.g.ipw[3], ji, jk, jl) * v1 * v1 + AT(c.g.ipw[4], ji, jk, jl) * w0 * w0 + AT(c.g.ipw[5], ji, jk, jl) * w1 * w1;
 }
 
+// ---- thread -> grid point maps of the stencil kernels ---------------------------------------
+#define KIDX                                                        \
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;         \
+  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
+  const int jl = blockIdx.z + 1;
+
+// Tile order for kernels that read many values of the levels above and below: the kernel is
+// launched on a 1-D grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8
+// XCDs.  XCD k takes a contiguous range of tiles, the level running fastest, so the blocks of one
+// tile at adjacent levels run one after another on the same XCD and its L2 still holds the
+// shared layers.  Kernels that mostly stream level by level keep the x-fastest order, which
+// reads DRAM pages in sequence.
+__device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int &by, int &bz) {
+  const int n = gx * gy * gz, b = blockIdx.x;
+  const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, r = n % N_XCD;
+  const int t = k * q + (k < r ? k : r) + s;  // tile index, level fastest
+  bz = t % gz;
+  const int txy = t / gz;
+  bx = txy % gx;
+  by = txy / gx;
+}
+#define KIDX_ZFAST(ex, ey, ez)                                                            \
+  int bx_, by_, bz_;                                                                      \
+  tile_zfast(((ex) + BX - 1) / BX, ((ey) + (int)blockDim.y - 1) / (int)blockDim.y, (ez), bx_, by_, bz_); \
+  const int jk = bx_ * blockDim.x + threadIdx.x + 1;                                      \
+  const int ji = by_ * blockDim.y + threadIdx.y + 1;                                      \
+  const int jl = bz_ + 1;
+
+// rel_vort_z_local, mo_vorticities.F90:423-538, at any dual point (ji <= ny+1, jk <= nx+1).  The
+// dual points of row ny+1 and column nx+1 use their own metrics (rel_vort, :177-184, evaluates
+// them; calc_pot_vort then overwrites its eta_z there with row 1 and column 1, :391-417).
+template <class R>
+__device__ R rel_vort_z(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+  const int ny = c.G.ny, nx = c.G.nx;
+  const int j_i[4] = {ji, wrapm(ji, ny), ji, ji == ny + 1 ? 1 : ji};
+  const int j_k[4] = {jk == nx + 1 ? 1 : jk, jk, wrapm(jk, nx), jk};
+  const R sgn[4] = {R(1.0), -R(1.0), -R(1.0), R(1.0)};
+  R r = R(0.0);
+  if (jl <= c.G.nflat) {
+    r = r + sgn[0] * AT(c.g.dy, j_i[0], j_k[0], jl) * ATR(s.wind_v, j_i[0], j_k[0], jl) +
+        sgn[1] * AT(c.g.dx, j_i[1], j_k[1], jl) * ATR(s.wind_u, j_i[1], j_k[1], jl) +
+        sgn[2] * AT(c.g.dy, j_i[2], j_k[2], jl) * ATR(s.wind_v, j_i[2], j_k[2], jl) +
+        sgn[3] * AT(c.g.dx, j_i[3], j_k[3], jl) * ATR(s.wind_u, j_i[3], j_k[3], jl);
+    return r;
+  }
+  const R zd = AT(c.g.z_area_dual_z, ji, jk, jl);
+#pragma unroll
+  for (int jm = 0; jm < 4; ++jm) {
+    const bool isv = (jm == 0 || jm == 2);
+    const Fld<const R> zf = isv ? c.g.z_v : c.g.z_u;
+    const Fld<const double> wf = isv ? Fld<const double>(s.wind_v) : Fld<const double>(s.wind_u);
+    const Fld<const R> len = isv ? c.g.dy : c.g.dx;
+    const R zp = AT(zf, j_i[jm], j_k[jm], jl);
+    const R l_rescale = (R(R_E) + zd) / (R(R_E) + zp);
+    const R delta_z = zd - zp;
+    int sh = 1;
+    if (delta_z > R(0.0) || jl == c.G.nl) sh = -1;
+    if (jl == 1) sh = 1;
+    const R vg = (ATR(wf, j_i[jm], j_k[jm], jl) - ATR(wf, j_i[jm], j_k[jm], jl + sh)) / (zp - AT(zf, j_i[jm], j_k[jm], jl + sh));
+    r = r + sgn[jm] * l_rescale * AT(len, j_i[jm], j_k[jm], jl) * (ATR(wf, j_i[jm], j_k[jm], jl) + delta_z * vg);
+  }
+  return r;
+}
+
+template <class R>
+__device__ __forceinline__ R rhog(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+  return ATR(s.rho[c.G.ncond], ji, jk, jl);
+}
+
 }  // namespace lgame

@@ -23,6 +23,13 @@ template <class R> __global__ void k_scalar_flux(const CtxT<R> c, const StateDev
 template <class R> __global__ void k_scalar_div(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
+template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_coef(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_dual(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_mom(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_flux_temp(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_flux_mass(const CtxT<R> c, const StateDev s, int jc);
+template <class R> __global__ void k_diff_div(const CtxT<R> c, Fld<R> out, int lvert);
 
 // Launch geometry of the horizontal stencil kernels: 64 x-points by BY rows per block (the
 // default; the tile table can choose 1, 2 or 4 rows per kernel).
@@ -32,7 +39,8 @@ constexpr int BY = 4;
 // stencil kernels of the step, the index of their tile-table entry
 enum KernelId {
   KI_TEMPERATURE, KI_THETA_FILL, KI_THETA_GRAD, KI_THETA_DIV, KI_THETA_EDGE, KI_PGRAD, KI_POT_VORT, KI_MASS_FLUX,
-  KI_VORT_FLUX, KI_KE, KI_KE_GRAD, KI_DISSIPATION, KI_VEC_TEND, KI_SCALAR_FLUX, KI_SCALAR_DIV, KI_COUNT
+  KI_VORT_FLUX, KI_KE, KI_KE_GRAD, KI_DISSIPATION, KI_VEC_TEND, KI_SCALAR_FLUX, KI_SCALAR_DIV, KI_DIFF_ZETA,
+  KI_DIFF_COEF, KI_DIFF_DUAL, KI_DIFF_MOM, KI_DIFF_FLUX, KI_DIFF_DIV, KI_COUNT
 };
 
 // rows per block of every stencil kernel (block = 64 x by threads)
@@ -42,8 +50,6 @@ struct Tiles {
     for (int &b : by) b = BY;
   }
 };
-// XCDs of the MI355X; blocks of a launch go to them round-robin.
-constexpr int N_XCD = 8;
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
 // The tracer solver: one wavefront per tracer of a 64-column group, two groups per block. Twelve

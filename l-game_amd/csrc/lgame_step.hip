@@ -10,33 +10,6 @@
 
 namespace lgame {
 
-#define KIDX                                                        \
-  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;         \
-  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
-  const int jl = blockIdx.z + 1;
-
-// Tile order for kernels that read many values of the levels above and below: the kernel is
-// launched on a 1-D grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8
-// XCDs.  XCD k takes a contiguous range of tiles, the level running fastest, so the blocks of one
-// tile at adjacent levels run one after another on the same XCD and its L2 still holds the
-// shared layers.  Kernels that mostly stream level by level keep the x-fastest order, which
-// reads DRAM pages in sequence.
-__device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int &by, int &bz) {
-  const int n = gx * gy * gz, b = blockIdx.x;
-  const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, r = n % N_XCD;
-  const int t = k * q + (k < r ? k : r) + s;  // tile index, level fastest
-  bz = t % gz;
-  const int txy = t / gz;
-  bx = txy % gx;
-  by = txy / gx;
-}
-#define KIDX_ZFAST(ex, ey, ez)                                                            \
-  int bx_, by_, bz_;                                                                      \
-  tile_zfast(((ex) + BX - 1) / BX, ((ey) + (int)blockDim.y - 1) / (int)blockDim.y, (ez), bx_, by_, bz_); \
-  const int jk = bx_ * blockDim.x + threadIdx.x + 1;                                      \
-  const int ji = by_ * blockDim.y + threadIdx.y + 1;                                      \
-  const int jl = bz_ + 1;
-
 // ---------------------------------------------------------------------------------------------
 // temperature_diagnostics, mo_derived.F90:18-38
 // ---------------------------------------------------------------------------------------------
@@ -327,45 +300,6 @@ __device__ R zeta_y_circ(const CtxT<R> &c, const StateDev &s, int ji, int jk, in
   return r;
 }
 
-// rel_vort_z_local, mo_vorticities.F90:423-538, for a canonical dual point (ji<=ny, jk<=nx)
-template <class R>
-__device__ R rel_vort_z(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  const int ny = c.G.ny, nx = c.G.nx;
-  const int j_i[4] = {ji, wrapm(ji, ny), ji, ji};
-  const int j_k[4] = {jk, jk, wrapm(jk, nx), jk};
-  const R sgn[4] = {R(1.0), -R(1.0), -R(1.0), R(1.0)};
-  R r = R(0.0);
-  if (jl <= c.G.nflat) {
-    r = r + sgn[0] * AT(c.g.dy, j_i[0], j_k[0], jl) * ATR(s.wind_v, j_i[0], j_k[0], jl) +
-        sgn[1] * AT(c.g.dx, j_i[1], j_k[1], jl) * ATR(s.wind_u, j_i[1], j_k[1], jl) +
-        sgn[2] * AT(c.g.dy, j_i[2], j_k[2], jl) * ATR(s.wind_v, j_i[2], j_k[2], jl) +
-        sgn[3] * AT(c.g.dx, j_i[3], j_k[3], jl) * ATR(s.wind_u, j_i[3], j_k[3], jl);
-    return r;
-  }
-  const R zd = AT(c.g.z_area_dual_z, ji, jk, jl);
-#pragma unroll
-  for (int jm = 0; jm < 4; ++jm) {
-    const bool isv = (jm == 0 || jm == 2);
-    const Fld<const R> zf = isv ? c.g.z_v : c.g.z_u;
-    const Fld<const double> wf = isv ? Fld<const double>(s.wind_v) : Fld<const double>(s.wind_u);
-    const Fld<const R> len = isv ? c.g.dy : c.g.dx;
-    const R zp = AT(zf, j_i[jm], j_k[jm], jl);
-    const R l_rescale = (R(R_E) + zd) / (R(R_E) + zp);
-    const R delta_z = zd - zp;
-    int sh = 1;
-    if (delta_z > R(0.0) || jl == c.G.nl) sh = -1;
-    if (jl == 1) sh = 1;
-    const R vg = (ATR(wf, j_i[jm], j_k[jm], jl) - ATR(wf, j_i[jm], j_k[jm], jl + sh)) / (zp - AT(zf, j_i[jm], j_k[jm], jl + sh));
-    r = r + sgn[jm] * l_rescale * AT(len, j_i[jm], j_k[jm], jl) * (ATR(wf, j_i[jm], j_k[jm], jl) + delta_z * vg);
-  }
-  return r;
-}
-
-template <class R>
-__device__ __forceinline__ R rhog(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  return ATR(s.rho[c.G.ncond], ji, jk, jl);
-}
-
 // density average of calc_pot_vort for horizontal vorticities, mo_vorticities.F90:240-266
 template <class R>
 __device__ R pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, int ka, int ib, int kb, int jl) {
@@ -598,24 +532,62 @@ __global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
   if (jk <= nx) AT(c.w.vsg_y, ji, jk, jl) = grad_hor_y(c, k2, ji, jk, jl);
 }
 
-// simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377 (momentum diffusion off: only the
-// Klemp layer acceleration enters the inner product)
+// simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377, and the vertical diffusion of w
+// of mom_diff_v (:277-289,320-328).  mom_diff_tend_z at the two levels of the cell is formed here
+// from the coefficient kv of k_diff_coef (vert_vert_mom_viscosity, mo_eff_diff_coeffs.F90:491-524)
+// and stored at the upper level; the Klemp layer acceleration is added to it and the inner
+// product with the wind gives the heating rate.  mom_diff_v's vertical diffusion of u, v and
+// horizontal diffusion of w multiply vert_hor_viscosity_u/_v, which the reference never sets
+// (vert_hor_mom_viscosity, :312-489, has no caller; control.F90:427-428 zeroes them): those terms
+// are exact zeros and are not formed.
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const StateDev s) {
   KIDX;
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  const ix_t i = LG_IX(c.G, ji, jk, jl), ib = LG_IX(c.G, ji, jk, jl + 1);
-  R wk0 = R(0.0), wk1 = R(0.0);
+  const Geom &G = c.G;
+  const int nl = G.nl;
+  if (jk > G.nx || ji > G.ny || jl > nl) return;
+  const ix_t i = LG_IX(G, ji, jk, jl), ib = LG_IX(G, ji, jk, jl + 1);
+  R mz0 = R(0.0), mz1 = R(0.0);
+  if (c.C.lmom_v) {
+    // rho*kv*(dw/dz) of layer l: add_vertical_div into a zeroed field, then vert_vert_mom_viscosity
+    auto sp = [&](int l) {
+      const ix_t q = LG_IX(G, ji, jk, l);
+      const R cu = l == 1 ? R(0.0) : R(s.wind_w[q]);
+      const R cl = l == nl ? R(0.0) : ATR(s.wind_w, ji, jk, l + 1);
+      const R comp_v = cu * c.g.area_z[q] - cl * AT(c.g.area_z, ji, jk, l + 1);
+      const R x = R(0.0) + comp_v / c.g.volume[q];
+      return R(s.rho[G.ncond][q]) * c.w.kv[q] * x;
+    };
+    const R sp_c = sp(jl);
+    if (jl >= 2) {
+      const ix_t ia = LG_IX(G, ji, jk, jl - 1);
+      mz0 = (sp(jl - 1) - sp_c) / c.g.dz[i] / (R(0.5) * (rho_sum(c, s, ia) + rho_sum(c, s, i)));
+      c.w.mom_z[i] = mz0;
+    }
+    if (jl < nl) mz1 = (sp_c - sp(jl + 1)) / c.g.dz[ib] / (R(0.5) * (rho_sum(c, s, i) + rho_sum(c, s, ib)));
+  }
+  if (!c.C.lklemp && !c.C.lmom_h) return;  // no heating rate without Klemp damping or momentum diffusion
+  R wk0 = mz0, wk1 = mz1;
   const R dt = R(c.C.dtime);
-  if (jl >= 2 && jl <= c.C.n_damping_levels) {
+  if (c.C.lklemp && jl >= 2 && jl <= c.C.n_damping_levels) {
     const R w = R(s.wind_w[i]);
-    wk0 = (w / c.g.klemp_pref[i] - w) / dt;
+    const R k = (w / c.g.klemp_pref[i] - w) / dt;
+    wk0 = c.C.lmom_v ? wk0 + k : k;
   }
-  if (jl + 1 >= 2 && jl + 1 <= c.C.n_damping_levels) {
+  if (c.C.lklemp && jl + 1 <= c.C.n_damping_levels) {
     const R w = R(s.wind_w[ib]);
-    wk1 = (w / c.g.klemp_pref[ib] - w) / dt;
+    const R k = (w / c.g.klemp_pref[ib] - w) / dt;
+    wk1 = c.C.lmom_v ? wk1 + k : k;
   }
-  const R ip = c.g.ipw[4][i] * R(s.wind_w[i]) * wk0 + c.g.ipw[5][i] * R(s.wind_w[ib]) * wk1;
+  R ip;
+  if (c.C.lmom_h) {
+    ip = AT(c.g.ipw[0], ji, jk, jl) * ATR(s.wind_u, ji, jk + 1, jl) * AT(c.w.mom_x, ji, jk + 1, jl) +
+         c.g.ipw[1][i] * R(s.wind_v[i]) * c.w.mom_y[i] + c.g.ipw[2][i] * R(s.wind_u[i]) * c.w.mom_x[i] +
+         AT(c.g.ipw[3], ji, jk, jl) * ATR(s.wind_v, ji + 1, jk, jl) * AT(c.w.mom_y, ji + 1, jk, jl) +
+         c.g.ipw[4][i] * R(s.wind_w[i]) * wk0 + c.g.ipw[5][i] * R(s.wind_w[ib]) * wk1;
+  } else {  // the horizontal terms are products with zero tendencies
+    ip = c.g.ipw[4][i] * R(s.wind_w[i]) * wk0 + c.g.ipw[5][i] * R(s.wind_w[ib]) * wk1;
+  }
   c.w.heating_diss[i] = -rho_sum(c, s, i) * ip;
 }
 
@@ -631,15 +603,18 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
   const R cur_hor = R(0.5) + R(IMPL_THERMO_WEIGHT), old_hor = R(1.0) - cur_hor, cur_ver = R(1.0) - R(IMPL_THERMO_WEIGHT);
   const ix_t i = LG_IX(c.G, ji, jk, jl);
+  const bool mom = c.C.lmom_h || c.C.lmom_v;  // diag%mom_diff_tend_* (zero arrays otherwise)
   if (ji <= ny) {
-    const R t = old_w * c.w.tend_u[i] +
-                     new_w * (old_hor * c.w.pold_u[i] - cur_hor * (c.w.pnl_u[i] + c.w.pl_u[i]) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i]);
+    R x = old_hor * c.w.pold_u[i] - cur_hor * (c.w.pnl_u[i] + c.w.pl_u[i]) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i];
+    if (mom) x = x + c.w.mom_x[i];
+    const R t = old_w * c.w.tend_u[i] + new_w * x;
     c.w.tend_u[i] = t;
     nw.wind_u[i] = old.wind_u[i] + c.C.dtime * (double)t;  // FP64 state update
   }
   if (jk <= nx) {
-    const R t = old_w * c.w.tend_v[i] +
-                     new_w * (old_hor * c.w.pold_v[i] - cur_hor * (c.w.pnl_v[i] + c.w.pl_v[i]) - R(0.5) * c.w.vsg_y[i] + c.w.pvt_y[i]);
+    R x = old_hor * c.w.pold_v[i] - cur_hor * (c.w.pnl_v[i] + c.w.pl_v[i]) - R(0.5) * c.w.vsg_y[i] + c.w.pvt_y[i];
+    if (mom) x = x + c.w.mom_y[i];
+    const R t = old_w * c.w.tend_v[i] + new_w * x;
     c.w.tend_v[i] = t;
     nw.wind_v[i] = old.wind_v[i] + c.C.dtime * (double)t;
   }
@@ -651,6 +626,7 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
       const R db = c.w.pgc[i];
       x = x + R(0.5) * (da + db) * c.g.gravity_m_v[i];
     }
+    if (mom) x = x + c.w.mom_z[i];
     c.w.tend_w[i] = old_w * c.w.tend_w[i] + new_w * x;
   }
 }
@@ -808,10 +784,17 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
   };
   const R div_g = div(fc_g, cu_g, cl_g);
   const R div_h_ = div(fc_h, cu_h, cl_h);
-  const R heat = c.w.heating_diss[i] + c.w.radiation_tendency[i] + c.w.csh[i];
-  const R rt_tend = -div_h_ + R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
-                                 (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
-  c.w.tend_rho[ncond][i] = -div_g;
+  R heat = c.w.heating_diss[i] + c.w.radiation_tendency[i] + c.w.csh[i];
+  if (c.C.ltemp_h) heat = heat + c.w.tdh[i];  // diag%temp_diff_heating (of this step's first substep)
+  R rt_tend = -div_h_ + R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
+                           (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
+  R tg = -div_g;
+  if (c.C.lmass_h) {  // diag%mass_diff_tendency of the gas, times theta_v in the rhotheta_v tendency
+    const R md = c.w.mdt[ncond][i];
+    tg = tg + md;
+    rt_tend = rt_tend + (R(0.0) + md) * (c.g.theta_v_bg[i] + R(ss.theta_v_pert[i]));
+  }
+  c.w.tend_rho[ncond][i] = tg;
   c.w.tend_rhotheta_v[i] = rt_tend;
   // tracers: upstream densities at the four faces and the two interfaces
   if (tracers) {
@@ -842,7 +825,8 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
         comp_v = R(r[iu]) * cu * az_u - R(r[il]) * cl * az_l;
       }
       const R d = (comp_h + comp_v) / V;
-      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * (-d);
+      const R x = c.C.lmass_h ? -d + c.w.mdt[q][i] : -d;
+      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * x;
     }
   }
   if (c.C.lfreeslip && jl == nl) {

@@ -29,12 +29,12 @@ class LgameDims(ctypes.Structure):
 
 class LgameCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
-        "dtime", "toa", "dx", "dy", "klemp_damp_max", "klemp_begin_rel", "dtime_rad")] + \
+        "dtime", "toa", "dx", "dy", "klemp_damp_max", "klemp_begin_rel", "dtime_rad", "c_s", "bg_shear")] + \
         [(n, ctypes.c_int) for n in (
             "theta_adv_order", "luse_bg_state", "llinear", "lcorio", "lmoist", "lklemp",
             "lmom_diff_h", "lmom_diff_v", "ltemp_diff_h", "ltemp_diff_v", "lmass_diff_h",
             "lmass_diff_v", "lprog_soil_temp", "lsfc_sensible_heat_flux", "lsfc_phase_trans",
-            "lpbl", "lperiodic", "lfreeslip", "lrad", "precision", "rad_mode")]
+            "lpbl", "lperiodic", "lfreeslip", "lrad", "precision", "rad_mode", "diff_coeff_scheme_h")]
 
 
 @dataclasses.dataclass
@@ -59,7 +59,16 @@ class RunConfig:
     dtime: float | None = None
     # &constituents
     lmoist: bool = False
-    # &diff (diffusion is outside the accelerated path: off)
+    # &diff (the reference turns all six diffusion switches on; the cases here choose)
+    lmom_diff_h: bool = False
+    lmom_diff_v: bool = False
+    ltemp_diff_h: bool = False
+    ltemp_diff_v: bool = False
+    lmass_diff_h: bool = False
+    lmass_diff_v: bool = False
+    diff_coeff_scheme_h: str = "smag"
+    c_s: float = 0.2
+    bg_shear: float = 1.5e-5
     lklemp: bool = True
     klemp_damp_max: float = 0.25
     klemp_begin_rel: float = 0.53
@@ -101,6 +110,16 @@ class RunConfig:
         # mo_run_nml.F90:93-94
         return 1.61 * math.sqrt(self.dx * self.dy) / 1000.0
 
+    def set_diffusion(self, on: bool = True):
+        """All six diffusion switches at once (the reference default is all on)."""
+        for k in ("lmom_diff_h", "lmom_diff_v", "ltemp_diff_h", "ltemp_diff_v", "lmass_diff_h", "lmass_diff_v"):
+            setattr(self, k, bool(on))
+        return self
+
+    @property
+    def diffusion(self) -> bool:
+        return any((self.lmom_diff_h, self.lmom_diff_v, self.ltemp_diff_h, self.lmass_diff_h))
+
     @property
     def gridpoints(self) -> int:
         return self.ny * self.nx * self.n_layers
@@ -122,6 +141,13 @@ class RunConfig:
         c.lcorio = int(self.lcorio)
         c.lmoist = int(self.lmoist)
         c.lklemp = int(self.lklemp)
+        for k in ("lmom_diff_h", "lmom_diff_v", "ltemp_diff_h", "ltemp_diff_v", "lmass_diff_h", "lmass_diff_v"):
+            setattr(c, k, int(getattr(self, k)))
+        c.c_s = self.c_s
+        c.bg_shear = self.bg_shear
+        if self.diff_coeff_scheme_h not in ("smag", "tke"):
+            raise ValueError("diff_coeff_scheme_h must be either smag or tke")
+        c.diff_coeff_scheme_h = 0 if self.diff_coeff_scheme_h == "smag" else 1
         c.lperiodic = int(self.lperiodic)
         c.lfreeslip = int(self.lfreeslip)
         c.lrad = int(self.lrad)
@@ -162,7 +188,12 @@ GRID_FIELDS = {
     "exner_bg": lambda ny, nx, nl: (ny, nx, nl),
     "gravity_m_v": lambda ny, nx, nl: (ny, nx, nl + 1),
     "gravity_potential": lambda ny, nx, nl: (ny, nx, nl),
+    "dx_dual": lambda ny, nx, nl: (ny + 1, nx, nl),
+    "dy_dual": lambda ny, nx, nl: (ny, nx + 1, nl),
+    "layer_thickness": lambda ny, nx, nl: (ny, nx, nl),
 }
+# scalar grid quantities that cross the boundary with the fields
+GRID_SCALARS = ("mean_velocity_area",)
 
 STATE_FIELDS = ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w")
 

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .config import GRID_FIELDS, STATE_FIELDS, RunConfig, state_shape
+from .config import GRID_FIELDS, GRID_SCALARS, STATE_FIELDS, RunConfig, state_shape
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "liblgame_gpu.so")
@@ -27,9 +27,10 @@ EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_sta
             "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
             "lgame_ensemble_destroy")
 
-N_TILES = 15
+N_TILES = 21
 TILE_KERNELS = ("temperature", "theta_fill", "theta_grad", "theta_div", "theta_edge", "pgrad", "pot_vort",
-                "mass_flux", "vort_flux", "ke", "ke_grad", "dissipation", "vec_tend", "scalar_flux", "scalar_div")
+                "mass_flux", "vort_flux", "ke", "ke_grad", "dissipation", "vec_tend", "scalar_flux", "scalar_div",
+                "diff_zeta", "diff_coef", "diff_dual", "diff_mom", "diff_flux", "diff_div")
 
 PROF_SOLVE_VER = 0
 PROF_SOLVE_GEN = 1
@@ -42,6 +43,16 @@ RAD_OUT_LW_OUT = 2
 RAD_SURFACE_FIELDS = ("lat_geo", "lon_geo", "sfc_albedo", "temp_sfc")
 
 LGAME_DIAG_TEMPERATURE = 0
+# diagnostics readable with Step.diag: id and extent as a function of (ny, nx, nl)
+DIAGS = {
+    "temperature": (0, lambda ny, nx, nl: (ny, nx, nl)),
+    "tke": (1, lambda ny, nx, nl: (ny, nx, nl)),
+    "viscosity": (2, lambda ny, nx, nl: (ny, nx, nl)),
+    "mom_diff_tend_x": (3, lambda ny, nx, nl: (ny, nx + 1, nl)),
+    "mom_diff_tend_z": (4, lambda ny, nx, nl: (ny, nx, nl + 1)),
+    "temp_diff_heating": (5, lambda ny, nx, nl: (ny, nx, nl)),
+    "heating_diss": (6, lambda ny, nx, nl: (ny, nx, nl)),
+}
 
 
 class LgameError(RuntimeError):
@@ -49,7 +60,7 @@ class LgameError(RuntimeError):
 
 
 class GridView(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in GRID_FIELDS]
+    _fields_ = [(n, ctypes.c_void_p) for n in GRID_FIELDS] + [(n, ctypes.c_double) for n in GRID_SCALARS]
 
 
 class RadView(ctypes.Structure):
@@ -134,7 +145,7 @@ class Step:
         self.cfg = cfg
         self.dims, self.ccfg = cfg.to_abi()
         keep = {n: _f64(grid[n]) for n in GRID_FIELDS}
-        gv = GridView(*[keep[n].ctypes.data for n in GRID_FIELDS])
+        gv = GridView(*[keep[n].ctypes.data for n in GRID_FIELDS], *[float(grid[n]) for n in GRID_SCALARS])
         h = ctypes.c_void_p()
         rc = self.lib.lgame_init(ctypes.byref(self.dims), ctypes.byref(self.ccfg), ctypes.byref(gv), ctypes.byref(h))
         if rc != 0:
@@ -271,9 +282,14 @@ class Step:
         self._check(self.lib.lgame_profile_reset(self.h), "lgame_profile_reset")
 
     def temperature(self) -> np.ndarray:
+        return self.diag("temperature")
+
+    def diag(self, name: str) -> np.ndarray:
+        """A carried diagnostic field (DIAGS) in the reference layout."""
+        which, shape = DIAGS[name]
         c = self.cfg
-        out = np.zeros((c.ny, c.nx, c.n_layers), order="F")
-        self._check(self.lib.lgame_get_diag(self.h, LGAME_DIAG_TEMPERATURE, out.ctypes.data), "lgame_get_diag")
+        out = np.zeros(shape(c.ny, c.nx, c.n_layers), order="F")
+        self._check(self.lib.lgame_get_diag(self.h, which, out.ctypes.data), "lgame_get_diag")
         return out
 
 

@@ -278,6 +278,15 @@ def grid_setup(cfg: RunConfig) -> dict:
     dz[:, :, nl] = 2.0 * (z_scalar[:, :, nl - 1] - z_w[:, :, nl])
     dz[:, :, 1:nl] = z_scalar[:, :, :nl - 1] - z_scalar[:, :, 1:]
     g["dz"] = dz
+    # dual-grid distances and layer thicknesses, :1012-1031 and :1063
+    if lplane:
+        g["dx_dual"] = np.full((ny + 1, nx, nl), dx, order="F")
+        g["dy_dual"] = np.full((ny, nx + 1, nl), dy, order="F")
+    else:
+        lat_dual = np.concatenate([lat_scalar + 0.5 * dlat, [lat_scalar[-1] - 0.5 * dlat]])
+        g["dx_dual"] = np.asfortranarray(dx * np.cos(lat_dual)[:, None, None] * (R_E + z_v) / R_E)
+        g["dy_dual"] = np.asfortranarray(dy * (R_E + z_u) / R_E)
+    g["layer_thickness"] = np.asfortranarray(z_w[:, :, :nl] - z_w[:, :, 1:])
 
     # areas and volumes, :1030-1315
     if lplane:
@@ -286,6 +295,8 @@ def grid_setup(cfg: RunConfig) -> dict:
         a_sfc = _patch_area(lat_scalar, dlon, dlat, False)[:, None] * (R_E + z_w[:, :, nl]) ** 2 / R_E ** 2
         area_z = np.asfortranarray(a_sfc[:, :, None] * (R_E + z_w) ** 2 / (R_E + z_w[:, :, nl:nl + 1]) ** 2)
     g["area_z"] = area_z
+    # area used by the turbulence parameterisations, :1111
+    g["mean_velocity_area"] = 2.0 * float(np.sum(area_z[:, :, nl])) / (ny * nx)
 
     def ext_x(a):  # values at the nx+1 u-columns: averages, extrapolated at both ends
         out = np.empty((a.shape[0], nx + 1) + a.shape[2:])

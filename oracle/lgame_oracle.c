@@ -63,10 +63,14 @@ typedef struct oracle {
   double dtime, toa, dxn, dyn, klemp_damp_max, klemp_begin_rel;
   int order, use_bg, llinear, lcorio, lmoist, lklemp, lfreeslip;
   int n_damping_levels;
+  /* diffusion switches and parameters (mo_diff_nml.F90) */
+  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_h_tke;
+  double c_s, bg_shear, eff_hor_res, mva;
   /* grid */
   F3 z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume, slope_x, slope_y;
   F3 area_dual_x, area_dual_y, area_dual_z, z_area_dual_z, fvec_x, fvec_y, fvec_z;
   F3 ebg_u, ebg_v, ebg_w, theta_v_bg, exner_bg, gravity_m_v, gravity_potential;
+  F3 dx_dual, dy_dual, layer_thickness, mva_f;
   double *ipw;    /* (6,ny,nx,nl) */
   double *trsk_u; /* (3,ny) */
   double *trsk_v; /* (2,ny+1) */
@@ -81,6 +85,8 @@ typedef struct oracle {
   F3 zeta_x, zeta_y, zeta_z, eta_x, eta_y, eta_z, radiation_tendency;
   F3 flux_u, flux_v, flux_div, decel, mom_x, mom_y, mom_z, heating_diss;
   F3 phase_trans_heating_rate, temp_diff_heating, condensates_sediment_heat, a_rain;
+  F3 tke, n_squared, viscosity, viscosity_molecular, viscosity_dual, flux_w;
+  F3 mass_coeff_h, mass_coeff_v, temp_coeff_h, temp_coeff_v;
   double *phase_trans_rates;  /* (ny,nx,nl,ncond+1) */
   double *mass_diff_tendency; /* (ny,nx,nl,nc)     */
   /* radiation: per-column surface data (ny,nx), outputs and switches */
@@ -89,7 +95,7 @@ typedef struct oracle {
   double *lat, *lon, *albedo, *tsfc, *sfc_sw_in, *sfc_lw_out;
   /* per-column work vectors */
   double *work;
-  double *allocs[160];
+  double *allocs[256];
   int nallocs;
 } oracle;
 
@@ -982,6 +988,264 @@ static void simple_dissipation_rate(oracle *o, const ostate *s) {
   for (size_t i = 0; i < n; ++i) o->heating_diss.p[i] = -rho_sum(o, s->rho, i) * o->heating_diss.p[i];
 }
 
+
+/* ------------------------------------------------------------------------------------------ */
+/* diffusion: subgrid_scale/mo_eff_diff_coeffs.F90, mo_tke.F90,                                */
+/* spatial_operators/mo_momentum_diff_diss.F90:25-331, mo_divergence_operators.F90:149-186     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* add_vertical_div, mo_divergence_operators.F90:149-186 */
+static void add_vertical_div(const oracle *o, F3 in, F3 out) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double cu = (jl == 1) ? 0.0 : A3(in, ji, jk, jl);
+        double cl = (jl == o->nl) ? 0.0 : A3(in, ji, jk, jl + 1);
+        double comp_v = cu * A3(o->area_z, ji, jk, jl) - cl * A3(o->area_z, ji, jk, jl + 1);
+        A3(out, ji, jk, jl) = A3(out, ji, jk, jl) + comp_v / A3(o->volume, ji, jk, jl);
+      }
+}
+
+/* tke2hor_diff_coeff, mo_eff_diff_coeffs.F90:668-683 */
+static double tke2hor(const oracle *o, double tke) {
+  double mean_velocity = sqrt(2.0 * tke);
+  double mean_free_path = o->eff_hor_res / 6.0;
+  return (1.0 / 6.0) * mean_free_path * mean_velocity;
+}
+
+/* tke2vert_diff_coeff, mo_eff_diff_coeffs.F90:685-718 */
+static double tke2vert(double tke, double n_squared, double layer_thickness) {
+  double tke_vert = (3.0 * 1e-3) * tke;
+  double mean_velocity = sqrt(2.0 * tke_vert);
+  double n_used = sqrt(n_squared > 1e-4 ? n_squared : 1e-4);
+  double mean_free_path = sqrt(2.0 * tke_vert) / n_used;
+  if (layer_thickness < mean_free_path) mean_free_path = layer_thickness;
+  return (1.0 / 6.0) * mean_free_path * mean_velocity;
+}
+
+/* update_n_squared, mo_eff_diff_coeffs.F90:616-666.  Levels 1 and n_levels of w_placeholder are
+   not written by grad_vert: the top and bottom layers take what the shared array holds. */
+static void update_n_squared(oracle *o, const ostate *s) {
+  size_t n = (size_t)o->ny * o->nx * o->nl, nw = (size_t)o->ny * o->nx * o->nlev;
+  for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = o->theta_v_bg.p[i] + s->theta_v_pert.p[i];
+  grad_vert(o, o->scalar_ph, o->w_ph);
+  for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = 1.0 / o->scalar_ph.p[i];
+  stv_v2(o, o->scalar_ph, o->w_ph);
+  for (size_t i = 0; i < nw; ++i) o->w_ph.p[i] = o->gravity_m_v.p[i] * o->w_ph.p[i];
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double v;
+        if (jl == 1)
+          v = A3(o->w_ph, ji, jk, jl);
+        else if (jl == o->nl)
+          v = A3(o->w_ph, ji, jk, jl + 1);
+        else
+          v = IPW(o, 5, ji, jk, jl) * A3(o->w_ph, ji, jk, jl) + IPW(o, 6, ji, jk, jl) * A3(o->w_ph, ji, jk, jl + 1);
+        A3(o->n_squared, ji, jk, jl) = v;
+      }
+}
+
+/* tke_update, mo_tke.F90:19-67 */
+static void tke_update(oracle *o, const ostate *s) {
+  grad_vert(o, o->tke, o->w_ph);
+  grad_hor(o, o->tke, o->u_ph, o->v_ph, o->w_ph);
+  inner_product(o, s->wind_u, s->wind_v, s->wind_w, o->u_ph, o->v_ph, o->w_ph, o->scalar_ph);
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double rg = rhog(o, s, ji, jk, jl);
+        double decay = 8.0 * (MPI * MPI) / o->mva * A3(o->viscosity, ji, jk, jl) / rg;
+        double t = A3(o->tke, ji, jk, jl);
+        t = t + o->dtime * (-A3(o->scalar_ph, ji, jk, jl) + A3(o->heating_diss, ji, jk, jl) / rg - decay * t);
+        if (t < 0.0) t = 0.0;
+        A3(o->tke, ji, jk, jl) = t;
+      }
+}
+
+/* hor_diff_coeff_postprocessing, mo_eff_diff_coeffs.F90:161-310 (periodic).  The molecular pass
+   of the reference is overwritten point for point by the turbulent pass, so only the latter is
+   formed; each edge and corner keeps the reference's order of the four terms. */
+static void hor_diff_coeff_postprocessing(oracle *o, const ostate *s) {
+  int ny = o->ny, nx = o->nx;
+  F3 v = o->viscosity, d = o->viscosity_dual;
+  F3 r = rho_slice(o, s->rho, o->ncond + 1);
+  for (int jl = 1; jl <= o->nl; ++jl) {
+    for (int ji = 2; ji <= ny; ++ji)
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(d, ji, jk, jl) = 0.25 * (A3(v, ji - 1, jk - 1, jl) + A3(v, ji - 1, jk, jl) + A3(v, ji, jk - 1, jl) + A3(v, ji, jk, jl));
+    for (int jk = 2; jk <= nx; ++jk) {
+      A3(d, 1, jk, jl) = 0.25 * (A3(v, 1, jk - 1, jl) + A3(v, ny, jk - 1, jl) + A3(v, 1, jk, jl) + A3(v, ny, jk, jl));
+      A3(d, ny + 1, jk, jl) = A3(d, 1, jk, jl);
+    }
+    for (int ji = 2; ji <= ny; ++ji) {
+      A3(d, ji, 1, jl) = 0.25 * (A3(v, ji - 1, nx, jl) + A3(v, ji - 1, 1, jl) + A3(v, ji, nx, jl) + A3(v, ji, 1, jl));
+      A3(d, ji, nx + 1, jl) = A3(d, ji, 1, jl);
+    }
+    A3(d, 1, 1, jl) = 0.25 * (A3(v, 1, 1, jl) + A3(v, ny, 1, jl) + A3(v, 1, nx, jl) + A3(v, ny, nx, jl));
+    A3(d, 1, nx + 1, jl) = A3(d, 1, 1, jl);
+    A3(d, ny + 1, 1, jl) = A3(d, 1, 1, jl);
+    A3(d, ny + 1, nx + 1, jl) = A3(d, 1, 1, jl);
+    /* multiplication by the averaged density */
+    for (int ji = 2; ji <= ny; ++ji)
+      for (int jk = 2; jk <= nx; ++jk)
+        A3(d, ji, jk, jl) =
+            0.25 * (A3(r, ji - 1, jk - 1, jl) + A3(r, ji - 1, jk, jl) + A3(r, ji, jk - 1, jl) + A3(r, ji, jk, jl)) * A3(d, ji, jk, jl);
+    for (int jk = 2; jk <= nx; ++jk) {
+      A3(d, 1, jk, jl) = A3(d, 1, jk, jl) * (0.25 * (A3(r, 1, jk - 1, jl) + A3(r, ny, jk - 1, jl) + A3(r, 1, jk, jl) + A3(r, ny, jk, jl)));
+      A3(d, ny + 1, jk, jl) = A3(d, 1, jk, jl);
+    }
+    for (int ji = 2; ji <= ny; ++ji) {
+      A3(d, ji, 1, jl) = A3(d, ji, 1, jl) * (0.25 * (A3(r, ji - 1, nx, jl) + A3(r, ji - 1, 1, jl) + A3(r, ji, nx, jl) + A3(r, ji, 1, jl)));
+      A3(d, ji, nx + 1, jl) = A3(d, ji, 1, jl);
+    }
+    A3(d, 1, 1, jl) = A3(d, 1, 1, jl) * (0.25 * (A3(r, ny, nx, jl) + A3(r, ny, 1, jl) + A3(r, 1, nx, jl) + A3(r, 1, 1, jl)));
+    A3(d, 1, nx + 1, jl) = A3(d, 1, 1, jl);
+    A3(d, ny + 1, 1, jl) = A3(d, 1, 1, jl);
+    A3(d, ny + 1, nx + 1, jl) = A3(d, 1, 1, jl);
+  }
+}
+
+/* hor_viscosity (_smag / _tke), mo_eff_diff_coeffs.F90:24-159.  The Smagorinsky form reads the
+   vorticity zeta_z and the horizontal divergence in scalar_placeholder. */
+static void hor_viscosity(oracle *o, const ostate *s) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(o->viscosity_molecular, ji, jk, jl) = calc_diff_coeff(A3(o->temperature, ji, jk, jl), rhog(o, s, ji, jk, jl));
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double e;
+        if (!o->scheme_h_tke) {
+          double vc = 0.25 * (A3(o->zeta_z, ji, jk, jl) + A3(o->zeta_z, ji, jk + 1, jl) + A3(o->zeta_z, ji + 1, jk, jl) +
+                              A3(o->zeta_z, ji + 1, jk + 1, jl));
+          double dv = A3(o->scalar_ph, ji, jk, jl);
+          double a = vc * vc + dv * dv, b = o->bg_shear * o->bg_shear;
+          e = o->c_s * o->mva * sqrt(a > b ? a : b);
+        } else {
+          e = tke2hor(o, A3(o->tke, ji, jk, jl));
+        }
+        e = A3(o->viscosity_molecular, ji, jk, jl) + e;
+        A3(o->viscosity, ji, jk, jl) = rhog(o, s, ji, jk, jl) * e;
+      }
+  hor_diff_coeff_postprocessing(o, s);
+}
+
+/* 0.5*(sum(rho(a,1:ncond+1)) + sum(rho(b,1:ncond+1))), the averaged density of
+   mo_momentum_diff_diss.F90:123-159,231-271,321-326 */
+static double mean_rho2(const oracle *o, const ostate *s, int ia, int ka, int la, int ib, int kb, int lb) {
+  size_t a = (size_t)(ia - 1) + (size_t)o->ny * ((size_t)(ka - 1) + (size_t)o->nx * (size_t)(la - 1));
+  size_t b = (size_t)(ib - 1) + (size_t)o->ny * ((size_t)(kb - 1) + (size_t)o->nx * (size_t)(lb - 1));
+  return 0.5 * (rho_sum(o, s->rho, a) + rho_sum(o, s->rho, b));
+}
+
+/* viscosity_dual*zeta_z at dual point (ji,jk), vertical difference around layer jl, halved
+   (mo_momentum_diff_diss.F90:75-82) */
+static double half_dvz(const oracle *o, int ji, int jk, int jl) {
+  int lu = jl - 1 < 1 ? 1 : jl - 1, ll = jl + 1 > o->nl ? o->nl : jl + 1;
+  return 0.5 * (A3(o->viscosity_dual, ji, jk, lu) * A3(o->zeta_z, ji, jk, lu) - A3(o->viscosity_dual, ji, jk, ll) * A3(o->zeta_z, ji, jk, ll)) /
+         (A3(o->z_area_dual_z, ji, jk, lu) - A3(o->z_area_dual_z, ji, jk, ll));
+}
+
+/* mom_diff_h, mo_momentum_diff_diss.F90:25-160 (periodic).  grad_hor's terrain correction reads
+   mom_diff_tend_z as the previous call of mom_diff_v left it. */
+static void mom_diff_h(oracle *o, const ostate *s) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  size_t n = (size_t)ny * nx * nl;
+  div_h(o, s->wind_u, s->wind_v, o->scalar_ph);
+  rel_vort(o, s);
+  hor_viscosity(o, s);
+  for (size_t i = 0; i < n; ++i) o->scalar_ph.p[i] = o->viscosity.p[i] * o->scalar_ph.p[i];
+  grad_hor(o, o->scalar_ph, o->mom_x, o->mom_y, o->mom_z);
+  F3 vd = o->viscosity_dual, z = o->zeta_z;
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx + 1; ++jk)
+      for (int ji = 1; ji <= ny; ++ji) {
+        double dd = A3(o->dy_dual, ji, jk, jl);
+        double u = (A3(vd, ji + 1, jk, jl) * A3(z, ji + 1, jk, jl) - A3(vd, ji, jk, jl) * A3(z, ji, jk, jl)) / dd;
+        double slope = (A3(o->z_area_dual_z, ji + 1, jk, jl) - A3(o->z_area_dual_z, ji, jk, jl)) / dd;
+        A3(o->u_ph, ji, jk, jl) = u - slope * (half_dvz(o, ji + 1, jk, jl) + half_dvz(o, ji, jk, jl));
+      }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk)
+      for (int ji = 1; ji <= ny + 1; ++ji) {
+        double dd = A3(o->dx_dual, ji, jk, jl);
+        double v = (A3(vd, ji, jk + 1, jl) * A3(z, ji, jk + 1, jl) - A3(vd, ji, jk, jl) * A3(z, ji, jk, jl)) / dd;
+        double slope = (A3(o->z_area_dual_z, ji, jk + 1, jl) - A3(o->z_area_dual_z, ji, jk, jl)) / dd;
+        A3(o->v_ph, ji, jk, jl) = v - slope * (half_dvz(o, ji, jk + 1, jl) + half_dvz(o, ji, jk, jl));
+      }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int ji = 1; ji <= ny; ++ji) {
+      for (int jk = 1; jk <= nx; ++jk)
+        A3(o->mom_x, ji, jk, jl) =
+            (A3(o->mom_x, ji, jk, jl) + A3(o->u_ph, ji, jk, jl)) / mean_rho2(o, s, ji, jk == 1 ? nx : jk - 1, jl, ji, jk, jl);
+      A3(o->mom_x, ji, nx + 1, jl) = A3(o->mom_x, ji, 1, jl);
+    }
+  for (int jl = 1; jl <= nl; ++jl)
+    for (int jk = 1; jk <= nx; ++jk) {
+      for (int ji = 1; ji <= ny; ++ji)
+        A3(o->mom_y, ji, jk, jl) =
+            (A3(o->mom_y, ji, jk, jl) + A3(o->v_ph, ji, jk, jl)) / mean_rho2(o, s, ji == 1 ? ny : ji - 1, jk, jl, ji, jk, jl);
+      A3(o->mom_y, ny + 1, jk, jl) = A3(o->mom_y, 1, jk, jl);
+    }
+}
+
+/* mom_diff_v, mo_momentum_diff_diss.F90:162-331.  Its vertical diffusion of u, v (:168-276) and
+   horizontal diffusion of w (:290-318) multiply vert_hor_viscosity_u/_v, which the reference
+   never sets (vert_hor_mom_viscosity, mo_eff_diff_coeffs.F90:312-489, has no caller), so they
+   stay zero (control.F90:427-428) and those terms add exact zeros; they are left out here.  What
+   remains is the vertical diffusion of w: add_vertical_div, vert_vert_mom_viscosity
+   (mo_eff_diff_coeffs.F90:491-524), grad_vert and the division by the averaged density. */
+static void mom_diff_v(oracle *o, const ostate *s) {
+  size_t n = (size_t)o->ny * o->nx * o->nl;
+  memset(o->scalar_ph.p, 0, n * sizeof(double));
+  add_vertical_div(o, s->wind_w, o->scalar_ph);
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double coeff = A3(o->viscosity_molecular, ji, jk, jl) +
+                       tke2vert(A3(o->tke, ji, jk, jl), A3(o->n_squared, ji, jk, jl), A3(o->layer_thickness, ji, jk, jl));
+        A3(o->scalar_ph, ji, jk, jl) = rhog(o, s, ji, jk, jl) * coeff * A3(o->scalar_ph, ji, jk, jl);
+      }
+  grad_vert(o, o->scalar_ph, o->mom_z);
+  for (int jl = 2; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(o->mom_z, ji, jk, jl) = A3(o->mom_z, ji, jk, jl) / mean_rho2(o, s, ji, jk, jl - 1, ji, jk, jl);
+}
+
+/* scalar_diff_coeffs, mo_eff_diff_coeffs.F90:526-614, for the combinations where horizontal
+   momentum diffusion is on (the only ones accepted). */
+static void scalar_diff_coeffs(oracle *o, const ostate *s) {
+  for (int jl = 1; jl <= o->nl; ++jl)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji) {
+        double rg = rhog(o, s, ji, jk, jl);
+        double kv = A3(o->viscosity_molecular, ji, jk, jl) +
+                    tke2vert(A3(o->tke, ji, jk, jl), A3(o->n_squared, ji, jk, jl), A3(o->layer_thickness, ji, jk, jl));
+        A3(o->mass_coeff_h, ji, jk, jl) = A3(o->viscosity, ji, jk, jl) / rg;
+        A3(o->mass_coeff_v, ji, jk, jl) = kv;
+        /* spec_heat_cap_diagnostics_v, mo_derived.F90:40-68: one relevant constituent */
+        double c_g_v = 0.0;
+        c_g_v = c_g_v + rg / rg * C_D_V;
+        A3(o->temp_coeff_h, ji, jk, jl) = c_g_v * A3(o->viscosity, ji, jk, jl);
+        A3(o->temp_coeff_v, ji, jk, jl) = rg * c_g_v * kv;
+      }
+}
+
+/* the diffusive flux divergence of one scalar field (mo_scalar_tend_expl.F90:66-99) */
+static void scalar_diffusion(oracle *o, F3 field, F3 coeff_h, F3 coeff_v, int lvert, F3 out) {
+  grad_vert(o, field, o->w_ph);
+  grad_hor(o, field, o->u_ph, o->v_ph, o->w_ph);
+  stv_h(o, coeff_h, o->u_ph, o->v_ph, o->flux_u, o->flux_v);
+  div_h(o, o->flux_u, o->flux_v, out);
+  if (lvert) {
+    stv_v(o, coeff_v, o->w_ph, o->flux_w);
+    add_vertical_div(o, o->flux_w, out);
+  }
+}
+
 /* vector_tend_expl, mo_vector_tend_expl.F90:30-182 */
 static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) {
   int ny = o->ny, nx = o->nx, nl = o->nl, nlev = o->nlev;
@@ -1009,7 +1273,13 @@ static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) 
     }
   }
   if (rk_step == 1) {
-    if (o->lklemp) simple_dissipation_rate(o, s);
+    if (o->lmom_h || o->lmass_h || o->ltemp_h) {
+      update_n_squared(o, s);
+      tke_update(o, s); /* diff_coeff_scheme_v is always "tke" (mo_diff_nml.F90:95-98) */
+    }
+    if (o->lmom_h) mom_diff_h(o, s);
+    if (o->lmom_v) mom_diff_v(o, s);
+    if (o->lmom_h || o->lklemp) simple_dissipation_rate(o, s);
   }
   double new_weight = (rk_step == 2) ? 0.5 : 1.0;
   double old_weight = 1.0 - new_weight;
@@ -1034,11 +1304,18 @@ static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) 
 /* time_stepping/mo_scalar_tend_expl.F90                                                       */
 /* ------------------------------------------------------------------------------------------ */
 
-/* scalar_tend_expl, mo_scalar_tend_expl.F90:24-181 (diffusion off) */
+/* scalar_tend_expl, mo_scalar_tend_expl.F90:24-181 */
 static void scalar_tend_expl(oracle *o, const ostate *old, const ostate *nw, int rk_step) {
   const ostate *ss = (rk_step == 1) ? old : nw;
   int ny = o->ny, nx = o->nx, nl = o->nl, nc = o->nc;
   size_t n = (size_t)ny * nx * nl, nu = (size_t)ny * (nx + 1) * nl, nv = (size_t)(ny + 1) * nx * nl;
+  if (rk_step == 1 && (o->lmass_h || o->ltemp_h)) scalar_diff_coeffs(o, ss);
+  if (o->ltemp_h) scalar_diffusion(o, o->temperature, o->temp_coeff_h, o->temp_coeff_v, o->ltemp_v, o->temp_diff_heating);
+  if (o->lmass_h && rk_step == 1)
+    for (int jc = 1; jc <= nc; ++jc) {
+      F3 md = rho_slice(o, o->mass_diff_tendency, jc);
+      scalar_diffusion(o, rho_slice(o, ss->rho, jc), o->mass_coeff_h, o->mass_coeff_v, o->lmass_v, md);
+    }
   for (int jc = 1; jc <= nc; ++jc) {
     double new_w = 1.0;
     if (rk_step == 2 && jc != o->ncond + 1) new_w = 0.5;
@@ -1522,9 +1799,9 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
     snprintf(g_err, sizeof g_err, "bad dimensions");
     return NULL;
   }
-  if (!cf->lperiodic || !cf->luse_bg_state || cf->lmom_diff_h || cf->lmom_diff_v || cf->ltemp_diff_h || cf->ltemp_diff_v ||
-      cf->lmass_diff_h || cf->lmass_diff_v || cf->lpbl || cf->lsfc_sensible_heat_flux || cf->lsfc_phase_trans || cf->lprog_soil_temp ||
-      (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
+  if (!cf->lperiodic || !cf->luse_bg_state || ((cf->ltemp_diff_h || cf->lmass_diff_h) && !cf->lmom_diff_h) ||
+      (cf->diff_coeff_scheme_h != 0 && cf->diff_coeff_scheme_h != 1) || cf->lpbl || cf->lsfc_sensible_heat_flux ||
+      cf->lsfc_phase_trans || cf->lprog_soil_temp || (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
     snprintf(g_err, sizeof g_err, "unsupported configuration");
     return NULL;
   }
@@ -1555,6 +1832,16 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   o->lmoist = cf->lmoist;
   o->lklemp = cf->lklemp;
   o->lfreeslip = cf->lfreeslip;
+  o->lmom_h = cf->lmom_diff_h != 0;
+  o->lmom_v = cf->lmom_diff_v != 0;
+  o->ltemp_h = cf->ltemp_diff_h != 0;
+  o->ltemp_v = cf->ltemp_diff_v != 0;
+  o->lmass_h = cf->lmass_diff_h != 0;
+  o->lmass_v = cf->lmass_diff_v != 0;
+  o->scheme_h_tke = cf->diff_coeff_scheme_h == 1;
+  o->c_s = cf->c_s;
+  o->bg_shear = cf->bg_shear;
+  o->eff_hor_res = sqrt(cf->dx * cf->dy); /* mo_run_nml.F90:93 */
 #define G3(f, a, b, c) o->f = f3alloc_o(o, a, b, c)
   G3(z_scalar, ny, nx, nl);
   G3(z_u, ny, nx + 1, nl);
@@ -1583,6 +1870,20 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   G3(exner_bg, ny, nx, nl);
   G3(gravity_m_v, ny, nx, nlev);
   G3(gravity_potential, ny, nx, nl);
+  G3(dx_dual, ny + 1, nx, nl);
+  G3(dy_dual, ny, nx + 1, nl);
+  G3(layer_thickness, ny, nx, nl);
+  G3(mva_f, 1, 1, 1);
+  G3(tke, ny, nx, nl);
+  G3(n_squared, ny, nx, nl);
+  G3(viscosity, ny, nx, nl);
+  G3(viscosity_molecular, ny, nx, nl);
+  G3(viscosity_dual, ny + 1, nx + 1, nl);
+  G3(flux_w, ny, nx, nlev);
+  G3(mass_coeff_h, ny, nx, nl);
+  G3(mass_coeff_v, ny, nx, nl);
+  G3(temp_coeff_h, ny, nx, nl);
+  G3(temp_coeff_v, ny, nx, nl);
   o->ipw = oalloc(o, (size_t)6 * ny * nx * nl);
   o->trsk_u = oalloc(o, (size_t)3 * ny);
   o->trsk_v = oalloc(o, (size_t)2 * (ny + 1));
@@ -1656,7 +1957,8 @@ static F3 *grid_field(oracle *o, const char *name) {
   if (!strcmp(name, #f)) return &o->f;
   GF(z_scalar) GF(z_u) GF(z_v) GF(z_w) GF(dx) GF(dy) GF(dz) GF(area_x) GF(area_y) GF(area_z) GF(volume) GF(slope_x) GF(slope_y)
   GF(area_dual_x) GF(area_dual_y) GF(area_dual_z) GF(z_area_dual_z) GF(fvec_x) GF(fvec_y) GF(fvec_z) GF(theta_v_bg) GF(exner_bg)
-  GF(gravity_m_v) GF(gravity_potential)
+  GF(gravity_m_v) GF(gravity_potential) GF(dx_dual) GF(dy_dual) GF(layer_thickness)
+  if (!strcmp(name, "mean_velocity_area")) return &o->mva_f;
   if (!strcmp(name, "exner_bg_grad_u")) return &o->ebg_u;
   if (!strcmp(name, "exner_bg_grad_v")) return &o->ebg_v;
   if (!strcmp(name, "exner_bg_grad_w")) return &o->ebg_w;
@@ -1685,6 +1987,7 @@ int orc_set_grid(oracle *o, const char *name, const double *src) {
 
 /* Must be called after z_w is set: n_damping_levels as in mo_grid_generator.F90:1050-1058. */
 void orc_finish_grid(oracle *o) {
+  o->mva = o->mva_f.p[0];
   o->n_damping_levels = 0;
   for (int jl = 1; jl <= o->nlev; ++jl) {
     double mx = -1e300;
@@ -1739,6 +2042,17 @@ int orc_get_diag(oracle *o, const char *name, double *dst) {
   if (!strcmp(name, "tend_wind_w")) f = &o->tend_w;
   if (!strcmp(name, "condensates_sediment_heat")) f = &o->condensates_sediment_heat;
   if (!strcmp(name, "radiation_tendency")) f = &o->radiation_tendency;
+  if (!strcmp(name, "tke")) f = &o->tke;
+  if (!strcmp(name, "n_squared")) f = &o->n_squared;
+  if (!strcmp(name, "viscosity")) f = &o->viscosity;
+  if (!strcmp(name, "mom_diff_tend_x")) f = &o->mom_x;
+  if (!strcmp(name, "mom_diff_tend_y")) f = &o->mom_y;
+  if (!strcmp(name, "mom_diff_tend_z")) f = &o->mom_z;
+  if (!strcmp(name, "temp_diff_heating")) f = &o->temp_diff_heating;
+  if (!strcmp(name, "mass_diff_tendency")) {
+    memcpy(dst, o->mass_diff_tendency, sizeof(double) * (size_t)o->ny * o->nx * o->nl * o->nc);
+    return 0;
+  }
   if (!f) return 1;
   memcpy(dst, f->p, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
   return 0;

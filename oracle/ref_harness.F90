@@ -7,12 +7,15 @@
 ! mo_column_solvers.  It exposes them to Python through bind(C) entry points so the C
 ! restatement in lgame_oracle.c can be checked against the reference arithmetic.
 !
-! mo_vector_tend_expl and mo_manage_pchevi cannot be built here without stand-ins (they reach
-! netCDF through mo_momentum_diff_diss -> mo_grid_generator, and RRTMGP through
-! mo_manage_radiation_calls), so the thin glue they contribute is restated below:
+! mo_vector_tend_expl, mo_momentum_diff_diss and mo_manage_pchevi cannot be built here without
+! stand-ins (they reach netCDF through mo_momentum_diff_diss -> mo_grid_generator, and RRTMGP
+! through mo_manage_radiation_calls), so the glue they contribute is restated below:
 !   * harness_vector_tend_expl follows mo_vector_tend_expl.F90:30-182 and calls the reference's
-!     calc_pot_vort, scalar_times_vector_h/v, calc_vorticity_flux_term, inner_product, grad_vert
-!     and grad_hor;
+!     calc_pot_vort, scalar_times_vector_h/v, calc_vorticity_flux_term, inner_product, grad_vert,
+!     grad_hor, update_n_squared and tke_update;
+!   * harness_mom_diff_h / harness_mom_diff_v follow mom_diff_h / mom_diff_v,
+!     mo_momentum_diff_diss.F90:25-331, and call the reference's div_h, rel_vort, hor_viscosity,
+!     grad_hor, grad_vert, add_vertical_div and vert_vert_mom_viscosity;
 !   * harness_dissipation follows simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377;
 !   * harness_pchevi follows pchevi, mo_manage_pchevi.F90:27-118.
 ! Everything else in a step (pressure gradient, scalar tendencies, both column solvers, the
@@ -24,10 +27,11 @@ module ref_harness
   use mo_definitions,        only: wp,t_grid,t_state,t_diag,t_tend
   use mo_constants,          only: impl_thermo_weight,M_PI
   use mo_run_nml,            only: ny,nx,n_layers,n_levels,n_oro_layers,n_flat_layers,dy,dx,dtime,toa, &
-                                   theta_adv_order,luse_bg_state,llinear,lcorio,lplane
+                                   theta_adv_order,luse_bg_state,llinear,lcorio,lplane,eff_hor_res
   use mo_constituents_nml,   only: lmoist,n_condensed_constituents,n_gaseous_constituents,n_constituents
   use mo_diff_nml,           only: lmom_diff_h,lmom_diff_v,ltemp_diff_h,ltemp_diff_v,lmass_diff_h,lmass_diff_v, &
-                                   lklemp,klemp_damp_max,klemp_begin_rel,diff_coeff_scheme_h,diff_coeff_scheme_v
+                                   lklemp,klemp_damp_max,klemp_begin_rel,diff_coeff_scheme_h,diff_coeff_scheme_v, &
+                                   c_s,bg_shear
   use mo_surface_nml,        only: nsoillays,lprog_soil_temp,lsfc_sensible_heat_flux,lsfc_phase_trans,lpbl
   use mo_bc_nml,             only: lperiodic,lfreeslip
   use mo_derived,            only: temperature_diagnostics
@@ -36,8 +40,10 @@ module ref_harness
   use mo_column_solvers,     only: three_band_solver_ver,three_band_solver_gen_densities,thomas_algorithm
   use mo_inner_product,      only: inner_product,w_free_slip
   use mo_gradient_operators, only: grad_hor,grad_vert
-  use mo_divergence_operators, only: div_h
-  use mo_vorticities,        only: calc_pot_vort
+  use mo_divergence_operators, only: div_h,add_vertical_div
+  use mo_vorticities,        only: calc_pot_vort,rel_vort
+  use mo_eff_diff_coeffs,    only: hor_viscosity,vert_vert_mom_viscosity,update_n_squared
+  use mo_tke,                only: tke_update
   use mo_multiplications,    only: scalar_times_vector_h,scalar_times_vector_v
   use mo_vorticity_flux,     only: calc_vorticity_flux_term
 
@@ -64,11 +70,13 @@ contains
   end subroutine alloc_state
 
   ! idims: ny,nx,n_layers,n_oro_layers,n_constituents,n_condensed_constituents,nsoillays
-  ! icfg:  theta_adv_order,luse_bg_state,llinear,lcorio,lmoist,lklemp,lperiodic,lfreeslip
-  ! rcfg:  dtime,toa,dx,dy,klemp_damp_max,klemp_begin_rel
+  ! icfg:  theta_adv_order,luse_bg_state,llinear,lcorio,lmoist,lklemp,lperiodic,lfreeslip,
+  !        lmom_diff_h,lmom_diff_v,ltemp_diff_h,ltemp_diff_v,lmass_diff_h,lmass_diff_v,
+  !        diff_coeff_scheme_h (0 smag, 1 tke)
+  ! rcfg:  dtime,toa,dx,dy,klemp_damp_max,klemp_begin_rel,c_s,bg_shear,mean_velocity_area
   subroutine ref_init(idims,icfg,rcfg) bind(C,name="ref_init")
-    integer(c_int), intent(in) :: idims(7),icfg(8)
-    real(c_double), intent(in) :: rcfg(6)
+    integer(c_int), intent(in) :: idims(7),icfg(15)
+    real(c_double), intent(in) :: rcfg(9)
 
     ny = idims(1); nx = idims(2); n_layers = idims(3); n_oro_layers = idims(4)
     n_constituents = idims(5); n_condensed_constituents = idims(6); nsoillays = idims(7)
@@ -86,9 +94,13 @@ contains
     dtime = rcfg(1); toa = rcfg(2); dx = rcfg(3); dy = rcfg(4)
     klemp_damp_max = rcfg(5); klemp_begin_rel = rcfg(6)
     lplane = .false.
-    lmom_diff_h = .false.; lmom_diff_v = .false.; ltemp_diff_h = .false.; ltemp_diff_v = .false.
-    lmass_diff_h = .false.; lmass_diff_v = .false.
-    diff_coeff_scheme_h = "smag"; diff_coeff_scheme_v = "tke"
+    lmom_diff_h = icfg(9)/=0; lmom_diff_v = icfg(10)/=0; ltemp_diff_h = icfg(11)/=0; ltemp_diff_v = icfg(12)/=0
+    lmass_diff_h = icfg(13)/=0; lmass_diff_v = icfg(14)/=0
+    diff_coeff_scheme_h = "smag"
+    if (icfg(15)==1) diff_coeff_scheme_h = "tke"
+    diff_coeff_scheme_v = "tke"
+    c_s = rcfg(7); bg_shear = rcfg(8)
+    eff_hor_res = sqrt(dx*dy)
     lprog_soil_temp = .false.; lsfc_sensible_heat_flux = .false.; lsfc_phase_trans = .false.; lpbl = .false.
 
     ! grid (sizes as in control.F90:101-157)
@@ -106,6 +118,9 @@ contains
     allocate(grid%exner_bg_grad_w(ny,nx,n_levels)); allocate(grid%theta_v_bg(ny,nx,n_layers))
     allocate(grid%exner_bg(ny,nx,n_layers)); allocate(grid%gravity_m_v(ny,nx,n_levels))
     allocate(grid%gravity_potential(ny,nx,n_layers))
+    allocate(grid%dx_dual(ny+1,nx,n_layers)); allocate(grid%dy_dual(ny,nx+1,n_layers))
+    allocate(grid%layer_thickness(ny,nx,n_layers))
+    grid%mean_velocity_area = rcfg(9)
     allocate(grid%land_fraction(ny,nx)); grid%land_fraction = 0._wp
     allocate(grid%lake_fraction(ny,nx)); grid%lake_fraction = 0._wp
 
@@ -164,6 +179,19 @@ contains
     allocate(diag%condensates_sediment_heat(ny,nx,n_layers)); diag%condensates_sediment_heat = 0._wp
     allocate(diag%mass_diff_tendency(ny,nx,n_layers,n_constituents)); diag%mass_diff_tendency = 0._wp
     allocate(diag%a_rain(ny,nx,n_layers)); diag%a_rain = 0._wp
+    allocate(diag%du_dz(ny,nx+1,n_levels)); diag%du_dz = 0._wp
+    allocate(diag%dv_dz(ny+1,nx,n_levels)); diag%dv_dz = 0._wp
+    allocate(diag%n_squared(ny,nx,n_layers)); diag%n_squared = 0._wp
+    allocate(diag%tke(ny,nx,n_layers)); diag%tke = 0._wp
+    allocate(diag%viscosity_molecular(ny,nx,n_layers)); diag%viscosity_molecular = 0._wp
+    allocate(diag%viscosity(ny,nx,n_layers)); diag%viscosity = 0._wp
+    allocate(diag%viscosity_dual(ny+1,nx+1,n_layers)); diag%viscosity_dual = 0._wp
+    allocate(diag%vert_hor_viscosity_u(ny,nx+1,n_levels)); diag%vert_hor_viscosity_u = 0._wp
+    allocate(diag%vert_hor_viscosity_v(ny+1,nx,n_levels)); diag%vert_hor_viscosity_v = 0._wp
+    allocate(diag%mass_diff_coeff_eff_h(ny,nx,n_layers)); diag%mass_diff_coeff_eff_h = 0._wp
+    allocate(diag%mass_diff_coeff_eff_v(ny,nx,n_layers)); diag%mass_diff_coeff_eff_v = 0._wp
+    allocate(diag%temp_diff_coeff_eff_h(ny,nx,n_layers)); diag%temp_diff_coeff_eff_h = 0._wp
+    allocate(diag%temp_diff_coeff_eff_v(ny,nx,n_layers)); diag%temp_diff_coeff_eff_v = 0._wp
   end subroutine ref_init
 
   ! grid field ids (Python side keeps the same table)
@@ -205,6 +233,10 @@ contains
       case(29); n = size(grid%gravity_m_v);   call c_f_pointer(src,p,[n]); grid%gravity_m_v = reshape(p,shape(grid%gravity_m_v))
       case(30); n = size(grid%gravity_potential); call c_f_pointer(src,p,[n])
                 grid%gravity_potential = reshape(p,shape(grid%gravity_potential))
+      case(31); n = size(grid%dx_dual);       call c_f_pointer(src,p,[n]); grid%dx_dual = reshape(p,shape(grid%dx_dual))
+      case(32); n = size(grid%dy_dual);       call c_f_pointer(src,p,[n]); grid%dy_dual = reshape(p,shape(grid%dy_dual))
+      case(33); n = size(grid%layer_thickness); call c_f_pointer(src,p,[n])
+                grid%layer_thickness = reshape(p,shape(grid%layer_thickness))
     end select
   end subroutine ref_set_grid
 
@@ -261,7 +293,9 @@ contains
     end select
   end subroutine ref_state_io
 
-  ! diag field ids: 1 temperature, 2 heating_diss, 3 pot_vort_tend_x, 4 pot_vort_tend_z
+  ! diag field ids: 1 temperature, 2 heating_diss, 3 pot_vort_tend_x, 4 pot_vort_tend_z,
+  ! 5 condensates_sediment_heat, 6 tke, 7 mom_diff_tend_x, 8 mom_diff_tend_y, 9 mom_diff_tend_z,
+  ! 10 temp_diff_heating, 11 mass_diff_tendency, 12 viscosity, 13 n_squared
   subroutine ref_get_diag(id,buf) bind(C,name="ref_get_diag")
     integer(c_int), value, intent(in) :: id
     type(c_ptr),    value, intent(in) :: buf
@@ -273,6 +307,16 @@ contains
       case(4); call c_f_pointer(buf,p,[size(diag%pot_vort_tend_z)]); p = reshape(diag%pot_vort_tend_z,[size(diag%pot_vort_tend_z)])
       case(5); call c_f_pointer(buf,p,[size(diag%condensates_sediment_heat)])
                p = reshape(diag%condensates_sediment_heat,[size(diag%condensates_sediment_heat)])
+      case(6); call c_f_pointer(buf,p,[size(diag%tke)]); p = reshape(diag%tke,[size(diag%tke)])
+      case(7); call c_f_pointer(buf,p,[size(diag%mom_diff_tend_x)]); p = reshape(diag%mom_diff_tend_x,[size(diag%mom_diff_tend_x)])
+      case(8); call c_f_pointer(buf,p,[size(diag%mom_diff_tend_y)]); p = reshape(diag%mom_diff_tend_y,[size(diag%mom_diff_tend_y)])
+      case(9); call c_f_pointer(buf,p,[size(diag%mom_diff_tend_z)]); p = reshape(diag%mom_diff_tend_z,[size(diag%mom_diff_tend_z)])
+      case(10); call c_f_pointer(buf,p,[size(diag%temp_diff_heating)])
+                p = reshape(diag%temp_diff_heating,[size(diag%temp_diff_heating)])
+      case(11); call c_f_pointer(buf,p,[size(diag%mass_diff_tendency)])
+                p = reshape(diag%mass_diff_tendency,[size(diag%mass_diff_tendency)])
+      case(12); call c_f_pointer(buf,p,[size(diag%viscosity)]); p = reshape(diag%viscosity,[size(diag%viscosity)])
+      case(13); call c_f_pointer(buf,p,[size(diag%n_squared)]); p = reshape(diag%n_squared,[size(diag%n_squared)])
     end select
   end subroutine ref_get_diag
 
@@ -305,7 +349,158 @@ contains
     diag%heating_diss = -sum(state%rho(:,:,:,1:n_condensed_constituents+1),4)*diag%heating_diss
   end subroutine harness_dissipation
 
-  ! vector_tend_expl restated (mo_vector_tend_expl.F90:30-182), diffusion and PBL off
+  ! the averaged density sum(rho(1:ncond+1)) of two cells, as mo_momentum_diff_diss.F90 forms it
+  pure function mean_rho2(state,ia,ka,la,ib,kb,lb) result(r)
+    type(t_state), intent(in) :: state
+    integer,       intent(in) :: ia,ka,la,ib,kb,lb
+    real(wp) :: r
+    r = 0.5_wp*(sum(state%rho(ia,ka,la,1:n_condensed_constituents+1)) &
+              + sum(state%rho(ib,kb,lb,1:n_condensed_constituents+1)))
+  end function mean_rho2
+
+  ! vertical derivative of viscosity_dual*zeta_z at dual point (ji,jk) around layer jl, halved
+  ! (the two terms of the terrain-following correction, mo_momentum_diff_diss.F90:75-82)
+  pure function half_dvz(ji,jk,jl) result(r)
+    integer, intent(in) :: ji,jk,jl
+    real(wp) :: r
+    integer  :: lu,ll
+    lu = max(1,jl-1)
+    ll = min(n_layers,jl+1)
+    r = 0.5_wp*(diag%viscosity_dual(ji,jk,lu)*diag%zeta_z(ji,jk,lu) - diag%viscosity_dual(ji,jk,ll)*diag%zeta_z(ji,jk,ll)) &
+        /(grid%z_area_dual_z(ji,jk,lu) - grid%z_area_dual_z(ji,jk,ll))
+  end function half_dvz
+
+  ! mom_diff_h restated (mo_momentum_diff_diss.F90:25-160), periodic boundaries
+  subroutine harness_mom_diff_h(state)
+    type(t_state), intent(in) :: state
+    integer  :: ji,jk,jl
+    real(wp) :: slope
+    call div_h(state%wind_u,state%wind_v,diag%scalar_placeholder,grid)
+    call rel_vort(state,diag,grid)
+    call hor_viscosity(state,diag,grid)
+    diag%scalar_placeholder = diag%viscosity*diag%scalar_placeholder
+    ! the terrain correction of grad_hor reads mom_diff_tend_z as it is at this point
+    call grad_hor(diag%scalar_placeholder,diag%mom_diff_tend_x,diag%mom_diff_tend_y,diag%mom_diff_tend_z,grid)
+    do jl=1,n_layers
+      do jk=1,nx+1
+        do ji=1,ny
+          diag%u_placeholder(ji,jk,jl) = (diag%viscosity_dual(ji+1,jk,jl)*diag%zeta_z(ji+1,jk,jl) - &
+                                          diag%viscosity_dual(ji,jk,jl)*diag%zeta_z(ji,jk,jl))/grid%dy_dual(ji,jk,jl)
+          slope = (grid%z_area_dual_z(ji+1,jk,jl) - grid%z_area_dual_z(ji,jk,jl))/grid%dy_dual(ji,jk,jl)
+          diag%u_placeholder(ji,jk,jl) = diag%u_placeholder(ji,jk,jl) - slope*(half_dvz(ji+1,jk,jl) + half_dvz(ji,jk,jl))
+        enddo
+      enddo
+    enddo
+    do jl=1,n_layers
+      do jk=1,nx
+        do ji=1,ny+1
+          diag%v_placeholder(ji,jk,jl) = (diag%viscosity_dual(ji,jk+1,jl)*diag%zeta_z(ji,jk+1,jl) - &
+                                          diag%viscosity_dual(ji,jk,jl)*diag%zeta_z(ji,jk,jl))/grid%dx_dual(ji,jk,jl)
+          slope = (grid%z_area_dual_z(ji,jk+1,jl) - grid%z_area_dual_z(ji,jk,jl))/grid%dx_dual(ji,jk,jl)
+          diag%v_placeholder(ji,jk,jl) = diag%v_placeholder(ji,jk,jl) - slope*(half_dvz(ji,jk+1,jl) + half_dvz(ji,jk,jl))
+        enddo
+      enddo
+    enddo
+    do jl=1,n_layers
+      do ji=1,ny
+        do jk=1,nx
+          diag%mom_diff_tend_x(ji,jk,jl) = (diag%mom_diff_tend_x(ji,jk,jl) + diag%u_placeholder(ji,jk,jl)) &
+                                           /mean_rho2(state,ji,merge(nx,jk-1,jk==1),jl,ji,jk,jl)
+        enddo
+        diag%mom_diff_tend_x(ji,nx+1,jl) = diag%mom_diff_tend_x(ji,1,jl)
+      enddo
+    enddo
+    do jl=1,n_layers
+      do jk=1,nx
+        do ji=1,ny
+          diag%mom_diff_tend_y(ji,jk,jl) = (diag%mom_diff_tend_y(ji,jk,jl) + diag%v_placeholder(ji,jk,jl)) &
+                                           /mean_rho2(state,merge(ny,ji-1,ji==1),jk,jl,ji,jk,jl)
+        enddo
+        diag%mom_diff_tend_y(ny+1,jk,jl) = diag%mom_diff_tend_y(1,jk,jl)
+      enddo
+    enddo
+  end subroutine harness_mom_diff_h
+
+  ! mom_diff_v restated (mo_momentum_diff_diss.F90:162-331), periodic boundaries
+  subroutine harness_mom_diff_v(state)
+    type(t_state), intent(in) :: state
+    integer :: ji,jk,jl,ka,ia
+    ! vertical gradients of u and v at the levels
+    do jl=2,n_layers
+      diag%du_dz(:,:,jl) = (state%wind_u(:,:,jl-1) - state%wind_u(:,:,jl))/(grid%z_u(:,:,jl-1) - grid%z_u(:,:,jl))
+      diag%dv_dz(:,:,jl) = (state%wind_v(:,:,jl-1) - state%wind_v(:,:,jl))/(grid%z_v(:,:,jl-1) - grid%z_v(:,:,jl))
+    enddo
+    diag%du_dz(:,:,1) = diag%du_dz(:,:,2)
+    diag%dv_dz(:,:,1) = diag%dv_dz(:,:,2)
+    do jk=1,nx
+      ka = merge(nx,jk-1,jk==1)
+      diag%du_dz(:,jk,n_levels) = state%wind_u(:,jk,n_layers)/(grid%z_u(:,jk,n_layers) &
+                                  - 0.5_wp*(grid%z_w(:,ka,n_levels)+grid%z_w(:,jk,n_levels)))
+    enddo
+    diag%du_dz(:,nx+1,n_levels) = diag%du_dz(:,1,n_levels)
+    do ji=1,ny
+      ia = merge(ny,ji-1,ji==1)
+      diag%dv_dz(ji,:,n_levels) = state%wind_v(ji,:,n_layers)/(grid%z_v(ji,:,n_layers) &
+                                  - 0.5_wp*(grid%z_w(ia,:,n_levels)+grid%z_w(ji,:,n_levels)))
+    enddo
+    diag%dv_dz(ny+1,:,n_levels) = diag%dv_dz(1,:,n_levels)
+    ! their vertical diffusion with vert_hor_viscosity_u/_v
+    do jl=1,n_layers
+      do ji=1,ny
+        do jk=1,nx
+          ka = merge(nx,jk-1,jk==1)
+          diag%mom_diff_tend_x(ji,jk,jl) = diag%mom_diff_tend_x(ji,jk,jl) &
+          + (diag%vert_hor_viscosity_u(ji,jk,jl)*diag%du_dz(ji,jk,jl) &
+          - diag%vert_hor_viscosity_u(ji,jk,jl+1)*diag%du_dz(ji,jk,jl+1)) &
+          /(0.5_wp*(grid%z_w(ji,ka,jl) + grid%z_w(ji,jk,jl) - grid%z_w(ji,ka,jl+1) - grid%z_w(ji,jk,jl+1))) &
+          /mean_rho2(state,ji,ka,jl,ji,jk,jl)
+        enddo
+        diag%mom_diff_tend_x(ji,nx+1,jl) = diag%mom_diff_tend_x(ji,1,jl)
+      enddo
+    enddo
+    do jl=1,n_layers
+      do jk=1,nx
+        do ji=1,ny
+          ia = merge(ny,ji-1,ji==1)
+          diag%mom_diff_tend_y(ji,jk,jl) = diag%mom_diff_tend_y(ji,jk,jl) &
+          + (diag%vert_hor_viscosity_v(ji,jk,jl)*diag%dv_dz(ji,jk,jl) &
+          - diag%vert_hor_viscosity_v(ji,jk,jl+1)*diag%dv_dz(ji,jk,jl+1)) &
+          /(0.5_wp*(grid%z_w(ia,jk,jl) + grid%z_w(ji,jk,jl) - grid%z_w(ia,jk,jl+1) - grid%z_w(ji,jk,jl+1))) &
+          /mean_rho2(state,ia,jk,jl,ji,jk,jl)
+        enddo
+        diag%mom_diff_tend_y(ny+1,jk,jl) = diag%mom_diff_tend_y(1,jk,jl)
+      enddo
+    enddo
+    ! vertical diffusion of w
+    diag%scalar_placeholder = 0._wp
+    call add_vertical_div(state%wind_w,diag%scalar_placeholder,grid)
+    call vert_vert_mom_viscosity(state,diag,grid)
+    call grad_vert(diag%scalar_placeholder,diag%mom_diff_tend_z,grid)
+    ! horizontal diffusion of w (w_placeholder is read as it stands, as in the reference)
+    do jl=1,n_layers
+      diag%scalar_placeholder(:,:,jl) = grid%inner_product_weights(5,:,:,jl)*state%wind_w(:,:,jl) &
+                                        + grid%inner_product_weights(6,:,:,jl)*state%wind_w(:,:,jl+1)
+    enddo
+    call grad_hor(diag%scalar_placeholder,diag%u_placeholder,diag%v_placeholder,diag%w_placeholder,grid)
+    do jl=1,n_layers
+      diag%u_placeholder(:,:,jl) = 0.5_wp*(diag%vert_hor_viscosity_u(:,:,jl) + diag%vert_hor_viscosity_u(:,:,jl+1)) &
+                                   *diag%u_placeholder(:,:,jl)
+      diag%v_placeholder(:,:,jl) = 0.5_wp*(diag%vert_hor_viscosity_v(:,:,jl) + diag%vert_hor_viscosity_v(:,:,jl+1)) &
+                                   *diag%v_placeholder(:,:,jl)
+    enddo
+    call div_h(diag%u_placeholder,diag%v_placeholder,diag%scalar_placeholder,grid)
+    do jl=2,n_layers
+      do jk=1,nx
+        do ji=1,ny
+          diag%mom_diff_tend_z(ji,jk,jl) = diag%mom_diff_tend_z(ji,jk,jl) &
+                                           + 0.5_wp*(diag%scalar_placeholder(ji,jk,jl-1) + diag%scalar_placeholder(ji,jk,jl))
+          diag%mom_diff_tend_z(ji,jk,jl) = diag%mom_diff_tend_z(ji,jk,jl)/mean_rho2(state,ji,jk,jl-1,ji,jk,jl)
+        enddo
+      enddo
+    enddo
+  end subroutine harness_mom_diff_v
+
+  ! vector_tend_expl restated (mo_vector_tend_expl.F90:30-182), PBL off
   subroutine harness_vector_tend_expl(state,rk_step,total_step_counter)
     type(t_state), intent(in) :: state
     integer,       intent(in) :: rk_step,total_step_counter
@@ -344,7 +539,19 @@ contains
       endif
     endif
     if (rk_step==1) then
-      if (lklemp) then
+      if (lmom_diff_h .or. lmass_diff_h .or. ltemp_diff_h) then
+        call update_n_squared(state,diag,grid)
+        if (diff_coeff_scheme_h=="tke" .or. diff_coeff_scheme_v=="tke") then
+          call tke_update(state,diag,grid)
+        endif
+      endif
+      if (lmom_diff_h) then
+        call harness_mom_diff_h(state)
+      endif
+      if (lmom_diff_v) then
+        call harness_mom_diff_v(state)
+      endif
+      if (lmom_diff_h .or. lklemp) then
         call harness_dissipation(state)
       endif
     endif

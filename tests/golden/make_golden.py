@@ -25,13 +25,21 @@ CASES = {
     "tiny_dry_o3_freeslip": ({"kind": "tiny", "kw": {"order": 3, "freeslip": True}}, 5),
     "tiny_moist_o3_freeslip": ({"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "cloud_frac": 0.3}, 5),
     "schaer_41x20": ({"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}}, 12),
+    # the six diffusion switches on (the reference default), Smagorinsky or TKE horizontal scheme
+    "tiny_moist_o3_freeslip_diff": ({"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True},
+                                     "cloud_frac": 0.3, "diffusion": True}, 5),
+    "tiny_dry_o2_diff_tke": ({"kind": "tiny", "kw": {}, "diffusion": True,
+                              "diff_kw": {"diff_coeff_scheme_h": "tke"}}, 6),
+    "schaer_41x20_diff": ({"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "diffusion": True}, 12),
 }
+# diagnostics stored with the diffusion cases (t_diag components carried between steps)
+DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_z", "heating_diss")
 
 
 def one(name):
     from compare_ref import build_case
     from pyoracle import RefHarness
-    from lgame.config import GRID_FIELDS
+    from lgame.config import GRID_FIELDS, GRID_SCALARS
 
     spec, steps = CASES[name]
     cfg, (g, st) = build_case(spec)
@@ -40,8 +48,14 @@ def one(name):
     r.run(0, steps)
     out = r.get_state(steps % 2)
     arrays = {f"grid_{k}": g[k] for k in GRID_FIELDS}
+    arrays.update({f"grid_{k}": np.array(g[k]) for k in GRID_SCALARS})
     arrays.update({f"in_{k}": v for k, v in st.items()})
     arrays.update({f"out_{k}": v for k, v in out.items()})
+    if spec.get("diffusion"):
+        ny, nx, nl = cfg.ny, cfg.nx, cfg.n_layers
+        shapes = {"tke": (ny, nx, nl), "viscosity": (ny, nx, nl), "mom_diff_tend_z": (ny, nx, nl + 1),
+                  "heating_diss": (ny, nx, nl)}
+        arrays.update({f"diag_{k}": r.get_diag(k, shapes[k]) for k in DIFF_DIAGS})
     meta = {"config": dataclasses.asdict(cfg), "steps": steps,
             "generator": "oracle/_ref/libref.so (reference modules, flang -O2, no OpenMP)"}
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), meta=np.array(json.dumps(meta)), **arrays)

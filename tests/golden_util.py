@@ -4,7 +4,7 @@ import os
 
 import numpy as np
 
-from lgame.config import GRID_FIELDS, STATE_FIELDS, RunConfig
+from lgame.config import GRID_FIELDS, GRID_SCALARS, STATE_FIELDS, RunConfig
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
@@ -17,9 +17,16 @@ def load(name):
     c["hills"] = tuple(tuple(h) for h in c["hills"])
     cfg = RunConfig(**c)
     grid = {k: np.asfortranarray(z[f"grid_{k}"]) for k in GRID_FIELDS}
+    grid.update({k: float(z[f"grid_{k}"]) for k in GRID_SCALARS})
     st_in = {k: np.asfortranarray(z[f"in_{k}"]) for k in STATE_FIELDS}
     st_out = {k: np.asfortranarray(z[f"out_{k}"]) for k in STATE_FIELDS}
     return cfg, grid, st_in, st_out, meta["steps"]
+
+
+def load_diags(name):
+    """The t_diag fields stored with a fixture (diffusion cases), {} otherwise."""
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    return {k[5:]: np.asfortranarray(z[k]) for k in z.files if k.startswith("diag_")}
 
 
 def max_rel(a, b):

@@ -49,7 +49,9 @@ def _init(mutate_dims=None, mutate_cfg=None):
     (lambda d: setattr(d, "ny", 2), None, ">= 3"),
     (None, lambda c: setattr(c, "theta_adv_order", 4), "theta_adv_order"),
     (None, lambda c: setattr(c, "lperiodic", 0), "lperiodic"),
-    (None, lambda c: setattr(c, "lmom_diff_h", 1), "diffusion"),
+    (None, lambda c: setattr(c, "ltemp_diff_h", 1), "without horizontal momentum diffusion"),
+    (None, lambda c: setattr(c, "diff_coeff_scheme_h", 2), "diff_coeff_scheme_h"),
+    (None, lambda c: setattr(c, "lmom_diff_h", 1), "mean_velocity_area"),
     (None, lambda c: setattr(c, "lmoist", 1), "constituent"),
     (None, lambda c: setattr(c, "dtime", 0.0), "dtime"),
 ])
@@ -61,4 +63,4 @@ def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
 
 
 def test_abi_version():
-    assert load_library().lgame_abi_version() == 1
+    assert load_library().lgame_abi_version() == 2

@@ -1,7 +1,7 @@
 """The C oracle against the golden fixtures written from the reference's compiled modules."""
 import pytest
 
-from golden_util import NAMES, load
+from golden_util import NAMES, load, load_diags
 from pyoracle import CpuOracle
 
 
@@ -16,3 +16,5 @@ def test_oracle_reproduces_golden_bitwise(name):
     got = o.get_state(steps % 2)
     for k in st_out:
         assert np.array_equal(got[k], st_out[k]), k
+    for k, v in load_diags(name).items():
+        assert np.array_equal(o.get_diag(k, v.shape), v), k

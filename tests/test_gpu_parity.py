@@ -34,6 +34,12 @@ def run_both(cfg, nsteps, cloud_frac=0.0, seed=None):
     return got, want, st
 
 
+def diff(cfg, **kw):
+    """cfg with the six diffusion switches on (the reference default), then the changes in kw."""
+    cfg = dataclasses.replace(cfg).set_diffusion(True)
+    return dataclasses.replace(cfg, **kw)
+
+
 CASES = {
     "tiny_dry_o2": (tiny_config(), 4, 0.0),
     "tiny_dry_o3_freeslip": (tiny_config(order=3, freeslip=True), 5, 0.0),
@@ -44,6 +50,17 @@ CASES = {
     "schaer_61x24": (schaer_config(nx=61, n_layers=24), 30, 0.0),
     "dry3d_33x33x20": (dry3d_config(n=33, n_layers=20), 8, 0.0),
     "dry3d_66x40x18_even": (dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66), 6, 0.0),
+    # diffusion and turbulence
+    "tiny_dry_o2_diff": (diff(tiny_config()), 4, 0.0),
+    "tiny_moist_o3_freeslip_diff": (diff(tiny_config(moist=True, order=3, freeslip=True)), 5, 0.3),
+    "tiny_dry_o3_diff_tke": (diff(tiny_config(order=3), diff_coeff_scheme_h="tke"), 8, 0.0),
+    "tiny_moist_diff_no_vertical": (diff(tiny_config(ny=9, nx=11, n_layers=7, moist=True), lmom_diff_v=False,
+                                         ltemp_diff_v=False, lmass_diff_h=False), 5, 0.2),
+    "tiny_plane_diff_mom_v_only": (diff(tiny_config(corio=False, plane=True), lmom_diff_h=False, ltemp_diff_h=False,
+                                        lmass_diff_h=False), 4, 0.0),
+    "schaer_61x24_diff": (diff(schaer_config(nx=61, n_layers=24)), 30, 0.0),
+    "dry3d_33x33x20_diff": (diff(dry3d_config(n=33, n_layers=20)), 8, 0.0),
+    "dry3d_66x40x18_even_diff": (diff(dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66)), 6, 0.0),
 }
 
 
@@ -56,6 +73,33 @@ def test_gpu_matches_oracle(name):
     print(f"{name}: max relative deviation {rel:.3e}, bitwise={bitwise}")
     assert rel <= TOL
     assert np.abs(got["wind_w"] - st["wind_w"]).max() > 0.0
+
+
+DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_x", "mom_diff_tend_z", "temp_diff_heating", "heating_diss")
+
+
+@pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "dry3d_33x33x20_diff"])
+def test_gpu_diffusion_diagnostics_match_oracle(name):
+    """The carried t_diag fields of the diffusion (TKE, viscosity, momentum diffusion tendencies,
+    heating rates) after N steps, field by field."""
+    cfg, n, cf = CASES[name]
+    grid, st = make_case(cfg, cloud_frac=cf)
+    print(f"case {name} steps={n}", flush=True)
+    g = Step(cfg, grid)
+    g.start(st)
+    g.run(0, n)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.run(0, n)
+    for k in DIFF_DIAGS:
+        a = g.diag(k)
+        b = o.get_diag(k, a.shape)
+        scale = max(float(np.abs(b).max()), 1e-300)
+        rel = float(np.abs(a - b).max()) / scale
+        print(f"{name} {k}: max relative deviation {rel:.3e}, bitwise={np.array_equal(a, b)}")
+        assert rel <= TOL, k
+        assert np.abs(b).max() > 0.0, k
+    g.close()
 
 
 @pytest.mark.parametrize("name", NAMES)

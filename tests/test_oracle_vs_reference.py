@@ -23,6 +23,16 @@ CASES = [
     {"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "steps": 5, "cloud_frac": 0.3},
     {"kind": "tiny", "kw": {"ny": 7, "nx": 5, "n_layers": 8, "llinear": True}, "steps": 3},
     {"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "steps": 20},
+    # diffusion and turbulence (the reference default switches), with the t_diag fields they carry
+    {"kind": "tiny", "kw": {}, "steps": 4, "diffusion": True},
+    {"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "steps": 5, "cloud_frac": 0.3,
+     "diffusion": True},
+    {"kind": "tiny", "kw": {"order": 3}, "steps": 8, "diffusion": True, "diff_kw": {"diff_coeff_scheme_h": "tke"}},
+    {"kind": "tiny", "kw": {"moist": True}, "steps": 5, "cloud_frac": 0.3, "diffusion": True,
+     "diff_kw": {"lmom_diff_v": False, "ltemp_diff_v": False, "lmass_diff_h": False}},
+    {"kind": "tiny", "kw": {"plane": True, "corio": False}, "steps": 4, "diffusion": True,
+     "diff_kw": {"lmom_diff_h": False, "ltemp_diff_h": False, "lmass_diff_h": False}},
+    {"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "steps": 12, "diffusion": True},
 ]
 
 
