@@ -1,6 +1,7 @@
 """Benchmark of the MI355X HEVI predictor-corrector step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--form both|mixed|fp64]
+                    [--diffusion]
 
 Runs the step on one GPU with the state resident in HBM, times exactly K steps between two
 stream synchronizations, and prints one JSON line (last line of stdout).  Progress goes to stderr.
@@ -13,7 +14,8 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
   ensemble        48 moist members of uneven size (nx, ny ~ U[64,384], 40-70 layers), mixed form,
                   autotuned block shapes, one graph launch per ensemble step vs serial launches
 Block shapes of the stencil kernels are chosen by lgame_autotune before the timed steps
-(--no-autotune keeps the default 64x4).
+(--no-autotune keeps the default 64x4).  --diffusion turns on the reference's default diffusion
+(momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients).
 
 Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
 and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
@@ -75,7 +77,9 @@ def workload(name):
 def describe(cfg):
     kind = "moist" if cfg.lmoist else "dry"
     rad = f", radiation every {cfg.dtime_rad:g} s" if cfg.lrad else ", radiation off"
-    return f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, theta order {cfg.theta_adv_order}{rad}"
+    diff = (f", diffusion on ({cfg.diff_coeff_scheme_h} horizontal, TKE vertical)" if cfg.diffusion else "")
+    return (f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, "
+            f"theta order {cfg.theta_adv_order}{rad}{diff}")
 
 
 def surrogate_mlp():
@@ -302,6 +306,8 @@ def main():
                     choices=["moist512", "dry128", "schaer", "surrogate-train", "ensemble"])
     ap.add_argument("--members", type=int, default=48)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--diffusion", action="store_true",
+                    help="turn on the six diffusion switches (the reference's default namelist)")
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
@@ -331,6 +337,9 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     cfg, sample_cfg = workload(args.workload)
+    if args.diffusion:
+        cfg = dataclasses.replace(cfg).set_diffusion(True)
+        sample_cfg = dataclasses.replace(sample_cfg).set_diffusion(True)
     forms = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.form]
     mlp = surrogate_mlp() if (cfg.lrad and "mixed" in forms) else None
     t0 = time.perf_counter()
@@ -381,6 +390,7 @@ def main():
         "config": {
             "workload": f"{args.workload}: {describe(cfg)}, {form_text}",
             "form": head,
+            "diffusion": cfg.diffusion,
             "grid_points": cells,
             "nonfinite_after_run": bad,
             "radiation_updates_timed": (1 + (args.steps - 1) // every) if every else 0,

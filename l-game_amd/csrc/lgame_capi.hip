@@ -320,15 +320,9 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     // temperature and mass diffusion with the coefficients of this substep (mo_scalar_tend_expl.F90:
     // 60-99).  The temperature diffusion of the second substep repeats the first one with the same
     // temperature and coefficients, so its result is kept.
-    if (rk == 1 && c.C.ltemp_h) {
-      LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux_temp, nx, ny, nl, c, old);
-      LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c, c.w.tdh, c.C.ltemp_v);
-    }
-    if (rk == 1 && c.C.lmass_h) {
-      for (int jc = 0; jc < c.G.nc; ++jc) {
-        LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux_mass, nx, ny, nl, c, old, jc);
-        LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c, c.w.mdt[jc], c.C.lmass_v);
-      }
+    if (rk == 1 && (c.C.ltemp_h || c.C.lmass_h)) {
+      LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux, nx, ny, nl, c, old);
+      LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
     }
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)
@@ -602,7 +596,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   constexpr int NGRID = 40;
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t ndiff = diffusion ? (size_t)14 + nc : 0;  // the diffusion fields of WorkT
+  const size_t ndiff = diffusion ? (size_t)12 + nc + 3 * (nc + 1) : 0;  // the diffusion fields of WorkT
   const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
@@ -699,10 +693,13 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
       W.mom_y = take();
       W.mom_z = take();
       W.tdh = take();
-      W.dfu = take();
-      W.dfv = take();
-      W.dfw = take();
+      W.rsum = take();
       for (int jc = 0; jc < MAX_CONST; ++jc) W.mdt[jc] = jc < nc ? take() : nullptr;
+      for (int q = 0; q <= MAX_CONST; ++q) {
+        W.dfu[q] = q <= nc ? take() : nullptr;
+        W.dfv[q] = q <= nc ? take() : nullptr;
+        W.dfw[q] = q <= nc ? take() : nullptr;
+      }
     }
     for (int q = 0; q < 12; ++q) W.col[q] = col[q];
     W.vfall_cw = col[12];

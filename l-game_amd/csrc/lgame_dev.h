@@ -104,8 +104,10 @@ struct WorkT {
   // (diag%temp_diff_heating) and mdt (diag%mass_diff_tendency).  Per step: zeta (relative
   // vorticity at the dual points), tke_next, kv (molecular plus vertical eddy viscosity of a
   // cell), vdiv (visc times the horizontal divergence), vz (viscosity_dual times zeta) and the
-  // diffusive fluxes dfu, dfv, dfw of one scalar field.
-  Fld<R> tke, tke_next, visc, kv, vdiv, zeta, vz, mom_x, mom_y, mom_z, tdh, mdt[MAX_CONST], dfu, dfv, dfw;
+  // diffusive fluxes dfu, dfv, dfw of the temperature (index 0) and the densities (1..nc).
+  // rsum is sum(rho(1:ncond+1)) of the old state, stored by k_temperature.
+  Fld<R> tke, tke_next, visc, kv, vdiv, zeta, vz, mom_x, mom_y, mom_z, tdh, mdt[MAX_CONST], rsum;
+  Fld<R> dfu[MAX_CONST + 1], dfv[MAX_CONST + 1], dfw[MAX_CONST + 1];
 };
 
 struct Cfg {

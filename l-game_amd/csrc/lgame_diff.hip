@@ -8,8 +8,8 @@
 //                 the vertical coefficient molecular + tke2vert and viscosity times divergence
 //   k_diff_dual   viscosity_dual times zeta_z (hor_diff_coeff_postprocessing, :161-310)
 //   k_diff_mom    mom_diff_tend_x/_y (mom_diff_h, mo_momentum_diff_diss.F90:25-160)
-//   k_diff_flux_* / k_diff_div   the diffusive flux and its divergence for the temperature and
-//                 for each constituent density (mo_scalar_tend_expl.F90:66-99)
+//   k_diff_flux / k_diff_div   the diffusive fluxes and their divergence for the temperature and
+//                 every constituent density (mo_scalar_tend_expl.F90:66-99)
 // The vertical diffusion of w (mom_diff_v) is formed inside k_dissipation, which reads it.
 // Every kernel keeps the reference's operation order (the library is built with
 // -ffp-contract=off), so the FP64 form is bit-identical to the reference.
@@ -49,10 +49,11 @@ __device__ __forceinline__ R tke2vert(R tke, R n_squared, R layer_thickness) {
   return R(1.0 / 6.0) * mean_free_path * mean_velocity;
 }
 
-// 0.5*(sum(rho(a,1:ncond+1)) + sum(rho(b,1:ncond+1))), mo_momentum_diff_diss.F90:123-159
+// 0.5*(sum(rho(a,1:ncond+1)) + sum(rho(b,1:ncond+1))), mo_momentum_diff_diss.F90:123-159, from
+// the sums k_temperature stored
 template <class R>
-__device__ __forceinline__ R mean_rho2(const CtxT<R> &c, const StateDev &s, ix_t a, ix_t b) {
-  return R(0.5) * (rho_sum(c, s, a) + rho_sum(c, s, b));
+__device__ __forceinline__ R mean_rho2(const CtxT<R> &c, ix_t a, ix_t b) {
+  return R(0.5) * (c.w.rsum[a] + c.w.rsum[b]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const Sta
     u = u - slope * (half_dvz(ji + 1, jk) + hd_c);
     R gx = (vdiv[i] - AT(vdiv, ji, km, jl)) / c.g.dx[i];
     if (jl > G.nflat) gx = gx - c.g.slope_x[i] * remap_x(c, mz, ji, jk, jl);
-    const R mx = (gx + u) / mean_rho2(c, s, LG_IX(G, ji, km, jl), i);
+    const R mx = (gx + u) / mean_rho2(c, LG_IX(G, ji, km, jl), i);
     c.w.mom_x[i] = mx;
     if (jk == 1) AT(c.w.mom_x, ji, nx + 1, jl) = mx;
   }
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const Sta
     v = v - slope * (half_dvz(ji, jk + 1) + hd_c);
     R gy = (AT(vdiv, jm, jk, jl) - vdiv[i]) / c.g.dy[i];
     if (jl > G.nflat) gy = gy - c.g.slope_y[i] * remap_y(c, mz, ji, jk, jl);
-    const R my = (gy + v) / mean_rho2(c, s, LG_IX(G, jm, jk, jl), i);
+    const R my = (gy + v) / mean_rho2(c, LG_IX(G, jm, jk, jl), i);
     c.w.mom_y[i] = my;
     if (ji == 1) AT(c.w.mom_y, ny + 1, jk, jl) = my;
   }
@@ -215,10 +216,18 @@ __global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const Sta
 
 // ---------------------------------------------------------------------------------------------
 // scalar diffusion (mo_scalar_tend_expl.F90:66-99 with scalar_diff_coeffs,
-// mo_eff_diff_coeffs.F90:526-614): the flux coefficient times grad_hor / grad_vert of the field
-// at the u, v and w points of each cell, then the divergence.  Coefficients: temperature
-// c_g_v*viscosity and rho_gas*c_g_v*kv, constituent densities viscosity/rho_gas and kv; c_g_v is
-// spec_heat_cap_diagnostics_v (mo_derived.F90:40-68) with its one relevant constituent.
+// mo_eff_diff_coeffs.F90:526-614) of the temperature (field 0) and of every constituent density
+// (fields 1..nc), in two passes over the cells that each treat all fields: the grid metrics and
+// the coefficients are loaded once per cell.
+//   k_diff_flux: flux coefficient times grad_hor (with its terrain correction) at the u and v
+//                points of the cell and times grad_vert at its upper w level
+//                (scalar_times_vector_h/_v, mo_multiplications.F90:18-66,289-312);
+//   k_diff_div:  div_h of the horizontal fluxes plus add_vertical_div of the vertical one
+//                (mo_divergence_operators.F90:16-68,149-186).
+// Coefficients: temperature c_g_v*viscosity and rho_gas*c_g_v*kv, densities viscosity/rho_gas
+// and kv; c_g_v is spec_heat_cap_diagnostics_v (mo_derived.F90:40-68) with its one relevant
+// constituent.  Column nx+1 and row ny+1 of the horizontal fluxes are copies of column 1 and
+// row 1 (scalar_times_vector_h stores them so).
 // ---------------------------------------------------------------------------------------------
 template <class R>
 __device__ __forceinline__ R c_g_v(R rg) {
@@ -227,69 +236,160 @@ __device__ __forceinline__ R c_g_v(R rg) {
   return r;
 }
 
-template <class R, bool TEMP, class P>
-__device__ __forceinline__ void diff_flux(const CtxT<R> &c, const StateDev &s, P f, bool lvert, int ji, int jk, int jl) {
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_flux(const CtxT<R> c, const StateDev s) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const Geom &G = c.G;
-  const int nx = G.nx, ny = G.ny;
+  const int nx = G.nx, ny = G.ny, nl = G.nl;
+  if (jk > nx || ji > ny || jl > nl) return;
+  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
+  const ix_t dl = (ix_t)G.lvl;
+  const ix_t i = LG_IX(G, ji, jk, jl), iw = LG_IX(G, ji, km, jl), is = LG_IX(G, jm, jk, jl);
+  const bool terr = jl > G.nflat;                 // grad_hor's terrain correction
+  const bool below = jl < nl || c.C.lfreeslip;    // remap_ver2hor reads level jl+1
+  const bool up = jl >= 2, dn = jl < nl;          // grad_vert is nonzero at level jl / jl+1
+  // metrics
+  const R dx = c.g.dx[i], dy = c.g.dy[i];
+  R sx = R(0.0), sy = R(0.0), p4i = R(0.0), p4w = R(0.0), p4s = R(0.0), p5i = R(0.0), p5w = R(0.0), p5s = R(0.0);
+  R dzi = R(1.0), dzw = R(1.0), dzs = R(1.0), dzi1 = R(1.0), dzw1 = R(1.0), dzs1 = R(1.0);
+  if (up) dzi = c.g.dz[i];
+  if (terr) {
+    sx = c.g.slope_x[i];
+    sy = c.g.slope_y[i];
+    p4i = c.g.ipw[4][i];
+    p4w = c.g.ipw[4][iw];
+    p4s = c.g.ipw[4][is];
+    if (up) {
+      dzw = c.g.dz[iw];
+      dzs = c.g.dz[is];
+    }
+    if (below) {
+      p5i = c.g.ipw[5][i];
+      p5w = c.g.ipw[5][iw];
+      p5s = c.g.ipw[5][is];
+    }
+    if (dn) {
+      dzi1 = c.g.dz[i + dl];
+      dzw1 = c.g.dz[iw + dl];
+      dzs1 = c.g.dz[is + dl];
+    }
+  }
+  // coefficients at the cell, its west and south neighbours and the cell above
   const Fld<const double> rgf = s.rho[G.ncond];
-  auto ch = [&](ix_t q) {
-    const R rg = R(rgf[q]);
-    return TEMP ? c_g_v(rg) * c.w.visc[q] : c.w.visc[q] / rg;
+  const R rgi = R(rgf[i]), rgw = R(rgf[iw]), rgs = R(rgf[is]);
+  const R vi = c.w.visc[i], vw = c.w.visc[iw], vs = c.w.visc[is];
+  const R kvi = c.w.kv[i];
+  R kva = R(0.0), rga = R(1.0);
+  if (up) {
+    kva = c.w.kv[i - dl];
+    rga = R(rgf[i - dl]);
+  }
+  // one field: gradients at the u, v and w points, times the averaged coefficients
+  auto one = [&](auto f, Fld<R> du, Fld<R> dv, Fld<R> dw, R chu, R chv, R cvw, bool lvert) {
+    const R fi = R(f[i]), fw = R(f[iw]), fs = R(f[is]);
+    R vgi0 = R(0.0), vgw0 = R(0.0), vgs0 = R(0.0), vgi1 = R(0.0), vgw1 = R(0.0), vgs1 = R(0.0);
+    if (up) vgi0 = (R(f[i - dl]) - fi) / dzi;
+    R gx = (fi - fw) / dx, gy = (fs - fi) / dy;
+    if (terr) {
+      if (up) {
+        vgw0 = (R(f[iw - dl]) - fw) / dzw;
+        vgs0 = (R(f[is - dl]) - fs) / dzs;
+      }
+      if (dn) {
+        vgi1 = (fi - R(f[i + dl])) / dzi1;
+        vgw1 = (fw - R(f[iw + dl])) / dzw1;
+        vgs1 = (fs - R(f[is + dl])) / dzs1;
+      }
+      R r = R(0.0);  // remap_ver2hor_x, mo_averaging.F90:56-95
+      r = r + p4w * vgw0;
+      r = r + p4i * vgi0;
+      if (below) {
+        r = r + p5w * vgw1;
+        r = r + p5i * vgi1;
+      }
+      gx = gx - sx * (R(0.5) * r);
+      r = R(0.0);  // remap_ver2hor_y, :97-136
+      r = r + p4s * vgs0;
+      r = r + p4i * vgi0;
+      if (below) {
+        r = r + p5s * vgs1;
+        r = r + p5i * vgi1;
+      }
+      gy = gy - sy * (R(0.5) * r);
+    }
+    const R fu = chu * gx, fv = chv * gy;
+    du[i] = fu;
+    if (jk == 1) AT(du, ji, nx + 1, jl) = fu;
+    dv[i] = fv;
+    if (ji == 1) AT(dv, ny + 1, jk, jl) = fv;
+    if (lvert && up) dw[i] = cvw * vgi0;
   };
-  auto cv = [&](ix_t q) {
-    const R rg = R(rgf[q]);
-    return TEMP ? rg * c_g_v(rg) * c.w.kv[q] : c.w.kv[q];
-  };
+  if (c.C.ltemp_h) {
+    const R chu = R(0.5) * (c_g_v(rgw) * vw + c_g_v(rgi) * vi);
+    const R chv = R(0.5) * (c_g_v(rgs) * vs + c_g_v(rgi) * vi);
+    const R cvw = R(0.5) * (rga * c_g_v(rga) * kva + rgi * c_g_v(rgi) * kvi);
+    one(Fld<const R>(c.w.temperature), c.w.dfu[0], c.w.dfv[0], c.w.dfw[0], chu, chv, cvw, c.C.ltemp_v != 0);
+  }
+  if (c.C.lmass_h) {
+    const R chu = R(0.5) * (vw / rgw + vi / rgi);
+    const R chv = R(0.5) * (vs / rgs + vi / rgi);
+    const R cvw = R(0.5) * (kva + kvi);
+    for (int jc = 0; jc < G.nc; ++jc)
+      one(Fld<const double>(s.rho[jc]), c.w.dfu[jc + 1], c.w.dfv[jc + 1], c.w.dfw[jc + 1], chu, chv, cvw, c.C.lmass_v != 0);
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  const Geom &G = c.G;
+  const int nl = G.nl, nflat = G.nflat;
+  if (jk > G.nx || ji > G.ny || jl > nl) return;
   const ix_t i = LG_IX(G, ji, jk, jl);
-  const R ch_c = ch(i);
-  {
-    const int km = wrapm(jk, nx);
-    const R fu = R(0.5) * (ch(LG_IX(G, ji, km, jl)) + ch_c) * grad_hor_x(c, f, ji, jk, jl);
-    c.w.dfu[i] = fu;
-    if (jk == 1) AT(c.w.dfu, ji, nx + 1, jl) = fu;
-  }
-  {
-    const int jm = wrapm(ji, ny);
-    const R fv = R(0.5) * (ch(LG_IX(G, jm, jk, jl)) + ch_c) * grad_hor_y(c, f, ji, jk, jl);
-    c.w.dfv[i] = fv;
-    if (ji == 1) AT(c.w.dfv, ny + 1, jk, jl) = fv;
-  }
-  if (lvert && jl >= 2) {
-    const VGrad<R, P> vg{f};
-    c.w.dfw[i] = R(0.5) * (cv(LG_IX(G, ji, jk, jl - 1)) + cv(i)) * vg(c, ji, jk, jl);
-  }
-}
-
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_flux_temp(const CtxT<R> c, const StateDev s) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  diff_flux<R, true>(c, s, Fld<const R>(c.w.temperature), c.C.ltemp_v != 0, ji, jk, jl);
-}
-
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_flux_mass(const CtxT<R> c, const StateDev s, int jc) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  diff_flux<R, false>(c, s, Fld<const double>(s.rho[jc]), c.C.lmass_v != 0, ji, jk, jl);
-}
-
-// div_h of the horizontal fluxes plus add_vertical_div of the vertical one
-// (mo_divergence_operators.F90:16-68,149-186)
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c, Fld<R> out, int lvert) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  const int nl = c.G.nl;
-  if (jk > c.G.nx || ji > c.G.ny || jl > nl) return;
-  const ix_t i = LG_IX(c.G, ji, jk, jl);
-  R d = div_h(c, Fld<const R>(c.w.dfu), Fld<const R>(c.w.dfv), ji, jk, jl);
-  if (lvert) {
-    const R cu = jl == 1 ? R(0.0) : c.w.dfw[i];
-    const R cl = jl == nl ? R(0.0) : AT(c.w.dfw, ji, jk, jl + 1);
-    const R comp_v = cu * c.g.area_z[i] - cl * AT(c.g.area_z, ji, jk, jl + 1);
-    d = d + comp_v / c.g.volume[i];
-  }
-  out[i] = d;
+  // which interfaces carry a contravariant correction, and which layers those need (as in
+  // k_scalar_div): vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1
+  const bool up = jl > nflat;
+  const bool lo = jl >= nflat && jl < nl;
+  const bool lay_m = up && jl != nflat + 1;
+  const bool lay_c = up || (lo && jl != nflat);
+  VccGeo<R> gm{}, gc{}, gp{};
+  if (lay_m) gm = vcc_geo(c, ji, jk, jl - 1);
+  if (lay_c) gc = vcc_geo(c, ji, jk, jl);
+  if (lo) gp = vcc_geo(c, ji, jk, jl + 1);
+  const R ax1 = AT(c.g.area_x, ji, jk + 1, jl), ay0 = c.g.area_y[i], ax0 = c.g.area_x[i], ay1 = AT(c.g.area_y, ji + 1, jk, jl);
+  const R az0 = c.g.area_z[i], az1 = AT(c.g.area_z, ji, jk, jl + 1), V = c.g.volume[i];
+  auto one = [&](int q, Fld<R> out, bool lvert) {
+    const Fld<const R> du = c.w.dfu[q], dv = c.w.dfv[q];
+    const Faces<R> fc = faces<R>(du, dv, G, ji, jk, jl);
+    R cu = R(0.0), cl = R(0.0);
+    if (up) {
+      if (lay_m) cu = vcc_sub(cu, gm, faces<R>(du, dv, G, ji, jk, jl - 1));
+      cu = vcc_sub(cu, gc, fc);
+    }
+    if (lo) {
+      if (jl != nflat) cl = vcc_sub(cl, gc, fc);
+      cl = vcc_sub(cl, gp, faces<R>(du, dv, G, ji, jk, jl + 1));
+    }
+    const R comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
+    R comp_v = R(0.0);
+    if (jl == nflat) {
+      comp_v = -cl * az1;
+    } else if (jl == nl) {
+      comp_v = cu * az0;
+    } else if (jl > nflat) {
+      comp_v = cu * az0 - cl * az1;
+    }
+    R d = (comp_h + comp_v) / V;
+    if (lvert) {
+      const R wu = jl == 1 ? R(0.0) : c.w.dfw[q][i];
+      const R wl = jl == nl ? R(0.0) : AT(c.w.dfw[q], ji, jk, jl + 1);
+      d = d + (wu * az0 - wl * az1) / V;
+    }
+    out[i] = d;
+  };
+  if (c.C.ltemp_h) one(0, c.w.tdh, c.C.ltemp_v != 0);
+  if (c.C.lmass_h)
+    for (int jc = 0; jc < G.nc; ++jc) one(jc + 1, c.w.mdt[jc], c.C.lmass_v != 0);
 }
 
 #define LG_INSTANTIATE(R)                                                                  \
@@ -297,9 +397,8 @@ __global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c, Fld<R> ou
   template __global__ void k_diff_coef<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_dual<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_mom<R>(const CtxT<R>, const StateDev);                   \
-  template __global__ void k_diff_flux_temp<R>(const CtxT<R>, const StateDev);             \
-  template __global__ void k_diff_flux_mass<R>(const CtxT<R>, const StateDev, int);        \
-  template __global__ void k_diff_div<R>(const CtxT<R>, Fld<R>, int);
+  template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
+  template __global__ void k_diff_div<R>(const CtxT<R>);
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE

@@ -327,4 +327,34 @@ __device__ __forceinline__ R rhog(const CtxT<R> &c, const StateDev &s, int ji, i
   return ATR(s.rho[c.G.ncond], ji, jk, jl);
 }
 
+// ---- vertical_contravariant_corr of flux fields at the faces of one cell -----------------------
+// slopes at the four faces of cell (ji, jk) and its inner-product weights 0..3, layer l
+template <class R>
+struct VccGeo {
+  R sx1, sy0, sx0, sy1, w0, w1, w2, w3;
+};
+template <class R>
+__device__ __forceinline__ VccGeo<R> vcc_geo(const CtxT<R> &c, int ji, int jk, int l) {
+  return VccGeo<R>{AT(c.g.slope_x, ji, jk + 1, l), AT(c.g.slope_y, ji, jk, l),    AT(c.g.slope_x, ji, jk, l),
+                   AT(c.g.slope_y, ji + 1, jk, l), AT(c.g.ipw[0], ji, jk, l),     AT(c.g.ipw[1], ji, jk, l),
+                   AT(c.g.ipw[2], ji, jk, l),      AT(c.g.ipw[3], ji, jk, l)};
+}
+
+// a vector field at the four faces of cell (ji, jk): x at jk+1, y at ji, x at jk, y at ji+1
+template <class R>
+struct Faces {
+  R e, n, w, s;
+};
+template <class R, class P>
+__device__ __forceinline__ Faces<R> faces(P vx, P vy, const Geom &G, int ji, int jk, int l) {
+  return Faces<R>{R(vx[LG_IX(G, ji, jk + 1, l)]), R(vy[LG_IX(G, ji, jk, l)]), R(vx[LG_IX(G, ji, jk, l)]),
+                  R(vy[LG_IX(G, ji + 1, jk, l)])};
+}
+
+// r minus the four terms of one layer of vertical_contravariant_corr (mo_averaging.F90:29-50)
+template <class R>
+__device__ __forceinline__ R vcc_sub(R r, const VccGeo<R> &g, const Faces<R> &f) {
+  return r - R(0.5) * f.e * g.sx1 * g.w0 - R(0.5) * f.n * g.sy0 * g.w1 - R(0.5) * f.w * g.sx0 * g.w2 - R(0.5) * f.s * g.sy1 * g.w3;
+}
+
 }  // namespace lgame

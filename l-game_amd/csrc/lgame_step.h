@@ -27,9 +27,8 @@ template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s
 template <class R> __global__ void k_diff_coef(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_dual(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_mom(const CtxT<R> c, const StateDev s);
-template <class R> __global__ void k_diff_flux_temp(const CtxT<R> c, const StateDev s);
-template <class R> __global__ void k_diff_flux_mass(const CtxT<R> c, const StateDev s, int jc);
-template <class R> __global__ void k_diff_div(const CtxT<R> c, Fld<R> out, int lvert);
+template <class R> __global__ void k_diff_flux(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_div(const CtxT<R> c);
 
 // Launch geometry of the horizontal stencil kernels: 64 x-points by BY rows per block (the
 // default; the tile table can choose 1, 2 or 4 rows per kernel).

@@ -29,6 +29,7 @@ __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const 
   c.w.temperature[i] = t;
   const R rs = rho_sum(c, s, i), rg = R(s.rho[ncond][i]);
   c.w.decel[i] = R(C_D_P) * rg / rs;
+  if (c.C.lmom_h || c.C.lmom_v) c.w.rsum[i] = rs;  // read by the momentum diffusion kernels
   if (c.C.lmoist) {
     c.w.pgc[i] = rg / rs - R(1.0);
     c.w.vfall_cw[i] = v_fall_liquid(c, s, ji, jk, jl, (double)t, CLOUD_WATER_RADIUS);
@@ -561,10 +562,10 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
     const R sp_c = sp(jl);
     if (jl >= 2) {
       const ix_t ia = LG_IX(G, ji, jk, jl - 1);
-      mz0 = (sp(jl - 1) - sp_c) / c.g.dz[i] / (R(0.5) * (rho_sum(c, s, ia) + rho_sum(c, s, i)));
+      mz0 = (sp(jl - 1) - sp_c) / c.g.dz[i] / (R(0.5) * (c.w.rsum[ia] + c.w.rsum[i]));
       c.w.mom_z[i] = mz0;
     }
-    if (jl < nl) mz1 = (sp_c - sp(jl + 1)) / c.g.dz[ib] / (R(0.5) * (rho_sum(c, s, i) + rho_sum(c, s, ib)));
+    if (jl < nl) mz1 = (sp_c - sp(jl + 1)) / c.g.dz[ib] / (R(0.5) * (c.w.rsum[i] + c.w.rsum[ib]));
   }
   if (!c.C.lklemp && !c.C.lmom_h) return;  // no heating rate without Klemp damping or momentum diffusion
   R wk0 = mz0, wk1 = mz1;
@@ -588,7 +589,8 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
   } else {  // the horizontal terms are products with zero tendencies
     ip = c.g.ipw[4][i] * R(s.wind_w[i]) * wk0 + c.g.ipw[5][i] * R(s.wind_w[ib]) * wk1;
   }
-  c.w.heating_diss[i] = -rho_sum(c, s, i) * ip;
+  const R rs = (c.C.lmom_h || c.C.lmom_v) ? c.w.rsum[i] : rho_sum(c, s, i);
+  c.w.heating_diss[i] = -rs * ip;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -680,35 +682,6 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_flux(const CtxT<R> c, const 
 // between the gas, theta_v and tracer fluxes: the vector memory path, not arithmetic, bounds
 // these stencils, so each repeated load costs time.  The expressions are those of div_h,
 // vertical_contravariant_corr and div_h_tracers, evaluated in the same order.
-
-// slopes at the four faces of cell (ji, jk) and its inner-product weights 0..3, layer l
-template <class R>
-struct VccGeo {
-  R sx1, sy0, sx0, sy1, w0, w1, w2, w3;
-};
-template <class R>
-__device__ __forceinline__ VccGeo<R> vcc_geo(const CtxT<R> &c, int ji, int jk, int l) {
-  return VccGeo<R>{AT(c.g.slope_x, ji, jk + 1, l), AT(c.g.slope_y, ji, jk, l),    AT(c.g.slope_x, ji, jk, l),
-                   AT(c.g.slope_y, ji + 1, jk, l), AT(c.g.ipw[0], ji, jk, l),     AT(c.g.ipw[1], ji, jk, l),
-                   AT(c.g.ipw[2], ji, jk, l),      AT(c.g.ipw[3], ji, jk, l)};
-}
-
-// a vector field at the four faces of cell (ji, jk): x at jk+1, y at ji, x at jk, y at ji+1
-template <class R>
-struct Faces {
-  R e, n, w, s;
-};
-template <class R, class P>
-__device__ __forceinline__ Faces<R> faces(P vx, P vy, const Geom &G, int ji, int jk, int l) {
-  return Faces<R>{R(vx[LG_IX(G, ji, jk + 1, l)]), R(vy[LG_IX(G, ji, jk, l)]), R(vx[LG_IX(G, ji, jk, l)]),
-                  R(vy[LG_IX(G, ji + 1, jk, l)])};
-}
-
-// r minus the four terms of one layer of vertical_contravariant_corr (mo_averaging.F90:29-50)
-template <class R>
-__device__ __forceinline__ R vcc_sub(R r, const VccGeo<R> &g, const Faces<R> &f) {
-  return r - R(0.5) * f.e * g.sx1 * g.w0 - R(0.5) * f.n * g.sy0 * g.w1 - R(0.5) * f.w * g.sx0 * g.w2 - R(0.5) * f.s * g.sy1 * g.w3;
-}
 
 // divergences -> tend%rho, tend%rhotheta_v (scalar_tend_expl, mo_scalar_tend_expl.F90:24-181,
 // with div_h, mo_divergence_operators.F90:16-68, and div_h_tracers with upstream face values,

@@ -26,6 +26,10 @@ CASES = {
     "schaer_61x24": (schaer_config(nx=61, n_layers=24), 60, 0.0),
     "dry3d_33x33x20": (dry3d_config(n=33, n_layers=20), 20, 0.0),
     "moist3d_33x33x30": (None, 20, 0.2),
+    # the reference's default diffusion (TKE, Smagorinsky viscosity, momentum/temperature/mass)
+    "tiny_moist_o3_freeslip_diff": (dataclasses.replace(tiny_config(moist=True, order=3, freeslip=True)).set_diffusion(),
+                                    10, 0.3),
+    "dry3d_33x33x20_diff": (dataclasses.replace(dry3d_config(n=33, n_layers=20)).set_diffusion(), 20, 0.0),
 }
 
 

@@ -111,3 +111,69 @@ def test_oracle_integrals_closed_form(moist):
     want[3] = 717.942189 * rho_c[gc] * temp * vol
     want[4:] = rho_c * vol
     np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+def _shear_case(axis):
+    """Flat periodic plane (dx != dy) at rest, with u = A sin(2 pi y/Ly) (axis 0) or
+    v = A sin(2 pi x/Lx) (axis 1), every diffusion switch on, TKE horizontal scheme."""
+    cfg = dataclasses.replace(tiny_config(ny=12, nx=10, n_layers=6, plane=True, corio=False), orography_id=0, hills=(),
+                              dx=2000.0, dy=1500.0, diff_coeff_scheme_h="tke").set_diffusion()
+    grid, st = make_case(cfg, perturbation=False)
+    amp = 3.0
+    if axis == 0:
+        prof = amp * np.sin(2.0 * np.pi * np.arange(cfg.ny) / cfg.ny)
+        st["wind_u"][:] = prof[:, None, None]
+    else:
+        prof = amp * np.sin(2.0 * np.pi * np.arange(cfg.nx) / cfg.nx)
+        st["wind_v"][:, :, :] = prof[None, :, None]
+    return cfg, grid, st
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_oracle_momentum_diffusion_is_the_discrete_laplacian(axis):
+    """mom_diff_h (mo_momentum_diff_diss.F90:25-160) on a divergence-free shear flow over flat
+    ground is the curl-of-vorticity term alone.  With a horizontally uniform viscosity (the TKE
+    scheme with zero TKE leaves the molecular part) it is the second difference of the wind along
+    the shear direction: the tendency over the wind is -visc*(4/d^2)*sin^2(pi/N) at every point,
+    with visc the density-weighted viscosity of the cell (diag%viscosity; viscosity_dual times
+    zeta divided by the averaged density reduces to it), d the grid distance across the shear
+    (dy for u(y), dx for v(x)) and N the number of points along it."""
+    cfg, grid, st = _shear_case(axis)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.run(0, 1)
+    ny, nx, nl = cfg.ny, cfg.nx, cfg.n_layers
+    visc = o.get_diag("viscosity", (ny, nx, nl))
+    assert np.ptp(visc, axis=(0, 1)).max() <= 1e-12 * np.abs(visc).max()  # uniform per layer
+    if axis == 0:
+        tend = o.get_diag("mom_diff_tend_x", (ny, nx + 1, nl))[:, :nx]
+        wind, d, n = st["wind_u"][:, :nx], cfg.dy, ny
+    else:
+        tend = o.get_diag("mom_diff_tend_y", (ny + 1, nx, nl))[:ny]
+        wind, d, n = st["wind_v"][:ny], cfg.dx, nx
+    mask = np.abs(wind) > 0.5
+    ratio = tend[mask] / wind[mask]
+    want = np.broadcast_to(-visc[0, 0, :] * 4.0 / d ** 2 * np.sin(np.pi / n) ** 2, wind.shape)[mask]
+    np.testing.assert_allclose(ratio, want, rtol=1e-9)
+
+
+def test_oracle_scalar_diffusion_conserves_heat_and_mass():
+    """The diffusive temperature heating and mass tendencies are divergences of fluxes: their
+    volume integrals vanish on the periodic plane whose terrain vanishes at the edges."""
+    cfg = conservative_case(True).set_diffusion()
+    grid, st = make_case(cfg, cloud_frac=0.1)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.run(0, 3)
+    ny, nx, nl, nc = cfg.ny, cfg.nx, cfg.n_layers, cfg.n_constituents
+    vol = grid["volume"]
+    tdh = o.get_diag("temp_diff_heating", (ny, nx, nl))
+    md = o.get_diag("mass_diff_tendency", (ny, nx, nl, nc))
+    assert np.abs(tdh).max() > 0.0
+    assert abs(np.sum(tdh * vol)) <= 1e-12 * np.sum(np.abs(tdh) * vol)
+    for jc in range(nc):
+        m = md[:, :, :, jc]
+        if np.abs(m).max() > 0.0:
+            assert abs(np.sum(m * vol)) <= 1e-12 * np.sum(np.abs(m) * vol), jc
+    tke = o.get_diag("tke", (ny, nx, nl))
+    assert tke.min() >= 0.0 and tke.max() > 0.0
