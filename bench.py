@@ -1,7 +1,7 @@
 """Benchmark of the MI355X HEVI predictor-corrector step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--form both|mixed|fp64]
-                    [--diffusion]
+                    [--diffusion] [--pbl]
 
 Runs the step on one GPU with the state resident in HBM, times exactly K steps between two
 stream synchronizations, and prints one JSON line (last line of stdout).  Progress goes to stderr.
@@ -15,7 +15,8 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
                   autotuned block shapes, one graph launch per ensemble step vs serial launches
 Block shapes of the stencil kernels are chosen by lgame_autotune before the timed steps
 (--no-autotune keeps the default 64x4).  --diffusion turns on the reference's default diffusion
-(momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients).
+(momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients); --pbl the
+PBL friction of the lowest-layer wind with the surface turbulence quantities.
 
 Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
 and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
@@ -78,6 +79,7 @@ def describe(cfg):
     kind = "moist" if cfg.lmoist else "dry"
     rad = f", radiation every {cfg.dtime_rad:g} s" if cfg.lrad else ", radiation off"
     diff = (f", diffusion on ({cfg.diff_coeff_scheme_h} horizontal, TKE vertical)" if cfg.diffusion else "")
+    diff += ", PBL friction on" if cfg.lpbl else ""
     return (f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, "
             f"theta order {cfg.theta_adv_order}{rad}{diff}")
 
@@ -308,6 +310,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--diffusion", action="store_true",
                     help="turn on the six diffusion switches (the reference's default namelist)")
+    ap.add_argument("--pbl", action="store_true",
+                    help="turn on the PBL friction and surface turbulence quantities (lpbl)")
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
@@ -340,6 +344,9 @@ def main():
     if args.diffusion:
         cfg = dataclasses.replace(cfg).set_diffusion(True)
         sample_cfg = dataclasses.replace(sample_cfg).set_diffusion(True)
+    if args.pbl:
+        cfg = dataclasses.replace(cfg, lpbl=True)
+        sample_cfg = dataclasses.replace(sample_cfg, lpbl=True)
     forms = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[args.form]
     mlp = surrogate_mlp() if (cfg.lrad and "mixed" in forms) else None
     t0 = time.perf_counter()
@@ -391,6 +398,7 @@ def main():
             "workload": f"{args.workload}: {describe(cfg)}, {form_text}",
             "form": head,
             "diffusion": cfg.diffusion,
+            "pbl": cfg.lpbl,
             "grid_points": cells,
             "nonfinite_after_run": bad,
             "radiation_updates_timed": (1 + (args.steps - 1) // every) if every else 0,

@@ -131,6 +131,12 @@ typedef struct lgame_dims {
  * updates the TKE a second time; lgame_init rejects those combinations with LGAME_EINVAL.
  * diff_coeff_scheme_v has one valid value in the reference ("tke", mo_diff_nml.F90:95-98), so it
  * has no field here.
+ *
+ * lpbl (mo_surface_nml.F90) turns on the surface turbulence quantities (update_sfc_turb_quantities,
+ * mo_pbl.F90:19-109) and the PBL friction of the lowest-layer wind (pbl_wind_tendency,
+ * :111-250), with h_prandtl and karman (mo_diff_nml.F90:39-44, defaults 100 m and 0.4).  The
+ * prognostic soil, the surface sensible heat flux and the surface phase transitions are not part
+ * of this library (lgame_init rejects them).
  */
 typedef struct lgame_cfg {
   double dtime;
@@ -142,6 +148,8 @@ typedef struct lgame_cfg {
   double dtime_rad;
   double c_s;                /* Smagorinsky parameter                                                  */
   double bg_shear;           /* minimum background shear [1/s]                                         */
+  double h_prandtl;          /* height of the Prandtl layer [m] (PBL)                                  */
+  double karman;             /* von Karman's constant (PBL)                                            */
   int theta_adv_order;       /* 2 or 3                                                                 */
   int luse_bg_state;
   int llinear;
@@ -201,6 +209,8 @@ typedef struct lgame_grid_view {
   const double *dx_dual;               /* (ny+1,nx,nl)   read only with diffusion */
   const double *dy_dual;               /* (ny,nx+1,nl)   read only with diffusion */
   const double *layer_thickness;       /* (ny,nx,nl)     read only with diffusion */
+  const double *land_fraction;         /* (ny,nx)        read only with lpbl      */
+  const double *roughness_length;      /* (ny,nx)        read only with lpbl      */
   double mean_velocity_area;           /* grid%mean_velocity_area, mo_grid_generator.F90:1111 */
 } lgame_grid_view;
 
@@ -245,6 +255,8 @@ typedef struct lgame_rad_view {
 #define LGAME_DIAG_MOM_DIFF_TEND_Z 4   /* diag%mom_diff_tend_z (ny,nx,nl+1)                         */
 #define LGAME_DIAG_TEMP_DIFF_HEATING 5 /* diag%temp_diff_heating (ny,nx,nl)                         */
 #define LGAME_DIAG_HEATING_DISS    6   /* diag%heating_diss (ny,nx,nl)                              */
+#define LGAME_DIAG_ROUGHNESS_LENGTH 7  /* diag%roughness_length (ny,nx), with lpbl                  */
+#define LGAME_DIAG_MONIN_OBUKHOV   8   /* diag%monin_obukhov_length (ny,nx), with lpbl              */
 
 /* Version of this ABI (LGAME_ABI_VERSION). */
 int lgame_abi_version(void);

@@ -114,6 +114,16 @@ double klemp_prefactor(const lgame_cfg &cf, double z) {
   return 1.0 + coeff * cf.dtime;
 }
 
+// diag%roughness_length = grid%roughness_length (control.F90:456-459): the carried surface
+// roughness starts from the grid's, at lgame_init and whenever the work arrays are zeroed again
+hipError_t init_surface(lgame_handle *h) {
+  if (!h->cfg.lpbl) return hipSuccess;
+  const size_t n = (size_t)h->c.G.lvl;
+  hipError_t e = h->mixed ? hipMemcpyAsync(h->cf.w.sfc.p, h->cf.g.roughness_length.p, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream)
+                          : hipMemcpyAsync(h->c.w.sfc.p, h->c.g.roughness_length.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
+}
+
 int upload(lgame_handle *h, double *dst, const double *host, int n1, int n2, int n3, int stride = 1, int ncomp = 1) {
   const size_t n = (size_t)n1 * n2 * n3 * stride;
   if (n > h->staging_elems) return fail(h, LGAME_EINVAL, "internal: staging buffer too small");
@@ -280,6 +290,8 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
   } while (0)
 
   LG_LAUNCH(KI_TEMPERATURE, k_temperature, nx, ny, nl, c, old);
+  // update_sfc_turb_quantities with state_old (mo_manage_pchevi.F90:52-54)
+  if (c.C.lpbl) LG_LAUNCH(KI_DIFF_DUAL, k_sfc_turb, nx, ny, 1, c, old);
   // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
   if (rad) {
     if (prof) prof->begin(3);
@@ -315,7 +327,9 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
       LG_LAUNCH(KI_DIFF_DUAL, k_diff_dual, nx, ny, nl, c, old);
       LG_LAUNCH_ZFAST(KI_DIFF_MOM, k_diff_mom, nx, ny, nl, c, old);
     }
-    if (rk == 1 && (c.C.lklemp || c.C.lmom_h || c.C.lmom_v)) LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
+    if (rk == 1 && c.C.lpbl) LG_LAUNCH(KI_DIFF_DUAL, k_pbl_wind, nx, ny, 1, c, old);
+    if (rk == 1 && (c.C.lklemp || c.C.lmom_h || c.C.lmom_v || c.C.lpbl))
+      LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
     LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
     // temperature and mass diffusion with the coefficients of this substep (mo_scalar_tend_expl.F90:
     // 60-99).  The temperature diffusion of the second substep repeats the first one with the same
@@ -504,10 +518,13 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
                 "(scalar_diff_coeffs, mo_eff_diff_coeffs.F90:540-590, then reads stale vorticity and divergence)");
   if (f.diff_coeff_scheme_h != 0 && f.diff_coeff_scheme_h != 1)
     return fail(nullptr, LGAME_EINVAL, "diff_coeff_scheme_h must be 0 (smag) or 1 (tke)");
-  const bool diffusion = f.lmom_diff_h || f.lmom_diff_v;  // temperature and mass diffusion need lmom_diff_h
+  // the diffusion work arrays (temperature and mass diffusion need lmom_diff_h; the PBL friction
+  // adds to the momentum diffusion tendencies)
+  const bool diffusion = f.lmom_diff_h || f.lmom_diff_v || f.lpbl;
   if (f.lmom_diff_h && !(grid->mean_velocity_area > 0.0)) return fail(nullptr, LGAME_EINVAL, "mean_velocity_area must be positive");
-  if (f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
-    return fail(nullptr, LGAME_EINVAL, "unsupported: PBL, surface fluxes and soil must be off");
+  if (f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
+    return fail(nullptr, LGAME_EINVAL, "unsupported: the surface sensible heat flux, surface phase transitions and soil must be off");
+  if (f.lpbl && !(f.karman > 0.0 && f.h_prandtl > 0.0)) return fail(nullptr, LGAME_EINVAL, "karman and h_prandtl must be positive");
   if (f.rad_mode != LGAME_RAD_OFF && f.rad_mode != LGAME_RAD_TEACHER && f.rad_mode != LGAME_RAD_SURROGATE)
     return fail(nullptr, LGAME_EINVAL, "unknown radiation mode %d", f.rad_mode);
   if (f.lrad && f.rad_mode == LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "lrad needs a radiation mode (LGAME_RAD_*)");
@@ -525,6 +542,9 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   NEED(exner_bg_grad_w) NEED(theta_v_bg) NEED(exner_bg) NEED(gravity_m_v) NEED(gravity_potential)
   if (f.lmom_diff_h) {
     NEED(dx_dual) NEED(dy_dual) NEED(layer_thickness)
+  }
+  if (f.lpbl) {
+    NEED(land_fraction) NEED(roughness_length)
   }
 #undef NEED
 
@@ -570,6 +590,9 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   C.bg_shear = f.bg_shear;
   C.eff_hor_res = std::sqrt(f.dx * f.dy);  // mo_run_nml.F90:93
   C.mva = grid->mean_velocity_area;
+  C.lpbl = f.lpbl != 0;
+  C.h_prandtl = f.h_prandtl;
+  C.karman = f.karman;
 
   // n_damping_levels, as mo_grid_generator.F90:1050-1058 derives it from z_w
   {
@@ -596,7 +619,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   constexpr int NGRID = 40;
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t ndiff = diffusion ? (size_t)12 + nc + 3 * (nc + 1) : 0;  // the diffusion fields of WorkT
+  const size_t ndiff = diffusion ? (size_t)13 + nc + 3 * (nc + 1) : 0;  // the diffusion fields of WorkT
   const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
@@ -694,6 +717,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
       W.mom_z = take();
       W.tdh = take();
       W.rsum = take();
+      W.sfc = take();
       for (int jc = 0; jc < MAX_CONST; ++jc) W.mdt[jc] = jc < nc ? take() : nullptr;
       for (int q = 0; q <= MAX_CONST; ++q) {
         W.dfu[q] = q <= nc ? take() : nullptr;
@@ -736,7 +760,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     G_Z_SCALAR, G_Z_U, G_Z_V, G_Z_W, G_DX, G_DY, G_DZ, G_AREA_X, G_AREA_Y, G_AREA_Z, G_VOLUME, G_SLOPE_X, G_SLOPE_Y,
     G_IPW, G_AREA_DUAL_X = G_IPW + 6, G_AREA_DUAL_Y, G_AREA_DUAL_Z, G_Z_AREA_DUAL_Z, G_FVEC_X, G_FVEC_Y, G_FVEC_Z,
     G_EBG_U, G_EBG_V, G_EBG_W, G_THETA_V_BG, G_EXNER_BG, G_GRAVITY_M_V, G_GRAVITY_POTENTIAL, G_KLEMP_PREF, G_DX_DUAL,
-    G_DY_DUAL, G_LAYER_THICKNESS, G_COUNT
+    G_DY_DUAL, G_LAYER_THICKNESS, G_LAND_FRACTION, G_ROUGHNESS_LENGTH, G_COUNT
   };
   static_assert(G_COUNT <= NGRID, "grid field budget");
   auto gf = [&](int q) { return grid64 + (size_t)q * fs; };
@@ -773,6 +797,10 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     rc = rc ? rc : upload(h, gf(G_DX_DUAL), grid->dx_dual, ny + 1, nx, nl);
     rc = rc ? rc : upload(h, gf(G_DY_DUAL), grid->dy_dual, ny, nx + 1, nl);
     rc = rc ? rc : upload(h, gf(G_LAYER_THICKNESS), grid->layer_thickness, ny, nx, nl);
+  }
+  if (f.lpbl) {
+    rc = rc ? rc : upload(h, gf(G_LAND_FRACTION), grid->land_fraction, ny, nx, 1);
+    rc = rc ? rc : upload(h, gf(G_ROUGHNESS_LENGTH), grid->roughness_length, ny, nx, 1);
   }
   if (!rc) {
     std::vector<double> kp((size_t)ny * nx * nlev);
@@ -835,6 +863,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     g.dx_dual = at(G_DX_DUAL);
     g.dy_dual = at(G_DY_DUAL);
     g.layer_thickness = at(G_LAYER_THICKNESS);
+    g.land_fraction = at(G_LAND_FRACTION);
+    g.roughness_length = at(G_ROUGHNESS_LENGTH);
     g.trsk_u = trsk;
     g.trsk_v = trsk + 3 * ny;
   };
@@ -845,6 +875,12 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     h->cf.C = h->c.C;
     assign_grid(h->cf.g, grid32, trsk32);
     h->cf.g64 = h->c.g;
+  }
+  e = init_surface(h);
+  if (e != hipSuccess) {
+    fail(nullptr, LGAME_EHIP, "surface initialisation: %s", hipGetErrorString(e));
+    lgame_finalize(h);
+    return LGAME_EHIP;
   }
   *out = h;
   return LGAME_OK;
@@ -982,7 +1018,7 @@ int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad) {
 int lgame_get_diag(lgame_handle *h, int which, double *out) {
   if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
   const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers;
-  const bool diffusion = h->cfg.lmom_diff_h || h->cfg.lmom_diff_v;
+  const bool diffusion = h->cfg.lmom_diff_h || h->cfg.lmom_diff_v || h->cfg.lpbl;
   auto get = [&](auto &w) -> int {
     switch (which) {
       case LGAME_DIAG_TEMPERATURE: return download(h, out, w.temperature.p, ny, nx, nl);
@@ -996,6 +1032,10 @@ int lgame_get_diag(lgame_handle *h, int which, double *out) {
       case LGAME_DIAG_MOM_DIFF_TEND_X: return download(h, out, w.mom_x.p, ny, nx + 1, nl);
       case LGAME_DIAG_MOM_DIFF_TEND_Z: return download(h, out, w.mom_z.p, ny, nx, nl + 1);
       case LGAME_DIAG_TEMP_DIFF_HEATING: return download(h, out, w.tdh.p, ny, nx, nl);
+      case LGAME_DIAG_ROUGHNESS_LENGTH:
+      case LGAME_DIAG_MONIN_OBUKHOV:
+        if (!h->cfg.lpbl) return fail(h, LGAME_EINVAL, "diagnostic %d exists only with lpbl", which);
+        return download(h, out, w.sfc.p + (which == LGAME_DIAG_MONIN_OBUKHOV ? 2 * (size_t)h->c.G.lvl : 0), ny, nx, 1);
       default: return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
     }
   };
@@ -1066,6 +1106,7 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
   hipError_t e = hipMemcpyAsync(h->slab, backup, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->slab + h->work_off, 0, h->work_bytes, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = init_surface(h);
   (void)hipFree(backup);
   if (e != hipSuccess) return fail(h, LGAME_EHIP, "autotune restore: %s", hipGetErrorString(e));
   if (rc) return fail(h, rc, "autotune step failed");

@@ -74,6 +74,7 @@ struct GridT {
   Fld<const R> exner_bg, gravity_m_v, gravity_potential;
   Fld<const R> klemp_pref;  // 1 + damping_coeff*dtime at w levels (mo_column_solvers.F90:190-196)
   Fld<const R> dx_dual, dy_dual, layer_thickness;  // read by the diffusion kernels only
+  Fld<const R> land_fraction, roughness_length;    // 2-D (level 1), read by the PBL kernels only
   const R *trsk_u;          // (3, ny) as in the reference
   const R *trsk_v;          // (2, ny+1)
 };
@@ -108,6 +109,8 @@ struct WorkT {
   // rsum is sum(rho(1:ncond+1)) of the old state, stored by k_temperature.
   Fld<R> tke, tke_next, visc, kv, vdiv, zeta, vz, mom_x, mom_y, mom_z, tdh, mdt[MAX_CONST], rsum;
   Fld<R> dfu[MAX_CONST + 1], dfv[MAX_CONST + 1], dfw[MAX_CONST + 1];
+  // PBL: diag%roughness_length, roughness_velocity, monin_obukhov_length at levels 1, 2, 3
+  Fld<R> sfc;
 };
 
 struct Cfg {
@@ -115,8 +118,8 @@ struct Cfg {
   int order, llinear, lcorio, lmoist, lklemp, lfreeslip, n_damping_levels;
   // diffusion (mo_diff_nml.F90); eff_hor_res = sqrt(dx*dy) (mo_run_nml.F90:93), mva =
   // grid%mean_velocity_area (mo_grid_generator.F90:1111)
-  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_tke;
-  double c_s, bg_shear, eff_hor_res, mva;
+  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_tke, lpbl;
+  double c_s, bg_shear, eff_hor_res, mva, h_prandtl, karman;
 };
 
 template <class R>

@@ -392,13 +392,132 @@ __global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
     for (int jc = 0; jc < G.nc; ++jc) one(jc + 1, c.w.mdt[jc], c.C.lmass_v != 0);
 }
 
+// ---------------------------------------------------------------------------------------------
+// planetary boundary layer (mo_pbl.F90): surface turbulence quantities of each column at the
+// start of the step, and the friction of the lowest-layer wind in the first substep.  sfc holds
+// the carried diag%roughness_length (level 1), roughness_velocity (2) and monin_obukhov_length (3)
+// as 2-D fields.  The transcendental functions are the device's, so these results agree with the
+// reference to rounding, not bit for bit.
+// ---------------------------------------------------------------------------------------------
+constexpr double EPSILON_SECURITY = 1e-10;  // mo_constants.F90:34
+constexpr double GRAVITY = 9.80616;         // mo_constants.F90:21
+
+template <class R>
+__device__ __forceinline__ R fmax2(R a, R b) { return a > b ? a : b; }
+template <class R>
+__device__ __forceinline__ R fmin2(R a, R b) { return a < b ? a : b; }
+
+// roughness_length_from_u10_sea, mo_pbl.F90:252-272
+template <class R>
+__device__ R roughness_length_from_u10_sea(R u10) {
+  const R swh = R(0.0248) * (u10 * u10);
+  const R period = R(0.729) * u10;
+  const R wavelength = R(GRAVITY) * (period * period) / R(2.0 * MPI);
+  const R r = R(1200.0) * swh * pow(swh / fmax2(wavelength, R(EPSILON_SECURITY)), R(4.5));
+  return fmax2(R(0.0001), r);
+}
+
+// roughness_velocity, mo_pbl.F90:344-366
+template <class R>
+__device__ R roughness_velocity(const CtxT<R> &c, R wind_speed, R z_agl, R z0) {
+  R den = log(z_agl / z0);
+  if (fabs(den) < R(EPSILON_SECURITY)) den = R(EPSILON_SECURITY);
+  return fmax2(R(EPSILON_SECURITY), wind_speed * R(c.C.karman) / den);
+}
+
+// psi_m, mo_pbl.F90:392-420
+template <class R>
+__device__ R psi_m(R eff, R l) {
+  R ll = l;
+  if (fabs(ll) < R(EPSILON_SECURITY)) ll = R(EPSILON_SECURITY);
+  if (ll < R(0.0)) {
+    const R x = pow(R(1.0) - R(15.0) * eff / ll, R(0.25));
+    return R(2.0) * log((R(1.0) + x) / R(2.0)) + log((R(1.0) + x * x) / R(2.0)) - R(2.0) * atan(x) + R(MPI / 2.0);
+  }
+  return -(R(4.7) * eff / ll);
+}
+
+// the PBL friction at a lowest-layer u or v point between cells a and b (mo_pbl.F90:125-160,
+// with momentum_flux_resistance, :318-342)
+template <class R>
+__device__ R pbl_friction(const CtxT<R> &c, R wind, R z_pt, ix_t a, ix_t b, ix_t as, ix_t bs, ix_t dl) {
+  const Fld<const R> vsq = c.w.v_squared, zw = c.g.z_w, sfc = c.w.sfc;
+  const R ws = R(0.5) * (sqrt(vsq[a]) + sqrt(vsq[b]));
+  const R z_agl = z_pt - R(0.5) * (zw[a + dl] + zw[b + dl]);
+  const R lt = R(0.5) * (zw[a] + zw[b]) - R(0.5) * (zw[a + dl] + zw[b + dl]);
+  const R z0 = R(0.5) * (sfc[as] + sfc[bs]);
+  const R l = R(0.5) * (sfc[as + 2 * dl] + sfc[bs + 2 * dl]);
+  const R hp = R(c.C.h_prandtl), dt = R(c.C.dtime);
+  const R used = fmin2(z_agl, hp);
+  R fr = R(1.0) / (R(c.C.karman) * roughness_velocity(c, ws, z_agl, z0)) * (log(used / z0) - psi_m(used, l));
+  if (fr < dt / z_agl) fr = dt / z_agl;
+  R f = R(1.0);
+  if (z_agl > hp) f = log(hp / z0) / log(z_agl / z0);
+  return f * wind / fr / lt;
+}
+
+// update_sfc_turb_quantities, mo_pbl.F90:19-109, one column per thread (no prognostic soil, so no
+// scalar flux resistance); v_squared is the kinetic energy the last momentum advection block left
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const StateDev s) {
+  KIDX;
+  const Geom &G = c.G;
+  const int nl = G.nl;
+  if (jk > G.nx || ji > G.ny || jl > 1) return;
+  const ix_t dl = (ix_t)G.lvl;
+  const ix_t i1 = LG_IX(G, ji, jk, 1), il = LG_IX(G, ji, jk, nl);
+  const Fld<R> sfc = c.w.sfc;
+  const R agl = c.g.z_scalar[il] - c.g.z_w[il + dl];
+  const R u_low = sqrt(c.w.v_squared[il]);
+  R z0 = sfc[i1];
+  const R u10 = u_low * log(R(10.0) / z0) / log(agl / z0);
+  const R lf = c.g.land_fraction[i1];
+  if (lf < R(1.0)) {
+    z0 = lf * c.g.roughness_length[i1] + (R(1.0) - lf) * roughness_length_from_u10_sea(u10);
+    sfc[i1] = z0;
+  }
+  const R us = roughness_velocity(c, u_low, agl, z0);
+  sfc[i1 + dl] = us;
+  const R th_l = c.g.theta_v_bg[il] + R(s.theta_v_pert[il]);
+  const R th_2 = c.g.theta_v_bg[il - dl] + R(s.theta_v_pert[il - dl]);
+  const R dz = c.g.z_scalar[il - dl] - c.g.z_scalar[il];
+  const R dth = (th_2 - th_l) / dz;
+  const R w_pert = u10 * fmax2(R(0.001), R(0.02) * (R(1.0) - dth / R(0.01)));
+  const R t_pert = -(R(0.2) * R(c.C.dtime) * w_pert * dth);
+  R wt = R(0.2) * w_pert * t_pert;
+  if (fabs(wt) < R(EPSILON_SECURITY)) wt = R(EPSILON_SECURITY);
+  sfc[i1 + 2 * dl] = -(th_l * (us * (us * us)) / (R(c.C.karman) * R(GRAVITY) * wt));
+}
+
+// pbl_wind_tendency, mo_pbl.F90:111-250: mom_diff_tend_x/_y of the lowest layer minus the
+// friction, at the u and v points of each bottom cell; column nx+1 and row ny+1 are copies
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const StateDev s) {
+  KIDX;
+  const Geom &G = c.G;
+  const int nx = G.nx, ny = G.ny, nl = G.nl;
+  if (jk > nx || ji > ny || jl > 1) return;
+  const ix_t dl = (ix_t)G.lvl;
+  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
+  const ix_t i = LG_IX(G, ji, jk, nl), iw = LG_IX(G, ji, km, nl), is = LG_IX(G, jm, jk, nl);
+  const ix_t i1 = LG_IX(G, ji, jk, 1), iw1 = LG_IX(G, ji, km, 1), is1 = LG_IX(G, jm, jk, 1);
+  const R mx = c.w.mom_x[i] - pbl_friction(c, R(s.wind_u[i]), c.g.z_u[i], iw, i, iw1, i1, dl);
+  const R my = c.w.mom_y[i] - pbl_friction(c, R(s.wind_v[i]), c.g.z_v[i], is, i, is1, i1, dl);
+  c.w.mom_x[i] = mx;
+  if (jk == 1) AT(c.w.mom_x, ji, nx + 1, nl) = mx;
+  c.w.mom_y[i] = my;
+  if (ji == 1) AT(c.w.mom_y, ny + 1, jk, nl) = my;
+}
+
 #define LG_INSTANTIATE(R)                                                                  \
   template __global__ void k_diff_zeta<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_coef<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_dual<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_mom<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
-  template __global__ void k_diff_div<R>(const CtxT<R>);
+  template __global__ void k_diff_div<R>(const CtxT<R>);                                   \
+  template __global__ void k_sfc_turb<R>(const CtxT<R>, const StateDev);                   \
+  template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE

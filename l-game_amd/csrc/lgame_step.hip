@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
     }
     if (jl < nl) mz1 = (sp_c - sp(jl + 1)) / c.g.dz[ib] / (R(0.5) * (c.w.rsum[i] + c.w.rsum[ib]));
   }
-  if (!c.C.lklemp && !c.C.lmom_h) return;  // no heating rate without Klemp damping or momentum diffusion
+  if (!c.C.lklemp && !c.C.lmom_h && !c.C.lpbl) return;  // mo_vector_tend_expl.F90:123-125
   R wk0 = mz0, wk1 = mz1;
   const R dt = R(c.C.dtime);
   if (c.C.lklemp && jl >= 2 && jl <= c.C.n_damping_levels) {
@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
     wk1 = c.C.lmom_v ? wk1 + k : k;
   }
   R ip;
-  if (c.C.lmom_h) {
+  if (c.C.lmom_h || c.C.lpbl) {
     ip = AT(c.g.ipw[0], ji, jk, jl) * ATR(s.wind_u, ji, jk + 1, jl) * AT(c.w.mom_x, ji, jk + 1, jl) +
          c.g.ipw[1][i] * R(s.wind_v[i]) * c.w.mom_y[i] + c.g.ipw[2][i] * R(s.wind_u[i]) * c.w.mom_x[i] +
          AT(c.g.ipw[3], ji, jk, jl) * ATR(s.wind_v, ji + 1, jk, jl) * AT(c.w.mom_y, ji + 1, jk, jl) +
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
   const R cur_hor = R(0.5) + R(IMPL_THERMO_WEIGHT), old_hor = R(1.0) - cur_hor, cur_ver = R(1.0) - R(IMPL_THERMO_WEIGHT);
   const ix_t i = LG_IX(c.G, ji, jk, jl);
-  const bool mom = c.C.lmom_h || c.C.lmom_v;  // diag%mom_diff_tend_* (zero arrays otherwise)
+  const bool mom = c.C.lmom_h || c.C.lmom_v || c.C.lpbl;  // diag%mom_diff_tend_* (zero arrays otherwise)
   if (ji <= ny) {
     R x = old_hor * c.w.pold_u[i] - cur_hor * (c.w.pnl_u[i] + c.w.pl_u[i]) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i];
     if (mom) x = x + c.w.mom_x[i];

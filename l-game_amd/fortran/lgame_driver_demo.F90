@@ -44,7 +44,7 @@ program lgame_driver_demo
   read(u) rc_
   ny = hdr(1); nx = hdr(2); nl = hdr(3); nc = hdr(5)
   dims = lgame_dims(hdr(1),hdr(2),hdr(3),hdr(4),hdr(5),hdr(6),hdr(7))
-  cfg = lgame_cfg(rc_(1),rc_(2),rc_(3),rc_(4),rc_(5),rc_(6),60._c_double*rc_(4)/1000._c_double,0.2_c_double,1.5e-5_c_double, &
+  cfg = lgame_cfg(rc_(1),rc_(2),rc_(3),rc_(4),rc_(5),rc_(6),60._c_double*rc_(4)/1000._c_double,0.2_c_double,1.5e-5_c_double,100._c_double,0.4_c_double, &
                   hdr(8),1,hdr(13),hdr(9),hdr(10),hdr(11),0,0,0,0,0,0,0,0,0,0,1,hdr(12),0,LGAME_PREC_FP64,LGAME_RAD_OFF,0)
 
   ! grid (extents as in control.F90:101-157)
@@ -68,7 +68,7 @@ program lgame_driver_demo
                        c_loc(ipw),c_loc(area_dual_x),c_loc(area_dual_y),c_loc(area_dual_z),c_loc(z_area_dual_z), &
                        c_loc(fvec_x),c_loc(fvec_y),c_loc(fvec_z),c_loc(trsk_u),c_loc(trsk_v),c_loc(ebg_u), &
                        c_loc(ebg_v),c_loc(ebg_w),c_loc(theta_v_bg),c_loc(exner_bg),c_loc(gravity_m_v), &
-                       c_loc(gravity_potential),c_null_ptr,c_null_ptr,c_null_ptr,0._c_double)
+                       c_loc(gravity_potential),c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,0._c_double)
   rc = lgame_init(dims,cfg,gv,h)
   call lgame_check(rc,c_null_ptr,"lgame_init")
 

@@ -23,7 +23,7 @@ module mo_lgame_gpu
   end type lgame_dims
 
   type, bind(C) :: lgame_cfg
-    real(c_double) :: dtime, toa, dx, dy, klemp_damp_max, klemp_begin_rel, dtime_rad, c_s, bg_shear
+    real(c_double) :: dtime, toa, dx, dy, klemp_damp_max, klemp_begin_rel, dtime_rad, c_s, bg_shear, h_prandtl, karman
     integer(c_int) :: theta_adv_order, luse_bg_state, llinear, lcorio, lmoist, lklemp
     integer(c_int) :: lmom_diff_h, lmom_diff_v, ltemp_diff_h, ltemp_diff_v, lmass_diff_h, lmass_diff_v
     integer(c_int) :: lprog_soil_temp, lsfc_sensible_heat_flux, lsfc_phase_trans, lpbl
@@ -35,7 +35,7 @@ module mo_lgame_gpu
     type(c_ptr) :: inner_product_weights, area_dual_x, area_dual_y, area_dual_z, z_area_dual_z
     type(c_ptr) :: fvec_x, fvec_y, fvec_z, trsk_weights_u, trsk_weights_v
     type(c_ptr) :: exner_bg_grad_u, exner_bg_grad_v, exner_bg_grad_w, theta_v_bg, exner_bg
-    type(c_ptr) :: gravity_m_v, gravity_potential, dx_dual, dy_dual, layer_thickness
+    type(c_ptr) :: gravity_m_v, gravity_potential, dx_dual, dy_dual, layer_thickness, land_fraction, roughness_length
     real(c_double) :: mean_velocity_area
   end type lgame_grid_view
 

@@ -55,9 +55,31 @@ def moist3d_config(n=513, n_layers=60, dtime=3.22, seed=3, **kw) -> RunConfig:
     return dataclasses.replace(cfg, lmoist=True)
 
 
+def land_mask(cfg: RunConfig, seed: int = 7):
+    """Seeded land/sea mask (set-up extension): land where a smoothed Gaussian field exceeds 0.3."""
+    from .init_state import _smooth_periodic
+
+    rng = np.random.default_rng(seed)
+    return _smooth_periodic(rng.standard_normal((cfg.ny, cfg.nx)), max(2.0, 20e3 / cfg.dx)) > 0.3
+
+
+def land_fraction(cfg: RunConfig, seed: int = 11):
+    """Seeded fractional land cover (set-up extension): a smoothed Gaussian field rescaled to
+    [0, 1], then 0 below 0.5 (sea), rising linearly to 1 at 0.9 (land), fractional between."""
+    from .init_state import _smooth_periodic
+
+    rng = np.random.default_rng(seed)
+    f = _smooth_periodic(rng.standard_normal((cfg.ny, cfg.nx)), max(1.0, 10e3 / cfg.dx))
+    f = (f - f.min()) / max(float(np.ptp(f)), 1e-300)
+    return np.asfortranarray(np.clip(2.5 * (f - 0.5), 0.0, 1.0))
+
+
 def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True, cloud_frac=0.0):
-    """Grid and initial state for ``cfg`` (perturbations only for the standard scenario)."""
+    """Grid and initial state for ``cfg`` (perturbations only for the standard scenario).  With
+    the PBL on, the grid's land fraction is the seeded field of land_fraction."""
     g = grid_setup(cfg)
+    if cfg.lpbl:
+        g["land_fraction"] = land_fraction(cfg)
     st = ideal_init(cfg, g)
     if cfg.scenario == "standard" and perturbation:
         perturb(st, g, cfg, seed=(seed if seed is not None else 2))
@@ -80,10 +102,7 @@ def radiation_surface(cfg: RunConfig, g: dict, st: dict, seed: int = 7) -> dict:
     of the grid, a land/sea mask from seeded random blobs (land where a smoothed Gaussian field
     exceeds 0.3), albedo 0.2 on land and 0.06 on sea, and a surface temperature 1 K (sea) or 3 K
     (land) above the temperature of the lowest layer of the initial state."""
-    from .init_state import _smooth_periodic
-
-    rng = np.random.default_rng(seed)
-    land = _smooth_periodic(rng.standard_normal((cfg.ny, cfg.nx)), max(2.0, 20e3 / cfg.dx)) > 0.3
+    land = land_mask(cfg, seed)
     nl = cfg.n_layers
     t_low = (g["theta_v_bg"][:, :, nl - 1] + st["theta_v_pert"][:, :, nl - 1]) * (g["exner_bg"][:, :, nl - 1] +
                                                                               st["exner_pert"][:, :, nl - 1])

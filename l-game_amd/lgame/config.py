@@ -29,7 +29,8 @@ class LgameDims(ctypes.Structure):
 
 class LgameCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
-        "dtime", "toa", "dx", "dy", "klemp_damp_max", "klemp_begin_rel", "dtime_rad", "c_s", "bg_shear")] + \
+        "dtime", "toa", "dx", "dy", "klemp_damp_max", "klemp_begin_rel", "dtime_rad", "c_s", "bg_shear",
+        "h_prandtl", "karman")] + \
         [(n, ctypes.c_int) for n in (
             "theta_adv_order", "luse_bg_state", "llinear", "lcorio", "lmoist", "lklemp",
             "lmom_diff_h", "lmom_diff_v", "ltemp_diff_h", "ltemp_diff_v", "lmass_diff_h",
@@ -69,6 +70,8 @@ class RunConfig:
     diff_coeff_scheme_h: str = "smag"
     c_s: float = 0.2
     bg_shear: float = 1.5e-5
+    h_prandtl: float = 100.0
+    karman: float = 0.4
     lklemp: bool = True
     klemp_damp_max: float = 0.25
     klemp_begin_rel: float = 0.53
@@ -76,6 +79,7 @@ class RunConfig:
     orography_id: int = 0
     lsleve: bool = False
     nsoillays: int = 5
+    lpbl: bool = False
     # &bc
     lperiodic: bool = True
     lfreeslip: bool = False
@@ -145,6 +149,9 @@ class RunConfig:
             setattr(c, k, int(getattr(self, k)))
         c.c_s = self.c_s
         c.bg_shear = self.bg_shear
+        c.h_prandtl = self.h_prandtl
+        c.karman = self.karman
+        c.lpbl = int(self.lpbl)
         if self.diff_coeff_scheme_h not in ("smag", "tke"):
             raise ValueError("diff_coeff_scheme_h must be either smag or tke")
         c.diff_coeff_scheme_h = 0 if self.diff_coeff_scheme_h == "smag" else 1
@@ -191,6 +198,8 @@ GRID_FIELDS = {
     "dx_dual": lambda ny, nx, nl: (ny + 1, nx, nl),
     "dy_dual": lambda ny, nx, nl: (ny, nx + 1, nl),
     "layer_thickness": lambda ny, nx, nl: (ny, nx, nl),
+    "land_fraction": lambda ny, nx, nl: (ny, nx),
+    "roughness_length": lambda ny, nx, nl: (ny, nx),
 }
 # scalar grid quantities that cross the boundary with the fields
 GRID_SCALARS = ("mean_velocity_area",)

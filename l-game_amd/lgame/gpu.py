@@ -52,6 +52,8 @@ DIAGS = {
     "mom_diff_tend_z": (4, lambda ny, nx, nl: (ny, nx, nl + 1)),
     "temp_diff_heating": (5, lambda ny, nx, nl: (ny, nx, nl)),
     "heating_diss": (6, lambda ny, nx, nl: (ny, nx, nl)),
+    "roughness_length": (7, lambda ny, nx, nl: (ny, nx)),
+    "monin_obukhov_length": (8, lambda ny, nx, nl: (ny, nx)),
 }
 
 

@@ -295,6 +295,9 @@ def grid_setup(cfg: RunConfig) -> dict:
         a_sfc = _patch_area(lat_scalar, dlon, dlat, False)[:, None] * (R_E + z_w[:, :, nl]) ** 2 / R_E ** 2
         area_z = np.asfortranarray(a_sfc[:, :, None] * (R_E + z_w) ** 2 / (R_E + z_w[:, :, nl:nl + 1]) ** 2)
     g["area_z"] = area_z
+    # surface: no land unless a case sets a mask (:326, :764-767); roughness length 0.08 m (:774)
+    g["land_fraction"] = np.zeros((ny, nx), order="F")
+    g["roughness_length"] = np.full((ny, nx), 0.08, order="F")
     # area used by the turbulence parameterisations, :1111
     g["mean_velocity_area"] = 2.0 * float(np.sum(area_z[:, :, nl])) / (ny * nx)
 

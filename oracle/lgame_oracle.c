@@ -64,13 +64,13 @@ typedef struct oracle {
   int order, use_bg, llinear, lcorio, lmoist, lklemp, lfreeslip;
   int n_damping_levels;
   /* diffusion switches and parameters (mo_diff_nml.F90) */
-  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_h_tke;
-  double c_s, bg_shear, eff_hor_res, mva;
+  int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_h_tke, lpbl;
+  double c_s, bg_shear, eff_hor_res, mva, h_prandtl, karman;
   /* grid */
   F3 z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume, slope_x, slope_y;
   F3 area_dual_x, area_dual_y, area_dual_z, z_area_dual_z, fvec_x, fvec_y, fvec_z;
   F3 ebg_u, ebg_v, ebg_w, theta_v_bg, exner_bg, gravity_m_v, gravity_potential;
-  F3 dx_dual, dy_dual, layer_thickness, mva_f;
+  F3 dx_dual, dy_dual, layer_thickness, mva_f, land_fraction, roughness_length;
   double *ipw;    /* (6,ny,nx,nl) */
   double *trsk_u; /* (3,ny) */
   double *trsk_v; /* (2,ny+1) */
@@ -87,6 +87,7 @@ typedef struct oracle {
   F3 phase_trans_heating_rate, temp_diff_heating, condensates_sediment_heat, a_rain;
   F3 tke, n_squared, viscosity, viscosity_molecular, viscosity_dual, flux_w;
   F3 mass_coeff_h, mass_coeff_v, temp_coeff_h, temp_coeff_v;
+  F3 z0, ustar, mo_len; /* diag%roughness_length, roughness_velocity, monin_obukhov_length (ny,nx) */
   double *phase_trans_rates;  /* (ny,nx,nl,ncond+1) */
   double *mass_diff_tendency; /* (ny,nx,nl,nc)     */
   /* radiation: per-column surface data (ny,nx), outputs and switches */
@@ -1246,6 +1247,116 @@ static void scalar_diffusion(oracle *o, F3 field, F3 coeff_h, F3 coeff_v, int lv
   }
 }
 
+
+/* ------------------------------------------------------------------------------------------ */
+/* subgrid_scale/mo_pbl.F90: surface turbulence quantities and the PBL wind tendency          */
+/* ------------------------------------------------------------------------------------------ */
+#define EPSILON_SECURITY 1e-10
+#define GRAVITY 9.80616
+
+static double fmax2(double a, double b) { return a > b ? a : b; } /* Fortran max(a,b) */
+static double fmin2(double a, double b) { return a < b ? a : b; } /* Fortran min(a,b) */
+
+/* roughness_length_from_u10_sea, mo_pbl.F90:252-272 */
+static double roughness_length_from_u10_sea(double u10) {
+  double swh = 0.0248 * (u10 * u10);
+  double period = 0.729 * u10;
+  double wavelength = GRAVITY * (period * period) / (2.0 * MPI);
+  double r = 1200.0 * swh * pow(swh / fmax2(wavelength, EPSILON_SECURITY), 4.5);
+  return fmax2(0.0001, r);
+}
+
+/* roughness_velocity, mo_pbl.F90:344-366 */
+static double roughness_velocity(const oracle *o, double wind_speed, double z_agl, double z0) {
+  double den = log(z_agl / z0);
+  if (fabs(den) < EPSILON_SECURITY) den = EPSILON_SECURITY;
+  double r = wind_speed * o->karman / den;
+  return fmax2(EPSILON_SECURITY, r);
+}
+
+/* psi_m, mo_pbl.F90:392-420 */
+static double psi_m(double eff, double l) {
+  double ll = l;
+  if (fabs(ll) < EPSILON_SECURITY) ll = EPSILON_SECURITY;
+  if (ll < 0.0) {
+    double x = pow(1.0 - 15.0 * eff / ll, 0.25);
+    return 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + MPI / 2.0;
+  }
+  return -(4.7 * eff / ll);
+}
+
+/* momentum_flux_resistance, mo_pbl.F90:318-342 */
+static double momentum_flux_resistance(const oracle *o, double wind_h, double z_agl, double z0, double l) {
+  double used = fmin2(z_agl, o->h_prandtl);
+  double r = 1.0 / (o->karman * roughness_velocity(o, wind_h, z_agl, z0)) * (log(used / z0) - psi_m(used, l));
+  if (r < o->dtime / z_agl) r = o->dtime / z_agl;
+  return r;
+}
+
+/* update_sfc_turb_quantities, mo_pbl.F90:19-109 (no prognostic soil: no scalar flux resistance).
+   v_squared is the kinetic energy the last momentum advection block left. */
+static void update_sfc_turb_quantities(oracle *o, const ostate *s) {
+  int nl = o->nl, nlev = o->nlev;
+  for (int jk = 1; jk <= o->nx; ++jk)
+    for (int ji = 1; ji <= o->ny; ++ji) {
+      double agl = A3(o->z_scalar, ji, jk, nl) - A3(o->z_w, ji, jk, nlev);
+      double u_low = sqrt(A3(o->v_squared, ji, jk, nl));
+      double z0 = A3(o->z0, ji, jk, 1);
+      double u10 = u_low * log(10.0 / z0) / log(agl / z0);
+      double lf = A3(o->land_fraction, ji, jk, 1);
+      if (lf < 1.0) {
+        z0 = lf * A3(o->roughness_length, ji, jk, 1) + (1.0 - lf) * roughness_length_from_u10_sea(u10);
+        A3(o->z0, ji, jk, 1) = z0;
+      }
+      double us = roughness_velocity(o, u_low, agl, z0);
+      A3(o->ustar, ji, jk, 1) = us;
+      double th_l = A3(o->theta_v_bg, ji, jk, nl) + A3(s->theta_v_pert, ji, jk, nl);
+      double th_2 = A3(o->theta_v_bg, ji, jk, nl - 1) + A3(s->theta_v_pert, ji, jk, nl - 1);
+      double dz = A3(o->z_scalar, ji, jk, nl - 1) - A3(o->z_scalar, ji, jk, nl);
+      double dth = (th_2 - th_l) / dz;
+      double w_pert = u10 * fmax2(0.001, 0.02 * (1.0 - dth / 0.01));
+      double t_pert = -(0.2 * o->dtime * w_pert * dth);
+      double wt = 0.2 * w_pert * t_pert;
+      if (fabs(wt) < EPSILON_SECURITY) wt = EPSILON_SECURITY;
+      A3(o->mo_len, ji, jk, 1) = -(th_l * (us * (us * us)) / (o->karman * GRAVITY * wt));
+    }
+}
+
+/* the PBL friction at one lowest-layer u or v point between cells a and b (mo_pbl.F90:125-160) */
+static double pbl_friction(const oracle *o, double wind, double z_pt, int ia, int ka, int ib, int kb) {
+  int nl = o->nl, nlev = o->nlev;
+  double ws = 0.5 * (sqrt(A3(o->v_squared, ia, ka, nl)) + sqrt(A3(o->v_squared, ib, kb, nl)));
+  double z_agl = z_pt - 0.5 * (A3(o->z_w, ia, ka, nlev) + A3(o->z_w, ib, kb, nlev));
+  double lt = 0.5 * (A3(o->z_w, ia, ka, nl) + A3(o->z_w, ib, kb, nl)) - 0.5 * (A3(o->z_w, ia, ka, nlev) + A3(o->z_w, ib, kb, nlev));
+  double z0 = 0.5 * (A3(o->z0, ia, ka, 1) + A3(o->z0, ib, kb, 1));
+  double l = 0.5 * (A3(o->mo_len, ia, ka, 1) + A3(o->mo_len, ib, kb, 1));
+  double fr = momentum_flux_resistance(o, ws, z_agl, z0, l);
+  double f = 1.0;
+  if (z_agl > o->h_prandtl) f = log(o->h_prandtl / z0) / log(z_agl / z0);
+  return f * wind / fr / lt;
+}
+
+/* pbl_wind_tendency, mo_pbl.F90:111-250 (periodic) */
+static void pbl_wind_tendency(oracle *o, const ostate *s) {
+  int ny = o->ny, nx = o->nx, nl = o->nl;
+  for (int ji = 1; ji <= ny; ++ji) {
+    for (int jk = 1; jk <= nx; ++jk) {
+      int km = jk == 1 ? nx : jk - 1;
+      A3(o->mom_x, ji, jk, nl) = A3(o->mom_x, ji, jk, nl) -
+                                 pbl_friction(o, A3(s->wind_u, ji, jk, nl), A3(o->z_u, ji, jk, nl), ji, km, ji, jk);
+    }
+    A3(o->mom_x, ji, nx + 1, nl) = A3(o->mom_x, ji, 1, nl);
+  }
+  for (int jk = 1; jk <= nx; ++jk) {
+    for (int ji = 1; ji <= ny; ++ji) {
+      int jm = ji == 1 ? ny : ji - 1;
+      A3(o->mom_y, ji, jk, nl) = A3(o->mom_y, ji, jk, nl) -
+                                 pbl_friction(o, A3(s->wind_v, ji, jk, nl), A3(o->z_v, ji, jk, nl), jm, jk, ji, jk);
+    }
+    A3(o->mom_y, ny + 1, jk, nl) = A3(o->mom_y, 1, jk, nl);
+  }
+}
+
 /* vector_tend_expl, mo_vector_tend_expl.F90:30-182 */
 static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) {
   int ny = o->ny, nx = o->nx, nl = o->nl, nlev = o->nlev;
@@ -1279,7 +1390,8 @@ static void vector_tend_expl(oracle *o, const ostate *s, int rk_step, int step) 
     }
     if (o->lmom_h) mom_diff_h(o, s);
     if (o->lmom_v) mom_diff_v(o, s);
-    if (o->lmom_h || o->lklemp) simple_dissipation_rate(o, s);
+    if (o->lpbl) pbl_wind_tendency(o, s);
+    if (o->lmom_h || o->lpbl || o->lklemp) simple_dissipation_rate(o, s);
   }
   double new_weight = (rk_step == 2) ? 0.5 : 1.0;
   double old_weight = 1.0 - new_weight;
@@ -1761,6 +1873,7 @@ static void update_rad_fluxes(oracle *o, const ostate *s, double t) {
 static void pchevi(oracle *o, ostate *old, ostate *nw, int step, int lrad_update, double t_0) {
   size_t nu = (size_t)o->ny * (o->nx + 1) * o->nl, nv = (size_t)(o->ny + 1) * o->nx * o->nl;
   temperature_diagnostics(o, old);
+  if (o->lpbl) update_sfc_turb_quantities(o, old);
   if (lrad_update && o->lrad) update_rad_fluxes(o, old, t_0);
   for (int rk = 1; rk <= 2; ++rk) {
     if (rk == 1) manage_p_grad(o, old, step == 0);
@@ -1800,7 +1913,7 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
     return NULL;
   }
   if (!cf->lperiodic || !cf->luse_bg_state || ((cf->ltemp_diff_h || cf->lmass_diff_h) && !cf->lmom_diff_h) ||
-      (cf->diff_coeff_scheme_h != 0 && cf->diff_coeff_scheme_h != 1) || cf->lpbl || cf->lsfc_sensible_heat_flux ||
+      (cf->diff_coeff_scheme_h != 0 && cf->diff_coeff_scheme_h != 1) || cf->lsfc_sensible_heat_flux ||
       cf->lsfc_phase_trans || cf->lprog_soil_temp || (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
     snprintf(g_err, sizeof g_err, "unsupported configuration");
     return NULL;
@@ -1842,6 +1955,9 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   o->c_s = cf->c_s;
   o->bg_shear = cf->bg_shear;
   o->eff_hor_res = sqrt(cf->dx * cf->dy); /* mo_run_nml.F90:93 */
+  o->lpbl = cf->lpbl != 0;
+  o->h_prandtl = cf->h_prandtl;
+  o->karman = cf->karman;
 #define G3(f, a, b, c) o->f = f3alloc_o(o, a, b, c)
   G3(z_scalar, ny, nx, nl);
   G3(z_u, ny, nx + 1, nl);
@@ -1874,6 +1990,11 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   G3(dy_dual, ny, nx + 1, nl);
   G3(layer_thickness, ny, nx, nl);
   G3(mva_f, 1, 1, 1);
+  G3(land_fraction, ny, nx, 1);
+  G3(roughness_length, ny, nx, 1);
+  G3(z0, ny, nx, 1);
+  G3(ustar, ny, nx, 1);
+  G3(mo_len, ny, nx, 1);
   G3(tke, ny, nx, nl);
   G3(n_squared, ny, nx, nl);
   G3(viscosity, ny, nx, nl);
@@ -1957,7 +2078,7 @@ static F3 *grid_field(oracle *o, const char *name) {
   if (!strcmp(name, #f)) return &o->f;
   GF(z_scalar) GF(z_u) GF(z_v) GF(z_w) GF(dx) GF(dy) GF(dz) GF(area_x) GF(area_y) GF(area_z) GF(volume) GF(slope_x) GF(slope_y)
   GF(area_dual_x) GF(area_dual_y) GF(area_dual_z) GF(z_area_dual_z) GF(fvec_x) GF(fvec_y) GF(fvec_z) GF(theta_v_bg) GF(exner_bg)
-  GF(gravity_m_v) GF(gravity_potential) GF(dx_dual) GF(dy_dual) GF(layer_thickness)
+  GF(gravity_m_v) GF(gravity_potential) GF(dx_dual) GF(dy_dual) GF(layer_thickness) GF(land_fraction) GF(roughness_length)
   if (!strcmp(name, "mean_velocity_area")) return &o->mva_f;
   if (!strcmp(name, "exner_bg_grad_u")) return &o->ebg_u;
   if (!strcmp(name, "exner_bg_grad_v")) return &o->ebg_v;
@@ -1988,6 +2109,8 @@ int orc_set_grid(oracle *o, const char *name, const double *src) {
 /* Must be called after z_w is set: n_damping_levels as in mo_grid_generator.F90:1050-1058. */
 void orc_finish_grid(oracle *o) {
   o->mva = o->mva_f.p[0];
+  /* diag%roughness_length = grid%roughness_length (control.F90:456-459) */
+  memcpy(o->z0.p, o->roughness_length.p, sizeof(double) * (size_t)o->ny * o->nx);
   o->n_damping_levels = 0;
   for (int jl = 1; jl <= o->nlev; ++jl) {
     double mx = -1e300;
@@ -2049,6 +2172,9 @@ int orc_get_diag(oracle *o, const char *name, double *dst) {
   if (!strcmp(name, "mom_diff_tend_y")) f = &o->mom_y;
   if (!strcmp(name, "mom_diff_tend_z")) f = &o->mom_z;
   if (!strcmp(name, "temp_diff_heating")) f = &o->temp_diff_heating;
+  if (!strcmp(name, "roughness_length")) f = &o->z0;
+  if (!strcmp(name, "roughness_velocity")) f = &o->ustar;
+  if (!strcmp(name, "monin_obukhov_length")) f = &o->mo_len;
   if (!strcmp(name, "mass_diff_tendency")) {
     memcpy(dst, o->mass_diff_tendency, sizeof(double) * (size_t)o->ny * o->nx * o->nl * o->nc);
     return 0;

@@ -22,13 +22,14 @@ REF_GRID_IDS = {name: i + 1 for i, name in enumerate([
     "slope_x", "slope_y", "inner_product_weights", "area_dual_x", "area_dual_y", "area_dual_z",
     "z_area_dual_z", "fvec_x", "fvec_y", "fvec_z", "trsk_weights_u", "trsk_weights_v",
     "exner_bg_grad_u", "exner_bg_grad_v", "exner_bg_grad_w", "theta_v_bg", "exner_bg",
-    "gravity_m_v", "gravity_potential", "dx_dual", "dy_dual", "layer_thickness"])}
+    "gravity_m_v", "gravity_potential", "dx_dual", "dy_dual", "layer_thickness", "land_fraction",
+    "roughness_length"])}
 REF_STATE_IDS = {"rho": 1, "rhotheta_v": 2, "theta_v_pert": 3, "exner_pert": 4, "wind_u": 5,
                  "wind_v": 6, "wind_w": 7}
 REF_DIAG_IDS = {"temperature": 1, "heating_diss": 2, "pot_vort_tend_x": 3, "pot_vort_tend_z": 4,
                 "condensates_sediment_heat": 5, "tke": 6, "mom_diff_tend_x": 7, "mom_diff_tend_y": 8,
                 "mom_diff_tend_z": 9, "temp_diff_heating": 10, "mass_diff_tendency": 11, "viscosity": 12,
-                "n_squared": 13}
+                "n_squared": 13, "roughness_length": 14, "roughness_velocity": 15, "monin_obukhov_length": 16}
 
 
 def _ptr(a):
@@ -205,10 +206,10 @@ class RefHarness:
         icfg = np.array([cfg.theta_adv_order, int(cfg.luse_bg_state), int(cfg.llinear), int(cfg.lcorio),
                          int(cfg.lmoist), int(cfg.lklemp), int(cfg.lperiodic), int(cfg.lfreeslip),
                          int(cfg.lmom_diff_h), int(cfg.lmom_diff_v), int(cfg.ltemp_diff_h), int(cfg.ltemp_diff_v),
-                         int(cfg.lmass_diff_h), int(cfg.lmass_diff_v), 0 if cfg.diff_coeff_scheme_h == "smag" else 1],
-                        dtype=np.int32)
+                         int(cfg.lmass_diff_h), int(cfg.lmass_diff_v), 0 if cfg.diff_coeff_scheme_h == "smag" else 1,
+                         int(cfg.lpbl)], dtype=np.int32)
         rcfg = np.array([cfg.dt, cfg.toa, cfg.dx, cfg.dy, cfg.klemp_damp_max, cfg.klemp_begin_rel, cfg.c_s,
-                         cfg.bg_shear, grid["mean_velocity_area"]])
+                         cfg.bg_shear, grid["mean_velocity_area"], cfg.h_prandtl, cfg.karman])
         lib.ref_init(_ptr(idims), _ptr(icfg), _ptr(rcfg))
         for name in GRID_FIELDS:
             a = _fortran(grid[name])

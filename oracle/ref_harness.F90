@@ -31,7 +31,7 @@ module ref_harness
   use mo_constituents_nml,   only: lmoist,n_condensed_constituents,n_gaseous_constituents,n_constituents
   use mo_diff_nml,           only: lmom_diff_h,lmom_diff_v,ltemp_diff_h,ltemp_diff_v,lmass_diff_h,lmass_diff_v, &
                                    lklemp,klemp_damp_max,klemp_begin_rel,diff_coeff_scheme_h,diff_coeff_scheme_v, &
-                                   c_s,bg_shear
+                                   c_s,bg_shear,h_prandtl,karman
   use mo_surface_nml,        only: nsoillays,lprog_soil_temp,lsfc_sensible_heat_flux,lsfc_phase_trans,lpbl
   use mo_bc_nml,             only: lperiodic,lfreeslip
   use mo_derived,            only: temperature_diagnostics
@@ -44,6 +44,7 @@ module ref_harness
   use mo_vorticities,        only: calc_pot_vort,rel_vort
   use mo_eff_diff_coeffs,    only: hor_viscosity,vert_vert_mom_viscosity,update_n_squared
   use mo_tke,                only: tke_update
+  use mo_pbl,                only: update_sfc_turb_quantities,pbl_wind_tendency
   use mo_multiplications,    only: scalar_times_vector_h,scalar_times_vector_v
   use mo_vorticity_flux,     only: calc_vorticity_flux_term
 
@@ -72,11 +73,11 @@ contains
   ! idims: ny,nx,n_layers,n_oro_layers,n_constituents,n_condensed_constituents,nsoillays
   ! icfg:  theta_adv_order,luse_bg_state,llinear,lcorio,lmoist,lklemp,lperiodic,lfreeslip,
   !        lmom_diff_h,lmom_diff_v,ltemp_diff_h,ltemp_diff_v,lmass_diff_h,lmass_diff_v,
-  !        diff_coeff_scheme_h (0 smag, 1 tke)
-  ! rcfg:  dtime,toa,dx,dy,klemp_damp_max,klemp_begin_rel,c_s,bg_shear,mean_velocity_area
+  !        diff_coeff_scheme_h (0 smag, 1 tke),lpbl
+  ! rcfg:  dtime,toa,dx,dy,klemp_damp_max,klemp_begin_rel,c_s,bg_shear,mean_velocity_area,h_prandtl,karman
   subroutine ref_init(idims,icfg,rcfg) bind(C,name="ref_init")
-    integer(c_int), intent(in) :: idims(7),icfg(15)
-    real(c_double), intent(in) :: rcfg(9)
+    integer(c_int), intent(in) :: idims(7),icfg(16)
+    real(c_double), intent(in) :: rcfg(11)
 
     ny = idims(1); nx = idims(2); n_layers = idims(3); n_oro_layers = idims(4)
     n_constituents = idims(5); n_condensed_constituents = idims(6); nsoillays = idims(7)
@@ -99,9 +100,9 @@ contains
     diff_coeff_scheme_h = "smag"
     if (icfg(15)==1) diff_coeff_scheme_h = "tke"
     diff_coeff_scheme_v = "tke"
-    c_s = rcfg(7); bg_shear = rcfg(8)
+    c_s = rcfg(7); bg_shear = rcfg(8); h_prandtl = rcfg(10); karman = rcfg(11)
     eff_hor_res = sqrt(dx*dy)
-    lprog_soil_temp = .false.; lsfc_sensible_heat_flux = .false.; lsfc_phase_trans = .false.; lpbl = .false.
+    lprog_soil_temp = .false.; lsfc_sensible_heat_flux = .false.; lsfc_phase_trans = .false.; lpbl = icfg(16)/=0
 
     ! grid (sizes as in control.F90:101-157)
     allocate(grid%z_scalar(ny,nx,n_layers)); allocate(grid%z_u(ny,nx+1,n_layers))
@@ -122,6 +123,7 @@ contains
     allocate(grid%layer_thickness(ny,nx,n_layers))
     grid%mean_velocity_area = rcfg(9)
     allocate(grid%land_fraction(ny,nx)); grid%land_fraction = 0._wp
+    allocate(grid%roughness_length(ny,nx)); grid%roughness_length = 0._wp
     allocate(grid%lake_fraction(ny,nx)); grid%lake_fraction = 0._wp
 
     call alloc_state(st1)
@@ -192,6 +194,10 @@ contains
     allocate(diag%mass_diff_coeff_eff_v(ny,nx,n_layers)); diag%mass_diff_coeff_eff_v = 0._wp
     allocate(diag%temp_diff_coeff_eff_h(ny,nx,n_layers)); diag%temp_diff_coeff_eff_h = 0._wp
     allocate(diag%temp_diff_coeff_eff_v(ny,nx,n_layers)); diag%temp_diff_coeff_eff_v = 0._wp
+    allocate(diag%roughness_length(ny,nx)); diag%roughness_length = 0._wp
+    allocate(diag%roughness_velocity(ny,nx)); diag%roughness_velocity = 0._wp
+    allocate(diag%monin_obukhov_length(ny,nx)); diag%monin_obukhov_length = 0._wp
+    allocate(diag%scalar_flux_resistance(ny,nx)); diag%scalar_flux_resistance = 0._wp
   end subroutine ref_init
 
   ! grid field ids (Python side keeps the same table)
@@ -237,12 +243,17 @@ contains
       case(32); n = size(grid%dy_dual);       call c_f_pointer(src,p,[n]); grid%dy_dual = reshape(p,shape(grid%dy_dual))
       case(33); n = size(grid%layer_thickness); call c_f_pointer(src,p,[n])
                 grid%layer_thickness = reshape(p,shape(grid%layer_thickness))
+      case(34); n = size(grid%land_fraction); call c_f_pointer(src,p,[n]); grid%land_fraction = reshape(p,shape(grid%land_fraction))
+      case(35); n = size(grid%roughness_length); call c_f_pointer(src,p,[n])
+                grid%roughness_length = reshape(p,shape(grid%roughness_length))
     end select
   end subroutine ref_set_grid
 
   ! n_damping_levels, as mo_grid_generator.F90:1050-1058 derives it from z_w
   subroutine ref_finish_grid() bind(C,name="ref_finish_grid")
     integer :: jl
+    ! initializing the diagnostic roughness length (control.F90:456-459)
+    diag%roughness_length = grid%roughness_length
     n_damping_levels = 0
     do jl=1,n_levels
       if (maxval(grid%z_w(:,:,jl))>klemp_begin_rel*toa) then
@@ -295,7 +306,8 @@ contains
 
   ! diag field ids: 1 temperature, 2 heating_diss, 3 pot_vort_tend_x, 4 pot_vort_tend_z,
   ! 5 condensates_sediment_heat, 6 tke, 7 mom_diff_tend_x, 8 mom_diff_tend_y, 9 mom_diff_tend_z,
-  ! 10 temp_diff_heating, 11 mass_diff_tendency, 12 viscosity, 13 n_squared
+  ! 10 temp_diff_heating, 11 mass_diff_tendency, 12 viscosity, 13 n_squared, 14 roughness_length,
+  ! 15 roughness_velocity, 16 monin_obukhov_length
   subroutine ref_get_diag(id,buf) bind(C,name="ref_get_diag")
     integer(c_int), value, intent(in) :: id
     type(c_ptr),    value, intent(in) :: buf
@@ -317,6 +329,12 @@ contains
                 p = reshape(diag%mass_diff_tendency,[size(diag%mass_diff_tendency)])
       case(12); call c_f_pointer(buf,p,[size(diag%viscosity)]); p = reshape(diag%viscosity,[size(diag%viscosity)])
       case(13); call c_f_pointer(buf,p,[size(diag%n_squared)]); p = reshape(diag%n_squared,[size(diag%n_squared)])
+      case(14); call c_f_pointer(buf,p,[size(diag%roughness_length)])
+                p = reshape(diag%roughness_length,[size(diag%roughness_length)])
+      case(15); call c_f_pointer(buf,p,[size(diag%roughness_velocity)])
+                p = reshape(diag%roughness_velocity,[size(diag%roughness_velocity)])
+      case(16); call c_f_pointer(buf,p,[size(diag%monin_obukhov_length)])
+                p = reshape(diag%monin_obukhov_length,[size(diag%monin_obukhov_length)])
     end select
   end subroutine ref_get_diag
 
@@ -500,7 +518,7 @@ contains
     enddo
   end subroutine harness_mom_diff_v
 
-  ! vector_tend_expl restated (mo_vector_tend_expl.F90:30-182), PBL off
+  ! vector_tend_expl restated (mo_vector_tend_expl.F90:30-182)
   subroutine harness_vector_tend_expl(state,rk_step,total_step_counter)
     type(t_state), intent(in) :: state
     integer,       intent(in) :: rk_step,total_step_counter
@@ -551,7 +569,10 @@ contains
       if (lmom_diff_v) then
         call harness_mom_diff_v(state)
       endif
-      if (lmom_diff_h .or. lklemp) then
+      if (lpbl) then
+        call pbl_wind_tendency(state,diag,grid)
+      endif
+      if (lmom_diff_h .or. lpbl .or. lklemp) then
         call harness_dissipation(state)
       endif
     endif
@@ -577,12 +598,15 @@ contains
     - 0.5_wp*diag%v_squared_grad_z + diag%pot_vort_tend_z + diag%p_grad_condensates_w + diag%mom_diff_tend_z)
   end subroutine harness_vector_tend_expl
 
-  ! pchevi restated (mo_manage_pchevi.F90:27-118): no radiation, surface or boundary updates
+  ! pchevi restated (mo_manage_pchevi.F90:27-118): no radiation or boundary updates
   subroutine harness_pchevi(state_old,state_new,total_step_counter)
     type(t_state), intent(inout) :: state_old,state_new
     integer,       intent(in)    :: total_step_counter
     integer :: rk_step
     call temperature_diagnostics(state_old,diag,grid)
+    if (lsfc_sensible_heat_flux .or. lsfc_phase_trans .or. lpbl) then
+      call update_sfc_turb_quantities(state_old,diag,grid)
+    endif
     do rk_step=1,2
       if (rk_step==1) then
         call manage_p_grad(state_old,diag,grid,total_step_counter==0)

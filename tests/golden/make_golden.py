@@ -31,6 +31,9 @@ CASES = {
     "tiny_dry_o2_diff_tke": ({"kind": "tiny", "kw": {}, "diffusion": True,
                               "diff_kw": {"diff_coeff_scheme_h": "tke"}}, 6),
     "schaer_41x20_diff": ({"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "diffusion": True}, 12),
+    # with the PBL friction and the surface turbulence quantities as well
+    "tiny_moist_o3_freeslip_diff_pbl": ({"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True},
+                                         "cloud_frac": 0.3, "diffusion": True, "diff_kw": {"lpbl": True}}, 6),
 }
 # diagnostics stored with the diffusion cases (t_diag components carried between steps)
 DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_z", "heating_diss")

@@ -30,6 +30,7 @@ CASES = {
     "tiny_moist_o3_freeslip_diff": (dataclasses.replace(tiny_config(moist=True, order=3, freeslip=True)).set_diffusion(),
                                     10, 0.3),
     "dry3d_33x33x20_diff": (dataclasses.replace(dry3d_config(n=33, n_layers=20)).set_diffusion(), 20, 0.0),
+    "dry3d_33x33x20_diff_pbl": (dataclasses.replace(dry3d_config(n=33, n_layers=20), lpbl=True).set_diffusion(), 20, 0.0),
 }
 
 

@@ -61,6 +61,10 @@ CASES = {
     "schaer_61x24_diff": (diff(schaer_config(nx=61, n_layers=24)), 30, 0.0),
     "dry3d_33x33x20_diff": (diff(dry3d_config(n=33, n_layers=20)), 8, 0.0),
     "dry3d_66x40x18_even_diff": (diff(dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66)), 6, 0.0),
+    # the PBL friction (with the device's log, pow and atan: agreement to rounding)
+    "tiny_dry_pbl": (dataclasses.replace(tiny_config(), lpbl=True), 5, 0.0),
+    "tiny_moist_o3_freeslip_diff_pbl": (diff(tiny_config(moist=True, order=3, freeslip=True), lpbl=True), 6, 0.3),
+    "dry3d_33x33x20_diff_pbl": (diff(dry3d_config(n=33, n_layers=20), lpbl=True), 8, 0.0),
 }
 
 
@@ -78,10 +82,12 @@ def test_gpu_matches_oracle(name):
 DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_x", "mom_diff_tend_z", "temp_diff_heating", "heating_diss")
 
 
-@pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "dry3d_33x33x20_diff"])
+@pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "dry3d_33x33x20_diff",
+                                  "dry3d_33x33x20_diff_pbl"])
 def test_gpu_diffusion_diagnostics_match_oracle(name):
-    """The carried t_diag fields of the diffusion (TKE, viscosity, momentum diffusion tendencies,
-    heating rates) after N steps, field by field."""
+    """The carried t_diag fields of the diffusion and the PBL (TKE, viscosity, momentum diffusion
+    tendencies, heating rates, roughness length, Monin-Obukhov length) after N steps, field by
+    field."""
     cfg, n, cf = CASES[name]
     grid, st = make_case(cfg, cloud_frac=cf)
     print(f"case {name} steps={n}", flush=True)
@@ -91,7 +97,7 @@ def test_gpu_diffusion_diagnostics_match_oracle(name):
     o = CpuOracle(cfg, grid)
     o.start(st)
     o.run(0, n)
-    for k in DIFF_DIAGS:
+    for k in DIFF_DIAGS + (("roughness_length", "monin_obukhov_length") if cfg.lpbl else ()):
         a = g.diag(k)
         b = o.get_diag(k, a.shape)
         scale = max(float(np.abs(b).max()), 1e-300)

@@ -33,6 +33,11 @@ CASES = [
     {"kind": "tiny", "kw": {"plane": True, "corio": False}, "steps": 4, "diffusion": True,
      "diff_kw": {"lmom_diff_h": False, "ltemp_diff_h": False, "lmass_diff_h": False}},
     {"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "steps": 12, "diffusion": True},
+    # the PBL friction with the surface turbulence quantities, alone and with diffusion
+    {"kind": "tiny", "kw": {}, "steps": 5, "diff_kw": {"lpbl": True}},
+    {"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "steps": 6, "cloud_frac": 0.3,
+     "diffusion": True, "diff_kw": {"lpbl": True}},
+    {"kind": "dry3d", "kw": {"n": 17, "n_layers": 12}, "steps": 8, "diffusion": True, "diff_kw": {"lpbl": True}},
 ]
 
 
