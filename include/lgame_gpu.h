@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define LGAME_ABI_VERSION 2
+#define LGAME_ABI_VERSION 3
 
 /* return codes */
 #define LGAME_OK          0
@@ -134,9 +134,15 @@ typedef struct lgame_dims {
  *
  * lpbl (mo_surface_nml.F90) turns on the surface turbulence quantities (update_sfc_turb_quantities,
  * mo_pbl.F90:19-109) and the PBL friction of the lowest-layer wind (pbl_wind_tendency,
- * :111-250), with h_prandtl and karman (mo_diff_nml.F90:39-44, defaults 100 m and 0.4).  The
- * prognostic soil, the surface sensible heat flux and the surface phase transitions are not part
- * of this library (lgame_init rejects them).
+ * :111-250), with h_prandtl and karman (mo_diff_nml.F90:39-44, defaults 100 m and 0.4); as in
+ * the reference it needs lmom_diff_h.  lprog_soil_temp adds the soil temperature rows to the
+ * vertical solve (mo_column_solvers.F90:229-336) with nsoillays in 2..16 layers;
+ * lsfc_sensible_heat_flux adds the sensible heat flux over soil and sea to the lowest layer
+ * (:81-120) and needs lprog_soil_temp (the scalar flux resistance it divides by exists only with
+ * the soil, mo_pbl.F90:96-104).  lsfc_phase_trans only turns on the surface turbulence
+ * quantities: the reference's surface phase transitions sit in a branch that never runs.  The
+ * prognostic soil with radiation is rejected (the radiation's surface temperature and fluxes
+ * would couple to the soil).
  */
 typedef struct lgame_cfg {
   double dtime;
@@ -209,9 +215,17 @@ typedef struct lgame_grid_view {
   const double *dx_dual;               /* (ny+1,nx,nl)   read only with diffusion */
   const double *dy_dual;               /* (ny,nx+1,nl)   read only with diffusion */
   const double *layer_thickness;       /* (ny,nx,nl)     read only with diffusion */
-  const double *land_fraction;         /* (ny,nx)        read only with lpbl      */
-  const double *roughness_length;      /* (ny,nx)        read only with lpbl      */
+  const double *land_fraction;         /* (ny,nx)        read only with the surface switches */
+  const double *roughness_length;      /* (ny,nx)        read only with the surface switches */
+  const double *lake_fraction;         /* (ny,nx)        read only with lprog_soil_temp     */
+  const double *sfc_rho_c;             /* (ny,nx)        volumetric heat capacity, same     */
+  const double *t_conduc_soil;         /* (ny,nx)        temperature conductivity, same     */
+  const double *t_const_soil;          /* (ny,nx)        temperature at z_t_const, same     */
+  const double *sst;                   /* (ny,nx)        diag%sst, constant over a run, same */
+  const double *z_soil_interface;      /* (nsoillays+1)  same                               */
+  const double *z_soil_center;         /* (nsoillays)    same                               */
   double mean_velocity_area;           /* grid%mean_velocity_area, mo_grid_generator.F90:1111 */
+  double z_t_const;                    /* grid%z_t_const, mo_grid_generator.F90:1363          */
 } lgame_grid_view;
 
 /* Prognostic state (t_state, src/mo_definitions.F90:94-105), reference layout. */
@@ -223,7 +237,7 @@ typedef struct lgame_state_view {
   double *wind_u;            /* (ny,nx+1,nl)              */
   double *wind_v;            /* (ny+1,nx,nl)              */
   double *wind_w;            /* (ny,nx,nl+1)              */
-  double *temperature_soil;  /* (ny,nx,nsoillays); may be NULL when nsoillays==0 or soil is off */
+  double *temperature_soil;  /* (ny,nx,nsoillays); read and written with lprog_soil_temp, else may be NULL */
 } lgame_state_view;
 
 typedef struct lgame_handle lgame_handle;
@@ -255,8 +269,11 @@ typedef struct lgame_rad_view {
 #define LGAME_DIAG_MOM_DIFF_TEND_Z 4   /* diag%mom_diff_tend_z (ny,nx,nl+1)                         */
 #define LGAME_DIAG_TEMP_DIFF_HEATING 5 /* diag%temp_diff_heating (ny,nx,nl)                         */
 #define LGAME_DIAG_HEATING_DISS    6   /* diag%heating_diss (ny,nx,nl)                              */
-#define LGAME_DIAG_ROUGHNESS_LENGTH 7  /* diag%roughness_length (ny,nx), with lpbl                  */
-#define LGAME_DIAG_MONIN_OBUKHOV   8   /* diag%monin_obukhov_length (ny,nx), with lpbl              */
+#define LGAME_DIAG_ROUGHNESS_LENGTH 7  /* diag%roughness_length (ny,nx), with the surface switches   */
+#define LGAME_DIAG_MONIN_OBUKHOV   8   /* diag%monin_obukhov_length (ny,nx), same                   */
+#define LGAME_DIAG_SCALAR_FLUX_RESISTANCE 9 /* diag%scalar_flux_resistance (ny,nx), with the sensible heat flux */
+#define LGAME_DIAG_PFD_SENS_SOIL   10  /* diag%power_flux_density_sens_soil (ny,nx), same           */
+#define LGAME_DIAG_PFD_SENS_SEA    11  /* diag%power_flux_density_sens_sea (ny,nx), same            */
 
 /* Version of this ABI (LGAME_ABI_VERSION). */
 int lgame_abi_version(void);

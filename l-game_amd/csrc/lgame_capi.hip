@@ -31,6 +31,7 @@ struct lgame_handle {
   double *red = nullptr;         // reduction partials
   unsigned long long *counter = nullptr;
   double *trsk = nullptr;        // TRSK weights, FP64 then FP32
+  double *soil_z = nullptr;      // soil grid: interfaces, centres, z_t_const (with the prognostic soil)
   hipStream_t stream = nullptr;
   int have_state[2] = {0, 0};
   int stepped = 0;
@@ -117,7 +118,7 @@ double klemp_prefactor(const lgame_cfg &cf, double z) {
 // diag%roughness_length = grid%roughness_length (control.F90:456-459): the carried surface
 // roughness starts from the grid's, at lgame_init and whenever the work arrays are zeroed again
 hipError_t init_surface(lgame_handle *h) {
-  if (!h->cfg.lpbl) return hipSuccess;
+  if (!h->c.C.lsfc_turb) return hipSuccess;
   const size_t n = (size_t)h->c.G.lvl;
   hipError_t e = h->mixed ? hipMemcpyAsync(h->cf.w.sfc.p, h->cf.g.roughness_length.p, n * sizeof(float), hipMemcpyDeviceToDevice, h->stream)
                           : hipMemcpyAsync(h->c.w.sfc.p, h->c.g.roughness_length.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
@@ -291,7 +292,7 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
 
   LG_LAUNCH(KI_TEMPERATURE, k_temperature, nx, ny, nl, c, old);
   // update_sfc_turb_quantities with state_old (mo_manage_pchevi.F90:52-54)
-  if (c.C.lpbl) LG_LAUNCH(KI_DIFF_DUAL, k_sfc_turb, nx, ny, 1, c, old);
+  if (c.C.lsfc_turb) LG_LAUNCH(KI_DIFF_DUAL, k_sfc_turb, nx, ny, 1, c, old);
   // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
   if (rad) {
     if (prof) prof->begin(3);
@@ -348,7 +349,9 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
     }
     LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx, ny, nl, c, S, nw);
     LG_LAUNCH_ZFAST(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
-    // three_band_solver_ver (mo_column_solvers.F90:21-351); w_free_slip is done in k_scalar_div
+    // three_band_solver_ver (mo_column_solvers.F90:21-351): the surface part, then the atmospheric
+    // columns; w_free_slip is done in k_scalar_div
+    if (c.C.lsoil || c.C.lsens) LG_LAUNCH(KI_DIFF_DUAL, k_surface, nx, ny, 1, c, old, nw, rk);
     if (prof) prof->begin(0);
     hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
     if (prof) prof->end(0);
@@ -518,13 +521,24 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
                 "(scalar_diff_coeffs, mo_eff_diff_coeffs.F90:540-590, then reads stale vorticity and divergence)");
   if (f.diff_coeff_scheme_h != 0 && f.diff_coeff_scheme_h != 1)
     return fail(nullptr, LGAME_EINVAL, "diff_coeff_scheme_h must be 0 (smag) or 1 (tke)");
+  if (f.lpbl && !f.lmom_diff_h)
+    return fail(nullptr, LGAME_EINVAL, "lpbl needs lmom_diff_h (the reference stops otherwise, mo_surface_nml.F90)");
+  if (f.lsfc_sensible_heat_flux && !f.lprog_soil_temp)
+    return fail(nullptr, LGAME_EINVAL,
+                "unsupported: the surface sensible heat flux without the prognostic soil (the scalar flux resistance it "
+                "divides by is computed only with the soil, mo_pbl.F90:96-104)");
+  if (f.lprog_soil_temp && (d.nsoillays < 2 || d.nsoillays > MAX_SOIL || d.nsoillays > d.n_layers + 1))
+    return fail(nullptr, LGAME_EINVAL, "with the prognostic soil, nsoillays must be in 2..%d and at most n_layers+1", MAX_SOIL);
+  if (f.lprog_soil_temp && d.n_layers < 4) return fail(nullptr, LGAME_EINVAL, "the prognostic soil needs n_layers >= 4");
+  if (f.lprog_soil_temp && f.lrad)
+    return fail(nullptr, LGAME_EINVAL, "unsupported: radiation together with the prognostic soil (surface temperature and fluxes)");
+  // the surface turbulence quantities (update_sfc_turb_quantities, mo_manage_pchevi.F90:52-54)
+  const bool sfc_turb = f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans;
   // the diffusion work arrays (temperature and mass diffusion need lmom_diff_h; the PBL friction
-  // adds to the momentum diffusion tendencies)
-  const bool diffusion = f.lmom_diff_h || f.lmom_diff_v || f.lpbl;
+  // adds to the momentum diffusion tendencies; the surface fields live there too)
+  const bool diffusion = f.lmom_diff_h || f.lmom_diff_v || sfc_turb;
   if (f.lmom_diff_h && !(grid->mean_velocity_area > 0.0)) return fail(nullptr, LGAME_EINVAL, "mean_velocity_area must be positive");
-  if (f.lsfc_sensible_heat_flux || f.lsfc_phase_trans || f.lprog_soil_temp)
-    return fail(nullptr, LGAME_EINVAL, "unsupported: the surface sensible heat flux, surface phase transitions and soil must be off");
-  if (f.lpbl && !(f.karman > 0.0 && f.h_prandtl > 0.0)) return fail(nullptr, LGAME_EINVAL, "karman and h_prandtl must be positive");
+  if (sfc_turb && !(f.karman > 0.0 && f.h_prandtl > 0.0)) return fail(nullptr, LGAME_EINVAL, "karman and h_prandtl must be positive");
   if (f.rad_mode != LGAME_RAD_OFF && f.rad_mode != LGAME_RAD_TEACHER && f.rad_mode != LGAME_RAD_SURROGATE)
     return fail(nullptr, LGAME_EINVAL, "unknown radiation mode %d", f.rad_mode);
   if (f.lrad && f.rad_mode == LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "lrad needs a radiation mode (LGAME_RAD_*)");
@@ -543,8 +557,11 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   if (f.lmom_diff_h) {
     NEED(dx_dual) NEED(dy_dual) NEED(layer_thickness)
   }
-  if (f.lpbl) {
+  if (sfc_turb || f.lprog_soil_temp) {
     NEED(land_fraction) NEED(roughness_length)
+  }
+  if (f.lprog_soil_temp) {
+    NEED(lake_fraction) NEED(sfc_rho_c) NEED(t_conduc_soil) NEED(t_const_soil) NEED(sst) NEED(z_soil_interface) NEED(z_soil_center)
   }
 #undef NEED
 
@@ -593,6 +610,10 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   C.lpbl = f.lpbl != 0;
   C.h_prandtl = f.h_prandtl;
   C.karman = f.karman;
+  C.lsoil = f.lprog_soil_temp != 0;
+  C.lsens = f.lsfc_sensible_heat_flux != 0;
+  C.lsfc_turb = sfc_turb;
+  C.nsoil = C.lsoil ? d.nsoillays : 0;
 
   // n_damping_levels, as mo_grid_generator.F90:1050-1058 derives it from z_w
   {
@@ -618,8 +639,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   h->mixed = mixed;
   constexpr int NGRID = 40;
   const size_t fs = (size_t)G.fsize;
-  const size_t nstate = (size_t)2 * (nc + 6);
-  const size_t ndiff = diffusion ? (size_t)13 + nc + 3 * (nc + 1) : 0;  // the diffusion fields of WorkT
+  const size_t nstate = (size_t)2 * (nc + 6 + (C.lsoil ? 1 : 0));
+  const size_t ndiff = diffusion ? (size_t)14 + nc + 3 * (nc + 1) : 0;  // the diffusion and surface fields of WorkT
   const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
@@ -651,6 +672,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     S.wind_u = take8();
     S.wind_v = take8();
     S.wind_w = take8();
+    S.temperature_soil = C.lsoil ? take8() : nullptr;
   }
   h->state_bytes = used;
   // grid: NGRID consecutive FP64 fields in a fixed order (the FP32 copy uses the same order)
@@ -718,6 +740,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
       W.tdh = take();
       W.rsum = take();
       W.sfc = take();
+      W.sfc2 = take();
       for (int jc = 0; jc < MAX_CONST; ++jc) W.mdt[jc] = jc < nc ? take() : nullptr;
       for (int q = 0; q <= MAX_CONST; ++q) {
         W.dfu[q] = q <= nc ? take() : nullptr;
@@ -760,7 +783,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     G_Z_SCALAR, G_Z_U, G_Z_V, G_Z_W, G_DX, G_DY, G_DZ, G_AREA_X, G_AREA_Y, G_AREA_Z, G_VOLUME, G_SLOPE_X, G_SLOPE_Y,
     G_IPW, G_AREA_DUAL_X = G_IPW + 6, G_AREA_DUAL_Y, G_AREA_DUAL_Z, G_Z_AREA_DUAL_Z, G_FVEC_X, G_FVEC_Y, G_FVEC_Z,
     G_EBG_U, G_EBG_V, G_EBG_W, G_THETA_V_BG, G_EXNER_BG, G_GRAVITY_M_V, G_GRAVITY_POTENTIAL, G_KLEMP_PREF, G_DX_DUAL,
-    G_DY_DUAL, G_LAYER_THICKNESS, G_LAND_FRACTION, G_ROUGHNESS_LENGTH, G_COUNT
+    G_DY_DUAL, G_LAYER_THICKNESS, G_LAND_FRACTION, G_ROUGHNESS_LENGTH, G_SOIL, G_COUNT
   };
   static_assert(G_COUNT <= NGRID, "grid field budget");
   auto gf = [&](int q) { return grid64 + (size_t)q * fs; };
@@ -798,9 +821,23 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     rc = rc ? rc : upload(h, gf(G_DY_DUAL), grid->dy_dual, ny, nx + 1, nl);
     rc = rc ? rc : upload(h, gf(G_LAYER_THICKNESS), grid->layer_thickness, ny, nx, nl);
   }
-  if (f.lpbl) {
+  if (sfc_turb || C.lsoil) {
     rc = rc ? rc : upload(h, gf(G_LAND_FRACTION), grid->land_fraction, ny, nx, 1);
     rc = rc ? rc : upload(h, gf(G_ROUGHNESS_LENGTH), grid->roughness_length, ny, nx, 1);
+  }
+  if (C.lsoil) {
+    const double *soil2d[5] = {grid->lake_fraction, grid->sfc_rho_c, grid->t_conduc_soil, grid->t_const_soil, grid->sst};
+    for (int q = 0; q < 5; ++q) rc = rc ? rc : upload(h, gf(G_SOIL) + (size_t)q * G.lvl, soil2d[q], ny, nx, 1);
+    const int ns = d.nsoillays;
+    std::vector<double> z((size_t)2 * ns + 2);
+    std::memcpy(z.data(), grid->z_soil_interface, sizeof(double) * (ns + 1));
+    std::memcpy(z.data() + ns + 1, grid->z_soil_center, sizeof(double) * ns);
+    z[2 * ns + 1] = grid->z_t_const;
+    if (!rc) {
+      e = hipMalloc(&h->soil_z, z.size() * sizeof(double));
+      if (e == hipSuccess) e = hipMemcpy(h->soil_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = fail(h, LGAME_EHIP, "soil grid: %s", hipGetErrorString(e));
+    }
   }
   if (!rc) {
     std::vector<double> kp((size_t)ny * nx * nlev);
@@ -865,6 +902,8 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     g.layer_thickness = at(G_LAYER_THICKNESS);
     g.land_fraction = at(G_LAND_FRACTION);
     g.roughness_length = at(G_ROUGHNESS_LENGTH);
+    g.soil = at(G_SOIL);
+    g.soil_z = h->soil_z;
     g.trsk_u = trsk;
     g.trsk_v = trsk + 3 * ny;
   };
@@ -893,6 +932,8 @@ static int state_io(lgame_handle *h, int slot, lgame_state_view *st, bool up) {
   StateDev &S = h->slot[slot];
   if (!st->rho || !st->rhotheta_v || !st->theta_v_pert || !st->exner_pert || !st->wind_u || !st->wind_v || !st->wind_w)
     return fail(h, LGAME_EINVAL, "null state field");
+  const bool soil = h->c.C.lsoil != 0;
+  if (soil && !st->temperature_soil) return fail(h, LGAME_EINVAL, "temperature_soil is null (the prognostic soil is on)");
   int rc = 0;
   const size_t ncell = (size_t)ny * nx * nl;
   for (int jc = 0; jc < nc && !rc; ++jc)
@@ -907,6 +948,9 @@ static int state_io(lgame_handle *h, int slot, lgame_state_view *st, bool up) {
     if (rc) break;
     rc = up ? upload(h, x.dev, x.host, x.n1, x.n2, x.n3) : download(h, x.host, x.dev, x.n1, x.n2, x.n3);
   }
+  if (!rc && soil)
+    rc = up ? upload(h, S.temperature_soil.p, st->temperature_soil, ny, nx, h->dims.nsoillays)
+            : download(h, st->temperature_soil, S.temperature_soil.p, ny, nx, h->dims.nsoillays);
   if (!rc && up) h->have_state[slot] = 1;
   return rc;
 }
@@ -1018,7 +1062,7 @@ int lgame_check_finite(lgame_handle *h, int slot, long long *n_bad) {
 int lgame_get_diag(lgame_handle *h, int which, double *out) {
   if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
   const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers;
-  const bool diffusion = h->cfg.lmom_diff_h || h->cfg.lmom_diff_v || h->cfg.lpbl;
+  const bool diffusion = h->cfg.lmom_diff_h || h->cfg.lmom_diff_v || h->c.C.lsfc_turb;
   auto get = [&](auto &w) -> int {
     switch (which) {
       case LGAME_DIAG_TEMPERATURE: return download(h, out, w.temperature.p, ny, nx, nl);
@@ -1034,8 +1078,14 @@ int lgame_get_diag(lgame_handle *h, int which, double *out) {
       case LGAME_DIAG_TEMP_DIFF_HEATING: return download(h, out, w.tdh.p, ny, nx, nl);
       case LGAME_DIAG_ROUGHNESS_LENGTH:
       case LGAME_DIAG_MONIN_OBUKHOV:
-        if (!h->cfg.lpbl) return fail(h, LGAME_EINVAL, "diagnostic %d exists only with lpbl", which);
+        if (!h->c.C.lsfc_turb)
+          return fail(h, LGAME_EINVAL, "diagnostic %d exists only with the surface turbulence quantities", which);
         return download(h, out, w.sfc.p + (which == LGAME_DIAG_MONIN_OBUKHOV ? 2 * (size_t)h->c.G.lvl : 0), ny, nx, 1);
+      case LGAME_DIAG_SCALAR_FLUX_RESISTANCE:
+      case LGAME_DIAG_PFD_SENS_SOIL:
+      case LGAME_DIAG_PFD_SENS_SEA:
+        if (!h->c.C.lsens) return fail(h, LGAME_EINVAL, "diagnostic %d exists only with the sensible heat flux", which);
+        return download(h, out, w.sfc2.p + (size_t)(which - LGAME_DIAG_SCALAR_FLUX_RESISTANCE) * (size_t)h->c.G.lvl, ny, nx, 1);
       default: return fail(h, LGAME_EINVAL, "unknown diagnostic %d", which);
     }
   };
@@ -1381,6 +1431,7 @@ void lgame_finalize(lgame_handle *h) {
   if (h->red) (void)hipFree(h->red);
   if (h->counter) (void)hipFree(h->counter);
   if (h->trsk) (void)hipFree(h->trsk);
+  if (h->soil_z) (void)hipFree(h->soil_z);
   for (void *p : {(void *)h->rad_mem, (void *)h->rad_x, (void *)h->rad_y, (void *)h->rad_counts, (void *)h->rad_sq})
     if (p) (void)hipFree(p);
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);

@@ -35,6 +35,8 @@ constexpr double IMPL_THERMO_WEIGHT = C_D_V / C_D_P;
 constexpr double MPI = 3.14159265358979323846;
 
 constexpr int MAX_CONST = 7;
+// soil layers the surface kernel holds in registers
+constexpr int MAX_SOIL = 16;
 
 // XCDs of the MI355X; blocks of a launch go to them round-robin.
 constexpr int N_XCD = 8;
@@ -74,7 +76,13 @@ struct GridT {
   Fld<const R> exner_bg, gravity_m_v, gravity_potential;
   Fld<const R> klemp_pref;  // 1 + damping_coeff*dtime at w levels (mo_column_solvers.F90:190-196)
   Fld<const R> dx_dual, dy_dual, layer_thickness;  // read by the diffusion kernels only
-  Fld<const R> land_fraction, roughness_length;    // 2-D (level 1), read by the PBL kernels only
+  Fld<const R> land_fraction, roughness_length;    // 2-D (level 1), read by the surface kernels only
+  // 2-D surface fields of the soil at levels 1..5: lake_fraction, sfc_rho_c, t_conduc_soil,
+  // t_const_soil and the sea surface temperature (diag%sst, constant over a run)
+  Fld<const R> soil;
+  // soil grid (FP64 in both forms): z_soil_interface(1:nsoillays+1), z_soil_center(1:nsoillays),
+  // z_t_const, consecutively
+  const double *soil_z;
   const R *trsk_u;          // (3, ny) as in the reference
   const R *trsk_v;          // (2, ny+1)
 };
@@ -83,6 +91,7 @@ using GridDev = GridT<double>;
 struct StateDev {
   Fld<double> rho[MAX_CONST];
   Fld<double> rhotheta_v, theta_v_pert, exner_pert, wind_u, wind_v, wind_w;
+  Fld<double> temperature_soil;  // soil layer jl at level jl (with the prognostic soil)
 };
 
 // Tendencies, carried diagnostics and scratch of one handle.
@@ -109,8 +118,10 @@ struct WorkT {
   // rsum is sum(rho(1:ncond+1)) of the old state, stored by k_temperature.
   Fld<R> tke, tke_next, visc, kv, vdiv, zeta, vz, mom_x, mom_y, mom_z, tdh, mdt[MAX_CONST], rsum;
   Fld<R> dfu[MAX_CONST + 1], dfv[MAX_CONST + 1], dfw[MAX_CONST + 1];
-  // PBL: diag%roughness_length, roughness_velocity, monin_obukhov_length at levels 1, 2, 3
-  Fld<R> sfc;
+  // surface turbulence: diag%roughness_length, roughness_velocity, monin_obukhov_length at
+  // levels 1, 2, 3; surface heat: diag%scalar_flux_resistance, power_flux_density_sens_soil and
+  // power_flux_density_sens_sea at levels 1, 2, 3 of sfc2
+  Fld<R> sfc, sfc2;
 };
 
 struct Cfg {
@@ -120,6 +131,10 @@ struct Cfg {
   // grid%mean_velocity_area (mo_grid_generator.F90:1111)
   int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_tke, lpbl;
   double c_s, bg_shear, eff_hor_res, mva, h_prandtl, karman;
+  // surface (mo_surface_nml.F90): prognostic soil with nsoil layers, sensible heat flux, and
+  // whether the surface turbulence quantities are updated (lpbl, the sensible heat flux or the
+  // surface phase transitions)
+  int lsoil, lsens, lsfc_turb, nsoil;
 };
 
 template <class R>

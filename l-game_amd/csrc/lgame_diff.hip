@@ -456,8 +456,30 @@ __device__ R pbl_friction(const CtxT<R> &c, R wind, R z_pt, ix_t a, ix_t b, ix_t
   return f * wind / fr / lt;
 }
 
-// update_sfc_turb_quantities, mo_pbl.F90:19-109, one column per thread (no prognostic soil, so no
-// scalar flux resistance); v_squared is the kinetic energy the last momentum advection block left
+// psi_h, mo_pbl.F90:368-390
+template <class R>
+__device__ R psi_h(R eff, R l) {
+  R ll = l;
+  if (fabs(ll) < R(EPSILON_SECURITY)) ll = R(EPSILON_SECURITY);
+  if (ll < R(0.0)) {
+    const R x = pow(R(1.0) - R(15.0) * eff / ll, R(0.25));
+    return R(2.0) * log((R(1.0) + x * x) / R(2.0));
+  }
+  return -(R(4.0) * eff / ll);
+}
+
+// scalar_flux_resistance, mo_pbl.F90:294-316
+template <class R>
+__device__ R scalar_flux_resistance(const CtxT<R> &c, R us, R z_agl, R z0, R l) {
+  const R used = fmin2(z_agl, R(c.C.h_prandtl));
+  R r = R(1.0) / (R(c.C.karman) * us) * (log(used / z0) - psi_h(used, l) + log(R(7.0)));
+  if (r < R(c.C.dtime) / z_agl) r = R(c.C.dtime) / z_agl;
+  return r;
+}
+
+// update_sfc_turb_quantities, mo_pbl.F90:19-109, one column per thread, with the scalar flux
+// resistance of the prognostic soil; v_squared is the kinetic energy the last momentum advection
+// block left
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const StateDev s) {
   KIDX;
@@ -486,7 +508,9 @@ __global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const Sta
   const R t_pert = -(R(0.2) * R(c.C.dtime) * w_pert * dth);
   R wt = R(0.2) * w_pert * t_pert;
   if (fabs(wt) < R(EPSILON_SECURITY)) wt = R(EPSILON_SECURITY);
-  sfc[i1 + 2 * dl] = -(th_l * (us * (us * us)) / (R(c.C.karman) * R(GRAVITY) * wt));
+  const R mo_len = -(th_l * (us * (us * us)) / (R(c.C.karman) * R(GRAVITY) * wt));
+  sfc[i1 + 2 * dl] = mo_len;
+  if (c.C.lsoil) c.w.sfc2[i1] = scalar_flux_resistance(c, us, agl, z0, mo_len);
 }
 
 // pbl_wind_tendency, mo_pbl.F90:111-250: mom_diff_tend_x/_y of the lowest layer minus the
@@ -509,6 +533,126 @@ __global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const Sta
   if (ji == 1) AT(c.w.mom_y, ny + 1, jk, nl) = my;
 }
 
+// ---------------------------------------------------------------------------------------------
+// the surface of three_band_solver_ver (mo_column_solvers.F90:21-351), one column per thread, in
+// FP64 in both forms: the sensible power flux densities over soil and sea and their heating of
+// the lowest layer (:81-120), and the soil temperature rows of the column's linear system
+// (:202-204, :229-300, :331-336).  In the reference's combined system the lower and upper
+// diagonals vanish at row n_layers-1 (c_vector(n_layers-1) = e_vector(n_layers-1) = 0), so the
+// elimination of the soil rows neither reads nor changes anything of the atmospheric rows, and
+// eliminating the soil rows on their own, as here, gives the same values.  Radiation is not
+// coupled to the soil here (lgame_init rejects the pair), so the short- and longwave surface
+// terms are zero.
+// ---------------------------------------------------------------------------------------------
+
+// the fields of a state that the sensible heat flux reads
+struct SfcState {
+  Fld<const double> ep, tvp, rho_g, rho_v, ts;
+};
+
+// the gas temperature of the lowest layer (mo_column_solvers.F90:84-93)
+__device__ __forceinline__ double t_gas_lowest(const GridT<double> &g, const SfcState &s, bool moist, ix_t il) {
+  double t = (g.exner_bg[il] + s.ep[il]) * (g.theta_v_bg[il] + s.tvp[il]);
+  if (moist) t = t / (1.0 + s.rho_v[il] / s.rho_g[il] * (M_D / M_V - 1.0));
+  return t;
+}
+
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  KIDX;
+  const Geom &G = c.G;
+  const int nl = G.nl;
+  if (jk > G.nx || ji > G.ny || jl > 1) return;
+  // the state fields read, picked one by one (a reference to either state argument would place
+  // both in scratch memory)
+  const bool moist = c.C.lmoist != 0;
+  const int gi = G.ncond, vi = moist ? G.ncond + 1 : G.ncond;
+  const SfcState so{old.exner_pert, old.theta_v_pert, old.rho[gi], old.rho[vi], old.temperature_soil};
+  const SfcState sn = rk == 1 ? so : SfcState{nw.exner_pert, nw.theta_v_pert, nw.rho[gi], nw.rho[vi], nw.temperature_soil};
+  const GridT<double> &g = c.g64;
+  const ix_t dl = (ix_t)G.lvl;
+  const ix_t i1 = LG_IX(G, ji, jk, 1), il = LG_IX(G, ji, jk, nl);
+  const double dt = c.C.dtime;
+  double ps = 0.0;  // diag%power_flux_density_sens_soil, zero without the sensible heat flux
+  if (c.C.lsens) {
+    const double t_old = t_gas_lowest(g, so, moist, il);
+    const double t_new = t_gas_lowest(g, sn, moist, il);
+    const double lf = g.land_fraction[i1], kf = g.soil[i1], sst = g.soil[i1 + 4 * dl];
+    const double rn = sn.rho_g[il], ro = so.rho_g[il];
+    const double sfr = (double)c.w.sfc2[i1];
+    ps = (lf + kf) * 0.5 * C_D_V * (rn * (t_old - so.ts[i1]) + ro * (t_new - sn.ts[i1])) / sfr;
+    const double pe = (1.0 - lf - kf) * 0.5 * C_D_V * (rn * (t_old - sst) + ro * (t_new - sst)) / sfr;
+    c.w.sfc2[i1 + dl] = (R)ps;
+    c.w.sfc2[i1 + 2 * dl] = (R)pe;
+    const double trt = (double)c.w.tend_rhotheta_v[il];
+    c.w.tend_rhotheta_v[il] = (R)(trt - g.area_z[il + dl] * (ps + pe) / ((g.exner_bg[il] + sn.ep[il]) * C_D_P) / g.volume[il]);
+  }
+  if (!c.C.lsoil) return;
+  // the soil rows: heat conduction between the layers and to the constant temperature at
+  // z_t_const below, the surface fluxes into the top layer, Crank-Nicolson in time
+  const int ns = c.C.nsoil;
+  const double *zi = g.soil_z, *zc = zi + ns + 1;
+  const double z_t_const = zc[ns];
+  const double rc = g.soil[i1 + dl], tc = g.soil[i1 + 2 * dl], tcs = g.soil[i1 + 3 * dl];
+  const double sw_in = 0.0, lw_out = 0.0, lat_lake = 0.0;  // no radiation here; no latent heat (phase transitions off)
+  // rows (:229-288) and the forward sweep of thomas_algorithm (:625-634), one row at a time: the
+  // explicit heat flux densities hfd(j) of the layer above and of this layer, and the soil
+  // temperatures of this layer and the one below, roll along
+  double ep[MAX_SOIL], rp[MAX_SOIL];
+  double t_cur = so.ts[i1];
+  double h_up = 0.0;    // hfd of the layer above
+  double c_prev = 0.0;  // c_vector of the row above (the lower diagonal entry of this row)
+#pragma unroll
+  for (int j = 0; j < MAX_SOIL; ++j) {
+    if (j < ns) {
+      double t_next = 0.0, h;
+      if (j < ns - 1) {
+        t_next = so.ts[i1 + (ix_t)(j + 1) * dl];
+        h = -rc * tc * (t_cur - t_next) / (zc[j] - zc[j + 1]);
+      } else {
+        h = -rc * tc * (t_cur - tcs) / (2.0 * (zc[j] - z_t_const));
+      }
+      double r;
+      if (j == 0)
+        r = t_cur + (ps + lat_lake + sw_in - lw_out + 0.5 * h) / ((zi[0] - zi[1]) * rc) * dt;
+      else
+        r = t_cur + 0.5 * (-h_up + h) / ((zi[j] - zi[j + 1]) * rc) * dt;
+      const double f = 0.5 * dt * rc * tc / ((zi[j] - zi[j + 1]) * rc);
+      double d;
+      if (j == 0)
+        d = 1.0 + f * 1.0 / (zc[j] - zc[j + 1]);
+      else if (j == ns - 1)
+        d = 1.0 + f * 1.0 / (zc[j - 1] - zc[j]);
+      else
+        d = 1.0 + f * (1.0 / (zc[j - 1] - zc[j]) + 1.0 / (zc[j] - zc[j + 1]));
+      double e = 0.0, cj = 0.0;
+      if (j < ns - 1) {
+        cj = -0.5 * dt * rc * tc / ((zi[j + 1] - zi[j + 2]) * rc) / (zc[j] - zc[j + 1]);
+        e = -0.5 * dt * rc * tc / ((zi[j] - zi[j + 1]) * rc) / (zc[j] - zc[j + 1]);
+      }
+      if (j == 0) {
+        ep[0] = e / d;
+        rp[0] = r / d;
+      } else {
+        if (j < ns - 1) ep[j] = e / (d - ep[j - 1] * c_prev);
+        rp[j] = (r - rp[j - 1] * c_prev) / (d - ep[j - 1] * c_prev);
+      }
+      c_prev = cj;
+      h_up = h;
+      t_cur = t_next;
+    }
+  }
+  // upward sweep (:636-639) and the new soil temperature (:331-336)
+  double x = 0.0;
+#pragma unroll
+  for (int j = MAX_SOIL - 1; j >= 0; --j) {
+    if (j < ns) {
+      x = j == ns - 1 ? rp[j] : rp[j] - ep[j] * x;
+      nw.temperature_soil[i1 + (ix_t)j * dl] = x;
+    }
+  }
+}
+
 #define LG_INSTANTIATE(R)                                                                  \
   template __global__ void k_diff_zeta<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_coef<R>(const CtxT<R>, const StateDev);                  \
@@ -517,7 +661,8 @@ __global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const Sta
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_div<R>(const CtxT<R>);                                   \
   template __global__ void k_sfc_turb<R>(const CtxT<R>, const StateDev);                   \
-  template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);
+  template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);                   \
+  template __global__ void k_surface<R>(const CtxT<R>, const StateDev, const StateDev, int);
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE

@@ -31,6 +31,7 @@ template <class R> __global__ void k_diff_flux(const CtxT<R> c, const StateDev s
 template <class R> __global__ void k_diff_div(const CtxT<R> c);
 template <class R> __global__ void k_sfc_turb(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_pbl_wind(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_surface(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 
 // Launch geometry of the horizontal stencil kernels: 64 x-points by BY rows per block (the
 // default; the tile table can choose 1, 2 or 4 rows per kernel).

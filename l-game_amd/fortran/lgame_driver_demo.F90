@@ -68,7 +68,8 @@ program lgame_driver_demo
                        c_loc(ipw),c_loc(area_dual_x),c_loc(area_dual_y),c_loc(area_dual_z),c_loc(z_area_dual_z), &
                        c_loc(fvec_x),c_loc(fvec_y),c_loc(fvec_z),c_loc(trsk_u),c_loc(trsk_v),c_loc(ebg_u), &
                        c_loc(ebg_v),c_loc(ebg_w),c_loc(theta_v_bg),c_loc(exner_bg),c_loc(gravity_m_v), &
-                       c_loc(gravity_potential),c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,0._c_double)
+                       c_loc(gravity_potential),c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr, &
+                       c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,0._c_double,0._c_double)
   rc = lgame_init(dims,cfg,gv,h)
   call lgame_check(rc,c_null_ptr,"lgame_init")
 

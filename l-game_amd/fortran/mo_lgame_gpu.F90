@@ -36,7 +36,8 @@ module mo_lgame_gpu
     type(c_ptr) :: fvec_x, fvec_y, fvec_z, trsk_weights_u, trsk_weights_v
     type(c_ptr) :: exner_bg_grad_u, exner_bg_grad_v, exner_bg_grad_w, theta_v_bg, exner_bg
     type(c_ptr) :: gravity_m_v, gravity_potential, dx_dual, dy_dual, layer_thickness, land_fraction, roughness_length
-    real(c_double) :: mean_velocity_area
+    type(c_ptr) :: lake_fraction, sfc_rho_c, t_conduc_soil, t_const_soil, sst, z_soil_interface, z_soil_center
+    real(c_double) :: mean_velocity_area, z_t_const
   end type lgame_grid_view
 
   type, bind(C) :: lgame_state_view

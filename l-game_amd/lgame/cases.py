@@ -74,17 +74,43 @@ def land_fraction(cfg: RunConfig, seed: int = 11):
     return np.asfortranarray(np.clip(2.5 * (f - 0.5), 0.0, 1.0))
 
 
+def soil_case(cfg: RunConfig, g: dict, st: dict, seed: int = 13) -> None:
+    """Lakes and a varied soil temperature (set-up extension), so that every term of the soil
+    budget is exercised: lakes cover 40 % of the non-land part of a seeded half of the columns
+    (the heat capacity and conductivity follow from the fractions as in the reference), and the
+    soil temperature is 280 K plus a seeded column offset of up to +-4 K at the surface, relaxing
+    with depth; the sea surface temperature is the initial surface soil temperature, as in
+    mo_set_initial_state.F90:246-247."""
+    from .grid import soil_properties
+
+    rng = np.random.default_rng(seed)
+    lakes = rng.random((cfg.ny, cfg.nx)) < 0.5
+    g["lake_fraction"] = np.asfortranarray(np.where(lakes, 0.4 * (1.0 - g["land_fraction"]), 0.0))
+    soil_properties(g)
+    ns = cfg.nsoillays
+    offset = 4.0 * (2.0 * rng.random((cfg.ny, cfg.nx)) - 1.0)
+    depth_weight = np.exp(-np.arange(ns) / 2.0)
+    st["temperature_soil"] = np.asfortranarray(280.0 + offset[:, :, None] * depth_weight[None, None, :])
+    g["sst"] = np.asfortranarray(st["temperature_soil"][:, :, 0].copy())
+
+
 def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True, cloud_frac=0.0):
     """Grid and initial state for ``cfg`` (perturbations only for the standard scenario).  With
-    the PBL on, the grid's land fraction is the seeded field of land_fraction."""
+    the surface physics on, the grid's land fraction is the seeded field of land_fraction; with
+    the prognostic soil, soil_case sets lakes and the soil temperature."""
+    from .grid import soil_properties
+
     g = grid_setup(cfg)
-    if cfg.lpbl:
+    if cfg.surface:
         g["land_fraction"] = land_fraction(cfg)
+        soil_properties(g)
     st = ideal_init(cfg, g)
     if cfg.scenario == "standard" and perturbation:
         perturb(st, g, cfg, seed=(seed if seed is not None else 2))
     if cfg.lmoist:
         add_moisture(st, g, cfg, seed=(seed if seed is not None else 3) + 1000, cloud_frac=cloud_frac)
+    if cfg.lprog_soil_temp:
+        soil_case(cfg, g, st)
     return g, st
 
 

@@ -79,6 +79,9 @@ class RunConfig:
     orography_id: int = 0
     lsleve: bool = False
     nsoillays: int = 5
+    lprog_soil_temp: bool = False
+    lsfc_sensible_heat_flux: bool = False
+    lsfc_phase_trans: bool = False
     lpbl: bool = False
     # &bc
     lperiodic: bool = True
@@ -125,6 +128,16 @@ class RunConfig:
         return any((self.lmom_diff_h, self.lmom_diff_v, self.ltemp_diff_h, self.lmass_diff_h))
 
     @property
+    def surface(self) -> bool:
+        """Whether the step reads the surface fields (land fraction, roughness length)."""
+        return self.lpbl or self.lsfc_sensible_heat_flux or self.lsfc_phase_trans or self.lprog_soil_temp
+
+    @property
+    def state_fields(self) -> tuple:
+        """The prognostic fields of a state: the soil temperature joins with the prognostic soil."""
+        return STATE_FIELDS + (("temperature_soil",) if self.lprog_soil_temp else ())
+
+    @property
     def gridpoints(self) -> int:
         return self.ny * self.nx * self.n_layers
 
@@ -152,6 +165,9 @@ class RunConfig:
         c.h_prandtl = self.h_prandtl
         c.karman = self.karman
         c.lpbl = int(self.lpbl)
+        c.lprog_soil_temp = int(self.lprog_soil_temp)
+        c.lsfc_sensible_heat_flux = int(self.lsfc_sensible_heat_flux)
+        c.lsfc_phase_trans = int(self.lsfc_phase_trans)
         if self.diff_coeff_scheme_h not in ("smag", "tke"):
             raise ValueError("diff_coeff_scheme_h must be either smag or tke")
         c.diff_coeff_scheme_h = 0 if self.diff_coeff_scheme_h == "smag" else 1
@@ -200,15 +216,23 @@ GRID_FIELDS = {
     "layer_thickness": lambda ny, nx, nl: (ny, nx, nl),
     "land_fraction": lambda ny, nx, nl: (ny, nx),
     "roughness_length": lambda ny, nx, nl: (ny, nx),
+    "lake_fraction": lambda ny, nx, nl: (ny, nx),
+    "sfc_rho_c": lambda ny, nx, nl: (ny, nx),
+    "t_conduc_soil": lambda ny, nx, nl: (ny, nx),
+    "t_const_soil": lambda ny, nx, nl: (ny, nx),
+    "sst": lambda ny, nx, nl: (ny, nx),  # diag%sst, constant over a run (mo_set_initial_state.F90:246-247)
+    "z_soil_interface": lambda ny, nx, nl, ns=5: (ns + 1,),
+    "z_soil_center": lambda ny, nx, nl, ns=5: (ns,),
 }
 # scalar grid quantities that cross the boundary with the fields
-GRID_SCALARS = ("mean_velocity_area",)
+GRID_SCALARS = ("mean_velocity_area", "z_t_const")
 
 STATE_FIELDS = ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w")
 
 
-def state_shape(name: str, ny: int, nx: int, nl: int, nc: int):
+def state_shape(name: str, ny: int, nx: int, nl: int, nc: int, ns: int = 5):
     return {
+        "temperature_soil": (ny, nx, ns),
         "rho": (ny, nx, nl, nc),
         "rhotheta_v": (ny, nx, nl),
         "theta_v_pert": (ny, nx, nl),

@@ -54,6 +54,9 @@ DIAGS = {
     "heating_diss": (6, lambda ny, nx, nl: (ny, nx, nl)),
     "roughness_length": (7, lambda ny, nx, nl: (ny, nx)),
     "monin_obukhov_length": (8, lambda ny, nx, nl: (ny, nx)),
+    "scalar_flux_resistance": (9, lambda ny, nx, nl: (ny, nx)),
+    "power_flux_density_sens_soil": (10, lambda ny, nx, nl: (ny, nx)),
+    "power_flux_density_sens_sea": (11, lambda ny, nx, nl: (ny, nx)),
 }
 
 
@@ -170,8 +173,9 @@ class Step:
             pass
 
     def _view(self, st: dict):
-        arrs = {k: _f64(st[k]) for k in STATE_FIELDS}
-        return arrs, StateView(*[arrs[k].ctypes.data for k in STATE_FIELDS], None)
+        arrs = {k: _f64(st[k]) for k in self.cfg.state_fields}
+        soil = arrs["temperature_soil"].ctypes.data if "temperature_soil" in arrs else None
+        return arrs, StateView(*[arrs[k].ctypes.data for k in STATE_FIELDS], soil)
 
     def set_state(self, slot: int, st: dict):
         arrs, v = self._view(st)
@@ -179,8 +183,10 @@ class Step:
 
     def get_state(self, slot: int) -> dict:
         c = self.cfg
-        out = {k: np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents), order="F") for k in STATE_FIELDS}
-        v = StateView(*[out[k].ctypes.data for k in STATE_FIELDS], None)
+        out = {k: np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents, c.nsoillays), order="F")
+               for k in c.state_fields}
+        soil = out["temperature_soil"].ctypes.data if "temperature_soil" in out else None
+        v = StateView(*[out[k].ctypes.data for k in STATE_FIELDS], soil)
         self._check(self.lib.lgame_get_state(self.h, slot, ctypes.byref(v)), "lgame_get_state")
         return out
 

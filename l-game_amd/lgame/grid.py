@@ -29,6 +29,17 @@ TROPO_HEIGHT = 11000.0
 INV_HEIGHT = 20000.0
 T_GRAD_INV = 0.001
 P_0_STANDARD = 101325.0
+T_0 = 273.15
+EPSILON_SECURITY = 1e-10
+RHO_H2O = 1024.0
+# surface and soil, mo_grid_generator.F90:748-757 and :1362-1363
+DENSITY_SOIL = 1442.0
+C_P_SOIL = 830.0
+C_P_WATER_SFC = 4184.0
+T_CONDUCTIVITY_WATER = 1.4e-7
+T_CONDUCTIVITY_SOIL = 7.5e-7
+SIGMA_SOIL = 0.352
+Z_T_CONST = -10.0
 
 
 def _f(shape):
@@ -187,6 +198,19 @@ def orography(cfg: RunConfig):
     return oro, oro_s
 
 
+def soil_properties(g: dict) -> None:
+    """Volumetric heat capacity and temperature conductivity of the surface from the land and lake
+    fractions, as mo_grid_generator.F90:746-800 sets them: water values where there is no land,
+    land/lake-weighted soil and water values where there is."""
+    land, lake = g["land_fraction"], g["lake_fraction"]
+    has_land = land > EPSILON_SECURITY
+    frac = np.where(has_land, land + lake, 1.0)
+    g["sfc_rho_c"] = np.asfortranarray(np.where(
+        has_land, (land * DENSITY_SOIL * C_P_SOIL + lake * RHO_H2O * C_P_WATER_SFC) / frac, RHO_H2O * C_P_WATER_SFC))
+    g["t_conduc_soil"] = np.asfortranarray(np.where(
+        has_land, (land * T_CONDUCTIVITY_SOIL + lake * T_CONDUCTIVITY_WATER) / frac, T_CONDUCTIVITY_WATER))
+
+
 def grid_setup(cfg: RunConfig) -> dict:
     """Grid metrics of grid_setup (mo_grid_generator.F90:29-1387); periodic boundaries."""
     if not cfg.lperiodic:
@@ -298,6 +322,21 @@ def grid_setup(cfg: RunConfig) -> dict:
     # surface: no land unless a case sets a mask (:326, :764-767); roughness length 0.08 m (:774)
     g["land_fraction"] = np.zeros((ny, nx), order="F")
     g["roughness_length"] = np.full((ny, nx), 0.08, order="F")
+    g["lake_fraction"] = np.zeros((ny, nx), order="F")
+    soil_properties(g)
+    # constant soil temperature at depth, :688; the sea surface temperature of the initial soil
+    # temperature, 280 K (mo_set_initial_state.F90:236-247)
+    g["t_const_soil"] = np.asfortranarray(T_0 + 25.0 * np.cos(2.0 * g["lat_geo_scalar"]))
+    g["sst"] = np.full((ny, nx), 280.0, order="F")
+    # the soil grid, :1360-1378
+    ns = max(cfg.nsoillays, 1)
+    zi = np.zeros(ns + 1)
+    for jl in range(2, ns + 2):
+        zi[jl - 1] = zi[jl - 2] + SIGMA_SOIL ** (ns - jl)
+    zi[1:] = Z_T_CONST / zi[ns] * zi[1:]
+    g["z_soil_interface"] = zi
+    g["z_soil_center"] = 0.5 * (zi[:-1] + zi[1:])
+    g["z_t_const"] = Z_T_CONST
     # area used by the turbulence parameterisations, :1111
     g["mean_velocity_area"] = 2.0 * float(np.sum(area_z[:, :, nl])) / (ny * nx)
 

@@ -18,8 +18,11 @@ R_V = 461.524879
 
 def _zeros(cfg: RunConfig):
     ny, nx, nl, nc = cfg.ny, cfg.nx, cfg.n_layers, cfg.n_constituents
-    return {k: np.zeros(state_shape(k, ny, nx, nl, nc), order="F") for k in
-            ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w")}
+    st = {k: np.zeros(state_shape(k, ny, nx, nl, nc), order="F") for k in
+          ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w")}
+    if cfg.lprog_soil_temp:  # 280 K in every soil layer, mo_set_initial_state.F90:236-242
+        st["temperature_soil"] = np.full((ny, nx, cfg.nsoillays), 280.0, order="F")
+    return st
 
 
 def w_free_slip(st: dict, g: dict) -> None:

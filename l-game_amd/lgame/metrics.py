@@ -17,8 +17,9 @@ def _pieces(st: dict):
     names = CONSTITUENTS if nc == 7 else ("gas",)
     for jc in range(nc):
         yield f"rho_{names[jc]}", rho[:, :, :, jc]
-    for k in ("rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w"):
-        yield k, st[k]
+    for k in ("rhotheta_v", "theta_v_pert", "exner_pert", "wind_u", "wind_v", "wind_w", "temperature_soil"):
+        if k in st:
+            yield k, st[k]
 
 
 def field_errors(got: dict, want: dict) -> dict:
@@ -38,4 +39,6 @@ def field_errors(got: dict, want: dict) -> dict:
 
 
 def worst(errors: dict, key: str) -> float:
-    return max(v[key] for v in errors.values())
+    """The largest error of kind `key`; infinite if any is NaN (max() would drop it)."""
+    vals = [v[key] for v in errors.values()]
+    return float("inf") if any(not np.isfinite(x) for x in vals) else max(vals)

@@ -50,14 +50,15 @@ def main():
         d = np.abs(a[k] - b[k]).max()
         res[k] = {"bitwise": bool(np.array_equal(a[k], b[k])), "maxabs": float(d),
                   "moved": float(np.abs(a[k] - st[k]).max())}
-    if spec.get("diffusion") or cfg.lpbl:
+    if spec.get("diffusion") or cfg.surface:
         ny, nx, nl, nc = cfg.ny, cfg.nx, cfg.n_layers, cfg.n_constituents
         for k, shp in (("tke", (ny, nx, nl)), ("viscosity", (ny, nx, nl)), ("n_squared", (ny, nx, nl)),
                        ("mom_diff_tend_x", (ny, nx + 1, nl)), ("mom_diff_tend_y", (ny + 1, nx, nl)),
                        ("mom_diff_tend_z", (ny, nx, nl + 1)), ("temp_diff_heating", (ny, nx, nl)),
                        ("heating_diss", (ny, nx, nl)), ("mass_diff_tendency", (ny, nx, nl, nc)),
                        ("roughness_length", (ny, nx)), ("roughness_velocity", (ny, nx)),
-                       ("monin_obukhov_length", (ny, nx))):
+                       ("monin_obukhov_length", (ny, nx)), ("scalar_flux_resistance", (ny, nx)),
+                       ("power_flux_density_sens_soil", (ny, nx)), ("power_flux_density_sens_sea", (ny, nx))):
             x, y = o.get_diag(k, shp), r.get_diag(k, shp)
             res["diag:" + k] = {"bitwise": bool(np.array_equal(x, y)), "maxabs": float(np.abs(x - y).max()),
                                 "moved": float(np.abs(x).max())}

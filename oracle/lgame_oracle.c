@@ -56,6 +56,7 @@ typedef struct {
 typedef struct {
   double *rho; /* (ny,nx,nl,nc) */
   F3 rhotheta_v, theta_v_pert, exner_pert, wind_u, wind_v, wind_w;
+  F3 temperature_soil; /* (ny,nx,max(nsoillays,1)) */
 } ostate;
 
 typedef struct oracle {
@@ -65,12 +66,14 @@ typedef struct oracle {
   int n_damping_levels;
   /* diffusion switches and parameters (mo_diff_nml.F90) */
   int lmom_h, lmom_v, ltemp_h, ltemp_v, lmass_h, lmass_v, scheme_h_tke, lpbl;
+  int lprog_soil, lsens, lphase, nsoil; /* mo_surface_nml.F90 */
   double c_s, bg_shear, eff_hor_res, mva, h_prandtl, karman;
   /* grid */
   F3 z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume, slope_x, slope_y;
   F3 area_dual_x, area_dual_y, area_dual_z, z_area_dual_z, fvec_x, fvec_y, fvec_z;
   F3 ebg_u, ebg_v, ebg_w, theta_v_bg, exner_bg, gravity_m_v, gravity_potential;
   F3 dx_dual, dy_dual, layer_thickness, mva_f, land_fraction, roughness_length;
+  F3 lake_fraction, sfc_rho_c, t_conduc_soil, t_const_soil, sst, z_soil_interface, z_soil_center, z_t_const;
   double *ipw;    /* (6,ny,nx,nl) */
   double *trsk_u; /* (3,ny) */
   double *trsk_v; /* (2,ny+1) */
@@ -88,6 +91,7 @@ typedef struct oracle {
   F3 tke, n_squared, viscosity, viscosity_molecular, viscosity_dual, flux_w;
   F3 mass_coeff_h, mass_coeff_v, temp_coeff_h, temp_coeff_v;
   F3 z0, ustar, mo_len; /* diag%roughness_length, roughness_velocity, monin_obukhov_length (ny,nx) */
+  F3 sfr, pfd_soil, pfd_sea; /* diag%scalar_flux_resistance, power_flux_density_sens_soil/_sea */
   double *phase_trans_rates;  /* (ny,nx,nl,ncond+1) */
   double *mass_diff_tendency; /* (ny,nx,nl,nc)     */
   /* radiation: per-column surface data (ny,nx), outputs and switches */
@@ -1285,6 +1289,25 @@ static double psi_m(double eff, double l) {
   return -(4.7 * eff / ll);
 }
 
+/* psi_h, mo_pbl.F90:368-390 */
+static double psi_h(double eff, double l) {
+  double ll = l;
+  if (fabs(ll) < EPSILON_SECURITY) ll = EPSILON_SECURITY;
+  if (ll < 0.0) {
+    double x = pow(1.0 - 15.0 * eff / ll, 0.25);
+    return 2.0 * log((1.0 + x * x) / 2.0);
+  }
+  return -(4.0 * eff / ll);
+}
+
+/* scalar_flux_resistance, mo_pbl.F90:294-316 */
+static double scalar_flux_resistance(const oracle *o, double us, double z_agl, double z0, double l) {
+  double used = fmin2(z_agl, o->h_prandtl);
+  double r = 1.0 / (o->karman * us) * (log(used / z0) - psi_h(used, l) + log(7.0));
+  if (r < o->dtime / z_agl) r = o->dtime / z_agl;
+  return r;
+}
+
 /* momentum_flux_resistance, mo_pbl.F90:318-342 */
 static double momentum_flux_resistance(const oracle *o, double wind_h, double z_agl, double z0, double l) {
   double used = fmin2(z_agl, o->h_prandtl);
@@ -1293,8 +1316,8 @@ static double momentum_flux_resistance(const oracle *o, double wind_h, double z_
   return r;
 }
 
-/* update_sfc_turb_quantities, mo_pbl.F90:19-109 (no prognostic soil: no scalar flux resistance).
-   v_squared is the kinetic energy the last momentum advection block left. */
+/* update_sfc_turb_quantities, mo_pbl.F90:19-109.  v_squared is the kinetic energy the last
+   momentum advection block left. */
 static void update_sfc_turb_quantities(oracle *o, const ostate *s) {
   int nl = o->nl, nlev = o->nlev;
   for (int jk = 1; jk <= o->nx; ++jk)
@@ -1320,6 +1343,11 @@ static void update_sfc_turb_quantities(oracle *o, const ostate *s) {
       if (fabs(wt) < EPSILON_SECURITY) wt = EPSILON_SECURITY;
       A3(o->mo_len, ji, jk, 1) = -(th_l * (us * (us * us)) / (o->karman * GRAVITY * wt));
     }
+  if (o->lprog_soil)
+    for (int jk = 1; jk <= o->nx; ++jk)
+      for (int ji = 1; ji <= o->ny; ++ji)
+        A3(o->sfr, ji, jk, 1) = scalar_flux_resistance(o, A3(o->ustar, ji, jk, 1), A3(o->z_scalar, ji, jk, nl) - A3(o->z_w, ji, jk, nlev),
+                                                       A3(o->z0, ji, jk, 1), A3(o->mo_len, ji, jk, 1));
 }
 
 /* the PBL friction at one lowest-layer u or v point between cells a and b (mo_pbl.F90:125-160) */
@@ -1486,17 +1514,47 @@ static void thomas(const double *c, const double *d, const double *e, const doub
   for (int j = n - 2; j >= 0; --j) x[j] = rp[j] - ep[j] * x[j + 1];
 }
 
-/* three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off) */
+/* the temperature of the gas in the lowest layer, mo_column_solvers.F90:84-93 */
+static double t_gas_lowest(const oracle *o, const ostate *s, int ji, int jk) {
+  int nl = o->nl;
+  double t = (A3(o->exner_bg, ji, jk, nl) + A3(s->exner_pert, ji, jk, nl)) * (A3(o->theta_v_bg, ji, jk, nl) + A3(s->theta_v_pert, ji, jk, nl));
+  if (o->lmoist) {
+    size_t n = (size_t)o->ny * o->nx * nl, i = (size_t)(ji - 1) + (size_t)o->ny * ((size_t)(jk - 1) + (size_t)o->nx * (size_t)(nl - 1));
+    t = t / (1.0 + s->rho[i + (size_t)(o->ncond + 1) * n] / s->rho[i + (size_t)o->ncond * n] * (M_D / M_V - 1.0));
+  }
+  return t;
+}
+
+/* three_band_solver_ver, mo_column_solvers.F90:21-351, with the surface sensible heat flux
+   (:81-120) and the soil temperature rows (:229-300), periodic */
 static void three_band_solver_ver(oracle *o, const ostate *old, ostate *nw, int rk_step) {
   const ostate *nu_ = (rk_step == 1) ? old : nw;
-  int ny = o->ny, nx = o->nx, nl = o->nl, g = o->ncond + 1;
+  int ny = o->ny, nx = o->nx, nl = o->nl, g = o->ncond + 1, ns = o->nsoil;
   double dt = o->dtime;
   const double pw = 0.5;
   double *wk = o->work;
-  double *c = wk, *d = c + nl, *e = d + nl, *r = e + nl, *x = r + nl, *rho_expl = x + nl, *rt_expl = rho_expl + nl;
-  double *ep_expl = rt_expl + nl, *tp_expl = ep_expl + nl, *rio = tp_expl + nl, *rie = rio + nl, *tin = rie + nl;
-  double *al = tin + nl, *be = al + nl, *ga = be + nl, *dp = ga + nl, *eprime = dp + nl, *rprime = eprime + nl;
+  const int m = nl + ns; /* stride of the work vectors: n_layers-1+nsoillays rows and more */
+  double *c = wk, *d = c + m, *e = d + m, *r = e + m, *x = r + m, *rho_expl = x + m, *rt_expl = rho_expl + m;
+  double *ep_expl = rt_expl + m, *tp_expl = ep_expl + m, *rio = tp_expl + m, *rie = rio + m, *tin = rie + m;
+  double *al = tin + m, *be = al + m, *ga = be + m, *dp = ga + m, *eprime = dp + m, *rprime = eprime + m, *hfd = rprime + m;
   F3 trho = rho_slice(o, o->tend_rho, g);
+  if (o->lsens) {
+    int nlev = o->nlev;
+    for (int ji = 1; ji <= ny; ++ji)
+      for (int jk = 1; jk <= nx; ++jk) {
+        double t_old = t_gas_lowest(o, old, ji, jk), t_new = t_gas_lowest(o, nu_, ji, jk);
+        double lf = A3(o->land_fraction, ji, jk, 1), kf = A3(o->lake_fraction, ji, jk, 1), sst = A3(o->sst, ji, jk, 1);
+        double rn = rhog(o, nu_, ji, jk, nl), ro = rhog(o, old, ji, jk, nl), sfr = A3(o->sfr, ji, jk, 1);
+        double ps = (lf + kf) * 0.5 * C_D_V *
+                    (rn * (t_old - A3(old->temperature_soil, ji, jk, 1)) + ro * (t_new - A3(nu_->temperature_soil, ji, jk, 1))) / sfr;
+        double pe = (1.0 - lf - kf) * 0.5 * C_D_V * (rn * (t_old - sst) + ro * (t_new - sst)) / sfr;
+        A3(o->pfd_soil, ji, jk, 1) = ps;
+        A3(o->pfd_sea, ji, jk, 1) = pe;
+        A3(o->tend_rhotheta_v, ji, jk, nl) = A3(o->tend_rhotheta_v, ji, jk, nl) - A3(o->area_z, ji, jk, nlev) * (ps + pe) /
+                                                                                     ((A3(o->exner_bg, ji, jk, nl) + A3(nu_->exner_pert, ji, jk, nl)) * C_D_P) /
+                                                                                     A3(o->volume, ji, jk, nl);
+      }
+  }
   for (int ji = 1; ji <= ny; ++ji)
     for (int jk = 1; jk <= nx; ++jk) {
       for (int jl = 1; jl <= nl; ++jl) {
@@ -1553,7 +1611,39 @@ static void three_band_solver_ver(oracle *o, const ostate *old, ostate *nw, int 
                (A3(o->z_scalar, ji, jk, jl) - A3(o->z_scalar, ji, jk, jl + 1)) / (IMPL_THERMO_WEIGHT * dt * C_D_P) * 0.5 *
                    A3(old->wind_w, ji, jk, jl + 1) / (A3(o->volume, ji, jk, jl + 1) * rio[j]) * dp[j];
       }
-      thomas(c, d, e, r, x, nl - 1, eprime, rprime);
+      /* soil rows (soil_switch: lprog_soil_temp, every column since land+lake fraction >= 0) */
+      int soil = o->lprog_soil;
+      if (soil) {
+        double rc = A3(o->sfc_rho_c, ji, jk, 1), tc = A3(o->t_conduc_soil, ji, jk, 1);
+        const double *zi = o->z_soil_interface.p, *zc = o->z_soil_center.p;
+        F3 ts = old->temperature_soil;
+        for (int jl = 1; jl <= ns - 1; ++jl)
+          hfd[jl - 1] = -rc * tc * (A3(ts, ji, jk, jl) - A3(ts, ji, jk, jl + 1)) / (zc[jl - 1] - zc[jl]);
+        hfd[ns - 1] = -rc * tc * (A3(ts, ji, jk, ns) - A3(o->t_const_soil, ji, jk, 1)) / (2.0 * (zc[ns - 1] - o->z_t_const.p[0]));
+        size_t ic = (size_t)(ji - 1) + (size_t)ny * (size_t)(jk - 1);
+        r[nl - 1] = A3(ts, ji, jk, 1) +
+                    (A3(o->pfd_soil, ji, jk, 1) + 0.0 + o->sfc_sw_in[ic] - o->sfc_lw_out[ic] + 0.5 * hfd[0]) / ((zi[0] - zi[1]) * rc) * dt;
+        for (int jl = 2; jl <= ns; ++jl)
+          r[nl - 2 + jl] = A3(ts, ji, jk, jl) + 0.5 * (-hfd[jl - 2] + hfd[jl - 1]) / ((zi[jl - 1] - zi[jl]) * rc) * dt;
+        for (int jl = 1; jl <= ns; ++jl) {
+          double f = 0.5 * dt * rc * tc / ((zi[jl - 1] - zi[jl]) * rc);
+          if (jl == 1)
+            d[jl + nl - 2] = 1.0 + f * 1.0 / (zc[jl - 1] - zc[jl]);
+          else if (jl == ns)
+            d[jl + nl - 2] = 1.0 + f * 1.0 / (zc[jl - 2] - zc[jl - 1]);
+          else
+            d[jl + nl - 2] = 1.0 + f * (1.0 / (zc[jl - 2] - zc[jl - 1]) + 1.0 / (zc[jl - 1] - zc[jl]));
+        }
+        c[nl - 2] = 0.0;
+        e[nl - 2] = 0.0;
+        for (int jl = 1; jl <= ns - 1; ++jl) {
+          c[jl + nl - 2] = -0.5 * dt * rc * tc / ((zi[jl] - zi[jl + 1]) * rc) / (zc[jl - 1] - zc[jl]);
+          e[jl + nl - 2] = -0.5 * dt * rc * tc / ((zi[jl - 1] - zi[jl]) * rc) / (zc[jl - 1] - zc[jl]);
+        }
+      }
+      thomas(c, d, e, r, x, nl - 1 + soil * ns, eprime, rprime);
+      if (soil)
+        for (int jl = 1; jl <= ns; ++jl) A3(nw->temperature_soil, ji, jk, jl) = x[nl - 2 + jl];
       for (int jl = 2; jl <= nl - 1; ++jl) {
         int j = jl - 1;
         double V = A3(o->volume, ji, jk, jl);
@@ -1873,7 +1963,7 @@ static void update_rad_fluxes(oracle *o, const ostate *s, double t) {
 static void pchevi(oracle *o, ostate *old, ostate *nw, int step, int lrad_update, double t_0) {
   size_t nu = (size_t)o->ny * (o->nx + 1) * o->nl, nv = (size_t)(o->ny + 1) * o->nx * o->nl;
   temperature_diagnostics(o, old);
-  if (o->lpbl) update_sfc_turb_quantities(o, old);
+  if (o->lsens || o->lphase || o->lpbl) update_sfc_turb_quantities(o, old);
   if (lrad_update && o->lrad) update_rad_fluxes(o, old, t_0);
   for (int rk = 1; rk <= 2; ++rk) {
     if (rk == 1) manage_p_grad(o, old, step == 0);
@@ -1900,6 +1990,7 @@ static void state_alloc(oracle *o, ostate *s) {
   s->wind_u = f3alloc_o(o, o->ny, o->nx + 1, o->nl);
   s->wind_v = f3alloc_o(o, o->ny + 1, o->nx, o->nl);
   s->wind_w = f3alloc_o(o, o->ny, o->nx, o->nlev);
+  s->temperature_soil = f3alloc_o(o, o->ny, o->nx, o->nsoil);
 }
 
 static char g_err[256];
@@ -1913,8 +2004,9 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
     return NULL;
   }
   if (!cf->lperiodic || !cf->luse_bg_state || ((cf->ltemp_diff_h || cf->lmass_diff_h) && !cf->lmom_diff_h) ||
-      (cf->diff_coeff_scheme_h != 0 && cf->diff_coeff_scheme_h != 1) || cf->lsfc_sensible_heat_flux ||
-      cf->lsfc_phase_trans || cf->lprog_soil_temp || (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
+      (cf->diff_coeff_scheme_h != 0 && cf->diff_coeff_scheme_h != 1) || (cf->lpbl && !cf->lmom_diff_h) ||
+      (cf->lsfc_sensible_heat_flux && !cf->lprog_soil_temp) || (cf->lprog_soil_temp && dm->nsoillays < 1) ||
+      (cf->theta_adv_order != 2 && cf->theta_adv_order != 3)) {
     snprintf(g_err, sizeof g_err, "unsupported configuration");
     return NULL;
   }
@@ -1956,6 +2048,10 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   o->bg_shear = cf->bg_shear;
   o->eff_hor_res = sqrt(cf->dx * cf->dy); /* mo_run_nml.F90:93 */
   o->lpbl = cf->lpbl != 0;
+  o->lprog_soil = cf->lprog_soil_temp != 0;
+  o->lsens = cf->lsfc_sensible_heat_flux != 0;
+  o->lphase = cf->lsfc_phase_trans != 0;
+  o->nsoil = dm->nsoillays > 0 ? dm->nsoillays : 1;
   o->h_prandtl = cf->h_prandtl;
   o->karman = cf->karman;
 #define G3(f, a, b, c) o->f = f3alloc_o(o, a, b, c)
@@ -1992,6 +2088,17 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
   G3(mva_f, 1, 1, 1);
   G3(land_fraction, ny, nx, 1);
   G3(roughness_length, ny, nx, 1);
+  G3(lake_fraction, ny, nx, 1);
+  G3(sfc_rho_c, ny, nx, 1);
+  G3(t_conduc_soil, ny, nx, 1);
+  G3(t_const_soil, ny, nx, 1);
+  G3(sst, ny, nx, 1);
+  G3(z_soil_interface, o->nsoil + 1, 1, 1);
+  G3(z_soil_center, o->nsoil, 1, 1);
+  G3(z_t_const, 1, 1, 1);
+  G3(sfr, ny, nx, 1);
+  G3(pfd_soil, ny, nx, 1);
+  G3(pfd_sea, ny, nx, 1);
   G3(z0, ny, nx, 1);
   G3(ustar, ny, nx, 1);
   G3(mo_len, ny, nx, 1);
@@ -2060,7 +2167,7 @@ oracle *orc_create(const lgame_dims *dm, const lgame_cfg *cf) {
 #undef G3
   o->phase_trans_rates = oalloc(o, (size_t)ny * nx * nl * (o->ncond + 1));
   o->mass_diff_tendency = oalloc(o, (size_t)ny * nx * nl * o->nc);
-  o->work = oalloc(o, (size_t)24 * (nl + 2));
+  o->work = oalloc(o, (size_t)24 * (nl + o->nsoil + 2));
   o->lrad = cf->lrad != 0;
   o->rad_mode = cf->rad_mode;
   o->dtime_rad = cf->dtime_rad;
@@ -2079,6 +2186,7 @@ static F3 *grid_field(oracle *o, const char *name) {
   GF(z_scalar) GF(z_u) GF(z_v) GF(z_w) GF(dx) GF(dy) GF(dz) GF(area_x) GF(area_y) GF(area_z) GF(volume) GF(slope_x) GF(slope_y)
   GF(area_dual_x) GF(area_dual_y) GF(area_dual_z) GF(z_area_dual_z) GF(fvec_x) GF(fvec_y) GF(fvec_z) GF(theta_v_bg) GF(exner_bg)
   GF(gravity_m_v) GF(gravity_potential) GF(dx_dual) GF(dy_dual) GF(layer_thickness) GF(land_fraction) GF(roughness_length)
+  GF(lake_fraction) GF(sfc_rho_c) GF(t_conduc_soil) GF(t_const_soil) GF(sst) GF(z_soil_interface) GF(z_soil_center) GF(z_t_const)
   if (!strcmp(name, "mean_velocity_area")) return &o->mva_f;
   if (!strcmp(name, "exner_bg_grad_u")) return &o->ebg_u;
   if (!strcmp(name, "exner_bg_grad_v")) return &o->ebg_v;
@@ -2128,7 +2236,7 @@ static double *state_field(oracle *o, int slot, const char *name, size_t *count)
     *count = (size_t)s->f.n1 * s->f.n2 * s->f.n3;                             \
     return s->f.p;                                                            \
   }
-  SF(rhotheta_v) SF(theta_v_pert) SF(exner_pert) SF(wind_u) SF(wind_v) SF(wind_w)
+  SF(rhotheta_v) SF(theta_v_pert) SF(exner_pert) SF(wind_u) SF(wind_v) SF(wind_w) SF(temperature_soil)
 #undef SF
   if (!strcmp(name, "rho")) {
     *count = (size_t)o->ny * o->nx * o->nl * o->nc;
@@ -2175,6 +2283,9 @@ int orc_get_diag(oracle *o, const char *name, double *dst) {
   if (!strcmp(name, "roughness_length")) f = &o->z0;
   if (!strcmp(name, "roughness_velocity")) f = &o->ustar;
   if (!strcmp(name, "monin_obukhov_length")) f = &o->mo_len;
+  if (!strcmp(name, "scalar_flux_resistance")) f = &o->sfr;
+  if (!strcmp(name, "power_flux_density_sens_soil")) f = &o->pfd_soil;
+  if (!strcmp(name, "power_flux_density_sens_sea")) f = &o->pfd_sea;
   if (!strcmp(name, "mass_diff_tendency")) {
     memcpy(dst, o->mass_diff_tendency, sizeof(double) * (size_t)o->ny * o->nx * o->nl * o->nc);
     return 0;
@@ -2249,6 +2360,7 @@ void orc_copy_slot(oracle *o, int from, int to) {
   memcpy(b->wind_u.p, a->wind_u.p, (size_t)o->ny * (o->nx + 1) * o->nl * sizeof(double));
   memcpy(b->wind_v.p, a->wind_v.p, (size_t)(o->ny + 1) * o->nx * o->nl * sizeof(double));
   memcpy(b->wind_w.p, a->wind_w.p, (size_t)o->ny * o->nx * o->nlev * sizeof(double));
+  memcpy(b->temperature_soil.p, a->temperature_soil.p, (size_t)o->ny * o->nx * o->nsoil * sizeof(double));
 }
 
 /*

@@ -23,13 +23,16 @@ REF_GRID_IDS = {name: i + 1 for i, name in enumerate([
     "z_area_dual_z", "fvec_x", "fvec_y", "fvec_z", "trsk_weights_u", "trsk_weights_v",
     "exner_bg_grad_u", "exner_bg_grad_v", "exner_bg_grad_w", "theta_v_bg", "exner_bg",
     "gravity_m_v", "gravity_potential", "dx_dual", "dy_dual", "layer_thickness", "land_fraction",
-    "roughness_length"])}
+    "roughness_length", "lake_fraction", "sfc_rho_c", "t_conduc_soil", "t_const_soil", "sst",
+    "z_soil_interface", "z_soil_center", "z_t_const"])}
 REF_STATE_IDS = {"rho": 1, "rhotheta_v": 2, "theta_v_pert": 3, "exner_pert": 4, "wind_u": 5,
-                 "wind_v": 6, "wind_w": 7}
+                 "wind_v": 6, "wind_w": 7, "temperature_soil": 8}
 REF_DIAG_IDS = {"temperature": 1, "heating_diss": 2, "pot_vort_tend_x": 3, "pot_vort_tend_z": 4,
                 "condensates_sediment_heat": 5, "tke": 6, "mom_diff_tend_x": 7, "mom_diff_tend_y": 8,
                 "mom_diff_tend_z": 9, "temp_diff_heating": 10, "mass_diff_tendency": 11, "viscosity": 12,
-                "n_squared": 13, "roughness_length": 14, "roughness_velocity": 15, "monin_obukhov_length": 16}
+                "n_squared": 13, "roughness_length": 14, "roughness_velocity": 15, "monin_obukhov_length": 16,
+                "scalar_flux_resistance": 17, "power_flux_density_sens_soil": 18,
+                "power_flux_density_sens_sea": 19}
 
 
 def _ptr(a):
@@ -106,11 +109,11 @@ class CpuOracle:
                 raise KeyError(k)
 
     def get_state(self, slot: int) -> dict:
-        from lgame.config import STATE_FIELDS, state_shape
+        from lgame.config import state_shape
         c = self.cfg
         out = {}
-        for k in STATE_FIELDS:
-            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents), order="F")
+        for k in c.state_fields:
+            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents, c.nsoillays), order="F")
             self.lib().orc_get_state(self.h, slot, k.encode(), _ptr(a))
             out[k] = a
         return out
@@ -207,12 +210,13 @@ class RefHarness:
                          int(cfg.lmoist), int(cfg.lklemp), int(cfg.lperiodic), int(cfg.lfreeslip),
                          int(cfg.lmom_diff_h), int(cfg.lmom_diff_v), int(cfg.ltemp_diff_h), int(cfg.ltemp_diff_v),
                          int(cfg.lmass_diff_h), int(cfg.lmass_diff_v), 0 if cfg.diff_coeff_scheme_h == "smag" else 1,
-                         int(cfg.lpbl)], dtype=np.int32)
+                         int(cfg.lpbl), int(cfg.lprog_soil_temp), int(cfg.lsfc_sensible_heat_flux),
+                         int(cfg.lsfc_phase_trans)], dtype=np.int32)
         rcfg = np.array([cfg.dt, cfg.toa, cfg.dx, cfg.dy, cfg.klemp_damp_max, cfg.klemp_begin_rel, cfg.c_s,
                          cfg.bg_shear, grid["mean_velocity_area"], cfg.h_prandtl, cfg.karman])
         lib.ref_init(_ptr(idims), _ptr(icfg), _ptr(rcfg))
-        for name in GRID_FIELDS:
-            a = _fortran(grid[name])
+        for name in list(GRID_FIELDS) + ["z_t_const"]:
+            a = _fortran(np.atleast_1d(np.asarray(grid[name], dtype=np.float64)))
             lib.ref_set_grid(REF_GRID_IDS[name], _ptr(a))
         lib.ref_finish_grid()
 
@@ -226,11 +230,11 @@ class RefHarness:
         self.lib.ref_run(first, nsteps)
 
     def get_state(self, slot: int) -> dict:
-        from lgame.config import STATE_FIELDS, state_shape
+        from lgame.config import state_shape
         c = self.cfg
         out = {}
-        for k in STATE_FIELDS:
-            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents), order="F")
+        for k in c.state_fields:
+            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents, c.nsoillays), order="F")
             self.lib.ref_state_io(slot, REF_STATE_IDS[k], 1, _ptr(a))
             out[k] = a
         return out

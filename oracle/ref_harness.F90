@@ -73,10 +73,10 @@ contains
   ! idims: ny,nx,n_layers,n_oro_layers,n_constituents,n_condensed_constituents,nsoillays
   ! icfg:  theta_adv_order,luse_bg_state,llinear,lcorio,lmoist,lklemp,lperiodic,lfreeslip,
   !        lmom_diff_h,lmom_diff_v,ltemp_diff_h,ltemp_diff_v,lmass_diff_h,lmass_diff_v,
-  !        diff_coeff_scheme_h (0 smag, 1 tke),lpbl
+  !        diff_coeff_scheme_h (0 smag, 1 tke),lpbl,lprog_soil_temp,lsfc_sensible_heat_flux,lsfc_phase_trans
   ! rcfg:  dtime,toa,dx,dy,klemp_damp_max,klemp_begin_rel,c_s,bg_shear,mean_velocity_area,h_prandtl,karman
   subroutine ref_init(idims,icfg,rcfg) bind(C,name="ref_init")
-    integer(c_int), intent(in) :: idims(7),icfg(16)
+    integer(c_int), intent(in) :: idims(7),icfg(19)
     real(c_double), intent(in) :: rcfg(11)
 
     ny = idims(1); nx = idims(2); n_layers = idims(3); n_oro_layers = idims(4)
@@ -102,7 +102,8 @@ contains
     diff_coeff_scheme_v = "tke"
     c_s = rcfg(7); bg_shear = rcfg(8); h_prandtl = rcfg(10); karman = rcfg(11)
     eff_hor_res = sqrt(dx*dy)
-    lprog_soil_temp = .false.; lsfc_sensible_heat_flux = .false.; lsfc_phase_trans = .false.; lpbl = icfg(16)/=0
+    lpbl = icfg(16)/=0; lprog_soil_temp = icfg(17)/=0; lsfc_sensible_heat_flux = icfg(18)/=0
+    lsfc_phase_trans = icfg(19)/=0
 
     ! grid (sizes as in control.F90:101-157)
     allocate(grid%z_scalar(ny,nx,n_layers)); allocate(grid%z_u(ny,nx+1,n_layers))
@@ -124,6 +125,12 @@ contains
     grid%mean_velocity_area = rcfg(9)
     allocate(grid%land_fraction(ny,nx)); grid%land_fraction = 0._wp
     allocate(grid%roughness_length(ny,nx)); grid%roughness_length = 0._wp
+    allocate(grid%sfc_rho_c(ny,nx)); grid%sfc_rho_c = 0._wp
+    allocate(grid%t_conduc_soil(ny,nx)); grid%t_conduc_soil = 0._wp
+    allocate(grid%t_const_soil(ny,nx)); grid%t_const_soil = 0._wp
+    allocate(grid%z_soil_interface(max(nsoillays,1)+1)); grid%z_soil_interface = 0._wp
+    allocate(grid%z_soil_center(max(nsoillays,1))); grid%z_soil_center = 0._wp
+    grid%z_t_const = 0._wp
     allocate(grid%lake_fraction(ny,nx)); grid%lake_fraction = 0._wp
 
     call alloc_state(st1)
@@ -198,6 +205,13 @@ contains
     allocate(diag%roughness_velocity(ny,nx)); diag%roughness_velocity = 0._wp
     allocate(diag%monin_obukhov_length(ny,nx)); diag%monin_obukhov_length = 0._wp
     allocate(diag%scalar_flux_resistance(ny,nx)); diag%scalar_flux_resistance = 0._wp
+    allocate(diag%sst(ny,nx)); diag%sst = 0._wp
+    allocate(diag%power_flux_density_sens_soil(ny,nx)); diag%power_flux_density_sens_soil = 0._wp
+    allocate(diag%power_flux_density_sens_sea(ny,nx)); diag%power_flux_density_sens_sea = 0._wp
+    allocate(diag%power_flux_density_lat_lake(ny,nx)); diag%power_flux_density_lat_lake = 0._wp
+    allocate(diag%power_flux_density_lat_sea(ny,nx)); diag%power_flux_density_lat_sea = 0._wp
+    allocate(diag%sfc_sw_in(ny,nx)); diag%sfc_sw_in = 0._wp
+    allocate(diag%sfc_lw_out(ny,nx)); diag%sfc_lw_out = 0._wp
   end subroutine ref_init
 
   ! grid field ids (Python side keeps the same table)
@@ -246,6 +260,14 @@ contains
       case(34); n = size(grid%land_fraction); call c_f_pointer(src,p,[n]); grid%land_fraction = reshape(p,shape(grid%land_fraction))
       case(35); n = size(grid%roughness_length); call c_f_pointer(src,p,[n])
                 grid%roughness_length = reshape(p,shape(grid%roughness_length))
+      case(36); n = size(grid%lake_fraction); call c_f_pointer(src,p,[n]); grid%lake_fraction = reshape(p,shape(grid%lake_fraction))
+      case(37); n = size(grid%sfc_rho_c); call c_f_pointer(src,p,[n]); grid%sfc_rho_c = reshape(p,shape(grid%sfc_rho_c))
+      case(38); n = size(grid%t_conduc_soil); call c_f_pointer(src,p,[n]); grid%t_conduc_soil = reshape(p,shape(grid%t_conduc_soil))
+      case(39); n = size(grid%t_const_soil); call c_f_pointer(src,p,[n]); grid%t_const_soil = reshape(p,shape(grid%t_const_soil))
+      case(40); n = size(diag%sst); call c_f_pointer(src,p,[n]); diag%sst = reshape(p,shape(diag%sst))
+      case(41); n = size(grid%z_soil_interface); call c_f_pointer(src,p,[n]); grid%z_soil_interface = p
+      case(42); n = size(grid%z_soil_center); call c_f_pointer(src,p,[n]); grid%z_soil_center = p
+      case(43); call c_f_pointer(src,p,[1]); grid%z_t_const = p(1)
     end select
   end subroutine ref_set_grid
 
@@ -272,7 +294,8 @@ contains
     endif
   end function sel
 
-  ! state field ids: 1 rho, 2 rhotheta_v, 3 theta_v_pert, 4 exner_pert, 5 wind_u, 6 wind_v, 7 wind_w
+  ! state field ids: 1 rho, 2 rhotheta_v, 3 theta_v_pert, 4 exner_pert, 5 wind_u, 6 wind_v, 7 wind_w,
+  ! 8 temperature_soil
   subroutine ref_state_io(slot,id,dir,buf) bind(C,name="ref_state_io")
     integer(c_int), value, intent(in) :: slot,id,dir
     type(c_ptr),    value, intent(in) :: buf
@@ -301,6 +324,13 @@ contains
       case(7)
         call c_f_pointer(buf,p,[size(s%wind_w)])
         if (dir==0) then; s%wind_w = reshape(p,shape(s%wind_w)); else; p = reshape(s%wind_w,[size(s%wind_w)]); endif
+      case(8)
+        call c_f_pointer(buf,p,[size(s%temperature_soil)])
+        if (dir==0) then
+          s%temperature_soil = reshape(p,shape(s%temperature_soil))
+        else
+          p = reshape(s%temperature_soil,[size(s%temperature_soil)])
+        endif
     end select
   end subroutine ref_state_io
 
@@ -335,6 +365,12 @@ contains
                 p = reshape(diag%roughness_velocity,[size(diag%roughness_velocity)])
       case(16); call c_f_pointer(buf,p,[size(diag%monin_obukhov_length)])
                 p = reshape(diag%monin_obukhov_length,[size(diag%monin_obukhov_length)])
+      case(17); call c_f_pointer(buf,p,[size(diag%scalar_flux_resistance)])
+                p = reshape(diag%scalar_flux_resistance,[size(diag%scalar_flux_resistance)])
+      case(18); call c_f_pointer(buf,p,[size(diag%power_flux_density_sens_soil)])
+                p = reshape(diag%power_flux_density_sens_soil,[size(diag%power_flux_density_sens_soil)])
+      case(19); call c_f_pointer(buf,p,[size(diag%power_flux_density_sens_sea)])
+                p = reshape(diag%power_flux_density_sens_sea,[size(diag%power_flux_density_sens_sea)])
     end select
   end subroutine ref_get_diag
 

@@ -34,6 +34,11 @@ CASES = {
     # with the PBL friction and the surface turbulence quantities as well
     "tiny_moist_o3_freeslip_diff_pbl": ({"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True},
                                          "cloud_frac": 0.3, "diffusion": True, "diff_kw": {"lpbl": True}}, 6),
+    # and the prognostic soil with the surface sensible heat flux
+    "tiny_moist_o3_freeslip_diff_pbl_soil": ({"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True},
+                                              "cloud_frac": 0.3, "diffusion": True,
+                                              "diff_kw": {"lpbl": True, "lprog_soil_temp": True,
+                                                          "lsfc_sensible_heat_flux": True}}, 6),
 }
 # diagnostics stored with the diffusion cases (t_diag components carried between steps)
 DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_z", "heating_diss")

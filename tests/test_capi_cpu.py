@@ -54,6 +54,11 @@ def _init(mutate_dims=None, mutate_cfg=None):
     (None, lambda c: setattr(c, "lmom_diff_h", 1), "mean_velocity_area"),
     (None, lambda c: setattr(c, "lmoist", 1), "constituent"),
     (None, lambda c: setattr(c, "dtime", 0.0), "dtime"),
+    (None, lambda c: setattr(c, "lpbl", 1), "lpbl needs lmom_diff_h"),
+    (None, lambda c: setattr(c, "lsfc_sensible_heat_flux", 1), "without the prognostic soil"),
+    (lambda d: setattr(d, "nsoillays", 1), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
+    (lambda d: setattr(d, "nsoillays", 17), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
+    (None, lambda c: (setattr(c, "lprog_soil_temp", 1), setattr(c, "lrad", 1)), "radiation together with the prognostic soil"),
 ])
 def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
     rc, err, h = _init(mutate_dims, mutate_cfg)
@@ -63,4 +68,4 @@ def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
 
 
 def test_abi_version():
-    assert load_library().lgame_abi_version() == 2
+    assert load_library().lgame_abi_version() == 3

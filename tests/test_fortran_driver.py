@@ -14,6 +14,11 @@ from lgame.config import GRID_FIELDS, STATE_FIELDS, state_shape
 from pyoracle import CpuOracle
 
 
+# the grid arrays the demo reads: the dynamical core's 30, in the order of include/lgame_gpu.h
+DEMO_GRID_FIELDS = tuple(GRID_FIELDS)[:30]
+assert DEMO_GRID_FIELDS[-1] == "gravity_potential"
+
+
 def write_case(path, cfg, grid, st):
     hdr = np.array([cfg.ny, cfg.nx, cfg.n_layers, cfg.n_oro_layers, cfg.n_constituents, cfg.n_condensed_constituents,
                     cfg.nsoillays, cfg.theta_adv_order, int(cfg.lcorio), int(cfg.lmoist), int(cfg.lklemp),
@@ -22,7 +27,7 @@ def write_case(path, cfg, grid, st):
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
         f.write(rcfg.tobytes())
-        for k in GRID_FIELDS:
+        for k in DEMO_GRID_FIELDS:
             f.write(np.asfortranarray(grid[k]).tobytes(order="F"))
         for k in STATE_FIELDS:
             f.write(np.asfortranarray(st[k]).tobytes(order="F"))

@@ -31,6 +31,8 @@ CASES = {
                                     10, 0.3),
     "dry3d_33x33x20_diff": (dataclasses.replace(dry3d_config(n=33, n_layers=20)).set_diffusion(), 20, 0.0),
     "dry3d_33x33x20_diff_pbl": (dataclasses.replace(dry3d_config(n=33, n_layers=20), lpbl=True).set_diffusion(), 20, 0.0),
+    "dry3d_33x33x20_diff_pbl_soil": (dataclasses.replace(dry3d_config(n=33, n_layers=20), lpbl=True, lprog_soil_temp=True,
+                                                         lsfc_sensible_heat_flux=True).set_diffusion(), 20, 0.0),
 }
 
 

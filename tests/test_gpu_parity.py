@@ -62,9 +62,20 @@ CASES = {
     "dry3d_33x33x20_diff": (diff(dry3d_config(n=33, n_layers=20)), 8, 0.0),
     "dry3d_66x40x18_even_diff": (diff(dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66)), 6, 0.0),
     # the PBL friction (with the device's log, pow and atan: agreement to rounding)
-    "tiny_dry_pbl": (dataclasses.replace(tiny_config(), lpbl=True), 5, 0.0),
+    "tiny_dry_pbl_mom_h": (dataclasses.replace(tiny_config(), lpbl=True, lmom_diff_h=True), 5, 0.0),
     "tiny_moist_o3_freeslip_diff_pbl": (diff(tiny_config(moist=True, order=3, freeslip=True), lpbl=True), 6, 0.3),
     "dry3d_33x33x20_diff_pbl": (diff(dry3d_config(n=33, n_layers=20), lpbl=True), 8, 0.0),
+    # the prognostic soil: conduction alone, with the sensible heat flux (whose scalar flux
+    # resistance uses the device's log and pow), with the PBL and all diffusion, 7 soil layers
+    "tiny_dry_soil": (dataclasses.replace(tiny_config(), lprog_soil_temp=True), 5, 0.0),
+    "tiny_moist_soil_sens": (dataclasses.replace(tiny_config(moist=True), lprog_soil_temp=True,
+                                                 lsfc_sensible_heat_flux=True), 5, 0.3),
+    "tiny_moist_o3_freeslip_diff_pbl_soil": (diff(tiny_config(moist=True, order=3, freeslip=True), lpbl=True,
+                                                  lprog_soil_temp=True, lsfc_sensible_heat_flux=True,
+                                                  lsfc_phase_trans=True), 6, 0.3),
+    "dry3d_33x33x20_diff_pbl_soil7": (diff(dry3d_config(n=33, n_layers=20, nsoillays=7), lpbl=True,
+                                           lprog_soil_temp=True, lsfc_sensible_heat_flux=True), 8, 0.0),
+    "tiny_dry_phase_trans": (dataclasses.replace(tiny_config(), lsfc_phase_trans=True), 4, 0.0),
 }
 
 
@@ -77,13 +88,15 @@ def test_gpu_matches_oracle(name):
     print(f"{name}: max relative deviation {rel:.3e}, bitwise={bitwise}")
     assert rel <= TOL
     assert np.abs(got["wind_w"] - st["wind_w"]).max() > 0.0
+    if "temperature_soil" in want:
+        assert np.abs(got["temperature_soil"] - st["temperature_soil"]).max() > 0.0
 
 
 DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_x", "mom_diff_tend_z", "temp_diff_heating", "heating_diss")
 
 
 @pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "dry3d_33x33x20_diff",
-                                  "dry3d_33x33x20_diff_pbl"])
+                                  "dry3d_33x33x20_diff_pbl", "dry3d_33x33x20_diff_pbl_soil7"])
 def test_gpu_diffusion_diagnostics_match_oracle(name):
     """The carried t_diag fields of the diffusion and the PBL (TKE, viscosity, momentum diffusion
     tendencies, heating rates, roughness length, Monin-Obukhov length) after N steps, field by
@@ -97,7 +110,10 @@ def test_gpu_diffusion_diagnostics_match_oracle(name):
     o = CpuOracle(cfg, grid)
     o.start(st)
     o.run(0, n)
-    for k in DIFF_DIAGS + (("roughness_length", "monin_obukhov_length") if cfg.lpbl else ()):
+    surface = ("roughness_length", "monin_obukhov_length") if cfg.lpbl else ()
+    if cfg.lsfc_sensible_heat_flux:
+        surface += ("scalar_flux_resistance", "power_flux_density_sens_soil", "power_flux_density_sens_sea")
+    for k in DIFF_DIAGS + surface:
         a = g.diag(k)
         b = o.get_diag(k, a.shape)
         scale = max(float(np.abs(b).max()), 1e-300)

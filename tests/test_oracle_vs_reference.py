@@ -33,11 +33,23 @@ CASES = [
     {"kind": "tiny", "kw": {"plane": True, "corio": False}, "steps": 4, "diffusion": True,
      "diff_kw": {"lmom_diff_h": False, "ltemp_diff_h": False, "lmass_diff_h": False}},
     {"kind": "schaer", "kw": {"nx": 41, "n_layers": 20}, "steps": 12, "diffusion": True},
-    # the PBL friction with the surface turbulence quantities, alone and with diffusion
-    {"kind": "tiny", "kw": {}, "steps": 5, "diff_kw": {"lpbl": True}},
+    # the PBL friction with the surface turbulence quantities, with horizontal momentum diffusion
+    # only (the reference requires it with lpbl) and with all diffusion switches
+    {"kind": "tiny", "kw": {}, "steps": 5, "diff_kw": {"lpbl": True, "lmom_diff_h": True}},
     {"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "steps": 6, "cloud_frac": 0.3,
      "diffusion": True, "diff_kw": {"lpbl": True}},
     {"kind": "dry3d", "kw": {"n": 17, "n_layers": 12}, "steps": 8, "diffusion": True, "diff_kw": {"lpbl": True}},
+    # the prognostic soil: heat conduction alone, with the surface sensible heat flux, and with the
+    # PBL and all diffusion; the surface phase-transition switch (surface turbulence quantities)
+    {"kind": "tiny", "kw": {}, "steps": 5, "diff_kw": {"lprog_soil_temp": True}},
+    {"kind": "tiny", "kw": {"moist": True}, "steps": 5, "cloud_frac": 0.3,
+     "diff_kw": {"lprog_soil_temp": True, "lsfc_sensible_heat_flux": True}},
+    {"kind": "tiny", "kw": {"moist": True, "order": 3, "freeslip": True}, "steps": 6, "cloud_frac": 0.3,
+     "diffusion": True, "diff_kw": {"lpbl": True, "lprog_soil_temp": True, "lsfc_sensible_heat_flux": True,
+                                    "lsfc_phase_trans": True}},
+    {"kind": "dry3d", "kw": {"n": 17, "n_layers": 12, "nsoillays": 7}, "steps": 8, "diffusion": True,
+     "diff_kw": {"lpbl": True, "lprog_soil_temp": True, "lsfc_sensible_heat_flux": True}},
+    {"kind": "tiny", "kw": {}, "steps": 4, "diff_kw": {"lsfc_phase_trans": True}},
 ]
 
 
