@@ -1,7 +1,7 @@
 """Benchmark of the MI355X HEVI predictor-corrector step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--form both|mixed|fp64]
-                    [--diffusion] [--pbl]
+                    [--diffusion] [--pbl] [--physics]
 
 Runs the step on one GPU with the state resident in HBM, times exactly K steps between two
 stream synchronizations, and prints one JSON line (last line of stdout).  Progress goes to stderr.
@@ -16,7 +16,9 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
 Block shapes of the stencil kernels are chosen by lgame_autotune before the timed steps
 (--no-autotune keeps the default 64x4).  --diffusion turns on the reference's default diffusion
 (momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients); --pbl the
-PBL friction of the lowest-layer wind with the surface turbulence quantities.
+PBL friction of the lowest-layer wind with the surface turbulence quantities; --physics all of
+the reference's default namelist (diffusion, PBL, 5-layer prognostic soil, surface sensible heat
+flux).
 
 Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
 and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
@@ -80,6 +82,7 @@ def describe(cfg):
     rad = f", radiation every {cfg.dtime_rad:g} s" if cfg.lrad else ", radiation off"
     diff = (f", diffusion on ({cfg.diff_coeff_scheme_h} horizontal, TKE vertical)" if cfg.diffusion else "")
     diff += ", PBL friction on" if cfg.lpbl else ""
+    diff += f", prognostic soil ({cfg.nsoillays} layers) with the surface sensible heat flux" if cfg.lprog_soil_temp else ""
     return (f"{kind} {cfg.ny}x{cfg.nx}x{cfg.n_layers}, dx {cfg.dx:g} m, dt {cfg.dt:g} s, "
             f"theta order {cfg.theta_adv_order}{rad}{diff}")
 
@@ -312,6 +315,9 @@ def main():
                     help="turn on the six diffusion switches (the reference's default namelist)")
     ap.add_argument("--pbl", action="store_true",
                     help="turn on the PBL friction and surface turbulence quantities (lpbl)")
+    ap.add_argument("--physics", action="store_true",
+                    help="the reference's default namelist: the six diffusion switches, lpbl, the prognostic "
+                         "soil (5 layers), the surface sensible heat flux and lsfc_phase_trans")
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"],
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
@@ -341,6 +347,11 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     cfg, sample_cfg = workload(args.workload)
+    if args.physics:
+        args.diffusion = args.pbl = True
+        surface = {"lprog_soil_temp": True, "lsfc_sensible_heat_flux": True, "lsfc_phase_trans": True, "nsoillays": 5}
+        cfg = dataclasses.replace(cfg, **surface)
+        sample_cfg = dataclasses.replace(sample_cfg, **surface)
     if args.diffusion:
         cfg = dataclasses.replace(cfg).set_diffusion(True)
         sample_cfg = dataclasses.replace(sample_cfg).set_diffusion(True)
@@ -399,6 +410,7 @@ def main():
             "form": head,
             "diffusion": cfg.diffusion,
             "pbl": cfg.lpbl,
+            "soil": cfg.lprog_soil_temp,
             "grid_points": cells,
             "nonfinite_after_run": bad,
             "radiation_updates_timed": (1 + (args.steps - 1) // every) if every else 0,

@@ -445,6 +445,10 @@ int rad_alloc(lgame_handle *h) {
   HIPCHK(h, hipMemset(h->rad_counts, 0, 2 * sizeof(unsigned long long)));
   HIPCHK(h, hipMalloc(&h->rad_sq, sizeof(double)));
   HIPCHK(h, hipMemset(h->rad_sq, 0, sizeof(double)));
+  if (h->c.C.lsoil && h->cfg.lrad) {  // the soil reads the surface fluxes of the radiation updates
+    h->c.C.rad_sw_in = h->cf.C.rad_sw_in = h->rad_mem + 4 * ncol;
+    h->c.C.rad_lw_out = h->cf.C.rad_lw_out = h->rad_mem + 5 * ncol;
+  }
   return 0;
 }
 
@@ -530,8 +534,6 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   if (f.lprog_soil_temp && (d.nsoillays < 2 || d.nsoillays > MAX_SOIL || d.nsoillays > d.n_layers + 1))
     return fail(nullptr, LGAME_EINVAL, "with the prognostic soil, nsoillays must be in 2..%d and at most n_layers+1", MAX_SOIL);
   if (f.lprog_soil_temp && d.n_layers < 4) return fail(nullptr, LGAME_EINVAL, "the prognostic soil needs n_layers >= 4");
-  if (f.lprog_soil_temp && f.lrad)
-    return fail(nullptr, LGAME_EINVAL, "unsupported: radiation together with the prognostic soil (surface temperature and fluxes)");
   // the surface turbulence quantities (update_sfc_turb_quantities, mo_manage_pchevi.F90:52-54)
   const bool sfc_turb = f.lpbl || f.lsfc_sensible_heat_flux || f.lsfc_phase_trans;
   // the diffusion work arrays (temperature and mass diffusion need lmom_diff_h; the PBL friction

@@ -135,6 +135,9 @@ struct Cfg {
   // whether the surface turbulence quantities are updated (lpbl, the sensible heat flux or the
   // surface phase transitions)
   int lsoil, lsens, lsfc_turb, nsoil;
+  // the surface shortwave input and longwave output of the last radiation update (device column
+  // order (ji-1)*nx + (jk-1)), read by the soil; null without radiation
+  const double *rad_sw_in, *rad_lw_out;
 };
 
 template <class R>

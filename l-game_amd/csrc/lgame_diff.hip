@@ -540,9 +540,8 @@ __global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const Sta
 // (:202-204, :229-300, :331-336).  In the reference's combined system the lower and upper
 // diagonals vanish at row n_layers-1 (c_vector(n_layers-1) = e_vector(n_layers-1) = 0), so the
 // elimination of the soil rows neither reads nor changes anything of the atmospheric rows, and
-// eliminating the soil rows on their own, as here, gives the same values.  Radiation is not
-// coupled to the soil here (lgame_init rejects the pair), so the short- and longwave surface
-// terms are zero.
+// eliminating the soil rows on their own, as here, gives the same values.  The short- and
+// longwave surface terms are those of the last radiation update, zero without radiation.
 // ---------------------------------------------------------------------------------------------
 
 // the fields of a state that the sensible heat flux reads
@@ -594,7 +593,12 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
   const double *zi = g.soil_z, *zc = zi + ns + 1;
   const double z_t_const = zc[ns];
   const double rc = g.soil[i1 + dl], tc = g.soil[i1 + 2 * dl], tcs = g.soil[i1 + 3 * dl];
-  const double sw_in = 0.0, lw_out = 0.0, lat_lake = 0.0;  // no radiation here; no latent heat (phase transitions off)
+  // diag%sfc_sw_in and sfc_lw_out of the last radiation update (zero without radiation); the
+  // latent heat flux of lakes is zero (the surface phase transitions never run in the reference)
+  const int col = (ji - 1) * G.nx + (jk - 1);
+  const double sw_in = c.C.rad_sw_in ? c.C.rad_sw_in[col] : 0.0;
+  const double lw_out = c.C.rad_lw_out ? c.C.rad_lw_out[col] : 0.0;
+  const double lat_lake = 0.0;
   // rows (:229-288) and the forward sweep of thomas_algorithm (:625-634), one row at a time: the
   // explicit heat flux densities hfd(j) of the layer above and of this layer, and the soil
   // temperatures of this layer and the one below, roll along

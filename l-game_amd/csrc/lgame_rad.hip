@@ -108,7 +108,13 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
   }
   const double mu0 = coszenith(r.lat[col], r.lon[col], t_rad);
   const double albedo = r.albedo[col];
-  const double t_sfc = rad_clip(r.tsfc[col]);
+  double t_sfc_in = r.tsfc[col];
+  if (c.C.lsoil) {  // temp_sfc_full of the sea and soil surface, mo_manage_radiation_calls.F90:36-42
+    const ix_t i1 = LG_IX(c.G, ji, jk, 1), dl = (ix_t)c.G.lvl;
+    const double sea = 1.0 - c.g64.land_fraction[i1] - c.g64.soil[i1];
+    t_sfc_in = sea * c.g64.soil[i1 + 4 * dl] + (1.0 - sea) * s.temperature_soil[i1];
+  }
+  const double t_sfc = rad_clip(t_sfc_in);
   const double p_sfc = spi[LG_IX(c.G, ji, jk, nlev)];
   if (x) {
     x[5 * nl] = (float)t_sfc;

@@ -1886,7 +1886,12 @@ static void rad_column(oracle *o, const ostate *s, int ji, int jk, double mu0, d
     else
       pi[jl - 1] = 0.5 * (p[jl - 2] + p[jl - 1]);
   }
-  const double t_sfc = rad_clip(o->tsfc[col]);
+  double t_sfc_in = o->tsfc[col];
+  if (o->lprog_soil) { /* temp_sfc_full of the sea and soil surface, mo_manage_radiation_calls.F90:36-42 */
+    const double sea = 1.0 - A3(o->land_fraction, ji, jk, 1) - A3(o->lake_fraction, ji, jk, 1);
+    t_sfc_in = sea * A3(o->sst, ji, jk, 1) + (1.0 - sea) * A3(s->temperature_soil, ji, jk, 1);
+  }
+  const double t_sfc = rad_clip(t_sfc_in);
   /* gray two-stream longwave */
   for (int jl = 1; jl <= nl; ++jl) {
     const double dm = (pi[jl] - pi[jl - 1]) / LGAME_RAD_GRAVITY;
@@ -1942,7 +1947,9 @@ static void rad_column(oracle *o, const ostate *s, int ji, int jk, double mu0, d
 }
 
 /* update_rad_fluxes, mo_manage_radiation_calls.F90:17-59: all columns at time_coordinate +
-   dtime_rad/2 (the surface temperature comes in directly: soil and sea are not prognostic here) */
+   dtime_rad/2.  The surface temperature is the sea and soil mix of the reference with the
+   prognostic soil, else it comes in directly (without the soil the reference's mix is of the
+   constant initial soil temperature and sea surface temperature). */
 static void update_rad_fluxes(oracle *o, const ostate *s, double t) {
   const double t_rad = t + 0.5 * o->dtime_rad;
   for (int jk = 1; jk <= o->nx; ++jk)

@@ -58,7 +58,6 @@ def _init(mutate_dims=None, mutate_cfg=None):
     (None, lambda c: setattr(c, "lsfc_sensible_heat_flux", 1), "without the prognostic soil"),
     (lambda d: setattr(d, "nsoillays", 1), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
     (lambda d: setattr(d, "nsoillays", 17), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
-    (None, lambda c: (setattr(c, "lprog_soil_temp", 1), setattr(c, "lrad", 1)), "radiation together with the prognostic soil"),
 ])
 def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
     rc, err, h = _init(mutate_dims, mutate_cfg)

@@ -15,9 +15,9 @@ pytestmark = pytest.mark.gpu
 T_EPOCH = 1576843200.0
 
 
-def _case(moist, nl=12, cloud=0.02):
+def _case(moist, nl=12, cloud=0.02, **kw):
     cfg = dataclasses.replace(tiny_config(ny=7, nx=9, n_layers=nl, moist=moist), lrad=True, rad_mode=RAD_TEACHER,
-                              dtime_rad=6.0)
+                              dtime_rad=6.0, **kw)
     g, st = make_case(cfg, cloud_frac=cloud if moist else 0.0)
     return cfg, g, st, radiation_surface(cfg, g, st)
 
@@ -42,9 +42,14 @@ def test_rad_columns_match_oracle(moist, hours):
     np.testing.assert_allclose(yg, yo, rtol=2e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("moist", [False, True])
-def test_steps_with_radiation_match_oracle(moist):
-    cfg, g, st, sf = _case(moist)
+SOIL = {"lprog_soil_temp": True, "lsfc_sensible_heat_flux": True}
+
+
+@pytest.mark.parametrize("moist,soil", [(False, False), (True, False), (True, True)])
+def test_steps_with_radiation_match_oracle(moist, soil):
+    """With the prognostic soil, the radiation sees the soil and sea surface temperature and the
+    soil takes the radiation's surface fluxes."""
+    cfg, g, st, sf = _case(moist, **(SOIL if soil else {}))
     n = 6
     print(f"steps with radiation moist={moist} n={n}", flush=True)
     s = Step(cfg, g)
@@ -65,6 +70,8 @@ def test_steps_with_radiation_match_oracle(moist):
     rel = max_rel(got, want)
     print(f"  max relative deviation {rel:.3e}")
     assert rel <= 1e-12
+    if soil:
+        assert np.abs(got["temperature_soil"] - st["temperature_soil"]).max() > 0.0
     np.testing.assert_allclose(rt_g, rt_o, rtol=1e-12, atol=1e-12 * np.abs(rt_o).max())
     np.testing.assert_allclose(sw_g, sw_o, rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(lw_g, lw_o, rtol=1e-12)
