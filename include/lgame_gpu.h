@@ -294,6 +294,16 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
 int lgame_set_state(lgame_handle *h, int slot, const lgame_state_view *state);
 
 /*
+ * coeff_1 * (state in slot_1) + coeff_2 * (state in slot_2), computed on the device and written
+ * to a host state view: the densities, rhotheta_v, Exner pressure, winds and soil temperature
+ * combine linearly; theta_v_pert is rhotheta_v / rho_gas - theta_v_bg of the combined state.
+ * Replaces linear_combine_two_states (src/spatial_operators/mo_linear_combination.F90:15-37),
+ * which the reference's time loop calls to interpolate the state to an output time
+ * (control.F90:539).  slot_1 and slot_2 may be equal.
+ */
+int lgame_linear_combine(lgame_handle *h, int slot_1, int slot_2, double coeff_1, double coeff_2, lgame_state_view *out);
+
+/*
  * Device -> host copy of resident slot 0 or 1 (the reference refreshes host state only before
  * write_output / write_out_integrals, control.F90:529-550).
  */

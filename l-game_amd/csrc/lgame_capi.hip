@@ -96,6 +96,21 @@ __global__ void k_to_ref(double *dst, const T *src, int n1, int n2, int n3, Geom
   dst[(size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1))] = (double)src[LG_IX(G, ji, jk, jl)];
 }
 
+// c1*a + c2*b of one field, written in the reference layout (linear_combine_two_states,
+// mo_linear_combination.F90:27-35); with tb, the field is rhotheta/rho_gas - theta_v_bg of the
+// combined rhotheta (a, b) and gas densities (ga, gb) instead
+__global__ void k_combine_to_ref(double *dst, const double *a, const double *b, const double *ga, const double *gb, const double *tb,
+                                 double c1, double c2, int n1, int n2, int n3, Geom G) {
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int jl = blockIdx.z + 1;
+  if (jk > n2 || ji > n1 || jl > n3) return;
+  const ix_t i = LG_IX(G, ji, jk, jl);
+  double v = c1 * a[i] + c2 * b[i];
+  if (tb) v = v / (c1 * ga[i] + c2 * gb[i]) - tb[i];
+  dst[(size_t)(ji - 1) + (size_t)n1 * ((size_t)(jk - 1) + (size_t)n2 * (size_t)(jl - 1))] = v;
+}
+
 // FP32 copy of a block of FP64 fields (the grid of the mixed form)
 __global__ void k_to_float(float *dst, const double *src, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = (float)src[i];
@@ -955,6 +970,48 @@ static int state_io(lgame_handle *h, int slot, lgame_state_view *st, bool up) {
             : download(h, st->temperature_soil, S.temperature_soil.p, ny, nx, h->dims.nsoillays);
   if (!rc && up) h->have_state[slot] = 1;
   return rc;
+}
+
+int lgame_linear_combine(lgame_handle *h, int slot_1, int slot_2, double coeff_1, double coeff_2, lgame_state_view *out) {
+  if (!h || !out) return fail(h, LGAME_EINVAL, "null argument");
+  for (int s : {slot_1, slot_2})
+    if ((s != 0 && s != 1) || !h->have_state[s]) return fail(h, LGAME_ESTATE, "slot %d holds no state", s);
+  if (!out->rho || !out->rhotheta_v || !out->theta_v_pert || !out->exner_pert || !out->wind_u || !out->wind_v || !out->wind_w)
+    return fail(h, LGAME_EINVAL, "null state field");
+  const bool soil = h->c.C.lsoil != 0;
+  if (soil && !out->temperature_soil) return fail(h, LGAME_EINVAL, "temperature_soil is null (the prognostic soil is on)");
+  const int ny = h->dims.ny, nx = h->dims.nx, nl = h->dims.n_layers, nc = h->dims.n_constituents;
+  const StateDev &A = h->slot[slot_1], &B = h->slot[slot_2];
+  const double *tb = h->c.g64.theta_v_bg.p;
+  const int g = h->c.G.ncond;
+  struct F {
+    double *host;
+    const double *a, *b;
+    int n1, n2, n3;
+    bool theta;
+  };
+  std::vector<F> fs;
+  const size_t ncell = (size_t)ny * nx * nl;
+  for (int jc = 0; jc < nc; ++jc) fs.push_back({out->rho + jc * ncell, A.rho[jc].p, B.rho[jc].p, ny, nx, nl, false});
+  fs.push_back({out->rhotheta_v, A.rhotheta_v.p, B.rhotheta_v.p, ny, nx, nl, false});
+  fs.push_back({out->theta_v_pert, A.rhotheta_v.p, B.rhotheta_v.p, ny, nx, nl, true});
+  fs.push_back({out->exner_pert, A.exner_pert.p, B.exner_pert.p, ny, nx, nl, false});
+  fs.push_back({out->wind_u, A.wind_u.p, B.wind_u.p, ny, nx + 1, nl, false});
+  fs.push_back({out->wind_v, A.wind_v.p, B.wind_v.p, ny + 1, nx, nl, false});
+  fs.push_back({out->wind_w, A.wind_w.p, B.wind_w.p, ny, nx, nl + 1, false});
+  if (soil) fs.push_back({out->temperature_soil, A.temperature_soil.p, B.temperature_soil.p, ny, nx, h->dims.nsoillays, false});
+  for (const F &f : fs) {
+    const size_t n = (size_t)f.n1 * f.n2 * f.n3;
+    if (n > h->staging_elems) return fail(h, LGAME_EINVAL, "internal: staging buffer too small");
+    const dim3 blk(BX, BY, 1), grd((f.n2 + BX - 1) / BX, (f.n1 + BY - 1) / BY, f.n3);
+    hipLaunchKernelGGL(k_combine_to_ref, grd, blk, 0, h->stream, h->staging, f.a, f.b, f.theta ? (const double *)A.rho[g].p : nullptr,
+                       f.theta ? (const double *)B.rho[g].p : nullptr, f.theta ? tb : nullptr, coeff_1, coeff_2, f.n1, f.n2, f.n3,
+                       h->c.G);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(f.host, h->staging, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return LGAME_OK;
 }
 
 int lgame_set_state(lgame_handle *h, int slot, const lgame_state_view *state) {

@@ -72,6 +72,16 @@ module mo_lgame_gpu
       type(lgame_state_view),             intent(inout) :: state
     end function lgame_get_state
 
+    ! state_write = coeff_1*slot_1 + coeff_2*slot_2 (linear_combine_two_states, control.F90:539)
+    integer(c_int) function lgame_linear_combine(handle,slot_1,slot_2,coeff_1,coeff_2,state) &
+                                                 bind(C,name="lgame_linear_combine")
+      import :: c_int,c_ptr,c_double,lgame_state_view
+      type(c_ptr),            value, intent(in)    :: handle
+      integer(c_int),         value, intent(in)    :: slot_1,slot_2
+      real(c_double),         value, intent(in)    :: coeff_1,coeff_2
+      type(lgame_state_view),        intent(inout) :: state
+    end function lgame_linear_combine
+
     integer(c_int) function lgame_pchevi(handle,slot_old,slot_new,total_step_counter,lrad_update,t_0) &
                                          bind(C,name="lgame_pchevi")
       import :: c_int,c_ptr,c_double

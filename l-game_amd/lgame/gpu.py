@@ -17,7 +17,7 @@ from .config import GRID_FIELDS, GRID_SCALARS, STATE_FIELDS, RunConfig, state_sh
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "liblgame_gpu.so")
 
-EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_state", "lgame_pchevi",
+EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_state", "lgame_linear_combine", "lgame_pchevi",
             "lgame_pchevi_host", "lgame_sync", "lgame_integrals", "lgame_check_finite", "lgame_get_diag",
             "lgame_finalize", "lgame_last_error", "lgame_profile", "lgame_profile_read", "lgame_profile_reset",
             "lgame_rad_set_surface", "lgame_rad_columns", "lgame_rad_get", "lgame_rad_attach_mlp", "lgame_rad_stats",
@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_init.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
     lib.lgame_set_state.argtypes = [vp, ci, vp]
     lib.lgame_get_state.argtypes = [vp, ci, vp]
+    lib.lgame_linear_combine.argtypes = [vp, ci, ci, cd, cd, vp]
     lib.lgame_pchevi.argtypes = [vp, ci, ci, ci, ci, cd]
     lib.lgame_pchevi_host.argtypes = [vp, vp, vp, ci, ci, cd]
     lib.lgame_sync.argtypes = [vp]
@@ -188,6 +189,18 @@ class Step:
         soil = out["temperature_soil"].ctypes.data if "temperature_soil" in out else None
         v = StateView(*[out[k].ctypes.data for k in STATE_FIELDS], soil)
         self._check(self.lib.lgame_get_state(self.h, slot, ctypes.byref(v)), "lgame_get_state")
+        return out
+
+    def linear_combine(self, slot_1: int, slot_2: int, coeff_1: float, coeff_2: float) -> dict:
+        """coeff_1*slot_1 + coeff_2*slot_2 as a host state (linear_combine_two_states,
+        mo_linear_combination.F90:15-37; the output-time interpolation of control.F90:539)."""
+        c = self.cfg
+        out = {k: np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents, c.nsoillays), order="F")
+               for k in c.state_fields}
+        soil = out["temperature_soil"].ctypes.data if "temperature_soil" in out else None
+        v = StateView(*[out[k].ctypes.data for k in STATE_FIELDS], soil)
+        self._check(self.lib.lgame_linear_combine(self.h, slot_1, slot_2, coeff_1, coeff_2, ctypes.byref(v)),
+                    "lgame_linear_combine")
         return out
 
     def start(self, st: dict):

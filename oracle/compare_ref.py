@@ -62,6 +62,12 @@ def main():
             x, y = o.get_diag(k, shp), r.get_diag(k, shp)
             res["diag:" + k] = {"bitwise": bool(np.array_equal(x, y)), "maxabs": float(np.abs(x - y).max()),
                                 "moved": float(np.abs(x).max())}
+    if "combine" in spec:  # the output-time interpolation of control.F90:539
+        c1, c2 = spec["combine"]
+        x, y = o.linear_combine(0, 1, c1, c2), r.linear_combine(0, 1, c1, c2)
+        for k in x:
+            res["combine:" + k] = {"bitwise": bool(np.array_equal(x[k], y[k])), "maxabs": float(np.abs(x[k] - y[k]).max()),
+                                   "moved": float(np.abs(x[k]).max())}
     print(json.dumps(res))
 
 

@@ -2356,6 +2356,30 @@ int orc_run(oracle *o, int first, int nsteps) {
   return 0;
 }
 
+/* linear_combine_two_states, mo_linear_combination.F90:15-37: dst (a state's fields in the
+   reference layout, as orc_get_state returns them) = c1*(slot s1) + c2*(slot s2), theta_v_pert
+   from the combined rhotheta_v and gas density */
+int orc_linear_combine(oracle *o, int s1, int s2, double c1, double c2, const char *name, double *dst) {
+  if (s1 < 0 || s1 > 1 || s2 < 0 || s2 > 1) return -1;
+  size_t n = 0;
+  if (!strcmp(name, "theta_v_pert")) {
+    size_t nr = 0, nrho = 0;
+    const double *a = state_field(o, s1, "rhotheta_v", &nr), *b = state_field(o, s2, "rhotheta_v", &nr);
+    const double *ra = state_field(o, s1, "rho", &nrho), *rb = state_field(o, s2, "rho", &nrho);
+    const size_t off = (size_t)o->ncond * nr; /* the gas density, constituent n_condensed_constituents+1 */
+    for (size_t i = 0; i < nr; ++i) {
+      const double rt = c1 * a[i] + c2 * b[i];
+      const double rho = c1 * ra[off + i] + c2 * rb[off + i];
+      dst[i] = rt / rho - o->theta_v_bg.p[i];
+    }
+    return 0;
+  }
+  const double *a = state_field(o, s1, name, &n), *b = state_field(o, s2, name, &n);
+  if (!a || !b) return -1;
+  for (size_t i = 0; i < n; ++i) dst[i] = c1 * a[i] + c2 * b[i];
+  return 0;
+}
+
 /* state_2 = state_1 (control.F90:500) */
 void orc_copy_slot(oracle *o, int from, int to) {
   size_t n = (size_t)o->ny * o->nx * o->nl;

@@ -124,6 +124,22 @@ class CpuOracle:
             raise KeyError(name)
         return a
 
+    def linear_combine(self, slot_1: int, slot_2: int, coeff_1: float, coeff_2: float) -> dict:
+        """coeff_1*slot_1 + coeff_2*slot_2 (linear_combine_two_states) as a state dict."""
+        from lgame.config import state_shape
+        c = self.cfg
+        lib = self.lib()
+        lib.orc_linear_combine.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_char_p, ctypes.c_void_p]
+        lib.orc_linear_combine.restype = ctypes.c_int
+        out = {}
+        for k in c.state_fields:
+            a = np.zeros(state_shape(k, c.ny, c.nx, c.n_layers, c.n_constituents, c.nsoillays), order="F")
+            if lib.orc_linear_combine(self.h, slot_1, slot_2, coeff_1, coeff_2, k.encode(), _ptr(a)) != 0:
+                raise KeyError(k)
+            out[k] = a
+        return out
+
     def start(self, st: dict):
         """Load the initial state into slot 0 and copy it to slot 1 (control.F90:500)."""
         self.set_state(0, st)
@@ -243,6 +259,12 @@ class RefHarness:
         a = np.zeros(shape, order="F")
         self.lib.ref_get_diag(REF_DIAG_IDS[name], _ptr(a))
         return a
+
+    def linear_combine(self, slot_1: int, slot_2: int, coeff_1: float, coeff_2: float) -> dict:
+        """The reference's linear_combine_two_states into the harness's third state, read back."""
+        self.lib.ref_linear_combine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        self.lib.ref_linear_combine(slot_1, slot_2, coeff_1, coeff_2)
+        return self.get_state(2)
 
 
 def thomas_reference(c, d, e, r):

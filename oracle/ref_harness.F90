@@ -47,11 +47,12 @@ module ref_harness
   use mo_pbl,                only: update_sfc_turb_quantities,pbl_wind_tendency
   use mo_multiplications,    only: scalar_times_vector_h,scalar_times_vector_v
   use mo_vorticity_flux,     only: calc_vorticity_flux_term
+  use mo_linear_combination, only: linear_combine_two_states
 
   implicit none
 
   type(t_grid),  save, target :: grid
-  type(t_state), save, target :: st1,st2
+  type(t_state), save, target :: st1,st2,st3
   type(t_tend),  save         :: tend
   type(t_diag),  save         :: diag
   integer,       save         :: n_damping_levels
@@ -135,6 +136,7 @@ contains
 
     call alloc_state(st1)
     call alloc_state(st2)
+    call alloc_state(st3)
 
     allocate(tend%rho(ny,nx,n_layers,n_constituents)); tend%rho = 0._wp
     allocate(tend%rhotheta_v(ny,nx,n_layers)); tend%rhotheta_v = 0._wp
@@ -289,8 +291,10 @@ contains
     type(t_state), pointer :: s
     if (slot==0) then
       s => st1
-    else
+    elseif (slot==1) then
       s => st2
+    else
+      s => st3
     endif
   end function sel
 
@@ -680,6 +684,14 @@ contains
       endif
     enddo
   end subroutine ref_run
+
+  ! slot 2 = coeff_1*(slot slot_1) + coeff_2*(slot slot_2), linear_combine_two_states
+  ! (mo_linear_combination.F90:15-37), as control.F90:539 interpolates to an output time
+  subroutine ref_linear_combine(slot_1,slot_2,coeff_1,coeff_2) bind(C,name="ref_linear_combine")
+    integer(c_int), value, intent(in) :: slot_1,slot_2
+    real(c_double), value, intent(in) :: coeff_1,coeff_2
+    call linear_combine_two_states(sel(slot_1),sel(slot_2),st3,coeff_1,coeff_2,grid)
+  end subroutine ref_linear_combine
 
   ! state_2 = state_1 (control.F90:500)
   subroutine ref_copy_slot0_to_1() bind(C,name="ref_copy_slot0_to_1")

@@ -198,3 +198,27 @@ def test_step_argument_errors():
     g.start(st)
     with pytest.raises(LgameError, match="distinct"):
         g.pchevi(0, 0, 0)
+
+
+def test_linear_combine_matches_oracle():
+    """linear_combine_two_states of the two time levels (the output-time interpolation of
+    control.F90:539) on the device against the oracle, bit for bit; with coefficients (1, 0) it
+    gives back the first state (theta_v_pert included: the solver forms it the same way)."""
+    cfg = dataclasses.replace(tiny_config(moist=True), lprog_soil_temp=True)
+    grid, st = make_case(cfg, cloud_frac=0.3)
+    print("linear combination, tiny moist with soil, 3 steps", flush=True)
+    g = Step(cfg, grid)
+    g.start(st)
+    g.run(0, 3)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.run(0, 3)
+    for c1, c2 in ((0.3125, 0.6875), (-0.5, 1.5)):
+        a, b = g.linear_combine(0, 1, c1, c2), o.linear_combine(0, 1, c1, c2)
+        for k in b:
+            assert np.array_equal(a[k], b[k]), (c1, c2, k)
+    same = g.linear_combine(1, 0, 1.0, 0.0)
+    state = g.get_state(1)
+    for k in state:
+        assert np.array_equal(same[k], state[k]), k
+    g.close()
