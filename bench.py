@@ -13,6 +13,7 @@ Workloads (synthetic initial states, see l-game_amd/lgame/cases.py):
   surrogate-train harvest radiation columns from the model and train the 304 -> 3x512 -> 62 MLP
   ensemble        48 moist members of uneven size (nx, ny ~ U[64,384], 40-70 layers), mixed form,
                   autotuned block shapes, one graph launch per ensemble step vs serial launches
+  grid-setup      grid generation (grid_setup, bg_setup) of the moist512 grid on the device
 Block shapes of the stencil kernels are chosen by lgame_autotune before the timed steps
 (--no-autotune keeps the default 64x4).  --diffusion turns on the reference's default diffusion
 (momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients); --pbl the
@@ -240,6 +241,47 @@ def ensemble_members(n, seed0=5000):
     return out
 
 
+def bench_grid(args):
+    """Grid generation (grid_setup and bg_setup) on the device for the moist 512x512x60 grid:
+    lgame_grid_setup including the copies of every field to host memory, against the host
+    restatement (lgame.grid, numpy) on a 128x128x60 sample."""
+    from lgame.cases import moist3d_config
+    from lgame.grid import grid_setup
+    from lgame.gpu import grid_setup_device
+
+    cfg = moist3d_config(n=512, n_layers=60, dtime=3.22)
+    points = cfg.gridpoints
+    for _ in range(args.warmup):
+        grid_setup_device(cfg)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        grid_setup_device(cfg)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    log(f"device grid 512x512x60: median {1e3 * t:.1f} ms over {len(times)} calls")
+    cpu = None
+    if not args.no_cpu:
+        sample = moist3d_config(n=128, n_layers=60, dtime=3.22)
+        t0 = time.perf_counter()
+        grid_setup(sample)
+        tc = time.perf_counter() - t0
+        cpu = {"value": sample.gridpoints / tc, "unit": "grid points/s", "cores": 1, "kind": "port",
+               "sample": f"lgame.grid.grid_setup (numpy, one thread), moist 128x128x60, {tc:.1f} s"}
+        log(f"host grid 128x128x60: {tc:.2f} s")
+    out = {
+        "metric": "grid points generated per second (grid_setup and bg_setup) at 1 GPU",
+        "value": points / t, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "grid-setup: moist 512x512x60 grid, 5 Gaussian hills, SLEVE, rotated lat-lon, "
+                               "including the copies of all fields to host memory", "grid_points": points},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_ensemble(args):
     """Config-5 ensemble: aggregate throughput with one graph launch per step vs serial launches."""
     from lgame.cases import make_case
@@ -308,7 +350,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="moist512",
-                    choices=["moist512", "dry128", "schaer", "surrogate-train", "ensemble"])
+                    choices=["moist512", "dry128", "schaer", "surrogate-train", "ensemble", "grid-setup"])
     ap.add_argument("--members", type=int, default=48)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--diffusion", action="store_true",
@@ -340,6 +382,8 @@ def main():
         return bench_training(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "grid-setup":
+        return bench_grid(args)
 
     from lgame.cases import make_case, radiation_surface
     from lgame.gpu import PROF_RADIATION, PROF_SOLVE_VER, PROF_STEP

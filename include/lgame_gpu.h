@@ -277,6 +277,51 @@ typedef struct lgame_rad_view {
 #define LGAME_DIAG_PFD_SENS_SOIL   10  /* diag%power_flux_density_sens_soil (ny,nx), same           */
 #define LGAME_DIAG_PFD_SENS_SEA    11  /* diag%power_flux_density_sens_sea (ny,nx), same            */
 
+/*
+ * Grid generation on the device (grid_setup and bg_setup, src/io/mo_grid_generator.F90:29-1457,
+ * for the analytic orographies of :326-361 and periodic boundaries).  Parameters mirror the
+ * &run and &surface namelist entries the generator reads (mo_run_nml.F90, mo_surface_nml.F90);
+ * orography_id 5 is a set-up extension: a sum of Gaussian hills (x0, y0, height, half width in
+ * metres, in `hills`, 4 per hill) on the periodic domain.
+ */
+typedef struct lgame_grid_params {
+  double dx, dy, toa, stretching_parameter;
+  double lat_center, lon_center;  /* rotation of the grid's pole [rad]                           */
+  int orography_id;               /* 0 flat, 2 Schaer, 3, 4, 5 Gaussian hills                    */
+  int lsleve, lplane, lfreeslip;
+  int n_hills;
+  const double *hills;            /* (4, n_hills), read with orography_id 5                     */
+} lgame_grid_params;
+
+/*
+ * Output of lgame_grid_setup: the arrays of lgame_grid_view in the same order and reference
+ * layout, written by the library, then the geographic coordinates of the scalar points (used
+ * by the radiation) and the two scalars.  The surface is set as the reference's generator sets
+ * it without land (land and lake fraction 0, roughness length 0.08 m, water heat capacity and
+ * conductivity, t_const_soil from the latitude) with a sea surface temperature of 280 K (the
+ * initial soil temperature, mo_set_initial_state.F90:236-247).
+ */
+typedef struct lgame_grid_out {
+  double *z_scalar, *z_u, *z_v, *z_w, *dx, *dy, *dz, *area_x, *area_y, *area_z, *volume, *slope_x, *slope_y;
+  double *inner_product_weights, *area_dual_x, *area_dual_y, *area_dual_z, *z_area_dual_z;
+  double *fvec_x, *fvec_y, *fvec_z, *trsk_weights_u, *trsk_weights_v;
+  double *exner_bg_grad_u, *exner_bg_grad_v, *exner_bg_grad_w, *theta_v_bg, *exner_bg, *gravity_m_v, *gravity_potential;
+  double *dx_dual, *dy_dual, *layer_thickness, *land_fraction, *roughness_length;
+  double *lake_fraction, *sfc_rho_c, *t_conduc_soil, *t_const_soil, *sst, *z_soil_interface, *z_soil_center;
+  double *lat_geo_scalar, *lon_geo_scalar;  /* (ny,nx) */
+  double mean_velocity_area;
+  double z_t_const;
+} lgame_grid_out;
+
+/*
+ * Fill *out with the grid of dims and params, computed on device 0 and copied to the caller's
+ * arrays (every pointer of *out must be valid).  Replaces grid_setup and bg_setup
+ * (mo_grid_generator.F90:29-1457) for the analytic orographies.  Returns LGAME_EINVAL for
+ * unsupported parameters or, as the reference stops, for levels that are not monotonic or
+ * orography that reaches the lowest flat level (:936-951).
+ */
+int lgame_grid_setup(const lgame_dims *dims, const lgame_grid_params *params, lgame_grid_out *out);
+
 /* Version of this ABI (LGAME_ABI_VERSION). */
 int lgame_abi_version(void);
 

@@ -11,6 +11,7 @@
 #include "../../include/lgame_gpu.h"
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_grid.h"
 
 using namespace lgame;
 
@@ -516,6 +517,15 @@ int lgame_profile_reset(lgame_handle *h) {
 }
 
 int lgame_abi_version(void) { return LGAME_ABI_VERSION; }
+
+int lgame_grid_setup(const lgame_dims *dims, const lgame_grid_params *params, lgame_grid_out *out) {
+  g_init_error.clear();
+  if (!dims || !params || !out) return fail(nullptr, LGAME_EINVAL, "null argument");
+  std::string err;
+  const int rc = grid_generate(*dims, *params, *out, err);
+  if (rc) return fail(nullptr, rc, "%s", err.c_str());
+  return LGAME_OK;
+}
 
 const char *lgame_last_error(const lgame_handle *h) { return h ? h->err.c_str() : g_init_error.c_str(); }
 

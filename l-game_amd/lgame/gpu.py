@@ -17,7 +17,7 @@ from .config import GRID_FIELDS, GRID_SCALARS, STATE_FIELDS, RunConfig, state_sh
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "liblgame_gpu.so")
 
-EXPORTED = ("lgame_abi_version", "lgame_init", "lgame_set_state", "lgame_get_state", "lgame_linear_combine", "lgame_pchevi",
+EXPORTED = ("lgame_abi_version", "lgame_grid_setup", "lgame_init", "lgame_set_state", "lgame_get_state", "lgame_linear_combine", "lgame_pchevi",
             "lgame_pchevi_host", "lgame_sync", "lgame_integrals", "lgame_check_finite", "lgame_get_diag",
             "lgame_finalize", "lgame_last_error", "lgame_profile", "lgame_profile_read", "lgame_profile_reset",
             "lgame_rad_set_surface", "lgame_rad_columns", "lgame_rad_get", "lgame_rad_attach_mlp", "lgame_rad_stats",
@@ -68,6 +68,43 @@ class GridView(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in GRID_FIELDS] + [(n, ctypes.c_double) for n in GRID_SCALARS]
 
 
+class GridParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("dx", "dy", "toa", "stretching_parameter", "lat_center", "lon_center")] + \
+        [(n, ctypes.c_int) for n in ("orography_id", "lsleve", "lplane", "lfreeslip", "n_hills")] + [("hills", ctypes.c_void_p)]
+
+
+GRID_GEO = ("lat_geo_scalar", "lon_geo_scalar")
+
+
+class GridOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in tuple(GRID_FIELDS) + GRID_GEO] + [(n, ctypes.c_double) for n in GRID_SCALARS]
+
+
+def grid_setup_device(cfg: RunConfig) -> dict:
+    """The grid of cfg generated on the device (lgame_grid_setup, grid_setup and bg_setup of
+    mo_grid_generator.F90:29-1457): the same dict as lgame.grid.grid_setup, including the
+    geographic coordinates of the scalar points."""
+    if not cfg.lperiodic:
+        raise ValueError("only periodic grids are generated")
+    lib = load_library()
+    ny, nx, nl = cfg.ny, cfg.nx, cfg.n_layers
+    ns = max(cfg.nsoillays, 1)
+    out = {k: np.zeros(GRID_FIELDS[k](ny, nx, nl, ns) if k.startswith("z_soil") else GRID_FIELDS[k](ny, nx, nl), order="F")
+           for k in GRID_FIELDS}
+    out.update({k: np.zeros((ny, nx), order="F") for k in GRID_GEO})
+    hills = np.ascontiguousarray(np.array(cfg.hills, dtype=np.float64).reshape(-1))
+    p = GridParams(cfg.dx, cfg.dy, cfg.toa, cfg.stretching_parameter, cfg.lat_center, cfg.lon_center,
+                   cfg.orography_id, int(cfg.lsleve), int(cfg.lplane), int(cfg.lfreeslip), len(cfg.hills),
+                   hills.ctypes.data if len(cfg.hills) else None)
+    o = GridOut(*[out[k].ctypes.data for k in tuple(GRID_FIELDS) + GRID_GEO])
+    d, _ = cfg.to_abi()
+    rc = lib.lgame_grid_setup(ctypes.byref(d), ctypes.byref(p), ctypes.byref(o))
+    if rc != 0:
+        raise LgameError(f"lgame_grid_setup failed ({rc}): {lib.lgame_last_error(None).decode()}")
+    out.update({k: float(getattr(o, k)) for k in GRID_SCALARS})
+    return out
+
+
 class RadView(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in RAD_SURFACE_FIELDS]
 
@@ -90,6 +127,7 @@ def load_library(path: str = LIB_PATH):
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.lgame_abi_version.restype = ci
     lib.lgame_init.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
+    lib.lgame_grid_setup.argtypes = [vp, vp, vp]
     lib.lgame_set_state.argtypes = [vp, ci, vp]
     lib.lgame_get_state.argtypes = [vp, ci, vp]
     lib.lgame_linear_combine.argtypes = [vp, ci, ci, cd, cd, vp]
