@@ -40,6 +40,24 @@ module mo_lgame_gpu
     real(c_double) :: mean_velocity_area, z_t_const
   end type lgame_grid_view
 
+  ! lgame_grid_setup: generator parameters and the t_grid arrays it fills
+  type, bind(C) :: lgame_grid_params
+    real(c_double) :: dx, dy, toa, stretching_parameter, lat_center, lon_center
+    integer(c_int) :: orography_id, lsleve, lplane, lfreeslip, n_hills
+    type(c_ptr)    :: hills
+  end type lgame_grid_params
+
+  type, bind(C) :: lgame_grid_out
+    type(c_ptr) :: z_scalar, z_u, z_v, z_w, dx, dy, dz, area_x, area_y, area_z, volume, slope_x, slope_y
+    type(c_ptr) :: inner_product_weights, area_dual_x, area_dual_y, area_dual_z, z_area_dual_z
+    type(c_ptr) :: fvec_x, fvec_y, fvec_z, trsk_weights_u, trsk_weights_v
+    type(c_ptr) :: exner_bg_grad_u, exner_bg_grad_v, exner_bg_grad_w, theta_v_bg, exner_bg
+    type(c_ptr) :: gravity_m_v, gravity_potential, dx_dual, dy_dual, layer_thickness, land_fraction, roughness_length
+    type(c_ptr) :: lake_fraction, sfc_rho_c, t_conduc_soil, t_const_soil, sst, z_soil_interface, z_soil_center
+    type(c_ptr) :: lat_geo_scalar, lon_geo_scalar
+    real(c_double) :: mean_velocity_area, z_t_const
+  end type lgame_grid_out
+
   type, bind(C) :: lgame_state_view
     type(c_ptr) :: rho, rhotheta_v, theta_v_pert, exner_pert, wind_u, wind_v, wind_w, temperature_soil
   end type lgame_state_view
@@ -49,6 +67,14 @@ module mo_lgame_gpu
     integer(c_int) function lgame_abi_version() bind(C,name="lgame_abi_version")
       import :: c_int
     end function lgame_abi_version
+
+    ! grid_setup and bg_setup (mo_grid_generator.F90:29-1457) on the device
+    integer(c_int) function lgame_grid_setup(dims,params,grid) bind(C,name="lgame_grid_setup")
+      import :: c_int,lgame_dims,lgame_grid_params,lgame_grid_out
+      type(lgame_dims),        intent(in)    :: dims
+      type(lgame_grid_params), intent(in)    :: params
+      type(lgame_grid_out),    intent(inout) :: grid
+    end function lgame_grid_setup
 
     integer(c_int) function lgame_init(dims,cfg,grid,handle) bind(C,name="lgame_init")
       import :: c_int,c_ptr,lgame_dims,lgame_cfg,lgame_grid_view
