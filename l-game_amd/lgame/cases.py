@@ -94,13 +94,14 @@ def soil_case(cfg: RunConfig, g: dict, st: dict, seed: int = 13) -> None:
     g["sst"] = np.asfortranarray(st["temperature_soil"][:, :, 0].copy())
 
 
-def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True, cloud_frac=0.0):
+def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True, cloud_frac=0.0, grid=None):
     """Grid and initial state for ``cfg`` (perturbations only for the standard scenario).  With
     the surface physics on, the grid's land fraction is the seeded field of land_fraction; with
-    the prognostic soil, soil_case sets lakes and the soil temperature."""
+    the prognostic soil, soil_case sets lakes and the soil temperature.  ``grid`` is a grid of
+    ``cfg`` made elsewhere (lgame.gpu.grid_setup_device), else the host generator makes it."""
     from .grid import soil_properties
 
-    g = grid_setup(cfg)
+    g = grid if grid is not None else grid_setup(cfg)
     if cfg.surface:
         g["land_fraction"] = land_fraction(cfg)
         soil_properties(g)
