@@ -57,6 +57,7 @@ METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
 T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
 NO_AUTOTUNE = False
+MIXED_RAD = "surrogate"
 
 
 def log(msg):
@@ -106,13 +107,14 @@ def make_step(cfg, grid, surface, form, mlp):
     from lgame.gpu import Step
 
     if form == "mixed":
-        c = dataclasses.replace(cfg, precision=PREC_MIXED, rad_mode=RAD_SURROGATE if cfg.lrad else cfg.rad_mode)
+        rad_mode = (RAD_TEACHER if MIXED_RAD == "solver" else RAD_SURROGATE) if cfg.lrad else cfg.rad_mode
+        c = dataclasses.replace(cfg, precision=PREC_MIXED, rad_mode=rad_mode)
     else:
         c = dataclasses.replace(cfg, precision=PREC_FP64, rad_mode=RAD_TEACHER if cfg.lrad else cfg.rad_mode)
     s = Step(c, grid)
     if cfg.lrad:
         s.set_rad_surface(surface)
-        if form == "mixed":
+        if form == "mixed" and MIXED_RAD == "surrogate":
             s.attach_mlp(mlp, tol_k_per_day=1.0, audit_fraction=0.01, seed=7)
     return s
 
@@ -239,6 +241,64 @@ def ensemble_members(n, seed0=5000):
         cfg = moist3d_config(n=nx, n_layers=nl, dtime=3.22, seed=seed0 + i)
         out.append(dataclasses.replace(cfg, ny=ny, nx=nx))
     return out
+
+
+def bench_accuracy(args, cfg, grid, st, surface, mlp):
+    """The mixed form + surrogate against the plain FP64 form (bitwise equal to the oracle on every
+    tested configuration) over args.accuracy_hours simulated hours from the same initial state,
+    radiation at the model's cadence: per-field RMS and max error relative to the FP64 fields,
+    every simulated hour.  Beside it, the same FP64 step from an initial state whose u differs by
+    1e-7 relative: the growth of that error is the flow's own, which no form of the step can
+    avoid."""
+    from lgame.metrics import field_errors, worst
+
+    n = int(round(args.accuracy_hours * 3600.0 / cfg.dt))
+    per_hour = max(1, int(round(3600.0 / cfg.dt)))
+    every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
+    st_p = dict(st)
+    st_p["wind_u"] = st["wind_u"] * (1.0 + 1e-7)
+    runs = {"fp64": make_step(cfg, grid, surface, "fp64", None), "mixed": make_step(cfg, grid, surface, "mixed", mlp),
+            "fp64_perturbed": make_step(cfg, grid, surface, "fp64", None)}
+    for name, step in runs.items():
+        step.start(st_p if name == "fp64_perturbed" else st)
+    history = []
+    t0 = time.perf_counter()
+    for k0 in range(0, n, per_hour):
+        m = min(per_hour, n - k0)
+        for k in range(k0, k0 + m):
+            lrad = bool(every) and k % every == 0
+            for step in runs.values():
+                step.pchevi(k % 2, 1 - k % 2, k, lrad, T_EPOCH + k * cfg.dt)
+        k = k0 + m
+        ref = runs["fp64"].get_state(k % 2)
+        row = {"hours": k * cfg.dt / 3600.0}
+        for name in ("mixed", "fp64_perturbed"):
+            bad = runs[name].count_nonfinite(k % 2)
+            err = field_errors(runs[name].get_state(k % 2), ref) if bad == 0 else None
+            row[name] = {"nonfinite": bad, "worst_rms": worst(err, "rms_rel") if err else None,
+                         "worst_max": worst(err, "max_rel") if err else None, "fields": err}
+        history.append(row)
+        log(f"{row['hours']:.2f} h: mixed worst rms {row['mixed']['worst_rms']}, perturbed FP64 worst rms "
+            f"{row['fp64_perturbed']['worst_rms']}, {time.perf_counter() - t0:.0f} s")
+    rad = runs["mixed"].rad_stats() if cfg.lrad else {}
+    for step in runs.values():
+        step.close()
+    last = history[-1]
+    out = {
+        "metric": "mixed + surrogate field error against the plain FP64 step after simulated hours",
+        "value": last["mixed"]["worst_rms"], "unit": "relative RMS error (worst field)", "n_gpus": 1, "steps": n,
+        "warmup": 0, "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {describe(cfg)}", "simulated_hours": args.accuracy_hours},
+        "worst_max_rel": last["mixed"]["worst_max"],
+        "perturbed_fp64_worst_rms": last["fp64_perturbed"]["worst_rms"],
+        "surrogate": rad,
+        "hourly": [{"hours": r["hours"], "mixed_worst_rms": r["mixed"]["worst_rms"],
+                    "mixed_worst_max": r["mixed"]["worst_max"], "mixed_nonfinite": r["mixed"]["nonfinite"],
+                    "perturbed_fp64_worst_rms": r["fp64_perturbed"]["worst_rms"]} for r in history],
+        "fields": last["mixed"]["fields"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def bench_grid(args):
@@ -371,13 +431,19 @@ def main():
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--save-surrogate", default="")
+    ap.add_argument("--mixed-radiation", default="surrogate", choices=["surrogate", "solver"],
+                    help="radiation of the mixed form: the MLP surrogate (default) or the gray two-stream solver")
+    ap.add_argument("--accuracy-hours", type=float, default=0.0,
+                    help="instead of timing: run both forms for this many simulated hours and report the mixed "
+                         "form's field errors against the FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
         return 2
-    global NO_AUTOTUNE
+    global NO_AUTOTUNE, MIXED_RAD
     NO_AUTOTUNE = args.no_autotune
+    MIXED_RAD = args.mixed_radiation
     if args.workload == "surrogate-train":
         return bench_training(args)
     if args.workload == "ensemble":
@@ -409,6 +475,8 @@ def main():
     grid, st = make_case(cfg)
     surface = radiation_surface(cfg, grid, st) if cfg.lrad else None
     log(f"set-up done in {time.perf_counter() - t0:.1f} s")
+    if args.accuracy_hours > 0.0:
+        return bench_accuracy(args, cfg, grid, st, surface, mlp if mlp is not None or not cfg.lrad else surrogate_mlp())
     cells = cfg.gridpoints
     res = {}
     for form in forms:

@@ -68,8 +68,13 @@ extern "C" {
  *   LGAME_RAD_SURROGATE: the per-column MLP attached with lgame_rad_attach_mlp; each call also
  *                        re-runs the solver on an audit sample of the columns and gives every
  *                        audited column whose heating-rate RMSE exceeds the tolerance the
- *                        solver's result (lgame_rad_stats counts both).
+ *                        solver's result (lgame_rad_stats counts both).  A column whose
+ *                        predicted outputs leave the network's training envelope (any output
+ *                        more than LGAME_RAD_ENVELOPE standard deviations of its training
+ *                        targets from their mean, or not finite) is always given the solver's
+ *                        result.
  */
+#define LGAME_RAD_ENVELOPE   6.0
 #define LGAME_RAD_OFF        0
 #define LGAME_RAD_TEACHER    1
 #define LGAME_RAD_SURROGATE  2
@@ -469,8 +474,9 @@ typedef struct lgame_mlp lgame_mlp;
  */
 int lgame_rad_attach_mlp(lgame_handle *h, lgame_mlp *m, double tol_k_per_day, double audit_fraction, unsigned long long seed);
 
-/* Counters since lgame_init: radiation updates, audited columns, columns given back to the
-   solver, and the RMSE [K/day] of the surrogate's heating rates over all audited columns. */
+/* Counters since lgame_init: radiation updates, columns of the random audit sample inside the
+   training envelope, columns given back to the solver (over the tolerance or outside the
+   envelope), and the RMSE [K/day] of the surrogate's heating rates over the counted sample. */
 int lgame_rad_stats(lgame_handle *h, long long *calls, long long *audited, long long *fallbacks, double *audit_rmse);
 
 /* ---- the surrogate network: a per-column MLP trained and evaluated with BF16 MFMA kernels ----

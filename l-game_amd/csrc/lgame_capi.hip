@@ -447,6 +447,7 @@ RadDev rad_dev(lgame_handle *h) {
   r.sq = h->rad_sq;
   r.tol = h->rad_tol;
   r.audit_thresh = h->rad_thresh;
+  if (h->mlp) mlp_out_norm(h->mlp, &r.y_mean, &r.y_scale);
   return r;
 }
 

@@ -604,6 +604,11 @@ int mlp_dims(const lgame_mlp *m, int *n_in, int *n_out) {
   return 0;
 }
 
+void mlp_out_norm(const lgame_mlp *m, const float **mean, const float **scale) {
+  *mean = m->y_mean;
+  *scale = m->y_scale;
+}
+
 }  // namespace lgame
 
 extern "C" {

@@ -21,6 +21,7 @@ struct RadDev {
   float *x;           // [col][5*nl+4]
   float *y;           // [col][nl+2]
   const float *ypred; // [col][nl+2], the surrogate's outputs
+  const float *y_mean, *y_scale;  // the surrogate's output normalisation (training envelope)
   unsigned long long *counts;  // [0] audited columns, [1] columns given back to the solver
   double *sq;                  // sum over audited columns and layers of the squared heating-rate error
   double tol;                  // per-column heating-rate RMSE tolerance [K/day]

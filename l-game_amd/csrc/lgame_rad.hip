@@ -56,8 +56,16 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
   if (col >= nx * ny) return;
   const int jk = col % nx + 1, ji = col / nx + 1;
   const bool audit = flags & RAD_AUDIT;
-  if (audit && rad_hash((unsigned)col ^ (unsigned)(r.call * 0x9E3779B9ull)) >= r.audit_thresh) return;
   const int nin = 5 * nl + 4, nout = nl + 2;
+  // a prediction outside the training envelope (or not finite) always goes back to the solver
+  bool outside = false;
+  if (audit && r.y_scale) {
+    const float *yc = r.ypred + (size_t)col * nout;
+    for (int o = 0; o < nout; ++o)
+      if (!(fabsf(yc[o] - r.y_mean[o]) <= (float)LGAME_RAD_ENVELOPE * r.y_scale[o])) outside = true;
+  }
+  const bool sampled = rad_hash((unsigned)col ^ (unsigned)(r.call * 0x9E3779B9ull)) < r.audit_thresh;
+  if (audit && !outside && !sampled) return;
   // per-level scratch: temperature, pressure, vapour/liquid/ice mixing ratios, water paths,
   // interface pressure, LW transmissivity, source, SW optical depth, SW heating
   const Fld<double> sT = c.w.col[0], sp = c.w.col[1], sqv = c.w.col[2], slwp = c.w.col[3], siwp = c.w.col[4];
@@ -197,9 +205,11 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
     bool write = (flags & RAD_TEND) != 0;
     if (audit) {
       const double rmse = sqrt(se / nl);
-      atomicAdd(r.counts + 0, 1ull);
-      atomicAdd(r.sq, se);
-      if (rmse > r.tol) {  // this column goes back to the solver
+      if (sampled && !outside) {  // the error statistic of the random sample inside the envelope
+        atomicAdd(r.counts + 0, 1ull);
+        atomicAdd(r.sq, se);
+      }
+      if (outside || rmse > r.tol) {  // this column goes back to the solver
         atomicAdd(r.counts + 1, 1ull);
         write = true;
       }
