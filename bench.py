@@ -54,7 +54,8 @@ STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step 
 # profiles/moist512_fp64_hbm_traffic.md holds the commands and the per-kernel table.
 SOLVER_TRAFFIC = {"moist512": 4.369e+09}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
-SURROGATE_WEIGHTS = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
+DEFAULT_SURROGATE = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
+SURROGATE_WEIGHTS = DEFAULT_SURROGATE
 T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
 NO_AUTOTUNE = False
 MIXED_RAD = "surrogate"
@@ -96,6 +97,8 @@ def surrogate_mlp():
     if os.path.exists(SURROGATE_WEIGHTS):
         log(f"radiation surrogate: {os.path.relpath(SURROGATE_WEIGHTS, ROOT)}")
         return surrogate.load(SURROGATE_WEIGHTS)
+    if SURROGATE_WEIGHTS != DEFAULT_SURROGATE:
+        raise SystemExit(f"no surrogate weights at {SURROGATE_WEIGHTS}")
     log("radiation surrogate: no trained weights, training a small one on the GPU")
     m, _, rep = surrogate.build(states=8, epochs=30)
     log(f"surrogate trained: {rep}")
@@ -208,8 +211,10 @@ def bench_training(args):
     """The surrogate training run: harvest columns from the model, train with BF16 MFMA GEMMs."""
     from lgame import surrogate
 
-    m, norm, rep = surrogate.build(states=args.train_states, epochs=args.train_epochs, batch=args.train_batch,
-                                   lr=args.train_lr, progress=log)
+    horizon = int(round(args.train_horizon_hours * 3600.0 / 3.22))
+    m, norm, rep = surrogate.build(states=args.train_states, calls=args.train_calls, times=args.train_times,
+                                   epochs=args.train_epochs, batch=args.train_batch, lr=args.train_lr, progress=log,
+                                   horizon_steps=horizon)
     if args.save_surrogate:
         surrogate.save(args.save_surrogate, m, norm)
         log(f"saved {args.save_surrogate}")
@@ -217,7 +222,9 @@ def bench_training(args):
         "metric": "surrogate training samples/s at 1 GPU (BF16 MFMA, FP32 Adam)",
         "value": rep["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": rep["epochs"], "warmup": 0,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
-        "config": {"workload": "surrogate-train: columns from 33x33x60 moist runs, 304 -> 3x512 GELU -> 62",
+        "config": {"workload": "surrogate-train: columns from 33x33x60 moist runs"
+                               + (f" over {args.train_horizon_hours:g} simulated hours" if horizon else "")
+                               + ", 304 -> 3x512 GELU -> 62",
                    "batch": rep["batch"], "columns": rep["train_columns"]},
         "roofline": {"kernel": "k_gemm (all MLP GEMMs of training)", "bound": "mfma", "achieved": rep["train_tflops"],
                      "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": rep["train_tflops"] / BF16_PEAK_TFLOPS,
@@ -430,7 +437,12 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=60)
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--train-calls", type=int, default=2, help="radiation updates harvested per training run")
+    ap.add_argument("--train-times", type=int, default=6, help="times of day per harvested update")
+    ap.add_argument("--train-horizon-hours", type=float, default=0.0,
+                    help="spread the harvested updates of each run over this many simulated hours")
     ap.add_argument("--save-surrogate", default="")
+    ap.add_argument("--surrogate", default="", help="surrogate weights (default: l-game_amd/data/rad_surrogate.npz)")
     ap.add_argument("--mixed-radiation", default="surrogate", choices=["surrogate", "solver"],
                     help="radiation of the mixed form: the MLP surrogate (default) or the gray two-stream solver")
     ap.add_argument("--accuracy-hours", type=float, default=0.0,
@@ -441,9 +453,11 @@ def main():
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
         return 2
-    global NO_AUTOTUNE, MIXED_RAD
+    global NO_AUTOTUNE, MIXED_RAD, SURROGATE_WEIGHTS
     NO_AUTOTUNE = args.no_autotune
     MIXED_RAD = args.mixed_radiation
+    if args.surrogate:
+        SURROGATE_WEIGHTS = os.path.abspath(args.surrogate)
     if args.workload == "surrogate-train":
         return bench_training(args)
     if args.workload == "ensemble":

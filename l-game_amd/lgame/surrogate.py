@@ -20,19 +20,26 @@ HIDDEN, N_HIDDEN = 512, 3
 T_EPOCH = 1576843200.0
 
 
-def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_max=0.01, progress=None):
+def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_max=0.01, progress=None,
+            horizon_steps=0):
     """Columns (x, y) from runs of the moist n x n x 60 case with the solver as radiation.
 
     Each run starts from a seeded perturbed state whose cloud condensates, surface albedo and
     surface temperature vary from column to column (smooth random fields), and steps with
     radiation updates every `steps_between` steps.  At every update the inputs and solver outputs
     of all columns are taken for the update's own time and for `times - 1` further random times
-    of day (the solver is a function of the state and the sun's position).
+    of day (the solver is a function of the state and the sun's position).  With horizon_steps,
+    the `calls` updates are spread over that many steps of the run, so that the columns also
+    come from the developed flow (stronger vertical motion, displaced temperature and moisture)
+    that a long run reaches.
     """
+    if horizon_steps > 0 and calls > 1:
+        steps_between = max(steps_between, horizon_steps // (calls - 1))
     from .init_state import _smooth_periodic
 
     rng = np.random.default_rng(seed)
     xs, ys = [], []
+    dropped = [0]
     for k in range(n_states):
         cfg = moist3d_config(n=n, n_layers=LEVELS, dtime=3.22, seed=int(rng.integers(1, 10 ** 6)))
         cfg = dataclasses.replace(cfg, lrad=True, rad_mode=RAD_TEACHER, dtime_rad=steps_between * cfg.dt)
@@ -61,10 +68,17 @@ def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_m
                 ys.append(y)
             s.run(step, steps_between, t_init=t0)
             step += steps_between
+            if c + 1 < calls and s.count_nonfinite(step % 2):  # a run that went unstable gives no more columns
+                dropped[0] += 1
+                break
         s.close()
-        if progress and (k + 1) % 128 == 0:
+        if progress and (k + 1) % 16 == 0:
             progress(f"harvested {k + 1} of {n_states} states")
-    return np.concatenate(xs), np.concatenate(ys)
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    ok = np.all(np.isfinite(x), axis=1) & np.all(np.isfinite(y), axis=1)
+    if progress and (dropped[0] or not ok.all()):
+        progress(f"harvest: {dropped[0]} runs stopped early (non-finite state), {int((~ok).sum())} columns dropped")
+    return x[ok], y[ok]
 
 
 def fit_norm(x, y):
@@ -108,13 +122,17 @@ def load(path) -> Mlp:
     return m
 
 
-def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, test_states=3, progress=None):
-    """Harvest, train and evaluate; returns (mlp, norm, report)."""
+def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, test_states=3, progress=None,
+          horizon_steps=0):
+    """Harvest, train and evaluate; returns (mlp, norm, report).  The test columns come from
+    separate seeded runs over the same horizon."""
     import time
 
     t0 = time.perf_counter()
-    xtr, ytr = harvest(n=33, n_states=states, calls=calls, times=times, seed=seed, progress=progress)
-    xte, yte = harvest(n=33, n_states=test_states, calls=1, times=times, seed=seed + 1000)
+    xtr, ytr = harvest(n=33, n_states=states, calls=calls, times=times, seed=seed, progress=progress,
+                       horizon_steps=horizon_steps)
+    xte, yte = harvest(n=33, n_states=test_states, calls=calls if horizon_steps else 1, times=times, seed=seed + 1000,
+                       horizon_steps=horizon_steps)
     t_harvest = time.perf_counter() - t0
     norm = fit_norm(xtr, ytr)
     m, loss, seconds = train(xtr, ytr, epochs=epochs, batch=batch, lr=lr, seed=seed, norm=norm)
@@ -123,6 +141,7 @@ def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, t
     flops = 6.0 * n_params * len(xtr) * epochs  # forward 2, backward 4 flops per parameter and sample
     report = {
         "train_columns": int(len(xtr)), "test_columns": int(len(xte)), "epochs": epochs, "batch": batch, "lr": lr,
+        "harvest_calls_per_run": calls, "harvest_horizon_steps": horizon_steps,
         "harvest_s": t_harvest, "train_s": seconds, "samples_per_s": len(xtr) * epochs / seconds,
         "train_tflops": flops / seconds / 1e12, "params": n_params,
         "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
