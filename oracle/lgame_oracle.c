@@ -11,14 +11,19 @@
  *   mo_scalar_tend_expl.F90, mo_column_solvers.F90;
  *   spatial_operators/mo_gradient_operators.F90, mo_divergence_operators.F90,
  *   mo_multiplications.F90, mo_averaging.F90, mo_vorticities.F90, mo_vorticity_flux.F90,
- *   mo_inner_product.F90, mo_momentum_diff_diss.F90 (simple_dissipation_rate);
+ *   mo_inner_product.F90, mo_momentum_diff_diss.F90 (mom_diff_h, mom_diff_v,
+ *   simple_dissipation_rate), mo_linear_combination.F90;
+ *   subgrid_scale/mo_tke.F90, mo_eff_diff_coeffs.F90, mo_pbl.F90;
  *   constituents/mo_derived.F90, mo_dictionary.F90; io/mo_write_out.F90 (budget sums).
  * Arrays keep the reference layout (column-major, 1-based indices through the A3 macro).
  * Build with -O2 -ffp-contract=off: the reference build has no FMA contraction.
  *
- * Supported configuration: periodic lateral boundaries, luse_bg_state, diffusion/PBL/surface
- * fluxes/soil off (the reference switches used by the Schaer and standard-atmosphere runs).
- * Anything else is rejected by orc_create.  Radiation: the coupler steps of
+ * Supported configuration: periodic lateral boundaries, luse_bg_state; diffusion (every switch
+ * set in which horizontal temperature and mass diffusion come with horizontal momentum
+ * diffusion), the PBL (with lmom_diff_h), the prognostic soil, the surface sensible heat flux
+ * (with the soil) and lsfc_phase_trans.  Anything else is rejected by orc_create.  Every
+ * function is pinned bitwise against the reference's compiled modules (oracle/ref_harness.F90)
+ * or against golden fixtures written from them.  Radiation: the coupler steps of
  * radiation/mo_rterrtmgp_coupler.F90 are restated; RTE+RRTMGP, which is not in the reference tree,
  * is replaced by the gray two-stream solver that include/lgame_gpu.h specifies (its outputs are
  * pinned by closed forms and flux conservation, not by the reference).
