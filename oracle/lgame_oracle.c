@@ -21,9 +21,12 @@
  * Supported configuration: periodic lateral boundaries, luse_bg_state; diffusion (every switch
  * set in which horizontal temperature and mass diffusion come with horizontal momentum
  * diffusion), the PBL (with lmom_diff_h), the prognostic soil, the surface sensible heat flux
- * (with the soil) and lsfc_phase_trans.  Anything else is rejected by orc_create.  Every
- * function is pinned bitwise against the reference's compiled modules (oracle/ref_harness.F90)
- * or against golden fixtures written from them.  Radiation: the coupler steps of
+ * (with the soil) and lsfc_phase_trans.  Anything else is rejected by orc_create.  The step is
+ * pinned bitwise against the reference's compiled modules driven by oracle/ref_harness.F90
+ * (which restates the glue of the modules that do not build here: the step sequence of
+ * mo_manage_pchevi.F90 and mo_vector_tend_expl.F90, and mom_diff_h, mom_diff_v and
+ * simple_dissipation_rate, the latter pinned by closed forms in the tests) and against golden
+ * fixtures written from them.  Radiation: the coupler steps of
  * radiation/mo_rterrtmgp_coupler.F90 are restated; RTE+RRTMGP, which is not in the reference tree,
  * is replaced by the gray two-stream solver that include/lgame_gpu.h specifies (its outputs are
  * pinned by closed forms and flux conservation, not by the reference).
