@@ -57,6 +57,8 @@ def _init(mutate_dims=None, mutate_cfg=None):
     (None, lambda c: setattr(c, "lpbl", 1), "lpbl needs lmom_diff_h"),
     (None, lambda c: setattr(c, "lsfc_sensible_heat_flux", 1), "without the prognostic soil"),
     (lambda d: setattr(d, "nsoillays", 1), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
+    (lambda d: (setattr(d, "ny", 6000), setattr(d, "nx", 6000), setattr(d, "n_layers", 30), setattr(d, "n_oro_layers", 20)),
+     None, "exceeds 4 GiB"),
     (lambda d: setattr(d, "nsoillays", 17), lambda c: setattr(c, "lprog_soil_temp", 1), "nsoillays"),
 ])
 def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):

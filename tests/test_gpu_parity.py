@@ -76,6 +76,9 @@ CASES = {
     "dry3d_33x33x20_diff_pbl_soil7": (diff(dry3d_config(n=33, n_layers=20, nsoillays=7), lpbl=True,
                                            lprog_soil_temp=True, lsfc_sensible_heat_flux=True), 8, 0.0),
     "tiny_dry_phase_trans": (dataclasses.replace(tiny_config(), lsfc_phase_trans=True), 4, 0.0),
+    # the smallest domains the step accepts
+    "minimal_3x3x3": (tiny_config(ny=3, nx=3, n_layers=3), 4, 0.0),
+    "minimal_3x4x4_moist_o3_diff": (diff(tiny_config(ny=3, nx=4, n_layers=4, moist=True, order=3)), 4, 0.2),
 }
 
 

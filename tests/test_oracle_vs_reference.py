@@ -50,6 +50,10 @@ CASES = [
     {"kind": "dry3d", "kw": {"n": 17, "n_layers": 12, "nsoillays": 7}, "steps": 8, "diffusion": True,
      "diff_kw": {"lpbl": True, "lprog_soil_temp": True, "lsfc_sensible_heat_flux": True}},
     {"kind": "tiny", "kw": {}, "steps": 4, "diff_kw": {"lsfc_phase_trans": True}},
+    # the smallest domains the step accepts
+    {"kind": "tiny", "kw": {"ny": 3, "nx": 3, "n_layers": 3}, "steps": 4},
+    {"kind": "tiny", "kw": {"ny": 3, "nx": 4, "n_layers": 4, "moist": True, "order": 3}, "steps": 4, "cloud_frac": 0.2,
+     "diffusion": True},
     # the linear combination of the two time levels (output-time interpolation)
     {"kind": "tiny", "kw": {"moist": True}, "steps": 3, "cloud_frac": 0.3, "combine": [0.3125, 0.6875],
      "diff_kw": {"lprog_soil_temp": True}},
