@@ -350,7 +350,7 @@ def bench_grid(args):
 
 
 def bench_ensemble(args):
-    """Config-5 ensemble: aggregate throughput with one graph launch per step vs serial launches."""
+    """The 48-member ensemble: aggregate throughput with one graph launch per step vs serial launches."""
     from lgame.cases import make_case
     from lgame.config import PREC_MIXED
     from lgame.gpu import Ensemble, Step, grid_setup_device
@@ -456,7 +456,7 @@ def main():
                          "form's field errors against the FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: the step runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
         return 2
     global NO_AUTOTUNE, MIXED_RAD, SURROGATE_WEIGHTS
