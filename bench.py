@@ -457,7 +457,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the step runs on one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU sharding is a design note, not part of the build", file=sys.stderr)
+              "multi-GPU sharding is a design note, not implemented", file=sys.stderr)
         return 2
     global NO_AUTOTUNE, MIXED_RAD, SURROGATE_WEIGHTS
     NO_AUTOTUNE = args.no_autotune
