@@ -102,3 +102,35 @@ def test_mixed_integrals_are_fp64_sums():
     np.testing.assert_allclose(i0[[0, 1, 2]], want[[0, 1, 2]], rtol=1e-13)
     np.testing.assert_allclose(i0[4:], want[4:], rtol=1e-13)
     np.testing.assert_allclose(i0[3], want[3], rtol=1e-6)
+
+
+def test_mixed_form_with_radiation_and_soil():
+    """The mixed form with the two-stream radiation coupled to the prognostic soil (surface
+    temperature from the soil and sea, surface fluxes into the soil) against the oracle."""
+    from lgame.cases import radiation_surface
+    from lgame.config import RAD_TEACHER
+
+    cfg = dataclasses.replace(tiny_config(ny=7, nx=9, n_layers=12, moist=True), lrad=True, rad_mode=RAD_TEACHER,
+                              dtime_rad=6.0, lprog_soil_temp=True, lsfc_sensible_heat_flux=True)
+    grid, st = make_case(cfg, cloud_frac=0.02)
+    sf = radiation_surface(cfg, grid, st)
+    n, t0 = 8, 1576843200.0
+    print(f"mixed with radiation and soil: {cfg.ny}x{cfg.nx}x{cfg.n_layers}, {n} steps", flush=True)
+    o = CpuOracle(cfg, grid)
+    o.start(st)
+    o.set_rad_surface(sf)
+    o.run_rad(0, n, t0)
+    want = o.get_state(n % 2)
+    g = Step(dataclasses.replace(cfg, precision=PREC_MIXED), grid)
+    g.start(st)
+    g.set_rad_surface(sf)
+    g.run(0, n, t_init=t0)
+    got = g.get_state(n % 2)
+    assert g.count_nonfinite(n % 2) == 0
+    g.close()
+    err = field_errors(got, want)
+    for k, v in err.items():
+        print(f"  {k:18s} rms {v['rms_rel']:.3e}  max {v['max_rel']:.3e}")
+    assert worst(err, "rms_rel") < RMS_TOL
+    assert worst(err, "max_rel") < MAX_TOL
+    assert np.abs(got["temperature_soil"] - st["temperature_soil"]).max() > 0.0
