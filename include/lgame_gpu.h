@@ -145,7 +145,8 @@ typedef struct lgame_dims {
  * lsfc_sensible_heat_flux adds the sensible heat flux over soil and sea to the lowest layer
  * (:81-120) and needs lprog_soil_temp (the scalar flux resistance it divides by exists only with
  * the soil, mo_pbl.F90:96-104).  lsfc_phase_trans only turns on the surface turbulence
- * quantities: the reference's surface phase transitions sit in a branch that never runs.  With
+ * quantities: the reference's surface phase transitions sit in calc_h2otracers_source_rates
+ * (mo_phase_trans.F90:22-330), which nothing in the reference calls.  With
  * the prognostic soil and radiation, the radiation's surface temperature is the reference's mix
  * of the sea surface temperature and the top soil layer (mo_manage_radiation_calls.F90:36-42,
  * the temp_sfc of lgame_rad_view is not read) and the soil takes the surface short- and
