@@ -594,7 +594,8 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
   const double z_t_const = zc[ns];
   const double rc = g.soil[i1 + dl], tc = g.soil[i1 + 2 * dl], tcs = g.soil[i1 + 3 * dl];
   // diag%sfc_sw_in and sfc_lw_out of the last radiation update (zero without radiation); the
-  // latent heat flux of lakes is zero (the surface phase transitions never run in the reference)
+  // latent heat flux of lakes is zero (nothing in the reference calls calc_h2otracers_source_rates,
+  // mo_phase_trans.F90:22-330, where it is set)
   const int col = (ji - 1) * G.nx + (jk - 1);
   const double sw_in = c.C.rad_sw_in ? c.C.rad_sw_in[col] : 0.0;
   const double lw_out = c.C.rad_lw_out ? c.C.rad_lw_out[col] : 0.0;
