@@ -38,7 +38,10 @@ def test_autotune_keeps_results():
 def test_ensemble_members_match_oracle(mode):
     cfgs = [tiny_config(ny=5, nx=7, n_layers=6, moist=True, order=3),
             tiny_config(ny=9, nx=11, n_layers=8),
-            dry3d_config(n=17, n_layers=10)]
+            dry3d_config(n=17, n_layers=10),
+            # a member with the reference's default physics: diffusion, PBL, prognostic soil
+            dataclasses.replace(tiny_config(ny=7, nx=9, n_layers=8, moist=True), lpbl=True, lprog_soil_temp=True,
+                                lsfc_sensible_heat_flux=True, lsfc_phase_trans=True).set_diffusion()]
     cases = [make_case(c, cloud_frac=0.1 if c.lmoist else 0.0) for c in cfgs]
     steps = []
     for c, (g, st) in zip(cfgs, cases):
