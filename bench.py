@@ -214,7 +214,7 @@ def bench_training(args):
     horizon = int(round(args.train_horizon_hours * 3600.0 / 3.22))
     m, norm, rep = surrogate.build(states=args.train_states, calls=args.train_calls, times=args.train_times,
                                    epochs=args.train_epochs, batch=args.train_batch, lr=args.train_lr, progress=log,
-                                   horizon_steps=horizon)
+                                   horizon_steps=horizon, hidden=args.train_hidden)
     if args.save_surrogate:
         surrogate.save(args.save_surrogate, m, norm)
         log(f"saved {args.save_surrogate}")
@@ -224,7 +224,7 @@ def bench_training(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "surrogate-train: columns from 33x33x60 moist runs"
                                + (f" over {args.train_horizon_hours:g} simulated hours" if horizon else "")
-                               + ", 304 -> 3x512 GELU -> 62",
+                               + f", 304 -> 3x{args.train_hidden} GELU -> 62",
                    "batch": rep["batch"], "columns": rep["train_columns"]},
         "roofline": {"kernel": "k_gemm (all MLP GEMMs of training)", "bound": "mfma", "achieved": rep["train_tflops"],
                      "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": rep["train_tflops"] / BF16_PEAK_TFLOPS,
@@ -445,6 +445,7 @@ def main():
     ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--train-calls", type=int, default=2, help="radiation updates harvested per training run")
     ap.add_argument("--train-times", type=int, default=6, help="times of day per harvested update")
+    ap.add_argument("--train-hidden", type=int, default=512, help="width of the surrogate's hidden layers")
     ap.add_argument("--train-horizon-hours", type=float, default=0.0,
                     help="spread the harvested updates of each run over this many simulated hours")
     ap.add_argument("--save-surrogate", default="")

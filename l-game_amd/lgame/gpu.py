@@ -363,6 +363,7 @@ class Mlp:
             raise LgameError(f"lgame_mlp_create failed ({rc})")
         self.m = m
         self.n_in, self.n_out = n_in, n_out
+        self.hidden, self.n_hidden_layers = hidden, n_hidden_layers
         self.n_params = int(self.lib.lgame_mlp_n_params(m))
 
     def _check(self, rc, what):

@@ -111,7 +111,7 @@ def baseline_rmse(y_train, y_test, levels=LEVELS):
 
 def save(path, m: Mlp, norm):
     np.savez(path, params=m.get_params(), x_mean=norm[0], x_scale=norm[1], y_mean=norm[2], y_scale=norm[3],
-             n_in=m.n_in, n_out=m.n_out, hidden=HIDDEN, n_hidden=N_HIDDEN)
+             n_in=m.n_in, n_out=m.n_out, hidden=m.hidden, n_hidden=m.n_hidden_layers)
 
 
 def load(path) -> Mlp:
@@ -123,7 +123,7 @@ def load(path) -> Mlp:
 
 
 def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, test_states=3, progress=None,
-          horizon_steps=0):
+          horizon_steps=0, hidden=HIDDEN, n_hidden=N_HIDDEN):
     """Harvest, train and evaluate; returns (mlp, norm, report).  The test columns come from
     separate seeded runs over the same horizon."""
     import time
@@ -135,7 +135,8 @@ def build(states=24, calls=2, times=6, epochs=60, batch=8192, lr=1e-3, seed=0, t
                        horizon_steps=horizon_steps)
     t_harvest = time.perf_counter() - t0
     norm = fit_norm(xtr, ytr)
-    m, loss, seconds = train(xtr, ytr, epochs=epochs, batch=batch, lr=lr, seed=seed, norm=norm)
+    m, loss, seconds = train(xtr, ytr, epochs=epochs, batch=batch, lr=lr, seed=seed, norm=norm, hidden=hidden,
+                             n_hidden=n_hidden)
     pred = m.forward(xte)
     n_params = m.n_params
     flops = 6.0 * n_params * len(xtr) * epochs  # forward 2, backward 4 flops per parameter and sample
