@@ -235,21 +235,6 @@ def bench_training(args):
     return 0
 
 
-def ensemble_members(n, seed0=5000):
-    """Member configurations: seeded sizes nx, ny ~ U[64,384] (odd), layers ~ U{40..70}."""
-    from lgame.cases import moist3d_config
-
-    out = []
-    for i in range(n):
-        rng = __import__("numpy").random.default_rng(seed0 + i)
-        nx = int(rng.integers(64, 385)) | 1
-        ny = int(rng.integers(64, 385)) | 1
-        nl = int(rng.integers(40, 71))
-        cfg = moist3d_config(n=nx, n_layers=nl, dtime=3.22, seed=seed0 + i)
-        out.append(dataclasses.replace(cfg, ny=ny, nx=nx))
-    return out
-
-
 def bench_accuracy(args, cfg, grid, st, surface, mlp):
     """The mixed form + surrogate against the plain FP64 form (bitwise equal to the oracle on every
     tested configuration) over args.accuracy_hours simulated hours from the same initial state,
@@ -354,6 +339,8 @@ def bench_ensemble(args):
     from lgame.cases import make_case
     from lgame.config import PREC_MIXED
     from lgame.gpu import Ensemble, Step, grid_setup_device
+
+    from lgame.cases import ensemble_members
 
     cfgs = ensemble_members(args.members)
     steps, cells = [], []

@@ -55,6 +55,21 @@ def moist3d_config(n=513, n_layers=60, dtime=3.22, seed=3, **kw) -> RunConfig:
     return dataclasses.replace(cfg, lmoist=True)
 
 
+def ensemble_members(n, seed0=5000):
+    """The batched-ensemble workload: n moist members with seeded sizes nx, ny ~ U[64,384]
+    (made odd, as the reference grid generator requires) and layers ~ U{40..70}, member i from
+    seed seed0 + i (hills, perturbations and humidity), dx 2 km, dt 3.22 s."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed0 + i)
+        nx = int(rng.integers(64, 385)) | 1
+        ny = int(rng.integers(64, 385)) | 1
+        nl = int(rng.integers(40, 71))
+        cfg = moist3d_config(n=nx, n_layers=nl, dtime=3.22, seed=seed0 + i)
+        out.append(dataclasses.replace(cfg, ny=ny, nx=nx))
+    return out
+
+
 def land_mask(cfg: RunConfig, seed: int = 7):
     """Seeded land/sea mask (set-up extension): land where a smoothed Gaussian field exceeds 0.3."""
     from .init_state import _smooth_periodic

@@ -8,6 +8,7 @@ surface fluxes (include/lgame_gpu.h, LGAME_RAD_N_IN / LGAME_RAD_N_OUT).
 from __future__ import annotations
 
 import dataclasses
+import os
 
 import numpy as np
 
@@ -18,6 +19,8 @@ from .gpu import Mlp, Step
 N_IN, N_OUT, LEVELS = 5 * 60 + 4, 60 + 2, 60
 HIDDEN, N_HIDDEN = 512, 3
 T_EPOCH = 1576843200.0
+# the committed weights of the 304 -> 3x512 -> 62 network
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data", "rad_surrogate.npz")
 
 
 def harvest(n=33, n_states=4, calls=2, times=6, steps_between=4, seed=0, cloud_max=0.01, progress=None,

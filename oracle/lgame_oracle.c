@@ -21,17 +21,39 @@
  * Supported configuration: periodic lateral boundaries, luse_bg_state; diffusion (every switch
  * set in which horizontal temperature and mass diffusion come with horizontal momentum
  * diffusion), the PBL (with lmom_diff_h), the prognostic soil, the surface sensible heat flux
- * (with the soil) and lsfc_phase_trans.  Anything else is rejected by orc_create.  The step is
- * pinned bitwise against the reference's compiled modules driven by oracle/ref_harness.F90
- * (which restates the glue of the modules that do not build here: the step sequence of
- * mo_manage_pchevi.F90 and mo_vector_tend_expl.F90, and mom_diff_h, mom_diff_v and
- * simple_dissipation_rate, the latter pinned by closed forms in the tests) and against golden
- * fixtures written from them.  Radiation: the coupler steps of
- * radiation/mo_rterrtmgp_coupler.F90 are restated; RTE+RRTMGP, which is not in the reference tree,
- * is replaced by the gray two-stream solver that include/lgame_gpu.h specifies (its outputs are
- * pinned by closed forms and flux conservation, not by the reference).
+ * (with the soil) and lsfc_phase_trans.  Anything else is rejected by orc_create.  Radiation: the
+ * coupler steps of radiation/mo_rterrtmgp_coupler.F90 are restated; RTE+RRTMGP, which is not in the
+ * reference tree, is replaced by the gray two-stream solver that include/lgame_gpu.h specifies.
+ *
+ * How each part is pinned (tests/ file: what it checks):
+ *   operators (grad_*, div_h*, add_vertical_div, stv_*, theta_v_adv_3rd_order, inner_product,
+ *   w_free_slip, rel_vort, calc_pot_vort, calc_vorticity_flux_term), temperature_diagnostics and
+ *   the derived quantities, manage_p_grad, p_grad_condensates_v, scalar_tend_expl,
+ *   three_band_solver_ver (with the soil rows and the sensible heat flux),
+ *   three_band_solver_gen_densities, update_n_squared, tke_update, hor_viscosity and its dual-point
+ *   postprocessing, scalar_diff_coeffs, the PBL functions, linear_combine_two_states:
+ *       test_oracle_vs_reference.py -- bitwise against the reference's own modules, compiled
+ *       from their sources (oracle/Makefile, oracle/ref_harness.F90), on 23 configurations;
+ *       test_golden.py -- bitwise against fixtures written from that build;
+ *   thomas: test_oracle_properties.py (the reference routine against a dense solve);
+ *   pchevi and the tendency combination of vector_tend_expl (restated glue):
+ *       test_oracle_pinning.py -- the budgets of the whole reference program's Schaer run,
+ *       the pressure-gradient extrapolation weights from c_v/c_p applied to the compiled
+ *       manage_p_grad, the inertial oscillation recurrence (averaged, lagged Coriolis term), and
+ *       the energy neutrality of the vorticity flux with the mass fluxes formed here;
+ *   mom_diff_h: test_oracle_properties.py (second difference of a shear flow);
+ *   mom_diff_v: test_oracle_pinning.py (second derivative of a quadratic w(z));
+ *   simple_dissipation_rate: test_oracle_pinning.py (shear flow and Klemp layer in closed form);
+ *   budget sums (orc_integrals): test_oracle_properties.py (uniform states); mass conservation
+ *       and the resting atmosphere: test_oracle_properties.py;
+ *   the radiation coupler steps, coszenith and the sea/soil surface temperature:
+ *       test_oracle_radiation.py (closed forms, an independent formulation of the two-stream
+ *       sums, flux conservation);
+ *   the grids the oracle runs on (l-game_amd/lgame/grid.py): test_grid_closed_forms.py.
+ * Parity unpinned: RTE+RRTMGP itself (absent; the gray solver stands in for it).
  */
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2276,37 +2298,64 @@ int orc_get_state(oracle *o, int slot, const char *name, double *dst) {
   return 0;
 }
 
+/* The diagnostic fields readable (and, for the closed-form checks, writable) by name. */
+static F3 *diag_field(oracle *o, const char *name) {
+  static const struct {
+    const char *name;
+    size_t off;
+  } map[] = {
+#define DF(n, m) {n, offsetof(oracle, m)}
+      DF("temperature", temperature), DF("heating_diss", heating_diss), DF("pot_vort_tend_x", pvt_x),
+      DF("pot_vort_tend_y", pvt_y), DF("pot_vort_tend_z", pvt_z), DF("p_grad_acc_old_u", pold_u),
+      DF("p_grad_acc_old_v", pold_v), DF("p_grad_acc_neg_nl_u", pnl_u), DF("p_grad_acc_neg_nl_v", pnl_v),
+      DF("p_grad_acc_neg_nl_w", pnl_w), DF("p_grad_acc_neg_l_u", pl_u), DF("p_grad_acc_neg_l_v", pl_v),
+      DF("p_grad_acc_neg_l_w", pl_w), DF("p_grad_condensates_w", pgc_w), DF("v_squared_grad_x", vsg_x),
+      DF("v_squared_grad_y", vsg_y), DF("v_squared_grad_z", vsg_z), DF("u_placeholder", u_ph),
+      DF("v_placeholder", v_ph), DF("w_placeholder", w_ph), DF("eta_x", eta_x), DF("eta_y", eta_y),
+      DF("eta_z", eta_z), DF("tend_wind_u", tend_u), DF("tend_wind_v", tend_v), DF("tend_wind_w", tend_w),
+      DF("condensates_sediment_heat", condensates_sediment_heat), DF("radiation_tendency", radiation_tendency),
+      DF("tke", tke), DF("n_squared", n_squared), DF("viscosity", viscosity),
+      DF("viscosity_molecular", viscosity_molecular), DF("mom_diff_tend_x", mom_x), DF("mom_diff_tend_y", mom_y),
+      DF("mom_diff_tend_z", mom_z), DF("temp_diff_heating", temp_diff_heating), DF("roughness_length", z0),
+      DF("roughness_velocity", ustar), DF("monin_obukhov_length", mo_len), DF("scalar_flux_resistance", sfr),
+      DF("power_flux_density_sens_soil", pfd_soil), DF("power_flux_density_sens_sea", pfd_sea),
+#undef DF
+  };
+  for (size_t q = 0; q < sizeof(map) / sizeof(map[0]); ++q)
+    if (!strcmp(name, map[q].name)) return (F3 *)((char *)o + map[q].off);
+  return NULL;
+}
+
 int orc_get_diag(oracle *o, const char *name, double *dst) {
-  F3 *f = NULL;
-  if (!strcmp(name, "temperature")) f = &o->temperature;
-  if (!strcmp(name, "heating_diss")) f = &o->heating_diss;
-  if (!strcmp(name, "pot_vort_tend_x")) f = &o->pvt_x;
-  if (!strcmp(name, "pot_vort_tend_y")) f = &o->pvt_y;
-  if (!strcmp(name, "pot_vort_tend_z")) f = &o->pvt_z;
-  if (!strcmp(name, "p_grad_acc_old_u")) f = &o->pold_u;
-  if (!strcmp(name, "p_grad_acc_old_v")) f = &o->pold_v;
-  if (!strcmp(name, "tend_wind_w")) f = &o->tend_w;
-  if (!strcmp(name, "condensates_sediment_heat")) f = &o->condensates_sediment_heat;
-  if (!strcmp(name, "radiation_tendency")) f = &o->radiation_tendency;
-  if (!strcmp(name, "tke")) f = &o->tke;
-  if (!strcmp(name, "n_squared")) f = &o->n_squared;
-  if (!strcmp(name, "viscosity")) f = &o->viscosity;
-  if (!strcmp(name, "mom_diff_tend_x")) f = &o->mom_x;
-  if (!strcmp(name, "mom_diff_tend_y")) f = &o->mom_y;
-  if (!strcmp(name, "mom_diff_tend_z")) f = &o->mom_z;
-  if (!strcmp(name, "temp_diff_heating")) f = &o->temp_diff_heating;
-  if (!strcmp(name, "roughness_length")) f = &o->z0;
-  if (!strcmp(name, "roughness_velocity")) f = &o->ustar;
-  if (!strcmp(name, "monin_obukhov_length")) f = &o->mo_len;
-  if (!strcmp(name, "scalar_flux_resistance")) f = &o->sfr;
-  if (!strcmp(name, "power_flux_density_sens_soil")) f = &o->pfd_soil;
-  if (!strcmp(name, "power_flux_density_sens_sea")) f = &o->pfd_sea;
   if (!strcmp(name, "mass_diff_tendency")) {
     memcpy(dst, o->mass_diff_tendency, sizeof(double) * (size_t)o->ny * o->nx * o->nl * o->nc);
     return 0;
   }
+  F3 *f = diag_field(o, name);
   if (!f) return 1;
   memcpy(dst, f->p, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
+  return 0;
+}
+
+/* Test hooks of the closed-form checks: a diagnostic field set by name, and one subroutine of the
+   step run on a state slot with the diagnostics as they stand. */
+int orc_set_diag(oracle *o, const char *name, const double *src) {
+  F3 *f = diag_field(o, name);
+  if (!f) return 1;
+  memcpy(f->p, src, sizeof(double) * (size_t)f->n1 * f->n2 * f->n3);
+  return 0;
+}
+
+int orc_call(oracle *o, const char *name, int slot, int rk_step, int total_step_counter) {
+  if (slot < 0 || slot > 1) return 2;
+  ostate *s = &o->st[slot];
+  if (!strcmp(name, "temperature_diagnostics")) temperature_diagnostics(o, s);
+  else if (!strcmp(name, "manage_p_grad")) manage_p_grad(o, s, total_step_counter == 0);
+  else if (!strcmp(name, "vector_tend_expl")) vector_tend_expl(o, s, rk_step, total_step_counter);
+  else if (!strcmp(name, "mom_diff_v")) mom_diff_v(o, s);
+  else if (!strcmp(name, "simple_dissipation_rate")) simple_dissipation_rate(o, s);
+  else if (!strcmp(name, "calc_pot_vort")) calc_pot_vort(o, s);
+  else return 1;
   return 0;
 }
 

@@ -32,7 +32,10 @@ REF_DIAG_IDS = {"temperature": 1, "heating_diss": 2, "pot_vort_tend_x": 3, "pot_
                 "mom_diff_tend_z": 9, "temp_diff_heating": 10, "mass_diff_tendency": 11, "viscosity": 12,
                 "n_squared": 13, "roughness_length": 14, "roughness_velocity": 15, "monin_obukhov_length": 16,
                 "scalar_flux_resistance": 17, "power_flux_density_sens_soil": 18,
-                "power_flux_density_sens_sea": 19}
+                "power_flux_density_sens_sea": 19, "p_grad_acc_neg_nl_u": 20, "p_grad_acc_neg_nl_v": 21,
+                "p_grad_acc_neg_nl_w": 22, "p_grad_acc_neg_l_u": 23, "p_grad_acc_neg_l_v": 24,
+                "p_grad_acc_neg_l_w": 25, "p_grad_acc_old_u": 26, "p_grad_acc_old_v": 27,
+                "p_grad_condensates_w": 28}
 
 
 def _ptr(a):
@@ -62,6 +65,10 @@ class CpuOracle:
             lib.orc_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
             lib.orc_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
             lib.orc_get_diag.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_set_diag.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_set_diag.restype = ctypes.c_int
+            lib.orc_call.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            lib.orc_call.restype = ctypes.c_int
             lib.orc_finish_grid.argtypes = [ctypes.c_void_p]
             lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
             lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -123,6 +130,20 @@ class CpuOracle:
         if self.lib().orc_get_diag(self.h, name.encode(), _ptr(a)) != 0:
             raise KeyError(name)
         return a
+
+    def set_diag(self, name: str, a):
+        """A diagnostic field overwritten (test hook of the closed-form checks)."""
+        a = _fortran(a)
+        if self.lib().orc_set_diag(self.h, name.encode(), _ptr(a)) != 0:
+            raise KeyError(name)
+
+    def call(self, name: str, slot: int, rk_step: int = 1, counter: int = 1):
+        """One subroutine of the step on a state slot (test hook of the closed-form checks):
+        temperature_diagnostics, manage_p_grad (lfirst when counter is 0), vector_tend_expl,
+        mom_diff_v, simple_dissipation_rate, calc_pot_vort."""
+        lib = self.lib()
+        if lib.orc_call(self.h, name.encode(), slot, rk_step, counter) != 0:
+            raise KeyError(name)
 
     def linear_combine(self, slot_1: int, slot_2: int, coeff_1: float, coeff_2: float) -> dict:
         """coeff_1*slot_1 + coeff_2*slot_2 (linear_combine_two_states) as a state dict."""
@@ -244,6 +265,16 @@ class RefHarness:
 
     def run(self, first: int, nsteps: int):
         self.lib.ref_run(first, nsteps)
+
+    def set_state(self, slot: int, st: dict):
+        for k, v in st.items():
+            a = _fortran(v)
+            self.lib.ref_state_io(slot, REF_STATE_IDS[k], 0, _ptr(a))
+
+    def manage_p_grad(self, slot: int, lfirst: bool):
+        """The reference's manage_p_grad and calc_p_grad_condensates_v on a state slot."""
+        self.lib.ref_manage_p_grad.argtypes = [ctypes.c_int, ctypes.c_int]
+        self.lib.ref_manage_p_grad(slot, int(lfirst))
 
     def get_state(self, slot: int) -> dict:
         from lgame.config import state_shape

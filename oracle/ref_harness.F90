@@ -341,7 +341,9 @@ contains
   ! diag field ids: 1 temperature, 2 heating_diss, 3 pot_vort_tend_x, 4 pot_vort_tend_z,
   ! 5 condensates_sediment_heat, 6 tke, 7 mom_diff_tend_x, 8 mom_diff_tend_y, 9 mom_diff_tend_z,
   ! 10 temp_diff_heating, 11 mass_diff_tendency, 12 viscosity, 13 n_squared, 14 roughness_length,
-  ! 15 roughness_velocity, 16 monin_obukhov_length
+  ! 15 roughness_velocity, 16 monin_obukhov_length, 17 scalar_flux_resistance,
+  ! 18 power_flux_density_sens_soil, 19 power_flux_density_sens_sea, 20-22 p_grad_acc_neg_nl_u/v/w,
+  ! 23-25 p_grad_acc_neg_l_u/v/w, 26-27 p_grad_acc_old_u/v, 28 p_grad_condensates_w
   subroutine ref_get_diag(id,buf) bind(C,name="ref_get_diag")
     integer(c_int), value, intent(in) :: id
     type(c_ptr),    value, intent(in) :: buf
@@ -375,6 +377,24 @@ contains
                 p = reshape(diag%power_flux_density_sens_soil,[size(diag%power_flux_density_sens_soil)])
       case(19); call c_f_pointer(buf,p,[size(diag%power_flux_density_sens_sea)])
                 p = reshape(diag%power_flux_density_sens_sea,[size(diag%power_flux_density_sens_sea)])
+      case(20); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_nl_u)])
+                p = reshape(diag%p_grad_acc_neg_nl_u,[size(diag%p_grad_acc_neg_nl_u)])
+      case(21); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_nl_v)])
+                p = reshape(diag%p_grad_acc_neg_nl_v,[size(diag%p_grad_acc_neg_nl_v)])
+      case(22); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_nl_w)])
+                p = reshape(diag%p_grad_acc_neg_nl_w,[size(diag%p_grad_acc_neg_nl_w)])
+      case(23); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_l_u)])
+                p = reshape(diag%p_grad_acc_neg_l_u,[size(diag%p_grad_acc_neg_l_u)])
+      case(24); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_l_v)])
+                p = reshape(diag%p_grad_acc_neg_l_v,[size(diag%p_grad_acc_neg_l_v)])
+      case(25); call c_f_pointer(buf,p,[size(diag%p_grad_acc_neg_l_w)])
+                p = reshape(diag%p_grad_acc_neg_l_w,[size(diag%p_grad_acc_neg_l_w)])
+      case(26); call c_f_pointer(buf,p,[size(diag%p_grad_acc_old_u)])
+                p = reshape(diag%p_grad_acc_old_u,[size(diag%p_grad_acc_old_u)])
+      case(27); call c_f_pointer(buf,p,[size(diag%p_grad_acc_old_v)])
+                p = reshape(diag%p_grad_acc_old_v,[size(diag%p_grad_acc_old_v)])
+      case(28); call c_f_pointer(buf,p,[size(diag%p_grad_condensates_w)])
+                p = reshape(diag%p_grad_condensates_w,[size(diag%p_grad_condensates_w)])
     end select
   end subroutine ref_get_diag
 
@@ -692,6 +712,14 @@ contains
     real(c_double), value, intent(in) :: coeff_1,coeff_2
     call linear_combine_two_states(sel(slot_1),sel(slot_2),st3,coeff_1,coeff_2,grid)
   end subroutine ref_linear_combine
+
+  ! the reference's manage_p_grad (mo_p_grad.F90:20-104) and calc_p_grad_condensates_v
+  ! (:106-120) on a state slot, with the diagnostics as they stand
+  subroutine ref_manage_p_grad(slot,lfirst) bind(C,name="ref_manage_p_grad")
+    integer(c_int), value, intent(in) :: slot,lfirst
+    call manage_p_grad(sel(slot),diag,grid,lfirst/=0)
+    call calc_p_grad_condensates_v(sel(slot),diag,grid)
+  end subroutine ref_manage_p_grad
 
   ! state_2 = state_1 (control.F90:500)
   subroutine ref_copy_slot0_to_1() bind(C,name="ref_copy_slot0_to_1")

@@ -141,3 +141,33 @@ def test_oracle_step_applies_radiation():
     assert np.abs(rt).max() > 0.0
     d = np.abs(runs[1]["rhotheta_v"] - runs[0]["rhotheta_v"]).max()
     assert d > 0.0
+
+
+def test_oracle_radiation_surface_temperature_with_the_soil():
+    """With the prognostic soil the radiation sees the surface temperature of the sea and the soil
+    (mo_manage_radiation_calls.F90:36-42): the sea surface temperature where there is neither land
+    nor lake, the soil's top layer where land and lake cover the column, and in between the mean
+    weighted by the sea fraction (the temperature clipping of the coupler, 160-350 K, does not
+    act on these values)."""
+    cfg = dataclasses.replace(tiny_config(ny=5, nx=6), lrad=True, rad_mode=RAD_TEACHER, dtime_rad=10.0,
+                              lprog_soil_temp=True)
+    g, st = make_case(cfg)
+    ny, nx, nl = cfg.ny, cfg.nx, cfg.n_layers
+    land = np.zeros((ny, nx))
+    land[1] = 1.0
+    land[2] = 0.25
+    lake = np.zeros((ny, nx))
+    lake[2] = 0.35
+    lake[3] = 1.0
+    g["land_fraction"], g["lake_fraction"] = np.asfortranarray(land), np.asfortranarray(lake)
+    g["sst"] = np.asfortranarray(np.full((ny, nx), 291.5))
+    st["temperature_soil"][:, :, 0] = 283.25
+    o = CpuOracle(cfg, g)
+    o.start(st)
+    o.set_rad_surface(radiation_surface(cfg, g, st))
+    x, _ = o.rad_columns(0, T_EPOCH)
+    tsfc = x[:, 5 * nl].reshape(nx, ny).T  # columns in Fortran order (ji fastest)
+    np.testing.assert_array_equal(tsfc[0], 291.5)  # open sea
+    np.testing.assert_array_equal(tsfc[1], 283.25)  # land
+    np.testing.assert_array_equal(tsfc[3], 283.25)  # lake
+    np.testing.assert_allclose(tsfc[2], 0.4 * 291.5 + 0.6 * 283.25, rtol=1e-14)
