@@ -98,11 +98,12 @@ extern "C" {
 #define LGAME_RAD_K_SW_ICE        0.01        /* m^2 g^-1                                        */
 
 /*
- * Surrogate column interface (n_layers = 60): inputs are 5 profiles of 60 layers (temperature
- * [K], pressure [Pa], vapour, liquid and ice mixing ratios [kg/kg]; liquid = rain + cloud water,
- * ice = snow + cloud ice + graupel) followed by 4 scalars (surface temperature [K], mu_0,
- * surface albedo, surface pressure [Pa]); outputs are the 60 radiative heating rates [K/day]
- * followed by sfc_sw_in and sfc_lw_out [W m^-2].
+ * Surrogate column interface (LGAME_RAD_LEVELS = 60 levels): inputs are 5 profiles of 60 levels
+ * (temperature [K], pressure [Pa], vapour, liquid and ice mixing ratios [kg/kg]; liquid = rain +
+ * cloud water, ice = snow + cloud ice + graupel) followed by 4 scalars (surface temperature [K],
+ * mu_0, surface albedo, surface pressure [Pa]); outputs are the 60 radiative heating rates [K/day]
+ * followed by sfc_sw_in and sfc_lw_out [W m^-2].  A domain of 60 layers maps them one to one;
+ * another layer count is resampled linearly in the layer index (lgame_rad_attach_mlp).
  */
 #define LGAME_RAD_LEVELS     60
 #define LGAME_RAD_N_IN       (5 * LGAME_RAD_LEVELS + 4)
@@ -269,6 +270,29 @@ typedef struct lgame_rad_view {
 #define LGAME_INT_INTERNAL   3   /* sum(c_d_v*rho_g*temperature*volume)                     */
 #define LGAME_INT_MASS0      4   /* masses: slot LGAME_INT_MASS0 + jc-1 for constituent jc  */
 
+/*
+ * Checkpoint of the fields the step carries from one call to the next besides the two state
+ * slots: the pressure-gradient terms of the last first substep (their negated sum is the next
+ * step's old gradient, mo_p_grad.F90:38-43), the lagged momentum advection and the kinetic energy
+ * (computed only at the corrector, mo_vector_tend_expl.F90:58; the surface turbulence of the next
+ * step reads the latter, mo_pbl.F90:19-109), the sediment heating, the dissipative heating (the next
+ * step's TKE production, mo_tke.F90:19-67), the radiation tendency and the
+ * surface fluxes of the last radiation update with the count of updates (which picks the audit
+ * sample), and with diffusion or the surface physics the TKE, the viscosity, the momentum-
+ * diffusion tendencies, the diffusive heating and mass tendencies and the surface turbulence
+ * quantities.  Saved together with the two states (lgame_get_state) and loaded into a handle of
+ * the same configuration (lgame_set_state, lgame_checkpoint_load), a run continues bit for bit at
+ * the next step counter.  The blob is opaque (device layout and precision of the handle) and
+ * checked on load (LGAME_EINVAL for another configuration or a short buffer).
+ * No counterpart in the reference: its restart (restart, mo_set_initial_state.F90:259, called
+ * at control.F90:470-471) reads the state only and starts the time loop at counter 0 again
+ * (control.F90:506), so it recomputes the old pressure gradient and the advection from the
+ * restart state; callers that need a restart to reproduce an uninterrupted run use this.
+ */
+int lgame_checkpoint_size(lgame_handle *h, long long *bytes);
+int lgame_checkpoint_save(lgame_handle *h, void *buf, long long bytes);
+int lgame_checkpoint_load(lgame_handle *h, const void *buf, long long bytes);
+
 /* Diagnostic fields readable with lgame_get_diag (reference t_diag components). */
 #define LGAME_DIAG_TEMPERATURE     0   /* diag%temperature (ny,nx,nl), of the last step's state_old */
 #define LGAME_DIAG_TKE             1   /* diag%tke (ny,nx,nl)                                       */
@@ -374,13 +398,20 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
 
 /*
  * Strict drop-in form: upload state_old and state_new, run one step, download state_new.
- * Mirrors the reference call signature with host arrays; costs two PCIe round trips.
+ * Mirrors the reference call signature with host arrays; costs two PCIe round trips.  Returns
+ * LGAME_ENONFINITE when the step produced a non-finite value (as lgame_sync).
  */
 int lgame_pchevi_host(lgame_handle *h, const lgame_state_view *state_old,
                       lgame_state_view *state_new, int total_step_counter, int lrad_update,
                       double t_0);
 
-/* Wait for all work queued on the handle's stream. */
+/*
+ * Wait for all work queued on the handle's stream.  Returns LGAME_ENONFINITE if any step since
+ * the last lgame_set_state produced a NaN or Inf in the new state: the corrector's column solvers
+ * check every value they write (the densities, rhotheta_v, Exner pressure, theta_v and w), at no
+ * extra memory traffic.  This is the reference's crash check (mo_write_out.F90:63-67, at every
+ * output) made per step; the driver prints lgame_last_error and stops, as the reference does.
+ */
 int lgame_sync(lgame_handle *h);
 
 /*
@@ -467,11 +498,17 @@ int lgame_rad_get(lgame_handle *h, int which, double *out);
 typedef struct lgame_mlp lgame_mlp;
 
 /*
- * Use a trained MLP (n_in = LGAME_RAD_N_IN, n_out = LGAME_RAD_N_OUT, n_layers = 60) as the
- * radiation of a handle configured with LGAME_RAD_SURROGATE.  At every update a fraction
- * audit_fraction of the columns (a different sample each call, from seed) is recomputed with the
- * solver; an audited column whose heating-rate RMSE exceeds tol_k_per_day takes the solver's
- * result.  The handle keeps a pointer to m: m must outlive the handle's use of it.
+ * Use a trained MLP (n_in = LGAME_RAD_N_IN, n_out = LGAME_RAD_N_OUT, with its normalisation set
+ * by lgame_mlp_set_norm, else LGAME_EINVAL) as the radiation of a handle configured with
+ * LGAME_RAD_SURROGATE.  The network works on LGAME_RAD_LEVELS levels: a domain of another layer
+ * count is resampled, linearly in the layer index, to the network's levels and its heating rates
+ * back to the domain's layers.  At every update a fraction audit_fraction of the columns (a
+ * different sample each call, from seed) is recomputed with the solver; an audited column whose
+ * heating-rate RMSE exceeds tol_k_per_day takes the solver's result, and so does every column whose
+ * prediction leaves LGAME_RAD_ENVELOPE standard deviations of the training targets.  The handle
+ * allocates its own inference buffers here (about 260 MB for 65,536 or more columns), so the step
+ * allocates nothing and handles may share one network (weights are only read; training it while
+ * handles use it is not allowed).  The handle keeps a pointer to m: m must outlive its use.
  */
 int lgame_rad_attach_mlp(lgame_handle *h, lgame_mlp *m, double tol_k_per_day, double audit_fraction, unsigned long long seed);
 

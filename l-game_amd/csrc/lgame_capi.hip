@@ -30,7 +30,7 @@ struct lgame_handle {
   double *staging = nullptr;     // device staging for host<->device transposes
   size_t staging_elems = 0;
   double *red = nullptr;         // reduction partials
-  unsigned long long *counter = nullptr;
+  unsigned long long *counter = nullptr;  // [0] lgame_check_finite's count, [1] the step's non-finite flag
   double *trsk = nullptr;        // TRSK weights, FP64 then FP32
   double *soil_z = nullptr;      // soil grid: interfaces, centres, z_t_const (with the prognostic soil)
   hipStream_t stream = nullptr;
@@ -46,8 +46,11 @@ struct lgame_handle {
   float *rad_x = nullptr, *rad_y = nullptr;
   unsigned long long *rad_counts = nullptr;  // audited, fallbacks (device)
   double *rad_sq = nullptr;                  // squared-error sum (device)
+  int *rad_list = nullptr;                   // the audit pass's columns (device)
+  unsigned *rad_list_count = nullptr;
   int rad_surface = 0;
   lgame_mlp *mlp = nullptr;
+  MlpWork *mlp_work = nullptr;  // this handle's inference buffers (made by lgame_rad_attach_mlp)
   double rad_tol = 0.0;
   unsigned rad_thresh = 0;
   unsigned long long rad_seed = 0, rad_calls = 0;
@@ -280,7 +283,7 @@ inline dim3 grid3(int nxr, int nyr, int nzr) { return dim3((nxr + BX - 1) / BX, 
 namespace lgame {
 
 template <class R>
-void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
+int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
                     const RadCall *rad, const Tiles &tiles) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const dim3 g_col((nx * ny + BCOL - 1) / BCOL), b_col(BCOL);
@@ -312,8 +315,9 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
   // update_rad_fluxes with state_old (mo_manage_pchevi.F90:56-58)
   if (rad) {
     if (prof) prof->begin(3);
-    enqueue_radiation(c, old, *rad, st);
+    const int rc = enqueue_radiation(c, old, *rad, st);
     if (prof) prof->end(3);
+    if (rc) return rc;
   }
   for (int rk = 1; rk <= 2; ++rk) {
     const StateDev &S = rk == 1 ? old : nw;
@@ -380,11 +384,12 @@ void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, i
   }
 #undef LG_LAUNCH
 #undef LG_LAUNCH_ZFAST
+  return 0;
 }
 
-template void enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
+template int enqueue_pchevi<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
                                      const RadCall *, const Tiles &);
-template void enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
+template int enqueue_pchevi<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, hipStream_t, ProfHook *,
                                     const RadCall *, const Tiles &);
 
 }  // namespace lgame
@@ -445,6 +450,8 @@ RadDev rad_dev(lgame_handle *h) {
   r.ypred = h->rad_y;
   r.counts = h->rad_counts;
   r.sq = h->rad_sq;
+  r.list = h->rad_list;
+  r.list_count = h->rad_list_count;
   r.tol = h->rad_tol;
   r.audit_thresh = h->rad_thresh;
   if (h->mlp) mlp_out_norm(h->mlp, &r.y_mean, &r.y_scale);
@@ -456,8 +463,13 @@ int rad_alloc(lgame_handle *h) {
   const size_t ncol = (size_t)h->dims.ny * h->dims.nx, nl = h->dims.n_layers;
   HIPCHK(h, hipMalloc(&h->rad_mem, 6 * ncol * sizeof(double)));
   HIPCHK(h, hipMemset(h->rad_mem, 0, 6 * ncol * sizeof(double)));
-  HIPCHK(h, hipMalloc(&h->rad_x, ncol * (5 * nl + 4) * sizeof(float)));
-  HIPCHK(h, hipMalloc(&h->rad_y, ncol * (nl + 2) * sizeof(float)));
+  // columns of the harvest (5 nl + 4 inputs, nl + 2 outputs) or of the network (its own sizes)
+  const size_t nin = 5 * nl + 4 > LGAME_RAD_N_IN ? 5 * nl + 4 : LGAME_RAD_N_IN;
+  const size_t nout = nl + 2 > LGAME_RAD_N_OUT ? nl + 2 : LGAME_RAD_N_OUT;
+  HIPCHK(h, hipMalloc(&h->rad_x, ncol * nin * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->rad_y, ncol * nout * sizeof(float)));
+  HIPCHK(h, hipMalloc(&h->rad_list, ncol * sizeof(int)));
+  HIPCHK(h, hipMalloc(&h->rad_list_count, sizeof(unsigned)));
   HIPCHK(h, hipMalloc(&h->rad_counts, 2 * sizeof(unsigned long long)));
   HIPCHK(h, hipMemset(h->rad_counts, 0, 2 * sizeof(unsigned long long)));
   HIPCHK(h, hipMalloc(&h->rad_sq, sizeof(double)));
@@ -571,8 +583,6 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     return fail(nullptr, LGAME_EINVAL, "unknown radiation mode %d", f.rad_mode);
   if (f.lrad && f.rad_mode == LGAME_RAD_OFF) return fail(nullptr, LGAME_EINVAL, "lrad needs a radiation mode (LGAME_RAD_*)");
   if (f.lrad && !(f.dtime_rad > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime_rad must be positive");
-  if (f.lrad && f.rad_mode == LGAME_RAD_SURROGATE && d.n_layers != LGAME_RAD_LEVELS)
-    return fail(nullptr, LGAME_EINVAL, "the radiation surrogate needs n_layers = %d", LGAME_RAD_LEVELS);
   if (f.precision != LGAME_PREC_FP64 && f.precision != LGAME_PREC_MIXED)
     return fail(nullptr, LGAME_EINVAL, "unknown precision mode %d", f.precision);
   if (!(f.dtime > 0.0) || !(f.toa > 0.0)) return fail(nullptr, LGAME_EINVAL, "dtime and toa must be positive");
@@ -798,7 +808,9 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   h->staging_elems = (size_t)d.ny * d.nx * d.n_layers * (nc > 6 ? nc : 6) + (size_t)(d.ny + 1) * (d.nx + 1) * (d.n_layers + 1);
   if (e == hipSuccess) e = hipMalloc(&h->staging, h->staging_elems * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->red, (size_t)2 * NRED * RED_GRID * sizeof(double) + NRED * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&h->counter, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemsetAsync(h->counter, 0, 2 * sizeof(unsigned long long), h->stream);
+  h->c.C.nonfinite = h->counter + 1;
   if (e == hipSuccess) e = hipMalloc(&h->trsk, (size_t)(3 * d.ny + 2 * (d.ny + 1)) * (sizeof(double) + sizeof(float)));
   if (e != hipSuccess) {
     fail(nullptr, LGAME_ENOMEM, "device allocation: %s", hipGetErrorString(e));
@@ -1026,7 +1038,9 @@ int lgame_linear_combine(lgame_handle *h, int slot_1, int slot_2, double coeff_1
 }
 
 int lgame_set_state(lgame_handle *h, int slot, const lgame_state_view *state) {
-  return state_io(h, slot, const_cast<lgame_state_view *>(state), true);
+  const int rc = state_io(h, slot, const_cast<lgame_state_view *>(state), true);
+  if (!rc) HIPCHK(h, hipMemsetAsync(h->counter + 1, 0, sizeof(unsigned long long), h->stream));  // a new state
+  return rc;
 }
 
 int lgame_get_state(lgame_handle *h, int slot, lgame_state_view *state) {
@@ -1052,16 +1066,19 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
     rc.dev = rad_dev(h);
     rc.dev.call = h->rad_calls;
     rc.mlp = h->mlp;
+    rc.work = h->mlp_work;
     rad = &rc;
     ++h->rad_calls;
   }
   EventProf ep(h);
   ProfHook *ph = h->prof ? &ep : nullptr;
   if (ph) ep.begin(LGAME_PROF_STEP);
+  int erc;
   if (h->mixed)
-    enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
+    erc = enqueue_pchevi(h->cf, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
   else
-    enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
+    erc = enqueue_pchevi(h->c, h->slot[slot_old], h->slot[slot_new], total_step_counter, h->stream, ph, rad, h->tiles);
+  if (erc) return fail(h, erc, "the radiation update could not be enqueued (%s)", hipGetErrorString(hipGetLastError()));
   if (ph) {
     ep.end(LGAME_PROF_STEP);
     if (ep.failed) return fail(h, LGAME_EHIP, "event recording failed");
@@ -1071,6 +1088,8 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
   return LGAME_OK;
 }
 
+static int nonfinite_check(lgame_handle *h);
+
 int lgame_pchevi_host(lgame_handle *h, const lgame_state_view *state_old, lgame_state_view *state_new, int total_step_counter,
                       int lrad_update, double t_0) {
   const int a = total_step_counter % 2 == 0 ? 0 : 1, b = 1 - a;
@@ -1078,13 +1097,23 @@ int lgame_pchevi_host(lgame_handle *h, const lgame_state_view *state_old, lgame_
   if (!rc) rc = lgame_set_state(h, b, state_new);
   if (!rc) rc = lgame_pchevi(h, a, b, total_step_counter, lrad_update, t_0);
   if (!rc) rc = lgame_get_state(h, b, state_new);
+  if (!rc) rc = nonfinite_check(h);
   return rc;
+}
+
+// The step's own check (mo_write_out.F90:63-67 checks the state for NaN at every output): the
+// corrector's column solvers flag every new state value that is not finite.
+static int nonfinite_check(lgame_handle *h) {
+  unsigned long long flag = 0;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(&flag, h->counter + 1, sizeof flag, hipMemcpyDeviceToHost));
+  if (flag) return fail(h, LGAME_ENONFINITE, "a step produced a non-finite value in the state (the model crashed)");
+  return LGAME_OK;
 }
 
 int lgame_sync(lgame_handle *h) {
   if (!h) return fail(h, LGAME_EINVAL, "null handle");
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return LGAME_OK;
+  return nonfinite_check(h);
 }
 
 int lgame_integrals(lgame_handle *h, int slot, double *out) {
@@ -1112,6 +1141,129 @@ int lgame_integrals(lgame_handle *h, int slot, double *out) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   tmp[LGAME_INT_KINETIC] = 0.5 * tmp[LGAME_INT_KINETIC];
   std::memcpy(out, tmp, nq * sizeof(double));
+  return LGAME_OK;
+}
+
+// ---- checkpoint of the carried fields ------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// The device fields a step carries to the next besides the two state slots, each with the bytes
+// it occupies: the pressure-gradient terms whose negated sum becomes the next step's old
+// gradient (mo_p_grad.F90:38-43), the lagged momentum advection with the mass fluxes and the
+// kinetic energy it was formed from (mo_vector_tend_expl.F90:58; the kinetic energy of the lowest
+// layer also drives the next step's surface turbulence, mo_pbl.F90:19-109), the sediment
+// heating, the dissipative heating (the
+// next step's TKE production, mo_tke.F90:19-67), the radiation tendency, and with diffusion or
+// the surface physics the TKE, viscosity, momentum-diffusion tendencies, heating and
+// mass-diffusion tendencies and the surface turbulence quantities; then the radiation's surface
+// fluxes.
+struct Carried {
+  void *p;
+  size_t bytes;
+};
+
+std::vector<Carried> carried_fields(lgame_handle *h) {
+  std::vector<Carried> v;
+  const size_t es = h->mixed ? sizeof(float) : sizeof(double);
+  const size_t box = (size_t)h->c.G.fsize * es, lvl = (size_t)h->c.G.lvl * es;
+  auto add_work = [&](const auto &w) {
+    for (auto f : {w.pnl_u, w.pnl_v, w.pnl_w, w.pl_u, w.pl_v, w.pl_w, w.pold_u, w.pold_v, w.vsg_x, w.vsg_y, w.vsg_z, w.pvt_x,
+                   w.pvt_y, w.pvt_z, w.u_ph, w.v_ph, w.w_ph, w.v_squared, w.csh, w.heating_diss, w.radiation_tendency, w.tke,
+                   w.visc, w.mom_x, w.mom_y, w.mom_z, w.tdh})
+      if (f.p) v.push_back({(void *)f.p, box});
+    for (int jc = 0; jc < h->dims.n_constituents; ++jc)
+      if (w.mdt[jc].p) v.push_back({(void *)w.mdt[jc].p, box});
+    for (auto f : {w.sfc, w.sfc2})
+      if (f.p) v.push_back({(void *)f.p, 3 * lvl});
+  };
+  if (h->mixed)
+    add_work(h->cf.w);
+  else
+    add_work(h->c.w);
+  if (h->rad_mem) {
+    const size_t ncol = (size_t)h->dims.ny * h->dims.nx;
+    v.push_back({(void *)(h->rad_mem + 4 * ncol), 2 * ncol * sizeof(double)});
+  }
+  return v;
+}
+
+// header of a checkpoint: what it was made from, checked on load
+struct CkptHeader {
+  char magic[8];
+  int abi, precision, ny, nx, nl, nc, nfields, has_rad;
+  long long payload;
+  unsigned long long rad_calls;
+};
+
+long long checkpoint_bytes(lgame_handle *h) {
+  long long n = (long long)sizeof(CkptHeader);
+  for (auto &f : carried_fields(h)) n += (long long)f.bytes;
+  return n;
+}
+
+CkptHeader checkpoint_header(lgame_handle *h) {
+  CkptHeader hd{};
+  std::memcpy(hd.magic, "LGCKPT1", 8);
+  hd.abi = LGAME_ABI_VERSION;
+  hd.precision = h->mixed ? 1 : 0;
+  hd.ny = h->dims.ny;
+  hd.nx = h->dims.nx;
+  hd.nl = h->dims.n_layers;
+  hd.nc = h->dims.n_constituents;
+  auto f = carried_fields(h);
+  hd.nfields = (int)f.size();
+  hd.has_rad = h->rad_mem ? 1 : 0;
+  hd.payload = checkpoint_bytes(h) - (long long)sizeof(CkptHeader);
+  hd.rad_calls = h->rad_calls;
+  return hd;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lgame_checkpoint_size(lgame_handle *h, long long *bytes) {
+  if (!h || !bytes) return fail(h, LGAME_EINVAL, "null argument");
+  *bytes = checkpoint_bytes(h);
+  return LGAME_OK;
+}
+
+int lgame_checkpoint_save(lgame_handle *h, void *buf, long long bytes) {
+  if (!h || !buf) return fail(h, LGAME_EINVAL, "null argument");
+  if (bytes < checkpoint_bytes(h)) return fail(h, LGAME_EINVAL, "the buffer holds %lld bytes, the checkpoint needs %lld", bytes,
+                                               checkpoint_bytes(h));
+  const CkptHeader hd = checkpoint_header(h);
+  char *out = (char *)buf;
+  std::memcpy(out, &hd, sizeof hd);
+  out += sizeof hd;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto &f : carried_fields(h)) {
+    HIPCHK(h, hipMemcpy(out, f.p, f.bytes, hipMemcpyDeviceToHost));
+    out += f.bytes;
+  }
+  return LGAME_OK;
+}
+
+int lgame_checkpoint_load(lgame_handle *h, const void *buf, long long bytes) {
+  if (!h || !buf) return fail(h, LGAME_EINVAL, "null argument");
+  if (bytes < (long long)sizeof(CkptHeader)) return fail(h, LGAME_EINVAL, "not a checkpoint (too short)");
+  CkptHeader hd;
+  std::memcpy(&hd, buf, sizeof hd);
+  const CkptHeader want = checkpoint_header(h);
+  if (std::memcmp(hd.magic, want.magic, 8) != 0 || hd.abi != want.abi) return fail(h, LGAME_EINVAL, "not a checkpoint of this library");
+  if (hd.precision != want.precision || hd.ny != want.ny || hd.nx != want.nx || hd.nl != want.nl || hd.nc != want.nc ||
+      hd.nfields != want.nfields || hd.has_rad != want.has_rad || hd.payload != want.payload)
+    return fail(h, LGAME_EINVAL, "the checkpoint was made by a handle of another configuration");
+  if (bytes < checkpoint_bytes(h)) return fail(h, LGAME_EINVAL, "the checkpoint is truncated");
+  const char *in = (const char *)buf + sizeof hd;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto &f : carried_fields(h)) {
+    HIPCHK(h, hipMemcpy(f.p, in, f.bytes, hipMemcpyHostToDevice));
+    in += f.bytes;
+  }
+  h->rad_calls = hd.rad_calls;
   return LGAME_OK;
 }
 
@@ -1198,9 +1350,9 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
       }
       const int a = s % 2, b = 1 - a;
       if (h->mixed)
-        enqueue_pchevi(h->cf, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
+        (void)enqueue_pchevi(h->cf, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
       else
-        enqueue_pchevi(h->c, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
+        (void)enqueue_pchevi(h->c, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
     }
     if (rc || ep.failed || hipStreamSynchronize(h->stream) != hipSuccess) {
       rc = rc ? rc : LGAME_EHIP;
@@ -1308,7 +1460,10 @@ int lgame_ensemble_create(lgame_handle *const *members, int n, lgame_ensemble **
   *out = nullptr;
   lgame_ensemble *e = new lgame_ensemble();
   for (int q = 0; q < n; ++q) {
-    if (!members[q]) {
+    // a member listed twice would be stepped twice per ensemble step on one stream
+    bool dup = false;
+    for (int p = 0; p < q; ++p) dup = dup || members[p] == members[q];
+    if (!members[q] || dup) {
       delete e;
       return LGAME_EINVAL;
     }
@@ -1457,13 +1612,23 @@ int lgame_rad_attach_mlp(lgame_handle *h, lgame_mlp *m, double tol_k_per_day, do
   if (h->cfg.rad_mode != LGAME_RAD_SURROGATE) return fail(h, LGAME_EINVAL, "the handle is not configured for the surrogate");
   int n_in = 0, n_out = 0;
   mlp_dims(m, &n_in, &n_out);
-  if (h->dims.n_layers != LGAME_RAD_LEVELS || n_in != LGAME_RAD_N_IN || n_out != LGAME_RAD_N_OUT)
-    return fail(h, LGAME_EINVAL, "the network must map %d inputs to %d outputs on %d layers", LGAME_RAD_N_IN, LGAME_RAD_N_OUT,
-                LGAME_RAD_LEVELS);
+  if (n_in != LGAME_RAD_N_IN || n_out != LGAME_RAD_N_OUT)
+    return fail(h, LGAME_EINVAL, "the network must map %d inputs to %d outputs", LGAME_RAD_N_IN, LGAME_RAD_N_OUT);
+  if (h->dims.n_layers < 2) return fail(h, LGAME_EINVAL, "the surrogate needs at least 2 layers");
+  if (!mlp_has_norm(m))
+    return fail(h, LGAME_EINVAL, "the network has no normalisation (lgame_mlp_set_norm): its output scales define the envelope");
   if (!(tol_k_per_day > 0.0) || !(audit_fraction >= 0.0 && audit_fraction <= 1.0))
     return fail(h, LGAME_EINVAL, "tolerance must be positive and the audit fraction in [0, 1]");
   int rc = rad_alloc(h);
   if (rc) return rc;
+  // the handle's own inference buffers: the step allocates nothing, and handles sharing the
+  // network do not share activations
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  mlp_work_destroy(h->mlp_work);
+  h->mlp_work = nullptr;
+  const long long ncol = (long long)h->dims.ny * h->dims.nx;
+  if (mlp_work_create(m, ncol < 65536 ? ncol : 65536, &h->mlp_work))
+    return fail(h, LGAME_ENOMEM, "inference buffers of the surrogate");
   h->mlp = m;
   h->rad_tol = tol_k_per_day;
   h->rad_thresh = audit_fraction >= 1.0 ? 0xffffffffu : (unsigned)(audit_fraction * 4294967296.0);
@@ -1502,8 +1667,10 @@ void lgame_finalize(lgame_handle *h) {
   if (h->counter) (void)hipFree(h->counter);
   if (h->trsk) (void)hipFree(h->trsk);
   if (h->soil_z) (void)hipFree(h->soil_z);
-  for (void *p : {(void *)h->rad_mem, (void *)h->rad_x, (void *)h->rad_y, (void *)h->rad_counts, (void *)h->rad_sq})
+  for (void *p : {(void *)h->rad_mem, (void *)h->rad_x, (void *)h->rad_y, (void *)h->rad_counts, (void *)h->rad_sq,
+                  (void *)h->rad_list, (void *)h->rad_list_count})
     if (p) (void)hipFree(p);
+  mlp_work_destroy(h->mlp_work);
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -138,6 +138,8 @@ struct Cfg {
   // the surface shortwave input and longwave output of the last radiation update (device column
   // order (ji-1)*nx + (jk-1)), read by the soil; null without radiation
   const double *rad_sw_in, *rad_lw_out;
+  // set (to 1) by the corrector's column solvers when a new state value is not finite
+  unsigned long long *nonfinite;
 };
 
 template <class R>

@@ -366,6 +366,8 @@ struct lgame_mlp {
   int n_in, n_out, hidden, nh;
   std::vector<MlpLayer> L;
   float *x_mean = nullptr, *x_scale = nullptr, *y_mean = nullptr, *y_scale = nullptr;
+  bool has_norm = false;  // lgame_mlp_set_norm was called (the outputs' scales define the envelope)
+  lgame::MlpWork *host_work = nullptr;  // inference buffers of lgame_mlp_forward
   // activations for up to `cap` rows: per layer Z and A (BF16), gradient buffers
   long long cap = 0;
   std::vector<uint16_t *> Z, A, AT;  // AT: activations transposed, [feature][row]
@@ -381,6 +383,16 @@ struct lgame_mlp {
   hipStream_t stream = nullptr;
   std::string err;
 };
+
+namespace lgame {
+// Inference buffers of one user of the network: normalised input, hidden activations, outputs.
+struct MlpWork {
+  long long rows = 0;
+  uint16_t *X = nullptr;
+  std::vector<uint16_t *> A;
+  float *Yf = nullptr;
+};
+}  // namespace lgame
 
 namespace {
 
@@ -403,45 +415,65 @@ int mfail(lgame_mlp *m, int code, const char *fmt, ...) {
 inline int pad64(int n) { return (n + 63) / 64 * 64; }
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-int ensure_cap(lgame_mlp *m, long long rows) {
-  if (rows <= m->cap) return 0;
+void free_training_buffers(lgame_mlp *m) {
   for (auto p : m->Z) (void)hipFree(p);
   for (auto p : m->A) (void)hipFree(p);
   for (auto p : m->AT) (void)hipFree(p);
   m->Z.clear();
   m->A.clear();
   m->AT.clear();
-  for (void *p : {(void *)m->X, (void *)m->XT, (void *)m->dA, (void *)m->dB, (void *)m->dYb, (void *)m->dTa, (void *)m->dTb,
-                  (void *)m->gpart, (void *)m->Yf, (void *)m->Yt, (void *)m->partial, (void *)m->xin, (void *)m->yout,
-                  (void *)m->rows})
-    if (p) (void)hipFree(p);
+  for (void **p : {(void **)&m->X, (void **)&m->XT, (void **)&m->dA, (void **)&m->dB, (void **)&m->dYb, (void **)&m->dTa,
+                   (void **)&m->dTb, (void **)&m->gpart, (void **)&m->Yf, (void **)&m->Yt, (void **)&m->partial,
+                   (void **)&m->xin, (void **)&m->yout, (void **)&m->rows}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  m->cap = 0;
+}
+
+// Activation and gradient buffers of training for `rows` rows.  On a failed allocation every
+// buffer is freed and the capacity is 0, so no later call uses a stale or freed buffer.
+int ensure_cap(lgame_mlp *m, long long rows) {
+  if (rows <= m->cap) return 0;
+  free_training_buffers(m);
   const int hp = pad64(m->hidden), ip = pad64(m->n_in), op = pad64(m->n_out);
   const int wmax = hp > ip ? hp : ip;
-  m->cap = rows;
-  for (int l = 0; l < m->nh; ++l) {
-    uint16_t *z, *a, *at;
-    MCHK(m, hipMalloc(&z, (size_t)rows * hp * 2));
-    MCHK(m, hipMalloc(&a, (size_t)rows * hp * 2));
-    MCHK(m, hipMalloc(&at, (size_t)rows * hp * 2));
+  bool ok = true;
+  auto alloc = [&](void **p, size_t bytes) {
+    if (ok && hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      ok = false;
+    }
+  };
+  for (int l = 0; l < m->nh && ok; ++l) {
+    uint16_t *z = nullptr, *a = nullptr, *at = nullptr;
+    alloc((void **)&z, (size_t)rows * hp * 2);
+    alloc((void **)&a, (size_t)rows * hp * 2);
+    alloc((void **)&at, (size_t)rows * hp * 2);
     m->Z.push_back(z);
     m->A.push_back(a);
     m->AT.push_back(at);
   }
-  MCHK(m, hipMalloc(&m->X, (size_t)rows * ip * 2));
-  MCHK(m, hipMemset(m->X, 0, (size_t)rows * ip * 2));
-  MCHK(m, hipMalloc(&m->XT, (size_t)rows * ip * 2));
-  MCHK(m, hipMalloc(&m->dA, (size_t)rows * wmax * 2));
-  MCHK(m, hipMalloc(&m->dB, (size_t)rows * wmax * 2));
-  MCHK(m, hipMalloc(&m->dTa, (size_t)rows * wmax * 2));
-  MCHK(m, hipMalloc(&m->dTb, (size_t)rows * wmax * 2));
-  MCHK(m, hipMalloc(&m->gpart, (size_t)MAX_SPLITS * wmax * wmax * 4));
-  MCHK(m, hipMalloc(&m->dYb, (size_t)rows * op * 2));
-  MCHK(m, hipMalloc(&m->Yf, (size_t)rows * op * 4));
-  MCHK(m, hipMalloc(&m->Yt, (size_t)rows * op * 4));
-  MCHK(m, hipMalloc(&m->partial, (size_t)nblk(rows * op) * 4));
-  MCHK(m, hipMalloc(&m->xin, (size_t)rows * m->n_in * 4));
-  MCHK(m, hipMalloc(&m->yout, (size_t)rows * m->n_out * 4));
-  MCHK(m, hipMalloc(&m->rows, (size_t)rows * 4));
+  alloc((void **)&m->X, (size_t)rows * ip * 2);
+  alloc((void **)&m->XT, (size_t)rows * ip * 2);
+  alloc((void **)&m->dA, (size_t)rows * wmax * 2);
+  alloc((void **)&m->dB, (size_t)rows * wmax * 2);
+  alloc((void **)&m->dTa, (size_t)rows * wmax * 2);
+  alloc((void **)&m->dTb, (size_t)rows * wmax * 2);
+  alloc((void **)&m->gpart, (size_t)MAX_SPLITS * wmax * wmax * 4);
+  alloc((void **)&m->dYb, (size_t)rows * op * 2);
+  alloc((void **)&m->Yf, (size_t)rows * op * 4);
+  alloc((void **)&m->Yt, (size_t)rows * op * 4);
+  alloc((void **)&m->partial, (size_t)nblk(rows * op) * 4);
+  alloc((void **)&m->xin, (size_t)rows * m->n_in * 4);
+  alloc((void **)&m->yout, (size_t)rows * m->n_out * 4);
+  alloc((void **)&m->rows, (size_t)rows * 4);
+  if (ok) ok = hipMemset(m->X, 0, (size_t)rows * ip * 2) == hipSuccess;
+  if (!ok) {
+    free_training_buffers(m);
+    return mfail(m, LGAME_ENOMEM, "activation buffers for %lld rows", rows);
+  }
+  m->cap = rows;
   return 0;
 }
 
@@ -514,6 +546,36 @@ void forward(lgame_mlp *m, const uint16_t *X, int B, bool train) {
   }
 }
 
+// the forward pass of inference on a workspace's buffers, enqueued on `st`
+void forward_infer(const lgame_mlp *m, MlpWork *w, int B, hipStream_t st) {
+  const uint16_t *in = w->X;
+  int in_p = pad64(m->n_in);
+  for (int l = 0; l <= m->nh; ++l) {
+    const MlpLayer &Ly = m->L[l];
+    NtArgs g{};
+    g.A = in;
+    g.lda = in_p;
+    g.B = Ly.wbf;
+    g.ldb = Ly.in_p;
+    g.M = B;
+    g.N = Ly.out_p;
+    g.K = Ly.in_p;
+    g.bias = Ly.b;
+    if (l < m->nh) {
+      g.Aout = w->A[l];
+      g.ldz = Ly.out_p;
+      g.ldt = B;
+      gemm_nt<EPI_BIAS_GELU>(st, g);
+      in = w->A[l];
+      in_p = Ly.out_p;
+    } else {
+      g.C = w->Yf;
+      g.ldc = Ly.out_p;
+      gemm_nt<EPI_BIAS_F32>(st, g);
+    }
+  }
+}
+
 // backward pass from the output gradient (dYb [B][out_p] and dTa = its transpose) with the
 // activations of the last training forward pass; XT is the transposed input
 void backward(lgame_mlp *m, const uint16_t *XT, int B) {
@@ -578,25 +640,59 @@ namespace lgame {
 
 // Device-side inference used by the radiation surrogate: x (ncol x n_in, FP32, device) ->
 // y (ncol x n_out, FP32, device), in chunks of the activation capacity, on `stream`.
-int mlp_infer_device(lgame_mlp *m, const float *x, long long ncol, float *y, hipStream_t stream) {
-  const long long chunk = 65536;
-  if (ensure_cap(m, chunk)) return LGAME_ENOMEM;
-  hipStream_t saved = m->stream;
-  m->stream = stream;
+void mlp_work_destroy(MlpWork *w) {
+  if (!w) return;
+  for (auto p : w->A)
+    if (p) (void)hipFree(p);
+  if (w->X) (void)hipFree(w->X);
+  if (w->Yf) (void)hipFree(w->Yf);
+  delete w;
+}
+
+int mlp_work_create(const lgame_mlp *m, long long rows, MlpWork **out) {
+  *out = nullptr;
+  rows = (rows + 63) / 64 * 64;
+  if (rows < 64) rows = 64;
+  MlpWork *w = new MlpWork();
+  w->rows = rows;
+  const int hp = pad64(m->hidden), ip = pad64(m->n_in), op = pad64(m->n_out);
+  bool ok = hipMalloc(&w->X, (size_t)rows * ip * 2) == hipSuccess;
+  // the padding columns of the input stay zero (k_norm_in writes the first n_in of each row)
+  ok = ok && hipMemset(w->X, 0, (size_t)rows * ip * 2) == hipSuccess;
+  for (int l = 0; l < m->nh && ok; ++l) {
+    uint16_t *a = nullptr;
+    ok = hipMalloc(&a, (size_t)rows * hp * 2) == hipSuccess;
+    w->A.push_back(a);
+  }
+  ok = ok && hipMalloc(&w->Yf, (size_t)rows * op * 4) == hipSuccess;
+  if (!ok) {
+    mlp_work_destroy(w);
+    return LGAME_ENOMEM;
+  }
+  *out = w;
+  return 0;
+}
+
+// x (ncol x n_in, FP32, device) -> y (ncol x n_out, FP32, device) on `stream`, in passes of the
+// workspace's rows.  Allocates nothing; the network's weights are only read, so users with
+// their own workspaces may run it at the same time on different streams.
+int mlp_infer_device(const lgame_mlp *m, MlpWork *w, const float *x, long long ncol, float *y, hipStream_t stream) {
+  const long long chunk = w->rows;
   const int ip = pad64(m->n_in), op = pad64(m->n_out);
   for (long long c0 = 0; c0 < ncol; c0 += chunk) {
     const long long nc = ncol - c0 < chunk ? ncol - c0 : chunk;
     const int B = (int)((nc + 63) / 64 * 64);
-    // rows beyond nc are left from earlier chunks: finite, and their outputs are not copied
+    // rows beyond nc hold earlier inputs (or zeros): finite, and their outputs are not copied
     hipLaunchKernelGGL(k_norm_in, dim3(nblk(nc * ip)), dim3(256), 0, stream, x + c0 * m->n_in, nc, m->n_in, ip, m->x_mean,
-                       m->x_scale, m->X);
-    forward(m, m->X, B, false);
-    hipLaunchKernelGGL(k_denorm_out, dim3(nblk(nc * m->n_out)), dim3(256), 0, stream, m->Yf, nc, m->n_out, op, m->y_mean,
+                       m->x_scale, w->X);
+    forward_infer(m, w, B, stream);
+    hipLaunchKernelGGL(k_denorm_out, dim3(nblk(nc * m->n_out)), dim3(256), 0, stream, w->Yf, nc, m->n_out, op, m->y_mean,
                        m->y_scale, y + c0 * m->n_out);
   }
-  m->stream = saved;
   return hipGetLastError() == hipSuccess ? 0 : LGAME_EHIP;
 }
+
+bool mlp_has_norm(const lgame_mlp *m) { return m->has_norm; }
 
 int mlp_dims(const lgame_mlp *m, int *n_in, int *n_out) {
   *n_in = m->n_in;
@@ -732,6 +828,7 @@ int lgame_mlp_set_norm(lgame_mlp *m, const float *x_mean, const float *x_scale, 
   MCHK(m, hipMemcpy(m->x_scale, x_scale, (size_t)m->n_in * 4, hipMemcpyHostToDevice));
   MCHK(m, hipMemcpy(m->y_mean, y_mean, (size_t)m->n_out * 4, hipMemcpyHostToDevice));
   MCHK(m, hipMemcpy(m->y_scale, y_scale, (size_t)m->n_out * 4, hipMemcpyHostToDevice));
+  m->has_norm = true;
   return LGAME_OK;
 }
 
@@ -740,10 +837,11 @@ int lgame_mlp_forward(lgame_mlp *m, const float *x, long long n, float *y) {
   if (n == 0) return LGAME_OK;
   const long long chunk = 65536;
   if (ensure_cap(m, chunk)) return LGAME_ENOMEM;
+  if (!m->host_work && mlp_work_create(m, chunk, &m->host_work)) return mfail(m, LGAME_ENOMEM, "inference buffers");
   for (long long c0 = 0; c0 < n; c0 += chunk) {
     const long long nc = n - c0 < chunk ? n - c0 : chunk;
     MCHK(m, hipMemcpyAsync(m->xin, x + c0 * m->n_in, (size_t)nc * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
-    int rc = mlp_infer_device(m, m->xin, nc, m->yout, m->stream);
+    int rc = mlp_infer_device(m, m->host_work, m->xin, nc, m->yout, m->stream);
     if (rc) return mfail(m, rc, "inference failed");
     MCHK(m, hipMemcpyAsync(y + c0 * m->n_out, m->yout, (size_t)nc * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
@@ -908,6 +1006,7 @@ const char *lgame_mlp_last_error(const lgame_mlp *m) { return m ? m->err.c_str()
 void lgame_mlp_destroy(lgame_mlp *m) {
   if (!m) return;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  mlp_work_destroy(m->host_work);
   for (auto &Ly : m->L)
     for (void *p : {(void *)Ly.w, (void *)Ly.b, (void *)Ly.mw, (void *)Ly.vw, (void *)Ly.mb, (void *)Ly.vb, (void *)Ly.gw,
                     (void *)Ly.gb, (void *)Ly.wbf, (void *)Ly.wbfT})

@@ -18,9 +18,11 @@ enum RadFlags {
 struct RadDev {
   const double *lat, *lon, *albedo, *tsfc;
   double *sw_in, *lw_out;
-  float *x;           // [col][5*nl+4]
+  float *x;           // [col][5*nl+4] (RAD_X), or [col][LGAME_RAD_N_IN] (k_rad_gather)
   float *y;           // [col][nl+2]
-  const float *ypred; // [col][nl+2], the surrogate's outputs
+  const float *ypred; // [col][LGAME_RAD_N_OUT], the surrogate's outputs on its own levels
+  int *list;              // the columns of the audit pass (a column c outside the envelope as ~c)
+  unsigned *list_count;   // number of entries in list
   const float *y_mean, *y_scale;  // the surrogate's output normalisation (training envelope)
   unsigned long long *counts;  // [0] audited columns, [1] columns given back to the solver
   double *sq;                  // sum over audited columns and layers of the squared heating-rate error
@@ -33,13 +35,15 @@ struct RadCall {
   int mode;      // LGAME_RAD_TEACHER or LGAME_RAD_SURROGATE
   double t_rad;  // time coordinate of the call, t_0 + dtime_rad/2
   RadDev dev;
-  lgame_mlp *mlp;
+  const lgame_mlp *mlp;
+  MlpWork *work;  // the handle's own inference buffers
 };
 
 template <class R> __global__ void k_rad(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad, int flags);
 
-// update_rad_fluxes (mo_manage_radiation_calls.F90:17-59) for the state `old`, on `stream`
+// update_rad_fluxes (mo_manage_radiation_calls.F90:17-59) for the state `old`, on `stream`;
+// returns 0 or an LGAME_E* code (the launches or the inference failed)
 template <class R>
-void enqueue_radiation(const CtxT<R> &c, const StateDev &old, const RadCall &rc, hipStream_t stream);
+int enqueue_radiation(const CtxT<R> &c, const StateDev &old, const RadCall &rc, hipStream_t stream);
 
 }  // namespace lgame

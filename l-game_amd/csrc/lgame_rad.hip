@@ -7,6 +7,12 @@
 // wavefront are coalesced); per-level intermediates live in the handle's FP64 column scratch,
 // which is free at the start of a step.  All radiation arithmetic is FP64 in both forms; the
 // result goes to radiation_tendency in the stencil type R.
+//
+// The surrogate path: k_rad_gather forms the network's column inputs on its LGAME_RAD_LEVELS
+// levels (a column of another layer count is resampled linearly in the layer index) and writes
+// them row-major through an LDS tile, the network runs on the handle's own buffers, k_rad_apply
+// maps its heating rates back to the column's layers, k_rad_select lists the audit sample and the
+// columns outside the training envelope, and k_rad re-runs the solver on that list only.
 #include "lgame_ops.h"
 #include "lgame_rad.h"
 #include "lgame_step.h"
@@ -49,23 +55,52 @@ __device__ __forceinline__ unsigned rad_hash(unsigned x) {
   return x;
 }
 
+__device__ __forceinline__ bool rad_sampled(const RadDev &r, int col) {
+  return rad_hash((unsigned)col ^ (unsigned)(r.call * 0x9E3779B9ull)) < r.audit_thresh;
+}
+
+// The network sees LGAME_RAD_LEVELS levels.  Level q of the network lies at layer position
+// q (nl - 1)/(L - 1) of a column of nl layers (0-based), layer l at network position
+// l (L - 1)/(nl - 1): the index below and the weight of the one above, linear in the index.
+__device__ __forceinline__ void rad_position(int i, int from, int to, int &lo, double &w) {
+  if (from == to) {
+    lo = i;
+    w = 0.0;
+    return;
+  }
+  const double t = (double)i * (double)(to - 1) / (double)(from - 1);
+  lo = (int)t;
+  if (lo > to - 2) lo = to - 2;
+  w = t - (double)lo;
+}
+
+// the surrogate's heating rate [K/day] at layer jl (1-based) of a column of nl layers
+__device__ __forceinline__ double rad_pred_layer(const float *yp, int jl, int nl) {
+  constexpr int L = LGAME_RAD_LEVELS;
+  if (nl == L) return (double)yp[jl - 1];
+  int q;
+  double w;
+  rad_position(jl - 1, nl, L, q, w);
+  return (1.0 - w) * (double)yp[q] + w * (double)yp[q + 1];
+}
+
 template <class R>
 __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad, int flags) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, ncond = c.G.ncond;
-  if (col >= nx * ny) return;
-  const int jk = col % nx + 1, ji = col / nx + 1;
   const bool audit = flags & RAD_AUDIT;
-  const int nin = 5 * nl + 4, nout = nl + 2;
-  // a prediction outside the training envelope (or not finite) always goes back to the solver
-  bool outside = false;
-  if (audit && r.y_scale) {
-    const float *yc = r.ypred + (size_t)col * nout;
-    for (int o = 0; o < nout; ++o)
-      if (!(fabsf(yc[o] - r.y_mean[o]) <= (float)LGAME_RAD_ENVELOPE * r.y_scale[o])) outside = true;
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  bool outside = false, sampled = false;
+  if (audit) {  // the columns k_rad_select listed: the audit sample and those outside the envelope
+    if (col >= (int)*r.list_count) return;
+    const int e = r.list[col];
+    outside = e < 0;
+    col = outside ? ~e : e;
+    sampled = rad_sampled(r, col);
+  } else if (col >= nx * ny) {
+    return;
   }
-  const bool sampled = rad_hash((unsigned)col ^ (unsigned)(r.call * 0x9E3779B9ull)) < r.audit_thresh;
-  if (audit && !outside && !sampled) return;
+  const int jk = col % nx + 1, ji = col / nx + 1;
+  const int nin = 5 * nl + 4, nout = nl + 2;
   // per-level scratch: temperature, pressure, vapour/liquid/ice mixing ratios, water paths,
   // interface pressure, LW transmissivity, source, SW optical depth, SW heating
   const Fld<double> sT = c.w.col[0], sp = c.w.col[1], sqv = c.w.col[2], slwp = c.w.col[3], siwp = c.w.col[4];
@@ -185,7 +220,7 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
       sup[i] = up;
     }
     double se = 0.0;
-    const float *yp = audit ? r.ypred + (size_t)col * nout : nullptr;
+    const float *yp = audit ? r.ypred + (size_t)col * LGAME_RAD_N_OUT : nullptr;
     float *y = (flags & RAD_Y) ? r.y + (size_t)col * nout : nullptr;
     for (int jl = 1; jl <= nl; ++jl) {
       const ix_t i = LG_IX(c.G, ji, jk, jl), ib = LG_IX(c.G, ji, jk, jl + 1);
@@ -194,7 +229,7 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
       const double hr = 86400.0 * q / (rg[i] * C_D_P);
       if (y) y[jl - 1] = (float)hr;
       if (yp) {
-        const double e = (double)yp[jl - 1] - hr;
+        const double e = rad_pred_layer(yp, jl, nl) - hr;
         se = se + e * e;
       }
     }
@@ -219,44 +254,148 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
         const ix_t i = LG_IX(c.G, ji, jk, jl);
         c.w.radiation_tendency[i] = (R)sq[i];
       }
-      r.sw_in[col] = sw;
+      // the reference stores the surface shortwave flux of day columns only, night columns keep
+      // the previous call's value (mo_rterrtmgp_coupler.F90:430-432)
+      if (mu0 > 0.0) r.sw_in[col] = sw;
       r.lw_out[col] = lw;
     }
   }
 }
 
-// radiation_tendency and surface fluxes from the surrogate's outputs (heating rates in K/day)
+// the surrogate's column inputs (T, p, q_v, q_l, q_i on the network's levels, then the surface
+// temperature, the cosine of the zenith angle, the albedo and the surface pressure), the same
+// quantities k_rad forms with RAD_X.  Each lane forms its column's row in an LDS tile; the block
+// then writes its rows, which are contiguous in x, as one coalesced range.
+constexpr int RAD_TILE_LD = LGAME_RAD_N_IN + 1;  // odd row stride: the lanes' rows hit different banks
+
 template <class R>
-__global__ void __launch_bounds__(BCOL) k_rad_apply(const CtxT<R> c, const StateDev s, const RadDev r) {
+__global__ void __launch_bounds__(BCOL) k_rad_gather(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+  constexpr int L = LGAME_RAD_LEVELS;
+  __shared__ float tile[BCOL * RAD_TILE_LD];
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, ncond = c.G.ncond;
+  const int ncol = nx * ny, col0 = blockIdx.x * BCOL, col = col0 + (int)threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *r.list_count = 0u;  // k_rad_select appends after this launch
+  if (col < ncol) {
+    const int jk = col % nx + 1, ji = col / nx + 1;
+    float *row = tile + threadIdx.x * RAD_TILE_LD;
+    const Fld<const double> rg = s.rho[ncond];
+    // the five per-layer inputs of layer jl, as k_rad forms them
+    auto feat = [&](int jl, double f[5]) {
+      const ix_t i = LG_IX(c.G, ji, jk, jl);
+      const double T = (double)c.w.temperature[i];
+      const double rho_g = rg[i];
+      f[0] = rad_clip(T);
+      f[1] = R_D * rho_g * T;
+      f[2] = f[3] = f[4] = 0.0;
+      if (c.C.lmoist) {
+        const double r1 = s.rho[0][i], r2 = s.rho[1][i], r3 = s.rho[2][i], r4 = s.rho[3][i], r5 = s.rho[4][i];
+        f[2] = s.rho[ncond + 1][i] / rho_g;
+        f[3] = (r2 + r4) / rho_g;
+        f[4] = (r1 + r3 + r5) / rho_g;
+      }
+    };
+    for (int q = 0; q < L; ++q) {
+      double f[5];
+      if (nl == L) {
+        feat(q + 1, f);
+      } else {
+        int lo;
+        double w, g[5];
+        rad_position(q, L, nl, lo, w);
+        feat(lo + 1, f);
+        feat(lo + 2, g);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) f[k] = (1.0 - w) * f[k] + w * g[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 5; ++k) row[k * L + q] = (float)f[k];
+    }
+    double t_sfc_in = r.tsfc[col];
+    if (c.C.lsoil) {  // temp_sfc_full of the sea and soil surface, mo_manage_radiation_calls.F90:36-42
+      const ix_t i1 = LG_IX(c.G, ji, jk, 1), dl = (ix_t)c.G.lvl;
+      const double sea = 1.0 - c.g64.land_fraction[i1] - c.g64.soil[i1];
+      t_sfc_in = sea * c.g64.soil[i1 + 4 * dl] + (1.0 - sea) * s.temperature_soil[i1];
+    }
+    const ix_t ib = LG_IX(c.G, ji, jk, nl);
+    const double p_low = R_D * rg[ib] * (double)c.w.temperature[ib];
+    const double p_sfc = p_low * exp(-(AT(c.g64.z_w, ji, jk, nlev) - AT(c.g64.z_scalar, ji, jk, nl)) / LGAME_RAD_SCALE_HEIGHT);
+    row[5 * L] = (float)rad_clip(t_sfc_in);
+    row[5 * L + 1] = (float)coszenith(r.lat[col], r.lon[col], t_rad);
+    row[5 * L + 2] = (float)r.albedo[col];
+    row[5 * L + 3] = (float)p_sfc;
+  }
+  __syncthreads();
+  const int nrows = ncol - col0 < BCOL ? ncol - col0 : BCOL;
+  float *dst = r.x + (size_t)col0 * LGAME_RAD_N_IN;
+  for (int e = threadIdx.x; e < nrows * LGAME_RAD_N_IN; e += BCOL) {
+    const int rw = e / LGAME_RAD_N_IN, k = e - rw * LGAME_RAD_N_IN;
+    dst[e] = tile[rw * RAD_TILE_LD + k];
+  }
+}
+
+// radiation_tendency and surface fluxes from the surrogate's outputs (heating rates in K/day on
+// the network's levels, mapped to the column's layers)
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_rad_apply(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+  constexpr int L = LGAME_RAD_LEVELS;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (col >= nx * ny) return;
   const int jk = col % nx + 1, ji = col / nx + 1;
-  const float *yp = r.ypred + (size_t)col * (nl + 2);
+  const float *yp = r.ypred + (size_t)col * LGAME_RAD_N_OUT;
   const Fld<const double> rg = s.rho[c.G.ncond];
   for (int jl = 1; jl <= nl; ++jl) {
     const ix_t i = LG_IX(c.G, ji, jk, jl);
-    c.w.radiation_tendency[i] = (R)((double)yp[jl - 1] * rg[i] * C_D_P / 86400.0);
+    c.w.radiation_tendency[i] = (R)(rad_pred_layer(yp, jl, nl) * rg[i] * C_D_P / 86400.0);
   }
-  r.sw_in[col] = yp[nl];
-  r.lw_out[col] = yp[nl + 1];
+  // day columns only, as the solver path (mo_rterrtmgp_coupler.F90:430-432)
+  if (coszenith(r.lat[col], r.lon[col], t_rad) > 0.0) r.sw_in[col] = yp[L];
+  r.lw_out[col] = yp[L + 1];
+}
+
+// the audit list: every column of the hashed sample and every column whose prediction leaves
+// the training envelope (or is not finite), appended in wavefront batches
+__global__ void __launch_bounds__(BCOL) k_rad_select(const RadDev r, int ncol) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false, outside = false;
+  if (col < ncol) {
+    const float *yc = r.ypred + (size_t)col * LGAME_RAD_N_OUT;
+    for (int o = 0; o < LGAME_RAD_N_OUT; ++o)
+      if (!(fabsf(yc[o] - r.y_mean[o]) <= (float)LGAME_RAD_ENVELOPE * r.y_scale[o])) outside = true;
+    take = outside || rad_sampled(r, col);
+  }
+  const unsigned long long mask = __ballot(take);  // every lane of the wavefront takes part
+  if (!mask) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  const int lane = (int)__lane_id();
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(r.list_count, (unsigned)__popcll(mask));
+  base = __shfl(base, leader);
+  if (take) {
+    const unsigned below = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    r.list[base + below] = outside ? ~col : col;
+  }
 }
 
 template <class R>
-void enqueue_radiation(const CtxT<R> &c, const StateDev &old, const RadCall &rc, hipStream_t st) {
-  const dim3 g((c.G.nx * c.G.ny + BCOL - 1) / BCOL), b(BCOL);
+int enqueue_radiation(const CtxT<R> &c, const StateDev &old, const RadCall &rc, hipStream_t st) {
+  const int ncol = c.G.nx * c.G.ny;
+  const dim3 g((ncol + BCOL - 1) / BCOL), b(BCOL);
   if (rc.mode == LGAME_RAD_TEACHER) {
     hipLaunchKernelGGL(k_rad<R>, g, b, 0, st, c, old, rc.dev, rc.t_rad, RAD_TEND);
   } else {
-    hipLaunchKernelGGL(k_rad<R>, g, b, 0, st, c, old, rc.dev, rc.t_rad, RAD_X);
-    mlp_infer_device(rc.mlp, rc.dev.x, (long long)c.G.nx * c.G.ny, const_cast<float *>(rc.dev.ypred), st);
-    hipLaunchKernelGGL(k_rad_apply<R>, g, b, 0, st, c, old, rc.dev);
+    hipLaunchKernelGGL(k_rad_gather<R>, g, b, 0, st, c, old, rc.dev, rc.t_rad);
+    const int rc_inf = mlp_infer_device(rc.mlp, rc.work, rc.dev.x, (long long)ncol, const_cast<float *>(rc.dev.ypred), st);
+    if (rc_inf) return rc_inf;
+    hipLaunchKernelGGL(k_rad_apply<R>, g, b, 0, st, c, old, rc.dev, rc.t_rad);
+    hipLaunchKernelGGL(k_rad_select, g, b, 0, st, rc.dev, ncol);
     hipLaunchKernelGGL(k_rad<R>, g, b, 0, st, c, old, rc.dev, rc.t_rad, RAD_AUDIT);
   }
+  return hipGetLastError() == hipSuccess ? 0 : LGAME_EHIP;
 }
 
-template void enqueue_radiation<double>(const CtxT<double> &, const StateDev &, const RadCall &, hipStream_t);
-template void enqueue_radiation<float>(const CtxT<float> &, const StateDev &, const RadCall &, hipStream_t);
+template int enqueue_radiation<double>(const CtxT<double> &, const StateDev &, const RadCall &, hipStream_t);
+template int enqueue_radiation<float>(const CtxT<float> &, const StateDev &, const RadCall &, hipStream_t);
 template __global__ void k_rad<double>(const CtxT<double>, const StateDev, const RadDev, double, int);
 template __global__ void k_rad<float>(const CtxT<float>, const StateDev, const RadDev, double, int);
 

@@ -213,6 +213,7 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
   };
   double x_low = 0.0;  // x(Lr) of the row below the current layer
   double tin_low = 0.0;
+  bool finite = true;  // the corrector's new values, for the step's non-finite flag
   double rho_below = 0.0;    // new rho of the layer below
   double rho_o_below = 0.0;  // old rho of the layer below
   BackIn bn = load_back(nl);
@@ -253,13 +254,17 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
       const double rin = 0.5 * (rho_new + rho_below);
       nw.wind_w[LG_IX(c.G, ji, jk, Lr + 1)] = (2.0 * x_low / b.az_b - rin * b.wo_b) / rio;
     }
-    nw.exner_pert[i] = b.ep_o + V * ga * (rt_new - b.rt_o);
-    nw.theta_v_pert[i] = rt_new / rho_new - b.tb;
+    const double ep_new = b.ep_o + V * ga * (rt_new - b.rt_o);
+    const double tp_new = rt_new / rho_new - b.tb;
+    nw.exner_pert[i] = ep_new;
+    nw.theta_v_pert[i] = tp_new;
+    if (RK == 2) finite = finite && isfinite(rho_new) && isfinite(rt_new) && isfinite(ep_new) && isfinite(tp_new);
     rho_below = rho_new;
     rho_o_below = b.rho_o;
     x_low = x_up;
     tin_low = tin_u;
   }
+  if (RK == 2 && !finite) atomicOr(c.C.nonfinite, 1ull);
 }
 
 template <class R>
@@ -543,13 +548,15 @@ __device__ __forceinline__ void gen_sweep(const CtxT<R> &c, const StateDev &old,
 
 // Back substitution of one tracer.  It is the same for all tracer kinds and is called once, after
 // the kind-specific sweeps, so the kernel holds one copy of its code.
-__device__ __forceinline__ void gen_back(const Geom &G, Fld<const double> srp, Fld<const double> sep, Fld<double> rn, int ji, int jk) {
+__device__ __forceinline__ void gen_back(const Geom &G, Fld<const double> srp, Fld<const double> sep, Fld<double> rn, int ji, int jk,
+                                         unsigned long long *nonfinite) {
   const int nl = G.nl;
   // back substitution and clipping of round-off negatives (mo_column_solvers.F90:584-596).
   // The scratch rows come in chunks of BACK_K levels into two register buffers: one chunk's
   // 2*BACK_K loads are in flight while the other chunk is substituted.  Only the chunks that
   // reach above level 1 need bounds checks.
   double x = 0.0;
+  bool finite = true;
   double ar[BACK_K], ae[BACK_K], br[BACK_K], be[BACK_K];
   auto fetch = [&](auto guard, int top, double *pr, double *pe) {
 #pragma unroll
@@ -569,6 +576,7 @@ __device__ __forceinline__ void gen_back(const Geom &G, Fld<const double> srp, F
       if (!decltype(guard)::value || jl >= 1) {
         x = (jl == nl) ? pr[q] : pr[q] - pe[q] * x;
         rn[LG_IX(G, ji, jk, jl)] = x < 0.0 ? 0.0 : x;
+        finite = finite && isfinite(x);
       }
     }
   };
@@ -587,6 +595,7 @@ __device__ __forceinline__ void gen_back(const Geom &G, Fld<const double> srp, F
     subst(checked, top, ar, ae);
     subst(checked, top - BACK_K, br, be);
   }
+  if (nonfinite && !finite) atomicOr(nonfinite, 1ull);
 }
 
 
@@ -638,7 +647,7 @@ __global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const Ctx
   if (!active) return;
   const int jc = t < NCS ? t + 1 : 7;
   const bool shared_ep = t == 4 || t == 5;  // graupel, vapour
-  gen_back(c.G, c.w.col[2 * t + 1], c.w.col[shared_ep ? 2 : 2 * t], nw.rho[jc - 1], ji, jk);
+  gen_back(c.G, c.w.col[2 * t + 1], c.w.col[shared_ep ? 2 : 2 * t], nw.rho[jc - 1], ji, jk, rk == 2 ? c.C.nonfinite : nullptr);
 }
 
 template __global__ void k_solve_ver<double>(const CtxT<double>, const StateDev, const StateDev, int);

@@ -72,9 +72,10 @@ struct ProfHook {
 };
 
 // Enqueue one HEVI predictor-corrector step (pchevi, mo_manage_pchevi.F90:27-118) on `stream`;
-// `rad` is non-null when the step updates the radiation (lrad_update).
+// `rad` is non-null when the step updates the radiation (lrad_update).  Returns 0, or the
+// LGAME_E* code of a radiation update that could not be enqueued.
 template <class R>
-void enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int total_step_counter, hipStream_t stream,
+int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int total_step_counter, hipStream_t stream,
                     ProfHook *prof, const RadCall *rad, const Tiles &tiles);
 
 }  // namespace lgame

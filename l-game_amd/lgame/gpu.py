@@ -25,7 +25,7 @@ EXPORTED = ("lgame_abi_version", "lgame_grid_setup", "lgame_init", "lgame_set_st
             "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_gradient", "lgame_mlp_load_data",
             "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy", "lgame_autotune", "lgame_set_tiles",
             "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
-            "lgame_ensemble_destroy")
+            "lgame_ensemble_destroy", "lgame_checkpoint_size", "lgame_checkpoint_save", "lgame_checkpoint_load")
 
 N_TILES = 21
 TILE_KERNELS = ("temperature", "theta_fill", "theta_grad", "theta_div", "theta_edge", "pgrad", "pot_vort",
@@ -173,6 +173,9 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_ensemble_last_error.restype = ctypes.c_char_p
     lib.lgame_ensemble_destroy.argtypes = [vp]
     lib.lgame_ensemble_destroy.restype = None
+    lib.lgame_checkpoint_size.argtypes = [vp, ctypes.POINTER(ll)]
+    lib.lgame_checkpoint_save.argtypes = [vp, vp, ll]
+    lib.lgame_checkpoint_load.argtypes = [vp, vp, ll]
     _lib = lib
     return lib
 
@@ -315,6 +318,18 @@ class Step:
 
     def sync(self):
         self._check(self.lib.lgame_sync(self.h), "lgame_sync")
+
+    def checkpoint(self) -> bytes:
+        """The carried fields besides the two states (lgame_checkpoint_save), as bytes."""
+        n = ctypes.c_longlong(0)
+        self._check(self.lib.lgame_checkpoint_size(self.h, ctypes.byref(n)), "lgame_checkpoint_size")
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.lib.lgame_checkpoint_save(self.h, buf, n), "lgame_checkpoint_save")
+        return buf.raw
+
+    def restore(self, blob: bytes):
+        """Load a checkpoint of a handle of the same configuration (lgame_checkpoint_load)."""
+        self._check(self.lib.lgame_checkpoint_load(self.h, blob, len(blob)), "lgame_checkpoint_load")
 
     def integrals(self, slot: int) -> np.ndarray:
         out = np.zeros(4 + self.cfg.n_constituents)

@@ -1953,7 +1953,9 @@ static void rad_column(oracle *o, const ostate *s, int ji, int jk, double mu0, d
   for (int jl = nl; jl >= 1; --jl) up[jl - 1] = up[jl] * tl[jl - 1] + B[jl - 1] * (1.0 - tl[jl - 1]);
   for (int jl = 1; jl <= nl; ++jl)
     q[jl - 1] = q[jl - 1] + (up[jl] - up[jl - 1] + dn[jl - 1] - dn[jl]) / (A3(o->z_w, ji, jk, jl) - A3(o->z_w, ji, jk, jl + 1));
-  *sw_in = sw;
+  /* the surface shortwave flux of day columns only; night columns keep the previous call's value
+     (mo_rterrtmgp_coupler.F90:430-432) */
+  if (mu0 > 0.0) *sw_in = sw;
   *lw_out = up[nl] - dn[nl];
   if (x) {
     for (int jl = 0; jl < nl; ++jl) {
@@ -2326,6 +2328,13 @@ static F3 *diag_field(oracle *o, const char *name) {
   return NULL;
 }
 
+/* number of values of a diagnostic field (0 for an unknown name) */
+long long orc_diag_size(oracle *o, const char *name) {
+  if (!strcmp(name, "mass_diff_tendency")) return (long long)o->ny * o->nx * o->nl * o->nc;
+  F3 *f = diag_field(o, name);
+  return f ? (long long)f->n1 * f->n2 * f->n3 : 0;
+}
+
 int orc_get_diag(oracle *o, const char *name, double *dst) {
   if (!strcmp(name, "mass_diff_tendency")) {
     memcpy(dst, o->mass_diff_tendency, sizeof(double) * (size_t)o->ny * o->nx * o->nl * o->nc);
@@ -2390,7 +2399,7 @@ void orc_rad_columns(oracle *o, int slot, double t, double *x, double *y) {
     for (int ji = 1; ji <= o->ny; ++ji) {
       const int col = (ji - 1) + o->ny * (jk - 1);
       const double mu0 = orc_coszenith(o->lat[col], o->lon[col], t);
-      double q[512], a, b;
+      double q[512], a = 0.0, b = 0.0;
       rad_column(o, &o->st[slot], ji, jk, mu0, q, &a, &b, x + (size_t)col * nin, y + (size_t)col * nout);
     }
 }

@@ -65,6 +65,8 @@ class CpuOracle:
             lib.orc_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
             lib.orc_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
             lib.orc_get_diag.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            lib.orc_diag_size.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+            lib.orc_diag_size.restype = ctypes.c_longlong
             lib.orc_set_diag.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
             lib.orc_set_diag.restype = ctypes.c_int
             lib.orc_call.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -125,8 +127,16 @@ class CpuOracle:
             out[k] = a
         return out
 
+    def _check_size(self, name: str, count: int):
+        n = self.lib().orc_diag_size(self.h, name.encode())
+        if n == 0:
+            raise KeyError(name)
+        if n != count:
+            raise ValueError(f"diagnostic {name} holds {n} values, not {count}")
+
     def get_diag(self, name: str, shape) -> np.ndarray:
         a = np.zeros(shape, order="F")
+        self._check_size(name, a.size)
         if self.lib().orc_get_diag(self.h, name.encode(), _ptr(a)) != 0:
             raise KeyError(name)
         return a
@@ -134,6 +144,7 @@ class CpuOracle:
     def set_diag(self, name: str, a):
         """A diagnostic field overwritten (test hook of the closed-form checks)."""
         a = _fortran(a)
+        self._check_size(name, a.size)
         if self.lib().orc_set_diag(self.h, name.encode(), _ptr(a)) != 0:
             raise KeyError(name)
 

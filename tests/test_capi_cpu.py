@@ -88,3 +88,14 @@ def test_grid_setup_rejects_invalid_parameters():
     p = GridParams(2000.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 0, 1, 0, 0, 0, None)
     assert lib.lgame_grid_setup(ctypes.byref(d), ctypes.byref(p), ctypes.byref(out)) == 1
     assert "null" in lib.lgame_last_error(None).decode()
+
+
+def test_ensemble_rejects_a_member_listed_twice():
+    """A handle listed twice would be stepped twice per ensemble step on one stream; the
+    ensemble refuses it before touching any member (the pointers are compared, not used)."""
+    lib = load_library()
+    fake = ctypes.create_string_buffer(64)
+    members = (ctypes.c_void_p * 3)(ctypes.addressof(fake), ctypes.addressof(fake) + 8, ctypes.addressof(fake))
+    e = ctypes.c_void_p()
+    assert lib.lgame_ensemble_create(members, 3, ctypes.byref(e)) == 1  # LGAME_EINVAL
+    assert not e.value
