@@ -26,12 +26,15 @@ and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, d
 and budget sums) with the MLP radiation surrogate and its 1% audit.  Radiation updates follow
 the model's cadence (dtime_rad / dtime steps), the first one inside the first timed step.  The
 headline value is the mixed form; `forms` lists both and `speedup_mixed_vs_fp64` their ratio.
-The roofline object is the implicit vertical solver (three_band_solver_ver, FP64 in both forms),
+The roofline object is the implicit vertical solver (three_band_solver_ver) of the plain FP64 form,
 timed with HIP events on the library's own stream; its algorithmic bytes are 152 B (predictor)
-and 176 B (corrector) per grid point (state, tendency and grid arrays streamed once, FP64).
-The CPU baseline is the oracle (C restatement, one thread) on a bounded sample of the same
-workload type, with the same radiation schedule; each GPU form is run on that sample and its
-per-field RMS and max error against the oracle are reported in `field_error`.
+and 176 B (corrector) per grid point (state, tendency and grid arrays streamed once, FP64; the
+mixed form reads its three tendency arrays as FP32, 12 B fewer).  `step_roofline` puts the whole
+FP64 step against the 1296 B per grid-point step model, beside the measured traffic.
+The CPU baseline is the oracle (the C restatement) on all host cores, as independent instances of
+a bounded sample of the same workload type, one per thread, with the same radiation schedule;
+each GPU form is run on that sample and its per-field RMS and max error against the oracle are
+reported in `field_error`.
 """
 from __future__ import annotations
 
@@ -42,17 +45,29 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "l-game_amd"))
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-SOLVER_BYTES = {1: 152.0, 2: 176.0}          # per grid point, FP64, predictor / corrector
-STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step traffic model
+# algorithmic bytes of three_band_solver_ver per grid point (predictor, corrector): the FP64 form
+# reads 14 (17) arrays and writes 5, all FP64; the mixed form reads its three tendency arrays
+# (rho_gas, rhotheta_v, w) as FP32, 12 B fewer
+SOLVER_BYTES = {"fp64": {1: 152.0, 2: 176.0}, "mixed": {1: 140.0, 2: 164.0}}
+STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step traffic model (FP64)
 # HBM bytes per k_solve_ver dispatch, averaged over predictor and corrector, from two rocprofv3
-# --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB device copy gives);
-# profiles/moist512_fp64_hbm_traffic.md holds the commands and the per-kernel table.
-SOLVER_TRAFFIC = {"moist512": 4.369e+09}
+# --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB device copy gives), per
+# form of the step; profiles/ holds the commands and the per-kernel tables.
+SOLVER_TRAFFIC = {("moist512", "fp64"): 4.369e+09, ("moist512", "mixed"): 4.128e+09}
+# measured HBM bytes per grid-point step of the whole FP64 step (same counters, all kernels)
+STEP_TRAFFIC = {"moist512": 3947.0}
+# the reference's own CPU path, flang -O2 -fopenmp on 8 cores (the survey's measurement; the
+# reference tree does not travel, so it cannot be timed beside the GPU)
+REFERENCE_CPU = {"dry128": {"value": 3.19e6, "grid": "129x129x50 dry", "cores": 8},
+                 "moist512": {"value": 2.05e6, "grid": "513x513x60 dry, no radiation", "cores": 8},
+                 "schaer": {"value": 1.78e6, "grid": "201x3x40 Schaer", "cores": 8}}
 METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated field error"
 DEFAULT_SURROGATE = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
 SURROGATE_WEIGHTS = DEFAULT_SURROGATE
@@ -156,8 +171,21 @@ def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
     return elapsed, bad, prof, dict(rad, tiles=tiles)
 
 
-def cpu_baseline(cfg, nsteps, forms, mlp):
-    """Oracle timing on the host (one thread) plus the GPU forms' per-field error on the same sample."""
+def host_threads():
+    """Host cores this process may use, at most 16 (the share of the host a one-GPU job gets)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(cfg, nsteps, forms, mlp, threads):
+    """The oracle on all host cores: `threads` independent instances of the same sample, one per
+    thread (the ctypes calls release the interpreter lock, so they run in parallel), timed
+    together; plus the GPU forms' per-field error against the oracle on that sample."""
+    import threading
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from lgame.cases import make_case, radiation_surface
     from lgame.metrics import field_errors, worst
@@ -165,17 +193,32 @@ def cpu_baseline(cfg, nsteps, forms, mlp):
 
     grid, st = make_case(cfg)
     surface = radiation_surface(cfg, grid, st)
-    o = CpuOracle(cfg, grid)
-    o.start(st)
-    o.set_rad_surface(surface)
     every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
-    lib = o.lib()
+    orcs = []
+    for _ in range(threads):
+        o = CpuOracle(cfg, grid)
+        o.start(st)
+        o.set_rad_surface(surface)
+        orcs.append(o)
+    lib = orcs[0].lib()
+
+    def run(o):
+        for k in range(nsteps):
+            lrad = int(bool(every) and k % every == 0)
+            lib.orc_step_rad(o.h, k % 2, 1 - k % 2, k, lrad, T_EPOCH + k * cfg.dt)
+
+    pool = [threading.Thread(target=run, args=(o,)) for o in orcs]
     t0 = time.perf_counter()
-    for k in range(nsteps):
-        lrad = int(bool(every) and k % every == 0)
-        lib.orc_step_rad(o.h, k % 2, 1 - k % 2, k, lrad, T_EPOCH + k * cfg.dt)
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
     dt = time.perf_counter() - t0
-    want = o.get_state(nsteps % 2)
+    want = orcs[0].get_state(nsteps % 2)
+    same = all(all(np.array_equal(o.get_state(nsteps % 2)[k], want[k]) for k in want) for o in orcs[1:])
+    if not same:
+        raise RuntimeError("the oracle instances disagree")
+    del orcs
     errs = {}
     for form in forms:
         g = make_step(cfg, grid, surface, form, mlp)
@@ -186,7 +229,7 @@ def cpu_baseline(cfg, nsteps, forms, mlp):
         e = field_errors(got, want)
         errs[form] = {"max_rel": worst(e, "max_rel"), "rms_rel": worst(e, "rms_rel"),
                       "per_field": {k: [v["rms_rel"], v["max_rel"]] for k, v in e.items()}}
-    return cfg.gridpoints * nsteps / dt, dt, errs
+    return threads * cfg.gridpoints * nsteps / dt, dt, errs
 
 
 def copy_calibration(nbytes):
@@ -496,9 +539,13 @@ def main():
     head = "mixed" if "mixed" in res else "fp64"
     elapsed, bad, prof, rad = res[head]
     value = cells * args.steps / elapsed
-    ms_solver, n_solver = prof[PROF_SOLVE_VER]
+    # the roofline object: the FP64 form's column solver where it was timed (the FP64 solves of the
+    # north star), else the timed form's with its own byte count
+    rf = "fp64" if "fp64" in res else head
+    ms_solver, n_solver = res[rf][2][PROF_SOLVE_VER]
     ms_step, n_step = prof[PROF_STEP]
-    solver_bytes = cells * (SOLVER_BYTES[1] + SOLVER_BYTES[2]) * (n_solver // 2)
+    sb = SOLVER_BYTES[rf]
+    solver_bytes = cells * (sb[1] + sb[2]) * (n_solver // 2)
     solver_gbs = solver_bytes / (ms_solver * 1e-3) / 1e9 if ms_solver > 0 else 0.0
     avg_launch_us = 1e3 * ms_solver / max(n_solver, 1)
     step_model = STEP_BYTES["moist" if cfg.lmoist else "dry"]
@@ -540,32 +587,38 @@ def main():
                       "radiation_update_us": rad_us[f], "radiation": r[3]}
                   for f, r in res.items()},
         "roofline": {
-            "kernel": "k_solve_ver (three_band_solver_ver, both substeps, FP64)",
+            "kernel": f"k_solve_ver (three_band_solver_ver, both substeps, FP64 solve) of the {rf} form",
             "bound": "hbm",
             "achieved": solver_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": solver_gbs / HBM_PEAK_GBS,
-            "traffic": SOLVER_TRAFFIC.get(args.workload),
+            "traffic": SOLVER_TRAFFIC.get((args.workload, rf)) if not (cfg.diffusion or cfg.lpbl) else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
-            "algorithmic_bytes_per_point": {"predictor": SOLVER_BYTES[1], "corrector": SOLVER_BYTES[2]},
+            "algorithmic_bytes_per_point": {"predictor": sb[1], "corrector": sb[2]},
             "avg_launch_us": avg_launch_us,
             "launches_timed": n_solver,
         },
         "step_roofline": {
-            "bytes_per_gp_step_model_fp64": step_model,
-            "achieved_gbs_fp64_model": step_model * value / 1e9,
+            "form": "fp64",
+            "bytes_per_gp_step_model": step_model,
+            "bytes_per_gp_step_measured": STEP_TRAFFIC.get(args.workload) if not (cfg.diffusion or cfg.lpbl) else None,
+            "achieved_gbs_model": step_model * cells * args.steps / res["fp64"][0] / 1e9 if "fp64" in res else None,
+            "frac_model": step_model * cells * args.steps / res["fp64"][0] / 1e9 / HBM_PEAK_GBS if "fp64" in res else None,
             "event_ms_per_step": ms_step / max(n_step, 1),
         },
     }
     if "fp64" in res and "mixed" in res:
         out["speedup_mixed_vs_fp64"] = res["fp64"][0] / res["mixed"][0]
     if not args.no_cpu:
-        log(f"CPU baseline: oracle on {describe(sample_cfg)} for {args.cpu_steps} steps")
-        v, secs, errs = cpu_baseline(sample_cfg, args.cpu_steps, forms, mlp)
+        nt = host_threads()
+        log(f"CPU baseline: oracle on {describe(sample_cfg)} for {args.cpu_steps} steps, {nt} instances on {nt} threads")
+        v, secs, errs = cpu_baseline(sample_cfg, args.cpu_steps, forms, mlp, nt)
         out["cpu_baseline"] = {
-            "value": v, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/lgame_oracle.c, {describe(sample_cfg)}, {args.cpu_steps} steps, {secs:.1f} s",
+            "value": v, "unit": "grid-point-steps/s", "cores": nt, "kind": "port",
+            "sample": f"oracle/lgame_oracle.c (the C restatement), {nt} independent instances of {describe(sample_cfg)} "
+                      f"on {nt} host threads, {args.cpu_steps} steps each, {secs:.1f} s",
+            "reference_8_cores": REFERENCE_CPU.get(args.workload),
         }
         out["field_error"] = {"vs": "FP64 oracle (gray two-stream radiation) on the CPU-baseline sample",
                               "steps": args.cpu_steps, **errs}

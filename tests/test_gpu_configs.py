@@ -19,7 +19,7 @@ import pytest
 import xxhash
 
 from lgame.cases import dry3d_config, ensemble_members, make_case, radiation_surface, schaer_config
-from lgame.config import PREC_FP64, PREC_MIXED, RAD_SURROGATE, RAD_TEACHER
+from lgame.config import PREC_FP64, PREC_MIXED, RAD_SURROGATE
 from lgame.gpu import Ensemble, Mlp, Step, grid_setup_device
 from lgame.metrics import field_errors, worst
 from golden_util import max_rel
@@ -205,14 +205,13 @@ def _digest(st):
 
 @pytest.mark.timeout(900)
 def test_ensemble_48_uneven_members_graph_equals_serial():
-    """48 moist members (nx, ny ~ U[64,384], 40-70 layers), mixed form with radiation: one graph
-    launch per ensemble step against serial launches, over 4 steps whose second updates the
-    radiation; every member's state is bitwise the same.  Members of 60 layers take the surrogate
-    (trained on 60-layer columns) with its 1 % audit, the others the two-stream solver."""
+    """48 moist members (nx, ny ~ U[64,384], 40-70 layers), mixed form with the radiation
+    surrogate and its 1 % audit (members of other than 60 layers resampled to the network's
+    levels): one graph launch per ensemble step against serial launches, over 4 steps whose
+    second updates the radiation; every member's state is bitwise the same."""
     from lgame import surrogate
 
-    cfgs = [dataclasses.replace(c, precision=PREC_MIXED, lrad=True, dtime_rad=900.0,
-                                rad_mode=RAD_SURROGATE if c.n_layers == 60 else RAD_TEACHER)
+    cfgs = [dataclasses.replace(c, precision=PREC_MIXED, lrad=True, dtime_rad=900.0, rad_mode=RAD_SURROGATE)
             for c in ensemble_members(48)]
     mlp = surrogate.load(surrogate.DEFAULT_WEIGHTS)
     n, t0 = 4, 1576843200.0 + 3 * 3600.0
@@ -224,8 +223,7 @@ def test_ensemble_48_uneven_members_graph_equals_serial():
             g, st = make_case(c, grid=g)
             s = Step(c, g)
             s.set_rad_surface(radiation_surface(c, g, st))
-            if c.rad_mode == RAD_SURROGATE:
-                s.attach_mlp(mlp, tol_k_per_day=1.0, audit_fraction=0.01, seed=7)
+            s.attach_mlp(mlp, tol_k_per_day=1.0, audit_fraction=0.01, seed=7)
             s.start(st)
             steps.append(s)
         print(f"mode {mode}: {len(steps)} members, {sum(c.gridpoints for c in cfgs) / 1e6:.1f} M points", flush=True)
