@@ -531,6 +531,17 @@ int lgame_mlp_get_params(lgame_mlp *m, float *params, long long n);
 int lgame_mlp_set_norm(lgame_mlp *m, const float *x_mean, const float *x_scale, const float *y_mean, const float *y_scale);
 /* y (n, n_out) = network(x (n, n_in)), host arrays, physical units */
 int lgame_mlp_forward(lgame_mlp *m, const float *x, long long n, float *y);
+/*
+ * lgame_mlp_forward that allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and results; the passes run in one static device
+ * array of 8 MiB per GPU (held until the process ends), whose address is looked up once per GPU,
+ * so a pass holds as many rows as fit (1,536 for the 304 -> 3x512 -> 62 network).  Returns
+ * LGAME_EINVAL when 64 rows of the network do not fit (a hidden width above about 21,000 at three
+ * hidden layers) or when another device than the network's is current.  Calls that overlap on one
+ * GPU share the array and race: callers serialise them.
+ */
+int lgame_mlp_forward_noalloc(lgame_mlp *m, const float *x, long long n, float *y);
 /* mean-squared error of the normalised outputs over n rows (a multiple of 64) and its gradient
    with respect to every parameter */
 int lgame_mlp_gradient(lgame_mlp *m, const float *x, const float *y, long long n, float *grad, float *loss);

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -367,6 +368,7 @@ struct lgame_mlp {
   std::vector<MlpLayer> L;
   float *x_mean = nullptr, *x_scale = nullptr, *y_mean = nullptr, *y_scale = nullptr;
   bool has_norm = false;  // lgame_mlp_set_norm was called (the outputs' scales define the envelope)
+  int device = 0;         // the GPU the network lives on (current at lgame_mlp_create)
   lgame::MlpWork *host_work = nullptr;  // inference buffers of lgame_mlp_forward
   // activations for up to `cap` rows: per layer Z and A (BF16), gradient buffers
   long long cap = 0;
@@ -385,12 +387,17 @@ struct lgame_mlp {
 };
 
 namespace lgame {
-// Inference buffers of one user of the network: normalised input, hidden activations, outputs.
-struct MlpWork {
+// Buffers of one inference pass of up to `rows` rows: the normalised input (BF16, padded width),
+// every hidden layer's activations (BF16) and the outputs (FP32, padded width).
+struct InferBufs {
   long long rows = 0;
   uint16_t *X = nullptr;
-  std::vector<uint16_t *> A;
+  uint16_t *A[8] = {};
   float *Yf = nullptr;
+};
+// Inference buffers of one user of the network, allocated by mlp_work_create.
+struct MlpWork {
+  InferBufs b;
 };
 }  // namespace lgame
 
@@ -547,7 +554,7 @@ void forward(lgame_mlp *m, const uint16_t *X, int B, bool train) {
 }
 
 // the forward pass of inference on a workspace's buffers, enqueued on `st`
-void forward_infer(const lgame_mlp *m, MlpWork *w, int B, hipStream_t st) {
+void forward_infer(const lgame_mlp *m, const InferBufs *w, int B, hipStream_t st) {
   const uint16_t *in = w->X;
   int in_p = pad64(m->n_in);
   for (int l = 0; l <= m->nh; ++l) {
@@ -640,12 +647,52 @@ namespace lgame {
 
 // Device-side inference used by the radiation surrogate: x (ncol x n_in, FP32, device) ->
 // y (ncol x n_out, FP32, device), in chunks of the activation capacity, on `stream`.
+}  // namespace lgame
+
+// ---- the scratch of lgame_mlp_forward_noalloc ------------------------------------------------
+// One inference pass of R rows needs the FP32 input and output rows, the normalised BF16 input,
+// every hidden layer's BF16 activations and the padded FP32 outputs; R is the largest multiple of
+// 64 (at most 4096) that fits.  Every GPU the library runs on holds one copy of this array until
+// the process ends; calls that overlap on one GPU share it and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[MAX_DEVICES] = {};  // the array's address on each device
+
+// The device address of `scratch` on the current device, looked up once per device with that
+// device current (the array's name is only a host-side stand-in).
+int scratch_address(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return LGAME_EHIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LGAME_EHIP;
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+__global__ void k_zero_u16(uint16_t *p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+namespace lgame {
+
 void mlp_work_destroy(MlpWork *w) {
   if (!w) return;
-  for (auto p : w->A)
+  for (auto p : w->b.A)
     if (p) (void)hipFree(p);
-  if (w->X) (void)hipFree(w->X);
-  if (w->Yf) (void)hipFree(w->Yf);
+  if (w->b.X) (void)hipFree(w->b.X);
+  if (w->b.Yf) (void)hipFree(w->b.Yf);
   delete w;
 }
 
@@ -654,17 +701,13 @@ int mlp_work_create(const lgame_mlp *m, long long rows, MlpWork **out) {
   rows = (rows + 63) / 64 * 64;
   if (rows < 64) rows = 64;
   MlpWork *w = new MlpWork();
-  w->rows = rows;
+  w->b.rows = rows;
   const int hp = pad64(m->hidden), ip = pad64(m->n_in), op = pad64(m->n_out);
-  bool ok = hipMalloc(&w->X, (size_t)rows * ip * 2) == hipSuccess;
+  bool ok = hipMalloc(&w->b.X, (size_t)rows * ip * 2) == hipSuccess;
   // the padding columns of the input stay zero (k_norm_in writes the first n_in of each row)
-  ok = ok && hipMemset(w->X, 0, (size_t)rows * ip * 2) == hipSuccess;
-  for (int l = 0; l < m->nh && ok; ++l) {
-    uint16_t *a = nullptr;
-    ok = hipMalloc(&a, (size_t)rows * hp * 2) == hipSuccess;
-    w->A.push_back(a);
-  }
-  ok = ok && hipMalloc(&w->Yf, (size_t)rows * op * 4) == hipSuccess;
+  ok = ok && hipMemset(w->b.X, 0, (size_t)rows * ip * 2) == hipSuccess;
+  for (int l = 0; l < m->nh && ok; ++l) ok = hipMalloc(&w->b.A[l], (size_t)rows * hp * 2) == hipSuccess;
+  ok = ok && hipMalloc(&w->b.Yf, (size_t)rows * op * 4) == hipSuccess;
   if (!ok) {
     mlp_work_destroy(w);
     return LGAME_ENOMEM;
@@ -676,7 +719,7 @@ int mlp_work_create(const lgame_mlp *m, long long rows, MlpWork **out) {
 // x (ncol x n_in, FP32, device) -> y (ncol x n_out, FP32, device) on `stream`, in passes of the
 // workspace's rows.  Allocates nothing; the network's weights are only read, so users with
 // their own workspaces may run it at the same time on different streams.
-int mlp_infer_device(const lgame_mlp *m, MlpWork *w, const float *x, long long ncol, float *y, hipStream_t stream) {
+static int infer_passes(const lgame_mlp *m, const InferBufs *w, const float *x, long long ncol, float *y, hipStream_t stream) {
   const long long chunk = w->rows;
   const int ip = pad64(m->n_in), op = pad64(m->n_out);
   for (long long c0 = 0; c0 < ncol; c0 += chunk) {
@@ -690,6 +733,10 @@ int mlp_infer_device(const lgame_mlp *m, MlpWork *w, const float *x, long long n
                        m->y_scale, y + c0 * m->n_out);
   }
   return hipGetLastError() == hipSuccess ? 0 : LGAME_EHIP;
+}
+
+int mlp_infer_device(const lgame_mlp *m, MlpWork *w, const float *x, long long ncol, float *y, hipStream_t stream) {
+  return infer_passes(m, &w->b, x, ncol, y, stream);
 }
 
 bool mlp_has_norm(const lgame_mlp *m) { return m->has_norm; }
@@ -714,6 +761,10 @@ int lgame_mlp_create(int n_in, int hidden, int n_hidden_layers, int n_out, unsig
   *out = nullptr;
   if (n_in < 1 || hidden < 1 || n_hidden_layers < 1 || n_out < 1 || n_hidden_layers > 8) return LGAME_EINVAL;
   lgame_mlp *m = new lgame_mlp();
+  if (hipGetDevice(&m->device) != hipSuccess) {
+    delete m;
+    return LGAME_EHIP;
+  }
   m->n_in = n_in;
   m->n_out = n_out;
   m->hidden = hidden;
@@ -844,6 +895,52 @@ int lgame_mlp_forward(lgame_mlp *m, const float *x, long long n, float *y) {
     int rc = mlp_infer_device(m, m->host_work, m->xin, nc, m->yout, m->stream);
     if (rc) return mfail(m, rc, "inference failed");
     MCHK(m, hipMemcpyAsync(y + c0 * m->n_out, m->yout, (size_t)nc * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+  }
+  return LGAME_OK;
+}
+
+// lgame_mlp_forward without any allocation: the passes run in the static scratch array above.
+int lgame_mlp_forward_noalloc(lgame_mlp *m, const float *x, long long n, float *y) {
+  if (!m || !x || !y || n < 0) return LGAME_EINVAL;
+  if (n == 0) return LGAME_OK;
+  int dev = -1;
+  MCHK(m, hipGetDevice(&dev));
+  if (dev != m->device) return mfail(m, LGAME_EINVAL, "call with the network's device (%d) current, not %d", m->device, dev);
+  unsigned char *base = nullptr;
+  if (scratch_address(&base)) return mfail(m, LGAME_EHIP, "device address of the inference scratch");
+  const int ip = pad64(m->n_in), hp = pad64(m->hidden), op = pad64(m->n_out);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  auto need = [&](long long R) {
+    return al((size_t)R * m->n_in * 4) + al((size_t)R * m->n_out * 4) + al((size_t)R * ip * 2) +
+           (size_t)m->nh * al((size_t)R * hp * 2) + al((size_t)R * op * 4);
+  };
+  long long R = 4096;
+  while (R >= 64 && need(R) > NOALLOC_SCRATCH_BYTES) R -= 64;
+  if (R < 64)
+    return mfail(m, LGAME_EINVAL, "the network needs %zu bytes of scratch for 64 rows; lgame_mlp_forward_noalloc has %zu",
+                 need(64), NOALLOC_SCRATCH_BYTES);
+  unsigned char *q = base;
+  auto carve = [&](size_t bytes) {
+    unsigned char *r = q;
+    q += al(bytes);
+    return r;
+  };
+  float *xin = (float *)carve((size_t)R * m->n_in * 4);
+  float *yout = (float *)carve((size_t)R * m->n_out * 4);
+  InferBufs b;
+  b.rows = R;
+  b.X = (uint16_t *)carve((size_t)R * ip * 2);
+  for (int l = 0; l < m->nh; ++l) b.A[l] = (uint16_t *)carve((size_t)R * hp * 2);
+  b.Yf = (float *)carve((size_t)R * op * 4);
+  // the scratch keeps the last call's contents: the input's padding columns must start at zero
+  hipLaunchKernelGGL(k_zero_u16, dim3(nblk((long long)R * ip)), dim3(256), 0, m->stream, b.X, (long long)R * ip);
+  for (long long c0 = 0; c0 < n; c0 += R) {
+    const long long nc = n - c0 < R ? n - c0 : R;
+    MCHK(m, hipMemcpyAsync(xin, x + c0 * m->n_in, (size_t)nc * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
+    int rc = infer_passes(m, &b, xin, nc, yout, m->stream);
+    if (rc) return mfail(m, rc, "inference failed");
+    MCHK(m, hipMemcpyAsync(y + c0 * m->n_out, yout, (size_t)nc * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
   }
   return LGAME_OK;

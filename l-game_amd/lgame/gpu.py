@@ -22,7 +22,7 @@ EXPORTED = ("lgame_abi_version", "lgame_grid_setup", "lgame_init", "lgame_set_st
             "lgame_finalize", "lgame_last_error", "lgame_profile", "lgame_profile_read", "lgame_profile_reset",
             "lgame_rad_set_surface", "lgame_rad_columns", "lgame_rad_get", "lgame_rad_attach_mlp", "lgame_rad_stats",
             "lgame_mlp_create", "lgame_mlp_n_params", "lgame_mlp_set_params", "lgame_mlp_get_params",
-            "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_gradient", "lgame_mlp_load_data",
+            "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_forward_noalloc", "lgame_mlp_gradient", "lgame_mlp_load_data",
             "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy", "lgame_autotune", "lgame_set_tiles",
             "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
             "lgame_ensemble_destroy", "lgame_checkpoint_size", "lgame_checkpoint_save", "lgame_checkpoint_load")
@@ -157,6 +157,7 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_mlp_get_params.argtypes = [vp, vp, ll]
     lib.lgame_mlp_set_norm.argtypes = [vp, vp, vp, vp, vp]
     lib.lgame_mlp_forward.argtypes = [vp, vp, ll, vp]
+    lib.lgame_mlp_forward_noalloc.argtypes = [vp, vp, ll, vp]
     lib.lgame_mlp_gradient.argtypes = [vp, vp, vp, ll, vp, fp]
     lib.lgame_mlp_load_data.argtypes = [vp, vp, vp, ll]
     lib.lgame_mlp_train.argtypes = [vp, ci, ci, ctypes.c_float, ull, vp, ctypes.POINTER(cd)]
@@ -414,6 +415,14 @@ class Mlp:
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
         self._check(self.lib.lgame_mlp_forward(self.m, x.ctypes.data, x.shape[0], y.ctypes.data), "lgame_mlp_forward")
+        return y
+
+    def forward_noalloc(self, x) -> np.ndarray:
+        """forward through lgame_mlp_forward_noalloc (no device allocation)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self._check(self.lib.lgame_mlp_forward_noalloc(self.m, x.ctypes.data, x.shape[0], y.ctypes.data),
+                    "lgame_mlp_forward_noalloc")
         return y
 
     def gradient(self, x, y):
