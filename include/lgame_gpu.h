@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define LGAME_ABI_VERSION 3
+#define LGAME_ABI_VERSION 4
 
 /* return codes */
 #define LGAME_OK          0
@@ -312,15 +312,20 @@ int lgame_checkpoint_load(lgame_handle *h, const void *buf, long long bytes);
  * for the analytic orographies of :326-361 and periodic boundaries).  Parameters mirror the
  * &run and &surface namelist entries the generator reads (mo_run_nml.F90, mo_surface_nml.F90);
  * orography_id 5 is a set-up extension: a sum of Gaussian hills (x0, y0, height, half width in
- * metres, in `hills`, 4 per hill) on the periodic domain.
+ * metres, in `hills`, 4 per hill) on the periodic domain.  orography_id 1 is the reference's
+ * real orography (:336-616) with the orography interpolated to the grid given in `oro` (the
+ * reference reads it from ETOPO files, which it does not ship): the device smooths it with
+ * smooth_hor_scalar2 (:1512-1557, inverse-distance weights over all other points, O((nx ny)^2))
+ * and keeps a tenth of the detail with SLEVE (:605-616); the land fraction stays 0.
  */
 typedef struct lgame_grid_params {
   double dx, dy, toa, stretching_parameter;
   double lat_center, lon_center;  /* rotation of the grid's pole [rad]                           */
-  int orography_id;               /* 0 flat, 2 Schaer, 3, 4, 5 Gaussian hills                    */
+  int orography_id;               /* 0 flat, 1 given (smoothed), 2 Schaer, 3, 4, 5 Gaussian hills */
   int lsleve, lplane, lfreeslip;
   int n_hills;
   const double *hills;            /* (4, n_hills), read with orography_id 5                     */
+  const double *oro;              /* (ny,nx) orography [m], read with orography_id 1            */
 } lgame_grid_params;
 
 /*

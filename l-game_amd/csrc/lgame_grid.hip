@@ -35,6 +35,7 @@ constexpr double T_CONDUCTIVITY_WATER = 1.4e-7;
 constexpr double SIGMA_SOIL = 0.352;
 constexpr double Z_T_CONST = -10.0;
 constexpr double PI = 3.14159265358979323846;
+constexpr double EPSILON_SECURITY = 1e-10;
 // the standard atmosphere of mo_set_initial_state.F90:384-422
 constexpr double LAPSE_RATE = 0.0065;
 constexpr double SURFACE_TEMP = 288.15;
@@ -48,6 +49,7 @@ struct Gen {
   double dx, dy, toa, stretch, dlat, dlon, lat_lu, lon_lu;
   double rot[9];
   const double *hills;
+  const double *oro, *oro_s;  // orography_id 1: the orography and its smoothed part, (ny,nx)
   int *bad;  // set when a column's levels are not monotonic or its orography reaches the flat levels
 };
 
@@ -134,11 +136,60 @@ __device__ inline double geopot(double h, bool lplane) {
   return -GRAVITY * (R_E * R_E) / (R_E + h) + GRAVITY * R_E;
 }
 
-// orography and its smoothed part at column (ji, jk) (mo_grid_generator.F90:326-361)
+// cos and sin of the latitude of every row of scalar points, interleaved
+__global__ void k_lat_trig(const Gen p, double *trig) {
+  const int jn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jn >= p.ny) return;
+  trig[2 * jn] = cos(lat_scalar(p, jn));
+  trig[2 * jn + 1] = sin(lat_scalar(p, jn));
+}
+
+// smooth_hor_scalar2 (mo_grid_generator.F90:1512-1557): every point becomes the mean of all the
+// others weighted by 1/(d + EPSILON_SECURITY)^1.5, d the great-circle angle between the grid
+// coordinates (calculate_distance_h, :1582-1596).  One thread per target point sums over the
+// sources in the reference's order (x outer, y inner); the source indices are the same for every
+// lane, so the source values and the per-row cos/sin of the latitude (trig) are wave-uniform
+// loads.  x^1.5 is x sqrt(x) (within an ulp of pow).  raw and the result are (ny,nx), Fortran
+// order.
+__global__ void k_smooth_oro(const Gen p, const double *__restrict__ trig, const double *__restrict__ raw,
+                             double *out) {
+  const int ji = blockIdx.x * blockDim.x + threadIdx.x, jk = blockIdx.y;
+  if (ji >= p.ny) return;
+  const double lon_a = lon_scalar(p, jk);
+  const double cla = trig[2 * ji], sla = trig[2 * ji + 1];
+  double acc = 0.0, sum_w = 0.0;
+  for (int jm = 0; jm < p.nx; ++jm) {
+    const double cdl = cos(lon_scalar(p, jm) - lon_a);
+    const double *col = raw + (size_t)jm * p.ny;
+    for (int jn = 0; jn < p.ny; ++jn) {
+      const double cosang = cla * trig[2 * jn] * cdl + sla * trig[2 * jn + 1];
+      const double base = 2.0 * asin(sqrt(0.5 - 0.5 * cosang)) + EPSILON_SECURITY;
+      const double weight = (jm == jk && jn == ji) ? 0.0 : 1.0 / (base * sqrt(base));
+      acc = acc + weight * col[jn];
+      sum_w = sum_w + weight;
+    }
+  }
+  out[F3(ji, jk, 0, p.ny, p.nx)] = acc / sum_w;
+}
+
+// the orography of orography_id 1 from its smoothed part (mo_grid_generator.F90:605-616): with
+// SLEVE a tenth of the unsmoothed detail is kept, else only the smoothed field
+__global__ void k_blend_oro(const Gen p, double *oro, const double *oro_s) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)p.ny * p.nx) return;
+  oro[i] = p.lsleve ? oro_s[i] + 0.1 * (oro[i] - oro_s[i]) : oro_s[i];
+}
+
+// orography and its smoothed part at column (ji, jk) (mo_grid_generator.F90:326-361, 605-616)
 __device__ void orography(const Gen &p, int ji, int jk, double &oro, double &oro_s) {
   const double x = p.dx * (jk + 1) - (p.nx / 2 + 1) * p.dx;
   oro = 0.0;
   oro_s = 0.0;
+  if (p.oro_id == 1) {
+    oro = p.oro[F3(ji, jk, 0, p.ny, p.nx)];
+    oro_s = p.oro_s[F3(ji, jk, 0, p.ny, p.nx)];
+    return;
+  }
   if (p.oro_id == 2) {
     const double cc = cos(PI * x / 4000.0);
     oro = 250.0 * exp(-(x * x) / (5000.0 * 5000.0)) * (cc * cc);
@@ -493,8 +544,12 @@ int grid_generate(const lgame_dims &d, const lgame_grid_params &gp, lgame_grid_o
     return LGAME_EINVAL;
   }
   const int id = gp.orography_id;
-  if (id != 0 && id != 2 && id != 3 && id != 4 && id != 5) {
-    err = "orography_id must be 0, 2, 3, 4 or 5 (the others read orography files)";
+  if (id < 0 || id > 5) {
+    err = "orography_id must be 0 to 5";
+    return LGAME_EINVAL;
+  }
+  if (id == 1 && !gp.oro) {
+    err = "orography_id 1 needs the orography (oro): the reference reads it from ETOPO files";
     return LGAME_EINVAL;
   }
   if (id == 5 && (gp.n_hills < 0 || (gp.n_hills > 0 && !gp.hills))) {
@@ -571,13 +626,14 @@ int grid_generate(const lgame_dims &d, const lgame_grid_params &gp, lgame_grid_o
   size_t total = 0;
   for (auto &f : fields) total += f.n;
   char *block = nullptr;
-  double *hills = nullptr;
+  double *hills = nullptr, *oro = nullptr;
   int *bad = nullptr;
   hipStream_t st = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
     if (block) (void)hipFree(block);
     if (hills) (void)hipFree(hills);
+    if (oro) (void)hipFree(oro);
     if (bad) (void)hipFree(bad);
     if (st) (void)hipStreamDestroy(st);
   };
@@ -596,9 +652,21 @@ int grid_generate(const lgame_dims &d, const lgame_grid_params &gp, lgame_grid_o
     e = hipMalloc(&hills, sizeof(double) * 4 * p.n_hills);
     if (e == hipSuccess) e = hipMemcpyAsync(hills, gp.hills, sizeof(double) * 4 * p.n_hills, hipMemcpyHostToDevice, st);
   }
+  if (e == hipSuccess && id == 1) {  // the orography and its smoothed part
+    e = hipMalloc(&oro, (2 * c2 + 2 * (size_t)ny) * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(oro, gp.oro, c2 * sizeof(double), hipMemcpyHostToDevice, st);
+  }
   if (e != hipSuccess) return hip_fail(e, "device set-up");
   p.hills = hills;
   p.bad = bad;
+  if (id == 1) {
+    p.oro = oro;
+    p.oro_s = oro + c2;
+    double *trig = oro + 2 * c2;
+    hipLaunchKernelGGL(k_lat_trig, dim3((ny + 63) / 64), dim3(64), 0, st, p, trig);
+    hipLaunchKernelGGL(k_smooth_oro, dim3((ny + 63) / 64, nx), dim3(64), 0, st, p, trig, oro, oro + c2);
+    hipLaunchKernelGGL(k_blend_oro, dim3((unsigned)((c2 + 255) / 256)), dim3(256), 0, st, p, oro, oro + c2);
+  }
   size_t off = 0;
   for (auto &f : fields) {
     *f.dev = reinterpret_cast<double *>(block) + off;

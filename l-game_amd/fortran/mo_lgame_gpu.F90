@@ -45,6 +45,7 @@ module mo_lgame_gpu
     real(c_double) :: dx, dy, toa, stretching_parameter, lat_center, lon_center
     integer(c_int) :: orography_id, lsleve, lplane, lfreeslip, n_hills
     type(c_ptr)    :: hills
+    type(c_ptr)    :: oro  ! (ny,nx) orography with orography_id 1
   end type lgame_grid_params
 
   type, bind(C) :: lgame_grid_out

@@ -70,7 +70,8 @@ class GridView(ctypes.Structure):
 
 class GridParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("dx", "dy", "toa", "stretching_parameter", "lat_center", "lon_center")] + \
-        [(n, ctypes.c_int) for n in ("orography_id", "lsleve", "lplane", "lfreeslip", "n_hills")] + [("hills", ctypes.c_void_p)]
+        [(n, ctypes.c_int) for n in ("orography_id", "lsleve", "lplane", "lfreeslip", "n_hills")] + \
+        [("hills", ctypes.c_void_p), ("oro", ctypes.c_void_p)]
 
 
 GRID_GEO = ("lat_geo_scalar", "lon_geo_scalar")
@@ -80,10 +81,11 @@ class GridOut(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in tuple(GRID_FIELDS) + GRID_GEO] + [(n, ctypes.c_double) for n in GRID_SCALARS]
 
 
-def grid_setup_device(cfg: RunConfig) -> dict:
+def grid_setup_device(cfg: RunConfig, oro=None) -> dict:
     """The grid of cfg generated on the device (lgame_grid_setup, grid_setup and bg_setup of
     mo_grid_generator.F90:29-1457): the same dict as lgame.grid.grid_setup, including the
-    geographic coordinates of the scalar points."""
+    geographic coordinates of the scalar points.  With orography_id 1, `oro` (ny, nx) is the
+    orography before the smoothing."""
     if not cfg.lperiodic:
         raise ValueError("only periodic grids are generated")
     lib = load_library()
@@ -93,9 +95,10 @@ def grid_setup_device(cfg: RunConfig) -> dict:
            for k in GRID_FIELDS}
     out.update({k: np.zeros((ny, nx), order="F") for k in GRID_GEO})
     hills = np.ascontiguousarray(np.array(cfg.hills, dtype=np.float64).reshape(-1))
+    oro_f = _f64(oro) if oro is not None else None
     p = GridParams(cfg.dx, cfg.dy, cfg.toa, cfg.stretching_parameter, cfg.lat_center, cfg.lon_center,
                    cfg.orography_id, int(cfg.lsleve), int(cfg.lplane), int(cfg.lfreeslip), len(cfg.hills),
-                   hills.ctypes.data if len(cfg.hills) else None)
+                   hills.ctypes.data if len(cfg.hills) else None, oro_f.ctypes.data if oro_f is not None else None)
     o = GridOut(*[out[k].ctypes.data for k in tuple(GRID_FIELDS) + GRID_GEO])
     d, _ = cfg.to_abi()
     rc = lib.lgame_grid_setup(ctypes.byref(d), ctypes.byref(p), ctypes.byref(o))

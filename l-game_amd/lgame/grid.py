@@ -165,13 +165,53 @@ def _grad_hor(s, rz, g, cfg):
     return rx, ry
 
 
-def orography(cfg: RunConfig):
-    """Surface height and its smoothed part (mo_grid_generator.F90:326-361)."""
+def grid_coordinates(cfg: RunConfig):
+    """lat_scalar(ny), lon_scalar(nx): the grid's own (unrotated) coordinates (mo_grid_generator.F90:213-240)."""
+    dlat, dlon = cfg.dy / R_E, cfg.dx / R_E
+    lat = (cfg.ny - 1.0) / 2.0 * dlat - dlat * np.arange(cfg.ny)
+    lon = -(cfg.nx - 1.0) / 2.0 * dlon + dlon * np.arange(cfg.nx)
+    return lat, lon
+
+
+def smooth_hor_scalar2(a, cfg: RunConfig, rows=64):
+    """smooth_hor_scalar2, mo_grid_generator.F90:1512-1557: every point becomes the mean of all the
+    others weighted by 1/(d + EPSILON_SECURITY)**1.5, d the great-circle distance on the unit
+    sphere (calculate_distance_h, :1582-1596) between the points' grid coordinates.  O((ny nx)^2):
+    the targets go in blocks of `rows` y rows."""
+    a = np.asarray(a, dtype=np.float64)
+    ny, nx = a.shape
+    lat, lon = grid_coordinates(cfg)
+    out = _f((ny, nx))
+    latb = lat[:, None]  # sources (jn, jm)
+    lonb = lon[None, :]
+    for i0 in range(0, ny, rows):
+        for jk in range(nx):
+            ia = np.arange(i0, min(ny, i0 + rows))
+            la = lat[ia][:, None, None]
+            cosang = np.cos(la) * np.cos(latb)[None] * np.cos(lonb - lon[jk])[None] + np.sin(la) * np.sin(latb)[None]
+            with np.errstate(invalid="ignore"):  # the point itself may round to cos > 1; dropped below
+                d = 2.0 * 1.0 * np.arcsin(np.sqrt(0.5 - 0.5 * cosang))
+            w = 1.0 / (d + EPSILON_SECURITY) ** 1.5
+            w[np.arange(len(ia)), ia, jk] = 0.0  # the point itself is left out
+            out[ia, jk] = np.sum(w * a[None], axis=(1, 2)) / np.sum(w, axis=(1, 2))
+    return out
+
+
+def orography(cfg: RunConfig, oro_raw=None):
+    """Surface height and its smoothed part (mo_grid_generator.F90:326-361; orography_id 1 is the
+    reference's real orography, given here in oro_raw instead of the ETOPO files, smoothed and
+    blended as :605-616)."""
     ny, nx, dx, dy = cfg.ny, cfg.nx, cfg.dx, cfg.dy
     oro = _f((ny, nx))
     oro_s = _f((ny, nx))
     jk = np.arange(1, nx + 1)
     x = dx * jk - (nx // 2 + 1) * dx
+    if cfg.orography_id == 1:
+        if oro_raw is None:
+            raise ValueError("orography_id 1 needs the orography (oro_raw)")
+        oro_s[:] = smooth_hor_scalar2(oro_raw, cfg)
+        oro[:] = oro_s + 0.1 * (np.asarray(oro_raw) - oro_s) if cfg.lsleve else oro_s
+        return oro, oro_s
     if cfg.orography_id == 0:
         pass
     elif cfg.orography_id == 2:
@@ -211,8 +251,9 @@ def soil_properties(g: dict) -> None:
         has_land, (land * T_CONDUCTIVITY_SOIL + lake * T_CONDUCTIVITY_WATER) / frac, T_CONDUCTIVITY_WATER))
 
 
-def grid_setup(cfg: RunConfig) -> dict:
-    """Grid metrics of grid_setup (mo_grid_generator.F90:29-1387); periodic boundaries."""
+def grid_setup(cfg: RunConfig, oro_raw=None) -> dict:
+    """Grid metrics of grid_setup (mo_grid_generator.F90:29-1387); periodic boundaries.  With
+    orography_id 1, oro_raw (ny, nx) is the orography before the smoothing."""
     if not cfg.lperiodic:
         raise ValueError("only periodic grids are set up here")
     ny, nx, nl = cfg.ny, cfg.nx, cfg.n_layers
@@ -250,7 +291,7 @@ def grid_setup(cfg: RunConfig) -> dict:
     g["lon_geo_scalar"] = np.asfortranarray(lon_geo)
 
     # vertical grid, :894-951
-    oro, oro_s = orography(cfg)
+    oro, oro_s = orography(cfg, oro_raw)
     jl = np.arange(1, nlev + 1, dtype=np.float64)
     z_rel = 1.0 - (jl - 1.0) / nl
     A = z_rel ** cfg.stretching_parameter * toa

@@ -69,7 +69,7 @@ def test_init_rejects_invalid_configuration(mutate_dims, mutate_cfg, msg):
 
 
 def test_abi_version():
-    assert load_library().lgame_abi_version() == 3
+    assert load_library().lgame_abi_version() == 4
 
 
 def test_grid_setup_rejects_invalid_parameters():
@@ -79,7 +79,8 @@ def test_grid_setup_rejects_invalid_parameters():
     d, _ = tiny_config().to_abi()
     buf = ctypes.create_string_buffer(8 * 8)
     out = GridOut(*([ctypes.addressof(buf)] * 44))
-    for p, msg in ((GridParams(2000.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 1, 1, 0, 0, 0, None), "orography_id"),
+    for p, msg in ((GridParams(2000.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 6, 1, 0, 0, 0, None), "orography_id"),
+                   (GridParams(2000.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 1, 1, 0, 0, 0, None, None), "ETOPO"),
                    (GridParams(0.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 0, 1, 0, 0, 0, None), "positive"),
                    (GridParams(2000.0, 2000.0, 12000.0, 1.3, 0.0, 0.0, 5, 1, 0, 0, 2, None), "hills")):
         assert lib.lgame_grid_setup(ctypes.byref(d), ctypes.byref(p), ctypes.byref(out)) == 1

@@ -111,3 +111,52 @@ def test_background_state_is_in_discrete_hydrostatic_balance(plane):
     # the geopotential: g z on the plane, the inverse-square law on the sphere
     want = GRAVITY * z if plane else -GRAVITY * R_E ** 2 / (R_E + z) + GRAVITY * R_E
     np.testing.assert_allclose(phi, want, rtol=1e-13)
+
+
+def _smooth_case():
+    return dataclasses.replace(tiny_config(ny=9, nx=11, n_layers=6), orography_id=1, dx=3000.0, dy=2000.0)
+
+
+def test_orography_smoothing_closed_forms():
+    """smooth_hor_scalar2 (mo_grid_generator.F90:1512-1557): a constant stays constant (the
+    weights are normalised), the smoothing is linear, and a single raised point p gives at every
+    other point q the weight of the pair over the sum of q's weights, w = 1/(d + 1e-10)^1.5 with
+    d the great-circle angle between the grid coordinates, here from the haversine formula (the
+    reference uses the spherical law of cosines; the two agree to about 1e-8 at 2 km)."""
+    from lgame.grid import grid_coordinates, smooth_hor_scalar2
+
+    cfg = _smooth_case()
+    ny, nx = cfg.ny, cfg.nx
+    np.testing.assert_allclose(smooth_hor_scalar2(np.full((ny, nx), 731.5), cfg), 731.5, rtol=1e-14)
+    rng = np.random.default_rng(2)
+    a, b = rng.random((ny, nx)), rng.random((ny, nx))
+    np.testing.assert_allclose(smooth_hor_scalar2(a + 2.0 * b, cfg),
+                               smooth_hor_scalar2(a, cfg) + 2.0 * smooth_hor_scalar2(b, cfg), rtol=1e-13)
+    lat, lon = grid_coordinates(cfg)
+    pi_, pk = 3, 4
+    delta = np.zeros((ny, nx))
+    delta[pi_, pk] = 1.0
+    got = smooth_hor_scalar2(delta, cfg)
+    la, lo = np.meshgrid(lat, lon, indexing="ij")
+    for qi, qk in ((0, 0), (3, 5), (8, 10), (pi_, pk)):
+        hav = np.sin(0.5 * (la - lat[qi])) ** 2 + np.cos(la) * np.cos(lat[qi]) * np.sin(0.5 * (lo - lon[qk])) ** 2
+        w = 1.0 / (2.0 * np.arcsin(np.sqrt(hav)) + 1e-10) ** 1.5
+        w[qi, qk] = 0.0
+        want = w[pi_, pk] / w.sum()
+        np.testing.assert_allclose(got[qi, qk], want, rtol=1e-6)
+    # the raised point itself gets only its neighbours' zeros
+    assert got[pi_, pk] == 0.0
+
+
+def test_smoothed_orography_in_the_grid():
+    """orography_id 1: with SLEVE the surface is the smoothed field plus a tenth of the detail,
+    without SLEVE the smoothed field (mo_grid_generator.F90:605-616)."""
+    from lgame.grid import smooth_hor_scalar2
+
+    cfg = _smooth_case()
+    rng = np.random.default_rng(5)
+    raw = 200.0 * rng.random((cfg.ny, cfg.nx))
+    sm = smooth_hor_scalar2(raw, cfg)
+    for sleve, want in ((True, sm + 0.1 * (raw - sm)), (False, sm)):
+        g = grid_setup(dataclasses.replace(cfg, lsleve=sleve), oro_raw=raw)
+        np.testing.assert_allclose(g["z_w"][:, :, -1], want, rtol=1e-14)
