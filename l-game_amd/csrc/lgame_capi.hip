@@ -367,10 +367,11 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
       LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 1);
       LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
     }
-    LG_LAUNCH(KI_SCALAR_FLUX, k_scalar_flux, nx, ny, nl, c, S, nw);
-    LG_LAUNCH_ZFAST(KI_SCALAR_DIV, k_scalar_div, nx, ny, nl, c, S, nw, rk);
+    if (pk) prof->begin(PROF_KERNEL0 + KI_SCALAR_DIV);
+    launch_scalar_col(c, S, nw, rk, tiles.by[KI_SCALAR_DIV], tiles.seg, st);
+    if (pk) prof->end(PROF_KERNEL0 + KI_SCALAR_DIV);
     // three_band_solver_ver (mo_column_solvers.F90:21-351): the surface part, then the atmospheric
-    // columns; w_free_slip is done in k_scalar_div
+    // columns; w_free_slip is done in k_scalar_col
     if (c.C.lsoil || c.C.lsens) LG_LAUNCH(KI_DIFF_DUAL, k_surface, nx, ny, 1, c, old, nw, rk);
     if (prof) prof->begin(0);
     hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
@@ -621,6 +622,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     delete h;
     return fail(nullptr, LGAME_EINVAL, "one field of %lld points exceeds 4 GiB", G.fsize);
   }
+  h->tiles.seg = default_segments(G);
   Cfg &C = h->c.C;
   C.dtime = f.dtime;
   C.toa = f.toa;
@@ -679,7 +681,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6 + (C.lsoil ? 1 : 0));
   const size_t ndiff = diffusion ? (size_t)14 + nc + 3 * (nc + 1) : 0;  // the diffusion and surface fields of WorkT
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 4 + 2 + ndiff;
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
   h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
@@ -758,10 +760,6 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
       W.tu[q] = take();
       W.tv[q] = take();
     }
-    W.fu = take();
-    W.fv = take();
-    W.thu = take();
-    W.thv = take();
     W.decel = take();
     W.pgc = take();
     if (diffusion) {
@@ -1336,9 +1334,9 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
     best_ms[k] = 1e300;
     best_by[k] = BY;
   }
-  int rc = 0;
-  for (int ci = 0; ci < 3 && !rc; ++ci) {
-    for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = cand[ci];
+  // run `steps` steps with the current tile table and sum each kernel's time
+  auto measure = [&](double *ms) {
+    int rc = 0;
     h->ev_marks.clear();
     h->ev_next = 0;
     EventProf ep(h, true);
@@ -1354,26 +1352,43 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
       else
         (void)enqueue_pchevi(h->c, h->slot[a], h->slot[b], s, h->stream, &ep, (const RadCall *)nullptr, h->tiles);
     }
-    if (rc || ep.failed || hipStreamSynchronize(h->stream) != hipSuccess) {
-      rc = rc ? rc : LGAME_EHIP;
-      break;
-    }
-    double ms[KI_COUNT] = {0.0};
+    if (rc || ep.failed || hipStreamSynchronize(h->stream) != hipSuccess) return rc ? rc : LGAME_EHIP;
+    for (int k = 0; k < KI_COUNT; ++k) ms[k] = 0.0;
     for (auto &m : h->ev_marks) {
       const int g = m.first - PROF_KERNEL0;
       if (g < 0 || g >= KI_COUNT) continue;
       float t = 0.0f;
       if (hipEventElapsedTime(&t, h->ev_pool[m.second.first], h->ev_pool[m.second.second]) == hipSuccess) ms[g] += t;
     }
-    for (int k = 0; k < KI_COUNT; ++k)
+    h->ev_marks.clear();
+    h->ev_next = 0;
+    return 0;
+  };
+  int rc = 0;
+  double ms[KI_COUNT];
+  for (int ci = 0; ci < 3 && !rc; ++ci) {
+    for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = cand[ci];
+    rc = measure(ms);
+    for (int k = 0; k < KI_COUNT && !rc; ++k)
       if (ms[k] > 0.0 && ms[k] < best_ms[k]) {
         best_ms[k] = ms[k];
         best_by[k] = cand[ci];
       }
   }
-  h->ev_marks.clear();
-  h->ev_next = 0;
   for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = best_by[k];
+  // the vertical segments of the column-marching scalar kernel, with its best rows per block
+  int best_seg = h->tiles.seg;
+  double best_seg_ms = 1e300;
+  for (int sg = 1; sg <= 8 && !rc; sg *= 2) {
+    if (sg > 1 && h->c.G.nl / sg < 4) break;
+    h->tiles.seg = sg;
+    rc = measure(ms);
+    if (!rc && ms[KI_SCALAR_DIV] > 0.0 && ms[KI_SCALAR_DIV] < best_seg_ms) {
+      best_seg_ms = ms[KI_SCALAR_DIV];
+      best_seg = sg;
+    }
+  }
+  h->tiles.seg = best_seg;
   // back to the state before tuning: both slots, and zero carried work arrays, as at lgame_init
   hipError_t e = hipMemcpyAsync(h->slab, backup, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->slab + h->work_off, 0, h->work_bytes, h->stream);
@@ -1392,6 +1407,20 @@ int lgame_set_tiles(lgame_handle *h, const int *rows) {
   for (int k = 0; k < KI_COUNT; ++k)
     if (rows[k] != 1 && rows[k] != 2 && rows[k] != 4) return fail(h, LGAME_EINVAL, "rows per block must be 1, 2 or 4");
   for (int k = 0; k < KI_COUNT; ++k) h->tiles.by[k] = rows[k];
+  return LGAME_OK;
+}
+
+int lgame_set_segments(lgame_handle *h, int nseg) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  if (nseg != 1 && nseg != 2 && nseg != 4 && nseg != 8) return fail(h, LGAME_EINVAL, "segments must be 1, 2, 4 or 8");
+  if (nseg > 1 && h->c.G.nl / nseg < 4) return fail(h, LGAME_EINVAL, "fewer than 4 levels per segment");
+  h->tiles.seg = nseg;
+  return LGAME_OK;
+}
+
+int lgame_get_segments(lgame_handle *h, int *nseg) {
+  if (!h || !nseg) return fail(h, LGAME_EINVAL, "null argument");
+  *nseg = h->tiles.seg;
   return LGAME_OK;
 }
 

@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
   if (jk > G.nx || ji > G.ny || jl > nl) return;
   const ix_t i = LG_IX(G, ji, jk, jl);
   // which interfaces carry a contravariant correction, and which layers those need (as in
-  // k_scalar_div): vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1
+  // the scalar divergence): vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1
   const bool up = jl > nflat;
   const bool lo = jl >= nflat && jl < nl;
   const bool lay_m = up && jl != nflat + 1;

@@ -19,8 +19,6 @@ template <class R> __global__ void k_ke(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_ke_grad(const CtxT<R> c);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
-template <class R> __global__ void k_scalar_flux(const CtxT<R> c, const StateDev ss, const StateDev nw);
-template <class R> __global__ void k_scalar_div(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s);
@@ -45,13 +43,29 @@ enum KernelId {
   KI_DIFF_COEF, KI_DIFF_DUAL, KI_DIFF_MOM, KI_DIFF_FLUX, KI_DIFF_DIV, KI_COUNT
 };
 
-// rows per block of every stencil kernel (block = 64 x by threads)
+// rows per block of every stencil kernel (block = 64 x by threads), and the number of vertical
+// segments of the column-marching scalar kernel
 struct Tiles {
   int by[KI_COUNT];
+  int seg = 1;
   Tiles() {
     for (int &b : by) b = BY;
   }
 };
+
+// The column-marching scalar_tend_expl (k_scalar_col): 64-lane rows of which 62 lanes are cells
+// and two the west and east halo; `rows` rows per block, the levels cut into `nseg` segments.
+constexpr int SCOL_CELLS = 62;
+// segments that give a domain about four wavefronts per SIMD of the MI355X (1024 SIMDs), with at
+// least four levels per segment
+inline int default_segments(const Geom &G) {
+  const long long waves = (long long)((G.nx + SCOL_CELLS - 1) / SCOL_CELLS) * G.ny;
+  int sg = 1;
+  while (waves * sg < 4096 && sg < 8 && G.nl / (2 * sg) >= 4) sg *= 2;
+  return sg;
+}
+template <class R>
+void launch_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw, int rk, int rows, int nseg, hipStream_t st);
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
 // The tracer solver: one wavefront per tracer of a 64-column group, two groups per block. Twelve

@@ -633,182 +633,8 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   }
 }
 
-// ---------------------------------------------------------------------------------------------
-// scalar_tend_expl, mo_scalar_tend_expl.F90:24-181
-// ---------------------------------------------------------------------------------------------
-
-// gas mass flux with the new winds and the theta_v flux at u and v points.  Column nx+1 and row
-// ny+1 are copies of column 1 and row 1; the threads of column 1 and row 1 store them, and the
-// launch covers the nx by ny cells.
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_scalar_flux(const CtxT<R> c, const StateDev ss, const StateDev nw) {
-  KIDX;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx || ji > ny || jl > nl) return;
-  const bool o3 = c.C.order == 3;
-  const Fld<const double> rg = ss.rho[c.G.ncond];
-  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
-  const R rC = ATR(rg, ji, jk, jl);
-  const R thC = AT(c.g.theta_v_bg, ji, jk, jl) + ATR(ss.theta_v_pert, ji, jk, jl);
-  // u point
-  const R fu = R(0.5) * (ATR(rg, ji, km, jl) + rC) * ATR(nw.wind_u, ji, jk, jl);
-  R thu;
-  if (o3)
-    thu = AT(c.w.tu[0], ji, jk, jl) * fu;
-  else
-    thu = R(0.5) * ((AT(c.g.theta_v_bg, ji, km, jl) + ATR(ss.theta_v_pert, ji, km, jl)) + thC) * fu;
-  // v point
-  const R fv = R(0.5) * (ATR(rg, jm, jk, jl) + rC) * ATR(nw.wind_v, ji, jk, jl);
-  R thv;
-  if (o3)
-    thv = AT(c.w.tv[0], ji, jk, jl) * fv;
-  else
-    thv = R(0.5) * ((AT(c.g.theta_v_bg, jm, jk, jl) + ATR(ss.theta_v_pert, jm, jk, jl)) + thC) * fv;
-  AT(c.w.fu, ji, jk, jl) = fu;
-  AT(c.w.thu, ji, jk, jl) = thu;
-  AT(c.w.fv, ji, jk, jl) = fv;
-  AT(c.w.thv, ji, jk, jl) = thv;
-  if (jk == 1) {
-    AT(c.w.fu, ji, nx + 1, jl) = fu;
-    AT(c.w.thu, ji, nx + 1, jl) = thu;
-  }
-  if (ji == 1) {
-    AT(c.w.fv, ny + 1, jk, jl) = fv;
-    AT(c.w.thv, ny + 1, jk, jl) = thv;
-  }
-}
-
-// The scalar divergences below load every metric and face value once per cell and share them
-// between the gas, theta_v and tracer fluxes: the vector memory path, not arithmetic, bounds
-// these stencils, so each repeated load costs time.  The expressions are those of div_h,
-// vertical_contravariant_corr and div_h_tracers, evaluated in the same order.
-
-// divergences -> tend%rho, tend%rhotheta_v (scalar_tend_expl, mo_scalar_tend_expl.F90:24-181,
-// with div_h, mo_divergence_operators.F90:16-68, and div_h_tracers with upstream face values,
-// :70-147 and scalar_times_vector_h_upstream, mo_multiplications.F90:206-287); plus
-// w_free_slip (mo_inner_product.F90:52-75) of the new state, which only depends on the already
-// final u, v of this substep.
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, ncond = c.G.ncond, nflat = c.G.nflat, nc = c.G.nc;
-  if (jk > nx || ji > ny || jl > nl) return;
-  const Geom &G = c.G;
-  const ix_t i = LG_IX(G, ji, jk, jl);
-  const bool tracers = nc > 1;
-  const Fld<const double> wu = nw.wind_u, wv = nw.wind_v;
-  // which interfaces carry a contravariant correction, and which layers those need
-  // (vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1)
-  const bool up = jl > nflat;                  // interface jl (jl = nl included)
-  const bool lo = jl >= nflat && jl < nl;      // interface jl+1
-  const bool lay_m = up && jl != nflat + 1;    // layer jl-1
-  const bool lay_c = up || (lo && jl != nflat);  // layer jl
-  // Flux faces of layer jl (needed by the horizontal divergence anyway).  The corrections of
-  // the upper and the lower interface are formed one after the other, with the layers above
-  // and below loaded only for their interface, so that few FP64 values are live at once.
-  const Faces<R> fc_g = faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl);
-  const Faces<R> fc_h = faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl);
-  Faces<R> fc_w{};
-  if (tracers) fc_w = faces<R>(wu, wv, G, ji, jk, jl);
-  VccGeo<R> gc{};
-  if (lay_c) gc = vcc_geo(c, ji, jk, jl);
-  R cu_g = R(0.0), cu_h = R(0.0), cu_w = R(0.0), cl_g = R(0.0), cl_h = R(0.0), cl_w = R(0.0);
-  if (up) {  // vcc at interface jl: layer jl-1 (unless jl = nflat+1), then layer jl
-    if (lay_m) {
-      const VccGeo<R> gm = vcc_geo(c, ji, jk, jl - 1);
-      cu_g = vcc_sub(R(0.0), gm, faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl - 1));
-      cu_h = vcc_sub(R(0.0), gm, faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl - 1));
-      if (tracers) cu_w = vcc_sub(R(0.0), gm, faces<R>(wu, wv, G, ji, jk, jl - 1));
-    }
-    cu_g = vcc_sub(cu_g, gc, fc_g);
-    cu_h = vcc_sub(cu_h, gc, fc_h);
-    if (tracers) cu_w = vcc_sub(cu_w, gc, fc_w);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  if (lo) {  // vcc at interface jl+1: layer jl (unless jl = nflat), then layer jl+1
-    if (jl != nflat) {
-      cl_g = vcc_sub(R(0.0), gc, fc_g);
-      cl_h = vcc_sub(R(0.0), gc, fc_h);
-      if (tracers) cl_w = vcc_sub(R(0.0), gc, fc_w);
-    }
-    const VccGeo<R> gp = vcc_geo(c, ji, jk, jl + 1);
-    cl_g = vcc_sub(cl_g, gp, faces<R>(c.w.fu, c.w.fv, G, ji, jk, jl + 1));
-    cl_h = vcc_sub(cl_h, gp, faces<R>(c.w.thu, c.w.thv, G, ji, jk, jl + 1));
-    if (tracers) cl_w = vcc_sub(cl_w, gp, faces<R>(wu, wv, G, ji, jk, jl + 1));
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  const R ax1 = AT(c.g.area_x, ji, jk + 1, jl), ay0 = AT(c.g.area_y, ji, jk, jl);
-  const R ax0 = AT(c.g.area_x, ji, jk, jl), ay1 = AT(c.g.area_y, ji + 1, jk, jl);
-  const R az_u = up ? AT(c.g.area_z, ji, jk, jl) : R(0.0);
-  const R az_l = lo ? AT(c.g.area_z, ji, jk, jl + 1) : R(0.0);
-  const R V = AT(c.g.volume, ji, jk, jl);
-  // div_h with the corrections cu, cl of this cell
-  auto div = [&](const Faces<R> &fc, R cu, R cl) {
-    const R comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
-    R comp_v = R(0.0);
-    if (jl == nflat) {
-      comp_v = -cl * az_l;
-    } else if (jl == nl) {
-      comp_v = cu * az_u;
-    } else if (jl > nflat) {
-      comp_v = cu * az_u - cl * az_l;
-    }
-    return (comp_h + comp_v) / V;
-  };
-  const R div_g = div(fc_g, cu_g, cl_g);
-  const R div_h_ = div(fc_h, cu_h, cl_h);
-  R heat = c.w.heating_diss[i] + c.w.radiation_tendency[i] + c.w.csh[i];
-  if (c.C.ltemp_h) heat = heat + c.w.tdh[i];  // diag%temp_diff_heating (of this step's first substep)
-  R rt_tend = -div_h_ + R(C_D_V) * R(ss.rho[ncond][i]) / c_v_mass_weighted_air(c, ss, c.w.temperature[i], i) * heat /
-                           (R(C_D_P) * (c.g.exner_bg[i] + R(ss.exner_pert[i])));
-  R tg = -div_g;
-  if (c.C.lmass_h) {  // diag%mass_diff_tendency of the gas, times theta_v in the rhotheta_v tendency
-    const R md = c.w.mdt[ncond][i];
-    tg = tg + md;
-    rt_tend = rt_tend + (R(0.0) + md) * (c.g.theta_v_bg[i] + R(ss.theta_v_pert[i]));
-  }
-  c.w.tend_rho[ncond][i] = tg;
-  c.w.tend_rhotheta_v[i] = rt_tend;
-  // tracers: upstream densities at the four faces and the two interfaces
-  if (tracers) {
-    const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
-    const int k1 = (jk + 1 == nx + 1) ? 1 : jk + 1;  // face jk+1 (the copy of face 1 when jk = nx)
-    const int j1 = (ji + 1 == ny + 1) ? 1 : ji + 1;
-    const R ue = ATR(wu, ji, k1, jl), uw = fc_w.w, vn = fc_w.n, vs = ATR(wv, j1, jk, jl);
-    const ix_t re = ue >= R(0.0) ? LG_IX(G, ji, wrapm(k1, nx), jl) : LG_IX(G, ji, k1, jl);
-    const ix_t rw = uw >= R(0.0) ? LG_IX(G, ji, wrapm(jk, nx), jl) : LG_IX(G, ji, jk, jl);
-    const ix_t rn = vn >= R(0.0) ? LG_IX(G, ji, jk, jl) : LG_IX(G, wrapm(ji, ny), jk, jl);
-    const ix_t rs = vs >= R(0.0) ? LG_IX(G, j1, jk, jl) : LG_IX(G, wrapm(j1, ny), jk, jl);
-    // the corrections where div_h_tracers uses them, zero elsewhere
-    const R cu = (jl == nl || jl > nflat) && jl != nflat ? cu_w : R(0.0);
-    const R cl = jl == nflat || (jl > nflat && jl != nl) ? cl_w : R(0.0);
-    // the upstream layer of each interface, as an offset from this cell
-    const ix_t iu = cu <= R(0.0) ? LG_IX(G, ji, jk, jl - 1) : i;
-    const ix_t il = cl <= R(0.0) ? i : LG_IX(G, ji, jk, jl + 1);
-    for (int q = 0; q < nc; ++q) {
-      if (q == ncond) continue;
-      const Fld<const double> r = ss.rho[q];
-      const R comp_h = (R(r[re]) * ue) * ax1 + (R(r[rn]) * vn) * ay0 - (R(r[rw]) * uw) * ax0 - (R(r[rs]) * vs) * ay1;
-      R comp_v = R(0.0);
-      if (jl == nflat) {
-        comp_v = -R(r[il]) * cl * az_l;
-      } else if (jl == nl) {
-        comp_v = R(r[iu]) * cu * az_u;
-      } else if (jl > nflat) {
-        comp_v = R(r[iu]) * cu * az_u - R(r[il]) * cl * az_l;
-      }
-      const R d = (comp_h + comp_v) / V;
-      const R x = c.C.lmass_h ? -d + c.w.mdt[q][i] : -d;
-      c.w.tend_rho[q][i] = old_w * c.w.tend_rho[q][i] + new_w * x;
-    }
-  }
-  if (c.C.lfreeslip && jl == nl) {
-    AT(nw.wind_w, ji, jk, nl + 1) = AT(c.g.ipw[0], ji, jk, nl) * ATR(wu, ji, jk + 1, nl) * AT(c.g.slope_x, ji, jk + 1, nl) +
-                                    AT(c.g.ipw[1], ji, jk, nl) * ATR(wv, ji, jk, nl) * AT(c.g.slope_y, ji, jk, nl) +
-                                    AT(c.g.ipw[2], ji, jk, nl) * ATR(wu, ji, jk, nl) * AT(c.g.slope_x, ji, jk, nl) +
-                                    AT(c.g.ipw[3], ji, jk, nl) * ATR(wv, ji + 1, jk, nl) * AT(c.g.slope_y, ji + 1, jk, nl);
-  }
-}
+// scalar_tend_expl (mo_scalar_tend_expl.F90:24-181) is the column-marching k_scalar_col of
+// lgame_scalar.hip.
 
 // The plain form (FP64 stencils) and the mixed form (FP32 stencils).
 #define LG_INSTANTIATE(R)                                                                                  \
@@ -824,9 +650,8 @@ __global__ void __launch_bounds__(BX * BY) k_scalar_div(const CtxT<R> c, const S
   template __global__ void k_ke<R>(const CtxT<R>, const StateDev);                                       \
   template __global__ void k_ke_grad<R>(const CtxT<R>);                                                  \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
-  template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);            \
-  template __global__ void k_scalar_flux<R>(const CtxT<R>, const StateDev, const StateDev);              \
-  template __global__ void k_scalar_div<R>(const CtxT<R>, const StateDev, const StateDev, int);
+  template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);
+
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE

@@ -13,8 +13,8 @@ from .gpu import LIB_PATH, PKG_ROOT
 
 REPO = os.path.dirname(PKG_ROOT)
 CSRC = os.path.join(PKG_ROOT, "csrc")
-SOURCES = ["lgame_capi.hip", "lgame_step.hip", "lgame_diff.hip", "lgame_solve.hip", "lgame_rad.hip", "lgame_mlp.hip",
-           "lgame_grid.hip"]
+SOURCES = ["lgame_capi.hip", "lgame_step.hip", "lgame_scalar.hip", "lgame_diff.hip", "lgame_solve.hip", "lgame_rad.hip",
+           "lgame_mlp.hip", "lgame_grid.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

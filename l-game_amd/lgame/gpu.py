@@ -24,6 +24,7 @@ EXPORTED = ("lgame_abi_version", "lgame_grid_setup", "lgame_init", "lgame_set_st
             "lgame_mlp_create", "lgame_mlp_n_params", "lgame_mlp_set_params", "lgame_mlp_get_params",
             "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_forward_noalloc", "lgame_mlp_gradient", "lgame_mlp_load_data",
             "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy", "lgame_autotune", "lgame_set_tiles",
+            "lgame_set_segments", "lgame_get_segments",
             "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
             "lgame_ensemble_destroy", "lgame_checkpoint_size", "lgame_checkpoint_save", "lgame_checkpoint_load")
 
@@ -170,6 +171,8 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_mlp_destroy.restype = None
     lib.lgame_autotune.argtypes = [vp, ci, vp]
     lib.lgame_set_tiles.argtypes = [vp, vp]
+    lib.lgame_set_segments.argtypes = [vp, ci]
+    lib.lgame_get_segments.argtypes = [vp, ctypes.POINTER(ci)]
     lib.lgame_ensemble_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
     lib.lgame_ensemble_step.argtypes = [vp, ci, ci, cd, ci]
     lib.lgame_ensemble_sync.argtypes = [vp]
@@ -286,6 +289,17 @@ class Step:
     def set_tiles(self, rows):
         r = np.ascontiguousarray([rows[k] for k in TILE_KERNELS] if isinstance(rows, dict) else rows, dtype=np.int32)
         self._check(self.lib.lgame_set_tiles(self.h, r.ctypes.data), "lgame_set_tiles")
+
+    @property
+    def segments(self) -> int:
+        """Vertical segments of the column-marching scalar kernel."""
+        n = ctypes.c_int(0)
+        self._check(self.lib.lgame_get_segments(self.h, ctypes.byref(n)), "lgame_get_segments")
+        return n.value
+
+    @segments.setter
+    def segments(self, nseg: int):
+        self._check(self.lib.lgame_set_segments(self.h, int(nseg)), "lgame_set_segments")
 
     # ---- radiation ----
     def set_rad_surface(self, surface: dict):

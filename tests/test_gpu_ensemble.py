@@ -1,12 +1,13 @@
-"""Launch tuning and ensembles: the autotuner's block shapes and the graph-captured ensemble step
-must not change any result (every member stays bitwise equal to its own oracle run)."""
+"""Launch tuning and ensembles: the autotuner's block shapes, the vertical segments of the
+column-marching scalar kernel and the graph-captured ensemble step must not change any result
+(every member stays bitwise equal to its own oracle run)."""
 import dataclasses
 
 import numpy as np
 import pytest
 
 from lgame.cases import dry3d_config, make_case, tiny_config
-from lgame.gpu import TILE_KERNELS, Ensemble, Step
+from lgame.gpu import TILE_KERNELS, Ensemble, LgameError, Step
 from pyoracle import CpuOracle
 
 pytestmark = pytest.mark.gpu
@@ -32,6 +33,35 @@ def test_autotune_keeps_results():
     s.close()
     want = _oracle(cfg, g, st, 5)
     assert all(np.array_equal(got[k], want[k]) for k in want)
+
+
+def test_scalar_kernel_segments_and_rows_keep_results():
+    """The scalar divergence marches down 62-cell row tiles: a domain of 3 tiles (the last one
+    partial) with an odd row count, cut into 1, 2 and 4 vertical segments on 1, 2 and 4 rows per
+    block, every combination bitwise equal to the oracle over 4 steps (moist, 3rd-order theta,
+    free slip, terrain-following layers from level 5, where the 4-segment cut starts its second
+    segment)."""
+    cfg = dataclasses.replace(tiny_config(ny=7, nx=150, n_layers=16, moist=True, order=3, freeslip=True), n_oro_layers=12)
+    g, st = make_case(cfg, cloud_frac=0.3)
+    n = 4
+    want = _oracle(cfg, g, st, n)
+    for rows in (1, 2, 4):
+        for seg in (1, 2, 4):
+            s = Step(cfg, g)
+            s.set_tiles({k: rows for k in TILE_KERNELS})
+            s.segments = seg
+            assert s.segments == seg
+            s.start(st)
+            s.run(0, n)
+            got = s.get_state(n % 2)
+            s.close()
+            print(f"rows {rows}, segments {seg}", flush=True)
+            assert all(np.array_equal(got[k], want[k]) for k in want), (rows, seg)
+    s = Step(cfg, g)
+    for bad in (3, 8):  # not a power of two; fewer than 4 levels per segment
+        with pytest.raises(LgameError, match="segment"):
+            s.segments = bad
+    s.close()
 
 
 @pytest.mark.parametrize("mode", [Ensemble.GRAPH, Ensemble.SERIAL])
