@@ -473,13 +473,17 @@ int lgame_set_segments(lgame_handle *h, int nseg);
 int lgame_get_segments(lgame_handle *h, int *nseg);
 
 /* ---- ensembles ----
- * Independent domains (handles, each with its own grid, size and state) advanced together.  A
- * step of the ensemble is one launch of a graph captured across all members' streams (one branch
- * per member), so the members run concurrently and the per-kernel launch cost is paid once per
- * ensemble step; steps at counter 0 and steps with a radiation update run eagerly.  mode 0 = the
- * graph launch, mode 1 = the members one after another with eager launches (the comparison).
- * The handles must stay alive and unchanged in configuration while the ensemble exists.  No
- * reference counterpart (the reference runs one domain per process). */
+ * Independent domains (handles, each with its own grid, size and state) advanced together.
+ * mode 2 = batched: members whose steps run the same kernel sequence (same form, switches and
+ * surrogate) form a group, and each kernel of the step is one launch over all members of the
+ * group, with a work list of every member's blocks (radiation updates included: one gather, one
+ * surrogate inference over all members' columns, one apply and one audit).  mode 0 = one launch
+ * of a graph captured across all members' streams (one branch per member); steps at counter 0
+ * and steps with a radiation update run eagerly.  mode 1 = the members one after another with
+ * eager launches (the comparison).  Every mode gives the same bits.  The handles must stay alive
+ * and unchanged in configuration, tiles and segments while the ensemble exists (the batched
+ * groups are made at the first mode-2 step).  No reference counterpart (the reference runs one
+ * domain per process). */
 typedef struct lgame_ensemble lgame_ensemble;
 int lgame_ensemble_create(lgame_handle *const *members, int n, lgame_ensemble **out);
 int lgame_ensemble_step(lgame_ensemble *e, int total_step_counter, int lrad_update, double t_0, int mode);

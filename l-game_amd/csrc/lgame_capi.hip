@@ -11,6 +11,7 @@
 #include "../../include/lgame_gpu.h"
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 #include "lgame_grid.h"
 
 using namespace lgame;
@@ -1427,6 +1428,24 @@ int lgame_get_segments(lgame_handle *h, int *nseg) {
 // ---- ensembles -----------------------------------------------------------------------------
 }  // extern "C"
 
+// Members of an ensemble whose steps run the same kernel sequence (same form and the same
+// switches of enqueue_pchevi's branches, the same surrogate), advanced with one launch per phase.
+struct BatchGroup {
+  std::vector<lgame_handle *> m;
+  bool mixed = false, rad = false, surrogate = false, o3 = false, moist = false;
+  Cfg C{};                       // the switches (of the first member)
+  void *mem = nullptr;           // MemberDev<R>[n] on the device
+  void *host = nullptr;          // its host copy (pinned), rewritten at radiation updates
+  size_t mem_bytes = 0;
+  hipEvent_t uploaded = nullptr; // the last upload of `host` has been read
+  int *prefix = nullptr;         // [PH_COUNT][n+1] first block of each member, per phase
+  int total[PH_COUNT] = {0};     // blocks of each phase
+  Tiles tiles;                   // rows per block (of the largest member)
+  float *rx = nullptr, *ry = nullptr;  // the surrogate's inputs and outputs of every member's columns
+  MlpWork *work = nullptr;
+  long long cols = 0;
+};
+
 struct lgame_ensemble {
   std::vector<lgame_handle *> m;
   hipStream_t master = nullptr;
@@ -1434,6 +1453,8 @@ struct lgame_ensemble {
   std::vector<hipEvent_t> join;
   hipGraphExec_t exec[2] = {nullptr, nullptr};  // one graph per ping-pong parity
   hipGraph_t graph[2] = {nullptr, nullptr};
+  std::vector<BatchGroup> groups;               // the batched launch (mode 2), made at its first step
+  bool grouped = false;
   std::string err;
 };
 
@@ -1480,6 +1501,236 @@ int capture(lgame_ensemble *e, int parity) {
   return 0;
 }
 
+// the branches of enqueue_pchevi a handle's step takes, and its form: members with the same key
+// run the same kernels in the same order
+unsigned long long seq_key(const lgame_handle *h) {
+  const Cfg &C = h->c.C;
+  const int bits[] = {h->mixed, C.order == 3, C.llinear, C.lcorio, C.lmom_h, C.lmom_v, C.lklemp, C.lpbl, C.ltemp_h,
+                      C.lmass_h, C.lsoil, C.lsens, C.lsfc_turb, h->c.G.nc > 1, h->cfg.lrad,
+                      h->cfg.lrad && h->cfg.rad_mode == LGAME_RAD_SURROGATE};
+  unsigned long long k = 0;
+  for (int b : bits) k = 2 * k + (b ? 1 : 0);
+  return k;
+}
+
+void free_group(BatchGroup &g) {
+  if (g.mem) (void)hipFree(g.mem);
+  if (g.host) (void)hipHostFree(g.host);
+  if (g.prefix) (void)hipFree(g.prefix);
+  if (g.rx) (void)hipFree(g.rx);
+  if (g.ry) (void)hipFree(g.ry);
+  if (g.work) mlp_work_destroy(g.work);
+  if (g.uploaded) (void)hipEventDestroy(g.uploaded);
+  g = BatchGroup();
+}
+
+template <class R>
+const CtxT<R> &ctx_of(const lgame_handle *h);
+template <>
+const CtxT<double> &ctx_of<double>(const lgame_handle *h) {
+  return h->c;
+}
+template <>
+const CtxT<float> &ctx_of<float>(const lgame_handle *h) {
+  return h->cf;
+}
+
+// the device table of a group: members, prefix sums of every phase's blocks, and for the
+// surrogate the concatenated input and output rows of all members' columns
+template <class R>
+int build_group(lgame_ensemble *e, BatchGroup &g) {
+  const int n = (int)g.m.size();
+  const lgame_handle *ref = g.m[0];
+  for (auto *h : g.m)
+    if ((long long)h->c.G.nx * h->c.G.ny * h->c.G.nl > (long long)ref->c.G.nx * ref->c.G.ny * ref->c.G.nl) ref = h;
+  g.tiles = ref->tiles;
+  g.mem_bytes = sizeof(MemberDev<R>) * n;
+  ECHK(e, hipHostMalloc(&g.host, g.mem_bytes, hipHostMallocDefault));
+  ECHK(e, hipMalloc(&g.mem, g.mem_bytes));
+  ECHK(e, hipMalloc(&g.prefix, sizeof(int) * PH_COUNT * (n + 1)));
+  ECHK(e, hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming));
+  if (g.surrogate) {
+    for (auto *h : g.m) g.cols += (long long)h->c.G.nx * h->c.G.ny;
+    ECHK(e, hipMalloc(&g.rx, (size_t)g.cols * LGAME_RAD_N_IN * sizeof(float)));
+    ECHK(e, hipMalloc(&g.ry, (size_t)g.cols * LGAME_RAD_N_OUT * sizeof(float)));
+    const int rc = mlp_work_create(g.m[0]->mlp, g.cols < 262144 ? g.cols : 262144, &g.work);
+    if (rc) return efail(e, rc, "surrogate inference buffers of the group");
+  }
+  MemberDev<R> *M = static_cast<MemberDev<R> *>(g.host);
+  // The column-marching scalar kernel cuts a small domain's columns into segments to fill the
+  // GPU; in one launch over the whole group the members' rows fill it together, so the segments
+  // (each with one extra layer) are dropped when the group has four wavefronts per SIMD without
+  // them.
+  long long waves = 0;
+  for (auto *h : g.m) waves += (long long)((h->c.G.nx + SCOL_CELLS - 1) / SCOL_CELLS) * h->c.G.ny;
+  const bool one_seg = waves >= 4096;
+  long long row = 0;
+  for (int q = 0; q < n; ++q) {
+    lgame_handle *h = g.m[q];
+    MemberDev<R> &d = M[q];
+    std::memset(static_cast<void *>(&d), 0, sizeof d);
+    d.c = ctx_of<R>(h);
+    d.s[0] = h->slot[0];
+    d.s[1] = h->slot[1];
+    d.seg = one_seg ? 1 : h->tiles.seg;
+    if (g.rad) {
+      d.rad = rad_dev(h);
+      if (g.surrogate) {
+        d.rad.x = g.rx + (size_t)row * LGAME_RAD_N_IN;
+        d.rad.ypred = g.ry + (size_t)row * LGAME_RAD_N_OUT;
+      }
+    }
+    row += (long long)h->c.G.nx * h->c.G.ny;
+  }
+  std::vector<int> pf((size_t)PH_COUNT * (n + 1), 0);
+  for (int ph = 0; ph < PH_COUNT; ++ph) {
+    const int tile = phase_tile(ph);
+    const int by = tile >= 0 ? g.tiles.by[tile] : 1;
+    int *p = pf.data() + (size_t)ph * (n + 1);
+    for (int q = 0; q < n; ++q) {
+      int gx, gy, gz;
+      phase_grid(ph, g.m[q]->c.G, by, M[q].seg, gx, gy, gz);
+      p[q + 1] = p[q] + gx * gy * gz;
+    }
+    g.total[ph] = p[n];
+  }
+  ECHK(e, hipMemcpyAsync(g.prefix, pf.data(), pf.size() * sizeof(int), hipMemcpyHostToDevice, e->master));
+  ECHK(e, hipMemcpyAsync(g.mem, g.host, g.mem_bytes, hipMemcpyHostToDevice, e->master));
+  ECHK(e, hipEventRecord(g.uploaded, e->master));
+  ECHK(e, hipStreamSynchronize(e->master));  // pf leaves scope
+  return 0;
+}
+
+int build_groups(lgame_ensemble *e) {
+  for (auto *h : e->m) {
+    bool placed = false;
+    for (auto &g : e->groups)
+      if (!placed && seq_key(g.m[0]) == seq_key(h) && g.m[0]->mlp == h->mlp) {
+        g.m.push_back(h);
+        placed = true;
+      }
+    if (!placed) {
+      e->groups.emplace_back();
+      BatchGroup &g = e->groups.back();
+      g.m.push_back(h);
+      g.mixed = h->mixed;
+      g.C = h->c.C;
+      g.o3 = h->c.C.order == 3;
+      g.moist = h->c.G.nc > 1;
+      g.rad = h->cfg.lrad != 0;
+      g.surrogate = g.rad && h->cfg.rad_mode == LGAME_RAD_SURROGATE;
+    }
+  }
+  for (auto &g : e->groups) {
+    const int rc = g.mixed ? build_group<float>(e, g) : build_group<double>(e, g);
+    if (rc) return rc;
+  }
+  e->grouped = true;
+  return 0;
+}
+
+// One step of a group: enqueue_pchevi's sequence with every phase one launch over all members.
+template <class R>
+int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int step, bool rad_update, double t_0) {
+  const int n = (int)g.m.size();
+  const hipStream_t st = e->master;
+  const MemberDev<R> *M = static_cast<const MemberDev<R> *>(g.mem);
+  const Cfg &C = g.C;
+  auto launch = [&](int ph, const BatchArgs &a) {
+    const dim3 grd((unsigned)g.total[ph]);
+    const int *pf = g.prefix + (size_t)ph * (n + 1);
+    dim3 blk(BX, 1, 1);
+    if (ph == PH_SOLVE_VER)
+      blk = dim3(BCOL);
+    else if (ph == PH_SOLVE_GEN)
+      blk = dim3(BCOL, NTR * GEN_GROUPS);
+    else
+      blk = dim3(BX, g.tiles.by[phase_tile(ph)]);
+    if (ph == PH_SCALAR) {
+      batch_launch_scalar<R>(grd, blk, st, M, pf, n, a, g.moist, g.o3);
+      return;
+    }
+    if (!batch_launch_step<R>(ph, grd, blk, st, M, pf, n, a) && !batch_launch_diff<R>(ph, grd, blk, st, M, pf, n, a))
+      (void)batch_launch_solve<R>(ph, grd, blk, st, M, pf, n, a);
+  };
+  BatchArgs base{a_slot, b_slot, a_slot, 1, step == 0 ? 1 : 0, 0, 0};
+  launch(PH_TEMPERATURE, base);
+  // update_sfc_turb_quantities and update_rad_fluxes with state_old (mo_manage_pchevi.F90:52-58)
+  if (C.lsfc_turb) launch(PH_SFC_TURB, base);
+  if (rad_update && g.rad) {
+    // the call index and time of every member (mo_manage_radiation_calls.F90:53), then one
+    // upload of the table, ordered after the launches that read the previous one
+    ECHK(e, hipEventSynchronize(g.uploaded));
+    MemberDev<R> *H = static_cast<MemberDev<R> *>(g.host);
+    for (int q = 0; q < n; ++q) {
+      lgame_handle *h = g.m[q];
+      H[q].rad.call = h->rad_calls++;
+      H[q].t_rad = t_0 + 0.5 * h->cfg.dtime_rad;
+    }
+    ECHK(e, hipMemcpyAsync(g.mem, g.host, g.mem_bytes, hipMemcpyHostToDevice, st));
+    ECHK(e, hipEventRecord(g.uploaded, st));
+    const dim3 grd((unsigned)g.total[PH_RAD]), blk(BCOL);
+    const int *pf = g.prefix + (size_t)PH_RAD * (n + 1);
+    BatchArgs a = base;
+    if (!g.surrogate) {
+      a.flags = RAD_TEND;
+      batch_launch_radiation<R>(RPH_SOLVER, grd, blk, st, M, pf, n, a);
+    } else {
+      batch_launch_radiation<R>(RPH_GATHER, grd, blk, st, M, pf, n, a);
+      const int rc = mlp_infer_device(g.m[0]->mlp, g.work, g.rx, g.cols, g.ry, st);
+      if (rc) return efail(e, rc, "surrogate inference of the group");
+      batch_launch_radiation<R>(RPH_APPLY, grd, blk, st, M, pf, n, a);
+      batch_launch_radiation<R>(RPH_SELECT, grd, blk, st, M, pf, n, a);
+      a.flags = RAD_AUDIT;
+      batch_launch_radiation<R>(RPH_SOLVER, grd, blk, st, M, pf, n, a);
+    }
+  }
+  for (int rk = 1; rk <= 2; ++rk) {
+    BatchArgs a = base;
+    a.rk = rk;
+    a.state_slot = rk == 1 ? a_slot : b_slot;
+    if (rk == 1) {
+      if (g.o3) {
+        BatchArgs t = a;
+        t.state_slot = a_slot;
+        t.nfield = 2;
+        for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
+      }
+      launch(PH_PGRAD, a);
+    }
+    if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
+      launch(PH_POT_VORT, a);
+      launch(PH_MASS_FLUX, a);
+      launch(PH_VORT_FLUX, a);
+      if (!C.llinear) {
+        launch(PH_KE, a);
+        launch(PH_KE_GRAD, a);
+      }
+    }
+    if (rk == 1 && C.lmom_h)
+      for (int ph : {PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM}) launch(ph, a);
+    if (rk == 1 && C.lpbl) launch(PH_PBL_WIND, a);
+    if (rk == 1 && (C.lklemp || C.lmom_h || C.lmom_v || C.lpbl)) launch(PH_DISSIPATION, a);
+    launch(PH_VEC_TEND, a);
+    if (rk == 1 && (C.ltemp_h || C.lmass_h)) {
+      launch(PH_DIFF_FLUX, a);
+      launch(PH_DIFF_DIV, a);
+    }
+    if (g.o3 && rk == 2) {
+      BatchArgs t = a;
+      t.state_slot = b_slot;
+      t.nfield = 1;
+      for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
+    }
+    launch(PH_SCALAR, a);
+    if (C.lsoil || C.lsens) launch(PH_SURFACE, a);
+    launch(PH_SOLVE_VER, a);
+    if (g.moist) launch(PH_SOLVE_GEN, a);
+  }
+  ECHK(e, hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1522,6 +1773,26 @@ int lgame_ensemble_step(lgame_ensemble *e, int total_step_counter, int lrad_upda
   const int a = total_step_counter % 2, b = 1 - a;
   for (auto *h : e->m)
     if (!h->have_state[0] || !h->have_state[1]) return efail(e, LGAME_ESTATE, "every member needs both state slots");
+  if (mode == 2) {  // one launch per phase over every member of a group
+    for (auto *h : e->m)
+      if (h->cfg.lrad && lrad_update && (!h->rad_surface || (h->cfg.rad_mode == LGAME_RAD_SURROGATE && !h->mlp)))
+        return efail(e, LGAME_ESTATE, "a member's radiation is not set up (surface data, surrogate)");
+    if (!e->grouped) {
+      const int rc = build_groups(e);
+      if (rc) return rc;
+    }
+    for (size_t q = 0; q < e->m.size(); ++q) {
+      ECHK(e, hipEventRecord(e->join[q], e->m[q]->stream));
+      ECHK(e, hipStreamWaitEvent(e->master, e->join[q], 0));
+    }
+    for (auto &g : e->groups) {
+      const int rc = g.mixed ? enqueue_group<float>(e, g, a, b, total_step_counter, lrad_update != 0, t_0)
+                             : enqueue_group<double>(e, g, a, b, total_step_counter, lrad_update != 0, t_0);
+      if (rc) return rc;
+    }
+    for (auto *h : e->m) h->stepped = 1;
+    return LGAME_OK;
+  }
   if (mode == 0 && total_step_counter > 0 && !lrad_update) {
     if (!e->exec[a]) {
       int rc = capture(e, a);
@@ -1564,6 +1835,7 @@ void lgame_ensemble_destroy(lgame_ensemble *e) {
     if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
     if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
   }
+  for (auto &g : e->groups) free_group(g);
   for (auto ev : e->join)
     if (ev) (void)hipEventDestroy(ev);
   if (e->fork) (void)hipEventDestroy(e->fork);

@@ -15,6 +15,7 @@
 // -ffp-contract=off), so the FP64 form is bit-identical to the reference.
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 
 namespace lgame {
 
@@ -61,10 +62,14 @@ __device__ __forceinline__ R mean_rho2(const CtxT<R> &c, ix_t a, ix_t b) {
 // row ny+1 and column nx+1 have their own metrics (mo_vorticities.F90:177-184).
 // ---------------------------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_zeta(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_diff_zeta(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX_ZFAST(c.G.nx + 1, c.G.ny + 1, c.G.nl);
   if (jk > c.G.nx + 1 || ji > c.G.ny + 1 || jl > c.G.nl) return;
   AT(c.w.zeta, ji, jk, jl) = rel_vort_z(c, s, ji, jk, jl) / AT(c.g.area_dual_z, ji, jk, jl);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_zeta(const CtxT<R> c, const StateDev s) {
+  b_diff_zeta(c, s, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -80,7 +85,7 @@ __device__ __forceinline__ R n2_level(const CtxT<R> &c, int ji, int jk, int L, R
 }
 
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_coef(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_diff_coef(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx || ji > ny || jl > nl) return;
@@ -130,6 +135,10 @@ __global__ void __launch_bounds__(BX * BY) k_diff_coef(const CtxT<R> c, const St
   c.w.kv[i] = mol + tke2vert(t, n2, c.g.layer_thickness[i]);
   c.w.vdiv[i] = visc * div;
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_coef(const CtxT<R> c, const StateDev s) {
+  b_diff_coef(c, s, bid_here());
+}
 
 // ---------------------------------------------------------------------------------------------
 // viscosity_dual * zeta_z at the dual points.  viscosity_dual averages the four cells around the
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(BX * BY) k_diff_coef(const CtxT<R> c, const St
 // ny+1 and column nx+1 is that of row 1 and column 1; zeta_z there is their own.
 // ---------------------------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_dual(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_diff_dual(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny;
   if (jk > nx || ji > ny || jl > c.G.nl) return;
@@ -164,6 +173,10 @@ __global__ void __launch_bounds__(BX * BY) k_diff_dual(const CtxT<R> c, const St
   if (jk == 1) AT(c.w.vz, ji, nx + 1, jl) = vd * AT(c.w.zeta, ji, nx + 1, jl);
   if (ji == 1 && jk == 1) AT(c.w.vz, ny + 1, nx + 1, jl) = vd * AT(c.w.zeta, ny + 1, nx + 1, jl);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_dual(const CtxT<R> c, const StateDev s) {
+  b_diff_dual(c, s, bid_here());
+}
 
 // ---------------------------------------------------------------------------------------------
 // mom_diff_h (mo_momentum_diff_diss.F90:25-160): the gradient of viscosity times divergence plus
@@ -174,7 +187,7 @@ __global__ void __launch_bounds__(BX * BY) k_diff_dual(const CtxT<R> c, const St
 // carried one.
 // ---------------------------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_diff_mom(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx || ji > ny || jl > nl) return;
@@ -213,6 +226,10 @@ __global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const Sta
   }
   c.w.tke[i] = c.w.tke_next[i];
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_mom(const CtxT<R> c, const StateDev s) {
+  b_diff_mom(c, s, bid_here());
+}
 
 // ---------------------------------------------------------------------------------------------
 // scalar diffusion (mo_scalar_tend_expl.F90:66-99 with scalar_diff_coeffs,
@@ -237,7 +254,7 @@ __device__ __forceinline__ R c_g_v(R rg) {
 }
 
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_flux(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_diff_flux(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny, nl = G.nl;
@@ -338,9 +355,13 @@ __global__ void __launch_bounds__(BX * BY) k_diff_flux(const CtxT<R> c, const St
       one(Fld<const double>(s.rho[jc]), c.w.dfu[jc + 1], c.w.dfv[jc + 1], c.w.dfw[jc + 1], chu, chv, cvw, c.C.lmass_v != 0);
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_flux(const CtxT<R> c, const StateDev s) {
+  b_diff_flux(c, s, bid_here());
+}
 
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
+__device__ __forceinline__ void b_diff_div(const CtxT<R> &c, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   const Geom &G = c.G;
   const int nl = G.nl, nflat = G.nflat;
@@ -390,6 +411,10 @@ __global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
   if (c.C.ltemp_h) one(0, c.w.tdh, c.C.ltemp_v != 0);
   if (c.C.lmass_h)
     for (int jc = 0; jc < G.nc; ++jc) one(jc + 1, c.w.mdt[jc], c.C.lmass_v != 0);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
+  b_diff_div(c, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -481,7 +506,7 @@ __device__ R scalar_flux_resistance(const CtxT<R> &c, R us, R z_agl, R z0, R l) 
 // resistance of the prognostic soil; v_squared is the kinetic energy the last momentum advection
 // block left
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_sfc_turb(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   const Geom &G = c.G;
   const int nl = G.nl;
@@ -512,11 +537,15 @@ __global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const Sta
   sfc[i1 + 2 * dl] = mo_len;
   if (c.C.lsoil) c.w.sfc2[i1] = scalar_flux_resistance(c, us, agl, z0, mo_len);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_sfc_turb(const CtxT<R> c, const StateDev s) {
+  b_sfc_turb(c, s, bid_here());
+}
 
 // pbl_wind_tendency, mo_pbl.F90:111-250: mom_diff_tend_x/_y of the lowest layer minus the
 // friction, at the u and v points of each bottom cell; column nx+1 and row ny+1 are copies
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_pbl_wind(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny, nl = G.nl;
@@ -531,6 +560,10 @@ __global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const Sta
   if (jk == 1) AT(c.w.mom_x, ji, nx + 1, nl) = mx;
   c.w.mom_y[i] = my;
   if (ji == 1) AT(c.w.mom_y, ny + 1, jk, nl) = my;
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pbl_wind(const CtxT<R> c, const StateDev s) {
+  b_pbl_wind(c, s, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -557,7 +590,7 @@ __device__ __forceinline__ double t_gas_lowest(const GridT<double> &g, const Sfc
 }
 
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+__device__ __forceinline__ void b_surface(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
   KIDX;
   const Geom &G = c.G;
   const int nl = G.nl;
@@ -657,6 +690,10 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
     }
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  b_surface(c, old, nw, rk, bid_here());
+}
 
 #define LG_INSTANTIATE(R)                                                                  \
   template __global__ void k_diff_zeta<R>(const CtxT<R>, const StateDev);                  \
@@ -671,5 +708,39 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE
+
+// ---- the same kernels over a group of domains (one launch per phase) ------------------------
+LG_BATCH_KERNEL(diff_zeta, BX * BY, PH_DIFF_ZETA, b_diff_zeta(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(diff_coef, BX * BY, PH_DIFF_COEF, b_diff_coef(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(diff_dual, BX * BY, PH_DIFF_DUAL, b_diff_dual(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(diff_mom, BX * BY, PH_DIFF_MOM, b_diff_mom(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(diff_flux, BX * BY, PH_DIFF_FLUX, b_diff_flux(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(diff_div, BX * BY, PH_DIFF_DIV, b_diff_div(d.c, bid))
+LG_BATCH_KERNEL(sfc_turb, BX * BY, PH_SFC_TURB, b_sfc_turb(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(pbl_wind, BX * BY, PH_PBL_WIND, b_pbl_wind(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(surface, BX * BY, PH_SURFACE, b_surface(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
+
+template <class R>
+bool batch_launch_diff(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm, const BatchArgs &a) {
+#define LG_B(P, K)                                                  \
+  case P:                                                           \
+    hipLaunchKernelGGL(K<R>, g, b, 0, st, M, prefix, nm, a);        \
+    return true;
+  switch (ph) {
+    LG_B(PH_DIFF_ZETA, kb_diff_zeta)
+    LG_B(PH_DIFF_COEF, kb_diff_coef)
+    LG_B(PH_DIFF_DUAL, kb_diff_dual)
+    LG_B(PH_DIFF_MOM, kb_diff_mom)
+    LG_B(PH_DIFF_FLUX, kb_diff_flux)
+    LG_B(PH_DIFF_DIV, kb_diff_div)
+    LG_B(PH_SFC_TURB, kb_sfc_turb)
+    LG_B(PH_PBL_WIND, kb_pbl_wind)
+    LG_B(PH_SURFACE, kb_surface)
+    default: return false;
+  }
+#undef LG_B
+}
+template bool batch_launch_diff<double>(int, dim3, dim3, hipStream_t, const MemberDev<double> *, const int *, int, const BatchArgs &);
+template bool batch_launch_diff<float>(int, dim3, dim3, hipStream_t, const MemberDev<float> *, const int *, int, const BatchArgs &);
 
 }  // namespace lgame

@@ -259,10 +259,17 @@ __device__ __forceinline__ R ke_point(const CtxT<R> &c, const StateDev &s, int j
 }
 
 // ---- thread -> grid point maps of the stencil kernels ---------------------------------------
+// Every kernel body takes its block's coordinates `bid` as an argument: from blockIdx when the
+// kernel runs for one domain, from an ensemble's work list when one launch covers several.
+struct Bid {
+  int x, y, z;
+};
+__device__ __forceinline__ Bid bid_here() { return Bid{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z}; }
+
 #define KIDX                                                        \
-  const int jk = blockIdx.x * blockDim.x + threadIdx.x + 1;         \
-  const int ji = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
-  const int jl = blockIdx.z + 1;
+  const int jk = bid.x * blockDim.x + threadIdx.x + 1;              \
+  const int ji = bid.y * blockDim.y + threadIdx.y + 1;              \
+  const int jl = bid.z + 1;
 
 // Tile order for kernels that read many values of the levels above and below: the kernel is
 // launched on a 1-D grid of gx*gy*gz blocks, which the hardware deals out round-robin to the 8
@@ -270,8 +277,8 @@ __device__ __forceinline__ R ke_point(const CtxT<R> &c, const StateDev &s, int j
 // tile at adjacent levels run one after another on the same XCD and its L2 still holds the
 // shared layers.  Kernels that mostly stream level by level keep the x-fastest order, which
 // reads DRAM pages in sequence.
-__device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int &by, int &bz) {
-  const int n = gx * gy * gz, b = blockIdx.x;
+__device__ __forceinline__ void tile_zfast(int b, int gx, int gy, int gz, int &bx, int &by, int &bz) {
+  const int n = gx * gy * gz;
   const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, r = n % N_XCD;
   const int t = k * q + (k < r ? k : r) + s;  // tile index, level fastest
   bz = t % gz;
@@ -281,7 +288,7 @@ __device__ __forceinline__ void tile_zfast(int gx, int gy, int gz, int &bx, int 
 }
 #define KIDX_ZFAST(ex, ey, ez)                                                            \
   int bx_, by_, bz_;                                                                      \
-  tile_zfast(((ex) + BX - 1) / BX, ((ey) + (int)blockDim.y - 1) / (int)blockDim.y, (ez), bx_, by_, bz_); \
+  tile_zfast(bid.x, ((ex) + BX - 1) / BX, ((ey) + (int)blockDim.y - 1) / (int)blockDim.y, (ez), bx_, by_, bz_); \
   const int jk = bx_ * blockDim.x + threadIdx.x + 1;                                      \
   const int ji = by_ * blockDim.y + threadIdx.y + 1;                                      \
   const int jl = bz_ + 1;

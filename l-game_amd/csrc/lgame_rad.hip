@@ -16,6 +16,7 @@
 #include "lgame_ops.h"
 #include "lgame_rad.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 
 #include "../../include/lgame_gpu.h"
 
@@ -85,10 +86,10 @@ __device__ __forceinline__ double rad_pred_layer(const float *yp, int jl, int nl
 }
 
 template <class R>
-__global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad, int flags) {
+__device__ __forceinline__ void b_rad(const CtxT<R> &c, const StateDev &s, const RadDev &r, double t_rad, int flags, const Bid bid) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, ncond = c.G.ncond;
   const bool audit = flags & RAD_AUDIT;
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  int col = bid.x * blockDim.x + threadIdx.x;
   bool outside = false, sampled = false;
   if (audit) {  // the columns k_rad_select listed: the audit sample and those outside the envelope
     if (col >= (int)*r.list_count) return;
@@ -261,6 +262,10 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
     }
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad, int flags) {
+  b_rad(c, s, r, t_rad, flags, bid_here());
+}
 
 // the surrogate's column inputs (T, p, q_v, q_l, q_i on the network's levels, then the surface
 // temperature, the cosine of the zenith angle, the albedo and the surface pressure), the same
@@ -269,12 +274,12 @@ __global__ void __launch_bounds__(BCOL) k_rad(const CtxT<R> c, const StateDev s,
 constexpr int RAD_TILE_LD = LGAME_RAD_N_IN + 1;  // odd row stride: the lanes' rows hit different banks
 
 template <class R>
-__global__ void __launch_bounds__(BCOL) k_rad_gather(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+__device__ __forceinline__ void b_rad_gather(const CtxT<R> &c, const StateDev &s, const RadDev &r, double t_rad, const Bid bid) {
   constexpr int L = LGAME_RAD_LEVELS;
   __shared__ float tile[BCOL * RAD_TILE_LD];
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, ncond = c.G.ncond;
-  const int ncol = nx * ny, col0 = blockIdx.x * BCOL, col = col0 + (int)threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *r.list_count = 0u;  // k_rad_select appends after this launch
+  const int ncol = nx * ny, col0 = bid.x * BCOL, col = col0 + (int)threadIdx.x;
+  if (bid.x == 0 && threadIdx.x == 0) *r.list_count = 0u;  // k_rad_select appends after this launch
   if (col < ncol) {
     const int jk = col % nx + 1, ji = col / nx + 1;
     float *row = tile + threadIdx.x * RAD_TILE_LD;
@@ -332,13 +337,17 @@ __global__ void __launch_bounds__(BCOL) k_rad_gather(const CtxT<R> c, const Stat
     dst[e] = tile[rw * RAD_TILE_LD + k];
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_rad_gather(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+  b_rad_gather(c, s, r, t_rad, bid_here());
+}
 
 // radiation_tendency and surface fluxes from the surrogate's outputs (heating rates in K/day on
 // the network's levels, mapped to the column's layers)
 template <class R>
-__global__ void __launch_bounds__(BCOL) k_rad_apply(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+__device__ __forceinline__ void b_rad_apply(const CtxT<R> &c, const StateDev &s, const RadDev &r, double t_rad, const Bid bid) {
   constexpr int L = LGAME_RAD_LEVELS;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = bid.x * blockDim.x + threadIdx.x;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (col >= nx * ny) return;
   const int jk = col % nx + 1, ji = col / nx + 1;
@@ -352,11 +361,15 @@ __global__ void __launch_bounds__(BCOL) k_rad_apply(const CtxT<R> c, const State
   if (coszenith(r.lat[col], r.lon[col], t_rad) > 0.0) r.sw_in[col] = yp[L];
   r.lw_out[col] = yp[L + 1];
 }
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_rad_apply(const CtxT<R> c, const StateDev s, const RadDev r, double t_rad) {
+  b_rad_apply(c, s, r, t_rad, bid_here());
+}
 
 // the audit list: every column of the hashed sample and every column whose prediction leaves
 // the training envelope (or is not finite), appended in wavefront batches
-__global__ void __launch_bounds__(BCOL) k_rad_select(const RadDev r, int ncol) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_rad_select(const RadDev &r, int ncol, const Bid bid) {
+  const int col = bid.x * blockDim.x + threadIdx.x;
   bool take = false, outside = false;
   if (col < ncol) {
     const float *yc = r.ypred + (size_t)col * LGAME_RAD_N_OUT;
@@ -376,6 +389,7 @@ __global__ void __launch_bounds__(BCOL) k_rad_select(const RadDev r, int ncol) {
     r.list[base + below] = outside ? ~col : col;
   }
 }
+__global__ void __launch_bounds__(BCOL) k_rad_select(const RadDev r, int ncol) { b_rad_select(r, ncol, bid_here()); }
 
 template <class R>
 int enqueue_radiation(const CtxT<R> &c, const StateDev &old, const RadCall &rc, hipStream_t st) {
@@ -398,5 +412,25 @@ template int enqueue_radiation<double>(const CtxT<double> &, const StateDev &, c
 template int enqueue_radiation<float>(const CtxT<float> &, const StateDev &, const RadCall &, hipStream_t);
 template __global__ void k_rad<double>(const CtxT<double>, const StateDev, const RadDev, double, int);
 template __global__ void k_rad<float>(const CtxT<float>, const StateDev, const RadDev, double, int);
+
+// ---- the same kernels over a group of domains (one launch per phase) ------------------------
+LG_BATCH_KERNEL(rad_solver, BCOL, PH_RAD, b_rad(d.c, d.s[a.old_slot], d.rad, d.t_rad, a.flags, bid))
+LG_BATCH_KERNEL(rad_gather, BCOL, PH_RAD, b_rad_gather(d.c, d.s[a.old_slot], d.rad, d.t_rad, bid))
+LG_BATCH_KERNEL(rad_apply, BCOL, PH_RAD, b_rad_apply(d.c, d.s[a.old_slot], d.rad, d.t_rad, bid))
+LG_BATCH_KERNEL(rad_select, BCOL, PH_RAD, b_rad_select(d.rad, d.c.G.nx * d.c.G.ny, bid))
+
+template <class R>
+void batch_launch_radiation(int rph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm,
+                            const BatchArgs &a) {
+  switch (rph) {
+    case RPH_SOLVER: hipLaunchKernelGGL(kb_rad_solver<R>, g, b, 0, st, M, prefix, nm, a); break;
+    case RPH_GATHER: hipLaunchKernelGGL(kb_rad_gather<R>, g, b, 0, st, M, prefix, nm, a); break;
+    case RPH_APPLY: hipLaunchKernelGGL(kb_rad_apply<R>, g, b, 0, st, M, prefix, nm, a); break;
+    case RPH_SELECT: hipLaunchKernelGGL(kb_rad_select<R>, g, b, 0, st, M, prefix, nm, a); break;
+    default: break;
+  }
+}
+template void batch_launch_radiation<double>(int, dim3, dim3, hipStream_t, const MemberDev<double> *, const int *, int, const BatchArgs &);
+template void batch_launch_radiation<float>(int, dim3, dim3, hipStream_t, const MemberDev<float> *, const int *, int, const BatchArgs &);
 
 }  // namespace lgame

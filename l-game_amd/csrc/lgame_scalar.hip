@@ -21,6 +21,7 @@
 // extra layer) so that small domains still fill the GPU.
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 
 namespace lgame {
 
@@ -271,11 +272,11 @@ __device__ __forceinline__ void finish_cell(const CtxT<R> &c, const StateDev &nw
 // 64 lanes), dealt to the XCDs so that each takes a contiguous range of tiles with x fastest:
 // the rows above and below a tile are read by the blocks next to it on the same XCD.
 template <class R, bool TR, bool O3>
-__global__ void __launch_bounds__(BX * 4) k_scalar_col(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk, int nseg) {
+__device__ __forceinline__ void b_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw, int rk, int nseg, const Bid bid) {
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny, nl = G.nl;
   const int tx_n = (nx + SCOL_CELLS - 1) / SCOL_CELLS, ty_n = (ny + (int)blockDim.y - 1) / (int)blockDim.y;
-  const int n = tx_n * ty_n * nseg, b = blockIdx.x;
+  const int n = tx_n * ty_n * nseg, b = bid.x;
   const int k = b % N_XCD, s = b / N_XCD, q = n / N_XCD, rr = n % N_XCD;
   const int t = k * q + (k < rr ? k : rr) + s;
   const int tx = t % tx_n, ty = (t / tx_n) % ty_n, seg = t / (tx_n * ty_n);
@@ -317,6 +318,11 @@ __global__ void __launch_bounds__(BX * 4) k_scalar_col(const CtxT<R> c, const St
   }
 }
 
+template <class R, bool TR, bool O3>
+__global__ void __launch_bounds__(BX * 4) k_scalar_col(const CtxT<R> c, const StateDev ss, const StateDev nw, int rk, int nseg) {
+  b_scalar_col<R, TR, O3>(c, ss, nw, rk, nseg, bid_here());
+}
+
 template <class R>
 void launch_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw, int rk, int rows, int nseg, hipStream_t st) {
   const int tx_n = (c.G.nx + SCOL_CELLS - 1) / SCOL_CELLS, ty_n = (c.G.ny + rows - 1) / rows;
@@ -334,5 +340,31 @@ void launch_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw,
 
 template void launch_scalar_col<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, int, int, hipStream_t);
 template void launch_scalar_col<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, int, int, hipStream_t);
+
+// ---- the same kernel over a group of domains (one launch) --------------------------------------
+template <class R, bool TR, bool O3>
+__global__ void __launch_bounds__(BX * 4)
+    kb_scalar_col(const MemberDev<R> *__restrict__ M, const int *__restrict__ prefix, int nm, const BatchArgs a) {
+  Bid bid;
+  const MemberDev<R> &d = batch_block(M, prefix, nm, PH_SCALAR, bid);
+  b_scalar_col<R, TR, O3>(d.c, d.s[a.state_slot], d.s[a.new_slot], a.rk, d.seg, bid);
+}
+
+template <class R>
+void batch_launch_scalar(dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm, const BatchArgs &a,
+                         bool tr, bool o3) {
+  if (tr && o3)
+    hipLaunchKernelGGL((kb_scalar_col<R, true, true>), g, b, 0, st, M, prefix, nm, a);
+  else if (tr)
+    hipLaunchKernelGGL((kb_scalar_col<R, true, false>), g, b, 0, st, M, prefix, nm, a);
+  else if (o3)
+    hipLaunchKernelGGL((kb_scalar_col<R, false, true>), g, b, 0, st, M, prefix, nm, a);
+  else
+    hipLaunchKernelGGL((kb_scalar_col<R, false, false>), g, b, 0, st, M, prefix, nm, a);
+}
+template void batch_launch_scalar<double>(dim3, dim3, hipStream_t, const MemberDev<double> *, const int *, int, const BatchArgs &,
+                                          bool, bool);
+template void batch_launch_scalar<float>(dim3, dim3, hipStream_t, const MemberDev<float> *, const int *, int, const BatchArgs &,
+                                         bool, bool);
 
 }  // namespace lgame

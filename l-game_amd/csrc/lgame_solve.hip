@@ -9,6 +9,7 @@
 
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 
 namespace lgame {
 
@@ -268,8 +269,8 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
 }
 
 template <class R>
-__global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_solve_ver(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
+  const int col = bid.x * blockDim.x + threadIdx.x;
   const int nx = c.G.nx, ny = c.G.ny;
   if (col >= nx * ny) return;
   const int jk = col % nx + 1, ji = col / nx + 1;
@@ -277,6 +278,10 @@ __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const State
     ver_column<R, 1>(c, old, nw, ji, jk);
   else
     ver_column<R, 2>(c, old, nw, ji, jk);
+}
+template <class R>
+__global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  b_solve_ver(c, old, nw, rk, bid_here());
 }
 
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
@@ -623,7 +628,7 @@ __device__ __forceinline__ void gen_dispatch(const CtxT<R> &c, const StateDev &o
 }
 
 template <class R>
-__global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+__device__ __forceinline__ void b_solve_gen(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
   // sediment-heating terms of the condensates: csh(jl) = ((0 + a_1) - b_1) + a_2 - b_2 ...,
   // b only at the lowest layer (the surface sink term)
   __shared__ GenShared sh;
@@ -632,7 +637,7 @@ __global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const Ctx
   // the compiler so keeps the tracer selection in scalar registers and scalar branches
   const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int t = wy % NTR, grp = wy / NTR;
-  const int col_raw = (blockIdx.x * GEN_GROUPS + grp) * BCOL + lane;
+  const int col_raw = (bid.x * GEN_GROUPS + grp) * BCOL + lane;
   const bool active = col_raw < nx * ny;
   const int col = active ? col_raw : nx * ny - 1;  // idle lanes mirror a valid column, store nothing
   const int jk = col % nx + 1, ji = col / nx + 1;
@@ -649,10 +654,33 @@ __global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const Ctx
   const bool shared_ep = t == 4 || t == 5;  // graupel, vapour
   gen_back(c.G, c.w.col[2 * t + 1], c.w.col[shared_ep ? 2 : 2 * t], nw.rho[jc - 1], ji, jk, rk == 2 ? c.C.nonfinite : nullptr);
 }
+template <class R>
+__global__ void __launch_bounds__(BCOL * NTR * GEN_GROUPS) k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  b_solve_gen(c, old, nw, rk, bid_here());
+}
 
 template __global__ void k_solve_ver<double>(const CtxT<double>, const StateDev, const StateDev, int);
 template __global__ void k_solve_ver<float>(const CtxT<float>, const StateDev, const StateDev, int);
 template __global__ void k_solve_gen<double>(const CtxT<double>, const StateDev, const StateDev, int);
 template __global__ void k_solve_gen<float>(const CtxT<float>, const StateDev, const StateDev, int);
+
+// ---- the same kernels over a group of domains (one launch per phase) ------------------------
+LG_BATCH_KERNEL(solve_ver, BCOL, PH_SOLVE_VER, b_solve_ver(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
+LG_BATCH_KERNEL(solve_gen, BCOL * NTR * GEN_GROUPS, PH_SOLVE_GEN, b_solve_gen(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
+
+template <class R>
+bool batch_launch_solve(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm, const BatchArgs &a) {
+  if (ph == PH_SOLVE_VER) {
+    hipLaunchKernelGGL(kb_solve_ver<R>, g, b, 0, st, M, prefix, nm, a);
+    return true;
+  }
+  if (ph == PH_SOLVE_GEN) {
+    hipLaunchKernelGGL(kb_solve_gen<R>, g, b, 0, st, M, prefix, nm, a);
+    return true;
+  }
+  return false;
+}
+template bool batch_launch_solve<double>(int, dim3, dim3, hipStream_t, const MemberDev<double> *, const int *, int, const BatchArgs &);
+template bool batch_launch_solve<float>(int, dim3, dim3, hipStream_t, const MemberDev<float> *, const int *, int, const BatchArgs &);
 
 }  // namespace lgame

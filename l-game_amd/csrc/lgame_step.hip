@@ -7,6 +7,7 @@
 // with per-level values in coalesced column scratch.
 #include "lgame_ops.h"
 #include "lgame_step.h"
+#include "lgame_batch.h"
 
 namespace lgame {
 
@@ -19,7 +20,7 @@ namespace lgame {
 // (mo_p_grad.F90:106-120) and the sink velocity of cloud water, which depends only on this state
 // and the temperature (the generalized-density solver of both substeps reads it).
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_temperature(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   const ix_t i = LG_IX(c.G, ji, jk, jl);
@@ -35,6 +36,10 @@ __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const 
     c.w.vfall_cw[i] = v_fall_liquid(c, s, ji, jk, jl, (double)t, CLOUD_WATER_RADIUS);
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const StateDev s) {
+  b_temperature(c, s, bid_here());
+}
 
 // ---------------------------------------------------------------------------------------------
 // theta_v_adv_3rd_order, mo_multiplications.F90:116-204, in three passes.
@@ -42,17 +47,21 @@ __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const 
 // ---------------------------------------------------------------------------------------------
 // pass 1a: scalar_placeholder = theta_v_bg + theta_v_pert (f=0) or theta_v_pert (f=1)
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_fill(const CtxT<R> c, const StateDev s, int nfield) {
+__device__ __forceinline__ void b_theta_fill(const CtxT<R> &c, const StateDev &s, int nfield, const Bid bid) {
   KIDX;
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   const ix_t i = LG_IX(c.G, ji, jk, jl);
   c.w.tth[0][i] = c.g.theta_v_bg[i] + R(s.theta_v_pert[i]);
   if (nfield > 1) c.w.tth[1][i] = R(s.theta_v_pert[i]);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_theta_fill(const CtxT<R> c, const StateDev s, int nfield) {
+  b_theta_fill(c, s, nfield, bid_here());
+}
 
 // pass 1b: u,v_placeholder = grad_hor(theta) with its grad_vert inline
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_grad(const CtxT<R> c, int nfield) {
+__device__ __forceinline__ void b_theta_grad(const CtxT<R> &c, int nfield, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny;
   if (jk > nx + 1 || ji > ny + 1 || jl > c.G.nl) return;
@@ -61,18 +70,26 @@ __global__ void __launch_bounds__(BX * BY) k_theta_grad(const CtxT<R> c, int nfi
     if (jk <= nx) AT(c.w.tgv[f], ji, jk, jl) = grad_hor_y(c, c.w.tth[f], ji, jk, jl);
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_theta_grad(const CtxT<R> c, int nfield) {
+  b_theta_grad(c, nfield, bid_here());
+}
 
 // pass 2: flux_density_div = div_h(u_placeholder, v_placeholder)
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_div(const CtxT<R> c, int nfield) {
+__device__ __forceinline__ void b_theta_div(const CtxT<R> &c, int nfield, const Bid bid) {
   KIDX;
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   for (int f = 0; f < nfield; ++f) AT(c.w.tfd[f], ji, jk, jl) = div_h(c, c.w.tgu[f], c.w.tgv[f], ji, jk, jl);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_theta_div(const CtxT<R> c, int nfield) {
+  b_theta_div(c, nfield, bid_here());
+}
 
 // pass 3: upwind-corrected edge values; the upwind direction comes from the given winds
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield) {
+__device__ __forceinline__ void b_theta_edge(const CtxT<R> &c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny;
   if (jk > nx + 1 || ji > ny + 1 || jl > c.G.nl) return;
@@ -101,6 +118,10 @@ __global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<con
       AT(c.w.tv[f], ji, jk, jl) = t;
     }
   }
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield) {
+  b_theta_edge(c, wind_u, wind_v, nfield, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -246,10 +267,14 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
 // reference evaluates them with the indices of column 1 and row 1); the threads of column 1 and
 // row 1 store them, and the launch covers the nx by ny by n_layers cells only.
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
+__device__ __forceinline__ void b_pgrad(const CtxT<R> &c, const StateDev &s, int lfirst, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   pgrad_interior(c, s, lfirst, ji, jk, jl);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
+  b_pgrad(c, s, lfirst, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -324,7 +349,7 @@ __device__ R pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, int ka, int i
 
 // calc_pot_vort (mo_vorticities.F90:189-421) with rel_vort (:19-187) inline
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_pot_vort(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   // The values of row ny+1 and column nx+1 are copies of row 1 and column 1 (the reference
   // evaluates them with the indices of row 1 and column 1): the threads of row 1 and column 1
   // store them, and the launch covers the nx by ny cells.  The three potential vorticities of a
@@ -363,12 +388,16 @@ __global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const Sta
   }
   if (ji == 1 && jk == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = ez;
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
+  b_pot_vort(c, s, bid_here());
+}
 
 // mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80).  The
 // u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1; the
 // threads of column 1 and row 1 store them, and the launch covers the nx by ny cells.
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_mass_flux(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   if (jk > nx || ji > ny || jl > nlev) return;
@@ -393,6 +422,10 @@ __global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const St
     if (ji == 1) AT(c.w.v_ph, ny + 1, jk, jl) = fv;
   }
   AT(c.w.w_ph, ji, jk, jl) = fw;
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s) {
+  b_mass_flux(c, s, bid_here());
 }
 
 // The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
@@ -449,17 +482,25 @@ __device__ __forceinline__ void vort_flux_interior(const CtxT<R> &c, int ji, int
 // indices of column 1 and row 1), so the threads of column 1 and row 1 store them too and the
 // launch covers the nx by ny cells only.
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c) {
+__device__ __forceinline__ void b_vort_flux(const CtxT<R> &c, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   vort_flux_interior(c, ji, jk, jl);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c) {
+  b_vort_flux(c, bid_here());
+}
 
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_ke(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_ke(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   AT(c.w.v_squared, ji, jk, jl) = ke_point(c, s, ji, jk, jl);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_ke(const CtxT<R> c, const StateDev s) {
+  b_ke(c, s, bid_here());
 }
 
 // grad_hor and grad_vert of the kinetic energy at the u, v and w points of one cell (ji <= ny,
@@ -519,7 +560,7 @@ __device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int j
 
 // grad_vert + grad_hor of the kinetic energy (mo_vector_tend_expl.F90:96-97)
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
+__device__ __forceinline__ void b_ke_grad(const CtxT<R> &c, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
@@ -532,6 +573,10 @@ __global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
   if (ji <= ny) AT(c.w.vsg_x, ji, jk, jl) = grad_hor_x(c, k2, ji, jk, jl);
   if (jk <= nx) AT(c.w.vsg_y, ji, jk, jl) = grad_hor_y(c, k2, ji, jk, jl);
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
+  b_ke_grad(c, bid_here());
+}
 
 // simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377, and the vertical diffusion of w
 // of mom_diff_v (:277-289,320-328).  mom_diff_tend_z at the two levels of the cell is formed here
@@ -542,7 +587,7 @@ __global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
 // (vert_hor_mom_viscosity, :312-489, has no caller; control.F90:427-428 zeroes them): those terms
 // are exact zeros and are not formed.
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const StateDev s) {
+__device__ __forceinline__ void b_dissipation(const CtxT<R> &c, const StateDev &s, const Bid bid) {
   KIDX;
   const Geom &G = c.G;
   const int nl = G.nl;
@@ -592,13 +637,17 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
   const R rs = (c.C.lmom_h || c.C.lmom_v) ? c.w.rsum[i] : rho_sum(c, s, i);
   c.w.heating_diss[i] = -rs * ip;
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const StateDev s) {
+  b_dissipation(c, s, bid_here());
+}
 
 // ---------------------------------------------------------------------------------------------
 // explicit wind tendencies (mo_vector_tend_expl.F90:128-180) and the forward update of u, v
 // (mo_manage_pchevi.F90:86-89)
 // ---------------------------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+__device__ __forceinline__ void b_vec_tend(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
@@ -632,6 +681,10 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
     c.w.tend_w[i] = old_w * c.w.tend_w[i] + new_w * x;
   }
 }
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  b_vec_tend(c, old, nw, rk, bid_here());
+}
 
 // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181) is the column-marching k_scalar_col of
 // lgame_scalar.hip.
@@ -655,5 +708,49 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE
+
+// ---- the same kernels over a group of domains (one launch per phase) ------------------------
+LG_BATCH_KERNEL(temperature, BX * BY, PH_TEMPERATURE, b_temperature(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(theta_fill, BX * BY, PH_THETA_FILL, b_theta_fill(d.c, d.s[a.state_slot], a.nfield, bid))
+LG_BATCH_KERNEL(theta_grad, BX * BY, PH_THETA_GRAD, b_theta_grad(d.c, a.nfield, bid))
+LG_BATCH_KERNEL(theta_div, BX * BY, PH_THETA_DIV, b_theta_div(d.c, a.nfield, bid))
+LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
+                b_theta_edge(d.c, Fld<const double>(d.s[a.state_slot].wind_u), Fld<const double>(d.s[a.state_slot].wind_v),
+                             a.nfield, bid))
+LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad(d.c, d.s[a.old_slot], a.lfirst, bid))
+LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
+LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], bid))
+LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, bid))
+LG_BATCH_KERNEL(ke, BX * BY, PH_KE, b_ke(d.c, d.s[a.state_slot], bid))
+LG_BATCH_KERNEL(ke_grad, BX * BY, PH_KE_GRAD, b_ke_grad(d.c, bid))
+LG_BATCH_KERNEL(dissipation, BX * BY, PH_DISSIPATION, b_dissipation(d.c, d.s[a.old_slot], bid))
+LG_BATCH_KERNEL(vec_tend, BX * BY, PH_VEC_TEND, b_vec_tend(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
+
+template <class R>
+bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm, const BatchArgs &a) {
+#define LG_B(P, K)                                                  \
+  case P:                                                           \
+    hipLaunchKernelGGL(K<R>, g, b, 0, st, M, prefix, nm, a);        \
+    return true;
+  switch (ph) {
+    LG_B(PH_TEMPERATURE, kb_temperature)
+    LG_B(PH_THETA_FILL, kb_theta_fill)
+    LG_B(PH_THETA_GRAD, kb_theta_grad)
+    LG_B(PH_THETA_DIV, kb_theta_div)
+    LG_B(PH_THETA_EDGE, kb_theta_edge)
+    LG_B(PH_PGRAD, kb_pgrad)
+    LG_B(PH_POT_VORT, kb_pot_vort)
+    LG_B(PH_MASS_FLUX, kb_mass_flux)
+    LG_B(PH_VORT_FLUX, kb_vort_flux)
+    LG_B(PH_KE, kb_ke)
+    LG_B(PH_KE_GRAD, kb_ke_grad)
+    LG_B(PH_DISSIPATION, kb_dissipation)
+    LG_B(PH_VEC_TEND, kb_vec_tend)
+    default: return false;
+  }
+#undef LG_B
+}
+template bool batch_launch_step<double>(int, dim3, dim3, hipStream_t, const MemberDev<double> *, const int *, int, const BatchArgs &);
+template bool batch_launch_step<float>(int, dim3, dim3, hipStream_t, const MemberDev<float> *, const int *, int, const BatchArgs &);
 
 }  // namespace lgame

@@ -465,9 +465,11 @@ class Mlp:
 
 
 class Ensemble:
-    """Independent domains advanced together, one graph launch per ensemble step."""
+    """Independent domains advanced together: BATCHED launches every kernel phase once over all
+    members (grouped by kernel sequence), GRAPH replays one graph with a branch per member,
+    SERIAL runs the members one after another."""
 
-    GRAPH, SERIAL = 0, 1
+    GRAPH, SERIAL, BATCHED = 0, 1, 2
 
     def __init__(self, members):
         self.lib = load_library()

@@ -9,8 +9,9 @@
   reported; dt 10 s (the value the workload names) turns the state non-finite.
 * Surrogate network 304 -> 3 x 512 -> 62 at batch 8192: the device gradient of the loss against
   the CPU oracle of the network.
-* 48 moist members of uneven size in the mixed form with the surrogate: one graph launch per step
-  against serial launches, bitwise, over steps that include a radiation update.
+* 48 moist members of uneven size in the mixed form with the surrogate: one launch per kernel phase
+  over all members, and one graph launch per step, against serial launches, bitwise, over steps
+  that include a radiation update.
 """
 import dataclasses
 
@@ -207,8 +208,10 @@ def _digest(st):
 def test_ensemble_48_uneven_members_graph_equals_serial():
     """48 moist members (nx, ny ~ U[64,384], 40-70 layers), mixed form with the radiation
     surrogate and its 1 % audit (members of other than 60 layers resampled to the network's
-    levels): one graph launch per ensemble step against serial launches, over 4 steps whose
-    second updates the radiation; every member's state is bitwise the same."""
+    levels): one launch per kernel phase over all members (the radiation update as one gather,
+    one inference over every member's columns and one audit), and one graph launch per ensemble
+    step, against serial launches, over 4 steps whose second updates the radiation; every
+    member's state is bitwise the same."""
     from lgame import surrogate
 
     cfgs = [dataclasses.replace(c, precision=PREC_MIXED, lrad=True, dtime_rad=900.0, rad_mode=RAD_SURROGATE)
@@ -216,7 +219,7 @@ def test_ensemble_48_uneven_members_graph_equals_serial():
     mlp = surrogate.load(surrogate.DEFAULT_WEIGHTS)
     n, t0 = 4, 1576843200.0 + 3 * 3600.0
     digests = {}
-    for mode in (Ensemble.GRAPH, Ensemble.SERIAL):
+    for mode in (Ensemble.BATCHED, Ensemble.GRAPH, Ensemble.SERIAL):
         steps = []
         for i, c in enumerate(cfgs):
             g = grid_setup_device(c)
@@ -238,6 +241,7 @@ def test_ensemble_48_uneven_members_graph_equals_serial():
         e.close()
         for s in steps:
             s.close()
-    same = sum(a == b for a, b in zip(digests[Ensemble.GRAPH], digests[Ensemble.SERIAL]))
-    print(f"members bitwise equal between graph and serial launches: {same}/48")
-    assert same == 48
+    for mode in (Ensemble.BATCHED, Ensemble.GRAPH):
+        same = sum(a == b for a, b in zip(digests[mode], digests[Ensemble.SERIAL]))
+        print(f"members bitwise equal between mode {mode} and serial launches: {same}/48")
+        assert same == 48

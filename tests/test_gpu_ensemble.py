@@ -64,7 +64,7 @@ def test_scalar_kernel_segments_and_rows_keep_results():
     s.close()
 
 
-@pytest.mark.parametrize("mode", [Ensemble.GRAPH, Ensemble.SERIAL])
+@pytest.mark.parametrize("mode", [Ensemble.GRAPH, Ensemble.SERIAL, Ensemble.BATCHED])
 def test_ensemble_members_match_oracle(mode):
     cfgs = [tiny_config(ny=5, nx=7, n_layers=6, moist=True, order=3),
             tiny_config(ny=9, nx=11, n_layers=8),
