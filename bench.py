@@ -27,11 +27,14 @@ and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, d
 and budget sums) with the MLP radiation surrogate and its 1% audit.  Radiation updates follow
 the model's cadence (dtime_rad / dtime steps), the first one inside the first timed step.  The
 headline value is the mixed form; `forms` lists both and `speedup_mixed_vs_fp64` their ratio.
-The roofline object is the implicit vertical solver (three_band_solver_ver) of the plain FP64 form,
-timed with HIP events on the library's own stream; its algorithmic bytes are 152 B (predictor)
-and 176 B (corrector) per grid point (state, tendency and grid arrays streamed once, FP64; the
-mixed form reads its three tendency arrays as FP32, 12 B fewer).  `step_roofline` puts the whole
-FP64 step against the 1296 B per grid-point step model, beside the measured traffic.
+Every kernel is timed with HIP events on the library's own stream in a separate pass
+(`forms.*.kernel_ms_per_step`).  `roofline` is the kernel that takes the most time per step in the
+headline form, `roofline_fp64` the same in the plain FP64 form, each against its algorithmic
+bytes per grid point (every array it must read or write counted once, at the width the form
+stores it: KERNEL_BYTES); `roofline_solver_ver` keeps the implicit vertical solver
+(three_band_solver_ver) of the FP64 form, 152 B (predictor) and 176 B (corrector) per grid point.
+`step_roofline` puts the whole FP64 step against the 1296 B per grid-point step model, beside
+the measured traffic.
 The CPU baseline is the oracle (the C restatement) on all host cores, as independent instances of
 a bounded sample of the same workload type, one per thread, with the same radiation schedule;
 each GPU form is run on that sample and its per-field RMS and max error against the oracle are
@@ -57,13 +60,37 @@ BF16_PEAK_TFLOPS = 2500.0
 # reads 14 (17) arrays and writes 5, all FP64; the mixed form reads its three tendency arrays
 # (rho_gas, rhotheta_v, w) as FP32, 12 B fewer
 SOLVER_BYTES = {"fp64": {1: 152.0, 2: 176.0}, "mixed": {1: 140.0, 2: 164.0}}
+# Algorithmic bytes per grid point and launch of the kernels that can dominate a step (moist,
+# theta order 2, no diffusion), per form; each is launched twice per step (predictor, corrector).
+# scalar_div (k_scalar_col, scalar_tend_expl): reads the gas and six tracer densities,
+#   theta_v_pert, exner_pert and the new u, v (11 FP64 arrays); theta_v_bg, area_x, area_y, area_z,
+#   volume, slope_x, slope_y, the four horizontal inner-product weights and exner_bg (12 grid
+#   arrays of the form's width); the heating terms (heating_diss, radiation, condensates),
+#   temperature and the six tracer tendencies it updates (10 work arrays); writes the seven density
+#   and the rhotheta_v tendencies (8 work arrays): FP64 88 + 96 + 80 + 64 = 328 B, mixed
+#   88 + 48 + 40 + 32 = 208 B.
+# solve_gen (k_solve_gen, three_band_solver_gen_densities): per tracer the old density, the new
+#   density and the tendency read and the new density written (6 x 32 B in FP64), shared per
+#   column old and new w, area_z, volume, the sink velocity of cloud water and the temperature
+#   read and the latent heating written (56 B): 248 B; the mixed form reads the six tendencies,
+#   the temperature and writes the heating at 4 B: 216 B.
+# solve_ver: the mean of the predictor and corrector counts of SOLVER_BYTES.
+KERNEL_BYTES = {"scalar_div": {"fp64": 328.0, "mixed": 208.0}, "solve_gen": {"fp64": 248.0, "mixed": 216.0},
+                "solve_ver": {"fp64": 164.0, "mixed": 152.0}}
+KERNEL_NAMES = {"scalar_div": "k_scalar_col (scalar_tend_expl, column-marching)",
+                "solve_gen": "k_solve_gen (three_band_solver_gen_densities, FP64 solve)",
+                "solve_ver": "k_solve_ver (three_band_solver_ver, FP64 solve)"}
 STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step traffic model (FP64)
 # HBM bytes per k_solve_ver dispatch, averaged over predictor and corrector, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB device copy gives), per
 # form of the step; profiles/ holds the commands and the per-kernel tables.
 SOLVER_TRAFFIC = {("moist512", "fp64"): 4.369e+09, ("moist512", "mixed"): 4.128e+09}
+# the same for the kernels of KERNEL_BYTES, (workload, kernel, form) -> bytes per launch
+KERNEL_TRAFFIC = {("moist512", "solve_ver", "fp64"): 4.369e+09, ("moist512", "solve_ver", "mixed"): 4.128e+09,
+                  ("moist512", "scalar_div", "fp64"): 5.694e+09, ("moist512", "scalar_div", "mixed"): 4.269e+09,
+                  ("moist512", "solve_gen", "fp64"): 5.652e+09, ("moist512", "solve_gen", "mixed"): 5.264e+09}
 # measured HBM bytes per grid-point step of the whole FP64 step (same counters, all kernels)
-STEP_TRAFFIC = {"moist512": 3947.0}
+STEP_TRAFFIC = {"moist512": 3575.0}
 # the reference's own CPU path, flang -O2 -fopenmp on 8 cores (the survey's measurement; the
 # reference tree does not travel, so it cannot be timed beside the GPU)
 REFERENCE_CPU = {"dry128": {"value": 3.19e6, "grid": "129x129x50 dry", "cores": 8},
@@ -148,7 +175,7 @@ def run_steps(step, cfg, first, n):
 
 def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
     """Throughput of one form of the step plus event timing of the FP64 column solver."""
-    from lgame.gpu import PROF_RADIATION, PROF_SOLVE_GEN, PROF_SOLVE_VER, PROF_STEP
+    from lgame.gpu import PROF_KERNEL0, PROF_RADIATION, PROF_SOLVE_GEN, PROF_SOLVE_VER, PROF_STEP, TILE_KERNELS
 
     step = make_step(cfg, grid, surface, form, mlp)
     step.start(st)
@@ -163,10 +190,17 @@ def time_form(cfg, grid, st, surface, form, mlp, steps, warmup):
     counter = warmup + steps
     bad = step.count_nonfinite(counter % 2)
     rad = step.rad_stats() if cfg.lrad else {}
-    # kernel timing with HIP events on the library's stream (separate pass)
-    step.profile(True)
-    run_steps(step, cfg, counter, max(2, min(steps, 6)))
+    # kernel timing with HIP events on the library's stream (separate pass, every kernel on its own)
+    step.profile(2)
+    nprof = max(2, min(steps, 6))
+    run_steps(step, cfg, counter, nprof)
     prof = {g: step.profile_read(g) for g in (PROF_SOLVE_VER, PROF_SOLVE_GEN, PROF_STEP, PROF_RADIATION)}
+    kernels = {"solve_ver": prof[PROF_SOLVE_VER], "solve_gen": prof[PROF_SOLVE_GEN]}
+    for i, name in enumerate(TILE_KERNELS):
+        ms, n = step.profile_read(PROF_KERNEL0 + i)
+        if n:
+            kernels[name] = (ms, n)
+    prof["kernels"] = {k: (ms / nprof, n // nprof) for k, (ms, n) in kernels.items() if n}  # per step
     step.profile(False)
     step.close()
     return elapsed, bad, prof, dict(rad, tiles=tiles)
@@ -562,8 +596,22 @@ def main():
     head = "mixed" if "mixed" in res else "fp64"
     elapsed, bad, prof, rad = res[head]
     value = cells * args.steps / elapsed
-    # the roofline object: the FP64 form's column solver where it was timed (the FP64 solves of the
-    # north star), else the timed form's with its own byte count
+    # the roofline objects: the kernel with the most time per step in each form
+    def kernel_roofline(form):
+        ks = res[form][2]["kernels"]
+        name = max(ks, key=lambda k: ks[k][0])
+        ms, n = ks[name]
+        b = KERNEL_BYTES.get(name, {}).get(form)
+        out_r = {"kernel": f"{KERNEL_NAMES.get(name, name)} of the {form} form", "bound": "hbm", "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "ms_per_step": ms, "launches_per_step": n, "avg_launch_us": 1e3 * ms / max(n, 1),
+                 "share_of_step": ms / (1e3 * res[form][0] / args.steps)}
+        if b is None:
+            return dict(out_r, achieved=None, frac=None, traffic=None)
+        gbs = b * cells * n / (ms * 1e-3) / 1e9
+        return dict(out_r, achieved=gbs, frac=gbs / HBM_PEAK_GBS, algorithmic_bytes_per_point=b,
+                    traffic=KERNEL_TRAFFIC.get((args.workload, name, form)) if not (cfg.diffusion or cfg.lpbl) else None,
+                    traffic_unit="bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)")
+
     rf = "fp64" if "fp64" in res else head
     ms_solver, n_solver = res[rf][2][PROF_SOLVE_VER]
     ms_step, n_step = prof[PROF_STEP]
@@ -607,9 +655,11 @@ def main():
         },
         "forms": {f: {"value": cells * args.steps / r[0], "ms_per_step": 1e3 * r[0] / args.steps,
                       "solver_ver_us": 1e3 * r[2][PROF_SOLVE_VER][0] / max(r[2][PROF_SOLVE_VER][1], 1),
-                      "radiation_update_us": rad_us[f], "radiation": r[3]}
+                      "radiation_update_us": rad_us[f], "radiation": r[3],
+                      "kernel_ms_per_step": {k: v[0] for k, v in sorted(r[2]["kernels"].items(), key=lambda kv: -kv[1][0])}}
                   for f, r in res.items()},
-        "roofline": {
+        "roofline": kernel_roofline(head),
+        "roofline_solver_ver": {
             "kernel": f"k_solve_ver (three_band_solver_ver, both substeps, FP64 solve) of the {rf} form",
             "bound": "hbm",
             "achieved": solver_gbs,
@@ -622,6 +672,7 @@ def main():
             "avg_launch_us": avg_launch_us,
             "launches_timed": n_solver,
         },
+        "roofline_fp64": kernel_roofline("fp64") if "fp64" in res else None,
         "step_roofline": {
             "form": "fp64",
             "bytes_per_gp_step_model": step_model,

@@ -439,13 +439,16 @@ int lgame_get_diag(lgame_handle *h, int which, double *out);
 /*
  * Event timing of kernel groups on the handle's stream.  No counterpart in the reference, which
  * only prints a model speed (control.F90:544-547); for profiling tools and benchmarks.
- * lgame_profile(h, 1) starts recording (and clears earlier records); lgame_profile_read waits for
- * the stream and returns the summed milliseconds and launch count of one group.
+ * lgame_profile(h, 1) starts recording (and clears earlier records); lgame_profile(h, 2) also
+ * times every stencil kernel on its own (group LGAME_PROF_KERNEL0 + its index in the kernel order
+ * of lgame_set_tiles; the scalar divergence kernel is index 14); lgame_profile_read waits for the
+ * stream and returns the summed milliseconds and launch count of one group.
  */
 #define LGAME_PROF_SOLVE_VER   0   /* three_band_solver_ver kernels                  */
 #define LGAME_PROF_SOLVE_GEN   1   /* three_band_solver_gen_densities kernels        */
 #define LGAME_PROF_STEP        2   /* whole lgame_pchevi calls                       */
 #define LGAME_PROF_RADIATION   3   /* radiation updates (solver or surrogate + audit) */
+#define LGAME_PROF_KERNEL0     8   /* + tile-table index: one stencil kernel (lgame_profile(h, 2)) */
 int lgame_profile(lgame_handle *h, int enable);
 int lgame_profile_read(lgame_handle *h, int group, double *total_ms, long long *count);
 int lgame_profile_reset(lgame_handle *h);

@@ -500,7 +500,7 @@ extern "C" {
 int lgame_profile(lgame_handle *h, int enable) {
   if (!h) return fail(h, LGAME_EINVAL, "null handle");
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->prof = enable != 0;
+  h->prof = enable == 2 ? 2 : (enable != 0 ? 1 : 0);
   h->ev_marks.clear();
   h->ev_next = 0;
   return LGAME_OK;
@@ -1069,7 +1069,7 @@ int lgame_pchevi(lgame_handle *h, int slot_old, int slot_new, int total_step_cou
     rad = &rc;
     ++h->rad_calls;
   }
-  EventProf ep(h);
+  EventProf ep(h, h->prof == 2);
   ProfHook *ph = h->prof ? &ep : nullptr;
   if (ph) ep.begin(LGAME_PROF_STEP);
   int erc;
@@ -1318,6 +1318,7 @@ int lgame_get_diag(lgame_handle *h, int which, double *out) {
 extern "C" {
 
 static_assert(KI_COUNT == LGAME_N_TILES, "tile table and LGAME_N_TILES");
+static_assert(PROF_KERNEL0 == LGAME_PROF_KERNEL0 && KI_SCALAR_DIV == 14, "per-kernel profile groups");
 
 int lgame_autotune(lgame_handle *h, int steps, int *rows) {
   if (!h) return fail(h, LGAME_EINVAL, "null handle");

@@ -136,6 +136,27 @@ module mo_lgame_gpu
       integer(c_long_long),      intent(out) :: n_bad
     end function lgame_check_finite
 
+    ! the checkpoint of the fields the step carries between steps (an exact restart)
+    integer(c_int) function lgame_checkpoint_size(handle,nbytes) bind(C,name="lgame_checkpoint_size")
+      import :: c_int,c_ptr,c_long_long
+      type(c_ptr),          value, intent(in)  :: handle
+      integer(c_long_long),        intent(out) :: nbytes
+    end function lgame_checkpoint_size
+
+    integer(c_int) function lgame_checkpoint_save(handle,buf,nbytes) bind(C,name="lgame_checkpoint_save")
+      import :: c_int,c_ptr,c_long_long
+      type(c_ptr),          value, intent(in) :: handle
+      type(c_ptr),          value, intent(in) :: buf
+      integer(c_long_long), value, intent(in) :: nbytes
+    end function lgame_checkpoint_save
+
+    integer(c_int) function lgame_checkpoint_load(handle,buf,nbytes) bind(C,name="lgame_checkpoint_load")
+      import :: c_int,c_ptr,c_long_long
+      type(c_ptr),          value, intent(in) :: handle
+      type(c_ptr),          value, intent(in) :: buf
+      integer(c_long_long), value, intent(in) :: nbytes
+    end function lgame_checkpoint_load
+
     subroutine lgame_finalize(handle) bind(C,name="lgame_finalize")
       import :: c_ptr
       type(c_ptr), value, intent(in) :: handle

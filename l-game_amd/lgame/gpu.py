@@ -37,6 +37,7 @@ PROF_SOLVE_VER = 0
 PROF_SOLVE_GEN = 1
 PROF_STEP = 2
 PROF_RADIATION = 3
+PROF_KERNEL0 = 8  # + index in TILE_KERNELS, with profile(2)
 
 RAD_OUT_TENDENCY = 0
 RAD_OUT_SW_IN = 1
@@ -361,7 +362,8 @@ class Step:
             self._check(rc, "lgame_check_finite")
         return int(n.value)
 
-    def profile(self, enable: bool = True):
+    def profile(self, enable: int = 1):
+        """1: time the solver, step and radiation groups; 2: also every stencil kernel."""
         self._check(self.lib.lgame_profile(self.h, int(enable)), "lgame_profile")
 
     def profile_read(self, group: int):
