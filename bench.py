@@ -100,6 +100,10 @@ METRIC = "grid-point-steps/s at 1 GPU: mixed+surrogate vs FP64 plain, at stated 
 DEFAULT_SURROGATE = os.path.join(ROOT, "l-game_amd", "data", "rad_surrogate.npz")
 SURROGATE_WEIGHTS = DEFAULT_SURROGATE
 T_EPOCH = 1576843200.0 + 3 * 3600.0  # a time of day with sunlit columns
+# the moist workloads start with cloud liquid (above 0 C) and cloud ice (below) (config 3 names both
+# as separate mass densities): up to 0.5% of the vapour density at saturation, times a smooth
+# random cloud cover per column (lgame.cases.cloud_patches), as in the surrogate's training columns
+CLOUD_FRAC = 0.005
 NO_AUTOTUNE = False
 MIXED_RAD = "surrogate"
 
@@ -124,7 +128,7 @@ def workload(name):
 
 
 def describe(cfg):
-    kind = "moist" if cfg.lmoist else "dry"
+    kind = f"moist (patchy cloud liquid and ice, up to {100 * CLOUD_FRAC:g}% of the vapour)" if cfg.lmoist else "dry"
     rad = f", radiation every {cfg.dtime_rad:g} s" if cfg.lrad else ", radiation off"
     diff = (f", diffusion on ({cfg.diff_coeff_scheme_h} horizontal, TKE vertical)" if cfg.diffusion else "")
     diff += ", PBL friction on" if cfg.lpbl else ""
@@ -222,11 +226,13 @@ def cpu_baseline(cfg, nsteps, forms, mlp, threads):
     import threading
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from lgame.cases import make_case, radiation_surface
+    from lgame.cases import cloud_patches, make_case, radiation_surface
     from lgame.metrics import field_errors, worst
     from pyoracle import CpuOracle
 
-    grid, st = make_case(cfg)
+    grid, st = make_case(cfg, cloud_frac=CLOUD_FRAC)
+    if cfg.lmoist:
+        cloud_patches(cfg, st)
     surface = radiation_surface(cfg, grid, st)
     every = max(1, int(round(cfg.dtime_rad / cfg.dt))) if cfg.lrad else 0
     orcs = []
@@ -559,7 +565,7 @@ def main():
     if args.workload == "grid-setup":
         return bench_grid(args)
 
-    from lgame.cases import make_case, radiation_surface
+    from lgame.cases import cloud_patches, make_case, radiation_surface
     from lgame.gpu import PROF_RADIATION, PROF_SOLVE_VER, PROF_STEP
 
     if args.copy_calibration:
@@ -580,7 +586,9 @@ def main():
     mlp = surrogate_mlp() if (cfg.lrad and "mixed" in forms) else None
     t0 = time.perf_counter()
     log(f"setting up {describe(cfg)}")
-    grid, st = make_case(cfg)
+    grid, st = make_case(cfg, cloud_frac=CLOUD_FRAC)
+    if cfg.lmoist:
+        cloud_patches(cfg, st)
     surface = radiation_surface(cfg, grid, st) if cfg.lrad else None
     log(f"set-up done in {time.perf_counter() - t0:.1f} s")
     if args.accuracy_hours > 0.0:

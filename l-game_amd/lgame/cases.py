@@ -130,6 +130,18 @@ def make_case(cfg: RunConfig, seed: int | None = None, perturbation: bool = True
     return g, st
 
 
+def cloud_patches(cfg: RunConfig, st: dict, seed: int = 17, offset: float = 0.0) -> None:
+    """Patchy clouds (set-up extension): scales the condensates of every column by a smooth
+    random cloud cover in [0, 1] (correlation length 2 grid points, the construction the
+    radiation surrogate's training columns use)."""
+    from .init_state import _smooth_periodic
+
+    rng = np.random.default_rng(seed)
+    cover = np.clip(_smooth_periodic(rng.standard_normal((cfg.ny, cfg.nx)), 2.0) * 0.6 + offset, 0.0, 1.0)
+    for jc in range(cfg.n_condensed_constituents):
+        st["rho"][:, :, :, jc] *= cover[:, :, None]
+
+
 def tiny_config(ny=5, nx=7, n_layers=6, moist=False, order=2, freeslip=False, corio=True, plane=False, **kw) -> RunConfig:
     cfg = RunConfig(ny=ny, nx=nx, n_layers=n_layers, n_oro_layers=n_layers - 2, dy=2000.0, dx=2000.0,
                     toa=12000.0, stretching_parameter=1.3, scenario="standard", lplane=plane,
