@@ -1550,13 +1550,7 @@ int build_group(lgame_ensemble *e, BatchGroup &g) {
   ECHK(e, hipMalloc(&g.mem, g.mem_bytes));
   ECHK(e, hipMalloc(&g.prefix, sizeof(int) * PH_COUNT * (n + 1)));
   ECHK(e, hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming));
-  if (g.surrogate) {
-    for (auto *h : g.m) g.cols += (long long)h->c.G.nx * h->c.G.ny;
-    ECHK(e, hipMalloc(&g.rx, (size_t)g.cols * LGAME_RAD_N_IN * sizeof(float)));
-    ECHK(e, hipMalloc(&g.ry, (size_t)g.cols * LGAME_RAD_N_OUT * sizeof(float)));
-    const int rc = mlp_work_create(g.m[0]->mlp, g.cols < 262144 ? g.cols : 262144, &g.work);
-    if (rc) return efail(e, rc, "surrogate inference buffers of the group");
-  }
+  for (auto *h : g.m) g.cols += (long long)h->c.G.nx * h->c.G.ny;
   MemberDev<R> *M = static_cast<MemberDev<R> *>(g.host);
   // The column-marching scalar kernel cuts a small domain's columns into segments to fill the
   // GPU; in one launch over the whole group the members' rows fill it together, so the segments
@@ -1565,7 +1559,6 @@ int build_group(lgame_ensemble *e, BatchGroup &g) {
   long long waves = 0;
   for (auto *h : g.m) waves += (long long)((h->c.G.nx + SCOL_CELLS - 1) / SCOL_CELLS) * h->c.G.ny;
   const bool one_seg = waves >= 4096;
-  long long row = 0;
   for (int q = 0; q < n; ++q) {
     lgame_handle *h = g.m[q];
     MemberDev<R> &d = M[q];
@@ -1573,15 +1566,7 @@ int build_group(lgame_ensemble *e, BatchGroup &g) {
     d.c = ctx_of<R>(h);
     d.s[0] = h->slot[0];
     d.s[1] = h->slot[1];
-    d.seg = one_seg ? 1 : h->tiles.seg;
-    if (g.rad) {
-      d.rad = rad_dev(h);
-      if (g.surrogate) {
-        d.rad.x = g.rx + (size_t)row * LGAME_RAD_N_IN;
-        d.rad.ypred = g.ry + (size_t)row * LGAME_RAD_N_OUT;
-      }
-    }
-    row += (long long)h->c.G.nx * h->c.G.ny;
+    d.seg = one_seg ? 1 : h->tiles.seg;  // the radiation part is filled at each radiation update
   }
   std::vector<int> pf((size_t)PH_COUNT * (n + 1), 0);
   for (int ph = 0; ph < PH_COUNT; ++ph) {
@@ -1624,9 +1609,49 @@ int build_groups(lgame_ensemble *e) {
   }
   for (auto &g : e->groups) {
     const int rc = g.mixed ? build_group<float>(e, g) : build_group<double>(e, g);
-    if (rc) return rc;
+    if (rc) {  // leave no half-made groups behind: the next step starts over
+      for (auto &q : e->groups) free_group(q);
+      e->groups.clear();
+      return rc;
+    }
   }
   e->grouped = true;
+  return 0;
+}
+
+// The radiation part of a group's device table at an update: every member's own radiation data,
+// with the surrogate's inputs and outputs in the group's matrices (made at the first update, with
+// the members' network, which must be one and the same), the call index and the time.
+template <class R>
+int prepare_group_radiation(lgame_ensemble *e, BatchGroup &g, double t_0) {
+  if (g.surrogate) {
+    const lgame_mlp *mlp = g.m[0]->mlp;
+    for (auto *h : g.m)
+      if (h->mlp != mlp) return efail(e, LGAME_ESTATE, "the members of a batched group need the same surrogate network");
+    if (!g.work) {
+      ECHK(e, hipMalloc(&g.rx, (size_t)g.cols * LGAME_RAD_N_IN * sizeof(float)));
+      ECHK(e, hipMalloc(&g.ry, (size_t)g.cols * LGAME_RAD_N_OUT * sizeof(float)));
+      const int rc = mlp_work_create(mlp, g.cols < 262144 ? g.cols : 262144, &g.work);
+      if (rc) return efail(e, rc, "surrogate inference buffers of the group");
+    }
+  }
+  ECHK(e, hipEventSynchronize(g.uploaded));  // the launches that read the previous table are done with it
+  MemberDev<R> *H = static_cast<MemberDev<R> *>(g.host);
+  long long row = 0;
+  for (size_t q = 0; q < g.m.size(); ++q) {
+    lgame_handle *h = g.m[q];
+    H[q].c = ctx_of<R>(h);  // with the surface-flux pointers of a radiation set up after the grouping
+    H[q].rad = rad_dev(h);
+    if (g.surrogate) {
+      H[q].rad.x = g.rx + (size_t)row * LGAME_RAD_N_IN;
+      H[q].rad.ypred = g.ry + (size_t)row * LGAME_RAD_N_OUT;
+    }
+    H[q].rad.call = h->rad_calls++;
+    H[q].t_rad = t_0 + 0.5 * h->cfg.dtime_rad;  // mo_manage_radiation_calls.F90:53
+    row += (long long)h->c.G.nx * h->c.G.ny;
+  }
+  ECHK(e, hipMemcpyAsync(g.mem, g.host, g.mem_bytes, hipMemcpyHostToDevice, e->master));
+  ECHK(e, hipEventRecord(g.uploaded, e->master));
   return 0;
 }
 
@@ -1659,17 +1684,8 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
   // update_sfc_turb_quantities and update_rad_fluxes with state_old (mo_manage_pchevi.F90:52-58)
   if (C.lsfc_turb) launch(PH_SFC_TURB, base);
   if (rad_update && g.rad) {
-    // the call index and time of every member (mo_manage_radiation_calls.F90:53), then one
-    // upload of the table, ordered after the launches that read the previous one
-    ECHK(e, hipEventSynchronize(g.uploaded));
-    MemberDev<R> *H = static_cast<MemberDev<R> *>(g.host);
-    for (int q = 0; q < n; ++q) {
-      lgame_handle *h = g.m[q];
-      H[q].rad.call = h->rad_calls++;
-      H[q].t_rad = t_0 + 0.5 * h->cfg.dtime_rad;
-    }
-    ECHK(e, hipMemcpyAsync(g.mem, g.host, g.mem_bytes, hipMemcpyHostToDevice, st));
-    ECHK(e, hipEventRecord(g.uploaded, st));
+    const int rrc = prepare_group_radiation<R>(e, g, t_0);
+    if (rrc) return rrc;
     const dim3 grd((unsigned)g.total[PH_RAD]), blk(BCOL);
     const int *pf = g.prefix + (size_t)PH_RAD * (n + 1);
     BatchArgs a = base;

@@ -92,3 +92,40 @@ def test_ensemble_members_match_oracle(mode):
     e.close()
     for s in steps:
         s.close()
+
+
+def test_batched_radiation_set_up_after_the_first_step():
+    """The batched groups are made at the first batched step; a radiation set up afterwards (surface
+    data, surrogate) is taken up at the first radiation update: the batched steps stay bitwise equal
+    to serial ones."""
+    from lgame import surrogate
+    from lgame.cases import moist3d_config, radiation_surface
+    from lgame.config import PREC_MIXED, RAD_SURROGATE
+
+    mlp = surrogate.load(surrogate.DEFAULT_WEIGHTS)
+    cfgs = [dataclasses.replace(moist3d_config(n=n, n_layers=60, seed=30 + n), precision=PREC_MIXED, lrad=True,
+                                rad_mode=RAD_SURROGATE, dtime_rad=900.0) for n in (33, 47)]
+    cases = [make_case(c, cloud_frac=0.005) for c in cfgs]
+    t0 = 1576843200.0 + 3 * 3600.0
+    out = {}
+    for mode in (Ensemble.BATCHED, Ensemble.SERIAL):
+        steps = []
+        for c, (g, st) in zip(cfgs, cases):
+            s = Step(c, g)
+            s.start(st)
+            steps.append(s)
+        e = Ensemble(steps)
+        e.step(0, mode=mode)
+        for c, (g, st), s in zip(cfgs, cases, steps):
+            s.set_rad_surface(radiation_surface(c, g, st))
+            s.attach_mlp(mlp, tol_k_per_day=1.0, audit_fraction=0.05, seed=3)
+        for k in range(1, 4):
+            e.step(k, lrad_update=(k == 1), t_0=t0 + k * cfgs[0].dt, mode=mode)
+        e.sync()
+        out[mode] = [s.get_state(0) for s in steps]
+        assert all(s.rad_stats()["calls"] == 1 for s in steps)
+        e.close()
+        for s in steps:
+            s.close()
+    for a, b in zip(out[Ensemble.BATCHED], out[Ensemble.SERIAL]):
+        assert all(np.array_equal(a[k], b[k]) for k in a)
