@@ -17,7 +17,7 @@ namespace lgame {
 
 enum Phase : int {
   PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
-  PH_MASS_FLUX, PH_VORT_FLUX, PH_KE, PH_KE_GRAD, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
+  PH_MASS_FLUX, PH_VORT_FLUX, PH_KE_GRAD, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
   PH_DISSIPATION, PH_VEC_TEND, PH_DIFF_FLUX, PH_DIFF_DIV, PH_SCALAR, PH_SURFACE, PH_SOLVE_VER, PH_SOLVE_GEN, PH_RAD,
   PH_COUNT
 };
@@ -35,7 +35,6 @@ inline int phase_tile(int ph) {
     case PH_POT_VORT: return KI_POT_VORT;
     case PH_MASS_FLUX: return KI_MASS_FLUX;
     case PH_VORT_FLUX: return KI_VORT_FLUX;
-    case PH_KE: return KI_KE;
     case PH_KE_GRAD: return KI_KE_GRAD;
     case PH_DIFF_ZETA: return KI_DIFF_ZETA;
     case PH_DIFF_COEF: return KI_DIFF_COEF;
@@ -65,6 +64,7 @@ struct BatchArgs {
   int old_slot, new_slot;  // state_old and state_new
   int state_slot;          // the slot of the phase's state argument (state_old, or state_new at rk 2)
   int rk, lfirst, nfield, flags;
+  int ke;  // the mass-flux kernel also forms the kinetic energy
 };
 
 // Blocks of phase ph for one member as (gx, gy, gz), with `by` rows per block for the stencil
@@ -76,7 +76,7 @@ __host__ __device__ inline void phase_grid(int ph, const Geom &G, int by, int se
   bool flat = false;  // a 1-D grid (tile_zfast kernels)
   gx = gy = gz = 0;
   switch (ph) {
-    case PH_TEMPERATURE: case PH_THETA_FILL: case PH_THETA_DIV: case PH_KE: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
+    case PH_TEMPERATURE: case PH_THETA_FILL: case PH_THETA_DIV: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
     case PH_SFC_TURB: case PH_PBL_WIND: case PH_SURFACE: ez = 1; break;
     case PH_THETA_GRAD: case PH_THETA_EDGE: case PH_KE_GRAD: case PH_VEC_TEND: ex = nx + 1; ey = ny + 1; break;
     case PH_MASS_FLUX: ez = nlev; break;
@@ -143,7 +143,6 @@ __device__ __forceinline__ const MemberDev<R> &batch_block(const MemberDev<R> *_
 LG_BATCH_LAUNCHER(batch_launch_step)
 LG_BATCH_LAUNCHER(batch_launch_diff)
 LG_BATCH_LAUNCHER(batch_launch_solve)
-LG_BATCH_LAUNCHER(batch_launch_rad)
 #undef LG_BATCH_LAUNCHER
 // the scalar kernel, with its moist and 3rd-order variants chosen by tr and o3
 template <class R>

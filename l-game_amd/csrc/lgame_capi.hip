@@ -335,12 +335,10 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
       LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
-      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S);
+      // the mass fluxes and, from the same winds, the kinetic energy
+      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
       LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c);
-      if (!c.C.llinear) {
-        LG_LAUNCH(KI_KE, k_ke, nx, ny, nl, c, S);
-        LG_LAUNCH(KI_KE_GRAD, k_ke_grad, nx + 1, ny + 1, nl, c);
-      }
+      if (!c.C.llinear) LG_LAUNCH(KI_KE_GRAD, k_ke_grad, nx + 1, ny + 1, nl, c);
     }
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
@@ -1679,7 +1677,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
     if (!batch_launch_step<R>(ph, grd, blk, st, M, pf, n, a) && !batch_launch_diff<R>(ph, grd, blk, st, M, pf, n, a))
       (void)batch_launch_solve<R>(ph, grd, blk, st, M, pf, n, a);
   };
-  BatchArgs base{a_slot, b_slot, a_slot, 1, step == 0 ? 1 : 0, 0, 0};
+  BatchArgs base{a_slot, b_slot, a_slot, 1, step == 0 ? 1 : 0, 0, 0, C.llinear ? 0 : 1};
   launch(PH_TEMPERATURE, base);
   // update_sfc_turb_quantities and update_rad_fluxes with state_old (mo_manage_pchevi.F90:52-58)
   if (C.lsfc_turb) launch(PH_SFC_TURB, base);
@@ -1717,12 +1715,9 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
     }
     if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
       launch(PH_POT_VORT, a);
-      launch(PH_MASS_FLUX, a);
+      launch(PH_MASS_FLUX, a);  // with the kinetic energy
       launch(PH_VORT_FLUX, a);
-      if (!C.llinear) {
-        launch(PH_KE, a);
-        launch(PH_KE_GRAD, a);
-      }
+      if (!C.llinear) launch(PH_KE_GRAD, a);
     }
     if (rk == 1 && C.lmom_h)
       for (int ph : {PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM}) launch(ph, a);

@@ -397,10 +397,13 @@ __global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const Sta
 // u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1; the
 // threads of column 1 and row 1 store them, and the launch covers the nx by ny cells.
 template <class R>
-__device__ __forceinline__ void b_mass_flux(const CtxT<R> &c, const StateDev &s, const Bid bid) {
+__device__ __forceinline__ void b_mass_flux(const CtxT<R> &c, const StateDev &s, int with_ke, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   if (jk > nx || ji > ny || jl > nlev) return;
+  // the kinetic energy of the cell (inner_product, mo_inner_product.F90:16-50) from the same wind
+  // values, before any store
+  if (with_ke && jl <= nl) AT(c.w.v_squared, ji, jk, jl) = ke_point(c, s, ji, jk, jl);
   const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
   R fu = R(0.0), fv = R(0.0), fw = R(0.0);
   if (jl <= nl) {
@@ -424,8 +427,8 @@ __device__ __forceinline__ void b_mass_flux(const CtxT<R> &c, const StateDev &s,
   AT(c.w.w_ph, ji, jk, jl) = fw;
 }
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s) {
-  b_mass_flux(c, s, bid_here());
+__global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke) {
+  b_mass_flux(c, s, with_ke, bid_here());
 }
 
 // The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
@@ -492,16 +495,6 @@ __global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c) {
   b_vort_flux(c, bid_here());
 }
 
-template <class R>
-__device__ __forceinline__ void b_ke(const CtxT<R> &c, const StateDev &s, const Bid bid) {
-  KIDX;
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  AT(c.w.v_squared, ji, jk, jl) = ke_point(c, s, ji, jk, jl);
-}
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_ke(const CtxT<R> c, const StateDev s) {
-  b_ke(c, s, bid_here());
-}
 
 // grad_hor and grad_vert of the kinetic energy at the u, v and w points of one cell (ji <= ny,
 // jk <= nx): the values and vertical gradients that the three share are loaded or formed once and
@@ -698,9 +691,8 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   template __global__ void k_theta_edge<R>(const CtxT<R>, Fld<const double>, Fld<const double>, int);   \
   template __global__ void k_pgrad<R>(const CtxT<R>, const StateDev, int);                               \
   template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev);                                 \
-  template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev);                                \
+  template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev, int);                           \
   template __global__ void k_vort_flux<R>(const CtxT<R>);                                                \
-  template __global__ void k_ke<R>(const CtxT<R>, const StateDev);                                       \
   template __global__ void k_ke_grad<R>(const CtxT<R>);                                                  \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
   template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);
@@ -719,9 +711,8 @@ LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
                              a.nfield, bid))
 LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad(d.c, d.s[a.old_slot], a.lfirst, bid))
 LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
-LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], bid))
+LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], a.ke, bid))
 LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, bid))
-LG_BATCH_KERNEL(ke, BX * BY, PH_KE, b_ke(d.c, d.s[a.state_slot], bid))
 LG_BATCH_KERNEL(ke_grad, BX * BY, PH_KE_GRAD, b_ke_grad(d.c, bid))
 LG_BATCH_KERNEL(dissipation, BX * BY, PH_DISSIPATION, b_dissipation(d.c, d.s[a.old_slot], bid))
 LG_BATCH_KERNEL(vec_tend, BX * BY, PH_VEC_TEND, b_vec_tend(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
@@ -742,7 +733,6 @@ bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R
     LG_B(PH_POT_VORT, kb_pot_vort)
     LG_B(PH_MASS_FLUX, kb_mass_flux)
     LG_B(PH_VORT_FLUX, kb_vort_flux)
-    LG_B(PH_KE, kb_ke)
     LG_B(PH_KE_GRAD, kb_ke_grad)
     LG_B(PH_DISSIPATION, kb_dissipation)
     LG_B(PH_VEC_TEND, kb_vec_tend)
