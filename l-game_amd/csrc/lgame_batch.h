@@ -17,7 +17,7 @@ namespace lgame {
 
 enum Phase : int {
   PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
-  PH_MASS_FLUX, PH_VORT_FLUX, PH_KE_GRAD, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
+  PH_MASS_FLUX, PH_VORT_FLUX, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
   PH_DISSIPATION, PH_VEC_TEND, PH_DIFF_FLUX, PH_DIFF_DIV, PH_SCALAR, PH_SURFACE, PH_SOLVE_VER, PH_SOLVE_GEN, PH_RAD,
   PH_COUNT
 };
@@ -35,7 +35,6 @@ inline int phase_tile(int ph) {
     case PH_POT_VORT: return KI_POT_VORT;
     case PH_MASS_FLUX: return KI_MASS_FLUX;
     case PH_VORT_FLUX: return KI_VORT_FLUX;
-    case PH_KE_GRAD: return KI_KE_GRAD;
     case PH_DIFF_ZETA: return KI_DIFF_ZETA;
     case PH_DIFF_COEF: return KI_DIFF_COEF;
     case PH_DIFF_MOM: return KI_DIFF_MOM;
@@ -64,7 +63,7 @@ struct BatchArgs {
   int old_slot, new_slot;  // state_old and state_new
   int state_slot;          // the slot of the phase's state argument (state_old, or state_new at rk 2)
   int rk, lfirst, nfield, flags;
-  int ke;  // the mass-flux kernel also forms the kinetic energy
+  int ke;  // the mass-flux kernel also forms the kinetic energy, the vorticity-flux kernel its gradient
 };
 
 // Blocks of phase ph for one member as (gx, gy, gz), with `by` rows per block for the stencil
@@ -78,7 +77,7 @@ __host__ __device__ inline void phase_grid(int ph, const Geom &G, int by, int se
   switch (ph) {
     case PH_TEMPERATURE: case PH_THETA_FILL: case PH_THETA_DIV: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
     case PH_SFC_TURB: case PH_PBL_WIND: case PH_SURFACE: ez = 1; break;
-    case PH_THETA_GRAD: case PH_THETA_EDGE: case PH_KE_GRAD: case PH_VEC_TEND: ex = nx + 1; ey = ny + 1; break;
+    case PH_THETA_GRAD: case PH_THETA_EDGE: case PH_VEC_TEND: ex = nx + 1; ey = ny + 1; break;
     case PH_MASS_FLUX: ez = nlev; break;
     case PH_PGRAD: case PH_VORT_FLUX: case PH_DIFF_COEF: case PH_DIFF_MOM: case PH_DIFF_FLUX: case PH_DIFF_DIV: flat = true; break;
     case PH_POT_VORT: ez = nlev; flat = true; break;

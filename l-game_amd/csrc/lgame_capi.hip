@@ -337,8 +337,8 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
       LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
       // the mass fluxes and, from the same winds, the kinetic energy
       LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
-      LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c);
-      if (!c.C.llinear) LG_LAUNCH(KI_KE_GRAD, k_ke_grad, nx + 1, ny + 1, nl, c);
+      // the vorticity flux term and, in the same cells, the gradient of the kinetic energy
+      LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c, c.C.llinear ? 0 : 1);
     }
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
@@ -1716,8 +1716,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
     if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
       launch(PH_POT_VORT, a);
       launch(PH_MASS_FLUX, a);  // with the kinetic energy
-      launch(PH_VORT_FLUX, a);
-      if (!C.llinear) launch(PH_KE_GRAD, a);
+      launch(PH_VORT_FLUX, a);  // with the gradient of the kinetic energy
     }
     if (rk == 1 && C.lmom_h)
       for (int ph : {PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM}) launch(ph, a);

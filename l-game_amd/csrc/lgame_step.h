@@ -14,8 +14,7 @@ template <class R> __global__ void k_theta_edge(const CtxT<R> c, Fld<const doubl
 template <class R> __global__ void k_pgrad(const CtxT<R> c, const StateDev s, int lfirst);
 template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke);
-template <class R> __global__ void k_vort_flux(const CtxT<R> c);
-template <class R> __global__ void k_ke_grad(const CtxT<R> c);
+template <class R> __global__ void k_vort_flux(const CtxT<R> c, int with_ke_grad);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);

@@ -485,14 +485,16 @@ __device__ __forceinline__ void vort_flux_interior(const CtxT<R> &c, int ji, int
 // indices of column 1 and row 1), so the threads of column 1 and row 1 store them too and the
 // launch covers the nx by ny cells only.
 template <class R>
-__device__ __forceinline__ void b_vort_flux(const CtxT<R> &c, const Bid bid) {
+__device__ __forceinline__ void b_vort_flux(const CtxT<R> &c, int with_ke_grad, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
   if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
   vort_flux_interior(c, ji, jk, jl);
+  // grad_vert + grad_hor of the kinetic energy (mo_vector_tend_expl.F90:96-97), the same cell
+  if (with_ke_grad) ke_grad_interior(c, ji, jk, jl);
 }
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c) {
-  b_vort_flux(c, bid_here());
+__global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c, int with_ke_grad) {
+  b_vort_flux(c, with_ke_grad, bid_here());
 }
 
 
@@ -526,50 +528,41 @@ __device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int j
       vS1 = vgrad(iS, kS, jl + 1);
     }
   }
-  R gx = (kC - kW) / c.g.dx[iC];
-  R gy = (kS - kC) / c.g.dy[iC];
+  const R gx_cov = (kC - kW) / c.g.dx[iC];
+  const R gy_cov = (kS - kC) / c.g.dy[iC];
+  R gx = gx_cov, gy = gy_cov, rx = R(0.0), ry = R(0.0);
   if (terr) {
-    R r = R(0.0);
-    r = r + c.g.ipw[4][iW] * vW;
-    r = r + c.g.ipw[4][iC] * vC;
+    rx = rx + c.g.ipw[4][iW] * vW;
+    rx = rx + c.g.ipw[4][iC] * vC;
     if (below) {
-      r = r + c.g.ipw[5][iW] * vW1;
-      r = r + c.g.ipw[5][iC] * vC1;
+      rx = rx + c.g.ipw[5][iW] * vW1;
+      rx = rx + c.g.ipw[5][iC] * vC1;
     }
-    gx = gx - c.g.slope_x[iC] * (R(0.5) * r);
-    r = R(0.0);
-    r = r + c.g.ipw[4][iS] * vS;
-    r = r + c.g.ipw[4][iC] * vC;
+    gx = gx - c.g.slope_x[iC] * (R(0.5) * rx);
+    ry = ry + c.g.ipw[4][iS] * vS;
+    ry = ry + c.g.ipw[4][iC] * vC;
     if (below) {
-      r = r + c.g.ipw[5][iS] * vS1;
-      r = r + c.g.ipw[5][iC] * vC1;
+      ry = ry + c.g.ipw[5][iS] * vS1;
+      ry = ry + c.g.ipw[5][iC] * vC1;
     }
-    gy = gy - c.g.slope_y[iC] * (R(0.5) * r);
+    gy = gy - c.g.slope_y[iC] * (R(0.5) * ry);
   }
   c.w.vsg_x[iC] = gx;
   c.w.vsg_y[iC] = gy;
   if (jl >= 2) c.w.vsg_z[iC] = vC;
+  // The u points of column nx+1 and the v points of row ny+1: grad_hor_x/_y there take the
+  // covariant part and the remap of column 1 / row 1 (their neighbours wrap to the same cells)
+  // and their own slope.
+  if (jk == 1) {
+    const ix_t iX = LG_IX(G, ji, nx + 1, jl);
+    c.w.vsg_x[iX] = terr ? gx_cov - c.g.slope_x[iX] * (R(0.5) * rx) : gx_cov;
+  }
+  if (ji == 1) {
+    const ix_t iY = LG_IX(G, ny + 1, jk, jl);
+    c.w.vsg_y[iY] = terr ? gy_cov - c.g.slope_y[iY] * (R(0.5) * ry) : gy_cov;
+  }
 }
 
-// grad_vert + grad_hor of the kinetic energy (mo_vector_tend_expl.F90:96-97)
-template <class R>
-__device__ __forceinline__ void b_ke_grad(const CtxT<R> &c, const Bid bid) {
-  KIDX;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
-  if (ji <= ny && jk <= nx) {
-    ke_grad_interior(c, ji, jk, jl);
-    return;
-  }
-  // the edge points: u at jk = nx+1 (own terrain metrics) and v at ji = ny+1
-  const Fld<const R> k2 = c.w.v_squared;
-  if (ji <= ny) AT(c.w.vsg_x, ji, jk, jl) = grad_hor_x(c, k2, ji, jk, jl);
-  if (jk <= nx) AT(c.w.vsg_y, ji, jk, jl) = grad_hor_y(c, k2, ji, jk, jl);
-}
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_ke_grad(const CtxT<R> c) {
-  b_ke_grad(c, bid_here());
-}
 
 // simple_dissipation_rate, mo_momentum_diff_diss.F90:333-377, and the vertical diffusion of w
 // of mom_diff_v (:277-289,320-328).  mom_diff_tend_z at the two levels of the cell is formed here
@@ -692,8 +685,7 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   template __global__ void k_pgrad<R>(const CtxT<R>, const StateDev, int);                               \
   template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev);                                 \
   template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev, int);                           \
-  template __global__ void k_vort_flux<R>(const CtxT<R>);                                                \
-  template __global__ void k_ke_grad<R>(const CtxT<R>);                                                  \
+  template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
   template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);
 
@@ -712,8 +704,7 @@ LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
 LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad(d.c, d.s[a.old_slot], a.lfirst, bid))
 LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
 LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], a.ke, bid))
-LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, bid))
-LG_BATCH_KERNEL(ke_grad, BX * BY, PH_KE_GRAD, b_ke_grad(d.c, bid))
+LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, a.ke, bid))
 LG_BATCH_KERNEL(dissipation, BX * BY, PH_DISSIPATION, b_dissipation(d.c, d.s[a.old_slot], bid))
 LG_BATCH_KERNEL(vec_tend, BX * BY, PH_VEC_TEND, b_vec_tend(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
 
@@ -733,7 +724,6 @@ bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R
     LG_B(PH_POT_VORT, kb_pot_vort)
     LG_B(PH_MASS_FLUX, kb_mass_flux)
     LG_B(PH_VORT_FLUX, kb_vort_flux)
-    LG_B(PH_KE_GRAD, kb_ke_grad)
     LG_B(PH_DISSIPATION, kb_dissipation)
     LG_B(PH_VEC_TEND, kb_vec_tend)
     default: return false;
