@@ -308,34 +308,77 @@ __global__ void __launch_bounds__(256) k_transpose16(const uint16_t *src, long l
   }
 }
 
-// gather a batch of rows of the normalised data set
-__global__ void k_gather(const uint16_t *X, const int *rows, int B, int ld, uint16_t *out) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)B * ld) return;
-  const int b = (int)(idx / ld), i = (int)(idx % ld);
-  out[idx] = X[(long long)rows[b] * ld + i];
+// A batch of rows of the normalised data set gathered into X [B][ld] and, through 32x32 LDS tiles,
+// its transpose XT [ld][ldt] (B and ld multiples of 32)
+__global__ void __launch_bounds__(256) k_gather_t(const uint16_t *DX, const int *rows, int ld, uint16_t *X, uint16_t *XT,
+                                                  long long ldt) {
+  __shared__ uint16_t tile[32][34];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    const uint16_t v = DX[(long long)rows[r] * ld + c];
+    X[(long long)r * ld + c] = v;
+    tile[j][tx] = v;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) XT[(long long)(c0 + j) * ldt + r0 + tx] = tile[tx][j];
 }
 
-// Adam on the FP32 master weights of one parameter block (padded [rows][ld] layout; entries
-// outside the valid rows/columns stay zero), then the BF16 copies [rows][ld] and [ld][ldt]
-__global__ void k_adam(float *w, float *m, float *v, const float *grad, uint16_t *wbf, uint16_t *wbfT, long long ldt, long long n,
-                       int ld, int valid_cols, int valid_rows, float lr, float b1, float b2, float eps, float c1, float c2) {
+
+
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;
+};
+
+// Adam (Kingma and Ba) on element i of an FP32 master-weight block, with the bias corrections
+// c1 = 1 - b1^t and c2 = 1 - b2^t; returns the new weight
+__device__ __forceinline__ float adam_elem(float *w, float *m, float *v, long long i, float gi, const AdamArgs &a) {
+  const float mi = a.b1 * m[i] + (1.0f - a.b1) * gi;
+  const float vi = a.b2 * v[i] + (1.0f - a.b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
+  w[i] = wi;
+  return wi;
+}
+
+// The weight gradient summed over its split-K partials in split order (as k_sum_splits), then Adam
+// on the padded [rows][ld] weight block (entries outside the valid rows/columns stay zero) and its
+// BF16 copies [rows][ld] and [ld][ldt]: one pass instead of two.
+__global__ void k_adam_splits(const float *part, int splits, long long n, float *w, float *m, float *v, uint16_t *wbf,
+                              uint16_t *wbfT, long long ldt, int ld, int valid_cols, int valid_rows, const AdamArgs a) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int col = (int)(i % ld);
   const long long row = i / ld;
   if (col >= valid_cols || row >= valid_rows) return;
-  const float gi = grad[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  if (wbf) {
-    wbf[i] = f2bf(wi);
-    wbfT[(long long)col * ldt + row] = f2bf(wi);
+  float gi = 0.0f;
+  for (int z = 0; z < splits; ++z) gi += part[(long long)z * n + i];
+  const float wi = adam_elem(w, m, v, i, gi, a);
+  wbf[i] = f2bf(wi);
+  wbfT[(long long)col * ldt + row] = f2bf(wi);
+}
+
+// The bias gradient as the row sum of the transposed output gradient (k_rowsum, one block per
+// output, the same reduction tree), then Adam on the bias.
+__global__ void __launch_bounds__(256) k_rowsum_adam(const uint16_t *T, long long ld, int len, int valid, float *b, float *mb,
+                                                      float *vb, const AdamArgs a) {
+  __shared__ float sh[256];
+  const uint16_t *row = T + (long long)blockIdx.x * ld;
+  float s = 0.0f;
+  for (int i = 8 * threadIdx.x; i < len; i += 8 * 256) {
+    const uint4 q4 = *reinterpret_cast<const uint4 *>(row + i);
+    const uint32_t wq[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s += bf2f((uint16_t)(wq[q] & 0xffffu)) + bf2f((uint16_t)(wq[q] >> 16));
   }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && (int)blockIdx.x < valid) (void)adam_elem(b, mb, vb, blockIdx.x, sh[0], a);
 }
 
 // BF16 copies of a [rows][ld] FP32 weight block: as it is and transposed ([ld][rows])
@@ -585,7 +628,10 @@ void forward_infer(const lgame_mlp *m, const InferBufs *w, int B, hipStream_t st
 
 // backward pass from the output gradient (dYb [B][out_p] and dTa = its transpose) with the
 // activations of the last training forward pass; XT is the transposed input
-void backward(lgame_mlp *m, const uint16_t *XT, int B) {
+// With `adam` the weight and bias gradients are not stored: each layer's parameters take their
+// Adam update as soon as its gradients are formed and its old weights are no longer needed (after
+// the data gradient through them).
+void backward(lgame_mlp *m, const uint16_t *XT, int B, const AdamArgs *adam = nullptr) {
   const uint16_t *dcur = m->dYb, *dcurT = m->dTa;
   // split-K of the weight gradients (K = B): enough blocks to fill the chip, >= 256 k values each
   int splits = 1;
@@ -606,9 +652,22 @@ void backward(lgame_mlp *m, const uint16_t *XT, int B) {
     gw.ldc = Ly.in_p;
     gw.split_stride = (long long)Ly.out_p * Ly.in_p;
     gemm_nt<EPI_F32>(m->stream, gw, splits);
-    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(gw.split_stride)), dim3(256), 0, m->stream, m->gpart, splits, gw.split_stride, Ly.gw);
-    hipLaunchKernelGGL(k_rowsum, dim3(Ly.out_p), dim3(256), 0, m->stream, dcurT, (long long)B, B, Ly.gb);
-    if (l == 0) break;
+    const long long nw = gw.split_stride;
+    auto update_weights = [&]() {
+      hipLaunchKernelGGL(k_adam_splits, dim3(nblk(nw)), dim3(256), 0, m->stream, (const float *)m->gpart, splits, nw, Ly.w, Ly.mw,
+                         Ly.vw, Ly.wbf, Ly.wbfT, (long long)Ly.out_p, Ly.in_p, Ly.in, Ly.out, *adam);
+    };
+    if (adam) {
+      hipLaunchKernelGGL(k_rowsum_adam, dim3(Ly.out_p), dim3(256), 0, m->stream, dcurT, (long long)B, B, Ly.out, Ly.b, Ly.mb, Ly.vb,
+                         *adam);
+    } else {
+      hipLaunchKernelGGL(k_sum_splits, dim3(nblk(nw)), dim3(256), 0, m->stream, m->gpart, splits, nw, Ly.gw);
+      hipLaunchKernelGGL(k_rowsum, dim3(Ly.out_p), dim3(256), 0, m->stream, dcurT, (long long)B, B, Ly.gb);
+    }
+    if (l == 0) {
+      if (adam) update_weights();
+      break;
+    }
     // data gradient through the GELU of layer l-1: d_prev[b][i] = (sum_o d[b][o] W[o][i]) gelu'(Z)
     uint16_t *dnext = (dcur == m->dA) ? m->dB : m->dA;
     uint16_t *dnextT = (dcurT == m->dTa) ? m->dTb : m->dTa;
@@ -626,6 +685,7 @@ void backward(lgame_mlp *m, const uint16_t *XT, int B) {
     gd.ldz = Ly.in_p;
     gd.ldt = B;
     gemm_nt<EPI_DGELU>(m->stream, gd);
+    if (adam) update_weights();  // the data gradient has read this layer's old weights
     dcur = dnext;
     dcurT = dnextT;
   }
@@ -1051,24 +1111,15 @@ int lgame_mlp_train(lgame_mlp *m, int epochs, int batch, float lr, unsigned long
     const int cur = ep % 2;
     for (long long b = 0; b < nb; ++b) {
       const int *rows = drows[cur] + b * batch;
-      hipLaunchKernelGGL(k_gather, dim3(nblk((long long)batch * ip)), dim3(256), 0, m->stream, m->DX, rows, batch, ip, m->X);
-      hipLaunchKernelGGL(k_transpose16, dim3(ip / 32, batch / 32), dim3(256), 0, m->stream, m->X, (long long)ip, batch, ip, m->XT,
+      hipLaunchKernelGGL(k_gather_t, dim3(ip / 32, batch / 32), dim3(256), 0, m->stream, m->DX, rows, ip, m->X, m->XT,
                          (long long)batch);
       forward(m, m->X, batch, true);
       hipLaunchKernelGGL(k_mse_grad, dim3(nparts), dim3(256), 0, m->stream, m->Yf, m->DY, rows, batch, m->n_out, op, m->dYb, m->dTa,
                          m->partial);
       hipLaunchKernelGGL(k_loss_add, dim3(1), dim3(256), 0, m->stream, (const float *)m->partial, nparts, dloss + ep);
-      backward(m, m->XT, batch);
       ++m->adam_t;
-      const float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-      for (auto &Ly : m->L) {
-        const long long nw = (long long)Ly.in_p * Ly.out_p;
-        hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(256), 0, m->stream, Ly.w, Ly.mw, Ly.vw, (const float *)Ly.gw, Ly.wbf, Ly.wbfT,
-                           (long long)Ly.out_p, nw, Ly.in_p, Ly.in, Ly.out, lr, b1, b2, eps, c1, c2);
-        hipLaunchKernelGGL(k_adam, dim3(nblk(Ly.out_p)), dim3(256), 0, m->stream, Ly.b, Ly.mb, Ly.vb, (const float *)Ly.gb,
-                           (uint16_t *)nullptr, (uint16_t *)nullptr, 0LL, (long long)Ly.out_p, Ly.out_p, Ly.out, 1, lr, b1, b2, eps, c1,
-                           c2);
-      }
+      const AdamArgs adam{lr, b1, b2, eps, 1.0f - std::pow(b1, (float)m->adam_t), 1.0f - std::pow(b2, (float)m->adam_t)};
+      backward(m, m->XT, batch, &adam);  // with each layer's Adam update
     }
     if (ep + 1 < epochs) {  // the next epoch's rows, drawn while this epoch runs
       const int nxt = (ep + 1) % 2;
