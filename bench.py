@@ -20,7 +20,8 @@ Block shapes of the stencil kernels are chosen by lgame_autotune before the time
 (momentum, temperature and mass, Smagorinsky horizontal and TKE vertical coefficients); --pbl the
 PBL friction of the lowest-layer wind with the surface turbulence quantities; --physics all of
 the reference's default namelist (diffusion, PBL, 5-layer prognostic soil, surface sensible heat
-flux).
+flux).  The default moist512 run also times both forms with that default physics
+(`reference_physics` in the JSON line; --no-physics-extra skips it).
 
 Two forms of the step are timed: the plain FP64 form with the gray two-stream radiation solver
 and the mixed form (FP32 stencils and explicit tendencies, FP64 column solves, density updates
@@ -530,6 +531,8 @@ def main():
                     help="which forms of the step to time (the headline value is the mixed form)")
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-physics-extra", action="store_true",
+                    help="moist512: skip timing the same workload with the reference's default physics")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (for HBM counter scaling)")
     ap.add_argument("--train-states", type=int, default=24)
@@ -692,6 +695,25 @@ def main():
     }
     if "fp64" in res and "mixed" in res:
         out["speedup_mixed_vs_fp64"] = res["fp64"][0] / res["mixed"][0]
+    if args.workload == "moist512" and not (args.physics or args.diffusion or args.pbl or args.no_physics_extra):
+        # the same workload with the reference's default namelist (diffusion, PBL, 5-layer soil,
+        # surface sensible heat flux, lsfc_phase_trans), timed the same way beside the headline
+        pcfg = dataclasses.replace(cfg, lprog_soil_temp=True, lsfc_sensible_heat_flux=True, lsfc_phase_trans=True,
+                                   nsoillays=5, lpbl=True).set_diffusion(True)
+        log(f"setting up {describe(pcfg)}")
+        pgrid, pst = make_case(pcfg, cloud_frac=CLOUD_FRAC)
+        cloud_patches(pcfg, pst)
+        psurface = radiation_surface(pcfg, pgrid, pst)
+        phys = {"workload": f"{args.workload}: {describe(pcfg)}"}
+        for form in forms:
+            el, bad_p, _, rad_p = time_form(pcfg, pgrid, pst, psurface, form, mlp, args.steps, args.warmup)
+            phys[form] = {"value": cells * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "nonfinite_after_run": bad_p,
+                          "radiation_audit_rmse_k_per_day": rad_p.get("audit_rmse_k_per_day")}
+            log(f"{form} with the reference's default physics: {1e3 * el / args.steps:.2f} ms/step")
+        if "fp64" in phys and "mixed" in phys:
+            phys["speedup_mixed_vs_fp64"] = phys["fp64"]["ms_per_step"] / phys["mixed"]["ms_per_step"]
+        out["reference_physics"] = phys
+        del pgrid, pst
     if not args.no_cpu:
         nt = host_threads()
         log(f"CPU baseline: oracle on {describe(sample_cfg)} for {args.cpu_steps} steps, {nt} instances on {nt} threads")
