@@ -334,7 +334,10 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
-      LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
+      if (nx % 2 == 0)
+        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort2, nx / 2, ny, nlev, c, S);
+      else
+        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
       // the mass fluxes and, from the same winds, the kinetic energy
       LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
       // the vorticity flux term and, in the same cells, the gradient of the kinetic energy

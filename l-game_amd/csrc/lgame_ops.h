@@ -22,6 +22,112 @@ namespace lgame {
 
 __device__ __forceinline__ int wrapm(int i, int n) { return i == 1 ? n : i - 1; }
 
+// ---- two x-adjacent points per thread -------------------------------------------------------
+// A stencil can be evaluated at the columns k and k+1 of one row at once (k odd, k+1 <= nx).  Its
+// values are then pairs: the loads of the own columns are single 2-element vector loads (aligned,
+// since k-1 is even and rows, levels and fields start at multiples of 16 elements), and the west
+// neighbours of the pair are the columns (west of k, k), whose second element the compiler takes
+// from the own pair.  Every operation on a pair is the same operation on each element, so each
+// element is bitwise the value of the single-point evaluation.  The helpers below take the x index
+// as a template parameter X: int (one point), XP (the columns k, k+1) or XG (one column per
+// element); VT<R, X> is the matching value type.
+template <class R>
+struct P2 {
+  R a, b;
+  P2() = default;
+  __device__ __forceinline__ P2(R x) : a(x), b(x) {}
+  __device__ __forceinline__ P2(R x, R y) : a(x), b(y) {}
+  friend __device__ __forceinline__ P2 operator+(P2 x, P2 y) { return P2(x.a + y.a, x.b + y.b); }
+  friend __device__ __forceinline__ P2 operator-(P2 x, P2 y) { return P2(x.a - y.a, x.b - y.b); }
+  friend __device__ __forceinline__ P2 operator*(P2 x, P2 y) { return P2(x.a * y.a, x.b * y.b); }
+  friend __device__ __forceinline__ P2 operator/(P2 x, P2 y) { return P2(x.a / y.a, x.b / y.b); }
+  friend __device__ __forceinline__ P2 operator-(P2 x) { return P2(-x.a, -x.b); }
+};
+struct XP {
+  int k;  // the columns k and k+1
+};
+struct XG {
+  int k0, k1;  // the column of each element
+};
+struct LP {
+  int a, b;  // the level of each element
+};
+template <class R, class X>
+struct ValOf {
+  using type = P2<R>;
+};
+template <class R>
+struct ValOf<R, int> {
+  using type = R;
+};
+template <class R, class X>
+using VT = typename ValOf<R, X>::type;
+
+// west neighbours: k+1 >= 2 never wraps
+__device__ __forceinline__ XG wrapm(XP x, int n) { return XG{wrapm(x.k, n), x.k}; }
+__device__ __forceinline__ XG wrapm(XG x, int n) { return XG{wrapm(x.k0, n), wrapm(x.k1, n)}; }
+// column nx+1 read as column 1: k+1 <= nx
+__device__ __forceinline__ int edge1(int k, int n) { return k == n + 1 ? 1 : k; }
+__device__ __forceinline__ XP edge1(XP x, int) { return x; }
+__device__ __forceinline__ XG edge1(XG x, int n) { return XG{edge1(x.k0, n), edge1(x.k1, n)}; }
+
+// loads of a field at one point, at a pair of columns, or at one column per element
+template <class T>
+__device__ __forceinline__ typename std::remove_cv<T>::type ldv(Fld<T> p, const Geom &G, int ji, int k, int jl) {
+  return p[LG_IX(G, ji, k, jl)];
+}
+template <class T>
+__device__ __forceinline__ P2<typename std::remove_cv<T>::type> ldv(Fld<T> p, const Geom &G, int ji, XP x, int jl) {
+  using E = typename std::remove_cv<T>::type;
+  using E2 = E __attribute__((ext_vector_type(2)));
+  const E2 v = *reinterpret_cast<const E2 *>(&p[LG_IX(G, ji, x.k, jl)]);
+  return P2<E>(v.x, v.y);
+}
+template <class T>
+__device__ __forceinline__ P2<typename std::remove_cv<T>::type> ldv(Fld<T> p, const Geom &G, int ji, XG x, int jl) {
+  using E = typename std::remove_cv<T>::type;
+  return P2<E>(p[LG_IX(G, ji, x.k0, jl)], p[LG_IX(G, ji, x.k1, jl)]);
+}
+// a load at one level per element
+template <class T>
+__device__ __forceinline__ typename std::remove_cv<T>::type ldv_l(Fld<T> p, const Geom &G, int ji, int k, int jl) {
+  return p[LG_IX(G, ji, k, jl)];
+}
+template <class T>
+__device__ __forceinline__ P2<typename std::remove_cv<T>::type> ldv_l(Fld<T> p, const Geom &G, int ji, XP x, LP l) {
+  using E = typename std::remove_cv<T>::type;
+  return P2<E>(p[LG_IX(G, ji, x.k, l.a)], p[LG_IX(G, ji, x.k + 1, l.b)]);
+}
+template <class T>
+__device__ __forceinline__ P2<typename std::remove_cv<T>::type> ldv_l(Fld<T> p, const Geom &G, int ji, XG x, LP l) {
+  using E = typename std::remove_cv<T>::type;
+  return P2<E>(p[LG_IX(G, ji, x.k0, l.a)], p[LG_IX(G, ji, x.k1, l.b)]);
+}
+template <class R, class E>
+__device__ __forceinline__ R cvt(E x) {
+  return static_cast<R>(x);
+}
+template <class R, class E>
+__device__ __forceinline__ P2<R> cvt(P2<E> x) {
+  return P2<R>(static_cast<R>(x.a), static_cast<R>(x.b));
+}
+// a store at one point or at a pair of columns
+template <class T>
+__device__ __forceinline__ void stv(Fld<T> p, const Geom &G, int ji, int k, int jl, T v) {
+  p[LG_IX(G, ji, k, jl)] = v;
+}
+template <class T>
+__device__ __forceinline__ void stv(Fld<T> p, const Geom &G, int ji, XP x, int jl, P2<T> v) {
+  using E2 = T __attribute__((ext_vector_type(2)));
+  E2 w;
+  w.x = v.a;
+  w.y = v.b;
+  *reinterpret_cast<E2 *>(&p[LG_IX(G, ji, x.k, jl)]) = w;
+}
+// generic forms of AT and ATR for any x index type
+#define LDG(p, ji, jk, jl) ldv(p, c.G, ji, jk, jl)
+#define LDGR(p, ji, jk, jl) cvt<R>(ldv(p, c.G, ji, jk, jl))
+
 // ---- vertical fields seen through remap_ver2hor -----------------------------------------
 // grad_vert of a scalar field (mo_gradient_operators.F90:65-83); the reference leaves levels 1
 // and n_levels of its result untouched and they hold zero whenever the remap reads them.
@@ -36,34 +142,39 @@ struct VGrad {
 template <class R, class P>
 struct VField {
   P w;
-  __device__ R operator()(const CtxT<R> &c, int ji, int jk, int jl) const { return ATR(w, ji, jk, jl); }
+  template <class X>
+  __device__ VT<R, X> operator()(const CtxT<R> &c, int ji, X jk, int jl) const {
+    return LDGR(w, ji, jk, jl);
+  }
 };
 
+// the two columns remap_ver2hor_x averages over: (nx, 1) at columns 1 and nx+1, else (jk-1, jk)
+__device__ __forceinline__ int remap_ka(int jk, int nx) { return (jk == 1 || jk == nx + 1) ? nx : jk - 1; }
+__device__ __forceinline__ int remap_kb(int jk, int nx) { return (jk == 1 || jk == nx + 1) ? 1 : jk; }
+__device__ __forceinline__ XG remap_ka(XP x, int nx) { return XG{remap_ka(x.k, nx), x.k}; }  // k+1 in 2..nx
+__device__ __forceinline__ XP remap_kb(XP x, int) { return x; }
+__device__ __forceinline__ XG remap_ka(XG x, int nx) { return XG{remap_ka(x.k0, nx), remap_ka(x.k1, nx)}; }
+__device__ __forceinline__ XG remap_kb(XG x, int nx) { return XG{remap_kb(x.k0, nx), remap_kb(x.k1, nx)}; }
+
 // remap_ver2hor_x, mo_averaging.F90:56-95 (periodic)
-template <class R, class VZ>
-__device__ R remap_x(const CtxT<R> &c, const VZ &vz, int ji, int jk, int jl) {
+template <class R, class VZ, class X>
+__device__ VT<R, X> remap_x(const CtxT<R> &c, const VZ &vz, int ji, X jk, int jl) {
   const int nx = c.G.nx;
-  int ka, kb;
-  if (jk == 1 || jk == nx + 1) {
-    ka = nx;
-    kb = 1;
-  } else {
-    ka = jk - 1;
-    kb = jk;
-  }
-  R r = R(0.0);
-  r = r + AT(c.g.ipw[4], ji, ka, jl) * vz(c, ji, ka, jl);
-  r = r + AT(c.g.ipw[4], ji, kb, jl) * vz(c, ji, kb, jl);
+  const auto ka = remap_ka(jk, nx);
+  const auto kb = remap_kb(jk, nx);
+  VT<R, X> r = R(0.0);
+  r = r + LDG(c.g.ipw[4], ji, ka, jl) * vz(c, ji, ka, jl);
+  r = r + LDG(c.g.ipw[4], ji, kb, jl) * vz(c, ji, kb, jl);
   if (jl < c.G.nl || c.C.lfreeslip) {
-    r = r + AT(c.g.ipw[5], ji, ka, jl) * vz(c, ji, ka, jl + 1);
-    r = r + AT(c.g.ipw[5], ji, kb, jl) * vz(c, ji, kb, jl + 1);
+    r = r + LDG(c.g.ipw[5], ji, ka, jl) * vz(c, ji, ka, jl + 1);
+    r = r + LDG(c.g.ipw[5], ji, kb, jl) * vz(c, ji, kb, jl + 1);
   }
   return R(0.5) * r;
 }
 
 // remap_ver2hor_y, mo_averaging.F90:97-136 (periodic)
-template <class R, class VZ>
-__device__ R remap_y(const CtxT<R> &c, const VZ &vz, int ji, int jk, int jl) {
+template <class R, class VZ, class X>
+__device__ VT<R, X> remap_y(const CtxT<R> &c, const VZ &vz, int ji, X jk, int jl) {
   const int ny = c.G.ny;
   int ia, ib;
   if (ji == 1 || ji == ny + 1) {
@@ -73,12 +184,12 @@ __device__ R remap_y(const CtxT<R> &c, const VZ &vz, int ji, int jk, int jl) {
     ia = ji - 1;
     ib = ji;
   }
-  R r = R(0.0);
-  r = r + AT(c.g.ipw[4], ia, jk, jl) * vz(c, ia, jk, jl);
-  r = r + AT(c.g.ipw[4], ib, jk, jl) * vz(c, ib, jk, jl);
+  VT<R, X> r = R(0.0);
+  r = r + LDG(c.g.ipw[4], ia, jk, jl) * vz(c, ia, jk, jl);
+  r = r + LDG(c.g.ipw[4], ib, jk, jl) * vz(c, ib, jk, jl);
   if (jl < c.G.nl || c.C.lfreeslip) {
-    r = r + AT(c.g.ipw[5], ia, jk, jl) * vz(c, ia, jk, jl + 1);
-    r = r + AT(c.g.ipw[5], ib, jk, jl) * vz(c, ib, jk, jl + 1);
+    r = r + LDG(c.g.ipw[5], ia, jk, jl) * vz(c, ia, jk, jl + 1);
+    r = r + LDG(c.g.ipw[5], ib, jk, jl) * vz(c, ib, jk, jl + 1);
   }
   return R(0.5) * r;
 }
@@ -151,16 +262,16 @@ __device__ R div_h(const CtxT<R> &c, P vx, P vy, int ji, int jk, int jl) {
 }
 
 // horizontal_covariant_x/_y, mo_averaging.F90:138-176 (vertical component = wind_w)
-template <class R>
-__device__ __forceinline__ R hcov_x(const CtxT<R> &c, Fld<const double> u, Fld<const double> w, int ji, int jk, int jl) {
-  R r = ATR(u, ji, jk, jl);
-  if (jl > c.G.nflat) r = r + AT(c.g.slope_x, ji, jk, jl) * remap_x(c, VField<R, Fld<const double>>{w}, ji, jk, jl);
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> hcov_x(const CtxT<R> &c, Fld<const double> u, Fld<const double> w, int ji, X jk, int jl) {
+  VT<R, X> r = LDGR(u, ji, jk, jl);
+  if (jl > c.G.nflat) r = r + LDG(c.g.slope_x, ji, jk, jl) * remap_x(c, VField<R, Fld<const double>>{w}, ji, jk, jl);
   return r;
 }
-template <class R>
-__device__ __forceinline__ R hcov_y(const CtxT<R> &c, Fld<const double> v, Fld<const double> w, int ji, int jk, int jl) {
-  R r = ATR(v, ji, jk, jl);
-  if (jl > c.G.nflat) r = r + AT(c.g.slope_y, ji, jk, jl) * remap_y(c, VField<R, Fld<const double>>{w}, ji, jk, jl);
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> hcov_y(const CtxT<R> &c, Fld<const double> v, Fld<const double> w, int ji, X jk, int jl) {
+  VT<R, X> r = LDGR(v, ji, jk, jl);
+  if (jl > c.G.nflat) r = r + LDG(c.g.slope_y, ji, jk, jl) * remap_y(c, VField<R, Fld<const double>>{w}, ji, jk, jl);
   return r;
 }
 
@@ -296,42 +407,60 @@ __device__ __forceinline__ void tile_zfast(int b, int gx, int gy, int gz, int &b
 // rel_vort_z_local, mo_vorticities.F90:423-538, at any dual point (ji <= ny+1, jk <= nx+1).  The
 // dual points of row ny+1 and column nx+1 use their own metrics (rel_vort, :177-184, evaluates
 // them; calc_pot_vort then overwrites its eta_z there with row 1 and column 1, :391-417).
+// rel_vort_z_local's level for the vertical gradient of one term: below (jl-1) when the dual
+// point lies above the term's point or at the lowest layer, above (jl+1) at the top layer and
+// otherwise
 template <class R>
-__device__ R rel_vort_z(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+__device__ __forceinline__ int rv_level(R delta_z, int jl, int nl) {
+  int sh = 1;
+  if (delta_z > R(0.0) || jl == nl) sh = -1;
+  if (jl == 1) sh = 1;
+  return jl + sh;
+}
+template <class R>
+__device__ __forceinline__ LP rv_level(P2<R> delta_z, int jl, int nl) {
+  return LP{rv_level(delta_z.a, jl, nl), rv_level(delta_z.b, jl, nl)};
+}
+// one term of rel_vort_z_local above the flat levels: sgn * l_rescale * len * (wind + delta_z * vg)
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> rv_term(const CtxT<R> &c, VT<R, X> zd, R sgn, Fld<const R> zf, Fld<const double> wf,
+                                             Fld<const R> len, int jj, X kk, int jl) {
+  using V = VT<R, X>;
+  const V zp = LDG(zf, jj, kk, jl);
+  const V l_rescale = (R(R_E) + zd) / (R(R_E) + zp);
+  const V delta_z = zd - zp;
+  const auto lv = rv_level(delta_z, jl, c.G.nl);
+  const V vg = (LDGR(wf, jj, kk, jl) - cvt<R>(ldv_l(wf, c.G, jj, kk, lv))) / (zp - ldv_l(zf, c.G, jj, kk, lv));
+  return sgn * l_rescale * LDG(len, jj, kk, jl) * (LDGR(wf, jj, kk, jl) + delta_z * vg);
+}
+template <class R, class X>
+__device__ VT<R, X> rel_vort_z(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  using V = VT<R, X>;
   const int ny = c.G.ny, nx = c.G.nx;
-  const int j_i[4] = {ji, wrapm(ji, ny), ji, ji == ny + 1 ? 1 : ji};
-  const int j_k[4] = {jk == nx + 1 ? 1 : jk, jk, wrapm(jk, nx), jk};
-  const R sgn[4] = {R(1.0), -R(1.0), -R(1.0), R(1.0)};
-  R r = R(0.0);
+  // the four faces of the dual cell: (row, column, sign), v at faces 0 and 2, u at 1 and 3
+  const int i0 = ji, i1 = wrapm(ji, ny), i2 = ji, i3 = ji == ny + 1 ? 1 : ji;
+  const auto k0 = edge1(jk, nx);
+  const X k1 = jk, k3 = jk;
+  const auto k2 = wrapm(jk, nx);
+  V r = R(0.0);
   if (jl <= c.G.nflat) {
-    r = r + sgn[0] * AT(c.g.dy, j_i[0], j_k[0], jl) * ATR(s.wind_v, j_i[0], j_k[0], jl) +
-        sgn[1] * AT(c.g.dx, j_i[1], j_k[1], jl) * ATR(s.wind_u, j_i[1], j_k[1], jl) +
-        sgn[2] * AT(c.g.dy, j_i[2], j_k[2], jl) * ATR(s.wind_v, j_i[2], j_k[2], jl) +
-        sgn[3] * AT(c.g.dx, j_i[3], j_k[3], jl) * ATR(s.wind_u, j_i[3], j_k[3], jl);
+    r = r + R(1.0) * LDG(c.g.dy, i0, k0, jl) * LDGR(s.wind_v, i0, k0, jl) +
+        -R(1.0) * LDG(c.g.dx, i1, k1, jl) * LDGR(s.wind_u, i1, k1, jl) +
+        -R(1.0) * LDG(c.g.dy, i2, k2, jl) * LDGR(s.wind_v, i2, k2, jl) +
+        R(1.0) * LDG(c.g.dx, i3, k3, jl) * LDGR(s.wind_u, i3, k3, jl);
     return r;
   }
-  const R zd = AT(c.g.z_area_dual_z, ji, jk, jl);
-#pragma unroll
-  for (int jm = 0; jm < 4; ++jm) {
-    const bool isv = (jm == 0 || jm == 2);
-    const Fld<const R> zf = isv ? c.g.z_v : c.g.z_u;
-    const Fld<const double> wf = isv ? Fld<const double>(s.wind_v) : Fld<const double>(s.wind_u);
-    const Fld<const R> len = isv ? c.g.dy : c.g.dx;
-    const R zp = AT(zf, j_i[jm], j_k[jm], jl);
-    const R l_rescale = (R(R_E) + zd) / (R(R_E) + zp);
-    const R delta_z = zd - zp;
-    int sh = 1;
-    if (delta_z > R(0.0) || jl == c.G.nl) sh = -1;
-    if (jl == 1) sh = 1;
-    const R vg = (ATR(wf, j_i[jm], j_k[jm], jl) - ATR(wf, j_i[jm], j_k[jm], jl + sh)) / (zp - AT(zf, j_i[jm], j_k[jm], jl + sh));
-    r = r + sgn[jm] * l_rescale * AT(len, j_i[jm], j_k[jm], jl) * (ATR(wf, j_i[jm], j_k[jm], jl) + delta_z * vg);
-  }
+  const V zd = LDG(c.g.z_area_dual_z, ji, jk, jl);
+  r = r + rv_term<R>(c, zd, R(1.0), c.g.z_v, s.wind_v, c.g.dy, i0, k0, jl);
+  r = r + rv_term<R>(c, zd, -R(1.0), c.g.z_u, s.wind_u, c.g.dx, i1, k1, jl);
+  r = r + rv_term<R>(c, zd, -R(1.0), c.g.z_v, s.wind_v, c.g.dy, i2, k2, jl);
+  r = r + rv_term<R>(c, zd, R(1.0), c.g.z_u, s.wind_u, c.g.dx, i3, k3, jl);
   return r;
 }
 
-template <class R>
-__device__ __forceinline__ R rhog(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  return ATR(s.rho[c.G.ncond], ji, jk, jl);
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> rhog(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  return LDGR(s.rho[c.G.ncond], ji, jk, jl);
 }
 
 // ---- vertical_contravariant_corr of flux fields at the faces of one cell -----------------------

@@ -13,6 +13,7 @@ template <class R> __global__ void k_theta_div(const CtxT<R> c, int nfield);
 template <class R> __global__ void k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield);
 template <class R> __global__ void k_pgrad(const CtxT<R> c, const StateDev s, int lfirst);
 template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s);  // nx even: two cells per thread
 template <class R> __global__ void k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke);
 template <class R> __global__ void k_vort_flux(const CtxT<R> c, int with_ke_grad);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);

@@ -283,114 +283,152 @@ __global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateD
 // ---------------------------------------------------------------------------------------------
 
 // circulation of zeta_x around its dual face, mo_vorticities.F90:33-93 (jl in 2..n_levels)
-template <class R>
-__device__ R zeta_x_circ(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+template <class R, class X>
+__device__ VT<R, X> zeta_x_circ(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
   const int ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const int jm = wrapm(ji, ny);
   if (jl <= nl) {
-    return AT(c.g.dz, jm, jk, jl) * ATR(s.wind_w, jm, jk, jl) - AT(c.g.dy, ji, jk, jl - 1) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, jl - 1) -
-           AT(c.g.dz, ji, jk, jl) * ATR(s.wind_w, ji, jk, jl) + AT(c.g.dy, ji, jk, jl) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, jl);
+    return LDG(c.g.dz, jm, jk, jl) * LDGR(s.wind_w, jm, jk, jl) - LDG(c.g.dy, ji, jk, jl - 1) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, jl - 1) -
+           LDG(c.g.dz, ji, jk, jl) * LDGR(s.wind_w, ji, jk, jl) + LDG(c.g.dy, ji, jk, jl) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, jl);
   }
-  R r = -AT(c.g.dy, ji, jk, nl) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, nl);
+  VT<R, X> r = -LDG(c.g.dy, ji, jk, nl) * hcov_y(c, s.wind_v, s.wind_w, ji, jk, nl);
   if (c.C.lfreeslip) {
-    r = r + R(0.5) * AT(c.g.dz, jm, jk, nlev) * ATR(s.wind_w, jm, jk, nlev) - R(0.5) * AT(c.g.dz, ji, jk, nlev) * ATR(s.wind_w, ji, jk, nlev) +
-        AT(c.g.dy, ji, jk, nl) *
-            (ATR(s.wind_v, ji, jk, nl) +
-             (ATR(s.wind_v, ji, jk, nl - 1) - ATR(s.wind_v, ji, jk, nl)) / (AT(c.g.z_v, ji, jk, nl - 1) - AT(c.g.z_v, ji, jk, nl)) *
-                 (R(0.5) * (AT(c.g.z_w, jm, jk, nlev) + AT(c.g.z_w, ji, jk, nlev)) - AT(c.g.z_v, ji, jk, nl)) +
-             R(0.5) * (ATR(s.wind_w, jm, jk, nlev) + ATR(s.wind_w, ji, jk, nlev)) * (AT(c.g.z_w, jm, jk, nlev) - AT(c.g.z_w, ji, jk, nlev)) /
-                 AT(c.g.dy, ji, jk, nl));
+    r = r + R(0.5) * LDG(c.g.dz, jm, jk, nlev) * LDGR(s.wind_w, jm, jk, nlev) - R(0.5) * LDG(c.g.dz, ji, jk, nlev) * LDGR(s.wind_w, ji, jk, nlev) +
+        LDG(c.g.dy, ji, jk, nl) *
+            (LDGR(s.wind_v, ji, jk, nl) +
+             (LDGR(s.wind_v, ji, jk, nl - 1) - LDGR(s.wind_v, ji, jk, nl)) / (LDG(c.g.z_v, ji, jk, nl - 1) - LDG(c.g.z_v, ji, jk, nl)) *
+                 (R(0.5) * (LDG(c.g.z_w, jm, jk, nlev) + LDG(c.g.z_w, ji, jk, nlev)) - LDG(c.g.z_v, ji, jk, nl)) +
+             R(0.5) * (LDGR(s.wind_w, jm, jk, nlev) + LDGR(s.wind_w, ji, jk, nlev)) * (LDG(c.g.z_w, jm, jk, nlev) - LDG(c.g.z_w, ji, jk, nlev)) /
+                 LDG(c.g.dy, ji, jk, nl));
   }
   return r;
 }
 
 // mo_vorticities.F90:105-165 (jl in 2..n_levels)
-template <class R>
-__device__ R zeta_y_circ(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
+template <class R, class X>
+__device__ VT<R, X> zeta_y_circ(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
   const int nx = c.G.nx, nl = c.G.nl, nlev = c.G.nlev;
-  const int km = wrapm(jk, nx);
+  const auto km = wrapm(jk, nx);
   if (jl <= nl) {
-    return -AT(c.g.dz, ji, jk, jl) * ATR(s.wind_w, ji, jk, jl) - AT(c.g.dx, ji, jk, jl) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, jl) +
-           AT(c.g.dz, ji, km, jl) * ATR(s.wind_w, ji, km, jl) + AT(c.g.dx, ji, jk, jl - 1) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, jl - 1);
+    return -LDG(c.g.dz, ji, jk, jl) * LDGR(s.wind_w, ji, jk, jl) - LDG(c.g.dx, ji, jk, jl) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, jl) +
+           LDG(c.g.dz, ji, km, jl) * LDGR(s.wind_w, ji, km, jl) + LDG(c.g.dx, ji, jk, jl - 1) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, jl - 1);
   }
-  R r = AT(c.g.dx, ji, jk, nl) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, nl);
+  VT<R, X> r = LDG(c.g.dx, ji, jk, nl) * hcov_x(c, s.wind_u, s.wind_w, ji, jk, nl);
   if (c.C.lfreeslip) {
-    r = r + R(0.5) * AT(c.g.dz, ji, km, nlev) * ATR(s.wind_w, ji, km, nlev) - R(0.5) * AT(c.g.dz, ji, jk, nlev) * ATR(s.wind_w, ji, jk, nlev) -
-        AT(c.g.dx, ji, jk, nl) *
-            (ATR(s.wind_u, ji, jk, nl) +
-             (ATR(s.wind_u, ji, jk, nl - 1) - ATR(s.wind_u, ji, jk, nl)) / (AT(c.g.z_u, ji, jk, nl - 1) - AT(c.g.z_u, ji, jk, nl)) *
-                 (R(0.5) * (AT(c.g.z_w, ji, km, nlev) + AT(c.g.z_w, ji, jk, nlev)) - AT(c.g.z_u, ji, jk, nl)) +
-             R(0.5) * (ATR(s.wind_w, ji, km, nlev) + ATR(s.wind_w, ji, jk, nlev)) * (AT(c.g.z_w, ji, jk, nlev) - AT(c.g.z_w, ji, km, nlev)) /
-                 AT(c.g.dx, ji, jk, nl));
+    r = r + R(0.5) * LDG(c.g.dz, ji, km, nlev) * LDGR(s.wind_w, ji, km, nlev) - R(0.5) * LDG(c.g.dz, ji, jk, nlev) * LDGR(s.wind_w, ji, jk, nlev) -
+        LDG(c.g.dx, ji, jk, nl) *
+            (LDGR(s.wind_u, ji, jk, nl) +
+             (LDGR(s.wind_u, ji, jk, nl - 1) - LDGR(s.wind_u, ji, jk, nl)) / (LDG(c.g.z_u, ji, jk, nl - 1) - LDG(c.g.z_u, ji, jk, nl)) *
+                 (R(0.5) * (LDG(c.g.z_w, ji, km, nlev) + LDG(c.g.z_w, ji, jk, nlev)) - LDG(c.g.z_u, ji, jk, nl)) +
+             R(0.5) * (LDGR(s.wind_w, ji, km, nlev) + LDGR(s.wind_w, ji, jk, nlev)) * (LDG(c.g.z_w, ji, jk, nlev) - LDG(c.g.z_w, ji, km, nlev)) /
+                 LDG(c.g.dx, ji, jk, nl));
   }
   return r;
 }
 
 // density average of calc_pot_vort for horizontal vorticities, mo_vorticities.F90:240-266
-template <class R>
-__device__ R pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, int ka, int ib, int kb, int jl) {
+template <class R, class XA, class XB>
+__device__ VT<R, XA> pv_den_h(const CtxT<R> &c, const StateDev &s, int ia, XA ka, int ib, XB kb, int jl) {
   if (jl == 1) {
     return R(0.5) * (rhog(c, s, ia, ka, jl) +
-                  (R(c.C.toa) - AT(c.g.z_scalar, ia, ka, jl)) * (rhog(c, s, ia, ka, jl) - rhog(c, s, ia, ka, jl + 1)) /
-                      (AT(c.g.z_scalar, ia, ka, jl) - AT(c.g.z_scalar, ia, ka, jl + 1)) +
+                  (R(c.C.toa) - LDG(c.g.z_scalar, ia, ka, jl)) * (rhog(c, s, ia, ka, jl) - rhog(c, s, ia, ka, jl + 1)) /
+                      (LDG(c.g.z_scalar, ia, ka, jl) - LDG(c.g.z_scalar, ia, ka, jl + 1)) +
                   rhog(c, s, ib, kb, jl) +
-                  (R(c.C.toa) - AT(c.g.z_scalar, ib, kb, jl)) * (rhog(c, s, ib, kb, jl) - rhog(c, s, ib, kb, jl + 1)) /
-                      (AT(c.g.z_scalar, ib, kb, jl) - AT(c.g.z_scalar, ib, kb, jl + 1)));
+                  (R(c.C.toa) - LDG(c.g.z_scalar, ib, kb, jl)) * (rhog(c, s, ib, kb, jl) - rhog(c, s, ib, kb, jl + 1)) /
+                      (LDG(c.g.z_scalar, ib, kb, jl) - LDG(c.g.z_scalar, ib, kb, jl + 1)));
   } else if (jl == c.G.nlev) {
     return R(0.5) * (rhog(c, s, ia, ka, jl - 1) +
-                  (AT(c.g.z_w, ia, ka, jl) - AT(c.g.z_scalar, ia, ka, jl - 1)) * (rhog(c, s, ia, ka, jl - 2) - rhog(c, s, ia, ka, jl - 1)) /
-                      (AT(c.g.z_scalar, ia, ka, jl - 2) - AT(c.g.z_scalar, ia, ka, jl - 1)) +
+                  (LDG(c.g.z_w, ia, ka, jl) - LDG(c.g.z_scalar, ia, ka, jl - 1)) * (rhog(c, s, ia, ka, jl - 2) - rhog(c, s, ia, ka, jl - 1)) /
+                      (LDG(c.g.z_scalar, ia, ka, jl - 2) - LDG(c.g.z_scalar, ia, ka, jl - 1)) +
                   rhog(c, s, ib, kb, jl - 1) +
-                  (AT(c.g.z_w, ib, kb, jl) - AT(c.g.z_scalar, ib, kb, jl - 1)) * (rhog(c, s, ib, kb, jl - 2) - rhog(c, s, ib, kb, jl - 1)) /
-                      (AT(c.g.z_scalar, ib, kb, jl - 2) - AT(c.g.z_scalar, ib, kb, jl - 1)));
+                  (LDG(c.g.z_w, ib, kb, jl) - LDG(c.g.z_scalar, ib, kb, jl - 1)) * (rhog(c, s, ib, kb, jl - 2) - rhog(c, s, ib, kb, jl - 1)) /
+                      (LDG(c.g.z_scalar, ib, kb, jl - 2) - LDG(c.g.z_scalar, ib, kb, jl - 1)));
   }
   return R(0.25) * (rhog(c, s, ia, ka, jl - 1) + rhog(c, s, ib, kb, jl - 1) + rhog(c, s, ia, ka, jl) + rhog(c, s, ib, kb, jl));
 }
 
-// calc_pot_vort (mo_vorticities.F90:189-421) with rel_vort (:19-187) inline
+// calc_pot_vort (mo_vorticities.F90:189-421) with rel_vort (:19-187) inline: the three potential
+// vorticities of cell (ji, jk, jl) (of two cells for X = XP), formed before any store so that the
+// loads they share are made once.  eta_z is zero above the layers.
+template <class R, class X>
+struct PotVort {
+  VT<R, X> ex, ey, ez;
+};
+template <class R, class X>
+__device__ __forceinline__ PotVort<R, X> pot_vort_at(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  using V = VT<R, X>;
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
+  const int jm = wrapm(ji, ny);
+  const auto km = wrapm(jk, nx);
+  const int le = jl == 1 ? 2 : jl;  // zeta_x,y(:,:,1) = zeta_x,y(:,:,2)
+  PotVort<R, X> o;
+  // eta_x
+  V zx = R(0.0);
+  if (!c.C.llinear) zx = zeta_x_circ(c, s, ji, jk, le) / LDG(c.g.area_dual_x, ji, jk, le);
+  o.ex = (c.C.lcorio ? zx + LDG(c.g.fvec_x, ji, jk, 1) : zx) / pv_den_h(c, s, jm, jk, ji, jk, jl);
+  // eta_y
+  V zy = R(0.0);
+  if (!c.C.llinear) zy = zeta_y_circ(c, s, ji, jk, le) / LDG(c.g.area_dual_y, ji, jk, le);
+  o.ey = (c.C.lcorio ? zy + LDG(c.g.fvec_y, ji, jk, 1) : zy) / pv_den_h(c, s, ji, km, ji, jk, jl);
+  // eta_z
+  o.ez = R(0.0);
+  if (jl <= nl) {
+    V z = R(0.0);
+    if (!c.C.llinear) z = rel_vort_z(c, s, ji, jk, jl) / LDG(c.g.area_dual_z, ji, jk, jl);
+    const V e = c.C.lcorio ? z + LDG(c.g.fvec_z, ji, jk, 1) : z;
+    o.ez = e / (R(0.25) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl) + rhog(c, s, jm, jk, jl) + rhog(c, s, jm, km, jl)));
+  }
+  return o;
+}
+// The values of row ny+1 and column nx+1 are copies of row 1 and column 1 (the reference
+// evaluates them with the indices of row 1 and column 1): the threads of row 1 and column 1
+// store them, and the launch covers the nx by ny cells.
 template <class R>
 __device__ __forceinline__ void b_pot_vort(const CtxT<R> &c, const StateDev &s, const Bid bid) {
-  // The values of row ny+1 and column nx+1 are copies of row 1 and column 1 (the reference
-  // evaluates them with the indices of row 1 and column 1): the threads of row 1 and column 1
-  // store them, and the launch covers the nx by ny cells.  The three potential vorticities of a
-  // cell are formed before any store, so the loads they share are made once.
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   if (jk > nx || ji > ny || jl > nlev) return;
-  const int jm = wrapm(ji, ny), km = wrapm(jk, nx);
-  const int le = jl == 1 ? 2 : jl;  // zeta_x,y(:,:,1) = zeta_x,y(:,:,2)
-  // eta_x
-  R zx = R(0.0);
-  if (!c.C.llinear) zx = zeta_x_circ(c, s, ji, jk, le) / AT(c.g.area_dual_x, ji, jk, le);
-  const R ex = (c.C.lcorio ? zx + c.g.fvec_x[LG_IX(c.G, ji, jk, 1)] : zx) / pv_den_h(c, s, jm, jk, ji, jk, jl);
-  // eta_y
-  R zy = R(0.0);
-  if (!c.C.llinear) zy = zeta_y_circ(c, s, ji, jk, le) / AT(c.g.area_dual_y, ji, jk, le);
-  const R ey = (c.C.lcorio ? zy + c.g.fvec_y[LG_IX(c.G, ji, jk, 1)] : zy) / pv_den_h(c, s, ji, km, ji, jk, jl);
-  // eta_z
-  R ez = R(0.0);
-  if (jl <= nl) {
-    R z = R(0.0);
-    if (!c.C.llinear) z = rel_vort_z(c, s, ji, jk, jl) / AT(c.g.area_dual_z, ji, jk, jl);
-    const R e = c.C.lcorio ? z + c.g.fvec_z[LG_IX(c.G, ji, jk, 1)] : z;
-    ez = e / (R(0.25) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl) + rhog(c, s, jm, jk, jl) + rhog(c, s, jm, km, jl)));
-  }
-  AT(c.w.eta_x, ji, jk, jl) = ex;
-  AT(c.w.eta_y, ji, jk, jl) = ey;
-  if (jl <= nl) AT(c.w.eta_z, ji, jk, jl) = ez;
+  const PotVort<R, int> o = pot_vort_at(c, s, ji, jk, jl);
+  AT(c.w.eta_x, ji, jk, jl) = o.ex;
+  AT(c.w.eta_y, ji, jk, jl) = o.ey;
+  if (jl <= nl) AT(c.w.eta_z, ji, jk, jl) = o.ez;
   if (ji == 1) {
-    AT(c.w.eta_x, ny + 1, jk, jl) = ex;
-    if (jl <= nl) AT(c.w.eta_z, ny + 1, jk, jl) = ez;
+    AT(c.w.eta_x, ny + 1, jk, jl) = o.ex;
+    if (jl <= nl) AT(c.w.eta_z, ny + 1, jk, jl) = o.ez;
   }
   if (jk == 1) {
-    AT(c.w.eta_y, ji, nx + 1, jl) = ey;
-    if (jl <= nl) AT(c.w.eta_z, ji, nx + 1, jl) = ez;
+    AT(c.w.eta_y, ji, nx + 1, jl) = o.ey;
+    if (jl <= nl) AT(c.w.eta_z, ji, nx + 1, jl) = o.ez;
   }
-  if (ji == 1 && jk == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = ez;
+  if (ji == 1 && jk == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = o.ez;
 }
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
   b_pot_vort(c, s, bid_here());
+}
+// The same with two x-adjacent cells per thread (nx even): the launch covers nx/2 by ny pairs.
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pot_vort2(const CtxT<R> c, const StateDev s) {
+  const Bid bid = bid_here();
+  KIDX_ZFAST(c.G.nx / 2, c.G.ny, c.G.nlev);
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
+  const int k = 2 * jk - 1;  // the pair's first column
+  if (k > nx || ji > ny || jl > nlev) return;
+  __builtin_assume(k >= 1 && k < nx);
+  const XP x{k};
+  const PotVort<R, XP> o = pot_vort_at(c, s, ji, x, jl);
+  stv(c.w.eta_x, c.G, ji, x, jl, o.ex);
+  stv(c.w.eta_y, c.G, ji, x, jl, o.ey);
+  if (jl <= nl) stv(c.w.eta_z, c.G, ji, x, jl, o.ez);
+  if (ji == 1) {
+    stv(c.w.eta_x, c.G, ny + 1, x, jl, o.ex);
+    if (jl <= nl) stv(c.w.eta_z, c.G, ny + 1, x, jl, o.ez);
+  }
+  if (k == 1) {
+    AT(c.w.eta_y, ji, nx + 1, jl) = o.ey.a;
+    if (jl <= nl) AT(c.w.eta_z, ji, nx + 1, jl) = o.ez.a;
+  }
+  if (ji == 1 && k == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = o.ez.a;
 }
 
 // mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80).  The
@@ -684,6 +722,7 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   template __global__ void k_theta_edge<R>(const CtxT<R>, Fld<const double>, Fld<const double>, int);   \
   template __global__ void k_pgrad<R>(const CtxT<R>, const StateDev, int);                               \
   template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev);                                 \
+  template __global__ void k_pot_vort2<R>(const CtxT<R>, const StateDev);                                \
   template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev, int);                           \
   template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
