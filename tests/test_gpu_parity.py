@@ -50,6 +50,11 @@ CASES = {
     "schaer_61x24": (schaer_config(nx=61, n_layers=24), 30, 0.0),
     "dry3d_33x33x20": (dry3d_config(n=33, n_layers=20), 8, 0.0),
     "dry3d_66x40x18_even": (dataclasses.replace(dry3d_config(n=40, n_layers=18), nx=66), 6, 0.0),
+    # even nx: the potential vorticity with two cells per thread
+    "tiny_even_moist_o3_freeslip": (tiny_config(nx=8, moist=True, order=3, freeslip=True), 5, 0.3),
+    "tiny_even_linear": (tiny_config(ny=7, nx=6, n_layers=8, llinear=True), 3, 0.0),
+    "tiny_even_plane_nocorio": (tiny_config(nx=10, corio=False, plane=True), 4, 0.0),
+    "schaer_60x24": (schaer_config(nx=60, n_layers=24), 30, 0.0),
     # diffusion and turbulence
     "tiny_dry_o2_diff": (diff(tiny_config()), 4, 0.0),
     "tiny_moist_o3_freeslip_diff": (diff(tiny_config(moist=True, order=3, freeslip=True)), 5, 0.3),
