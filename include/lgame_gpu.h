@@ -465,7 +465,9 @@ int lgame_profile_reset(lgame_handle *h);
  * diff_coef, diff_dual, diff_mom, diff_flux, diff_div (the last six run only with diffusion).
  * The scalar fluxes are formed inside the scalar divergence kernel, the kinetic energy inside the
  * mass-flux kernel and its gradient inside the vorticity-flux kernel, so the scalar_flux, ke and
- * ke_grad entries are kept for the order but not used.  The scalar divergence kernel marches down the columns; its
+ * ke_grad entries are kept for the order but not used.  The pgrad entry is the first substep's
+ * pressure gradient together with its wind update, vec_tend the second substep's wind update.
+ * The scalar divergence kernel marches down the columns; its
  * levels can be cut into 1, 2, 4 or 8 segments (at least 4 levels each) so that small domains
  * fill the GPU: lgame_init picks the segments that give about four wavefronts per SIMD,
  * lgame_autotune times 1, 2, 4 and 8 after the block shapes, lgame_set_segments sets them and

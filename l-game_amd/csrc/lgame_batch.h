@@ -15,6 +15,8 @@
 
 namespace lgame {
 
+// PH_PGRAD is the first substep's pressure gradient with its wind update (b_pgrad_vt); PH_VEC_TEND
+// the second substep's wind update.
 enum Phase : int {
   PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
   PH_MASS_FLUX, PH_VORT_FLUX, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,

@@ -330,7 +330,6 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
         LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 2);
         LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
       }
-      LG_LAUNCH_ZFAST(KI_PGRAD, k_pgrad, nx, ny, nl, c, old, step == 0 ? 1 : 0);
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
@@ -353,7 +352,12 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     if (rk == 1 && c.C.lpbl) LG_LAUNCH(KI_DIFF_DUAL, k_pbl_wind, nx, ny, 1, c, old);
     if (rk == 1 && (c.C.lklemp || c.C.lmom_h || c.C.lmom_v || c.C.lpbl))
       LG_LAUNCH(KI_DISSIPATION, k_dissipation, nx, ny, nl, c, old);
-    LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
+    // the first substep's pressure gradient (manage_p_grad) and, from its values, the wind update;
+    // no kernel in between reads or writes what the pressure gradient uses or forms
+    if (rk == 1)
+      LG_LAUNCH_ZFAST(KI_PGRAD, k_pgrad_vt, nx, ny, nl, c, old, nw, step == 0 ? 1 : 0);
+    else
+      LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
     // temperature and mass diffusion with the coefficients of this substep (mo_scalar_tend_expl.F90:
     // 60-99).  The temperature diffusion of the second substep repeats the first one with the same
     // temperature and coefficients, so its result is kept.
@@ -1714,7 +1718,6 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
         t.nfield = 2;
         for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
       }
-      launch(PH_PGRAD, a);
     }
     if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
       launch(PH_POT_VORT, a);
@@ -1725,7 +1728,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
       for (int ph : {PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM}) launch(ph, a);
     if (rk == 1 && C.lpbl) launch(PH_PBL_WIND, a);
     if (rk == 1 && (C.lklemp || C.lmom_h || C.lmom_v || C.lpbl)) launch(PH_DISSIPATION, a);
-    launch(PH_VEC_TEND, a);
+    launch(rk == 1 ? PH_PGRAD : PH_VEC_TEND, a);  // at rk 1 with the pressure gradient
     if (rk == 1 && (C.ltemp_h || C.lmass_h)) {
       launch(PH_DIFF_FLUX, a);
       launch(PH_DIFF_DIV, a);

@@ -11,13 +11,14 @@ template <class R> __global__ void k_theta_fill(const CtxT<R> c, const StateDev 
 template <class R> __global__ void k_theta_grad(const CtxT<R> c, int nfield);
 template <class R> __global__ void k_theta_div(const CtxT<R> c, int nfield);
 template <class R> __global__ void k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield);
-template <class R> __global__ void k_pgrad(const CtxT<R> c, const StateDev s, int lfirst);
 template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s);  // nx even: two cells per thread
 template <class R> __global__ void k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke);
 template <class R> __global__ void k_vort_flux(const CtxT<R> c, int with_ke_grad);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
+// the first substep's pressure gradient (manage_p_grad) with its wind update (k_vec_tend at rk 1)
+template <class R> __global__ void k_pgrad_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int lfirst);
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s);

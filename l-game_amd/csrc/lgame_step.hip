@@ -131,9 +131,17 @@ __global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<con
 // The pressure-gradient accelerations at the u, v and w points of one cell (ji <= ny, jk <= nx). Its west, south and upper neighbours and the cell itself are shared by the three
 // points, so every state value, metric and deceleration factor is loaded or computed once and
 // then combined with the expressions of grad_hor_x/_y, remap_ver2hor_x/_y and the w-point
-// gradient, in their order.
+// gradient, in their order.  Returns the values it stores at the cell's three points.
 template <class R>
-__device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev &s, int lfirst, int ji, int jk, int jl) {
+struct PGrad {
+  R nl_u, l_u, old_u;  // u point: nonlinear and linear parts, and the previous step's sum (pold)
+  R nl_v, l_v, old_v;  // v point
+  R nl_w, l_w;         // w point (levels 2..n_layers)
+};
+template <class R>
+__device__ __forceinline__ PGrad<R> pgrad_interior(const CtxT<R> &c, const StateDev &s, int lfirst, int ji, int jk, int jl) {
+  PGrad<R> o;
+  o.nl_w = o.l_w = R(0.0);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev, nflat = c.G.nflat;
   const bool o3 = c.C.order == 3;
   const Geom &G = c.G;
@@ -202,6 +210,9 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
       pold = -nlu - R(1.0) * llu;
       c.w.pold_u[iC] = pold;
     }
+    o.nl_u = nlu;
+    o.l_u = llu;
+    o.old_u = pold;
     if (jk == 1) {  // column nx+1 is the copy of column 1
       const ix_t iX = LG_IX(G, ji, nx + 1, jl);
       c.w.pnl_u[iX] = nlu;
@@ -244,6 +255,9 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
       pold = -nlv - R(1.0) * llv;
       c.w.pold_v[iC] = pold;
     }
+    o.nl_v = nlv;
+    o.l_v = llv;
+    o.old_v = pold;
     if (ji == 1) {  // row ny+1 is the copy of row 1
       const ix_t iY = LG_IX(G, ny + 1, jk, jl);
       c.w.pnl_v[iY] = nlv;
@@ -258,24 +272,14 @@ __device__ __forceinline__ void pgrad_interior(const CtxT<R> &c, const StateDev 
     const R t = R(0.5) * ((c.g.theta_v_bg[iA] + tpA) + thC) * vC;
     const R lw = R(0.5) * (tpA + tpC) * c.g.ebg_w[iC];
     const R d = R(0.5) * (decel_at(iA) + dC);
-    c.w.pnl_w[iC] = d * t;
-    c.w.pl_w[iC] = d * lw;
+    o.nl_w = d * t;
+    o.l_w = d * lw;
+    c.w.pnl_w[iC] = o.nl_w;
+    c.w.pl_w[iC] = o.l_w;
   }
+  return o;
 }
 
-// The u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1 (the
-// reference evaluates them with the indices of column 1 and row 1); the threads of column 1 and
-// row 1 store them, and the launch covers the nx by ny by n_layers cells only.
-template <class R>
-__device__ __forceinline__ void b_pgrad(const CtxT<R> &c, const StateDev &s, int lfirst, const Bid bid) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  pgrad_interior(c, s, lfirst, ji, jk, jl);
-}
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pgrad(const CtxT<R> c, const StateDev s, int lfirst) {
-  b_pgrad(c, s, lfirst, bid_here());
-}
 
 // ---------------------------------------------------------------------------------------------
 // Momentum advection block of vector_tend_expl (mo_vector_tend_expl.F90:58-99):
@@ -670,44 +674,92 @@ __global__ void __launch_bounds__(BX * BY) k_dissipation(const CtxT<R> c, const 
 // explicit wind tendencies (mo_vector_tend_expl.F90:128-180) and the forward update of u, v
 // (mo_manage_pchevi.F90:86-89)
 // ---------------------------------------------------------------------------------------------
+// the weights of a substep's tendency update
+template <class R>
+struct VtWeights {
+  R new_w, old_w, cur_hor, old_hor, cur_ver;
+  bool mom;  // diag%mom_diff_tend_* are added (zero arrays otherwise)
+};
+template <class R>
+__device__ __forceinline__ VtWeights<R> vt_weights(const CtxT<R> &c, int rk) {
+  VtWeights<R> w;
+  w.new_w = rk == 2 ? R(0.5) : R(1.0);
+  w.old_w = R(1.0) - w.new_w;
+  w.cur_hor = R(0.5) + R(IMPL_THERMO_WEIGHT);
+  w.old_hor = R(1.0) - w.cur_hor;
+  w.cur_ver = R(1.0) - R(IMPL_THERMO_WEIGHT);
+  w.mom = c.C.lmom_h || c.C.lmom_v || c.C.lpbl;
+  return w;
+}
+// the u point at index i, from its pressure-gradient terms (pold, pnl, pl)
+template <class R>
+__device__ __forceinline__ void vt_u(const CtxT<R> &c, const StateDev &old, const StateDev &nw, const VtWeights<R> &w, ix_t i,
+                                     R pold, R pnl, R pl) {
+  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i];
+  if (w.mom) x = x + c.w.mom_x[i];
+  const R t = w.old_w * c.w.tend_u[i] + w.new_w * x;
+  c.w.tend_u[i] = t;
+  nw.wind_u[i] = old.wind_u[i] + c.C.dtime * (double)t;  // FP64 state update
+}
+template <class R>
+__device__ __forceinline__ void vt_v(const CtxT<R> &c, const StateDev &old, const StateDev &nw, const VtWeights<R> &w, ix_t i,
+                                     R pold, R pnl, R pl) {
+  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * c.w.vsg_y[i] + c.w.pvt_y[i];
+  if (w.mom) x = x + c.w.mom_y[i];
+  const R t = w.old_w * c.w.tend_v[i] + w.new_w * x;
+  c.w.tend_v[i] = t;
+  nw.wind_v[i] = old.wind_v[i] + c.C.dtime * (double)t;
+}
+// the w point of cell (ji, jk, jl), jl >= 2
+template <class R>
+__device__ __forceinline__ void vt_w(const CtxT<R> &c, const VtWeights<R> &w, int ji, int jk, int jl, ix_t i, R pnl, R pl) {
+  R x = -w.cur_ver * (pnl + pl) - R(0.5) * c.w.vsg_z[i] + c.w.pvt_z[i];
+  if (c.C.lmoist) {  // calc_p_grad_condensates_v, mo_p_grad.F90:106-120 (identically 0 when dry)
+    const ix_t ia = LG_IX(c.G, ji, jk, jl - 1);
+    const R da = c.w.pgc[ia];  // rho_gas/rho_sum - 1 of the old state, from k_temperature
+    const R db = c.w.pgc[i];
+    x = x + R(0.5) * (da + db) * c.g.gravity_m_v[i];
+  }
+  if (w.mom) x = x + c.w.mom_z[i];
+  c.w.tend_w[i] = w.old_w * c.w.tend_w[i] + w.new_w * x;
+}
 template <class R>
 __device__ __forceinline__ void b_vec_tend(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
   if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
-  const R new_w = rk == 2 ? R(0.5) : R(1.0), old_w = R(1.0) - new_w;
-  const R cur_hor = R(0.5) + R(IMPL_THERMO_WEIGHT), old_hor = R(1.0) - cur_hor, cur_ver = R(1.0) - R(IMPL_THERMO_WEIGHT);
+  const VtWeights<R> w = vt_weights(c, rk);
   const ix_t i = LG_IX(c.G, ji, jk, jl);
-  const bool mom = c.C.lmom_h || c.C.lmom_v || c.C.lpbl;  // diag%mom_diff_tend_* (zero arrays otherwise)
-  if (ji <= ny) {
-    R x = old_hor * c.w.pold_u[i] - cur_hor * (c.w.pnl_u[i] + c.w.pl_u[i]) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i];
-    if (mom) x = x + c.w.mom_x[i];
-    const R t = old_w * c.w.tend_u[i] + new_w * x;
-    c.w.tend_u[i] = t;
-    nw.wind_u[i] = old.wind_u[i] + c.C.dtime * (double)t;  // FP64 state update
-  }
-  if (jk <= nx) {
-    R x = old_hor * c.w.pold_v[i] - cur_hor * (c.w.pnl_v[i] + c.w.pl_v[i]) - R(0.5) * c.w.vsg_y[i] + c.w.pvt_y[i];
-    if (mom) x = x + c.w.mom_y[i];
-    const R t = old_w * c.w.tend_v[i] + new_w * x;
-    c.w.tend_v[i] = t;
-    nw.wind_v[i] = old.wind_v[i] + c.C.dtime * (double)t;
-  }
-  if (ji <= ny && jk <= nx && jl >= 2) {
-    R x = -cur_ver * (c.w.pnl_w[i] + c.w.pl_w[i]) - R(0.5) * c.w.vsg_z[i] + c.w.pvt_z[i];
-    if (c.C.lmoist) {  // calc_p_grad_condensates_v, mo_p_grad.F90:106-120 (identically 0 when dry)
-      const ix_t ia = LG_IX(c.G, ji, jk, jl - 1);
-      const R da = c.w.pgc[ia];  // rho_gas/rho_sum - 1 of the old state, from k_temperature
-      const R db = c.w.pgc[i];
-      x = x + R(0.5) * (da + db) * c.g.gravity_m_v[i];
-    }
-    if (mom) x = x + c.w.mom_z[i];
-    c.w.tend_w[i] = old_w * c.w.tend_w[i] + new_w * x;
-  }
+  if (ji <= ny) vt_u(c, old, nw, w, i, c.w.pold_u[i], c.w.pnl_u[i], c.w.pl_u[i]);
+  if (jk <= nx) vt_v(c, old, nw, w, i, c.w.pold_v[i], c.w.pnl_v[i], c.w.pl_v[i]);
+  if (ji <= ny && jk <= nx && jl >= 2) vt_w(c, w, ji, jk, jl, i, c.w.pnl_w[i], c.w.pl_w[i]);
 }
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
   b_vec_tend(c, old, nw, rk, bid_here());
+}
+
+// The first substep's pressure gradient and wind update in one kernel: the pressure-gradient terms
+// of a cell's u, v and w points are stored for the second substep and used from registers for the
+// first one.  The thread of column 1 also updates the u point of column nx+1, and the thread of
+// row 1 the v point of row ny+1, whose pressure-gradient terms are the copies of theirs.
+template <class R>
+__device__ __forceinline__ void b_pgrad_vt(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int lfirst, const Bid bid) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  const int nx = c.G.nx, ny = c.G.ny;
+  if (jk > nx || ji > ny || jl > c.G.nl) return;
+  const PGrad<R> g = pgrad_interior(c, old, lfirst, ji, jk, jl);
+  const VtWeights<R> w = vt_weights(c, 1);
+  const ix_t i = LG_IX(c.G, ji, jk, jl);
+  vt_u(c, old, nw, w, i, g.old_u, g.nl_u, g.l_u);
+  if (jk == 1) vt_u(c, old, nw, w, LG_IX(c.G, ji, nx + 1, jl), g.old_u, g.nl_u, g.l_u);
+  vt_v(c, old, nw, w, i, g.old_v, g.nl_v, g.l_v);
+  if (ji == 1) vt_v(c, old, nw, w, LG_IX(c.G, ny + 1, jk, jl), g.old_v, g.nl_v, g.l_v);
+  if (jl >= 2) vt_w(c, w, ji, jk, jl, i, g.nl_w, g.l_w);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_pgrad_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int lfirst) {
+  b_pgrad_vt(c, old, nw, lfirst, bid_here());
 }
 
 // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181) is the column-marching k_scalar_col of
@@ -720,13 +772,13 @@ __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const Sta
   template __global__ void k_theta_grad<R>(const CtxT<R>, int);                                          \
   template __global__ void k_theta_div<R>(const CtxT<R>, int);                                           \
   template __global__ void k_theta_edge<R>(const CtxT<R>, Fld<const double>, Fld<const double>, int);   \
-  template __global__ void k_pgrad<R>(const CtxT<R>, const StateDev, int);                               \
   template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev);                                 \
   template __global__ void k_pot_vort2<R>(const CtxT<R>, const StateDev);                                \
   template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev, int);                           \
   template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
-  template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);
+  template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);                 \
+  template __global__ void k_pgrad_vt<R>(const CtxT<R>, const StateDev, const StateDev, int);
 
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
@@ -740,7 +792,7 @@ LG_BATCH_KERNEL(theta_div, BX * BY, PH_THETA_DIV, b_theta_div(d.c, a.nfield, bid
 LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
                 b_theta_edge(d.c, Fld<const double>(d.s[a.state_slot].wind_u), Fld<const double>(d.s[a.state_slot].wind_v),
                              a.nfield, bid))
-LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad(d.c, d.s[a.old_slot], a.lfirst, bid))
+LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad_vt(d.c, d.s[a.old_slot], d.s[a.new_slot], a.lfirst, bid))
 LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
 LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], a.ke, bid))
 LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, a.ke, bid))
