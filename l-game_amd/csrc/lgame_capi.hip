@@ -332,6 +332,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
       }
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
+    bool winds_done = false;  // the wind update was made with the vorticity flux term
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
       if (nx % 2 == 0)
         LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort2, nx / 2, ny, nlev, c, S);
@@ -339,8 +340,14 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
         LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
       // the mass fluxes and, from the same winds, the kinetic energy
       LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
-      // the vorticity flux term and, in the same cells, the gradient of the kinetic energy
-      LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c, c.C.llinear ? 0 : 1);
+      // the vorticity flux term and, in the same cells, the gradient of the kinetic energy; at
+      // rk 2 with the wind update, which follows it directly
+      if (rk == 2) {
+        LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux_vt, nx, ny, nl, c, old, nw, c.C.llinear ? 0 : 1);
+        winds_done = true;
+      } else {
+        LG_LAUNCH_ZFAST(KI_VORT_FLUX, k_vort_flux, nx, ny, nl, c, c.C.llinear ? 0 : 1);
+      }
     }
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
@@ -356,7 +363,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // no kernel in between reads or writes what the pressure gradient uses or forms
     if (rk == 1)
       LG_LAUNCH_ZFAST(KI_PGRAD, k_pgrad_vt, nx, ny, nl, c, old, nw, step == 0 ? 1 : 0);
-    else
+    else if (!winds_done)
       LG_LAUNCH(KI_VEC_TEND, k_vec_tend, nx + 1, ny + 1, nl, c, old, nw, rk);
     // temperature and mass diffusion with the coefficients of this substep (mo_scalar_tend_expl.F90:
     // 60-99).  The temperature diffusion of the second substep repeats the first one with the same
@@ -1719,16 +1726,22 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
         for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
       }
     }
+    bool winds_done = false;
     if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
       launch(PH_POT_VORT, a);
       launch(PH_MASS_FLUX, a);  // with the kinetic energy
-      launch(PH_VORT_FLUX, a);  // with the gradient of the kinetic energy
+      // with the gradient of the kinetic energy, and at rk 2 the wind update
+      launch(rk == 2 ? PH_VORT_FLUX_VT : PH_VORT_FLUX, a);
+      winds_done = rk == 2;
     }
     if (rk == 1 && C.lmom_h)
       for (int ph : {PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM}) launch(ph, a);
     if (rk == 1 && C.lpbl) launch(PH_PBL_WIND, a);
     if (rk == 1 && (C.lklemp || C.lmom_h || C.lmom_v || C.lpbl)) launch(PH_DISSIPATION, a);
-    launch(rk == 1 ? PH_PGRAD : PH_VEC_TEND, a);  // at rk 1 with the pressure gradient
+    if (rk == 1)
+      launch(PH_PGRAD, a);  // the pressure gradient with the wind update
+    else if (!winds_done)
+      launch(PH_VEC_TEND, a);
     if (rk == 1 && (C.ltemp_h || C.lmass_h)) {
       launch(PH_DIFF_FLUX, a);
       launch(PH_DIFF_DIV, a);

@@ -19,6 +19,8 @@ template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 // the first substep's pressure gradient (manage_p_grad) with its wind update (k_vec_tend at rk 1)
 template <class R> __global__ void k_pgrad_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int lfirst);
+// the second substep's vorticity flux term (and kinetic-energy gradient) with its wind update
+template <class R> __global__ void k_vort_flux_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int with_ke_grad);
 template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s);

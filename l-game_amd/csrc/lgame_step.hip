@@ -476,9 +476,13 @@ __global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const St
 // The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
 // mass-flux and weight value that two of the points share is loaded once, and the three results
 // are stored after all loads.  The expressions and their order are those of the reference's
-// per-point loops.
+// per-point loops.  Returns the three values.
 template <class R>
-__device__ __forceinline__ void vort_flux_interior(const CtxT<R> &c, int ji, int jk, int jl) {
+struct VortFlux {
+  R x, y, z;  // at the u, v and w points (z is zero at level 1)
+};
+template <class R>
+__device__ __forceinline__ VortFlux<R> vort_flux_interior(const CtxT<R> &c, int ji, int jk, int jl) {
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny;
   const Fld<const R> ez = c.w.eta_z, up = c.w.u_ph, vp = c.w.v_ph, wp = c.w.w_ph;
@@ -520,6 +524,7 @@ __device__ __forceinline__ void vort_flux_interior(const CtxT<R> &c, int ji, int
   if (jl >= 2) c.w.pvt_z[iC] = rw;
   if (jk == 1) c.w.pvt_x[LG_IX(G, ji, nx + 1, jl)] = ru;
   if (ji == 1) c.w.pvt_y[LG_IX(G, ny + 1, jk, jl)] = rv;
+  return VortFlux<R>{ru, rv, rw};
 }
 
 // calc_vorticity_flux_term, mo_vorticity_flux.F90:16-171.  The u points of column nx+1 and the
@@ -544,7 +549,13 @@ __global__ void __launch_bounds__(BX * BY) k_vort_flux(const CtxT<R> c, int with
 // jk <= nx): the values and vertical gradients that the three share are loaded or formed once and
 // combined with the expressions of grad_hor_x/_y and remap_ver2hor_x/_y, in their order.
 template <class R>
-__device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int jk, int jl) {
+struct KeGrad {
+  R x, y, z;  // at the u, v and w points (z is zero at level 1)
+  R x_edge;   // at the u point of column nx+1 (thread of column 1)
+  R y_edge;   // at the v point of row ny+1 (thread of row 1)
+};
+template <class R>
+__device__ __forceinline__ KeGrad<R> ke_grad_interior(const CtxT<R> &c, int ji, int jk, int jl) {
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny, nl = G.nl, nlev = G.nlev, nflat = G.nflat;
   const Fld<const R> k2 = c.w.v_squared;
@@ -589,6 +600,7 @@ __device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int j
     }
     gy = gy - c.g.slope_y[iC] * (R(0.5) * ry);
   }
+  KeGrad<R> o{gx, gy, jl >= 2 ? vC : R(0.0), R(0.0), R(0.0)};
   c.w.vsg_x[iC] = gx;
   c.w.vsg_y[iC] = gy;
   if (jl >= 2) c.w.vsg_z[iC] = vC;
@@ -597,12 +609,15 @@ __device__ __forceinline__ void ke_grad_interior(const CtxT<R> &c, int ji, int j
   // and their own slope.
   if (jk == 1) {
     const ix_t iX = LG_IX(G, ji, nx + 1, jl);
-    c.w.vsg_x[iX] = terr ? gx_cov - c.g.slope_x[iX] * (R(0.5) * rx) : gx_cov;
+    o.x_edge = terr ? gx_cov - c.g.slope_x[iX] * (R(0.5) * rx) : gx_cov;
+    c.w.vsg_x[iX] = o.x_edge;
   }
   if (ji == 1) {
     const ix_t iY = LG_IX(G, ny + 1, jk, jl);
-    c.w.vsg_y[iY] = terr ? gy_cov - c.g.slope_y[iY] * (R(0.5) * ry) : gy_cov;
+    o.y_edge = terr ? gy_cov - c.g.slope_y[iY] * (R(0.5) * ry) : gy_cov;
+    c.w.vsg_y[iY] = o.y_edge;
   }
+  return o;
 }
 
 
@@ -692,10 +707,11 @@ __device__ __forceinline__ VtWeights<R> vt_weights(const CtxT<R> &c, int rk) {
   return w;
 }
 // the u point at index i, from its pressure-gradient terms (pold, pnl, pl)
+// and the kinetic-energy gradient and vorticity flux term there (vsg, pvt)
 template <class R>
 __device__ __forceinline__ void vt_u(const CtxT<R> &c, const StateDev &old, const StateDev &nw, const VtWeights<R> &w, ix_t i,
-                                     R pold, R pnl, R pl) {
-  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * c.w.vsg_x[i] + c.w.pvt_x[i];
+                                     R pold, R pnl, R pl, R vsg, R pvt) {
+  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * vsg + pvt;
   if (w.mom) x = x + c.w.mom_x[i];
   const R t = w.old_w * c.w.tend_u[i] + w.new_w * x;
   c.w.tend_u[i] = t;
@@ -703,8 +719,8 @@ __device__ __forceinline__ void vt_u(const CtxT<R> &c, const StateDev &old, cons
 }
 template <class R>
 __device__ __forceinline__ void vt_v(const CtxT<R> &c, const StateDev &old, const StateDev &nw, const VtWeights<R> &w, ix_t i,
-                                     R pold, R pnl, R pl) {
-  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * c.w.vsg_y[i] + c.w.pvt_y[i];
+                                     R pold, R pnl, R pl, R vsg, R pvt) {
+  R x = w.old_hor * pold - w.cur_hor * (pnl + pl) - R(0.5) * vsg + pvt;
   if (w.mom) x = x + c.w.mom_y[i];
   const R t = w.old_w * c.w.tend_v[i] + w.new_w * x;
   c.w.tend_v[i] = t;
@@ -712,8 +728,9 @@ __device__ __forceinline__ void vt_v(const CtxT<R> &c, const StateDev &old, cons
 }
 // the w point of cell (ji, jk, jl), jl >= 2
 template <class R>
-__device__ __forceinline__ void vt_w(const CtxT<R> &c, const VtWeights<R> &w, int ji, int jk, int jl, ix_t i, R pnl, R pl) {
-  R x = -w.cur_ver * (pnl + pl) - R(0.5) * c.w.vsg_z[i] + c.w.pvt_z[i];
+__device__ __forceinline__ void vt_w(const CtxT<R> &c, const VtWeights<R> &w, int ji, int jk, int jl, ix_t i, R pnl, R pl, R vsg,
+                                     R pvt) {
+  R x = -w.cur_ver * (pnl + pl) - R(0.5) * vsg + pvt;
   if (c.C.lmoist) {  // calc_p_grad_condensates_v, mo_p_grad.F90:106-120 (identically 0 when dry)
     const ix_t ia = LG_IX(c.G, ji, jk, jl - 1);
     const R da = c.w.pgc[ia];  // rho_gas/rho_sum - 1 of the old state, from k_temperature
@@ -730,9 +747,9 @@ __device__ __forceinline__ void b_vec_tend(const CtxT<R> &c, const StateDev &old
   if (jk > nx + 1 || ji > ny + 1 || jl > nl) return;
   const VtWeights<R> w = vt_weights(c, rk);
   const ix_t i = LG_IX(c.G, ji, jk, jl);
-  if (ji <= ny) vt_u(c, old, nw, w, i, c.w.pold_u[i], c.w.pnl_u[i], c.w.pl_u[i]);
-  if (jk <= nx) vt_v(c, old, nw, w, i, c.w.pold_v[i], c.w.pnl_v[i], c.w.pl_v[i]);
-  if (ji <= ny && jk <= nx && jl >= 2) vt_w(c, w, ji, jk, jl, i, c.w.pnl_w[i], c.w.pl_w[i]);
+  if (ji <= ny) vt_u(c, old, nw, w, i, c.w.pold_u[i], c.w.pnl_u[i], c.w.pl_u[i], c.w.vsg_x[i], c.w.pvt_x[i]);
+  if (jk <= nx) vt_v(c, old, nw, w, i, c.w.pold_v[i], c.w.pnl_v[i], c.w.pl_v[i], c.w.vsg_y[i], c.w.pvt_y[i]);
+  if (ji <= ny && jk <= nx && jl >= 2) vt_w(c, w, ji, jk, jl, i, c.w.pnl_w[i], c.w.pl_w[i], c.w.vsg_z[i], c.w.pvt_z[i]);
 }
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
@@ -751,15 +768,56 @@ __device__ __forceinline__ void b_pgrad_vt(const CtxT<R> &c, const StateDev &old
   const PGrad<R> g = pgrad_interior(c, old, lfirst, ji, jk, jl);
   const VtWeights<R> w = vt_weights(c, 1);
   const ix_t i = LG_IX(c.G, ji, jk, jl);
-  vt_u(c, old, nw, w, i, g.old_u, g.nl_u, g.l_u);
-  if (jk == 1) vt_u(c, old, nw, w, LG_IX(c.G, ji, nx + 1, jl), g.old_u, g.nl_u, g.l_u);
-  vt_v(c, old, nw, w, i, g.old_v, g.nl_v, g.l_v);
-  if (ji == 1) vt_v(c, old, nw, w, LG_IX(c.G, ny + 1, jk, jl), g.old_v, g.nl_v, g.l_v);
-  if (jl >= 2) vt_w(c, w, ji, jk, jl, i, g.nl_w, g.l_w);
+  vt_u(c, old, nw, w, i, g.old_u, g.nl_u, g.l_u, c.w.vsg_x[i], c.w.pvt_x[i]);
+  if (jk == 1) {
+    const ix_t iX = LG_IX(c.G, ji, nx + 1, jl);
+    vt_u(c, old, nw, w, iX, g.old_u, g.nl_u, g.l_u, c.w.vsg_x[iX], c.w.pvt_x[iX]);
+  }
+  vt_v(c, old, nw, w, i, g.old_v, g.nl_v, g.l_v, c.w.vsg_y[i], c.w.pvt_y[i]);
+  if (ji == 1) {
+    const ix_t iY = LG_IX(c.G, ny + 1, jk, jl);
+    vt_v(c, old, nw, w, iY, g.old_v, g.nl_v, g.l_v, c.w.vsg_y[iY], c.w.pvt_y[iY]);
+  }
+  if (jl >= 2) vt_w(c, w, ji, jk, jl, i, g.nl_w, g.l_w, c.w.vsg_z[i], c.w.pvt_z[i]);
 }
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_pgrad_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int lfirst) {
   b_pgrad_vt(c, old, nw, lfirst, bid_here());
+}
+
+// The second substep's vorticity flux term (with the kinetic-energy gradient) and wind update in
+// one kernel: both terms are stored for the next step's first substep and used from registers
+// here.  The thread of column 1 also updates the u point of column nx+1, and the thread of row 1
+// the v point of row ny+1.  Without the kinetic-energy gradient (linear runs) its stored values
+// are read.
+template <class R>
+__device__ __forceinline__ void b_vort_flux_vt(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int with_ke_grad,
+                                               const Bid bid) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  const int nx = c.G.nx, ny = c.G.ny;
+  if (jk > nx || ji > ny || jl > c.G.nl) return;
+  const ix_t i = LG_IX(c.G, ji, jk, jl), iX = LG_IX(c.G, ji, nx + 1, jl), iY = LG_IX(c.G, ny + 1, jk, jl);
+  const VortFlux<R> f = vort_flux_interior(c, ji, jk, jl);
+  KeGrad<R> g;
+  if (with_ke_grad) {
+    g = ke_grad_interior(c, ji, jk, jl);
+  } else {
+    g.x = c.w.vsg_x[i];
+    g.y = c.w.vsg_y[i];
+    g.z = jl >= 2 ? c.w.vsg_z[i] : R(0.0);
+    g.x_edge = jk == 1 ? c.w.vsg_x[iX] : R(0.0);
+    g.y_edge = ji == 1 ? c.w.vsg_y[iY] : R(0.0);
+  }
+  const VtWeights<R> w = vt_weights(c, 2);
+  vt_u(c, old, nw, w, i, c.w.pold_u[i], c.w.pnl_u[i], c.w.pl_u[i], g.x, f.x);
+  if (jk == 1) vt_u(c, old, nw, w, iX, c.w.pold_u[iX], c.w.pnl_u[iX], c.w.pl_u[iX], g.x_edge, f.x);
+  vt_v(c, old, nw, w, i, c.w.pold_v[i], c.w.pnl_v[i], c.w.pl_v[i], g.y, f.y);
+  if (ji == 1) vt_v(c, old, nw, w, iY, c.w.pold_v[iY], c.w.pnl_v[iY], c.w.pl_v[iY], g.y_edge, f.y);
+  if (jl >= 2) vt_w(c, w, ji, jk, jl, i, c.w.pnl_w[i], c.w.pl_w[i], g.z, f.z);
+}
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_vort_flux_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int with_ke_grad) {
+  b_vort_flux_vt(c, old, nw, with_ke_grad, bid_here());
 }
 
 // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181) is the column-marching k_scalar_col of
@@ -778,7 +836,8 @@ __global__ void __launch_bounds__(BX * BY) k_pgrad_vt(const CtxT<R> c, const Sta
   template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
   template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);                 \
-  template __global__ void k_pgrad_vt<R>(const CtxT<R>, const StateDev, const StateDev, int);
+  template __global__ void k_pgrad_vt<R>(const CtxT<R>, const StateDev, const StateDev, int);                 \
+  template __global__ void k_vort_flux_vt<R>(const CtxT<R>, const StateDev, const StateDev, int);
 
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
@@ -796,6 +855,7 @@ LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad_vt(d.c, d.s[a.old_slot], d.s[a
 LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
 LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], a.ke, bid))
 LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, a.ke, bid))
+LG_BATCH_KERNEL(vort_flux_vt, BX * BY, PH_VORT_FLUX_VT, b_vort_flux_vt(d.c, d.s[a.old_slot], d.s[a.new_slot], a.ke, bid))
 LG_BATCH_KERNEL(dissipation, BX * BY, PH_DISSIPATION, b_dissipation(d.c, d.s[a.old_slot], bid))
 LG_BATCH_KERNEL(vec_tend, BX * BY, PH_VEC_TEND, b_vec_tend(d.c, d.s[a.old_slot], d.s[a.new_slot], a.rk, bid))
 
@@ -815,6 +875,7 @@ bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R
     LG_B(PH_POT_VORT, kb_pot_vort)
     LG_B(PH_MASS_FLUX, kb_mass_flux)
     LG_B(PH_VORT_FLUX, kb_vort_flux)
+    LG_B(PH_VORT_FLUX_VT, kb_vort_flux_vt)
     LG_B(PH_DISSIPATION, kb_dissipation)
     LG_B(PH_VEC_TEND, kb_vec_tend)
     default: return false;
