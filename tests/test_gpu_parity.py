@@ -53,6 +53,8 @@ CASES = {
     # even nx: the potential vorticity with two cells per thread
     "tiny_even_moist_o3_freeslip": (tiny_config(nx=8, moist=True, order=3, freeslip=True), 5, 0.3),
     "tiny_even_linear": (tiny_config(ny=7, nx=6, n_layers=8, llinear=True), 3, 0.0),
+    # no momentum advection at all: the second substep's wind update alone (k_vec_tend)
+    "tiny_linear_nocorio": (tiny_config(ny=7, nx=6, n_layers=8, llinear=True, corio=False), 3, 0.0),
     "tiny_even_plane_nocorio": (tiny_config(nx=10, corio=False, plane=True), 4, 0.0),
     "schaer_60x24": (schaer_config(nx=60, n_layers=24), 30, 0.0),
     # diffusion and turbulence
