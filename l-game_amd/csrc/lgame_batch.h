@@ -20,7 +20,7 @@ namespace lgame {
 // second substep's wind update alone (runs without momentum advection).
 enum Phase : int {
   PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
-  PH_MASS_FLUX, PH_VORT_FLUX, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
+  PH_VORT_FLUX, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
   PH_DISSIPATION, PH_VEC_TEND, PH_DIFF_FLUX, PH_DIFF_DIV, PH_SCALAR, PH_SURFACE, PH_SOLVE_VER, PH_SOLVE_GEN, PH_RAD,
   PH_VORT_FLUX_VT,
   PH_COUNT
@@ -37,7 +37,6 @@ inline int phase_tile(int ph) {
     case PH_THETA_EDGE: return KI_THETA_EDGE;
     case PH_PGRAD: return KI_PGRAD;
     case PH_POT_VORT: return KI_POT_VORT;
-    case PH_MASS_FLUX: return KI_MASS_FLUX;
     case PH_VORT_FLUX: case PH_VORT_FLUX_VT: return KI_VORT_FLUX;
     case PH_DIFF_ZETA: return KI_DIFF_ZETA;
     case PH_DIFF_COEF: return KI_DIFF_COEF;
@@ -82,7 +81,6 @@ __host__ __device__ inline void phase_grid(int ph, const Geom &G, int by, int se
     case PH_TEMPERATURE: case PH_THETA_FILL: case PH_THETA_DIV: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
     case PH_SFC_TURB: case PH_PBL_WIND: case PH_SURFACE: ez = 1; break;
     case PH_THETA_GRAD: case PH_THETA_EDGE: case PH_VEC_TEND: ex = nx + 1; ey = ny + 1; break;
-    case PH_MASS_FLUX: ez = nlev; break;
     case PH_PGRAD: case PH_VORT_FLUX: case PH_VORT_FLUX_VT: case PH_DIFF_COEF: case PH_DIFF_MOM: case PH_DIFF_FLUX: case PH_DIFF_DIV: flat = true; break;
     case PH_POT_VORT: ez = nlev; flat = true; break;
     case PH_DIFF_ZETA: ex = nx + 1; ey = ny + 1; flat = true; break;

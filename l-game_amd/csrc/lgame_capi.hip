@@ -334,12 +334,12 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
     bool winds_done = false;  // the wind update was made with the vorticity flux term
     if ((rk == 2 || step == 0) && (!c.C.llinear || c.C.lcorio)) {
+      // the potential vorticity, and from the same densities and winds the mass fluxes and the
+      // kinetic energy
       if (nx % 2 == 0)
-        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort2, nx / 2, ny, nlev, c, S);
+        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort2, nx / 2, ny, nlev, c, S, c.C.llinear ? 0 : 1);
       else
-        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S);
-      // the mass fluxes and, from the same winds, the kinetic energy
-      LG_LAUNCH(KI_MASS_FLUX, k_mass_flux, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
+        LG_LAUNCH_ZFAST(KI_POT_VORT, k_pot_vort, nx, ny, nlev, c, S, c.C.llinear ? 0 : 1);
       // the vorticity flux term and, in the same cells, the gradient of the kinetic energy; at
       // rk 2 with the wind update, which follows it directly
       if (rk == 2) {
@@ -1728,8 +1728,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
     }
     bool winds_done = false;
     if ((rk == 2 || step == 0) && (!C.llinear || C.lcorio)) {
-      launch(PH_POT_VORT, a);
-      launch(PH_MASS_FLUX, a);  // with the kinetic energy
+      launch(PH_POT_VORT, a);  // with the mass fluxes and the kinetic energy
       // with the gradient of the kinetic energy, and at rk 2 the wind update
       launch(rk == 2 ? PH_VORT_FLUX_VT : PH_VORT_FLUX, a);
       winds_done = rk == 2;

@@ -66,6 +66,21 @@ using VT = typename ValOf<R, X>::type;
 // west neighbours: k+1 >= 2 never wraps
 __device__ __forceinline__ XG wrapm(XP x, int n) { return XG{wrapm(x.k, n), x.k}; }
 __device__ __forceinline__ XG wrapm(XG x, int n) { return XG{wrapm(x.k0, n), wrapm(x.k1, n)}; }
+// the value of one cell, or of the pair's first cell
+template <class R>
+__device__ __forceinline__ R first(R v) {
+  return v;
+}
+template <class R>
+__device__ __forceinline__ R first(P2<R> v) {
+  return v.a;
+}
+// the cell (or the pair's first cell) is in column 1
+__device__ __forceinline__ bool col1(int k) { return k == 1; }
+__device__ __forceinline__ bool col1(XP x) { return x.k == 1; }
+// east neighbours: k+2 <= nx+1
+__device__ __forceinline__ int east(int k) { return k + 1; }
+__device__ __forceinline__ XG east(XP x) { return XG{x.k + 1, x.k + 2}; }
 // column nx+1 read as column 1: k+1 <= nx
 __device__ __forceinline__ int edge1(int k, int n) { return k == n + 1 ? 1 : k; }
 __device__ __forceinline__ XP edge1(XP x, int) { return x; }
@@ -360,13 +375,15 @@ __device__ __forceinline__ double v_fall_liquid(const CtxT<R> &c, const StateDev
 // radius of the cloud-water droplets whose sink velocity the generalized-density solver uses
 constexpr double CLOUD_WATER_RADIUS = 12.e-6;
 
-// inner_product of the wind with itself, mo_inner_product.F90:16-50
-template <class R>
-__device__ __forceinline__ R ke_point(const CtxT<R> &c, const StateDev &s, int ji, int jk, int jl) {
-  const R u1 = ATR(s.wind_u, ji, jk + 1, jl), v0 = ATR(s.wind_v, ji, jk, jl), u0 = ATR(s.wind_u, ji, jk, jl);
-  const R v1 = ATR(s.wind_v, ji + 1, jk, jl), w0 = ATR(s.wind_w, ji, jk, jl), w1 = ATR(s.wind_w, ji, jk, jl + 1);
-  return AT(c.g.ipw[0], ji, jk, jl) * u1 * u1 + AT(c.g.ipw[1], ji, jk, jl) * v0 * v0 + AT(c.g.ipw[2], ji, jk, jl) * u0 * u0 +
-         AT(c.g.ipw[3], ji, jk, jl) * v1 * v1 + AT(c.g.ipw[4], ji, jk, jl) * w0 * w0 + AT(c.g.ipw[5], ji, jk, jl) * w1 * w1;
+// inner_product of the wind with itself, mo_inner_product.F90:16-50, at one cell or a column pair
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> ke_point(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  using V = VT<R, X>;
+  const auto ke = east(jk);
+  const V u1 = LDGR(s.wind_u, ji, ke, jl), v0 = LDGR(s.wind_v, ji, jk, jl), u0 = LDGR(s.wind_u, ji, jk, jl);
+  const V v1 = LDGR(s.wind_v, ji + 1, jk, jl), w0 = LDGR(s.wind_w, ji, jk, jl), w1 = LDGR(s.wind_w, ji, jk, jl + 1);
+  return LDG(c.g.ipw[0], ji, jk, jl) * u1 * u1 + LDG(c.g.ipw[1], ji, jk, jl) * v0 * v0 + LDG(c.g.ipw[2], ji, jk, jl) * u0 * u0 +
+         LDG(c.g.ipw[3], ji, jk, jl) * v1 * v1 + LDG(c.g.ipw[4], ji, jk, jl) * w0 * w0 + LDG(c.g.ipw[5], ji, jk, jl) * w1 * w1;
 }
 
 // ---- thread -> grid point maps of the stencil kernels ---------------------------------------

@@ -11,9 +11,9 @@ template <class R> __global__ void k_theta_fill(const CtxT<R> c, const StateDev 
 template <class R> __global__ void k_theta_grad(const CtxT<R> c, int nfield);
 template <class R> __global__ void k_theta_div(const CtxT<R> c, int nfield);
 template <class R> __global__ void k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield);
-template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s);
-template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s);  // nx even: two cells per thread
-template <class R> __global__ void k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke);
+// the potential vorticity, the mass fluxes and (with_ke) the kinetic energy
+template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s, int with_ke);
+template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s, int with_ke);  // nx even: two cells per thread
 template <class R> __global__ void k_vort_flux(const CtxT<R> c, int with_ke_grad);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);

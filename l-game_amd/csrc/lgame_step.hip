@@ -384,15 +384,64 @@ __device__ __forceinline__ PotVort<R, X> pot_vort_at(const CtxT<R> &c, const Sta
   }
   return o;
 }
+// mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80), and the
+// kinetic energy (inner_product, mo_inner_product.F90:16-50) of cell (ji, jk, jl), or of the two
+// cells of a column pair, formed from the same density and wind values as the potential
+// vorticity
+template <class R, class X>
+struct MassFlux {
+  VT<R, X> fu, fv, fw, ke;
+};
+template <class R, class X>
+__device__ __forceinline__ MassFlux<R, X> mass_flux_at(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl, int with_ke) {
+  using V = VT<R, X>;
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
+  MassFlux<R, X> o;
+  o.ke = R(0.0);
+  if (with_ke && jl <= nl) o.ke = ke_point(c, s, ji, jk, jl);
+  const auto km = wrapm(jk, nx);
+  const int jm = wrapm(ji, ny);
+  o.fu = o.fv = o.fw = R(0.0);
+  if (jl <= nl) {
+    o.fu = R(0.5) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl)) * LDGR(s.wind_u, ji, jk, jl);
+    o.fv = R(0.5) * (rhog(c, s, jm, jk, jl) + rhog(c, s, ji, jk, jl)) * LDGR(s.wind_v, ji, jk, jl);
+  }
+  if (jl >= 2 && jl <= nl) {
+    o.fw = R(0.5) * (rhog(c, s, ji, jk, jl - 1) + rhog(c, s, ji, jk, jl)) * LDGR(s.wind_w, ji, jk, jl);
+  } else if (jl == nlev && c.C.lfreeslip) {
+    const V dv = rhog(c, s, ji, jk, nl) + (rhog(c, s, ji, jk, nl - 1) - rhog(c, s, ji, jk, nl)) /
+                                              (LDG(c.g.z_scalar, ji, jk, nl - 1) - LDG(c.g.z_scalar, ji, jk, nl)) *
+                                              (LDG(c.g.z_w, ji, jk, nlev) - LDG(c.g.z_scalar, ji, jk, nl));
+    o.fw = dv * LDGR(s.wind_w, ji, jk, nlev);
+  }
+  return o;
+}
+// The u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1.
+template <class R, class X>
+__device__ __forceinline__ void store_mass_flux(const CtxT<R> &c, int ji, X jk, int jl, int with_ke, const MassFlux<R, X> &m) {
+  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
+  if (jl <= nl) {
+    if (with_ke) stv(c.w.v_squared, c.G, ji, jk, jl, m.ke);
+    stv(c.w.u_ph, c.G, ji, jk, jl, m.fu);
+    stv(c.w.v_ph, c.G, ji, jk, jl, m.fv);
+    if (col1(jk)) AT(c.w.u_ph, ji, nx + 1, jl) = first(m.fu);
+    if (ji == 1) stv(c.w.v_ph, c.G, ny + 1, jk, jl, m.fv);
+  }
+  stv(c.w.w_ph, c.G, ji, jk, jl, m.fw);
+}
+
 // The values of row ny+1 and column nx+1 are copies of row 1 and column 1 (the reference
 // evaluates them with the indices of row 1 and column 1): the threads of row 1 and column 1
 // store them, and the launch covers the nx by ny cells.
+// The same kernels form the mass fluxes and (with_ke) the kinetic energy of the cells.
 template <class R>
-__device__ __forceinline__ void b_pot_vort(const CtxT<R> &c, const StateDev &s, const Bid bid) {
+__device__ __forceinline__ void b_pot_vort(const CtxT<R> &c, const StateDev &s, int with_ke, const Bid bid) {
   KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   if (jk > nx || ji > ny || jl > nlev) return;
   const PotVort<R, int> o = pot_vort_at(c, s, ji, jk, jl);
+  const MassFlux<R, int> m = mass_flux_at(c, s, ji, jk, jl, with_ke);
+  store_mass_flux(c, ji, jk, jl, with_ke, m);
   AT(c.w.eta_x, ji, jk, jl) = o.ex;
   AT(c.w.eta_y, ji, jk, jl) = o.ey;
   if (jl <= nl) AT(c.w.eta_z, ji, jk, jl) = o.ez;
@@ -407,12 +456,12 @@ __device__ __forceinline__ void b_pot_vort(const CtxT<R> &c, const StateDev &s, 
   if (ji == 1 && jk == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = o.ez;
 }
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s) {
-  b_pot_vort(c, s, bid_here());
+__global__ void __launch_bounds__(BX * BY) k_pot_vort(const CtxT<R> c, const StateDev s, int with_ke) {
+  b_pot_vort(c, s, with_ke, bid_here());
 }
 // The same with two x-adjacent cells per thread (nx even): the launch covers nx/2 by ny pairs.
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_pot_vort2(const CtxT<R> c, const StateDev s) {
+__global__ void __launch_bounds__(BX * BY) k_pot_vort2(const CtxT<R> c, const StateDev s, int with_ke) {
   const Bid bid = bid_here();
   KIDX_ZFAST(c.G.nx / 2, c.G.ny, c.G.nlev);
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
@@ -421,6 +470,8 @@ __global__ void __launch_bounds__(BX * BY) k_pot_vort2(const CtxT<R> c, const St
   __builtin_assume(k >= 1 && k < nx);
   const XP x{k};
   const PotVort<R, XP> o = pot_vort_at(c, s, ji, x, jl);
+  const MassFlux<R, XP> m = mass_flux_at(c, s, ji, x, jl, with_ke);
+  store_mass_flux(c, ji, x, jl, with_ke, m);
   stv(c.w.eta_x, c.G, ji, x, jl, o.ex);
   stv(c.w.eta_y, c.G, ji, x, jl, o.ey);
   if (jl <= nl) stv(c.w.eta_z, c.G, ji, x, jl, o.ez);
@@ -433,44 +484,6 @@ __global__ void __launch_bounds__(BX * BY) k_pot_vort2(const CtxT<R> c, const St
     if (jl <= nl) AT(c.w.eta_z, ji, nx + 1, jl) = o.ez.a;
   }
   if (ji == 1 && k == 1 && jl <= nl) AT(c.w.eta_z, ny + 1, nx + 1, jl) = o.ez.a;
-}
-
-// mass flux densities of the gas, scalar_times_vector_h/v (mo_vector_tend_expl.F90:64-80).  The
-// u points of column nx+1 and the v points of row ny+1 are copies of column 1 and row 1; the
-// threads of column 1 and row 1 store them, and the launch covers the nx by ny cells.
-template <class R>
-__device__ __forceinline__ void b_mass_flux(const CtxT<R> &c, const StateDev &s, int with_ke, const Bid bid) {
-  KIDX;
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  if (jk > nx || ji > ny || jl > nlev) return;
-  // the kinetic energy of the cell (inner_product, mo_inner_product.F90:16-50) from the same wind
-  // values, before any store
-  if (with_ke && jl <= nl) AT(c.w.v_squared, ji, jk, jl) = ke_point(c, s, ji, jk, jl);
-  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
-  R fu = R(0.0), fv = R(0.0), fw = R(0.0);
-  if (jl <= nl) {
-    fu = R(0.5) * (rhog(c, s, ji, km, jl) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_u, ji, jk, jl);
-    fv = R(0.5) * (rhog(c, s, jm, jk, jl) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_v, ji, jk, jl);
-  }
-  if (jl >= 2 && jl <= nl) {
-    fw = R(0.5) * (rhog(c, s, ji, jk, jl - 1) + rhog(c, s, ji, jk, jl)) * ATR(s.wind_w, ji, jk, jl);
-  } else if (jl == nlev && c.C.lfreeslip) {
-    const R dv = rhog(c, s, ji, jk, nl) + (rhog(c, s, ji, jk, nl - 1) - rhog(c, s, ji, jk, nl)) /
-                                              (AT(c.g.z_scalar, ji, jk, nl - 1) - AT(c.g.z_scalar, ji, jk, nl)) *
-                                              (AT(c.g.z_w, ji, jk, nlev) - AT(c.g.z_scalar, ji, jk, nl));
-    fw = dv * ATR(s.wind_w, ji, jk, nlev);
-  }
-  if (jl <= nl) {
-    AT(c.w.u_ph, ji, jk, jl) = fu;
-    AT(c.w.v_ph, ji, jk, jl) = fv;
-    if (jk == 1) AT(c.w.u_ph, ji, nx + 1, jl) = fu;
-    if (ji == 1) AT(c.w.v_ph, ny + 1, jk, jl) = fv;
-  }
-  AT(c.w.w_ph, ji, jk, jl) = fw;
-}
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_mass_flux(const CtxT<R> c, const StateDev s, int with_ke) {
-  b_mass_flux(c, s, with_ke, bid_here());
 }
 
 // The vorticity flux term at the u, v and w points of one cell (ji <= ny, jk <= nx): every eta,
@@ -830,9 +843,8 @@ __global__ void __launch_bounds__(BX * BY) k_vort_flux_vt(const CtxT<R> c, const
   template __global__ void k_theta_grad<R>(const CtxT<R>, int);                                          \
   template __global__ void k_theta_div<R>(const CtxT<R>, int);                                           \
   template __global__ void k_theta_edge<R>(const CtxT<R>, Fld<const double>, Fld<const double>, int);   \
-  template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev);                                 \
-  template __global__ void k_pot_vort2<R>(const CtxT<R>, const StateDev);                                \
-  template __global__ void k_mass_flux<R>(const CtxT<R>, const StateDev, int);                           \
+  template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev, int);                            \
+  template __global__ void k_pot_vort2<R>(const CtxT<R>, const StateDev, int);                           \
   template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
   template __global__ void k_dissipation<R>(const CtxT<R>, const StateDev);                              \
   template __global__ void k_vec_tend<R>(const CtxT<R>, const StateDev, const StateDev, int);                 \
@@ -852,8 +864,7 @@ LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
                 b_theta_edge(d.c, Fld<const double>(d.s[a.state_slot].wind_u), Fld<const double>(d.s[a.state_slot].wind_v),
                              a.nfield, bid))
 LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad_vt(d.c, d.s[a.old_slot], d.s[a.new_slot], a.lfirst, bid))
-LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], bid))
-LG_BATCH_KERNEL(mass_flux, BX * BY, PH_MASS_FLUX, b_mass_flux(d.c, d.s[a.state_slot], a.ke, bid))
+LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], a.ke, bid))
 LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, a.ke, bid))
 LG_BATCH_KERNEL(vort_flux_vt, BX * BY, PH_VORT_FLUX_VT, b_vort_flux_vt(d.c, d.s[a.old_slot], d.s[a.new_slot], a.ke, bid))
 LG_BATCH_KERNEL(dissipation, BX * BY, PH_DISSIPATION, b_dissipation(d.c, d.s[a.old_slot], bid))
@@ -873,7 +884,6 @@ bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R
     LG_B(PH_THETA_EDGE, kb_theta_edge)
     LG_B(PH_PGRAD, kb_pgrad)
     LG_B(PH_POT_VORT, kb_pot_vort)
-    LG_B(PH_MASS_FLUX, kb_mass_flux)
     LG_B(PH_VORT_FLUX, kb_vort_flux)
     LG_B(PH_VORT_FLUX_VT, kb_vort_flux_vt)
     LG_B(PH_DISSIPATION, kb_dissipation)
