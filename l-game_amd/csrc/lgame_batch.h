@@ -145,10 +145,10 @@ LG_BATCH_LAUNCHER(batch_launch_step)
 LG_BATCH_LAUNCHER(batch_launch_diff)
 LG_BATCH_LAUNCHER(batch_launch_solve)
 #undef LG_BATCH_LAUNCHER
-// the scalar kernel, with its moist and 3rd-order variants chosen by tr and o3
+// the scalar kernel, with its moist, 3rd-order and diffusion-input variants chosen by tr, o3 and diff
 template <class R>
 void batch_launch_scalar(dim3 g, dim3 b, hipStream_t st, const MemberDev<R> *M, const int *prefix, int nm, const BatchArgs &a,
-                         bool tr, bool o3);
+                         bool tr, bool o3, bool diff);
 
 // the radiation phases (all on the PH_RAD grid)
 enum RadPhase : int { RPH_SOLVER, RPH_GATHER, RPH_APPLY, RPH_SELECT };

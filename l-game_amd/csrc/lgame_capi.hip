@@ -1685,7 +1685,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
     else
       blk = dim3(BX, g.tiles.by[phase_tile(ph)]);
     if (ph == PH_SCALAR) {
-      batch_launch_scalar<R>(grd, blk, st, M, pf, n, a, g.moist, g.o3);
+      batch_launch_scalar<R>(grd, blk, st, M, pf, n, a, g.moist, g.o3, C.ltemp_h || C.lmass_h);
       return;
     }
     if (!batch_launch_step<R>(ph, grd, blk, st, M, pf, n, a) && !batch_launch_diff<R>(ph, grd, blk, st, M, pf, n, a))
