@@ -88,10 +88,10 @@ STEP_BYTES = {"dry": 920.0, "moist": 1296.0}  # per grid-point step, whole-step 
 SOLVER_TRAFFIC = {("moist512", "fp64"): 4.369e+09, ("moist512", "mixed"): 4.128e+09}
 # the same for the kernels of KERNEL_BYTES, (workload, kernel, form) -> bytes per launch
 KERNEL_TRAFFIC = {("moist512", "solve_ver", "fp64"): 4.369e+09, ("moist512", "solve_ver", "mixed"): 4.128e+09,
-                  ("moist512", "scalar_div", "fp64"): 5.710e+09, ("moist512", "scalar_div", "mixed"): 4.265e+09,
+                  ("moist512", "scalar_div", "fp64"): 5.932e+09, ("moist512", "scalar_div", "mixed"): 4.264e+09,
                   ("moist512", "solve_gen", "fp64"): 5.652e+09, ("moist512", "solve_gen", "mixed"): 5.263e+09}
 # measured HBM bytes per grid-point step of the whole FP64 step (same counters, all kernels)
-STEP_TRAFFIC = {"moist512": 3458.0}
+STEP_TRAFFIC = {"moist512": 3489.0}
 # the reference's own CPU path, flang -O2 -fopenmp on 8 cores (the survey's measurement; the
 # reference tree does not travel, so it cannot be timed beside the GPU)
 REFERENCE_CPU = {"dry128": {"value": 3.19e6, "grid": "129x129x50 dry", "cores": 8},
