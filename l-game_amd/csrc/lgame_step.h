@@ -17,7 +17,7 @@ template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s
 template <class R> __global__ void k_vort_flux(const CtxT<R> c, int with_ke_grad);
 template <class R> __global__ void k_dissipation(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_vec_tend(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
-// the first substep's pressure gradient (manage_p_grad) with its wind update (k_vec_tend at rk 1)
+// the first substep's pressure gradient (manage_p_grad) with its wind update (the work of k_vec_tend)
 template <class R> __global__ void k_pgrad_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int lfirst);
 // the second substep's vorticity flux term (and kinetic-energy gradient) with its wind update
 template <class R> __global__ void k_vort_flux_vt(const CtxT<R> c, const StateDev old, const StateDev nw, int with_ke_grad);
@@ -38,7 +38,10 @@ template <class R> __global__ void k_surface(const CtxT<R> c, const StateDev old
 constexpr int BX = 64;
 constexpr int BY = 4;
 
-// stencil kernels of the step, the index of their tile-table entry
+// stencil kernels of the step, the index of their tile-table entry.  The mass fluxes and the
+// kinetic energy are formed in the potential-vorticity kernel, the kinetic-energy gradient in the
+// vorticity-flux kernel and the scalar fluxes in the scalar column kernel: KI_MASS_FLUX, KI_KE,
+// KI_KE_GRAD and KI_SCALAR_FLUX keep their place in the table (and the C-ABI's order) unused.
 enum KernelId {
   KI_TEMPERATURE, KI_THETA_FILL, KI_THETA_GRAD, KI_THETA_DIV, KI_THETA_EDGE, KI_PGRAD, KI_POT_VORT, KI_MASS_FLUX,
   KI_VORT_FLUX, KI_KE, KI_KE_GRAD, KI_DISSIPATION, KI_VEC_TEND, KI_SCALAR_FLUX, KI_SCALAR_DIV, KI_DIFF_ZETA,
