@@ -85,8 +85,9 @@ __host__ __device__ inline void phase_grid(int ph, const Geom &G, int by, int se
     case PH_POT_VORT: ez = nlev; flat = true; break;
     case PH_DIFF_ZETA: ex = nx + 1; ey = ny + 1; flat = true; break;
     case PH_SCALAR: gx = ((nx + SCOL_CELLS - 1) / SCOL_CELLS) * ((ny + by - 1) / by) * seg; gy = gz = 1; return;
-    case PH_SOLVE_VER: case PH_RAD: gx = (nx * ny + BCOL - 1) / BCOL; gy = gz = 1; return;
-    case PH_SOLVE_GEN: gx = (nx * ny + BCOL * GEN_GROUPS - 1) / (BCOL * GEN_GROUPS); gy = gz = 1; return;
+    case PH_SOLVE_VER: gx = col_waves(G); gy = gz = 1; return;
+    case PH_RAD: gx = (nx * ny + BCOL - 1) / BCOL; gy = gz = 1; return;
+    case PH_SOLVE_GEN: gx = (col_waves(G) + GEN_GROUPS - 1) / GEN_GROUPS; gy = gz = 1; return;
     default: return;
   }
   gx = (ex + BX - 1) / BX;

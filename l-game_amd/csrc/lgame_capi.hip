@@ -287,7 +287,7 @@ template <class R>
 int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
                     const RadCall *rad, const Tiles &tiles) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  const dim3 g_col((nx * ny + BCOL - 1) / BCOL), b_col(BCOL);
+  const dim3 g_col((unsigned)col_waves(c.G)), b_col(BCOL);
   const bool o3 = c.C.order == 3;
   const bool pk = prof && prof->kernels;
   // a stencil kernel on a (64 x by) block grid covering (ex, ey, ez) points, by from the tile table
@@ -391,7 +391,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     if (prof) prof->end(0);
     if (c.G.nc > 1) {
       if (prof) prof->begin(1);
-      const dim3 g_gen((nx * ny + BCOL * GEN_GROUPS - 1) / (BCOL * GEN_GROUPS));
+      const dim3 g_gen((unsigned)((col_waves(c.G) + GEN_GROUPS - 1) / GEN_GROUPS));
       hipLaunchKernelGGL(k_solve_gen<R>, g_gen, dim3(BCOL, NTR * GEN_GROUPS), 0, st, c, old, nw, rk);
       if (prof) prof->end(1);
     }

@@ -270,10 +270,9 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
 
 template <class R>
 __device__ __forceinline__ void b_solve_ver(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, const Bid bid) {
-  const int col = bid.x * blockDim.x + threadIdx.x;
-  const int nx = c.G.nx, ny = c.G.ny;
-  if (col >= nx * ny) return;
-  const int jk = col % nx + 1, ji = col / nx + 1;
+  const int nx = c.G.nx, wpr = col_waves_per_row(nx);
+  const int ji = bid.x / wpr + 1, jk = (bid.x % wpr) * BCOL + (int)threadIdx.x + 1;
+  if (ji > c.G.ny || jk > nx) return;
   if (rk == 1)
     ver_column<R, 1>(c, old, nw, ji, jk);
   else
@@ -637,10 +636,10 @@ __device__ __forceinline__ void b_solve_gen(const CtxT<R> &c, const StateDev &ol
   // the compiler so keeps the tracer selection in scalar registers and scalar branches
   const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int t = wy % NTR, grp = wy / NTR;
-  const int col_raw = (bid.x * GEN_GROUPS + grp) * BCOL + lane;
-  const bool active = col_raw < nx * ny;
-  const int col = active ? col_raw : nx * ny - 1;  // idle lanes mirror a valid column, store nothing
-  const int jk = col % nx + 1, ji = col / nx + 1;
+  const int g = bid.x * GEN_GROUPS + grp, wpr = col_waves_per_row(nx);  // the group: wavefront g % wpr of row g / wpr
+  const int ji_raw = g / wpr + 1, jk_raw = (g % wpr) * BCOL + lane + 1;
+  const bool active = ji_raw <= ny && jk_raw <= nx;
+  const int ji = active ? ji_raw : ny, jk = active ? jk_raw : nx;  // idle lanes mirror a valid column, store nothing
   if (rk == 1)  // the sediment heating is diagnosed at the predictor only
     gen_dispatch<R, true>(c, old, nw, sh, t, grp, lane, active, ji, jk);
   else

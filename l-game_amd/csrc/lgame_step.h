@@ -73,6 +73,11 @@ template <class R>
 void launch_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw, int rk, int rows, int nseg, hipStream_t st);
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
+// The column solvers give each wavefront 64 consecutive columns of one row (the row's last
+// wavefront partly idle), so that a wavefront's loads of a level cover whole 128-byte lines of
+// the row rather than the end of one row and the start of the next.
+__host__ __device__ inline int col_waves_per_row(int nx) { return (nx + BCOL - 1) / BCOL; }
+__host__ __device__ inline int col_waves(const Geom &G) { return col_waves_per_row(G.nx) * G.ny; }
 // The tracer solver: one wavefront per tracer of a 64-column group, two groups per block. Twelve
 // wavefronts per block land three on each SIMD, which is what the kernel's registers allow; a
 // block of six would land 2,2,1,1 and leave room for only one block per CU.
