@@ -26,8 +26,8 @@ __device__ __forceinline__ int wrapm(int i, int n) { return i == 1 ? n : i - 1; 
 // A stencil can be evaluated at the columns k and k+1 of one row at once (k odd, k+1 <= nx).  Its
 // values are then pairs: the loads of the own columns are single 2-element vector loads (aligned,
 // since k-1 is even and rows, levels and fields start at multiples of 16 elements), and the west
-// neighbours of the pair are the columns (west of k, k), whose second element the compiler takes
-// from the own pair.  Every operation on a pair is the same operation on each element, so each
+// neighbours of the pair are the columns (west of k, k), the east ones (k+1, k+2), each loaded per
+// element.  Every operation on a pair is the same operation on each element, so each
 // element is bitwise the value of the single-point evaluation.  The helpers below take the x index
 // as a template parameter X: int (one point), XP (the columns k, k+1) or XG (one column per
 // element); VT<R, X> is the matching value type.
