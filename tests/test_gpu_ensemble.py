@@ -68,6 +68,9 @@ def test_scalar_kernel_segments_and_rows_keep_results():
 def test_ensemble_members_match_oracle(mode):
     cfgs = [tiny_config(ny=5, nx=7, n_layers=6, moist=True, order=3),
             tiny_config(ny=9, nx=11, n_layers=8),
+            # even nx: two cells per thread in the potential vorticity of a single-domain launch,
+            # one cell per thread in the batched one; nx not a multiple of 64 in the column solvers
+            tiny_config(ny=6, nx=70, n_layers=7, moist=True),
             dry3d_config(n=17, n_layers=10),
             # a member with the reference's default physics: diffusion, PBL, prognostic soil
             dataclasses.replace(tiny_config(ny=7, nx=9, n_layers=8, moist=True), lpbl=True, lprog_soil_temp=True,
