@@ -352,7 +352,10 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
       LG_LAUNCH_ZFAST(KI_DIFF_ZETA, k_diff_zeta, nx + 1, ny + 1, nl, c, old);
-      LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
+      if (nx % 2 == 0)
+        LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef2, nx / 2, ny, nl, c, old);
+      else
+        LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
       LG_LAUNCH(KI_DIFF_DUAL, k_diff_dual, nx, ny, nl, c, old);
       LG_LAUNCH_ZFAST(KI_DIFF_MOM, k_diff_mom, nx, ny, nl, c, old);
     }

@@ -19,34 +19,60 @@
 
 namespace lgame {
 
-// calc_diff_coeff, mo_derived.F90:189-216: molecular kinematic viscosity of air
+// square root and per-element selects of a value or a pair (each element as the one-point form)
 template <class R>
-__device__ __forceinline__ R calc_diff_coeff(R temperature, R density) {
+__device__ __forceinline__ R vsqrt(R x) {
+  return sqrt(x);
+}
+template <class R>
+__device__ __forceinline__ P2<R> vsqrt(P2<R> x) {
+  return P2<R>(sqrt(x.a), sqrt(x.b));
+}
+template <class F, class R>
+__device__ __forceinline__ R vmap(F f, R x) {
+  return f(x);
+}
+template <class F, class R>
+__device__ __forceinline__ P2<R> vmap(F f, P2<R> x) {
+  return P2<R>(f(x.a), f(x.b));
+}
+template <class F, class R>
+__device__ __forceinline__ R vmap(F f, R x, R y) {
+  return f(x, y);
+}
+template <class F, class R>
+__device__ __forceinline__ P2<R> vmap(F f, P2<R> x, P2<R> y) {
+  return P2<R>(f(x.a, y.a), f(x.b, y.b));
+}
+
+// calc_diff_coeff, mo_derived.F90:189-216: molecular kinematic viscosity of air
+template <class R, class V>
+__device__ __forceinline__ V calc_diff_coeff(V temperature, V density) {
   const R particle_radius = R(130e-12);
   const R particle_mass = R(M_D / N_A);
-  const R thermal_velocity = sqrt(R(8.0) * R(K_B) * temperature / (R(MPI) * particle_mass));
-  const R particle_density = density / particle_mass;
+  const V thermal_velocity = vsqrt(R(8.0) * R(K_B) * temperature / (R(MPI) * particle_mass));
+  const V particle_density = density / particle_mass;
   const R cross_section = R(4.0) * R(MPI) * (particle_radius * particle_radius);
-  const R mean_free_path = R(1.0) / (sqrt(R(2.0)) * particle_density * cross_section);
+  const V mean_free_path = R(1.0) / (sqrt(R(2.0)) * particle_density * cross_section);
   return R(1.0 / 3.0) * thermal_velocity * mean_free_path;
 }
 
 // tke2hor_diff_coeff, mo_eff_diff_coeffs.F90:668-683
-template <class R>
-__device__ __forceinline__ R tke2hor(const CtxT<R> &c, R tke) {
-  const R mean_velocity = sqrt(R(2.0) * tke);
+template <class R, class V>
+__device__ __forceinline__ V tke2hor(const CtxT<R> &c, V tke) {
+  const V mean_velocity = vsqrt(R(2.0) * tke);
   const R mean_free_path = R(c.C.eff_hor_res) / R(6.0);
   return R(1.0 / 6.0) * mean_free_path * mean_velocity;
 }
 
 // tke2vert_diff_coeff, mo_eff_diff_coeffs.F90:685-718
-template <class R>
-__device__ __forceinline__ R tke2vert(R tke, R n_squared, R layer_thickness) {
-  const R tke_vert = R(3.0 * 1e-3) * tke;
-  const R mean_velocity = sqrt(R(2.0) * tke_vert);
-  const R n_used = sqrt(n_squared > R(1e-4) ? n_squared : R(1e-4));
-  R mean_free_path = sqrt(R(2.0) * tke_vert) / n_used;
-  mean_free_path = layer_thickness < mean_free_path ? layer_thickness : mean_free_path;
+template <class R, class V>
+__device__ __forceinline__ V tke2vert(V tke, V n_squared, V layer_thickness) {
+  const V tke_vert = R(3.0 * 1e-3) * tke;
+  const V mean_velocity = vsqrt(R(2.0) * tke_vert);
+  const V n_used = vsqrt(vmap([](R n) { return n > R(1e-4) ? n : R(1e-4); }, n_squared));
+  V mean_free_path = vsqrt(R(2.0) * tke_vert) / n_used;
+  mean_free_path = vmap([](R l, R m) { return l < m ? l : m; }, layer_thickness, mean_free_path);
   return R(1.0 / 6.0) * mean_free_path * mean_velocity;
 }
 
@@ -77,67 +103,87 @@ __global__ void __launch_bounds__(BX * BY) k_diff_zeta(const CtxT<R> c, const St
 // ---------------------------------------------------------------------------------------------
 // gravity_m_v * (d theta_v/dz) / theta_v at level L (2 <= L <= n_layers), the w_placeholder of
 // update_n_squared (mo_eff_diff_coeffs.F90:636-648); th_a, th_b are theta_v of layers L-1, L.
-template <class R>
-__device__ __forceinline__ R n2_level(const CtxT<R> &c, int ji, int jk, int L, R th_a, R th_b) {
-  const R w = (th_a - th_b) / AT(c.g.dz, ji, jk, L);
-  const R inv_a = R(1.0) / th_a, inv_b = R(1.0) / th_b;
-  return AT(c.g.gravity_m_v, ji, jk, L) * (R(0.5) * (inv_a + inv_b) * w);
+template <class R, class X>
+__device__ __forceinline__ VT<R, X> n2_level(const CtxT<R> &c, int ji, X jk, int L, VT<R, X> th_a, VT<R, X> th_b) {
+  const VT<R, X> w = (th_a - th_b) / LDG(c.g.dz, ji, jk, L);
+  const VT<R, X> inv_a = R(1.0) / th_a, inv_b = R(1.0) / th_b;
+  return LDG(c.g.gravity_m_v, ji, jk, L) * (R(0.5) * (inv_a + inv_b) * w);
 }
 
-template <class R>
-__device__ __forceinline__ void b_diff_coef(const CtxT<R> &c, const StateDev &s, const Bid bid) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
-  const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl;
-  if (jk > nx || ji > ny || jl > nl) return;
-  const Geom &G = c.G;
-  const ix_t i = LG_IX(G, ji, jk, jl);
-  const R rg = R(s.rho[G.ncond][i]);
+// the coefficients of one cell (jk an int) or of the columns k, k+1 (jk an XP)
+template <class R, class X>
+__device__ __forceinline__ void diff_coef_at(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  using V = VT<R, X>;
+  const int nl = c.G.nl;
+  const auto ke = east(jk);
+  const V rg = LDGR(s.rho[c.G.ncond], ji, jk, jl);
   // N^2, update_n_squared (mo_eff_diff_coeffs.F90:616-666).  Its top and bottom layers take levels
   // 1 and n_levels of w_placeholder, which grad_vert does not write and which hold zero there.
-  R n2 = R(0.0);
+  V n2 = R(0.0);
   if (jl > 1 && jl < nl) {
-    const R th_m = AT(c.g.theta_v_bg, ji, jk, jl - 1) + ATR(s.theta_v_pert, ji, jk, jl - 1);
-    const R th_c = c.g.theta_v_bg[i] + R(s.theta_v_pert[i]);
-    const R th_p = AT(c.g.theta_v_bg, ji, jk, jl + 1) + ATR(s.theta_v_pert, ji, jk, jl + 1);
-    n2 = AT(c.g.ipw[4], ji, jk, jl) * n2_level(c, ji, jk, jl, th_m, th_c) +
-         AT(c.g.ipw[5], ji, jk, jl) * n2_level(c, ji, jk, jl + 1, th_c, th_p);
+    const V th_m = LDG(c.g.theta_v_bg, ji, jk, jl - 1) + LDGR(s.theta_v_pert, ji, jk, jl - 1);
+    const V th_c = LDG(c.g.theta_v_bg, ji, jk, jl) + LDGR(s.theta_v_pert, ji, jk, jl);
+    const V th_p = LDG(c.g.theta_v_bg, ji, jk, jl + 1) + LDGR(s.theta_v_pert, ji, jk, jl + 1);
+    n2 = LDG(c.g.ipw[4], ji, jk, jl) * n2_level(c, ji, jk, jl, th_m, th_c) +
+         LDG(c.g.ipw[5], ji, jk, jl) * n2_level(c, ji, jk, jl + 1, th_c, th_p);
   }
   // tke_update (mo_tke.F90:19-67): advection by the inner product of the wind with grad(tke),
   // production by the dissipation of the previous step, decay with the viscosity of the previous
   // step, clipping at zero
   const Fld<const R> tk = c.w.tke;
-  const R gx0 = grad_hor_x(c, tk, ji, jk, jl), gx1 = grad_hor_x(c, tk, ji, jk + 1, jl);
-  const R gy0 = grad_hor_y(c, tk, ji, jk, jl), gy1 = grad_hor_y(c, tk, ji + 1, jk, jl);
+  const V gx0 = grad_hor_x(c, tk, ji, jk, jl), gx1 = grad_hor_x(c, tk, ji, ke, jl);
+  const V gy0 = grad_hor_y(c, tk, ji, jk, jl), gy1 = grad_hor_y(c, tk, ji + 1, jk, jl);
   const VGrad<R, Fld<const R>> vg{tk};
-  const R gz0 = vg(c, ji, jk, jl), gz1 = vg(c, ji, jk, jl + 1);
-  const R adv = AT(c.g.ipw[0], ji, jk, jl) * ATR(s.wind_u, ji, jk + 1, jl) * gx1 + c.g.ipw[1][i] * R(s.wind_v[i]) * gy0 +
-                c.g.ipw[2][i] * R(s.wind_u[i]) * gx0 + AT(c.g.ipw[3], ji, jk, jl) * ATR(s.wind_v, ji + 1, jk, jl) * gy1 +
-                c.g.ipw[4][i] * R(s.wind_w[i]) * gz0 + c.g.ipw[5][i] * ATR(s.wind_w, ji, jk, jl + 1) * gz1;
-  const R t_old = tk[i];
-  const R decay = R(8.0 * (MPI * MPI)) / R(c.C.mva) * c.w.visc[i] / rg;
-  R t = t_old + R(c.C.dtime) * (-adv + c.w.heating_diss[i] / rg - decay * t_old);
-  t = t < R(0.0) ? R(0.0) : t;
+  const V gz0 = vg(c, ji, jk, jl), gz1 = vg(c, ji, jk, jl + 1);
+  const V adv = LDG(c.g.ipw[0], ji, jk, jl) * LDGR(s.wind_u, ji, ke, jl) * gx1 +
+                LDG(c.g.ipw[1], ji, jk, jl) * LDGR(s.wind_v, ji, jk, jl) * gy0 +
+                LDG(c.g.ipw[2], ji, jk, jl) * LDGR(s.wind_u, ji, jk, jl) * gx0 +
+                LDG(c.g.ipw[3], ji, jk, jl) * LDGR(s.wind_v, ji + 1, jk, jl) * gy1 +
+                LDG(c.g.ipw[4], ji, jk, jl) * LDGR(s.wind_w, ji, jk, jl) * gz0 +
+                LDG(c.g.ipw[5], ji, jk, jl) * LDGR(s.wind_w, ji, jk, jl + 1) * gz1;
+  const V t_old = LDG(tk, ji, jk, jl);
+  const V decay = R(8.0 * (MPI * MPI)) / R(c.C.mva) * LDG(c.w.visc, ji, jk, jl) / rg;
+  V t = t_old + R(c.C.dtime) * (-adv + LDG(c.w.heating_diss, ji, jk, jl) / rg - decay * t_old);
+  t = vmap([](R x) { return x < R(0.0) ? R(0.0) : x; }, t);
   // hor_viscosity (mo_eff_diff_coeffs.F90:24-159): molecular plus eddy viscosity, times rho
-  const R mol = calc_diff_coeff(c.w.temperature[i], rg);
-  const R div = div_h(c, Fld<const double>(s.wind_u), Fld<const double>(s.wind_v), ji, jk, jl);
-  R eddy;
+  const V mol = calc_diff_coeff<R>(LDG(c.w.temperature, ji, jk, jl), rg);
+  const V div = div_h(c, Fld<const double>(s.wind_u), Fld<const double>(s.wind_v), ji, jk, jl);
+  V eddy;
   if (!c.C.scheme_tke) {
     const Fld<const R> z = c.w.zeta;
-    const R vc = R(0.25) * (AT(z, ji, jk, jl) + AT(z, ji, jk + 1, jl) + AT(z, ji + 1, jk, jl) + AT(z, ji + 1, jk + 1, jl));
-    const R a = vc * vc + div * div, b = R(c.C.bg_shear) * R(c.C.bg_shear);
-    eddy = R(c.C.c_s) * R(c.C.mva) * sqrt(a > b ? a : b);
+    const V vc = R(0.25) * (LDG(z, ji, jk, jl) + LDG(z, ji, ke, jl) + LDG(z, ji + 1, jk, jl) + LDG(z, ji + 1, ke, jl));
+    const V a = vc * vc + div * div;
+    const R b = R(c.C.bg_shear) * R(c.C.bg_shear);
+    eddy = R(c.C.c_s) * R(c.C.mva) * vsqrt(vmap([b](R x) { return x > b ? x : b; }, a));
   } else {
     eddy = tke2hor(c, t);
   }
-  const R visc = rg * (mol + eddy);
-  c.w.tke_next[i] = t;
-  c.w.visc[i] = visc;
-  c.w.kv[i] = mol + tke2vert(t, n2, c.g.layer_thickness[i]);
-  c.w.vdiv[i] = visc * div;
+  const V visc = rg * (mol + eddy);
+  stv(c.w.tke_next, c.G, ji, jk, jl, t);
+  stv(c.w.visc, c.G, ji, jk, jl, visc);
+  stv(c.w.kv, c.G, ji, jk, jl, mol + tke2vert<R>(t, n2, LDG(c.g.layer_thickness, ji, jk, jl)));
+  stv(c.w.vdiv, c.G, ji, jk, jl, visc * div);
+}
+
+template <class R>
+__device__ __forceinline__ void b_diff_coef(const CtxT<R> &c, const StateDev &s, const Bid bid) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  diff_coef_at(c, s, ji, jk, jl);
 }
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_diff_coef(const CtxT<R> c, const StateDev s) {
   b_diff_coef(c, s, bid_here());
+}
+// nx even: the columns k = 2*jk-1 and k+1 per thread
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_coef2(const CtxT<R> c, const StateDev s) {
+  const Bid bid = bid_here();
+  KIDX_ZFAST(c.G.nx / 2, c.G.ny, c.G.nl);
+  const int k = 2 * jk - 1;
+  if (k > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  __builtin_assume(k >= 1 && k < c.G.nx);
+  diff_coef_at(c, s, ji, XP{k}, jl);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -698,6 +744,7 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
 #define LG_INSTANTIATE(R)                                                                  \
   template __global__ void k_diff_zeta<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_coef<R>(const CtxT<R>, const StateDev);                  \
+  template __global__ void k_diff_coef2<R>(const CtxT<R>, const StateDev);                 \
   template __global__ void k_diff_dual<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_mom<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \

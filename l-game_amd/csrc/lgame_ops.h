@@ -149,9 +149,10 @@ __device__ __forceinline__ void stv(Fld<T> p, const Geom &G, int ji, XP x, int j
 template <class R, class P>
 struct VGrad {
   P s;
-  __device__ R operator()(const CtxT<R> &c, int ji, int jk, int jl) const {
+  template <class X>
+  __device__ VT<R, X> operator()(const CtxT<R> &c, int ji, X jk, int jl) const {
     if (jl >= c.G.nlev || jl <= 1) return R(0.0);
-    return (ATR(s, ji, jk, jl - 1) - ATR(s, ji, jk, jl)) / AT(c.g.dz, ji, jk, jl);
+    return (LDGR(s, ji, jk, jl - 1) - LDGR(s, ji, jk, jl)) / LDG(c.g.dz, ji, jk, jl);
   }
 };
 template <class R, class P>
@@ -211,69 +212,71 @@ __device__ VT<R, X> remap_y(const CtxT<R> &c, const VZ &vz, int ji, X jk, int jl
 
 // grad_hor x-component (mo_gradient_operators.F90:17-63,85-130); at jk = nx+1 the covariant part
 // is the copy of jk = 1 and the terrain correction uses the metrics of jk = nx+1.
-template <class R, class P>
-__device__ __forceinline__ R grad_hor_x(const CtxT<R> &c, P s, int ji, int jk, int jl) {
+// (columns 1 and nx+1 take (s(1) - s(nx)) / dx(1): edge1 gives column 1 there, wrapm column nx)
+template <class R, class P, class X>
+__device__ __forceinline__ VT<R, X> grad_hor_x(const CtxT<R> &c, P s, int ji, X jk, int jl) {
   const int nx = c.G.nx;
-  R r;
-  if (jk == 1 || jk == nx + 1)
-    r = (ATR(s, ji, 1, jl) - ATR(s, ji, nx, jl)) / AT(c.g.dx, ji, 1, jl);
-  else
-    r = (ATR(s, ji, jk, jl) - ATR(s, ji, jk - 1, jl)) / AT(c.g.dx, ji, jk, jl);
-  if (jl > c.G.nflat) r = r - AT(c.g.slope_x, ji, jk, jl) * remap_x(c, VGrad<R, P>{s}, ji, jk, jl);
+  const auto ka = edge1(jk, nx);
+  const auto kb = wrapm(jk, nx);
+  VT<R, X> r = (LDGR(s, ji, ka, jl) - LDGR(s, ji, kb, jl)) / LDG(c.g.dx, ji, ka, jl);
+  if (jl > c.G.nflat) r = r - LDG(c.g.slope_x, ji, jk, jl) * remap_x(c, VGrad<R, P>{s}, ji, jk, jl);
   return r;
 }
 
-template <class R, class P>
-__device__ __forceinline__ R grad_hor_y(const CtxT<R> &c, P s, int ji, int jk, int jl) {
+template <class R, class P, class X>
+__device__ __forceinline__ VT<R, X> grad_hor_y(const CtxT<R> &c, P s, int ji, X jk, int jl) {
   const int ny = c.G.ny;
-  R r;
+  VT<R, X> r;
   if (ji == 1 || ji == ny + 1)
-    r = (ATR(s, ny, jk, jl) - ATR(s, 1, jk, jl)) / AT(c.g.dy, 1, jk, jl);
+    r = (LDGR(s, ny, jk, jl) - LDGR(s, 1, jk, jl)) / LDG(c.g.dy, 1, jk, jl);
   else
-    r = (ATR(s, ji - 1, jk, jl) - ATR(s, ji, jk, jl)) / AT(c.g.dy, ji, jk, jl);
-  if (jl > c.G.nflat) r = r - AT(c.g.slope_y, ji, jk, jl) * remap_y(c, VGrad<R, P>{s}, ji, jk, jl);
+    r = (LDGR(s, ji - 1, jk, jl) - LDGR(s, ji, jk, jl)) / LDG(c.g.dy, ji, jk, jl);
+  if (jl > c.G.nflat) r = r - LDG(c.g.slope_y, ji, jk, jl) * remap_y(c, VGrad<R, P>{s}, ji, jk, jl);
   return r;
 }
 
 // vertical_contravariant_corr, mo_averaging.F90:16-54
-template <class R, class P>
-__device__ R vcc(const CtxT<R> &c, P vx, P vy, int ji, int jk, int jl) {
-  R r = R(0.0);
+template <class R, class P, class X>
+__device__ VT<R, X> vcc(const CtxT<R> &c, P vx, P vy, int ji, X jk, int jl) {
+  const auto ke = east(jk);
+  VT<R, X> r = R(0.0);
   if (jl >= c.G.nflat + 1) {
     if (jl != c.G.nflat + 1) {
       const int l = jl - 1;
-      r = r - R(0.5) * ATR(vx, ji, jk + 1, l) * AT(c.g.slope_x, ji, jk + 1, l) * AT(c.g.ipw[0], ji, jk, l) -
-          R(0.5) * ATR(vy, ji, jk, l) * AT(c.g.slope_y, ji, jk, l) * AT(c.g.ipw[1], ji, jk, l) -
-          R(0.5) * ATR(vx, ji, jk, l) * AT(c.g.slope_x, ji, jk, l) * AT(c.g.ipw[2], ji, jk, l) -
-          R(0.5) * ATR(vy, ji + 1, jk, l) * AT(c.g.slope_y, ji + 1, jk, l) * AT(c.g.ipw[3], ji, jk, l);
+      r = r - R(0.5) * LDGR(vx, ji, ke, l) * LDG(c.g.slope_x, ji, ke, l) * LDG(c.g.ipw[0], ji, jk, l) -
+          R(0.5) * LDGR(vy, ji, jk, l) * LDG(c.g.slope_y, ji, jk, l) * LDG(c.g.ipw[1], ji, jk, l) -
+          R(0.5) * LDGR(vx, ji, jk, l) * LDG(c.g.slope_x, ji, jk, l) * LDG(c.g.ipw[2], ji, jk, l) -
+          R(0.5) * LDGR(vy, ji + 1, jk, l) * LDG(c.g.slope_y, ji + 1, jk, l) * LDG(c.g.ipw[3], ji, jk, l);
     }
-    r = r - R(0.5) * ATR(vx, ji, jk + 1, jl) * AT(c.g.slope_x, ji, jk + 1, jl) * AT(c.g.ipw[0], ji, jk, jl) -
-        R(0.5) * ATR(vy, ji, jk, jl) * AT(c.g.slope_y, ji, jk, jl) * AT(c.g.ipw[1], ji, jk, jl) -
-        R(0.5) * ATR(vx, ji, jk, jl) * AT(c.g.slope_x, ji, jk, jl) * AT(c.g.ipw[2], ji, jk, jl) -
-        R(0.5) * ATR(vy, ji + 1, jk, jl) * AT(c.g.slope_y, ji + 1, jk, jl) * AT(c.g.ipw[3], ji, jk, jl);
+    r = r - R(0.5) * LDGR(vx, ji, ke, jl) * LDG(c.g.slope_x, ji, ke, jl) * LDG(c.g.ipw[0], ji, jk, jl) -
+        R(0.5) * LDGR(vy, ji, jk, jl) * LDG(c.g.slope_y, ji, jk, jl) * LDG(c.g.ipw[1], ji, jk, jl) -
+        R(0.5) * LDGR(vx, ji, jk, jl) * LDG(c.g.slope_x, ji, jk, jl) * LDG(c.g.ipw[2], ji, jk, jl) -
+        R(0.5) * LDGR(vy, ji + 1, jk, jl) * LDG(c.g.slope_y, ji + 1, jk, jl) * LDG(c.g.ipw[3], ji, jk, jl);
   }
   return r;
 }
 
 // div_h, mo_divergence_operators.F90:16-68
-template <class R, class P>
-__device__ R div_h(const CtxT<R> &c, P vx, P vy, int ji, int jk, int jl) {
-  R comp_h = ATR(vx, ji, jk + 1, jl) * AT(c.g.area_x, ji, jk + 1, jl) + ATR(vy, ji, jk, jl) * AT(c.g.area_y, ji, jk, jl) -
-             ATR(vx, ji, jk, jl) * AT(c.g.area_x, ji, jk, jl) - ATR(vy, ji + 1, jk, jl) * AT(c.g.area_y, ji + 1, jk, jl);
-  R comp_v = R(0.0);
+template <class R, class P, class X>
+__device__ VT<R, X> div_h(const CtxT<R> &c, P vx, P vy, int ji, X jk, int jl) {
+  using V = VT<R, X>;
+  const auto ke = east(jk);
+  V comp_h = LDGR(vx, ji, ke, jl) * LDG(c.g.area_x, ji, ke, jl) + LDGR(vy, ji, jk, jl) * LDG(c.g.area_y, ji, jk, jl) -
+             LDGR(vx, ji, jk, jl) * LDG(c.g.area_x, ji, jk, jl) - LDGR(vy, ji + 1, jk, jl) * LDG(c.g.area_y, ji + 1, jk, jl);
+  V comp_v = R(0.0);
   const int nflat = c.G.nflat, nl = c.G.nl;
   if (jl == nflat) {
-    R cl = vcc(c, vx, vy, ji, jk, jl + 1);
-    comp_v = -cl * AT(c.g.area_z, ji, jk, jl + 1);
+    V cl = vcc(c, vx, vy, ji, jk, jl + 1);
+    comp_v = -cl * LDG(c.g.area_z, ji, jk, jl + 1);
   } else if (jl == nl) {
-    R cu = vcc(c, vx, vy, ji, jk, jl);
-    comp_v = cu * AT(c.g.area_z, ji, jk, jl);
+    V cu = vcc(c, vx, vy, ji, jk, jl);
+    comp_v = cu * LDG(c.g.area_z, ji, jk, jl);
   } else if (jl > nflat) {
-    R cu = vcc(c, vx, vy, ji, jk, jl);
-    R cl = vcc(c, vx, vy, ji, jk, jl + 1);
-    comp_v = cu * AT(c.g.area_z, ji, jk, jl) - cl * AT(c.g.area_z, ji, jk, jl + 1);
+    V cu = vcc(c, vx, vy, ji, jk, jl);
+    V cl = vcc(c, vx, vy, ji, jk, jl + 1);
+    comp_v = cu * LDG(c.g.area_z, ji, jk, jl) - cl * LDG(c.g.area_z, ji, jk, jl + 1);
   }
-  return (comp_h + comp_v) / AT(c.g.volume, ji, jk, jl);
+  return (comp_h + comp_v) / LDG(c.g.volume, ji, jk, jl);
 }
 
 // horizontal_covariant_x/_y, mo_averaging.F90:138-176 (vertical component = wind_w)

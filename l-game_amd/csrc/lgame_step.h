@@ -25,6 +25,7 @@ template <class R> __global__ void k_solve_ver(const CtxT<R> c, const StateDev o
 template <class R> __global__ void k_solve_gen(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
 template <class R> __global__ void k_diff_zeta(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_coef(const CtxT<R> c, const StateDev s);
+template <class R> __global__ void k_diff_coef2(const CtxT<R> c, const StateDev s);  // nx even: two cells per thread
 template <class R> __global__ void k_diff_dual(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_mom(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_flux(const CtxT<R> c, const StateDev s);

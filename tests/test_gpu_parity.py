@@ -61,6 +61,9 @@ CASES = {
     "tiny_dry_o2_diff": (diff(tiny_config()), 4, 0.0),
     "tiny_moist_o3_freeslip_diff": (diff(tiny_config(moist=True, order=3, freeslip=True)), 5, 0.3),
     "tiny_dry_o3_diff_tke": (diff(tiny_config(order=3), diff_coeff_scheme_h="tke"), 8, 0.0),
+    # even nx: the diffusion coefficients with two cells per thread, both horizontal schemes
+    "tiny_even_dry_o3_diff_tke": (diff(tiny_config(nx=8, order=3), diff_coeff_scheme_h="tke"), 8, 0.0),
+    "tiny_even_moist_o2_diff": (diff(tiny_config(ny=6, nx=12, n_layers=7, moist=True)), 5, 0.2),
     "tiny_moist_diff_no_vertical": (diff(tiny_config(ny=9, nx=11, n_layers=7, moist=True), lmom_diff_v=False,
                                          ltemp_diff_v=False, lmass_diff_h=False), 5, 0.2),
     "tiny_plane_diff_mom_v_only": (diff(tiny_config(corio=False, plane=True), lmom_diff_h=False, ltemp_diff_h=False,
@@ -105,7 +108,8 @@ def test_gpu_matches_oracle(name):
 DIFF_DIAGS = ("tke", "viscosity", "mom_diff_tend_x", "mom_diff_tend_z", "temp_diff_heating", "heating_diss")
 
 
-@pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "dry3d_33x33x20_diff",
+@pytest.mark.parametrize("name", ["tiny_moist_o3_freeslip_diff", "tiny_dry_o3_diff_tke", "tiny_even_dry_o3_diff_tke",
+                                  "tiny_even_moist_o2_diff", "dry3d_33x33x20_diff", "dry3d_66x40x18_even_diff",
                                   "dry3d_33x33x20_diff_pbl", "dry3d_33x33x20_diff_pbl_soil7"])
 def test_gpu_diffusion_diagnostics_match_oracle(name):
     """The carried t_diag fields of the diffusion and the PBL (TKE, viscosity, momentum diffusion
