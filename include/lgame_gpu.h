@@ -478,6 +478,14 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows);
 int lgame_set_tiles(lgame_handle *h, const int *rows);
 int lgame_set_segments(lgame_handle *h, int nseg);
 int lgame_get_segments(lgame_handle *h, int *nseg);
+/* The implicit vertical solver (three_band_solver_ver, mo_column_solvers.F90:21-351) runs in one
+ * pass per column (one lane a column) or in three passes: the rows of the system and the new
+ * state by one thread per (column, level), the Thomas recursion alone per column.  lgame_init
+ * picks three passes when the domain has fewer than 1024 wavefronts of 64 columns (fewer than the
+ * GPU's SIMDs) and at least 3 layers, one pass otherwise; lgame_set_column_passes (1 or 3) sets it
+ * and lgame_get_column_passes reads it.  Both give the same bits.  No reference counterpart. */
+int lgame_set_column_passes(lgame_handle *h, int passes);
+int lgame_get_column_passes(lgame_handle *h, int *passes);
 
 /* ---- ensembles ----
  * Independent domains (handles, each with its own grid, size and state) advanced together.

@@ -287,7 +287,6 @@ template <class R>
 int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int step, hipStream_t st, ProfHook *prof,
                     const RadCall *rad, const Tiles &tiles) {
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
-  const dim3 g_col((unsigned)col_waves(c.G)), b_col(BCOL);
   const bool o3 = c.C.order == 3;
   const bool pk = prof && prof->kernels;
   // a stencil kernel on a (64 x by) block grid covering (ex, ey, ez) points, by from the tile table
@@ -390,7 +389,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // columns; w_free_slip is done in k_scalar_col
     if (c.C.lsoil || c.C.lsens) LG_LAUNCH(KI_DIFF_DUAL, k_surface, nx, ny, 1, c, old, nw, rk);
     if (prof) prof->begin(0);
-    hipLaunchKernelGGL(k_solve_ver<R>, g_col, b_col, 0, st, c, old, nw, rk);
+    launch_solve_ver(c, old, nw, rk, tiles.ver_passes, st);
     if (prof) prof->end(0);
     if (c.G.nc > 1) {
       if (prof) prof->begin(1);
@@ -639,6 +638,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     return fail(nullptr, LGAME_EINVAL, "one field of %lld points exceeds 4 GiB", G.fsize);
   }
   h->tiles.seg = default_segments(G);
+  h->tiles.ver_passes = col_waves(G) < VER_SPLIT_WAVES && G.nl >= 3 ? 3 : 1;
   Cfg &C = h->c.C;
   C.dtime = f.dtime;
   C.toa = f.toa;
@@ -1438,6 +1438,20 @@ int lgame_set_segments(lgame_handle *h, int nseg) {
 int lgame_get_segments(lgame_handle *h, int *nseg) {
   if (!h || !nseg) return fail(h, LGAME_EINVAL, "null argument");
   *nseg = h->tiles.seg;
+  return LGAME_OK;
+}
+
+int lgame_set_column_passes(lgame_handle *h, int passes) {
+  if (!h) return fail(h, LGAME_EINVAL, "null handle");
+  if (passes != 1 && passes != 3) return fail(h, LGAME_EINVAL, "column passes must be 1 or 3");
+  if (passes == 3 && h->c.G.nl < 3) return fail(h, LGAME_EINVAL, "three passes need at least 3 layers");
+  h->tiles.ver_passes = passes;
+  return LGAME_OK;
+}
+
+int lgame_get_column_passes(lgame_handle *h, int *passes) {
+  if (!h || !passes) return fail(h, LGAME_EINVAL, "null argument");
+  *passes = h->tiles.ver_passes;
   return LGAME_OK;
 }
 

@@ -113,6 +113,49 @@ struct BackIn {
   double az_b, wo_b;                                     // interface Lr+1 (below the layer)
 };
 
+// row q of the interface mass-flux system (mo_column_solvers.F90:187-226): diagonal d, right-hand
+// side r, and (when layer q+2 exists) the coefficients c and e that couple it to rows q+1 and q-1;
+// L0, L1, L2 are layers q, q+1, q+2, F0 and F1 the interfaces q+1 and q+2
+struct Row {
+  double d, r, c, e;
+};
+__device__ __forceinline__ Row make_row(const Layer &L0, const Layer &L1, const Layer &L2, const Iface &F0, const Iface &F1,
+                                        bool has_next, double dt) {
+  const double K2 = IMPL_THERMO_WEIGHT * (dt * dt);
+  const double K1 = IMPL_THERMO_WEIGHT * dt * C_D_P;
+  Row w;
+  const double dzs = L0.z - L1.z;
+  const double deb = L0.eb - L1.eb;
+  w.d = -F0.tin * F0.tin * (L0.ga + L1.ga) + 0.5 * deb * (L1.al - L0.al + F0.tin * (L1.be - L0.be)) -
+        dzs / (K2 * C_D_P * F0.rio) * (2.0 / F0.az + dt * F0.wo * 0.5 * (-1.0 / L0.V + 1.0 / L1.V)) * F0.dp;
+  w.r = -(F0.wo + dt * F0.tw) * dzs / (K2 * C_D_P) + F0.tin * (L0.epe - L1.epe) / dt + 0.5 / dt * (L0.tpe + L1.tpe) * deb -
+        dzs / (K2 * C_D_P) * F0.wo * F0.rie / F0.rio;
+  w.c = 0.0;
+  w.e = 0.0;
+  if (has_next) {
+    w.c = F1.tin * L1.ga * F0.tin + 0.5 * (L1.eb - L2.eb) * (L1.al + L1.be * F0.tin) -
+          (L1.z - L2.z) / K1 * 0.5 * F1.wo / (L1.V * F1.rio) * F1.dp;
+    w.e = F0.tin * L1.ga * F1.tin - 0.5 * deb * (L1.al + L1.be * F1.tin) + dzs / K1 * 0.5 * F0.wo / (L1.V * F0.rio) * F0.dp;
+  }
+  return w;
+}
+
+// the new rho and rhotheta_v of layer Lr (mo_column_solvers.F90:307-320) from the solution at its
+// upper (x_up, row Lr-1) and lower (x_low, row Lr) interface and theta_v there
+__device__ __forceinline__ void new_densities(int Lr, int nl, double rhoe, double rte, double V, double dt, double x_up,
+                                              double x_low, double tin_u, double tin_low, double &rho_new, double &rt_new) {
+  if (Lr == nl) {
+    rho_new = rhoe - dt * x_up / V;
+    rt_new = rte - dt * tin_u * x_up / V;
+  } else if (Lr >= 2) {
+    rho_new = rhoe + dt * (-x_up + x_low) / V;
+    rt_new = rte + dt * (-tin_u * x_up + tin_low * x_low) / V;
+  } else {
+    rho_new = rhoe + dt * x_low / V;
+    rt_new = rte + dt * tin_low * x_low / V;
+  }
+}
+
 // three_band_solver_ver, mo_column_solvers.F90:21-351 (soil and surface heat flux off).
 //
 // The back substitution needs, per layer, rho_e, rhotheta_e, gamma and, per row, rho at the
@@ -128,8 +171,6 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
   const StateDev &nu = RK == 1 ? old : nw;
   const double dt = c.C.dtime;
   const Fld<double> sep = c.w.col[0], srp = c.w.col[1], stin = c.w.col[4], sga = c.w.col[5];
-  const double K2 = IMPL_THERMO_WEIGHT * (dt * dt);
-  const double K1 = IMPL_THERMO_WEIGHT * dt * C_D_P;
 
   // forward sweep: rows q = 1..nl-1 of the interface mass-flux system
   Layer L0 = make_layer(load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, 1)), RK, dt);
@@ -151,18 +192,8 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
       L2 = make_layer(cur, RK, dt);
       F1 = make_iface(L1, L2, cur);
     }
-    const double dzs = L0.z - L1.z;
-    const double deb = L0.eb - L1.eb;
-    const double d = -F0.tin * F0.tin * (L0.ga + L1.ga) + 0.5 * deb * (L1.al - L0.al + F0.tin * (L1.be - L0.be)) -
-                     dzs / (K2 * C_D_P * F0.rio) * (2.0 / F0.az + dt * F0.wo * 0.5 * (-1.0 / L0.V + 1.0 / L1.V)) * F0.dp;
-    const double r = -(F0.wo + dt * F0.tw) * dzs / (K2 * C_D_P) + F0.tin * (L0.epe - L1.epe) / dt + 0.5 / dt * (L0.tpe + L1.tpe) * deb -
-                     dzs / (K2 * C_D_P) * F0.wo * F0.rie / F0.rio;
-    double cq = 0.0, eq = 0.0;
-    if (has_next) {
-      cq = F1.tin * L1.ga * F0.tin + 0.5 * (L1.eb - L2.eb) * (L1.al + L1.be * F0.tin) -
-           (L1.z - L2.z) / K1 * 0.5 * F1.wo / (L1.V * F1.rio) * F1.dp;
-      eq = F0.tin * L1.ga * F1.tin - 0.5 * deb * (L1.al + L1.be * F1.tin) + dzs / K1 * 0.5 * F0.wo / (L1.V * F0.rio) * F0.dp;
-    }
+    const Row w = make_row(L0, L1, L2, F0, F1, has_next, dt);
+    const double d = w.d, r = w.r, cq = w.c, eq = w.e;
     // thomas_algorithm, mo_column_solvers.F90:622-632
     double epq, rpq;
     if (q == 1) {
@@ -236,18 +267,11 @@ __device__ __forceinline__ void ver_column(const CtxT<R> &c, const StateDev &old
     const double V = b.V;
     double rho_new, rt_new;
     double x_up = 0.0;
-    if (Lr == nl) {
+    if (Lr == nl)
       x_up = b.rp_u;  // x(nl-1)
-      rho_new = rhoe - dt * x_up / V;
-      rt_new = rte - dt * tin_u * x_up / V;
-    } else if (Lr >= 2) {
+    else if (Lr >= 2)
       x_up = b.rp_u - b.ep_u * x_low;
-      rho_new = rhoe + dt * (-x_up + x_low) / V;
-      rt_new = rte + dt * (-tin_u * x_up + tin_low * x_low) / V;
-    } else {
-      rho_new = rhoe + dt * x_low / V;
-      rt_new = rte + dt * tin_low * x_low / V;
-    }
+    new_densities(Lr, nl, rhoe, rte, V, dt, x_up, x_low, tin_u, tin_low, rho_new, rt_new);
     nw.rho[g][i] = rho_new;
     nw.rhotheta_v[i] = rt_new;
     if (Lr < nl) {  // w at the interface below this layer, level Lr+1
@@ -282,6 +306,198 @@ template <class R>
 __global__ void __launch_bounds__(BCOL) k_solve_ver(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
   b_solve_ver(c, old, nw, rk, bid_here());
 }
+
+// ---- the same solve in three passes, for domains with fewer column wavefronts than SIMDs -------
+// On such a domain k_solve_ver's wavefronts leave SIMDs idle and each runs its column's chain of
+// dependent FP64 operations alone.  Here the rows of the system (k_ver_rows) and the new state
+// (k_ver_update) are formed by one thread per (column, level), and only the Thomas recursion
+// (k_ver_thomas) runs per column.  Every value is formed by the same expressions as in ver_column
+// (make_layer, make_iface, make_row, new_densities), so the results are the same bits.
+// Column scratch: col[0] d, then c'; col[1] r, then r', then x; col[2] c; col[3] e (rows);
+// col[4] theta_v at the row's interface (make_iface's tin); col[5] gamma of each layer.
+template <class R, int RK>
+__device__ __forceinline__ void ver_row(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int ji, int jk, int q) {
+  const int nl = c.G.nl;
+  const StateDev &nu = RK == 1 ? old : nw;
+  const double dt = c.C.dtime;
+  const RawLayer raw1 = load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, q + 1));
+  const Layer L0 = make_layer(load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, q)), RK, dt);
+  const Layer L1 = make_layer(raw1, RK, dt);
+  const Iface F0 = make_iface(L0, L1, raw1);
+  const bool has_next = q + 2 <= nl;
+  Layer L2 = L1;
+  Iface F1 = F0;
+  if (has_next) {
+    const RawLayer raw2 = load_raw(c, old, nu, RK, LG_IX(c.G, ji, jk, q + 2));
+    L2 = make_layer(raw2, RK, dt);
+    F1 = make_iface(L1, L2, raw2);
+  }
+  const Row w = make_row(L0, L1, L2, F0, F1, has_next, dt);
+  const ix_t iq = LG_IX(c.G, ji, jk, q);
+  c.w.col[0][iq] = w.d;
+  c.w.col[1][iq] = w.r;
+  c.w.col[2][iq] = w.c;
+  c.w.col[3][iq] = w.e;
+  c.w.col[4][iq] = F0.tin;
+  c.w.col[5][iq] = L0.ga;
+  if (q == nl - 1) c.w.col[5][LG_IX(c.G, ji, jk, nl)] = L1.ga;
+}
+template <class R>
+__global__ void __launch_bounds__(BX * 4) k_ver_rows(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  const int jk = blockIdx.x * BX + threadIdx.x + 1, ji = blockIdx.y * blockDim.y + threadIdx.y + 1, q = blockIdx.z + 1;
+  if (jk > c.G.nx || ji > c.G.ny || q > c.G.nl - 1) return;
+  if (rk == 1)
+    ver_row<R, 1>(c, old, nw, ji, jk, q);
+  else
+    ver_row<R, 2>(c, old, nw, ji, jk, q);
+}
+
+// thomas_algorithm (mo_column_solvers.F90:606-640) on rows 1..nl-1 of one column; the rows are
+// loaded in chunks of VCH, the next chunk while the current one is eliminated
+constexpr int VCH = 8;
+__global__ void __launch_bounds__(BCOL) k_ver_thomas(const Geom G, Fld<double> s_d, Fld<double> s_r, Fld<const double> s_c,
+                                                   Fld<const double> s_e) {
+  const int wpr = col_waves_per_row(G.nx);
+  const int ji = blockIdx.x / wpr + 1, jk = (blockIdx.x % wpr) * BCOL + (int)threadIdx.x + 1;
+  if (ji > G.ny || jk > G.nx) return;
+  const int n = G.nl - 1;
+  const ix_t i0 = LG_IX(G, ji, jk, 1), lvl = (ix_t)G.lvl;
+  double d[2][VCH], r[2][VCH], cc[2][VCH], e[2][VCH];
+  auto load = [&](int b, int q0) {
+#pragma unroll
+    for (int j = 0; j < VCH; ++j)
+      if (q0 + j <= n) {
+        const ix_t i = i0 + (ix_t)(q0 + j - 1) * lvl;
+        d[b][j] = s_d[i];
+        r[b][j] = s_r[i];
+        cc[b][j] = s_c[i];
+        e[b][j] = s_e[i];
+      }
+  };
+  double ep_prev = 0.0, rp_prev = 0.0, c_prev = 0.0;
+  load(0, 1);
+  for (int q0 = 1; q0 <= n; q0 += 2 * VCH) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qa = q0 + h * VCH;  // the chunk in buffer h
+      if (qa > n) break;
+      if (qa + VCH <= n) load(1 - h, qa + VCH);
+#pragma unroll
+      for (int j = 0; j < VCH; ++j) {
+        const int q = qa + j;
+        if (q > n) break;
+        double epq, rpq;
+        if (q == 1) {
+          epq = e[h][j] / d[h][j];
+          rpq = r[h][j] / d[h][j];
+        } else {
+          const double den = d[h][j] - ep_prev * c_prev;
+          epq = e[h][j] / den;
+          rpq = (r[h][j] - rp_prev * c_prev) / den;
+        }
+        const ix_t i = i0 + (ix_t)(q - 1) * lvl;
+        s_d[i] = epq;
+        s_r[i] = rpq;
+        ep_prev = epq;
+        rp_prev = rpq;
+        c_prev = cc[h][j];
+      }
+    }
+  }
+  // back substitution from row n up, x into s_r
+  auto load_b = [&](int b, int qt) {  // rows qt, qt-1, ..., qt-VCH+1
+#pragma unroll
+    for (int j = 0; j < VCH; ++j)
+      if (qt - j >= 1) {
+        const ix_t i = i0 + (ix_t)(qt - j - 1) * lvl;
+        d[b][j] = s_d[i];
+        r[b][j] = s_r[i];
+      }
+  };
+  double x_low = 0.0;
+  load_b(0, n);
+  for (int q0 = n; q0 >= 1; q0 -= 2 * VCH) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qt = q0 - h * VCH;
+      if (qt < 1) break;
+      if (qt - VCH >= 1) load_b(1 - h, qt - VCH);
+#pragma unroll
+      for (int j = 0; j < VCH; ++j) {
+        const int q = qt - j;
+        if (q < 1) break;
+        const double x = q == n ? r[h][j] : r[h][j] - d[h][j] * x_low;
+        s_r[i0 + (ix_t)(q - 1) * lvl] = x;
+        x_low = x;
+      }
+    }
+  }
+}
+
+// the new state of layer Lr from the solution x (mo_column_solvers.F90:305-349)
+template <class R>
+__global__ void __launch_bounds__(BX * 4) k_ver_update(const CtxT<R> c, const StateDev old, const StateDev nw, int rk) {
+  const int jk = blockIdx.x * BX + threadIdx.x + 1, ji = blockIdx.y * blockDim.y + threadIdx.y + 1, Lr = blockIdx.z + 1;
+  const int nl = c.G.nl, g = c.G.ncond;
+  if (jk > c.G.nx || ji > c.G.ny || Lr > nl) return;
+  const double dt = c.C.dtime;
+  const Fld<const double> sx = c.w.col[1], stin = c.w.col[4], sga = c.w.col[5];
+  const ix_t i = LG_IX(c.G, ji, jk, Lr), lvl = (ix_t)c.G.lvl;
+  double x_up = 0.0, x_low = 0.0, tin_u = 0.0, tin_low = 0.0;
+  if (Lr >= 2) {
+    x_up = sx[i - lvl];
+    tin_u = stin[i - lvl];
+  }
+  if (Lr < nl) {
+    x_low = sx[i];
+    tin_low = stin[i];
+  }
+  const double V = c.g64.volume[i];
+  const double rho_o = old.rho[g][i], rt_o = old.rhotheta_v[i];
+  const double rhoe = rho_o + dt * (double)c.w.tend_rho[g][i];
+  const double rte = rt_o + dt * (double)c.w.tend_rhotheta_v[i];
+  double rho_new, rt_new;
+  new_densities(Lr, nl, rhoe, rte, V, dt, x_up, x_low, tin_u, tin_low, rho_new, rt_new);
+  nw.rho[g][i] = rho_new;
+  nw.rhotheta_v[i] = rt_new;
+  if (Lr < nl) {  // w at the interface below this layer, level Lr+1, with the new rho of the layer below
+    const ix_t ib = i + lvl;
+    const double V_b = c.g64.volume[ib];
+    const double rho_o_b = old.rho[g][ib];
+    const double rhoe_b = rho_o_b + dt * (double)c.w.tend_rho[g][ib];
+    const double rte_b = old.rhotheta_v[ib] + dt * (double)c.w.tend_rhotheta_v[ib];
+    const double x2 = Lr + 1 < nl ? sx[ib] : 0.0;
+    const double tin2 = Lr + 1 < nl ? stin[ib] : 0.0;
+    double rho_b, rt_b;
+    new_densities(Lr + 1, nl, rhoe_b, rte_b, V_b, dt, x_low, x2, tin_low, tin2, rho_b, rt_b);
+    const double rio = 0.5 * (rho_o + rho_o_b);
+    const double rin = 0.5 * (rho_new + rho_b);
+    nw.wind_w[ib] = (2.0 * x_low / c.g64.area_z[ib] - rin * old.wind_w[ib]) / rio;
+  }
+  const double ep_o = old.exner_pert[i];
+  const double ep_new = ep_o + V * sga[i] * (rt_new - rt_o);
+  const double tp_new = rt_new / rho_new - c.g64.theta_v_bg[i];
+  nw.exner_pert[i] = ep_new;
+  nw.theta_v_pert[i] = tp_new;
+  if (rk == 2 && !(isfinite(rho_new) && isfinite(rt_new) && isfinite(ep_new) && isfinite(tp_new)))
+    atomicOr(c.C.nonfinite, 1ull);
+}
+
+template <class R>
+void launch_solve_ver(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, int passes, hipStream_t st) {
+  const Geom &G = c.G;
+  if (passes != 3 || G.nl < 3) {
+    hipLaunchKernelGGL(k_solve_ver<R>, dim3((unsigned)col_waves(G)), dim3(BCOL), 0, st, c, old, nw, rk);
+    return;
+  }
+  const dim3 b(BX, 4), g_rows((G.nx + BX - 1) / BX, (G.ny + 3) / 4, G.nl - 1), g_up(g_rows.x, g_rows.y, G.nl);
+  hipLaunchKernelGGL(k_ver_rows<R>, g_rows, b, 0, st, c, old, nw, rk);
+  hipLaunchKernelGGL(k_ver_thomas, dim3((unsigned)col_waves(G)), dim3(BCOL), 0, st, G, c.w.col[0], c.w.col[1],
+                     Fld<const double>(c.w.col[2]), Fld<const double>(c.w.col[3]));
+  hipLaunchKernelGGL(k_ver_update<R>, g_up, b, 0, st, c, old, nw, rk);
+}
+template void launch_solve_ver<double>(const CtxT<double> &, const StateDev &, const StateDev &, int, int, hipStream_t);
+template void launch_solve_ver<float>(const CtxT<float> &, const StateDev &, const StateDev &, int, int, hipStream_t);
 
 __device__ __forceinline__ double v_fall_solid(double radius) { return (radius > (double)250.e-6f) ? 1.0 : 0.001; }
 

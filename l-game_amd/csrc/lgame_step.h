@@ -54,6 +54,7 @@ enum KernelId {
 struct Tiles {
   int by[KI_COUNT];
   int seg = 1;
+  int ver_passes = 1;  // the vertical solver in one column pass (k_solve_ver) or three (launch_solve_ver)
   Tiles() {
     for (int &b : by) b = BY;
   }
@@ -74,6 +75,11 @@ template <class R>
 void launch_scalar_col(const CtxT<R> &c, const StateDev &ss, const StateDev &nw, int rk, int rows, int nseg, hipStream_t st);
 // Lanes per block of the column solvers.
 constexpr int BCOL = 64;
+// The vertical solver runs in three passes (rows, Thomas recursion per column, new state) on
+// domains with fewer column wavefronts than this, the 1024 SIMDs of the MI355X.
+constexpr int VER_SPLIT_WAVES = 1024;
+template <class R>
+void launch_solve_ver(const CtxT<R> &c, const StateDev &old, const StateDev &nw, int rk, int passes, hipStream_t st);
 // The column solvers give each wavefront 64 consecutive columns of one row (the row's last
 // wavefront partly idle), so that a wavefront's loads of a level cover whole 128-byte lines of
 // the row rather than the end of one row and the start of the next.

@@ -24,7 +24,7 @@ EXPORTED = ("lgame_abi_version", "lgame_grid_setup", "lgame_init", "lgame_set_st
             "lgame_mlp_create", "lgame_mlp_n_params", "lgame_mlp_set_params", "lgame_mlp_get_params",
             "lgame_mlp_set_norm", "lgame_mlp_forward", "lgame_mlp_forward_noalloc", "lgame_mlp_gradient", "lgame_mlp_load_data",
             "lgame_mlp_train", "lgame_mlp_last_error", "lgame_mlp_destroy", "lgame_autotune", "lgame_set_tiles",
-            "lgame_set_segments", "lgame_get_segments",
+            "lgame_set_segments", "lgame_get_segments", "lgame_set_column_passes", "lgame_get_column_passes",
             "lgame_ensemble_create", "lgame_ensemble_step", "lgame_ensemble_sync", "lgame_ensemble_last_error",
             "lgame_ensemble_destroy", "lgame_checkpoint_size", "lgame_checkpoint_save", "lgame_checkpoint_load")
 
@@ -174,6 +174,8 @@ def load_library(path: str = LIB_PATH):
     lib.lgame_set_tiles.argtypes = [vp, vp]
     lib.lgame_set_segments.argtypes = [vp, ci]
     lib.lgame_get_segments.argtypes = [vp, ctypes.POINTER(ci)]
+    lib.lgame_set_column_passes.argtypes = [vp, ci]
+    lib.lgame_get_column_passes.argtypes = [vp, ctypes.POINTER(ci)]
     lib.lgame_ensemble_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
     lib.lgame_ensemble_step.argtypes = [vp, ci, ci, cd, ci]
     lib.lgame_ensemble_sync.argtypes = [vp]
@@ -301,6 +303,17 @@ class Step:
     @segments.setter
     def segments(self, nseg: int):
         self._check(self.lib.lgame_set_segments(self.h, int(nseg)), "lgame_set_segments")
+
+    @property
+    def column_passes(self) -> int:
+        """The vertical solver in one pass per column (1) or in three passes (3)."""
+        n = ctypes.c_int(0)
+        self._check(self.lib.lgame_get_column_passes(self.h, ctypes.byref(n)), "lgame_get_column_passes")
+        return n.value
+
+    @column_passes.setter
+    def column_passes(self, passes: int):
+        self._check(self.lib.lgame_set_column_passes(self.h, int(passes)), "lgame_set_column_passes")
 
     # ---- radiation ----
     def set_rad_surface(self, surface: dict):

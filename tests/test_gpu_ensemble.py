@@ -64,6 +64,49 @@ def test_scalar_kernel_segments_and_rows_keep_results():
     s.close()
 
 
+VER_CASES = {
+    "dry_o2": (tiny_config(ny=6, nx=70, n_layers=9), 4, 0.0),
+    "moist_o3_freeslip": (tiny_config(ny=9, nx=11, n_layers=7, moist=True, order=3, freeslip=True), 5, 0.3),
+    "three_layers": (tiny_config(ny=3, nx=4, n_layers=3, moist=True), 4, 0.2),
+    "diff_pbl_soil_sens": (dataclasses.replace(tiny_config(moist=True, order=3).set_diffusion(True), lpbl=True,
+                                               lprog_soil_temp=True, lsfc_sensible_heat_flux=True), 6, 0.3),
+}
+
+
+@pytest.mark.parametrize("name", list(VER_CASES))
+def test_vertical_solver_passes_keep_results(name):
+    """The vertical solver in one pass per column and in three passes (rows, Thomas recursion,
+    new state), each bitwise equal to the oracle; small domains default to three passes."""
+    cfg, n, cf = VER_CASES[name]
+    g, st = make_case(cfg, cloud_frac=cf)
+    want = _oracle(cfg, g, st, n)
+    for passes in (1, 3):
+        s = Step(cfg, g)
+        assert s.column_passes == 3  # fewer column wavefronts than SIMDs
+        s.column_passes = passes
+        assert s.column_passes == passes
+        s.start(st)
+        s.run(0, n)
+        got = s.get_state(n % 2)
+        s.close()
+        print(f"{name}: passes {passes}", flush=True)
+        assert all(np.array_equal(got[k], want[k]) for k in want), (name, passes)
+    s = Step(cfg, g)
+    with pytest.raises(LgameError, match="passes"):
+        s.column_passes = 2
+    s.close()
+
+
+def test_vertical_solver_default_passes_by_size():
+    """One pass per column once the domain has as many 64-column wavefronts as the GPU has SIMDs."""
+    for n, want in ((128, 3), (256, 1)):
+        cfg = dry3d_config(n=n, n_layers=10)
+        g, st = make_case(cfg)
+        s = Step(cfg, g)
+        assert s.column_passes == want, n
+        s.close()
+
+
 @pytest.mark.parametrize("mode", [Ensemble.GRAPH, Ensemble.SERIAL, Ensemble.BATCHED])
 def test_ensemble_members_match_oracle(mode):
     cfgs = [tiny_config(ny=5, nx=7, n_layers=6, moist=True, order=3),
