@@ -300,70 +300,74 @@ __device__ __forceinline__ R c_g_v(R rg) {
 }
 
 template <class R>
-__device__ __forceinline__ void b_diff_flux(const CtxT<R> &c, const StateDev &s, const Bid bid) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+__device__ __forceinline__ P2<R> c_g_v(P2<R> rg) {
+  return P2<R>(c_g_v(rg.a), c_g_v(rg.b));
+}
+
+// the fluxes of one cell (jk an int) or of the columns k, k+1 (jk an XP)
+template <class R, class X>
+__device__ __forceinline__ void diff_flux_at(const CtxT<R> &c, const StateDev &s, int ji, X jk, int jl) {
+  using V = VT<R, X>;
   const Geom &G = c.G;
   const int nx = G.nx, ny = G.ny, nl = G.nl;
-  if (jk > nx || ji > ny || jl > nl) return;
-  const int km = wrapm(jk, nx), jm = wrapm(ji, ny);
-  const ix_t dl = (ix_t)G.lvl;
-  const ix_t i = LG_IX(G, ji, jk, jl), iw = LG_IX(G, ji, km, jl), is = LG_IX(G, jm, jk, jl);
+  const auto km = wrapm(jk, nx);
+  const int jm = wrapm(ji, ny);
   const bool terr = jl > G.nflat;                 // grad_hor's terrain correction
   const bool below = jl < nl || c.C.lfreeslip;    // remap_ver2hor reads level jl+1
   const bool up = jl >= 2, dn = jl < nl;          // grad_vert is nonzero at level jl / jl+1
   // metrics
-  const R dx = c.g.dx[i], dy = c.g.dy[i];
-  R sx = R(0.0), sy = R(0.0), p4i = R(0.0), p4w = R(0.0), p4s = R(0.0), p5i = R(0.0), p5w = R(0.0), p5s = R(0.0);
-  R dzi = R(1.0), dzw = R(1.0), dzs = R(1.0), dzi1 = R(1.0), dzw1 = R(1.0), dzs1 = R(1.0);
-  if (up) dzi = c.g.dz[i];
+  const V dx = LDG(c.g.dx, ji, jk, jl), dy = LDG(c.g.dy, ji, jk, jl);
+  V sx = R(0.0), sy = R(0.0), p4i = R(0.0), p4w = R(0.0), p4s = R(0.0), p5i = R(0.0), p5w = R(0.0), p5s = R(0.0);
+  V dzi = R(1.0), dzw = R(1.0), dzs = R(1.0), dzi1 = R(1.0), dzw1 = R(1.0), dzs1 = R(1.0);
+  if (up) dzi = LDG(c.g.dz, ji, jk, jl);
   if (terr) {
-    sx = c.g.slope_x[i];
-    sy = c.g.slope_y[i];
-    p4i = c.g.ipw[4][i];
-    p4w = c.g.ipw[4][iw];
-    p4s = c.g.ipw[4][is];
+    sx = LDG(c.g.slope_x, ji, jk, jl);
+    sy = LDG(c.g.slope_y, ji, jk, jl);
+    p4i = LDG(c.g.ipw[4], ji, jk, jl);
+    p4w = LDG(c.g.ipw[4], ji, km, jl);
+    p4s = LDG(c.g.ipw[4], jm, jk, jl);
     if (up) {
-      dzw = c.g.dz[iw];
-      dzs = c.g.dz[is];
+      dzw = LDG(c.g.dz, ji, km, jl);
+      dzs = LDG(c.g.dz, jm, jk, jl);
     }
     if (below) {
-      p5i = c.g.ipw[5][i];
-      p5w = c.g.ipw[5][iw];
-      p5s = c.g.ipw[5][is];
+      p5i = LDG(c.g.ipw[5], ji, jk, jl);
+      p5w = LDG(c.g.ipw[5], ji, km, jl);
+      p5s = LDG(c.g.ipw[5], jm, jk, jl);
     }
     if (dn) {
-      dzi1 = c.g.dz[i + dl];
-      dzw1 = c.g.dz[iw + dl];
-      dzs1 = c.g.dz[is + dl];
+      dzi1 = LDG(c.g.dz, ji, jk, jl + 1);
+      dzw1 = LDG(c.g.dz, ji, km, jl + 1);
+      dzs1 = LDG(c.g.dz, jm, jk, jl + 1);
     }
   }
   // coefficients at the cell, its west and south neighbours and the cell above
   const Fld<const double> rgf = s.rho[G.ncond];
-  const R rgi = R(rgf[i]), rgw = R(rgf[iw]), rgs = R(rgf[is]);
-  const R vi = c.w.visc[i], vw = c.w.visc[iw], vs = c.w.visc[is];
-  const R kvi = c.w.kv[i];
-  R kva = R(0.0), rga = R(1.0);
+  const V rgi = LDGR(rgf, ji, jk, jl), rgw = LDGR(rgf, ji, km, jl), rgs = LDGR(rgf, jm, jk, jl);
+  const V vi = LDG(c.w.visc, ji, jk, jl), vw = LDG(c.w.visc, ji, km, jl), vs = LDG(c.w.visc, jm, jk, jl);
+  const V kvi = LDG(c.w.kv, ji, jk, jl);
+  V kva = R(0.0), rga = R(1.0);
   if (up) {
-    kva = c.w.kv[i - dl];
-    rga = R(rgf[i - dl]);
+    kva = LDG(c.w.kv, ji, jk, jl - 1);
+    rga = LDGR(rgf, ji, jk, jl - 1);
   }
   // one field: gradients at the u, v and w points, times the averaged coefficients
-  auto one = [&](auto f, Fld<R> du, Fld<R> dv, Fld<R> dw, R chu, R chv, R cvw, bool lvert) {
-    const R fi = R(f[i]), fw = R(f[iw]), fs = R(f[is]);
-    R vgi0 = R(0.0), vgw0 = R(0.0), vgs0 = R(0.0), vgi1 = R(0.0), vgw1 = R(0.0), vgs1 = R(0.0);
-    if (up) vgi0 = (R(f[i - dl]) - fi) / dzi;
-    R gx = (fi - fw) / dx, gy = (fs - fi) / dy;
+  auto one = [&](auto f, Fld<R> du, Fld<R> dv, Fld<R> dw, V chu, V chv, V cvw, bool lvert) {
+    const V fi = LDGR(f, ji, jk, jl), fw = LDGR(f, ji, km, jl), fs = LDGR(f, jm, jk, jl);
+    V vgi0 = R(0.0), vgw0 = R(0.0), vgs0 = R(0.0), vgi1 = R(0.0), vgw1 = R(0.0), vgs1 = R(0.0);
+    if (up) vgi0 = (LDGR(f, ji, jk, jl - 1) - fi) / dzi;
+    V gx = (fi - fw) / dx, gy = (fs - fi) / dy;
     if (terr) {
       if (up) {
-        vgw0 = (R(f[iw - dl]) - fw) / dzw;
-        vgs0 = (R(f[is - dl]) - fs) / dzs;
+        vgw0 = (LDGR(f, ji, km, jl - 1) - fw) / dzw;
+        vgs0 = (LDGR(f, jm, jk, jl - 1) - fs) / dzs;
       }
       if (dn) {
-        vgi1 = (fi - R(f[i + dl])) / dzi1;
-        vgw1 = (fw - R(f[iw + dl])) / dzw1;
-        vgs1 = (fs - R(f[is + dl])) / dzs1;
+        vgi1 = (fi - LDGR(f, ji, jk, jl + 1)) / dzi1;
+        vgw1 = (fw - LDGR(f, ji, km, jl + 1)) / dzw1;
+        vgs1 = (fs - LDGR(f, jm, jk, jl + 1)) / dzs1;
       }
-      R r = R(0.0);  // remap_ver2hor_x, mo_averaging.F90:56-95
+      V r = R(0.0);  // remap_ver2hor_x, mo_averaging.F90:56-95
       r = r + p4w * vgw0;
       r = r + p4i * vgi0;
       if (below) {
@@ -380,30 +384,47 @@ __device__ __forceinline__ void b_diff_flux(const CtxT<R> &c, const StateDev &s,
       }
       gy = gy - sy * (R(0.5) * r);
     }
-    const R fu = chu * gx, fv = chv * gy;
-    du[i] = fu;
-    if (jk == 1) AT(du, ji, nx + 1, jl) = fu;
-    dv[i] = fv;
-    if (ji == 1) AT(dv, ny + 1, jk, jl) = fv;
-    if (lvert && up) dw[i] = cvw * vgi0;
+    const V fu = chu * gx, fv = chv * gy;
+    stv(du, G, ji, jk, jl, fu);
+    if (col1(jk)) AT(du, ji, nx + 1, jl) = first(fu);
+    stv(dv, G, ji, jk, jl, fv);
+    if (ji == 1) stv(dv, G, ny + 1, jk, jl, fv);
+    if (lvert && up) stv(dw, G, ji, jk, jl, cvw * vgi0);
   };
   if (c.C.ltemp_h) {
-    const R chu = R(0.5) * (c_g_v(rgw) * vw + c_g_v(rgi) * vi);
-    const R chv = R(0.5) * (c_g_v(rgs) * vs + c_g_v(rgi) * vi);
-    const R cvw = R(0.5) * (rga * c_g_v(rga) * kva + rgi * c_g_v(rgi) * kvi);
+    const V chu = R(0.5) * (c_g_v(rgw) * vw + c_g_v(rgi) * vi);
+    const V chv = R(0.5) * (c_g_v(rgs) * vs + c_g_v(rgi) * vi);
+    const V cvw = R(0.5) * (rga * c_g_v(rga) * kva + rgi * c_g_v(rgi) * kvi);
     one(Fld<const R>(c.w.temperature), c.w.dfu[0], c.w.dfv[0], c.w.dfw[0], chu, chv, cvw, c.C.ltemp_v != 0);
   }
   if (c.C.lmass_h) {
-    const R chu = R(0.5) * (vw / rgw + vi / rgi);
-    const R chv = R(0.5) * (vs / rgs + vi / rgi);
-    const R cvw = R(0.5) * (kva + kvi);
+    const V chu = R(0.5) * (vw / rgw + vi / rgi);
+    const V chv = R(0.5) * (vs / rgs + vi / rgi);
+    const V cvw = R(0.5) * (kva + kvi);
     for (int jc = 0; jc < G.nc; ++jc)
       one(Fld<const double>(s.rho[jc]), c.w.dfu[jc + 1], c.w.dfv[jc + 1], c.w.dfw[jc + 1], chu, chv, cvw, c.C.lmass_v != 0);
   }
 }
+
+template <class R>
+__device__ __forceinline__ void b_diff_flux(const CtxT<R> &c, const StateDev &s, const Bid bid) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  diff_flux_at(c, s, ji, jk, jl);
+}
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_diff_flux(const CtxT<R> c, const StateDev s) {
   b_diff_flux(c, s, bid_here());
+}
+// nx even: the columns k = 2*jk-1 and k+1 per thread
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_flux2(const CtxT<R> c, const StateDev s) {
+  const Bid bid = bid_here();
+  KIDX_ZFAST(c.G.nx / 2, c.G.ny, c.G.nl);
+  const int k = 2 * jk - 1;
+  if (k > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  __builtin_assume(k >= 1 && k < c.G.nx);
+  diff_flux_at(c, s, ji, XP{k}, jl);
 }
 
 template <class R>
@@ -748,6 +769,7 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
   template __global__ void k_diff_dual<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_mom<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
+  template __global__ void k_diff_flux2<R>(const CtxT<R>, const StateDev);                 \
   template __global__ void k_diff_div<R>(const CtxT<R>);                                   \
   template __global__ void k_sfc_turb<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);                   \
