@@ -375,7 +375,10 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux2, nx / 2, ny, nl, c, old);
       else
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux, nx, ny, nl, c, old);
-      LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
+      if (nx % 2 == 0)
+        LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div2, nx / 2, ny, nl, c);
+      else
+        LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
     }
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)

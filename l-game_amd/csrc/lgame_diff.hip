@@ -427,29 +427,30 @@ __global__ void __launch_bounds__(BX * BY) k_diff_flux2(const CtxT<R> c, const S
   diff_flux_at(c, s, ji, XP{k}, jl);
 }
 
-template <class R>
-__device__ __forceinline__ void b_diff_div(const CtxT<R> &c, const Bid bid) {
-  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+// the divergences of one cell (jk an int) or of the columns k, k+1 (jk an XP)
+template <class R, class X>
+__device__ __forceinline__ void diff_div_at(const CtxT<R> &c, int ji, X jk, int jl) {
+  using V = VT<R, X>;
   const Geom &G = c.G;
   const int nl = G.nl, nflat = G.nflat;
-  if (jk > G.nx || ji > G.ny || jl > nl) return;
-  const ix_t i = LG_IX(G, ji, jk, jl);
+  const auto ke = east(jk);
   // which interfaces carry a contravariant correction, and which layers those need (as in
   // the scalar divergence): vcc at interface L >= nflat+1 uses layer L, and layer L-1 unless L = nflat+1
   const bool up = jl > nflat;
   const bool lo = jl >= nflat && jl < nl;
   const bool lay_m = up && jl != nflat + 1;
   const bool lay_c = up || (lo && jl != nflat);
-  VccGeo<R> gm{}, gc{}, gp{};
+  VccGeo<V> gm{}, gc{}, gp{};
   if (lay_m) gm = vcc_geo(c, ji, jk, jl - 1);
   if (lay_c) gc = vcc_geo(c, ji, jk, jl);
   if (lo) gp = vcc_geo(c, ji, jk, jl + 1);
-  const R ax1 = AT(c.g.area_x, ji, jk + 1, jl), ay0 = c.g.area_y[i], ax0 = c.g.area_x[i], ay1 = AT(c.g.area_y, ji + 1, jk, jl);
-  const R az0 = c.g.area_z[i], az1 = AT(c.g.area_z, ji, jk, jl + 1), V = c.g.volume[i];
+  const V ax1 = LDG(c.g.area_x, ji, ke, jl), ay0 = LDG(c.g.area_y, ji, jk, jl), ax0 = LDG(c.g.area_x, ji, jk, jl),
+          ay1 = LDG(c.g.area_y, ji + 1, jk, jl);
+  const V az0 = LDG(c.g.area_z, ji, jk, jl), az1 = LDG(c.g.area_z, ji, jk, jl + 1), Vol = LDG(c.g.volume, ji, jk, jl);
   auto one = [&](int q, Fld<R> out, bool lvert) {
     const Fld<const R> du = c.w.dfu[q], dv = c.w.dfv[q];
-    const Faces<R> fc = faces<R>(du, dv, G, ji, jk, jl);
-    R cu = R(0.0), cl = R(0.0);
+    const Faces<V> fc = faces<R>(du, dv, G, ji, jk, jl);
+    V cu = R(0.0), cl = R(0.0);
     if (up) {
       if (lay_m) cu = vcc_sub(cu, gm, faces<R>(du, dv, G, ji, jk, jl - 1));
       cu = vcc_sub(cu, gc, fc);
@@ -458,8 +459,8 @@ __device__ __forceinline__ void b_diff_div(const CtxT<R> &c, const Bid bid) {
       if (jl != nflat) cl = vcc_sub(cl, gc, fc);
       cl = vcc_sub(cl, gp, faces<R>(du, dv, G, ji, jk, jl + 1));
     }
-    const R comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
-    R comp_v = R(0.0);
+    const V comp_h = fc.e * ax1 + fc.n * ay0 - fc.w * ax0 - fc.s * ay1;
+    V comp_v = R(0.0);
     if (jl == nflat) {
       comp_v = -cl * az1;
     } else if (jl == nl) {
@@ -467,21 +468,39 @@ __device__ __forceinline__ void b_diff_div(const CtxT<R> &c, const Bid bid) {
     } else if (jl > nflat) {
       comp_v = cu * az0 - cl * az1;
     }
-    R d = (comp_h + comp_v) / V;
+    V d = (comp_h + comp_v) / Vol;
     if (lvert) {
-      const R wu = jl == 1 ? R(0.0) : c.w.dfw[q][i];
-      const R wl = jl == nl ? R(0.0) : AT(c.w.dfw[q], ji, jk, jl + 1);
-      d = d + (wu * az0 - wl * az1) / V;
+      V wu = R(0.0), wl = R(0.0);
+      if (jl != 1) wu = LDG(c.w.dfw[q], ji, jk, jl);
+      if (jl != nl) wl = LDG(c.w.dfw[q], ji, jk, jl + 1);
+      d = d + (wu * az0 - wl * az1) / Vol;
     }
-    out[i] = d;
+    stv(out, G, ji, jk, jl, d);
   };
   if (c.C.ltemp_h) one(0, c.w.tdh, c.C.ltemp_v != 0);
   if (c.C.lmass_h)
     for (int jc = 0; jc < G.nc; ++jc) one(jc + 1, c.w.mdt[jc], c.C.lmass_v != 0);
 }
+
+template <class R>
+__device__ __forceinline__ void b_diff_div(const CtxT<R> &c, const Bid bid) {
+  KIDX_ZFAST(c.G.nx, c.G.ny, c.G.nl);
+  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  diff_div_at(c, ji, jk, jl);
+}
 template <class R>
 __global__ void __launch_bounds__(BX * BY) k_diff_div(const CtxT<R> c) {
   b_diff_div(c, bid_here());
+}
+// nx even: the columns k = 2*jk-1 and k+1 per thread
+template <class R>
+__global__ void __launch_bounds__(BX * BY) k_diff_div2(const CtxT<R> c) {
+  const Bid bid = bid_here();
+  KIDX_ZFAST(c.G.nx / 2, c.G.ny, c.G.nl);
+  const int k = 2 * jk - 1;
+  if (k > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
+  __builtin_assume(k >= 1 && k < c.G.nx);
+  diff_div_at(c, ji, XP{k}, jl);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -771,6 +790,7 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_flux2<R>(const CtxT<R>, const StateDev);                 \
   template __global__ void k_diff_div<R>(const CtxT<R>);                                   \
+  template __global__ void k_diff_div2<R>(const CtxT<R>);                                  \
   template __global__ void k_sfc_turb<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_surface<R>(const CtxT<R>, const StateDev, const StateDev, int);

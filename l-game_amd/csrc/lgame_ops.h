@@ -485,32 +485,44 @@ __device__ __forceinline__ VT<R, X> rhog(const CtxT<R> &c, const StateDev &s, in
 
 // ---- vertical_contravariant_corr of flux fields at the faces of one cell -----------------------
 // slopes at the four faces of cell (ji, jk) and its inner-product weights 0..3, layer l
-template <class R>
+template <class V>
 struct VccGeo {
-  R sx1, sy0, sx0, sy1, w0, w1, w2, w3;
+  V sx1, sy0, sx0, sy1, w0, w1, w2, w3;
 };
-template <class R>
-__device__ __forceinline__ VccGeo<R> vcc_geo(const CtxT<R> &c, int ji, int jk, int l) {
-  return VccGeo<R>{AT(c.g.slope_x, ji, jk + 1, l), AT(c.g.slope_y, ji, jk, l),    AT(c.g.slope_x, ji, jk, l),
-                   AT(c.g.slope_y, ji + 1, jk, l), AT(c.g.ipw[0], ji, jk, l),     AT(c.g.ipw[1], ji, jk, l),
-                   AT(c.g.ipw[2], ji, jk, l),      AT(c.g.ipw[3], ji, jk, l)};
+template <class R, class X>
+__device__ __forceinline__ VccGeo<VT<R, X>> vcc_geo(const CtxT<R> &c, int ji, X jk, int l) {
+  const auto ke = east(jk);
+  return VccGeo<VT<R, X>>{LDG(c.g.slope_x, ji, ke, l), LDG(c.g.slope_y, ji, jk, l),     LDG(c.g.slope_x, ji, jk, l),
+                          LDG(c.g.slope_y, ji + 1, jk, l), LDG(c.g.ipw[0], ji, jk, l), LDG(c.g.ipw[1], ji, jk, l),
+                          LDG(c.g.ipw[2], ji, jk, l),      LDG(c.g.ipw[3], ji, jk, l)};
 }
 
 // a vector field at the four faces of cell (ji, jk): x at jk+1, y at ji, x at jk, y at ji+1
-template <class R>
+template <class V>
 struct Faces {
-  R e, n, w, s;
+  V e, n, w, s;
 };
-template <class R, class P>
-__device__ __forceinline__ Faces<R> faces(P vx, P vy, const Geom &G, int ji, int jk, int l) {
-  return Faces<R>{R(vx[LG_IX(G, ji, jk + 1, l)]), R(vy[LG_IX(G, ji, jk, l)]), R(vx[LG_IX(G, ji, jk, l)]),
-                  R(vy[LG_IX(G, ji + 1, jk, l)])};
+template <class R, class P, class X>
+__device__ __forceinline__ Faces<VT<R, X>> faces(P vx, P vy, const Geom &G, int ji, X jk, int l) {
+  return Faces<VT<R, X>>{cvt<R>(ldv(vx, G, ji, east(jk), l)), cvt<R>(ldv(vy, G, ji, jk, l)), cvt<R>(ldv(vx, G, ji, jk, l)),
+                         cvt<R>(ldv(vy, G, ji + 1, jk, l))};
 }
 
-// r minus the four terms of one layer of vertical_contravariant_corr (mo_averaging.F90:29-50)
+// the element type of a value or a pair
+template <class V>
+struct ElemOf {
+  using type = V;
+};
 template <class R>
-__device__ __forceinline__ R vcc_sub(R r, const VccGeo<R> &g, const Faces<R> &f) {
-  return r - R(0.5) * f.e * g.sx1 * g.w0 - R(0.5) * f.n * g.sy0 * g.w1 - R(0.5) * f.w * g.sx0 * g.w2 - R(0.5) * f.s * g.sy1 * g.w3;
+struct ElemOf<P2<R>> {
+  using type = R;
+};
+
+// r minus the four terms of one layer of vertical_contravariant_corr (mo_averaging.F90:29-50)
+template <class V>
+__device__ __forceinline__ V vcc_sub(V r, const VccGeo<V> &g, const Faces<V> &f) {
+  using E = typename ElemOf<V>::type;
+  return r - E(0.5) * f.e * g.sx1 * g.w0 - E(0.5) * f.n * g.sy0 * g.w1 - E(0.5) * f.w * g.sx0 * g.w2 - E(0.5) * f.s * g.sy1 * g.w3;
 }
 
 }  // namespace lgame

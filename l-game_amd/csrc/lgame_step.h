@@ -31,6 +31,7 @@ template <class R> __global__ void k_diff_mom(const CtxT<R> c, const StateDev s)
 template <class R> __global__ void k_diff_flux(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_diff_flux2(const CtxT<R> c, const StateDev s);  // nx even: two cells per thread
 template <class R> __global__ void k_diff_div(const CtxT<R> c);
+template <class R> __global__ void k_diff_div2(const CtxT<R> c);  // nx even: two cells per thread
 template <class R> __global__ void k_sfc_turb(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_pbl_wind(const CtxT<R> c, const StateDev s);
 template <class R> __global__ void k_surface(const CtxT<R> c, const StateDev old, const StateDev nw, int rk);
