@@ -289,6 +289,10 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const bool o3 = c.C.order == 3;
   const bool pk = prof && prof->kernels;
+  // the diffusion coefficients and divergence take two cells per thread in the FP64 form only: in
+  // the FP32 form the pair kernels were slower (0.80 against 1.06 and 0.72 against 0.86 ms per
+  // step at 512x512x60)
+  const bool pair64 = sizeof(R) == 8;
   // a stencil kernel on a (64 x by) block grid covering (ex, ey, ez) points, by from the tile table
 #define LG_LAUNCH(KI, kern, ex, ey, ez, ...)                                        \
   do {                                                                              \
@@ -351,7 +355,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
       LG_LAUNCH_ZFAST(KI_DIFF_ZETA, k_diff_zeta, nx + 1, ny + 1, nl, c, old);
-      if (nx % 2 == 0)
+      if (nx % 2 == 0 && pair64)
         LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef2, nx / 2, ny, nl, c, old);
       else
         LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
@@ -375,7 +379,7 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux2, nx / 2, ny, nl, c, old);
       else
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux, nx, ny, nl, c, old);
-      if (nx % 2 == 0)
+      if (nx % 2 == 0 && pair64)
         LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div2, nx / 2, ny, nl, c);
       else
         LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
