@@ -1516,11 +1516,16 @@ int efail(lgame_ensemble *e, int code, const char *fmt, ...) {
     if (e_ != hipSuccess) return efail(e, LGAME_EHIP, "%s: %s", #call, hipGetErrorString(e_));  \
   } while (0)
 
+// A member's step as a branch of the ensemble's graph.  The branches run concurrently and fill the
+// GPU together, so the vertical solver takes its one-pass form (the three-pass form of a small
+// domain alone reads its inputs twice: the 48-member ensemble step took 74.2 instead of 71.8 ms).
 void enqueue_member(lgame_handle *h, int a, int b, int step, const RadCall *rad) {
+  Tiles t = h->tiles;
+  t.ver_passes = 1;
   if (h->mixed)
-    enqueue_pchevi(h->cf, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, h->tiles);
+    enqueue_pchevi(h->cf, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, t);
   else
-    enqueue_pchevi(h->c, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, h->tiles);
+    enqueue_pchevi(h->c, h->slot[a], h->slot[b], step, h->stream, nullptr, rad, t);
 }
 
 // all members' steps of one parity as parallel branches of one graph (fork/join over the
