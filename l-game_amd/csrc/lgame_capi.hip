@@ -289,10 +289,6 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
   const int nx = c.G.nx, ny = c.G.ny, nl = c.G.nl, nlev = c.G.nlev;
   const bool o3 = c.C.order == 3;
   const bool pk = prof && prof->kernels;
-  // the diffusion coefficients and divergence take two cells per thread in the FP64 form only: in
-  // the FP32 form the pair kernels were slower (0.80 against 1.06 and 0.72 against 0.86 ms per
-  // step at 512x512x60)
-  const bool pair64 = sizeof(R) == 8;
   // a stencil kernel on a (64 x by) block grid covering (ex, ey, ez) points, by from the tile table
 #define LG_LAUNCH(KI, kern, ex, ey, ez, ...)                                        \
   do {                                                                              \
@@ -355,10 +351,17 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // momentum diffusion and dissipation, first substep only (mo_vector_tend_expl.F90:101-126)
     if (rk == 1 && c.C.lmom_h) {
       LG_LAUNCH_ZFAST(KI_DIFF_ZETA, k_diff_zeta, nx + 1, ny + 1, nl, c, old);
-      if (nx % 2 == 0 && pair64)
-        LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef2, nx / 2, ny, nl, c, old);
-      else
+      // the coefficients and the divergence take two cells per thread in the FP64 form only: in the
+      // FP32 form the pair kernels were slower (0.80 against 1.06 and 0.72 against 0.86 ms per step
+      // at 512x512x60)
+      if constexpr (sizeof(R) == 8) {
+        if (nx % 2 == 0)
+          LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef2, nx / 2, ny, nl, c, old);
+        else
+          LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
+      } else {
         LG_LAUNCH_ZFAST(KI_DIFF_COEF, k_diff_coef, nx, ny, nl, c, old);
+      }
       LG_LAUNCH(KI_DIFF_DUAL, k_diff_dual, nx, ny, nl, c, old);
       LG_LAUNCH_ZFAST(KI_DIFF_MOM, k_diff_mom, nx, ny, nl, c, old);
     }
@@ -379,10 +382,14 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux2, nx / 2, ny, nl, c, old);
       else
         LG_LAUNCH_ZFAST(KI_DIFF_FLUX, k_diff_flux, nx, ny, nl, c, old);
-      if (nx % 2 == 0 && pair64)
-        LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div2, nx / 2, ny, nl, c);
-      else
+      if constexpr (sizeof(R) == 8) {
+        if (nx % 2 == 0)
+          LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div2, nx / 2, ny, nl, c);
+        else
+          LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
+      } else {
         LG_LAUNCH_ZFAST(KI_DIFF_DIV, k_diff_div, nx, ny, nl, c);
+      }
     }
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)

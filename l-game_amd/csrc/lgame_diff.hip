@@ -784,19 +784,20 @@ __global__ void __launch_bounds__(BX * BY) k_surface(const CtxT<R> c, const Stat
 #define LG_INSTANTIATE(R)                                                                  \
   template __global__ void k_diff_zeta<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_coef<R>(const CtxT<R>, const StateDev);                  \
-  template __global__ void k_diff_coef2<R>(const CtxT<R>, const StateDev);                 \
   template __global__ void k_diff_dual<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_mom<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_diff_flux<R>(const CtxT<R>, const StateDev);                  \
   template __global__ void k_diff_flux2<R>(const CtxT<R>, const StateDev);                 \
   template __global__ void k_diff_div<R>(const CtxT<R>);                                   \
-  template __global__ void k_diff_div2<R>(const CtxT<R>);                                  \
   template __global__ void k_sfc_turb<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_pbl_wind<R>(const CtxT<R>, const StateDev);                   \
   template __global__ void k_surface<R>(const CtxT<R>, const StateDev, const StateDev, int);
 LG_INSTANTIATE(double)
 LG_INSTANTIATE(float)
 #undef LG_INSTANTIATE
+// the FP64 form only (the FP32 form keeps one cell per thread in these two)
+template __global__ void k_diff_coef2<double>(const CtxT<double>, const StateDev);
+template __global__ void k_diff_div2<double>(const CtxT<double>);
 
 // ---- the same kernels over a group of domains (one launch per phase) ------------------------
 LG_BATCH_KERNEL(diff_zeta, BX * BY, PH_DIFF_ZETA, b_diff_zeta(d.c, d.s[a.old_slot], bid))
