@@ -92,7 +92,7 @@ KERNEL_TRAFFIC = {("moist512", "solve_ver", "fp64"): 4.369e+09, ("moist512", "so
                   ("moist512", "solve_gen", "fp64"): 5.652e+09, ("moist512", "solve_gen", "mixed"): 5.263e+09}
 # measured HBM bytes per grid-point step of the whole FP64 step (same counters, all kernels)
 STEP_TRAFFIC = {"moist512": 3489.0}
-# the reference's own CPU path, flang -O2 -fopenmp on 8 cores (the survey's measurement; the
+# the reference's own CPU path, flang -O2 -fopenmp on 8 cores, measured on an 8-core Xeon host (the
 # reference tree does not travel, so it cannot be timed beside the GPU)
 REFERENCE_CPU = {"dry128": {"value": 3.19e6, "grid": "129x129x50 dry", "cores": 8},
                  "moist512": {"value": 2.05e6, "grid": "513x513x60 dry, no radiation", "cores": 8},

@@ -5,7 +5,7 @@ The glue in question (oracle/lgame_oracle.c; mo_vector_tend_expl, mo_momentum_di
 mo_manage_pchevi do not build here, see oracle/ref_harness.F90):
 * the step sequence of pchevi (mo_manage_pchevi.F90:27-118) and the tendency combination of
   vector_tend_expl (mo_vector_tend_expl.F90:58-180): the budgets of the reference program's own
-  Schaer run (the whole unmodified control.F90, recorded when the reference was surveyed), the
+  Schaer run (the whole unmodified control.F90, recorded from a flang build of the reference), the
   pressure-gradient extrapolation weights derived from c_v/c_p and applied to the compiled
   manage_p_grad, the inertial oscillation of a uniform flow on an f-plane (the predictor-corrector
   recurrence of the averaged, lagged Coriolis term), and the energy neutrality of the vorticity
@@ -36,7 +36,7 @@ C_D_V, C_D_P = 717.942189, 1005.0
 def test_oracle_schaer_run_matches_reference_program_budgets():
     """The Schaer case on the 201 x 3 x 40 plane for 1,119 steps of 0.805 s (a 10-minute span plus
     the time loop's 300 s, control.F90:507), as the whole reference program (control.F90 with
-    every step module) ran it when it was surveyed: the dry mass went from 1492336736843.482 to
+    every step module, built with flang) ran it: the dry mass went from 1492336736843.482 to
     1492336736843.394, the rhotheta_v integral drifted +2.7e-9 and the total energy -2.3e-8
     relative (write_out_integrals, mo_write_out.F90:226-304, 8 OpenMP threads).  The oracle must
     reproduce these to their printed precision; the last digit of the final mass may differ by the
