@@ -482,7 +482,8 @@ int lgame_get_segments(lgame_handle *h, int *nseg);
  * pass per column (one lane a column) or in three passes: the rows of the system and the new
  * state by one thread per (column, level), the Thomas recursion alone per column.  lgame_init
  * picks three passes when the domain has fewer than 1024 wavefronts of 64 columns (fewer than the
- * GPU's SIMDs) and at least 3 layers, one pass otherwise; lgame_set_column_passes (1 or 3) sets it
+ * GPU's SIMDs) and at least 3 layers, one pass otherwise; lgame_autotune times both on domains of
+ * fewer than 2048 such wavefronts and keeps the faster; lgame_set_column_passes (1 or 3) sets it
  * and lgame_get_column_passes reads it.  Both give the same bits.  No reference counterpart. */
 int lgame_set_column_passes(lgame_handle *h, int passes);
 int lgame_get_column_passes(lgame_handle *h, int *passes);

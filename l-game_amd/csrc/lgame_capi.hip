@@ -1368,8 +1368,9 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
     best_ms[k] = 1e300;
     best_by[k] = BY;
   }
-  // run `steps` steps with the current tile table and sum each kernel's time
-  auto measure = [&](double *ms) {
+  // run `steps` steps with the current tile table and sum each kernel's time (and the vertical
+  // solver's, when solver_ms is given)
+  auto measure = [&](double *ms, double *solver_ms = nullptr) {
     int rc = 0;
     h->ev_marks.clear();
     h->ev_next = 0;
@@ -1388,10 +1389,15 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
     }
     if (rc || ep.failed || hipStreamSynchronize(h->stream) != hipSuccess) return rc ? rc : LGAME_EHIP;
     for (int k = 0; k < KI_COUNT; ++k) ms[k] = 0.0;
+    if (solver_ms) *solver_ms = 0.0;
     for (auto &m : h->ev_marks) {
       const int g = m.first - PROF_KERNEL0;
-      if (g < 0 || g >= KI_COUNT) continue;
       float t = 0.0f;
+      if (m.first == 0 && solver_ms) {
+        if (hipEventElapsedTime(&t, h->ev_pool[m.second.first], h->ev_pool[m.second.second]) == hipSuccess) *solver_ms += t;
+        continue;
+      }
+      if (g < 0 || g >= KI_COUNT) continue;
       if (hipEventElapsedTime(&t, h->ev_pool[m.second.first], h->ev_pool[m.second.second]) == hipSuccess) ms[g] += t;
     }
     h->ev_marks.clear();
@@ -1423,6 +1429,22 @@ int lgame_autotune(lgame_handle *h, int steps, int *rows) {
     }
   }
   h->tiles.seg = best_seg;
+  // the vertical solver in one or three passes, on domains near the size where they trade places
+  if (!rc && h->c.G.nl >= 3 && col_waves(h->c.G) < 2 * VER_SPLIT_WAVES) {
+    int best_p = h->tiles.ver_passes;
+    double best_p_ms = 1e300;
+    for (int p : {1, 3}) {
+      if (rc) break;
+      h->tiles.ver_passes = p;
+      double sv = 0.0;
+      rc = measure(ms, &sv);
+      if (!rc && sv > 0.0 && sv < best_p_ms) {
+        best_p_ms = sv;
+        best_p = p;
+      }
+    }
+    h->tiles.ver_passes = best_p;
+  }
   // back to the state before tuning: both slots, and zero carried work arrays, as at lgame_init
   hipError_t e = hipMemcpyAsync(h->slab, backup, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->slab + h->work_off, 0, h->work_bytes, h->stream);
