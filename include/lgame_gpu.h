@@ -464,9 +464,9 @@ int lgame_profile_reset(lgame_handle *h);
  * mass_flux, vort_flux, ke, ke_grad, dissipation, vec_tend, scalar_flux, scalar_div, diff_zeta,
  * diff_coef, diff_dual, diff_mom, diff_flux, diff_div (the last six run only with diffusion).
  * The scalar fluxes are formed inside the scalar divergence kernel, the mass fluxes and the
- * kinetic energy inside the potential-vorticity kernel and the kinetic-energy gradient inside the
- * vorticity-flux kernel, so the mass_flux, scalar_flux, ke and ke_grad entries are kept for the
- * order but not used.  The pgrad entry is the first substep's
+ * kinetic energy inside the potential-vorticity kernel, the kinetic-energy gradient inside the
+ * vorticity-flux kernel and the full theta_v of the third-order theta where it is loaded, so the
+ * theta_fill, mass_flux, scalar_flux, ke and ke_grad entries are kept for the order but not used.  The pgrad entry is the first substep's
  * pressure gradient together with its wind update, vec_tend the second substep's wind update.
  * The scalar divergence kernel marches down the columns; its
  * levels can be cut into 1, 2, 4 or 8 segments (at least 4 levels each) so that small domains

@@ -19,7 +19,7 @@ namespace lgame {
 // the second substep's vorticity flux term with its wind update (b_vort_flux_vt); PH_VEC_TEND is the
 // second substep's wind update alone (runs without momentum advection).
 enum Phase : int {
-  PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
+  PH_TEMPERATURE, PH_SFC_TURB, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE, PH_PGRAD, PH_POT_VORT,
   PH_VORT_FLUX, PH_DIFF_ZETA, PH_DIFF_COEF, PH_DIFF_DUAL, PH_DIFF_MOM, PH_PBL_WIND,
   PH_DISSIPATION, PH_VEC_TEND, PH_DIFF_FLUX, PH_DIFF_DIV, PH_SCALAR, PH_SURFACE, PH_SOLVE_VER, PH_SOLVE_GEN, PH_RAD,
   PH_VORT_FLUX_VT,
@@ -31,7 +31,6 @@ inline int phase_tile(int ph) {
   switch (ph) {
     case PH_TEMPERATURE: return KI_TEMPERATURE;
     case PH_SFC_TURB: case PH_PBL_WIND: case PH_SURFACE: case PH_DIFF_DUAL: return KI_DIFF_DUAL;
-    case PH_THETA_FILL: return KI_THETA_FILL;
     case PH_THETA_GRAD: return KI_THETA_GRAD;
     case PH_THETA_DIV: return KI_THETA_DIV;
     case PH_THETA_EDGE: return KI_THETA_EDGE;
@@ -78,7 +77,7 @@ __host__ __device__ inline void phase_grid(int ph, const Geom &G, int by, int se
   bool flat = false;  // a 1-D grid (tile_zfast kernels)
   gx = gy = gz = 0;
   switch (ph) {
-    case PH_TEMPERATURE: case PH_THETA_FILL: case PH_THETA_DIV: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
+    case PH_TEMPERATURE: case PH_THETA_DIV: case PH_DIFF_DUAL: case PH_DISSIPATION: break;
     case PH_SFC_TURB: case PH_PBL_WIND: case PH_SURFACE: ez = 1; break;
     case PH_THETA_GRAD: case PH_THETA_EDGE: case PH_VEC_TEND: ex = nx + 1; ey = ny + 1; break;
     case PH_PGRAD: case PH_VORT_FLUX: case PH_VORT_FLUX_VT: case PH_DIFF_COEF: case PH_DIFF_MOM: case PH_DIFF_FLUX: case PH_DIFF_DIV: flat = true; break;

@@ -324,10 +324,9 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     if (rk == 1) {
       // manage_p_grad (mo_p_grad.F90:20-104); theta_v_adv_3rd_order of theta_v and theta_v_pert
       if (o3) {
-        LG_LAUNCH(KI_THETA_FILL, k_theta_fill, nx, ny, nl, c, old, 2);
-        LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, 2);
+        LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, old, 2);
         LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 2);
-        LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(old.wind_u), Fld<const double>(old.wind_v), 2);
+        LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, old, 2);
       }
     }
     // vector_tend_expl (mo_vector_tend_expl.F90:30-182)
@@ -394,10 +393,9 @@ int enqueue_pchevi(const CtxT<R> &c, const StateDev &old, const StateDev &nw, in
     // scalar_tend_expl (mo_scalar_tend_expl.F90:24-181); at rk 1 the theta_v edge values of the
     // pressure gradient are reused (same state, same winds)
     if (o3 && rk == 2) {
-      LG_LAUNCH(KI_THETA_FILL, k_theta_fill, nx, ny, nl, c, nw, 1);
-      LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, 1);
+      LG_LAUNCH(KI_THETA_GRAD, k_theta_grad, nx + 1, ny + 1, nl, c, nw, 1);
       LG_LAUNCH(KI_THETA_DIV, k_theta_div, nx, ny, nl, c, 1);
-      LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, Fld<const double>(nw.wind_u), Fld<const double>(nw.wind_v), 1);
+      LG_LAUNCH(KI_THETA_EDGE, k_theta_edge, nx + 1, ny + 1, nl, c, nw, 1);
     }
     if (pk) prof->begin(PROF_KERNEL0 + KI_SCALAR_DIV);
     launch_scalar_col(c, S, nw, rk, tiles.by[KI_SCALAR_DIV], tiles.seg, st);
@@ -714,7 +712,7 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
   const size_t fs = (size_t)G.fsize;
   const size_t nstate = (size_t)2 * (nc + 6 + (C.lsoil ? 1 : 0));
   const size_t ndiff = diffusion ? (size_t)14 + nc + 3 * (nc + 1) : 0;  // the diffusion and surface fields of WorkT
-  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 12 + 2 + ndiff;
+  const size_t nwork = (size_t)nc + 4 + 8 + 6 + 4 + 7 + 10 + 2 + ndiff;
   const size_t ncol = 13;  // 12 column-scratch fields and the cloud-water sink velocity
   const size_t wbytes = mixed ? sizeof(float) : sizeof(double);
   h->slab_bytes = (nstate + NGRID + ncol) * fs * sizeof(double) + nwork * fs * wbytes + (mixed ? NGRID * fs * sizeof(float) : 0);
@@ -786,7 +784,6 @@ int lgame_init(const lgame_dims *dims, const lgame_cfg *cfg, const lgame_grid_vi
     W.w_ph = take();
     W.v_squared = take();
     for (int q = 0; q < 2; ++q) {
-      W.tth[q] = take();
       W.tgu[q] = take();
       W.tgv[q] = take();
       W.tfd[q] = take();
@@ -1784,7 +1781,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
         BatchArgs t = a;
         t.state_slot = a_slot;
         t.nfield = 2;
-        for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
+        for (int ph : {PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
       }
     }
     bool winds_done = false;
@@ -1810,7 +1807,7 @@ int enqueue_group(lgame_ensemble *e, BatchGroup &g, int a_slot, int b_slot, int 
       BatchArgs t = a;
       t.state_slot = b_slot;
       t.nfield = 1;
-      for (int ph : {PH_THETA_FILL, PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
+      for (int ph : {PH_THETA_GRAD, PH_THETA_DIV, PH_THETA_EDGE}) launch(ph, t);
     }
     launch(PH_SCALAR, a);
     if (C.lsoil || C.lsens) launch(PH_SURFACE, a);

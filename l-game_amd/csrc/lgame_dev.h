@@ -103,7 +103,7 @@ struct WorkT {
   Fld<R> vsg_x, vsg_y, vsg_z, pvt_x, pvt_y, pvt_z;
   Fld<R> heating_diss, csh, temperature, radiation_tendency;
   Fld<R> eta_x, eta_y, eta_z, u_ph, v_ph, w_ph, v_squared;
-  Fld<R> tth[2], tgu[2], tgv[2], tfd[2], tu[2], tv[2];  // theta_v_adv_3rd_order: full, pert
+  Fld<R> tgu[2], tgv[2], tfd[2], tu[2], tv[2];  // theta_v_adv_3rd_order: full, pert
   Fld<R> decel, pgc;  // per cell of the old state: C_D_P*rho_gas/rho_sum and rho_gas/rho_sum - 1
   Fld<double> col[12];                                  // column-solver per-level scratch (FP64)
   Fld<double> vfall_cw;  // sink velocity of cloud water per layer (moist), with the temperature

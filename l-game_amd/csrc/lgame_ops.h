@@ -103,6 +103,18 @@ __device__ __forceinline__ P2<typename std::remove_cv<T>::type> ldv(Fld<T> p, co
   using E = typename std::remove_cv<T>::type;
   return P2<E>(p[LG_IX(G, ji, x.k0, jl)], p[LG_IX(G, ji, x.k1, jl)]);
 }
+// theta_v = theta_v_bg + theta_v_pert of a state, formed where it is loaded (the full theta_v of
+// theta_v_adv_3rd_order, mo_multiplications.F90:116-204, without a stored copy)
+template <class R>
+struct ThetaFull {
+  Fld<const R> bg;
+  Fld<const double> pert;
+};
+template <class R>
+__device__ __forceinline__ R ldv(ThetaFull<R> p, const Geom &G, int ji, int k, int jl) {
+  const ix_t i = LG_IX(G, ji, k, jl);
+  return p.bg[i] + R(p.pert[i]);
+}
 // a load at one level per element
 template <class T>
 __device__ __forceinline__ typename std::remove_cv<T>::type ldv_l(Fld<T> p, const Geom &G, int ji, int k, int jl) {

@@ -7,10 +7,9 @@
 namespace lgame {
 
 template <class R> __global__ void k_temperature(const CtxT<R> c, const StateDev s);
-template <class R> __global__ void k_theta_fill(const CtxT<R> c, const StateDev s, int nfield);
-template <class R> __global__ void k_theta_grad(const CtxT<R> c, int nfield);
+template <class R> __global__ void k_theta_grad(const CtxT<R> c, const StateDev s, int nfield);
 template <class R> __global__ void k_theta_div(const CtxT<R> c, int nfield);
-template <class R> __global__ void k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield);
+template <class R> __global__ void k_theta_edge(const CtxT<R> c, const StateDev s, int nfield);
 // the potential vorticity, the mass fluxes and (with_ke) the kinetic energy
 template <class R> __global__ void k_pot_vort(const CtxT<R> c, const StateDev s, int with_ke);
 template <class R> __global__ void k_pot_vort2(const CtxT<R> c, const StateDev s, int with_ke);  // nx even: two cells per thread
@@ -43,7 +42,8 @@ constexpr int BY = 4;
 
 // stencil kernels of the step, the index of their tile-table entry.  The mass fluxes and the
 // kinetic energy are formed in the potential-vorticity kernel, the kinetic-energy gradient in the
-// vorticity-flux kernel and the scalar fluxes in the scalar column kernel: KI_MASS_FLUX, KI_KE,
+// vorticity-flux kernel, the scalar fluxes in the scalar column kernel and the full theta_v of the
+// third-order theta where the gradient and edge kernels load it: KI_THETA_FILL, KI_MASS_FLUX, KI_KE,
 // KI_KE_GRAD and KI_SCALAR_FLUX keep their place in the table (and the C-ABI's order) unused.
 enum KernelId {
   KI_TEMPERATURE, KI_THETA_FILL, KI_THETA_GRAD, KI_THETA_DIV, KI_THETA_EDGE, KI_PGRAD, KI_POT_VORT, KI_MASS_FLUX,

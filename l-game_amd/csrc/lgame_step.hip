@@ -46,33 +46,25 @@ __global__ void __launch_bounds__(BX * BY) k_temperature(const CtxT<R> c, const 
 // field f: 0 = full theta_v (theta_v_bg + theta_v_pert), 1 = theta_v_pert.
 // ---------------------------------------------------------------------------------------------
 // pass 1a: scalar_placeholder = theta_v_bg + theta_v_pert (f=0) or theta_v_pert (f=1)
+// pass 1: u,v_placeholder = grad_hor(theta) with its grad_vert inline, theta being the full
+// theta_v of state s (field 0) or its theta_v_pert (field 1), formed at every load
 template <class R>
-__device__ __forceinline__ void b_theta_fill(const CtxT<R> &c, const StateDev &s, int nfield, const Bid bid) {
-  KIDX;
-  if (jk > c.G.nx || ji > c.G.ny || jl > c.G.nl) return;
-  const ix_t i = LG_IX(c.G, ji, jk, jl);
-  c.w.tth[0][i] = c.g.theta_v_bg[i] + R(s.theta_v_pert[i]);
-  if (nfield > 1) c.w.tth[1][i] = R(s.theta_v_pert[i]);
-}
-template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_fill(const CtxT<R> c, const StateDev s, int nfield) {
-  b_theta_fill(c, s, nfield, bid_here());
-}
-
-// pass 1b: u,v_placeholder = grad_hor(theta) with its grad_vert inline
-template <class R>
-__device__ __forceinline__ void b_theta_grad(const CtxT<R> &c, int nfield, const Bid bid) {
+__device__ __forceinline__ void b_theta_grad(const CtxT<R> &c, const StateDev &s, int nfield, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny;
   if (jk > nx + 1 || ji > ny + 1 || jl > c.G.nl) return;
-  for (int f = 0; f < nfield; ++f) {
-    if (ji <= ny) AT(c.w.tgu[f], ji, jk, jl) = grad_hor_x(c, c.w.tth[f], ji, jk, jl);
-    if (jk <= nx) AT(c.w.tgv[f], ji, jk, jl) = grad_hor_y(c, c.w.tth[f], ji, jk, jl);
+  const ThetaFull<R> full{c.g.theta_v_bg, Fld<const double>(s.theta_v_pert)};
+  const Fld<const double> pert = s.theta_v_pert;
+  if (ji <= ny) AT(c.w.tgu[0], ji, jk, jl) = grad_hor_x(c, full, ji, jk, jl);
+  if (jk <= nx) AT(c.w.tgv[0], ji, jk, jl) = grad_hor_y(c, full, ji, jk, jl);
+  if (nfield > 1) {
+    if (ji <= ny) AT(c.w.tgu[1], ji, jk, jl) = grad_hor_x(c, pert, ji, jk, jl);
+    if (jk <= nx) AT(c.w.tgv[1], ji, jk, jl) = grad_hor_y(c, pert, ji, jk, jl);
   }
 }
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_grad(const CtxT<R> c, int nfield) {
-  b_theta_grad(c, nfield, bid_here());
+__global__ void __launch_bounds__(BX * BY) k_theta_grad(const CtxT<R> c, const StateDev s, int nfield) {
+  b_theta_grad(c, s, nfield, bid_here());
 }
 
 // pass 2: flux_density_div = div_h(u_placeholder, v_placeholder)
@@ -89,18 +81,22 @@ __global__ void __launch_bounds__(BX * BY) k_theta_div(const CtxT<R> c, int nfie
 
 // pass 3: upwind-corrected edge values; the upwind direction comes from the given winds
 template <class R>
-__device__ __forceinline__ void b_theta_edge(const CtxT<R> &c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield, const Bid bid) {
+__device__ __forceinline__ void b_theta_edge(const CtxT<R> &c, const StateDev &s, int nfield, const Bid bid) {
   KIDX;
   const int nx = c.G.nx, ny = c.G.ny;
   if (jk > nx + 1 || ji > ny + 1 || jl > c.G.nl) return;
+  const Fld<const double> wind_u = s.wind_u, wind_v = s.wind_v;
   const R c8 = R(1.0) / R(8.0);
+  const ThetaFull<R> full{c.g.theta_v_bg, Fld<const double>(s.theta_v_pert)};
+  const Fld<const double> pert = s.theta_v_pert;
   for (int f = 0; f < nfield; ++f) {
-    const Fld<const R> th = c.w.tth[f];
+    // theta at (a, b): the full theta_v (field 0) or theta_v_pert (field 1)
+    auto th = [&](int a, int b) { return f == 0 ? ldv(full, c.G, a, b, jl) : R(AT(pert, a, b, jl)); };
     const Fld<const R> fd = c.w.tfd[f];
     if (ji <= ny) {
       const int k = (jk == nx + 1) ? 1 : jk;  // theta_v_u(:,nx+1,:) = theta_v_u(:,1,:)
       const int km = wrapm(k, nx);
-      R t = R(0.5) * (AT(th, ji, km, jl) + AT(th, ji, k, jl));
+      R t = R(0.5) * (th(ji, km) + th(ji, k));
       if (ATR(wind_u, ji, k, jl) >= R(0.0))
         t = t - c8 * R(c.C.dyn) * R(c.C.dxn) * AT(fd, ji, km, jl);
       else
@@ -110,7 +106,7 @@ __device__ __forceinline__ void b_theta_edge(const CtxT<R> &c, Fld<const double>
     if (jk <= nx) {
       const int j = (ji == ny + 1) ? 1 : ji;
       const int jm = wrapm(j, ny);
-      R t = R(0.5) * (AT(th, j, jk, jl) + AT(th, jm, jk, jl));
+      R t = R(0.5) * (th(j, jk) + th(jm, jk));
       if (ATR(wind_v, j, jk, jl) >= R(0.0))
         t = t - c8 * R(c.C.dyn) * R(c.C.dxn) * AT(fd, j, jk, jl);
       else
@@ -120,8 +116,8 @@ __device__ __forceinline__ void b_theta_edge(const CtxT<R> &c, Fld<const double>
   }
 }
 template <class R>
-__global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, Fld<const double> wind_u, Fld<const double> wind_v, int nfield) {
-  b_theta_edge(c, wind_u, wind_v, nfield, bid_here());
+__global__ void __launch_bounds__(BX * BY) k_theta_edge(const CtxT<R> c, const StateDev s, int nfield) {
+  b_theta_edge(c, s, nfield, bid_here());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -839,10 +835,9 @@ __global__ void __launch_bounds__(BX * BY) k_vort_flux_vt(const CtxT<R> c, const
 // The plain form (FP64 stencils) and the mixed form (FP32 stencils).
 #define LG_INSTANTIATE(R)                                                                                  \
   template __global__ void k_temperature<R>(const CtxT<R>, const StateDev);                              \
-  template __global__ void k_theta_fill<R>(const CtxT<R>, const StateDev, int);                          \
-  template __global__ void k_theta_grad<R>(const CtxT<R>, int);                                          \
+  template __global__ void k_theta_grad<R>(const CtxT<R>, const StateDev, int);                          \
   template __global__ void k_theta_div<R>(const CtxT<R>, int);                                           \
-  template __global__ void k_theta_edge<R>(const CtxT<R>, Fld<const double>, Fld<const double>, int);   \
+  template __global__ void k_theta_edge<R>(const CtxT<R>, const StateDev, int);                          \
   template __global__ void k_pot_vort<R>(const CtxT<R>, const StateDev, int);                            \
   template __global__ void k_pot_vort2<R>(const CtxT<R>, const StateDev, int);                           \
   template __global__ void k_vort_flux<R>(const CtxT<R>, int);                                           \
@@ -857,12 +852,9 @@ LG_INSTANTIATE(float)
 
 // ---- the same kernels over a group of domains (one launch per phase) ------------------------
 LG_BATCH_KERNEL(temperature, BX * BY, PH_TEMPERATURE, b_temperature(d.c, d.s[a.old_slot], bid))
-LG_BATCH_KERNEL(theta_fill, BX * BY, PH_THETA_FILL, b_theta_fill(d.c, d.s[a.state_slot], a.nfield, bid))
-LG_BATCH_KERNEL(theta_grad, BX * BY, PH_THETA_GRAD, b_theta_grad(d.c, a.nfield, bid))
+LG_BATCH_KERNEL(theta_grad, BX * BY, PH_THETA_GRAD, b_theta_grad(d.c, d.s[a.state_slot], a.nfield, bid))
 LG_BATCH_KERNEL(theta_div, BX * BY, PH_THETA_DIV, b_theta_div(d.c, a.nfield, bid))
-LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE,
-                b_theta_edge(d.c, Fld<const double>(d.s[a.state_slot].wind_u), Fld<const double>(d.s[a.state_slot].wind_v),
-                             a.nfield, bid))
+LG_BATCH_KERNEL(theta_edge, BX * BY, PH_THETA_EDGE, b_theta_edge(d.c, d.s[a.state_slot], a.nfield, bid))
 LG_BATCH_KERNEL(pgrad, BX * BY, PH_PGRAD, b_pgrad_vt(d.c, d.s[a.old_slot], d.s[a.new_slot], a.lfirst, bid))
 LG_BATCH_KERNEL(pot_vort, BX * BY, PH_POT_VORT, b_pot_vort(d.c, d.s[a.state_slot], a.ke, bid))
 LG_BATCH_KERNEL(vort_flux, BX * BY, PH_VORT_FLUX, b_vort_flux(d.c, a.ke, bid))
@@ -878,7 +870,6 @@ bool batch_launch_step(int ph, dim3 g, dim3 b, hipStream_t st, const MemberDev<R
     return true;
   switch (ph) {
     LG_B(PH_TEMPERATURE, kb_temperature)
-    LG_B(PH_THETA_FILL, kb_theta_fill)
     LG_B(PH_THETA_GRAD, kb_theta_grad)
     LG_B(PH_THETA_DIV, kb_theta_div)
     LG_B(PH_THETA_EDGE, kb_theta_edge)
